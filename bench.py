@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: sector H·v throughput on one MI355X.
+
+Workload (default): J1-J2 XXZ chain, L=28, nup=14, k=0, p=+1, z=+1
+(N = 361,270), J1=1, J2=0.4, Delta=1, FP64 matrix-free matvec.  A step is one
+matvec over the whole sector with x and y resident in HBM.  `value` is
+algorithmic bytes per matvec / time, with
+    bytes = N * (8 read x + 8 write y + 4 stored state + 8 stored 1/sqrt(D)).
+Beside it: the FP64 Lanczos ground state (k=1, tol 1e-10) of the same
+sector, the L=32 z+ FP32-stream / FP64 matvec pair (N = 300,540,195,
+implicit basis) and the CPU baseline: the oracle's restatement of QuSpin.jl's
+parent-space apply loop (operators.jl:2554-2589), single-threaded, timed on a
+bounded sample of parent states and extrapolated to the whole sector.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-c3] [--no-eigsh]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "quspin.jl_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def xxz_terms(L, J=1.0, delta=1.0, J2=0.0):
+    b1 = [(i, i % L + 1) for i in range(1, L + 1)]
+    terms = [("+-", [(J / 2, i, j) for i, j in b1]), ("-+", [(J / 2, i, j) for i, j in b1]),
+             ("zz", [(J * delta, i, j) for i, j in b1])]
+    if J2:
+        b2 = [(i, (i + 1) % L + 1) for i in range(1, L + 1)]
+        terms += [("+-", [(J2 / 2, i, j) for i, j in b2]), ("-+", [(J2 / 2, i, j) for i, j in b2]),
+                  ("zz", [(J2 * delta, i, j) for i, j in b2])]
+    return terms
+
+
+def time_matvec(op, x, y, steps, warmup):
+    """Average matvec time (s) with HIP events on the stream the kernel runs on."""
+    import torch
+    stream = torch.cuda.current_stream()
+    op.set_stream(stream.cuda_stream)
+    for _ in range(warmup):
+        op.matvec_device(x, y)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(steps):
+        op.matvec_device(x, y)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(end) * 1e-3 / steps, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-eigsh", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU; the multi-GPU sharding is a design note only "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import qsgpu
+
+    dev = qsgpu.device_info()
+    L = 28
+    print("bench: building C2 sector L=28 nup=14 k=0 p=+1 z=+1", file=sys.stderr, flush=True)
+    tb = time.perf_counter()
+    basis = qsgpu.SpinBasis1D(L, nup=14, pauli=False, kblock=0, pblock=1, zblock=1)
+    torch.cuda.synchronize()
+    t_basis = time.perf_counter() - tb
+    terms = xxz_terms(L, J=1.0, delta=1.0, J2=0.4)
+    op = qsgpu.QuantumLinearOperator(basis, terms)
+    N = basis.Ns
+    g = torch.Generator(device="cpu").manual_seed(1234)
+    x = torch.randn(N, generator=g, dtype=torch.float64).cuda()
+    x /= x.norm()
+    y = torch.zeros_like(x)
+    bytes_alg = N * (8 + 8 + 4 + 8)
+    print("bench: timing %d matvecs (N=%d)" % (args.steps, N), file=sys.stderr, flush=True)
+    t_mv, wall = time_matvec(op, x, y, args.steps, args.warmup)
+    gbps = bytes_alg / t_mv / 1e9
+    result = {
+        "metric": "sector H·v GB/s (% HBM peak)",
+        "value": round(gbps, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mv * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian vector; Hamiltonian from closed-form couplings)",
+        "config": {"workload": "C2: J1-J2 XXZ L=28 nup=14 k=0 p=+1 z=+1, N=%d, FP64 matrix-free matvec" % N,
+                   "L": L, "nup": 14, "J1": 1.0, "J2": 0.4, "Delta": 1.0, "sector_dim": N,
+                   "bytes_per_matvec": bytes_alg, "basis_build_s": round(t_basis, 4), "device": dev["name"]},
+        "roofline": {"bound": "hbm", "achieved": round(gbps, 3), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(gbps / HBM_PEAK_GBPS, 5), "traffic": None,
+                     "kernel": "k_matvec<uint32,double,real,STORED>",
+                     "note": "canonicalisation over the 112-element group dominates; integer-ALU bound at this N"},
+    }
+    if not args.no_eigsh:
+        print("bench: FP64 Lanczos ground state", file=sys.stderr, flush=True)
+        v0 = np.random.default_rng(0x51a7).standard_normal(N)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e64, _ = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0, return_eigenvectors=True)
+        t_e64 = time.perf_counter() - t0
+        nmv64 = qsgpu.eigsh.last_n_matvec
+        t0 = time.perf_counter()
+        e32, _ = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0, dtype=np.float32)
+        t_e32 = time.perf_counter() - t0
+        nmv32 = qsgpu.eigsh.last_n_matvec
+        result["eigsh"] = {"E0_fp64": float(e64[0]), "seconds_fp64": round(t_e64, 4), "matvecs_fp64": nmv64,
+                           "E0_fp32_stream": float(e32[0]), "seconds_fp32": round(t_e32, 4),
+                           "matvecs_fp32": nmv32, "abs_err_fp32_vs_fp64": abs(float(e32[0]) - float(e64[0]))}
+    if not args.no_c3:
+        print("bench: C3 L=32 z+ matvec pair", file=sys.stderr, flush=True)
+        b3 = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
+        op3 = qsgpu.QuantumLinearOperator(b3, xxz_terms(32, J=1.0, delta=1.0))
+        n3 = b3.Ns
+        out = {"sector_dim": n3}
+        for dt, name, esz in ((torch.float32, "fp32", 4), (torch.float64, "fp64", 8)):
+            x3 = torch.rand(n3, dtype=dt, device="cuda")
+            y3 = torch.zeros_like(x3)
+            t3, _ = time_matvec(op3, x3, y3, max(3, args.steps // 10), 2)
+            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(n3 * 2 * esz / t3 / 1e9, 2),
+                         "frac": round(n3 * 2 * esz / t3 / 1e9 / HBM_PEAK_GBPS, 4)}
+            del x3, y3
+            torch.cuda.empty_cache()
+        out["fp32_over_fp64"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)
+        result["c3_matvec"] = out
+    if not args.no_cpu:
+        import oracle
+        print("bench: CPU baseline (reference parent-apply loop, 1 thread, %d-state sample)" % args.cpu_sample,
+              file=sys.stderr, flush=True)
+        npar = math.comb(L, 14)
+        ts = oracle.bench_parent_apply(L, 14, False, terms, args.cpu_sample, 1)
+        t_full = ts * npar / args.cpu_sample
+        result["cpu_baseline"] = {
+            "value": round(bytes_alg / t_full / 1e9, 9), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "parent-space apply loop of operators.jl:2554-2589 over the first %d of %d parent states "
+                      "(%.1f s), extrapolated to one full matvec (%.1f s); lift/project SpMVs not counted"
+                      % (args.cpu_sample, npar, ts, t_full),
+            "speedup_gpu_over_cpu": round(t_full / t_mv, 1),
+        }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

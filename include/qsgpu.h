@@ -1,0 +1,164 @@
+/*
+ * qsgpu.h -- C ABI of libqsgpu.so, the MI355X (gfx950) implementation of
+ * QuSpin.jl's spin-1/2 matrix-free Hamiltonian action and the Krylov solvers
+ * that consume it.
+ *
+ * The reference has no FFI layer: its boundary is Julia's AbstractMatrix
+ * protocol (size / eltype / ishermitian / mul!) that QuantumLinearOperator
+ * implements (operators.jl:3128-3245) and that Arpack.eigs (operators.jl:3384)
+ * and the Krylov exponential (operators.jl:1642-1653) consume.  Each entry
+ * point below names the reference function it stands in for; a Julia
+ * `ccall` binding of every one is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *  - Every function returns a status code (QS_OK on success).  The message of
+ *    the last failure on the calling thread is read with qs_last_error().
+ *  - Handles are opaque and owned by the library; release them with the
+ *    matching *_destroy.  Calls on one handle are serialised by the caller;
+ *    calls on different handles may run concurrently.
+ *  - Sites are 1-based, as in QuSpin.jl.  Complex numbers are interleaved
+ *    (re, im) doubles.  Vectors are contiguous; a block of ncols vectors is
+ *    column-major with leading dimension Ns.
+ *  - `on_device` = 1 means the vector pointers are HIP device pointers that
+ *    the kernels read directly; 0 means host pointers that are staged.
+ */
+#ifndef QSGPU_H
+#define QSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the Julia shim maps them to the reference's exceptions. */
+#define QS_OK 0
+#define QS_EARG 1     /* ArgumentError      (e.g. basis.jl:655,686-687,717-723) */
+#define QS_EDIM 2     /* DimensionMismatch  (operators.jl:3154-3157) */
+#define QS_ENOCONV 3  /* ErrorException: Krylov did not converge (operators.jl:1782-1784) */
+#define QS_EOOM 4     /* device or host allocation failed */
+#define QS_EDEVICE 5  /* HIP runtime error */
+
+/* A symmetry block argument that is `nothing` in Julia. */
+#define QS_UNSET (-1000000)
+
+/* Element types of vectors crossing the boundary. */
+#define QS_F64 0   /* Float64 */
+#define QS_C128 1  /* ComplexF64 */
+#define QS_F32 2   /* Float32 storage (FP32-stream form) */
+#define QS_C64 3   /* ComplexF32 storage (FP32-stream form) */
+
+/* eigsh `which` (operators.jl:3375-3377). */
+#define QS_WHICH_SA 0
+#define QS_WHICH_LA 1
+#define QS_WHICH_SM 2
+#define QS_WHICH_LM 3
+#define QS_WHICH_BE 4
+
+/* qs_op_create flags (operators.jl:2492-2499 keyword arguments). */
+#define QS_CHECK_SYMM 1
+#define QS_CHECK_HERM 2
+#define QS_CHECK_PCON 4
+
+typedef struct qs_basis qs_basis;
+typedef struct qs_op qs_op;
+
+/* Last error message of the calling thread; returns its full length. */
+size_t qs_last_error(char *buf, size_t len);
+
+/* Library / device information: HIP device name and total HBM bytes. */
+int qs_device_info(char *name, size_t len, int64_t *hbm_bytes, int *n_cu);
+
+/* ---------------------------------------------------------------- basis --
+ * SpinBasis1D(L; nup, pauli, a, kblock, pblock, zblock, pzblock, zAblock,
+ * zBblock)  -- basis.jl:622-777.  nup = QS_UNSET enumerates every state.
+ * The sector is enumerated on the GPU (orbit-minimum canonical states in
+ * ascending order, which is the reference's column order).  flags: 0. */
+int qs_basis_create(int L, int nup, int pauli, int a, int kblock, int pblock, int zblock,
+                    int pzblock, int zAblock, int zBblock, int flags, qs_basis **out);
+int qs_basis_destroy(qs_basis *b);
+
+/* length(basis) / basis.Ns -- basis.jl:779 */
+int qs_basis_size(const qs_basis *b, int64_t *ns);
+
+/* states(basis) -- basis.jl:786: the reference's representative label per
+ * column (symmetry_basis.jl:215-232), in column order. */
+int qs_basis_states(const qs_basis *b, uint64_t *out);
+
+/* Orbit-minimum canonical state per column (the key the GPU kernels index
+ * by; equals states(basis) wherever the reference label is the orbit min). */
+int qs_basis_reps(const qs_basis *b, uint64_t *out);
+
+/* Per column: number of distinct parent states in the orbit, and the
+ * reference's normalization(basis, states(basis)[i]) (discrete_basis.jl:2401-2408). */
+int qs_basis_norms(const qs_basis *b, uint32_t *orbit_size, double *ref_norm);
+
+/* _basis_requires_complex (symmetry_basis.jl:624-629). */
+int qs_basis_is_complex(const qs_basis *b, int *is_complex);
+
+/* state_index(basis, s) -- basis.jl:1016-1027 on the column keys: index of
+ * the column whose orbit contains states[i] (0-based, -1 if none), and the
+ * amplitude <states[i]|column> (interleaved complex, may be NULL). */
+int qs_basis_lookup(const qs_basis *b, const uint64_t *states, int64_t n, int64_t *index,
+                    double *amplitude);
+
+/* ------------------------------------------------------------- operator --
+ * QuantumLinearOperator(basis, terms; diagonal, check_symm, check_herm,
+ * check_pcon) -- operators.jl:2492-2552.
+ *   opstr[t]        operator string of term t ("+-", "zz", "z", "x", "y", "I")
+ *   n_couplings[t]  couplings of term t
+ *   coeff           2 doubles (re, im) per coupling, terms in order
+ *   sites           len(opstr[t]) 1-based sites per coupling, terms in order
+ *   diag            NULL or 2*Ns doubles: the user `diagonal` keyword
+ * The operator realises the reference's P'·H·P semantics in the sector. */
+int qs_op_create(const qs_basis *b, int n_terms, const char *const *opstr,
+                 const int32_t *n_couplings, const double *coeff, const int32_t *sites,
+                 const double *diag, int flags, qs_op **out);
+int qs_op_destroy(qs_op *op);
+
+/* eltype / ishermitian / size -- operators.jl:3228-3245. */
+int qs_op_info(const qs_op *op, int *is_complex, int *is_hermitian, int64_t *ns);
+
+/* set_diagonal!(op, d) -- operators.jl:3301-3309 (2*Ns doubles). */
+int qs_op_set_diagonal(qs_op *op, const double *diag);
+
+/* Stream the operator launches on (hipStream_t); NULL selects the
+ * library's own stream.  qs_op_stream returns the one in use. */
+int qs_op_set_stream(qs_op *op, void *stream);
+int qs_op_stream(const qs_op *op, void **stream);
+
+/* mul!(Y, op, X, alpha, beta) for ncols column vectors --
+ * operators.jl:3147-3208; transposed/conjugated select transpose(op),
+ * conj(op) and adjoint(op) (operators.jl:3286-3296).
+ * dtype: QS_F64 / QS_C128 (FP64 form) or QS_F32 / QS_C64 (FP32 stream,
+ * FP64 row accumulation).  Real dtypes need a real operator. */
+int qs_op_matvec(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, double alpha_re,
+                 double alpha_im, double beta_re, double beta_im, int transposed, int conjugated,
+                 int on_device);
+
+/* eigsh(op; k, which, tol, maxiter, ncv, v0, return_eigenvectors) --
+ * operators.jl:3367-3408 (ARPACK "SA"/"LA"/... semantics).  Thick-restart
+ * Lanczos with full FP64-accumulated reorthogonalisation on the device.
+ *   dtype  QS_F64/QS_C128: FP64 vectors; QS_F32/QS_C64: FP32-stored Krylov
+ *          basis with FP64 dots, norms and tridiagonal solve.
+ *   tol    <= 0 selects machine epsilon (ARPACK default)
+ *   v0     NULL or Ns elements of the vector dtype (host)
+ *   vecs   NULL or Ns*k elements of the vector dtype (host, column-major)
+ *   vals   k doubles, or 2k (interleaved) when the operator is not Hermitian
+ * Returns QS_ENOCONV (with the converged count in *nconv) if fewer than k
+ * Ritz pairs converged within maxiter restarts. */
+int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, int ncv,
+             const void *v0, double *vals, void *vecs, int *nconv, int *n_matvec);
+
+/* _krylov_expmv_times(op, v, scales; tol, krylov_dim) -- operators.jl:1805-1893:
+ * out[:, j] = exp(scales[j] * op) * v for n_times complex scales.
+ * dtype QS_C128 (FP64) or QS_C64 (FP32-stored basis, FP64 reductions).
+ * v and out are host ComplexF64/ComplexF32 arrays (out: Ns x n_times). */
+int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const double *scales,
+                     double tol, int krylov_dim, void *out, int *n_matvec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSGPU_H */

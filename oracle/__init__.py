@@ -1,0 +1,250 @@
+"""CPU oracle for the spin-1/2 sector Hamiltonian action (test infrastructure).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this package, as the checker or as the timed CPU baseline.
+The product library (``libqsgpu.so``) never loads it.
+
+``qs_oracle.c`` restates QuSpin.jl's CPU algorithms (cited per function there);
+``krylov.py`` restates the Krylov exponential and Lanczos helpers in numpy.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libqsoracle.so")
+UNSET = -1000000
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = ctypes.CDLL(_LIB_PATH)
+    c_i64, c_int, c_dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    vp = ctypes.c_void_p
+    L.qo_last_error.restype = ctypes.c_char_p
+    L.qo_fixed_weight_states.argtypes = [c_int, c_int, vp]
+    L.qo_fixed_weight_states.restype = c_i64
+    L.qo_sector_build.argtypes = [c_int] * 10
+    L.qo_sector_build.restype = vp
+    for name in ("qo_sector_ncols", "qo_sector_nparent", "qo_sector_nnz"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = c_i64
+    L.qo_sector_reduced.argtypes = [vp]
+    L.qo_sector_reduced.restype = c_int
+    for name in ("qo_sector_parent_states", "qo_sector_labels"):
+        getattr(L, name).argtypes = [vp, vp]
+        getattr(L, name).restype = None
+    L.qo_sector_projector.argtypes = [vp, vp, vp, vp]
+    L.qo_sector_projector.restype = None
+    L.qo_sector_norms.argtypes = [vp, vp, vp]
+    L.qo_sector_norms.restype = None
+    L.qo_sector_free.argtypes = [vp]
+    L.qo_sector_free.restype = None
+    L.qo_op_create.argtypes = [vp, c_int, vp, vp, vp, vp, c_int, vp]
+    L.qo_op_create.restype = vp
+    L.qo_op_matvec.argtypes = [vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_int, c_int]
+    L.qo_op_matvec.restype = c_int
+    L.qo_op_free.argtypes = [vp]
+    L.qo_op_free.restype = None
+    L.qo_op_ncouplings.argtypes = [vp]
+    L.qo_op_ncouplings.restype = c_int
+    L.qo_csr_build.argtypes = [vp, c_int, vp, vp, vp, vp]
+    L.qo_csr_build.restype = vp
+    L.qo_csr_nnz.argtypes = [vp]
+    L.qo_csr_nnz.restype = c_i64
+    L.qo_csr_matvec.argtypes = [vp, vp, vp, c_dbl, c_dbl, c_int]
+    L.qo_csr_matvec.restype = None
+    L.qo_csr_export.argtypes = [vp, vp, vp, vp]
+    L.qo_csr_export.restype = None
+    L.qo_csr_free.argtypes = [vp]
+    L.qo_csr_free.restype = None
+    L.qo_bench_parent_apply.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, c_i64, c_int]
+    L.qo_bench_parent_apply.restype = c_dbl
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def fixed_weight_states(L, w):
+    """basis.jl:229-250 (w=None enumerates every L-bit word)."""
+    lb = lib()
+    ww = -1 if w is None else int(w)
+    n = lb.qo_fixed_weight_states(L, ww, None)
+    out = np.empty(n, dtype=np.uint64)
+    lb.qo_fixed_weight_states(L, ww, _ptr(out))
+    return out
+
+
+def _u(v):
+    return UNSET if v is None else int(v)
+
+
+class Sector:
+    """Reference-semantics SpinBasis1D (basis.jl:622-777): explicit projector P."""
+
+    def __init__(self, L, nup=None, pauli=True, a=1, kblock=None, pblock=None, zblock=None,
+                 pzblock=None, zAblock=None, zBblock=None):
+        lb = lib()
+        h = lb.qo_sector_build(L, _u(nup), int(bool(pauli)), a, _u(kblock), _u(pblock), _u(zblock),
+                               _u(pzblock), _u(zAblock), _u(zBblock))
+        if not h:
+            raise ValueError(lb.qo_last_error().decode())
+        self._h = h
+        self.L, self.nup, self.pauli = L, nup, pauli
+        self.Ns = lb.qo_sector_ncols(h)
+        self.npar = lb.qo_sector_nparent(h)
+        self.reduced = bool(lb.qo_sector_reduced(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().qo_sector_free(self._h)
+            self._h = None
+
+    def parent_states(self):
+        out = np.empty(self.npar, dtype=np.uint64)
+        lib().qo_sector_parent_states(self._h, _ptr(out))
+        return out
+
+    def states(self):
+        out = np.empty(self.Ns, dtype=np.uint64)
+        lib().qo_sector_labels(self._h, _ptr(out))
+        return out
+
+    def norms(self):
+        ref = np.empty(self.Ns, dtype=np.float64)
+        orb = np.empty(self.Ns, dtype=np.uint32)
+        lib().qo_sector_norms(self._h, _ptr(ref), _ptr(orb))
+        return orb, ref
+
+    def projector(self):
+        """Dense-free CSC pieces (colptr, row, values) of P, 0-based."""
+        if not self.reduced:
+            return None
+        nnz = lib().qo_sector_nnz(self._h)
+        colptr = np.empty(self.Ns + 1, dtype=np.int64)
+        row = np.empty(nnz, dtype=np.int64)
+        val = np.empty(nnz, dtype=np.complex128)
+        lib().qo_sector_projector(self._h, _ptr(colptr), _ptr(row), _ptr(val))
+        return colptr, row, val
+
+    def projector_dense(self):
+        if not self.reduced:
+            return np.eye(self.Ns, dtype=np.complex128)
+        colptr, row, val = self.projector()
+        P = np.zeros((self.npar, self.Ns), dtype=np.complex128)
+        for c in range(self.Ns):
+            P[row[colptr[c]:colptr[c + 1]], c] = val[colptr[c]:colptr[c + 1]]
+        return P
+
+
+def _flatten_terms(terms):
+    ops, ncoup, coef, sites = [], [], [], []
+    for op, couplings in terms:
+        ops.append(op.encode())
+        ncoup.append(len(couplings))
+        for c in couplings:
+            z = complex(c[0])
+            coef += [z.real, z.imag]
+            sites += [int(s) for s in c[1:]]
+    arr_ops = (ctypes.c_char_p * max(1, len(ops)))(*ops)
+    return (arr_ops, np.asarray(ncoup, dtype=np.int32), np.asarray(coef, dtype=np.float64),
+            np.asarray(sites if sites else [0], dtype=np.int32))
+
+
+class Operator:
+    """Reference-semantics QuantumLinearOperator (operators.jl:2492-2552)."""
+
+    def __init__(self, sector, terms, check_symm=True, diagonal=None):
+        lb = lib()
+        self.sector = sector
+        self._keep = _flatten_terms(terms)
+        ops, ncoup, coef, sites = self._keep
+        d = None
+        if diagonal is not None:
+            self._diag = np.ascontiguousarray(np.asarray(diagonal, dtype=np.complex128))
+            d = _ptr(self._diag)
+        h = lb.qo_op_create(sector._h, len(terms), ops, _ptr(ncoup), _ptr(coef), _ptr(sites),
+                            int(bool(check_symm)), d)
+        if not h:
+            raise ValueError(lb.qo_last_error().decode())
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().qo_op_free(self._h)
+            self._h = None
+
+    def matvec(self, x, y=None, alpha=1.0, beta=0.0, transposed=False, conjugated=False):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+        out = np.zeros(self.sector.Ns, dtype=np.complex128) if y is None else \
+            np.ascontiguousarray(np.asarray(y, dtype=np.complex128)).copy()
+        a, b = complex(alpha), complex(beta)
+        lib().qo_op_matvec(self._h, _ptr(x), _ptr(out), a.real, a.imag, b.real, b.imag,
+                           int(transposed), int(conjugated))
+        return out
+
+    def dense(self):
+        n = self.sector.Ns
+        M = np.zeros((n, n), dtype=np.complex128)
+        e = np.zeros(n, dtype=np.complex128)
+        for j in range(n):
+            e[:] = 0
+            e[j] = 1
+            M[:, j] = self.matvec(e)
+        return M
+
+
+class CSR:
+    """Assembled real CSR Hamiltonian on a basis without symmetry
+    (operators.jl:381-576, sparse_formats.jl:29-50,110-146)."""
+
+    def __init__(self, sector, terms):
+        lb = lib()
+        self._keep = _flatten_terms(terms)
+        ops, ncoup, coef, sites = self._keep
+        h = lb.qo_csr_build(sector._h, len(terms), ops, _ptr(ncoup), _ptr(coef), _ptr(sites))
+        if not h:
+            raise ValueError(lb.qo_last_error().decode())
+        self._h = h
+        self.n = sector.Ns
+        self.nnz = lb.qo_csr_nnz(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().qo_csr_free(self._h)
+            self._h = None
+
+    def matvec(self, x, y=None, alpha=1.0, beta=0.0, nthreads=1):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        out = np.zeros(self.n) if y is None else y
+        lib().qo_csr_matvec(self._h, _ptr(x), _ptr(out), float(alpha), float(beta), int(nthreads))
+        return out
+
+    def export(self):
+        rowptr = np.empty(self.n + 1, dtype=np.int64)
+        col = np.empty(self.nnz, dtype=np.int64)
+        val = np.empty(self.nnz, dtype=np.float64)
+        lib().qo_csr_export(self._h, _ptr(rowptr), _ptr(col), _ptr(val))
+        return rowptr, col, val
+
+
+def bench_parent_apply(L, nup, pauli, terms, n_sample, repeats=1):
+    """Seconds the reference's parent-space apply loop takes over the first
+    n_sample parent states (single thread); see qo_bench_parent_apply."""
+    ops, ncoup, coef, sites = _flatten_terms(terms)
+    return lib().qo_bench_parent_apply(L, nup, int(bool(pauli)), len(terms), ops, _ptr(ncoup), _ptr(coef),
+                                       _ptr(sites), int(n_sample), int(repeats))

@@ -1,0 +1,648 @@
+// qs_api.cpp -- the extern "C" boundary of libqsgpu (declared in include/qsgpu.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "qs_internal.h"
+
+struct qs_basis {
+    qs::BasisHost b;
+};
+struct qs_op {
+    qs::OpHost o;
+};
+
+namespace qs {
+
+const std::string &last_error();
+
+OpHost::~OpHost() {
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+namespace {
+
+template <typename F>
+int guard(F &&f) {
+    try {
+        f();
+        return QS_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return QS_EOOM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return QS_EDEVICE;
+    }
+}
+
+void check_ptr(const void *p, const char *name) {
+    if (!p) fail(QS_EARG, std::string(name) + " must not be NULL");
+}
+
+// host-side canonicalisation, used by qs_basis_lookup
+uint64_t canon_host(const BasisHost &b, uint64_t u, cd &chi) {
+    const GroupHost &g = b.group;
+    uint64_t best = ~0ULL;
+    chi = cd(1, 0);
+    for (int p = 0; p < g.npoint; p++) {
+        uint64_t v = apply_point(g, p, u);
+        for (int t = 0; t < g.nT; t++) {
+            uint64_t w = rotl(v, t * g.a, g.L);
+            if (w < best) {
+                best = w;
+                chi = g.lam[t] * g.pchar[p];
+            }
+        }
+    }
+    return best;
+}
+
+int64_t rank_host(uint64_t u) {
+    int64_t r = 0;
+    int k = 1;
+    while (u) {
+        int p = __builtin_ctzll(u);
+        r += binom(p, k);
+        k++;
+        u &= u - 1;
+    }
+    return r;
+}
+
+uint64_t unrank_host(int64_t i, int L, int w) {
+    uint64_t u = 0;
+    int p = L - 1;
+    for (int k = w; k >= 1; k--) {
+        while (binom(p, k) > i) p--;
+        u |= 1ULL << p;
+        i -= binom(p, k);
+        p--;
+    }
+    return u;
+}
+
+void host_reps(const BasisHost &b, std::vector<uint64_t> &out) {
+    out.resize((size_t)b.Ns);
+    if (b.mode == MODE_STORED) {
+        if (b.use64) {
+            QS_HIP(hipMemcpy(out.data(), b.reps64.get(), sizeof(uint64_t) * b.Ns, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<uint32_t> tmp((size_t)b.Ns);
+            QS_HIP(hipMemcpy(tmp.data(), b.reps32.get(), sizeof(uint32_t) * b.Ns, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < b.Ns; i++) out[i] = tmp[i];
+        }
+        return;
+    }
+    if (b.nup < 0) {
+        for (int64_t i = 0; i < b.Ns; i++) out[i] = (uint64_t)i;
+        return;
+    }
+    // walk the combinadic order with Gosper's successor
+    uint64_t s = unrank_host(0, b.L, b.nup);
+    for (int64_t i = 0; i < b.Ns; i++) {
+        out[i] = s;
+        if (i + 1 == b.Ns || s == 0) break;
+        uint64_t lowest = s & (~s + 1), nx = s + lowest;
+        s = nx | ((nx ^ s) >> (__builtin_ctzll(lowest) + 2));
+    }
+}
+
+bool structural_hermitian(const std::vector<Coupling> &cs) {
+    // operators.jl:2979-3007
+    std::map<std::pair<std::string, std::vector<int>>, cd> coef;
+    for (auto &c : cs) coef[{c.op, c.sites}] += c.coef;
+    for (auto &kv : coef) {
+        std::string adj = kv.first.first;
+        for (char &ch : adj) {
+            if (ch == '+') ch = '-';
+            else if (ch == '-') ch = '+';
+            else if (ch != 'I' && ch != 'x' && ch != 'y' && ch != 'z' && ch != 'n' && ch != '|') return false;
+        }
+        auto it = coef.find({adj, kv.first.second});
+        if (it == coef.end()) return false;
+        cd want = std::conj(kv.second);
+        double d = std::abs(it->second - want);
+        double tol = std::max(3e-12, 3e-12 * std::max(std::abs(it->second), std::abs(want)));
+        if (d > tol) return false;
+    }
+    return true;
+}
+
+bool pcon_ok(const std::vector<Coupling> &cs) {
+    // discrete_basis.jl:2333-2339
+    for (auto &c : cs) {
+        int change = 0;
+        for (char ch : c.op) {
+            if (ch == 'x' || ch == 'y') return false;
+            if (ch == '+') change++;
+            if (ch == '-') change--;
+        }
+        if (change != 0) return false;
+    }
+    return true;
+}
+
+// adjoint of a compiled coupling (same flip, shifted condition, conj amplitude)
+CTerm adjoint_term(const CTerm &t) {
+    CTerm a = t;
+    a.cval = t.cval ^ (t.flip & t.cmask);
+    a.A = std::conj(t.A);
+    if (__builtin_popcountll(t.flip & t.smask) & 1) a.A = -a.A;
+    return a;
+}
+
+void upload_terms(const std::vector<CTerm> &src, DevBuf<TermDev> &dst) {
+    std::vector<TermDev> h(std::max<size_t>(1, src.size()));
+    for (size_t i = 0; i < src.size(); i++) {
+        h[i].cmask = src[i].cmask;
+        h[i].cval = src[i].cval;
+        h[i].smask = src[i].smask;
+        h[i].are = src[i].A.real();
+        h[i].aim = src[i].A.imag();
+    }
+    dst.alloc(h.size());
+    QS_HIP(hipMemcpy(dst.get(), h.data(), sizeof(TermDev) * h.size(), hipMemcpyHostToDevice));
+}
+
+void upload_groups(const std::vector<GroupOfTerms> &src, DevBuf<GroupOfTerms> &dst) {
+    std::vector<GroupOfTerms> h(src);
+    if (h.empty()) h.push_back(GroupOfTerms{0, 0, 0});
+    dst.alloc(h.size());
+    QS_HIP(hipMemcpy(dst.get(), h.data(), sizeof(GroupOfTerms) * h.size(), hipMemcpyHostToDevice));
+}
+
+hipStream_t ensure_stream(OpHost &o) {
+    if (o.stream) return o.stream;
+    if (!o.own_stream) QS_HIP(hipStreamCreate(&o.own_stream));
+    o.stream = o.own_stream;
+    return o.stream;
+}
+
+void ensure_stage(DevBuf<char> &buf, size_t bytes) {
+    if (buf.n < bytes) buf.alloc(bytes);
+}
+
+}  // namespace
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+size_t qs_last_error(char *buf, size_t len) {
+    const std::string &e = last_error();
+    if (buf && len) {
+        size_t n = std::min(len - 1, e.size());
+        memcpy(buf, e.data(), n);
+        buf[n] = 0;
+    }
+    return e.size();
+}
+
+int qs_device_info(char *name, size_t len, int64_t *hbm_bytes, int *n_cu) {
+    return guard([&] {
+        int dev = 0;
+        QS_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        QS_HIP(hipGetDeviceProperties(&p, dev));
+        if (name && len) {
+            std::string s = std::string(p.name) + " (" + p.gcnArchName + ")";
+            size_t n = std::min(len - 1, s.size());
+            memcpy(name, s.data(), n);
+            name[n] = 0;
+        }
+        if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+        if (n_cu) *n_cu = p.multiProcessorCount;
+    });
+}
+
+int qs_basis_create(int L, int nup, int pauli, int a, int kblock, int pblock, int zblock, int pzblock, int zAblock,
+                    int zBblock, int flags, qs_basis **out) {
+    (void)flags;
+    return guard([&] {
+        check_ptr(out, "out");
+        *out = nullptr;
+        // basis.jl:652-665
+        if (L < 1 || L > 63) fail(QS_EARG, "L must be between 1 and 63");
+        if (nup != QS_UNSET && (nup < 0 || nup > L)) fail(QS_EARG, "nup must lie between 0 and L");
+        auto h = std::make_unique<qs_basis>();
+        BasisHost &b = h->b;
+        b.L = L;
+        b.nup = nup == QS_UNSET ? -1 : nup;
+        b.pauli = pauli ? 1 : 0;
+        b.a = a;
+        int blocks[6] = {kblock, pblock, zblock, pzblock, zAblock, zBblock};
+        memcpy(b.blocks, blocks, sizeof blocks);
+        if (b.nup < 0 && L > 40) fail(QS_EARG, "a basis without nup is limited to L <= 40 here");
+        build_group(b);
+        build_basis_device(b);
+        *out = h.release();
+    });
+}
+
+int qs_basis_destroy(qs_basis *b) {
+    delete b;
+    return QS_OK;
+}
+
+int qs_basis_size(const qs_basis *b, int64_t *ns) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(ns, "ns");
+        *ns = b->b.Ns;
+    });
+}
+
+int qs_basis_reps(const qs_basis *b, uint64_t *out) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(out, "out");
+        std::vector<uint64_t> r;
+        host_reps(b->b, r);
+        memcpy(out, r.data(), sizeof(uint64_t) * r.size());
+    });
+}
+
+int qs_basis_states(const qs_basis *b, uint64_t *out) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(out, "out");
+        BasisHost &bb = const_cast<BasisHost &>(b->b);
+        if (bb.mode != MODE_STORED) {
+            std::vector<uint64_t> r;
+            host_reps(bb, r);
+            memcpy(out, r.data(), sizeof(uint64_t) * r.size());
+            return;
+        }
+        if (!bb.labels_ready) compute_reference_labels(bb);
+        memcpy(out, bb.labels.data(), sizeof(uint64_t) * bb.labels.size());
+    });
+}
+
+int qs_basis_norms(const qs_basis *b, uint32_t *orbit_size, double *ref_norm) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        BasisHost &bb = const_cast<BasisHost &>(b->b);
+        const int64_t n = bb.Ns;
+        if (bb.mode == MODE_FULL) {
+            if (orbit_size) std::fill(orbit_size, orbit_size + n, 1u);
+            if (ref_norm) std::fill(ref_norm, ref_norm + n, 1.0);
+            return;
+        }
+        if (bb.mode == MODE_ZONLY) {
+            // column [1, z]/norm([1, z]) (symmetry_basis.jl:90-104), normalization 1/abs2(amp)
+            double nrm = std::sqrt(1.0 * 1.0 + 1.0 * 1.0);
+            double amp = 1.0 / nrm;
+            if (orbit_size) std::fill(orbit_size, orbit_size + n, 2u);
+            if (ref_norm) std::fill(ref_norm, ref_norm + n, 1.0 / (amp * amp));
+            return;
+        }
+        if (orbit_size) QS_HIP(hipMemcpy(orbit_size, bb.orbit.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        if (ref_norm) {
+            if (!bb.labels_ready) compute_reference_labels(bb);
+            memcpy(ref_norm, bb.ref_norm.data(), sizeof(double) * n);
+        }
+    });
+}
+
+int qs_basis_is_complex(const qs_basis *b, int *is_complex) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(is_complex, "is_complex");
+        *is_complex = b->b.is_complex;
+    });
+}
+
+int qs_basis_lookup(const qs_basis *b, const uint64_t *states, int64_t n, int64_t *index, double *amplitude) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        const BasisHost &bb = b->b;
+        std::vector<uint64_t> reps;
+        if (bb.mode == MODE_STORED) host_reps(bb, reps);
+        uint64_t mask = full_mask(bb.L);
+        for (int64_t i = 0; i < n; i++) {
+            uint64_t s = states[i];
+            int64_t idx = -1;
+            cd amp(0, 0);
+            bool valid = (s & ~mask) == 0 && (bb.nup < 0 || __builtin_popcountll(s) == bb.nup);
+            if (valid) {
+                if (bb.mode == MODE_FULL) {
+                    idx = bb.nup >= 0 ? rank_host(s) : (int64_t)s;
+                    amp = 1.0;
+                } else if (bb.mode == MODE_ZONLY) {
+                    uint64_t r = s;
+                    cd ch = 1.0;
+                    if (s >> (bb.L - 1) & 1) {
+                        r = ~s & mask;
+                        ch = (double)bb.zchar;
+                    }
+                    idx = bb.nup >= 0 ? rank_host(r) : (int64_t)r;
+                    amp = ch / std::sqrt(2.0);
+                } else {
+                    cd ch;
+                    uint64_t r = canon_host(bb, s, ch);
+                    auto it = std::lower_bound(reps.begin(), reps.end(), r);
+                    if (it != reps.end() && *it == r) {
+                        idx = it - reps.begin();
+                        uint32_t d;
+                        QS_HIP(hipMemcpy(&d, bb.orbit.get() + idx, sizeof d, hipMemcpyDeviceToHost));
+                        amp = ch / std::sqrt((double)d);
+                    }
+                }
+            }
+            if (index) index[i] = idx;
+            if (amplitude) {
+                amplitude[2 * i] = amp.real();
+                amplitude[2 * i + 1] = amp.imag();
+            }
+        }
+    });
+}
+
+int qs_op_create(const qs_basis *b, int n_terms, const char *const *opstr, const int32_t *n_couplings,
+                 const double *coeff, const int32_t *sites, const double *diag, int flags, qs_op **out) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(out, "out");
+        *out = nullptr;
+        const BasisHost &bb = b->b;
+        std::vector<Coupling> cs;
+        size_t ci = 0, si = 0;
+        bool any_complex = false;
+        for (int t = 0; t < n_terms; t++) {
+            std::string op(opstr[t]);
+            for (int j = 0; j < n_couplings[t]; j++) {
+                Coupling c;
+                c.op = op;
+                c.coef = cd(coeff[2 * ci], coeff[2 * ci + 1]);
+                ci++;
+                for (size_t q = 0; q < op.size(); q++) c.sites.push_back(sites[si++]);
+                if (c.coef.imag() != 0) any_complex = true;
+                cs.push_back(c);
+            }
+            if (op.find('y') != std::string::npos) any_complex = true;
+        }
+        auto h = std::make_unique<qs_op>();
+        OpHost &o = h->o;
+        o.basis = &bb;
+        // operators.jl:2519-2534: checks in the reference's order
+        bool herm = structural_hermitian(cs);
+        bool check_herm = flags & QS_CHECK_HERM, check_pcon = flags & QS_CHECK_PCON, check_symm = flags & QS_CHECK_SYMM;
+        if (check_herm && !herm) fail(QS_EARG, "operator list is not Hermitian");
+        if (check_pcon && bb.nup >= 0 && !pcon_ok(cs)) fail(QS_EARG, "operator list violates the selected particle sector");
+        std::vector<CTerm> compiled = compile_couplings(cs, bb.L, bb.pauli);
+        std::vector<CTerm> H = consolidate_cterms(compiled);
+        if (check_symm && bb.mode != MODE_FULL) {
+            std::vector<CTerm> avg = symmetrize_cterms(H, bb);
+            std::map<std::tuple<uint64_t, uint64_t, uint64_t, uint64_t>, cd> a1, a2;
+            for (auto &x : H) a1[std::make_tuple(x.flip, x.cmask, x.cval, x.smask)] = x.A;
+            for (auto &x : avg) a2[std::make_tuple(x.flip, x.cmask, x.cval, x.smask)] = x.A;
+            bool same = a1.size() == a2.size();
+            for (auto &kv : a1) {
+                auto it = a2.find(kv.first);
+                if (it == a2.end() || std::abs(it->second - kv.second) > 1e-12 * std::max(1.0, std::abs(kv.second)))
+                    same = false;
+            }
+            if (!same) fail(QS_EARG, "operator list violates the selected symmetry sector");
+        }
+        std::vector<CTerm> Havg = symmetrize_cterms(H, bb);
+        bool diag_complex = false;
+        if (diag) {
+            o.user_diag.resize((size_t)bb.Ns);
+            for (int64_t i = 0; i < bb.Ns; i++) {
+                o.user_diag[i] = cd(diag[2 * i], diag[2 * i + 1]);
+                if (diag[2 * i + 1] != 0) diag_complex = true;
+            }
+            o.has_user_diag = true;
+            if (check_herm && diag_complex) fail(QS_EARG, "a Hermitian operator requires a real diagonal");
+        }
+        o.is_complex = (any_complex || bb.is_complex || diag_complex || (flags & 8)) ? 1 : 0;
+        for (auto &x : Havg)
+            if (x.A.imag() != 0) o.is_complex = 1;
+        o.hermitian = check_herm ? 1 : ((herm && !diag_complex) ? 1 : 0);
+        // pull-form kernels: plain products apply the adjoint couplings
+        std::vector<CTerm> Hadj;
+        for (auto &x : Havg) Hadj.push_back(adjoint_term(x));
+        split_terms(Hadj, o.diag_terms, o.off_terms, o.groups);
+        split_terms(Havg, o.diag_terms_adj, o.off_terms_adj, o.groups_adj);
+        upload_terms(o.diag_terms, o.d_diag);
+        upload_terms(o.off_terms, o.d_off);
+        upload_groups(o.groups, o.d_groups);
+        upload_terms(o.diag_terms_adj, o.d_diag_adj);
+        upload_terms(o.off_terms_adj, o.d_off_adj);
+        upload_groups(o.groups_adj, o.d_groups_adj);
+        if (o.has_user_diag) {
+            o.d_user_diag.alloc((size_t)2 * bb.Ns);
+            QS_HIP(hipMemcpy(o.d_user_diag.get(), diag, sizeof(double) * 2 * bb.Ns, hipMemcpyHostToDevice));
+        }
+        ensure_stream(o);
+        *out = h.release();
+    });
+}
+
+int qs_op_destroy(qs_op *op) {
+    delete op;
+    return QS_OK;
+}
+
+int qs_op_info(const qs_op *op, int *is_complex, int *is_hermitian, int64_t *ns) {
+    return guard([&] {
+        check_ptr(op, "op");
+        if (is_complex) *is_complex = op->o.is_complex;
+        if (is_hermitian) *is_hermitian = op->o.hermitian;
+        if (ns) *ns = op->o.basis->Ns;
+    });
+}
+
+int qs_op_set_diagonal(qs_op *op, const double *diag) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(diag, "diag");
+        OpHost &o = op->o;
+        int64_t n = o.basis->Ns;
+        o.user_diag.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) o.user_diag[i] = cd(diag[2 * i], diag[2 * i + 1]);
+        o.has_user_diag = true;
+        o.d_user_diag.alloc((size_t)2 * n);
+        QS_HIP(hipMemcpy(o.d_user_diag.get(), diag, sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    });
+}
+
+int qs_op_set_stream(qs_op *op, void *stream) {
+    return guard([&] {
+        check_ptr(op, "op");
+        op->o.stream = stream ? (hipStream_t)stream : nullptr;
+        ensure_stream(op->o);
+    });
+}
+
+int qs_op_stream(const qs_op *op, void **stream) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(stream, "stream");
+        *stream = (void *)op->o.stream;
+    });
+}
+
+int qs_op_matvec(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, double alpha_re, double alpha_im,
+                 double beta_re, double beta_im, int transposed, int conjugated, int on_device) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(x, "x");
+        check_ptr(y, "y");
+        OpHost &o = op->o;
+        if (ncols < 0) fail(QS_EDIM, "ncols must be non-negative");
+        if (o.is_complex && !dtype_complex(dtype))
+            fail(QS_EARG, "a complex operator needs a complex vector dtype");
+        hipStream_t s = ensure_stream(o);
+        const int64_t n = o.basis->Ns;
+        const size_t esz = dtype_size(dtype), bytes = (size_t)n * ncols * esz;
+        const void *dx = x;
+        void *dy = y;
+        bool beta_zero = beta_re == 0 && beta_im == 0;
+        if (!on_device) {
+            ensure_stage(o.stage_x, bytes);
+            ensure_stage(o.stage_y, bytes);
+            QS_HIP(hipMemcpyAsync(o.stage_x.get(), x, bytes, hipMemcpyHostToDevice, s));
+            if (!beta_zero) QS_HIP(hipMemcpyAsync(o.stage_y.get(), y, bytes, hipMemcpyHostToDevice, s));
+            dx = o.stage_x.get();
+            dy = o.stage_y.get();
+        } else if (x == y) {
+            // the reference copies an aliased input (operators.jl:3158)
+            ensure_stage(o.stage_x, bytes);
+            QS_HIP(hipMemcpyAsync(o.stage_x.get(), x, bytes, hipMemcpyDeviceToDevice, s));
+            dx = o.stage_x.get();
+        }
+        // M x, M^dagger x, M^T x = conj(M^dagger conj x), conj(M) x = conj(M conj x)
+        bool adj = transposed != 0;
+        bool cj = (transposed != 0) != (conjugated != 0);
+        double ar = alpha_re, ai = cj ? -alpha_im : alpha_im, br = beta_re, bi = cj ? -beta_im : beta_im;
+        for (int64_t c = 0; c < ncols; c++) {
+            const char *xc = (const char *)dx + (size_t)c * n * esz;
+            char *yc = (char *)dy + (size_t)c * n * esz;
+            launch_matvec(o, dtype, xc, yc, ar, ai, br, bi, adj, cj, cj, s);
+        }
+        if (!on_device) {
+            QS_HIP(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, s));
+            QS_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0, double *vals,
+             void *vecs, int *nconv, int *n_matvec) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(vals, "vals");
+        OpHost &o = op->o;
+        const int64_t n = o.basis->Ns;
+        if (k < 1 || k > n) fail(QS_EARG, "k must lie in 1:Ns");
+        if (which < QS_WHICH_SA || which > QS_WHICH_BE) fail(QS_EARG, "which must be SA, LA, SM, LM, or BE");
+        if (o.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+        hipStream_t s = ensure_stream(o);
+        const size_t esz = dtype_size(dtype);
+        DevBuf<char> dv0, dvec;
+        if (v0) {
+            dv0.alloc((size_t)n * esz);
+            QS_HIP(hipMemcpyAsync(dv0.get(), v0, (size_t)n * esz, hipMemcpyHostToDevice, s));
+        }
+        if (vecs) dvec.alloc((size_t)n * k * esz);
+        EigshResult res;
+        if (k >= n - 1) {
+            // operators.jl:3379-3382: dense fallback for tiny problems
+            int nn = (int)n;
+            int cdt = dtype_complex(dtype) ? QS_C128 : QS_F64;
+            DevBuf<char> E((size_t)nn * nn * dtype_size(cdt)), Y((size_t)nn * nn * dtype_size(cdt));
+            std::vector<char> eh((size_t)nn * nn * dtype_size(cdt), 0);
+            for (int j = 0; j < nn; j++) {
+                if (cdt == QS_F64) ((double *)eh.data())[(size_t)j * nn + j] = 1.0;
+                else ((double *)eh.data())[2 * ((size_t)j * nn + j)] = 1.0;
+            }
+            QS_HIP(hipMemcpy(E.get(), eh.data(), eh.size(), hipMemcpyHostToDevice));
+            for (int j = 0; j < nn; j++)
+                launch_matvec(o, cdt, E.get() + (size_t)j * nn * dtype_size(cdt), Y.get() + (size_t)j * nn * dtype_size(cdt),
+                              1, 0, 0, 0, false, false, false, s);
+            QS_HIP(hipMemcpy(eh.data(), Y.get(), eh.size(), hipMemcpyDeviceToHost));
+            std::vector<cd> A((size_t)nn * nn);
+            for (int i = 0; i < nn; i++)
+                for (int j = 0; j < nn; j++)
+                    A[(size_t)i * nn + j] = cdt == QS_F64 ? cd(((double *)eh.data())[(size_t)j * nn + i], 0)
+                                                          : cd(((double *)eh.data())[2 * ((size_t)j * nn + i)],
+                                                               ((double *)eh.data())[2 * ((size_t)j * nn + i) + 1]);
+            std::vector<double> w;
+            std::vector<cd> Vv;
+            herm_eig(nn, A, w, Vv);
+            std::vector<int> idx(nn);
+            for (int i = 0; i < nn; i++) idx[i] = i;
+            auto key = [&](int i) {
+                switch (which) {
+                case QS_WHICH_LA: return -w[i];
+                case QS_WHICH_SM: return std::fabs(w[i]);
+                case QS_WHICH_LM: return -std::fabs(w[i]);
+                default: return w[i];
+                }
+            };
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b2) { return key(a) < key(b2); });
+            if (which == QS_WHICH_BE) {
+                std::vector<int> be;
+                for (int i = 0; i < k / 2; i++) be.push_back(i);
+                for (int i = 0; i < (k + 1) / 2; i++) be.push_back(nn - 1 - i);
+                idx = be;
+            }
+            for (int i = 0; i < k; i++) vals[i] = w[idx[i]];
+            if (vecs) {
+                std::vector<char> vh((size_t)n * k * esz);
+                for (int c = 0; c < k; c++)
+                    for (int r = 0; r < nn; r++) {
+                        cd z = Vv[(size_t)r * nn + idx[c]];
+                        size_t off = (size_t)c * n + r;
+                        switch (dtype) {
+                        case QS_F64: ((double *)vh.data())[off] = z.real(); break;
+                        case QS_F32: ((float *)vh.data())[off] = (float)z.real(); break;
+                        case QS_C128: ((double *)vh.data())[2 * off] = z.real(); ((double *)vh.data())[2 * off + 1] = z.imag(); break;
+                        case QS_C64: ((float *)vh.data())[2 * off] = (float)z.real(); ((float *)vh.data())[2 * off + 1] = (float)z.imag(); break;
+                        }
+                    }
+                memcpy(vecs, vh.data(), vh.size());
+            }
+            if (nconv) *nconv = k;
+            if (n_matvec) *n_matvec = nn;
+            return;
+        }
+        eigsh_device(o, dtype, k, which, tol, maxiter, ncv, v0 ? dv0.get() : nullptr, res, vecs ? dvec.get() : nullptr);
+        for (int i = 0; i < k; i++) vals[i] = res.vals[i];
+        if (vecs) QS_HIP(hipMemcpy(vecs, dvec.get(), (size_t)n * k * esz, hipMemcpyDeviceToHost));
+        if (nconv) *nconv = res.nconv;
+        if (n_matvec) *n_matvec = res.nmatvec;
+        if (res.nconv < k) fail(QS_ENOCONV, "eigsh: only " + std::to_string(res.nconv) + " of " + std::to_string(k) +
+                                                " Ritz values converged within maxiter");
+    });
+}
+
+int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const double *scales, double tol,
+                     int krylov_dim, void *out, int *n_matvec) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(v, "v");
+        check_ptr(out, "out");
+        OpHost &o = op->o;
+        hipStream_t s = ensure_stream(o);
+        const int64_t n = o.basis->Ns;
+        const size_t esz = dtype_size(dtype);
+        std::vector<cd> sc((size_t)n_times);
+        for (int64_t j = 0; j < n_times; j++) sc[j] = cd(scales[2 * j], scales[2 * j + 1]);
+        DevBuf<char> dv((size_t)n * esz), dout((size_t)n * std::max<int64_t>(1, n_times) * esz);
+        QS_HIP(hipMemcpyAsync(dv.get(), v, (size_t)n * esz, hipMemcpyHostToDevice, s));
+        int nmv = 0;
+        expm_times_device(o, dtype, dv.get(), sc, tol, krylov_dim, dout.get(), nmv);
+        QS_HIP(hipMemcpy(out, dout.get(), (size_t)n * n_times * esz, hipMemcpyDeviceToHost));
+        if (n_matvec) *n_matvec = nmv;
+    });
+}
+
+}  // extern "C"

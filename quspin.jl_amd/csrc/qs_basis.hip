@@ -1,0 +1,236 @@
+// qs_basis.hip -- GPU enumeration of spin-1/2 symmetry sectors.
+//
+// Reference: SpinBasis1D (basis.jl:622-777) enumerates the nup sector with
+// Gosper's hack (basis.jl:229-250), builds an explicit projector stage by
+// stage (symmetry_basis.jl:57-213) and orders columns by ascending orbit
+// minimum.  Here each lane unranks a chunk of parent states (combinadic),
+// walks it with Gosper's successor, keeps a state when it is the minimum of
+// its orbit under the whole group and the sector character is trivial on its
+// stabiliser, and a two-pass compaction (count, scan, write) emits the kept
+// orbit minima in ascending order -- the reference's column order.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "qs_device.h"
+
+namespace qs {
+
+namespace {
+
+constexpr int kChunk = 32;      // parent states per lane
+constexpr int kBlock = 256;
+
+template <typename S>
+__device__ __forceinline__ S gosper_next(S s) {
+    S lowest = s & (~s + 1);
+    S next = s + lowest;
+    return next | ((next ^ s) >> (ctz(lowest) + 2));
+}
+
+// Is u the orbit minimum with a character that is trivial on its stabiliser?
+// Returns the stabiliser size (0 if u is not a kept representative).
+template <typename S>
+__device__ int keep_rep(const GroupDev &g, S u, S mask) {
+    for (int p = 0; p < g.npoint; p++) {
+        S v = point_apply(g, p, u);
+        int sh = 0;
+        for (int t = 0; t < g.nT; t++) {
+            if (rotl_bits(v, sh, g.L, mask) < u) return 0;
+            sh += g.a;
+        }
+    }
+    int cnt = 0;
+    double cre = 0, cim = 0;
+    for (int p = 0; p < g.npoint; p++) {
+        S v = point_apply(g, p, u);
+        int sh = 0;
+        for (int t = 0; t < g.nT; t++) {
+            if (rotl_bits(v, sh, g.L, mask) == u) {
+                cnt++;
+                double lr = g.lam_re[t], li = g.lam_im[t];
+                double pr = g.pch_re[p], pi = g.pch_im[p];
+                cre += lr * pr - li * pi;
+                cim += lr * pi + li * pr;
+            }
+            sh += g.a;
+        }
+    }
+    return (cre * cre + cim * cim > 0.25) ? cnt : 0;
+}
+
+template <typename S>
+__device__ __forceinline__ S parent_state(int64_t idx, int L, int nup, const uint64_t *binom) {
+    if (nup < 0) return (S)idx;
+    return comb_unrank<S, uint64_t>(idx, L, nup, binom);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_count(GroupDev g, int nup, int64_t npar, S mask,
+                                                  const uint64_t *binom, uint32_t *block_count) {
+    __shared__ uint32_t red[kBlock];
+    int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    uint32_t cnt = 0;
+    if (base < npar) {
+        S u = parent_state<S>(base, g.L, nup, binom);
+        int64_t end = base + kChunk < npar ? base + kChunk : npar;
+        for (int64_t i = base; i < end; i++) {
+            if (keep_rep(g, u, mask)) cnt++;
+            if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : (S)(u + 1);
+        }
+    }
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_count[blockIdx.x] = red[0];
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_write(GroupDev g, int nup, int64_t npar, S mask,
+                                                  const uint64_t *binom, const uint64_t *block_offset,
+                                                  S *reps, uint32_t *orbit, double *inv_nrm) {
+    __shared__ uint32_t scan[kBlock];
+    int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    uint32_t cnt = 0;
+    S u0 = 0;
+    int64_t end = base;
+    if (base < npar) {
+        u0 = parent_state<S>(base, g.L, nup, binom);
+        end = base + kChunk < npar ? base + kChunk : npar;
+        S u = u0;
+        for (int64_t i = base; i < end; i++) {
+            if (keep_rep(g, u, mask)) cnt++;
+            if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : (S)(u + 1);
+        }
+    }
+    // exclusive scan of lane counts in lane order
+    scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t pos = block_offset[blockIdx.x] + scan[threadIdx.x] - cnt;
+    if (cnt == 0) return;
+    const int order = g.nT * g.npoint;
+    S u = u0;
+    for (int64_t i = base; i < end; i++) {
+        int st = keep_rep(g, u, mask);
+        if (st) {
+            uint32_t d = (uint32_t)(order / st);
+            reps[pos] = u;
+            orbit[pos] = d;
+            inv_nrm[pos] = 1.0 / sqrt((double)d);
+            pos++;
+        }
+        if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : (S)(u + 1);
+    }
+}
+
+template <typename S>
+__global__ void k_samples(const S *reps, int64_t n, int shift, int64_t nsample, S *samp) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsample) samp[i] = reps[i << shift];
+}
+
+GroupDev to_dev(const GroupHost &h) {
+    GroupDev d;
+    memset(&d, 0, sizeof d);
+    d.L = h.L;
+    d.a = h.a;
+    d.nT = h.nT;
+    d.npoint = h.npoint;
+    for (int p = 0; p < h.npoint; p++) {
+        d.prev[p] = h.prev[p];
+        d.pflip[p] = h.pflip[p];
+        d.pch_re[p] = h.pchar[p].real();
+        d.pch_im[p] = h.pchar[p].imag();
+    }
+    for (int t = 0; t < h.nT; t++) {
+        d.lam_re[t] = h.lam[t].real();
+        d.lam_im[t] = h.lam[t].imag();
+    }
+    return d;
+}
+
+template <typename S>
+void build_stored(BasisHost &b, int64_t npar) {
+    GroupDev g = to_dev(b.group);
+    S mask = (S)full_mask(b.L);
+    int64_t lanes = (npar + kChunk - 1) / kChunk;
+    int64_t nblk = (lanes + kBlock - 1) / kBlock;
+    if (nblk > 0x7fffffff) fail(QS_EARG, "sector too large");
+    DevBuf<uint32_t> counts((size_t)nblk);
+    hipLaunchKernelGGL((k_count<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
+                       b.binom.get(), counts.get());
+    QS_HIP(hipGetLastError());
+    std::vector<uint32_t> hc((size_t)nblk);
+    QS_HIP(hipMemcpy(hc.data(), counts.get(), sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> off((size_t)nblk);
+    uint64_t total = 0;
+    for (int64_t i = 0; i < nblk; i++) {
+        off[i] = total;
+        total += hc[i];
+    }
+    if (total == 0) fail(QS_EARG, "the requested symmetry sector is empty");
+    b.Ns = (int64_t)total;
+    DevBuf<uint64_t> d_off((size_t)nblk);
+    QS_HIP(hipMemcpy(d_off.get(), off.data(), sizeof(uint64_t) * nblk, hipMemcpyHostToDevice));
+    DevBuf<S> reps((size_t)total);
+    b.orbit.alloc((size_t)total);
+    b.inv_nrm.alloc((size_t)total);
+    hipLaunchKernelGGL((k_write<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
+                       b.binom.get(), d_off.get(), reps.get(), b.orbit.get(), b.inv_nrm.get());
+    QS_HIP(hipGetLastError());
+    // sampled search table: at most 4096 samples
+    int shift = 0;
+    while (((total + (1ULL << shift) - 1) >> shift) > 4096) shift++;
+    b.sample_shift = shift;
+    b.nsample = (int64_t)((total + (1ULL << shift) - 1) >> shift);
+    DevBuf<S> samp((size_t)b.nsample);
+    hipLaunchKernelGGL((k_samples<S>), dim3((unsigned)((b.nsample + 255) / 256)), dim3(256), 0, 0,
+                       reps.get(), (int64_t)total, shift, b.nsample, samp.get());
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipDeviceSynchronize());
+    if constexpr (sizeof(S) == 4) {
+        b.reps32 = std::move(reps);
+        b.samp32 = std::move(samp);
+    } else {
+        b.reps64 = std::move(reps);
+        b.samp64 = std::move(samp);
+    }
+}
+
+}  // namespace
+
+void build_basis_device(BasisHost &b) {
+    // binomial table C(n, k), n < 64, k <= 64, at [n * 65 + k]
+    std::vector<uint64_t> tab(64 * 65, 0);
+    for (int n = 0; n < 64; n++) {
+        tab[n * 65 + 0] = 1;
+        for (int k = 1; k <= n; k++) tab[n * 65 + k] = tab[(n - 1) * 65 + k - 1] + (k <= n - 1 ? tab[(n - 1) * 65 + k] : 0);
+    }
+    b.binom.alloc(tab.size());
+    QS_HIP(hipMemcpy(b.binom.get(), tab.data(), sizeof(uint64_t) * tab.size(), hipMemcpyHostToDevice));
+    b.use64 = b.L > 32;
+    int64_t npar = b.nup >= 0 ? binom(b.L, b.nup) : (int64_t)1 << b.L;
+    if (b.mode == MODE_FULL) {
+        b.Ns = npar;
+        return;
+    }
+    if (b.mode == MODE_ZONLY) {
+        // canonical states are the ones with bit L-1 clear: C(L-1, nup) of them
+        b.Ns = b.nup >= 0 ? binom(b.L - 1, b.nup) : (int64_t)1 << (b.L - 1);
+        if (b.Ns == 0) fail(QS_EARG, "the requested symmetry sector is empty");
+        return;
+    }
+    if (b.use64) build_stored<uint64_t>(b, npar);
+    else build_stored<uint32_t>(b, npar);
+}
+
+}  // namespace qs

@@ -1,0 +1,122 @@
+// qs_device.h -- device helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "qs_internal.h"
+
+namespace qs {
+
+// ---------------------------------------------------------- value types ---
+struct cplx_d { double re, im; };
+struct cplx_f { float re, im; };
+
+template <typename V> struct VTraits;
+template <> struct VTraits<double> {
+    static constexpr bool cplx = false;
+    __device__ static inline void load(const double *p, int64_t i, double &re, double &im) { re = p[i]; im = 0; }
+    __device__ static inline void store(double *p, int64_t i, double re, double) { p[i] = re; }
+};
+template <> struct VTraits<float> {
+    static constexpr bool cplx = false;
+    __device__ static inline void load(const float *p, int64_t i, double &re, double &im) { re = (double)p[i]; im = 0; }
+    __device__ static inline void store(float *p, int64_t i, double re, double) { p[i] = (float)re; }
+};
+template <> struct VTraits<cplx_d> {
+    static constexpr bool cplx = true;
+    __device__ static inline void load(const cplx_d *p, int64_t i, double &re, double &im) {
+        cplx_d v = p[i];
+        re = v.re;
+        im = v.im;
+    }
+    __device__ static inline void store(cplx_d *p, int64_t i, double re, double im) { p[i] = cplx_d{re, im}; }
+};
+template <> struct VTraits<cplx_f> {
+    static constexpr bool cplx = true;
+    __device__ static inline void load(const cplx_f *p, int64_t i, double &re, double &im) {
+        cplx_f v = p[i];
+        re = (double)v.re;
+        im = (double)v.im;
+    }
+    __device__ static inline void store(cplx_f *p, int64_t i, double re, double im) {
+        p[i] = cplx_f{(float)re, (float)im};
+    }
+};
+
+// ------------------------------------------------------------ bit helpers --
+__device__ __forceinline__ uint32_t rev_bits(uint32_t u, int L) { return __builtin_bitreverse32(u) >> (32 - L); }
+__device__ __forceinline__ uint64_t rev_bits(uint64_t u, int L) { return __builtin_bitreverse64(u) >> (64 - L); }
+__device__ __forceinline__ int popc(uint32_t u) { return __builtin_popcount(u); }
+__device__ __forceinline__ int popc(uint64_t u) { return __builtin_popcountll(u); }
+__device__ __forceinline__ int ctz(uint32_t u) { return __builtin_ctz(u); }
+__device__ __forceinline__ int ctz(uint64_t u) { return __builtin_ctzll(u); }
+
+// rotl within L bits; k in [0, L)
+__device__ __forceinline__ uint32_t rotl_bits(uint32_t u, int k, int L, uint32_t mask) {
+    uint64_t dd = ((uint64_t)u << L) | u;
+    return (uint32_t)(dd >> (L - k)) & mask;
+}
+__device__ __forceinline__ uint64_t rotl_bits(uint64_t u, int k, int L, uint64_t mask) {
+    return ((u << k) | (u >> (L - k))) & mask;
+}
+
+template <typename S>
+__device__ __forceinline__ S point_apply(const GroupDev &g, int p, S u) {
+    S v = g.prev[p] ? rev_bits(u, g.L) : u;
+    return v ^ (S)g.pflip[p];
+}
+
+// Orbit minimum of u under {T^t o X_p}; returns the (t, p) reaching it.
+template <typename S, bool TRACK>
+__device__ __forceinline__ S canonical(const GroupDev &g, S u, S mask, int &bt, int &bp) {
+    S best = (S)~(S)0;
+    bt = 0;
+    bp = 0;
+    for (int p = 0; p < g.npoint; p++) {
+        S v = point_apply(g, p, u);
+        int sh = 0;
+        for (int t = 0; t < g.nT; t++) {
+            S w = rotl_bits(v, sh, g.L, mask);
+            if (TRACK) {
+                if (w < best) { best = w; bt = t; bp = p; }
+            } else {
+                best = w < best ? w : best;
+            }
+            sh += g.a;
+        }
+    }
+    return best;
+}
+
+// Combinadic (colex) rank of a word with w set bits: sum_k C(p_k, k).
+// binom is laid out as C(n, k) at [n * 65 + k].
+template <typename S, typename B>
+__device__ __forceinline__ int64_t comb_rank(S u, const B *binom) {
+    int64_t r = 0;
+    int k = 1;
+    while (u) {
+        int p = ctz(u);
+        r += (int64_t)binom[p * 65 + k];
+        k++;
+        u &= u - 1;
+    }
+    return r;
+}
+
+template <typename S, typename B>
+__device__ __forceinline__ S comb_unrank(int64_t i, int L, int w, const B *binom) {
+    S u = 0;
+    int p = L - 1;
+    for (int k = w; k >= 1; k--) {
+        while ((int64_t)binom[p * 65 + k] > i) p--;
+        u |= (S)1 << p;
+        i -= (int64_t)binom[p * 65 + k];
+        p--;
+    }
+    return u;
+}
+
+}  // namespace qs

@@ -1,0 +1,626 @@
+// qs_host.cpp -- host-side logic of libqsgpu: symmetry groups, operator-term
+// compilation, group averaging (the P'HP semantics), and small dense FP64
+// linear algebra for the Krylov solvers.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+#include <tuple>
+
+#include "qs_internal.h"
+
+namespace qs {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error(code, msg); }
+const std::string &last_error() { return g_last_error; }
+
+uint64_t full_mask(int L) { return L >= 64 ? ~0ULL : ((1ULL << L) - 1); }
+
+uint64_t bit_reverse(uint64_t u, int L)
+{
+    uint64_t r = 0;
+    for (int i = 0; i < L; i++)
+        if (u >> i & 1) r |= 1ULL << (L - 1 - i);
+    return r;
+}
+
+uint64_t rotl(uint64_t u, int k, int L)
+{
+    k %= L;
+    if (k < 0) k += L;
+    if (k == 0) return u;
+    return ((u << k) | (u >> (L - k))) & full_mask(L);
+}
+
+uint64_t apply_point(const GroupHost &g, int p, uint64_t u)
+{
+    uint64_t v = g.prev[p] ? bit_reverse(u, g.L) : u;
+    return v ^ g.pflip[p];
+}
+
+int64_t binom(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    if (k > n - k) k = n - k;
+    int64_t r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+// --------------------------------------------------------------- groups --
+namespace {
+
+// A group element as an affine map on bits: u -> perm(u) ^ c.
+struct Affine {
+    std::vector<int> perm;   // bit i -> bit perm[i]
+    uint64_t c = 0;
+    bool operator<(const Affine &o) const { return std::tie(perm, c) < std::tie(o.perm, o.c); }
+    bool operator==(const Affine &o) const { return perm == o.perm && c == o.c; }
+};
+
+uint64_t permute_bits(const std::vector<int> &perm, uint64_t u)
+{
+    uint64_t r = 0;
+    for (size_t i = 0; i < perm.size(); i++)
+        if (u >> i & 1) r |= 1ULL << perm[i];
+    return r;
+}
+
+Affine compose(const Affine &g1, const Affine &g2)   // g1 o g2
+{
+    Affine r;
+    r.perm.resize(g1.perm.size());
+    for (size_t i = 0; i < g1.perm.size(); i++) r.perm[i] = g1.perm[g2.perm[i]];
+    r.c = permute_bits(g1.perm, g2.c) ^ g1.c;
+    return r;
+}
+
+Affine identity_map(int L)
+{
+    Affine a;
+    a.perm.resize(L);
+    for (int i = 0; i < L; i++) a.perm[i] = i;
+    return a;
+}
+
+bool close_to(cd a, cd b) { return std::abs(a - b) < 1e-9; }
+
+cd exact_root(int k, int n)
+{
+    // exp(2 pi i k / n) with exact values at the real and imaginary axes
+    k %= n;
+    if (k < 0) k += n;
+    if (k == 0) return cd(1, 0);
+    if (2 * k == n) return cd(-1, 0);
+    if (4 * k == n) return cd(0, 1);
+    if (4 * k == 3 * n) return cd(0, -1);
+    double ang = 2.0 * M_PI * (double)k / (double)n;
+    return cd(std::cos(ang), std::sin(ang));
+}
+
+}  // namespace
+
+void build_group(BasisHost &b)
+{
+    const int L = b.L;
+    const int kb = b.blocks[0];
+    const char *names[6] = {"kblock", "pblock", "zblock", "pzblock", "zAblock", "zBblock"};
+    bool requested = false;
+    for (int i = 0; i < 6; i++) requested |= b.blocks[i] != QS_UNSET;
+    GroupHost &g = b.group;
+    g = GroupHost();
+    g.L = L;
+    g.a = b.a;
+    g.lam[0] = cd(1, 0);
+    g.pchar[0] = cd(1, 0);
+    if (!requested) {
+        b.mode = MODE_FULL;
+        return;
+    }
+    // basis.jl:686-687, 717-723, 744-745
+    if (!(b.a > 0 && L % b.a == 0)) fail(QS_EARG, "a must be a positive divisor of L");
+    const int order = L / b.a;
+    if (b.blocks[1] != QS_UNSET && kb != QS_UNSET) {
+        int m = ((kb % order) + order) % order;
+        if (!(m == 0 || (order % 2 == 0 && m == order / 2)))
+            fail(QS_EARG, "pblock can be combined with kblock only at k=0 or pi");
+    }
+    for (int i = 1; i < 6; i++)
+        if (b.blocks[i] != QS_UNSET && b.blocks[i] != 1 && b.blocks[i] != -1)
+            fail(QS_EARG, std::string(names[i]) + " must be +1 or -1");
+    // generators that must keep the particle sector (symmetry_basis.jl:48-50,79-81)
+    uint64_t odd = 0, even = 0;
+    for (int site = 1; site <= L; site += 2) odd |= 1ULL << (site - 1);
+    for (int site = 2; site <= L; site += 2) even |= 1ULL << (site - 1);
+    if (b.nup >= 0) {
+        bool z_ok = 2 * b.nup == L;
+        if ((b.blocks[2] != QS_UNSET || b.blocks[3] != QS_UNSET) && !z_ok)
+            fail(QS_EARG, "symmetry transformation leaves the selected particle sector");
+        if (b.blocks[4] != QS_UNSET || b.blocks[5] != QS_UNSET)
+            fail(QS_EARG, "symmetry transformation leaves the selected particle sector");
+    }
+    if ((b.blocks[4] != QS_UNSET || b.blocks[5] != QS_UNSET) && kb != QS_UNSET && (b.a % 2 == 1))
+        fail(QS_EARG, "zAblock/zBblock combined with kblock need an even translation step a");
+
+    // generators with their characters
+    std::vector<std::pair<Affine, cd>> gens;
+    Affine rev = identity_map(L);
+    for (int i = 0; i < L; i++) rev.perm[i] = L - 1 - i;
+    if (kb != QS_UNSET) {
+        Affine t = identity_map(L);
+        for (int i = 0; i < L; i++) t.perm[i] = (i + b.a) % L;
+        gens.push_back({t, exact_root(kb, order)});
+    }
+    if (b.blocks[1] != QS_UNSET) gens.push_back({rev, cd(b.blocks[1], 0)});
+    if (b.blocks[2] != QS_UNSET) {
+        Affine z = identity_map(L);
+        z.c = full_mask(L);
+        gens.push_back({z, cd(b.blocks[2], 0)});
+    }
+    if (b.blocks[3] != QS_UNSET) {
+        Affine pz = rev;
+        pz.c = full_mask(L);
+        gens.push_back({pz, cd(b.blocks[3], 0)});
+    }
+    if (b.blocks[4] != QS_UNSET) {
+        Affine za = identity_map(L);
+        za.c = odd;
+        gens.push_back({za, cd(b.blocks[4], 0)});
+    }
+    if (b.blocks[5] != QS_UNSET) {
+        Affine zb = identity_map(L);
+        zb.c = even;
+        gens.push_back({zb, cd(b.blocks[5], 0)});
+    }
+    // closure by breadth-first search with character bookkeeping
+    std::map<Affine, cd> chi;
+    std::vector<Affine> queue{identity_map(L)};
+    chi[queue[0]] = cd(1, 0);
+    bool consistent = true;
+    for (size_t qi = 0; qi < queue.size(); qi++) {
+        Affine cur = queue[qi];
+        cd cc = chi[cur];
+        for (auto &gg : gens) {
+            Affine nx = compose(gg.first, cur);
+            cd nc = gg.second * cc;
+            auto it = chi.find(nx);
+            if (it == chi.end()) {
+                chi[nx] = nc;
+                queue.push_back(nx);
+                if (queue.size() > 4096) fail(QS_EARG, "symmetry group too large");
+            } else if (!close_to(it->second, nc)) {
+                consistent = false;
+            }
+        }
+    }
+    if (!consistent) fail(QS_EARG, "the requested symmetry sector is empty");
+
+    // decompose every element as T^t o X_p
+    g.nT = (kb != QS_UNSET) ? order : 1;
+    for (int t = 0; t < g.nT; t++) g.lam[t] = (kb != QS_UNSET) ? exact_root((int64_t)kb * t % order, order) : cd(1, 0);
+    std::vector<std::tuple<int, uint64_t, cd>> points;
+    for (auto &kv : chi) {
+        const Affine &e = kv.first;
+        bool placed = false;
+        for (int t = 0; t < g.nT && !placed; t++) {
+            int sh = t * b.a;
+            bool is_id = true, is_rev = true;
+            for (int i = 0; i < L; i++) {
+                int p = ((e.perm[i] - sh) % L + L) % L;
+                is_id &= p == i;
+                is_rev &= p == L - 1 - i;
+            }
+            if (!is_id && !is_rev) continue;
+            placed = true;
+            if (t != 0) continue;   // point ops are the t = 0 representatives
+            int r = is_id ? 0 : 1;
+            uint64_t f = e.c;
+            points.emplace_back(r, f, kv.second);
+        }
+        if (!placed) fail(QS_EARG, "unsupported symmetry combination");
+    }
+    if ((int)points.size() > kMaxPoint) fail(QS_EARG, "symmetry group too large");
+    // The kernels run over the pairs (t, p); when two pairs are the same map
+    // (tiny L) their characters must agree, otherwise the projection is zero.
+    for (auto &pt : points)
+        for (int t = 0; t < g.nT; t++) {
+            Affine x = identity_map(L);
+            if (std::get<0>(pt))
+                for (int i = 0; i < L; i++) x.perm[i] = L - 1 - i;
+            x.c = std::get<1>(pt);
+            Affine tr = identity_map(L);
+            for (int i = 0; i < L; i++) tr.perm[i] = (i + t * b.a) % L;
+            Affine e = compose(tr, x);
+            cd want = (kb != QS_UNSET ? exact_root((int64_t)kb * t % order, order) : cd(1, 0)) * std::get<2>(pt);
+            auto it = chi.find(e);
+            if (it == chi.end() || !close_to(it->second, want))
+                fail(QS_EARG, "the requested symmetry sector is empty");
+        }
+    std::sort(points.begin(), points.end(), [](auto &x, auto &y) {
+        return std::make_pair(std::get<0>(x), std::get<1>(x)) < std::make_pair(std::get<0>(y), std::get<1>(y));
+    });
+    g.npoint = (int)points.size();
+    for (int p = 0; p < g.npoint; p++) {
+        g.prev[p] = std::get<0>(points[p]);
+        g.pflip[p] = std::get<1>(points[p]);
+        g.pchar[p] = std::get<2>(points[p]);
+    }
+    b.is_complex = 0;
+    for (auto &kv : chi)
+        if (std::abs(kv.second.imag()) > 1e-12) b.is_complex = 1;
+    bool zonly = b.blocks[0] == QS_UNSET && b.blocks[1] == QS_UNSET && b.blocks[2] != QS_UNSET &&
+                 b.blocks[3] == QS_UNSET && b.blocks[4] == QS_UNSET && b.blocks[5] == QS_UNSET;
+    if (zonly) {
+        b.mode = MODE_ZONLY;
+        b.zchar = b.blocks[2];
+    } else {
+        b.mode = MODE_STORED;
+    }
+}
+
+// ----------------------------------------------------------------- terms --
+namespace {
+
+struct LocalAct {
+    bool alive;
+    int out;
+    cd amp;
+};
+
+LocalAct local_op(char op, int b, int pauli)
+{
+    // operators.jl:343-379
+    double sz = pauli ? 1.0 : 0.5, spm = pauli ? 2.0 : 1.0;
+    switch (op) {
+    case 'I': return {true, b, cd(1, 0)};
+    case 'z': return {true, b, cd(b ? sz : -sz, 0)};
+    case '+': return b ? LocalAct{false, b, cd(0, 0)} : LocalAct{true, 1, cd(spm, 0)};
+    case '-': return b ? LocalAct{true, 0, cd(spm, 0)} : LocalAct{false, b, cd(0, 0)};
+    case 'x': return {true, 1 - b, cd(sz, 0)};
+    case 'y': return {true, 1 - b, b ? cd(0, -sz) : cd(0, sz)};
+    }
+    fail(QS_EARG, std::string("unsupported spin operator '") + op + "'");
+}
+
+}  // namespace
+
+std::vector<CTerm> compile_couplings(const std::vector<Coupling> &cs, int L, int pauli)
+{
+    std::vector<CTerm> out;
+    for (const Coupling &c : cs) {
+        if (c.op.empty()) fail(QS_EARG, "operator string cannot be empty");
+        if (c.sites.size() != c.op.size()) fail(QS_EARG, "operator arity and coupling sites differ");
+        for (int s : c.sites)
+            if (s < 1 || s > L)
+                fail(QS_EARG, "site " + std::to_string(s) + " lies outside 1:" + std::to_string(L));
+        for (char ch : c.op) local_op(ch, 0, pauli);   // validates the characters
+        CTerm t{0, 0, 0, 0, c.coef};
+        bool dead = false;
+        std::set<int> seen;
+        for (size_t i = 0; i < c.sites.size() && !dead; i++) {
+            int site = c.sites[i];
+            if (seen.count(site)) continue;
+            seen.insert(site);
+            LocalAct res[2];
+            for (int b0 = 0; b0 < 2; b0++) {
+                LocalAct st{true, b0, cd(1, 0)};
+                for (int j = (int)c.op.size() - 1; j >= 0 && st.alive; j--) {
+                    if (c.sites[j] != site) continue;
+                    LocalAct la = local_op(c.op[j], st.out, pauli);
+                    if (!la.alive) { st.alive = false; break; }
+                    st.out = la.out;
+                    st.amp *= la.amp;
+                }
+                res[b0] = st;
+            }
+            uint64_t bit = 1ULL << (site - 1);
+            if (!res[0].alive && !res[1].alive) { dead = true; break; }
+            if (res[0].alive && res[1].alive) {
+                bool flip = res[0].out == 1;
+                if (flip != (res[1].out == 0)) fail(QS_EARG, "non-monomial local operator");
+                if (flip) t.flip |= bit;
+                cd r = res[1].amp / res[0].amp;
+                if (std::abs(r - cd(1, 0)) < 1e-15) {
+                } else if (std::abs(r + cd(1, 0)) < 1e-15) {
+                    t.smask |= bit;
+                } else {
+                    fail(QS_EARG, "unsupported local operator product");
+                }
+                t.A *= res[0].amp;
+            } else {
+                int v = res[0].alive ? 0 : 1;
+                t.cmask |= bit;
+                if (v) t.cval |= bit;
+                if (res[v].out != v) t.flip |= bit;
+                t.A *= res[v].amp;
+            }
+        }
+        if (dead || t.A == cd(0, 0)) continue;
+        out.push_back(t);
+    }
+    return out;
+}
+
+std::vector<CTerm> consolidate_cterms(const std::vector<CTerm> &t)
+{
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t, uint64_t>, size_t> idx;
+    std::vector<CTerm> out;
+    for (const CTerm &x : t) {
+        auto key = std::make_tuple(x.flip, x.cmask, x.cval, x.smask);
+        auto it = idx.find(key);
+        if (it == idx.end()) {
+            idx[key] = out.size();
+            out.push_back(x);
+        } else {
+            out[it->second].A += x.A;
+        }
+    }
+    std::vector<CTerm> kept;
+    for (auto &x : out)
+        if (x.A != cd(0, 0)) kept.push_back(x);
+    return kept;
+}
+
+namespace {
+
+// g O g^-1 for g(u) = pi(u) ^ f with pi = rotl(rev^r(.), sh)
+CTerm conjugate_term(const CTerm &t, int r, uint64_t fx, int sh, int L)
+{
+    auto pi = [&](uint64_t v) { return rotl(r ? bit_reverse(v, L) : v, sh, L); };
+    uint64_t f = rotl(fx, sh, L);
+    CTerm o;
+    o.flip = pi(t.flip);
+    o.cmask = pi(t.cmask);
+    o.cval = pi(t.cval) ^ (f & o.cmask);
+    o.smask = pi(t.smask);
+    o.A = t.A;
+    if (__builtin_popcountll(f & o.smask) & 1) o.A = -o.A;
+    return o;
+}
+
+bool same_dict(const std::vector<CTerm> &a, const std::vector<CTerm> &b)
+{
+    if (a.size() != b.size()) return false;
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t, uint64_t>, cd> m;
+    for (auto &x : a) m[std::make_tuple(x.flip, x.cmask, x.cval, x.smask)] = x.A;
+    for (auto &x : b) {
+        auto it = m.find(std::make_tuple(x.flip, x.cmask, x.cval, x.smask));
+        if (it == m.end() || it->second != x.A) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+std::vector<CTerm> symmetrize_cterms(const std::vector<CTerm> &t, const BasisHost &b)
+{
+    const GroupHost &g = b.group;
+    if (b.mode == MODE_FULL) return consolidate_cterms(t);
+    std::vector<CTerm> base = consolidate_cterms(t);
+    bool invariant = true;
+    for (int p = 0; p < g.npoint && invariant; p++)
+        for (int tt = 0; tt < g.nT && invariant; tt++) {
+            std::vector<CTerm> img;
+            for (auto &x : base) img.push_back(conjugate_term(x, g.prev[p], g.pflip[p], tt * g.a, g.L));
+            invariant = same_dict(consolidate_cterms(img), base);
+        }
+    if (invariant) return base;
+    // P'HP = P' H_avg P with H_avg the group average of H (the character
+    // drops out), so the projected action of a non-invariant term list is the
+    // action of its average.
+    std::vector<CTerm> all;
+    double w = 1.0 / (double)g.order();
+    for (int p = 0; p < g.npoint; p++)
+        for (int tt = 0; tt < g.nT; tt++)
+            for (auto &x : base) {
+                CTerm y = conjugate_term(x, g.prev[p], g.pflip[p], tt * g.a, g.L);
+                y.A *= w;
+                all.push_back(y);
+            }
+    return consolidate_cterms(all);
+}
+
+void split_terms(const std::vector<CTerm> &all, std::vector<CTerm> &diag, std::vector<CTerm> &off,
+                 std::vector<GroupOfTerms> &groups)
+{
+    diag.clear();
+    off.clear();
+    groups.clear();
+    std::vector<uint64_t> order;
+    std::map<uint64_t, std::vector<CTerm>> by;
+    for (auto &x : all) {
+        if (x.flip == 0) {
+            diag.push_back(x);
+            continue;
+        }
+        if (!by.count(x.flip)) order.push_back(x.flip);
+        by[x.flip].push_back(x);
+    }
+    for (uint64_t f : order) {
+        GroupOfTerms gr;
+        gr.flip = f;
+        gr.t0 = (int)off.size();
+        for (auto &x : by[f]) off.push_back(x);
+        gr.t1 = (int)off.size();
+        groups.push_back(gr);
+    }
+}
+
+// ---------------------------------------------------- dense eigensolvers --
+// Cyclic Jacobi for a real symmetric n x n matrix (row-major A, destroyed).
+// Eigenvalues ascending in w, eigenvectors in the columns of V (row-major).
+void sym_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V)
+{
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
+    auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double v = a(i, j) * a(i, j);
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-34 * tot || off == 0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = a(p, q);
+                if (apq == 0) continue;
+                double app = a(p, p), aqq = a(q, q);
+                double theta = (aqq - app) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = a(k, p), akq = a(k, q);
+                    a(k, p) = c * akp - s * akq;
+                    a(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = a(p, k), aqk = a(q, k);
+                    a(p, k) = c * apk - s * aqk;
+                    a(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = c * vkp - s * vkq;
+                    V[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+    w.resize(n);
+    std::vector<double> V2((size_t)n * n);
+    for (int j = 0; j < n; j++) {
+        w[j] = a(ord[j], ord[j]);
+        for (int k = 0; k < n; k++) V2[(size_t)k * n + j] = V[(size_t)k * n + ord[j]];
+    }
+    V.swap(V2);
+}
+
+// Complex Hermitian eigenproblem via the real symmetric embedding
+// [Re -Im; Im Re] (each eigenvalue appears twice; one copy is kept).
+void herm_eig(int n, std::vector<cd> &A, std::vector<double> &w, std::vector<cd> &V)
+{
+    int m = 2 * n;
+    std::vector<double> B((size_t)m * m);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            cd z = A[(size_t)i * n + j];
+            B[(size_t)i * m + j] = z.real();
+            B[(size_t)i * m + j + n] = -z.imag();
+            B[(size_t)(i + n) * m + j] = z.imag();
+            B[(size_t)(i + n) * m + j + n] = z.real();
+        }
+    std::vector<double> ww, VV;
+    sym_eig(m, B, ww, VV);
+    // pick n orthonormal complex vectors: take pairs in order, orthogonalise
+    w.clear();
+    V.assign((size_t)n * n, cd(0, 0));
+    int got = 0;
+    for (int j = 0; j < m && got < n; j++) {
+        std::vector<cd> v(n);
+        for (int k = 0; k < n; k++) v[k] = cd(VV[(size_t)k * m + j], VV[(size_t)(k + n) * m + j]);
+        for (int c = 0; c < got; c++) {
+            cd d = 0;
+            for (int k = 0; k < n; k++) d += std::conj(V[(size_t)k * n + c]) * v[k];
+            for (int k = 0; k < n; k++) v[k] -= d * V[(size_t)k * n + c];
+        }
+        double nr = 0;
+        for (int k = 0; k < n; k++) nr += std::norm(v[k]);
+        nr = std::sqrt(nr);
+        if (nr < 0.5) continue;
+        for (int k = 0; k < n; k++) V[(size_t)k * n + got] = v[k] / nr;
+        w.push_back(ww[j]);
+        got++;
+    }
+}
+
+// exp(A) for a small complex matrix: scaling and squaring with the [13/13]
+// Pade approximant (Higham, SIAM J. Matrix Anal. Appl. 26 (2005) 1179).
+void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
+{
+    auto mm = [n](const std::vector<cd> &X, const std::vector<cd> &Y) {
+        std::vector<cd> Z((size_t)n * n, cd(0, 0));
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < n; k++) {
+                cd x = X[(size_t)i * n + k];
+                if (x == cd(0, 0)) continue;
+                for (int j = 0; j < n; j++) Z[(size_t)i * n + j] += x * Y[(size_t)k * n + j];
+            }
+        return Z;
+    };
+    double norm1 = 0;
+    for (int j = 0; j < n; j++) {
+        double s = 0;
+        for (int i = 0; i < n; i++) s += std::abs(A[(size_t)i * n + j]);
+        norm1 = std::max(norm1, s);
+    }
+    const double theta13 = 5.371920351148152;
+    int s = 0;
+    if (norm1 > theta13) s = std::max(0, (int)std::ceil(std::log2(norm1 / theta13)));
+    double scale = std::ldexp(1.0, -s);
+    std::vector<cd> X(A);
+    for (auto &v : X) v *= scale;
+    static const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0,
+                                 1187353796428800.0,  129060195264000.0,   10559470521600.0,
+                                 670442572800.0,      33522128640.0,       1323241920.0,
+                                 40840800.0,          960960.0,            16380.0,
+                                 182.0,               1.0};
+    std::vector<cd> I((size_t)n * n, cd(0, 0));
+    for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1;
+    auto X2 = mm(X, X), X4 = mm(X2, X2), X6 = mm(X4, X2);
+    std::vector<cd> U1((size_t)n * n), V1((size_t)n * n), T((size_t)n * n), W((size_t)n * n);
+    for (size_t i = 0; i < I.size(); i++) {
+        T[i] = b[13] * X6[i] + b[11] * X4[i] + b[9] * X2[i];
+        W[i] = b[12] * X6[i] + b[10] * X4[i] + b[8] * X2[i];
+    }
+    auto T6 = mm(X6, T), W6 = mm(X6, W);
+    for (size_t i = 0; i < I.size(); i++) {
+        U1[i] = T6[i] + b[7] * X6[i] + b[5] * X4[i] + b[3] * X2[i] + b[1] * I[i];
+        V1[i] = W6[i] + b[6] * X6[i] + b[4] * X4[i] + b[2] * X2[i] + b[0] * I[i];
+    }
+    auto U = mm(X, U1);
+    // solve (V - U) E = (V + U)
+    std::vector<cd> P((size_t)n * n), Q((size_t)n * n);
+    for (size_t i = 0; i < I.size(); i++) {
+        P[i] = V1[i] - U[i];
+        Q[i] = V1[i] + U[i];
+    }
+    // Gaussian elimination with partial pivoting on P, applied to Q
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::abs(P[(size_t)r * n + c]) > std::abs(P[(size_t)piv * n + c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < n; j++) {
+                std::swap(P[(size_t)c * n + j], P[(size_t)piv * n + j]);
+                std::swap(Q[(size_t)c * n + j], Q[(size_t)piv * n + j]);
+            }
+        cd d = P[(size_t)c * n + c];
+        for (int r = c + 1; r < n; r++) {
+            cd f = P[(size_t)r * n + c] / d;
+            if (f == cd(0, 0)) continue;
+            for (int j = c; j < n; j++) P[(size_t)r * n + j] -= f * P[(size_t)c * n + j];
+            for (int j = 0; j < n; j++) Q[(size_t)r * n + j] -= f * Q[(size_t)c * n + j];
+        }
+    }
+    for (int c = n - 1; c >= 0; c--) {
+        cd d = P[(size_t)c * n + c];
+        for (int j = 0; j < n; j++) {
+            cd acc = Q[(size_t)c * n + j];
+            for (int k = c + 1; k < n; k++) acc -= P[(size_t)c * n + k] * Q[(size_t)k * n + j];
+            Q[(size_t)c * n + j] = acc / d;
+        }
+    }
+    E = Q;
+    for (int i = 0; i < s; i++) E = mm(E, E);
+}
+
+}  // namespace qs

@@ -1,0 +1,239 @@
+// qs_internal.h -- shared types of libqsgpu (host runtime + gfx950 kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qsgpu.h"
+
+namespace qs {
+
+using cd = std::complex<double>;
+
+// ------------------------------------------------------------ errors ------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+void set_last_error(const std::string &msg);
+
+#define QS_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            ::qs::fail(QS_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// --------------------------------------------------------- device memory --
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(QS_EOOM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *get() const { return p; }
+};
+
+// -------------------------------------------------------------- groups ----
+// A symmetry group element acts on an L-bit state as
+//   g(u) = rotl_L( rev_L^r(u) ^ f , t * a )
+// The group is enumerated as {T^t o X_p : t in [0, nT), p in point ops}.
+constexpr int kMaxPoint = 16;
+constexpr int kMaxRot = 64;
+
+struct GroupHost {
+    int L = 0, a = 1, nT = 1;
+    int npoint = 1;
+    int prev[kMaxPoint] = {0};
+    uint64_t pflip[kMaxPoint] = {0};
+    cd pchar[kMaxPoint];     // character of X_p
+    cd lam[kMaxRot];         // character of T^t
+    int order() const { return nT * npoint; }
+};
+
+// Device-side copy passed by value in kernel arguments.
+struct GroupDev {
+    int L, a, nT, npoint;
+    int prev[kMaxPoint];
+    uint64_t pflip[kMaxPoint];
+    double pch_re[kMaxPoint], pch_im[kMaxPoint];
+    double lam_re[kMaxRot], lam_im[kMaxRot];
+};
+
+// ------------------------------------------------------------- terms ------
+// A compiled operator coupling acting on a basis state s:
+//   alive iff (s & cmask) == cval;  s -> s ^ flip;
+//   amplitude = A * (-1)^popcount(s & smask)
+struct CTerm {
+    uint64_t flip, cmask, cval, smask;
+    cd A;
+};
+
+struct TermDev {
+    uint64_t cmask, cval, smask;
+    double are, aim;
+};
+struct GroupOfTerms {   // off-diagonal terms sharing one flip mask
+    uint64_t flip;
+    int t0, t1;
+};
+
+// Basis modes.
+enum BasisMode : int {
+    MODE_FULL = 0,   // no symmetry: column index = combinadic rank (or the state itself without nup)
+    MODE_ZONLY = 1,  // spin inversion only at nup = L/2 (or no nup): canonical = top bit clear
+    MODE_STORED = 2  // general group: sorted canonical states stored on the device
+};
+
+struct BasisHost {
+    int L = 0, nup = -1, pauli = 1, a = 1;
+    int blocks[6];               // k, p, z, pz, zA, zB (QS_UNSET if absent)
+    int mode = MODE_FULL;
+    int is_complex = 0;          // sector phases are complex
+    int use64 = 0;               // states need 64-bit words
+    int64_t Ns = 0;
+    int zchar = 1;               // MODE_ZONLY: character of the inversion
+    GroupHost group;
+    // MODE_STORED device arrays
+    DevBuf<uint32_t> reps32;
+    DevBuf<uint64_t> reps64;
+    DevBuf<double> inv_nrm;      // 1/sqrt(orbit size)
+    DevBuf<uint32_t> orbit;      // orbit size per column
+    // search acceleration: every 2^sample_shift-th representative
+    int sample_shift = 0;
+    int64_t nsample = 0;
+    DevBuf<uint32_t> samp32;
+    DevBuf<uint64_t> samp64;
+    // binomial table C(n, k) for n, k < 64 (device), combinadic ranking
+    DevBuf<uint64_t> binom;
+    // reference labels and normalizations (host, computed on demand)
+    std::vector<uint64_t> labels;
+    std::vector<double> ref_norm;
+    bool labels_ready = false;
+};
+
+struct OpHost {
+    const BasisHost *basis = nullptr;
+    int is_complex = 0;          // any complex matrix element
+    int hermitian = 1;
+    // action terms (P'HP semantics, group averaged), adjoint form for the pull kernel
+    std::vector<CTerm> diag_terms;
+    std::vector<CTerm> off_terms;
+    std::vector<GroupOfTerms> groups;
+    DevBuf<TermDev> d_diag, d_off;
+    DevBuf<GroupOfTerms> d_groups;
+    // adjoint operator (for transposed / adjoint products)
+    std::vector<CTerm> diag_terms_adj, off_terms_adj;
+    std::vector<GroupOfTerms> groups_adj;
+    DevBuf<TermDev> d_diag_adj, d_off_adj;
+    DevBuf<GroupOfTerms> d_groups_adj;
+    // user diagonal
+    std::vector<cd> user_diag;
+    bool has_user_diag = false;
+    DevBuf<double> d_user_diag;      // interleaved complex
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    // staging buffers for host vectors
+    DevBuf<char> stage_x, stage_y;
+    ~OpHost();
+};
+
+// ---------------------------------------------------------- host logic ---
+uint64_t full_mask(int L);
+uint64_t bit_reverse(uint64_t u, int L);
+uint64_t rotl(uint64_t u, int k, int L);
+uint64_t apply_point(const GroupHost &g, int p, uint64_t u);
+int64_t binom(int n, int k);
+
+void build_group(BasisHost &b);   // throws Error(QS_EARG) for invalid blocks
+void build_basis_device(BasisHost &b);
+void compute_reference_labels(BasisHost &b);
+
+struct Coupling {
+    std::string op;
+    std::vector<int> sites;   // 1-based
+    cd coef;
+};
+std::vector<CTerm> compile_couplings(const std::vector<Coupling> &cs, int L, int pauli);
+std::vector<CTerm> consolidate_cterms(const std::vector<CTerm> &t);
+std::vector<CTerm> symmetrize_cterms(const std::vector<CTerm> &t, const BasisHost &b);
+void split_terms(const std::vector<CTerm> &all, std::vector<CTerm> &diag, std::vector<CTerm> &off,
+                 std::vector<GroupOfTerms> &groups);
+
+// small dense FP64 linear algebra (host)
+void sym_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
+void herm_eig(int n, std::vector<cd> &A, std::vector<double> &w, std::vector<cd> &V);
+void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E);
+
+// --------------------------------------------------------- kernels -------
+// y = beta*y + alpha*K x on the sector; K = M (adjoint=false) or M^dagger.
+// conj_x / conj_y conjugate x on load and y on load and store.
+void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim,
+                   double bre, double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s);
+
+// vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
+struct VecWork {
+    DevBuf<double> partial;   // per-block partial sums
+    DevBuf<double> result;    // reduced values (device)
+    std::vector<double> host;
+};
+int vec_blocks(int64_t n);
+// h[j] = <V[:, j], w> for j < m (complex: conj(V) . w); V column-major with ld = n.
+void gram_dots(int dtype, const void *V, int64_t n, int m, const void *w, VecWork &wk,
+               hipStream_t s, std::vector<cd> &h);
+// w -= V[:, :m] h  and return ||w||^2
+double gram_update(int dtype, const void *V, int64_t n, int m, const std::vector<cd> &h, void *w,
+                   VecWork &wk, hipStream_t s);
+double vec_norm2(int dtype, const void *x, int64_t n, VecWork &wk, hipStream_t s);
+void vec_scale_copy(int dtype, const void *x, void *y, int64_t n, cd scale, hipStream_t s);
+void vec_axpby(int dtype, cd a, const void *x, cd b, void *y, int64_t n, hipStream_t s);
+// Y[:, :k] = V[:, :m] C  (C is m x k column-major, complex or real)
+void basis_combine(int dtype, const void *V, int64_t n, int m, const std::vector<cd> &C, int k,
+                   void *Y, hipStream_t s);
+void vec_random(int dtype, void *y, int64_t n, uint64_t seed, hipStream_t s);
+void vec_convert(int from_dtype, const void *x, int to_dtype, void *y, int64_t n, hipStream_t s);
+size_t dtype_size(int dtype);
+bool dtype_complex(int dtype);
+
+// Krylov solvers (qs_krylov.cpp); vectors are device buffers of `dtype`
+struct EigshResult {
+    std::vector<double> vals;    // k (re) or 2k (re, im)
+    int nconv = 0, nmatvec = 0;
+};
+void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0_dev,
+                  EigshResult &res, void *vecs_dev);
+// out_dev: Ns x n_times (column-major) of the complex dtype
+void expm_times_device(OpHost &op, int dtype, const void *v_dev, const std::vector<cd> &scales, double tol,
+                       int krylov_dim, void *out_dev, int &nmatvec);
+
+}  // namespace qs

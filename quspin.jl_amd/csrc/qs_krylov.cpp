@@ -1,0 +1,387 @@
+// qs_krylov.cpp -- device-resident Krylov solvers driven from the host.
+//
+// eigsh_device: replaces Arpack.eigs behind eigsh (operators.jl:3367-3408).
+//   Thick-restart Lanczos (Wu & Simon, SIAM J. Matrix Anal. Appl. 22 (2000)
+//   602) with two-pass classical Gram-Schmidt against the whole resident
+//   basis, FP64 accumulation, and ARPACK's symmetric convergence test
+//   |beta_m y_m,i| <= tol * max(eps^(2/3), |theta_i|).  The Krylov basis stays
+//   in HBM for the whole solve; only the small projected matrix visits the host.
+// expm_times_device: _krylov_expmv_times (operators.jl:1805-1893) with
+//   _krylov_factorization (:1655-1731), _krylov_project (:1733-1756) and the
+//   halving recursion of _krylov_expmv_vector (:1758-1803).
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "qs_internal.h"
+
+namespace qs {
+
+namespace {
+
+const double kEps = 2.220446049250313e-16;
+
+struct Krylov {
+    int dtype;
+    int64_t n;
+    size_t esz;
+    DevBuf<char> V;   // n x cols
+    int cols = 0;
+    Krylov(int dt, int64_t n_, int c) : dtype(dt), n(n_), esz(dtype_size(dt)), cols(c) {
+        V.alloc((size_t)n * c * esz);
+    }
+    void *col(int j) const { return (void *)(V.get() + (size_t)j * n * esz); }
+};
+
+void matvec(OpHost &op, int dtype, const void *x, void *y) {
+    launch_matvec(op, dtype, x, y, 1.0, 0.0, 0.0, 0.0, false, false, false, op.stream);
+}
+
+// w (already holding A v_j) orthogonalised against V[:, 0..j] twice;
+// returns h (sum of both passes) and ||w||.
+double orthogonalize(Krylov &K, int j, void *w, VecWork &wk, std::vector<cd> &h, hipStream_t s) {
+    std::vector<cd> h2;
+    gram_dots(K.dtype, K.V.get(), K.n, j + 1, w, wk, s, h);
+    gram_update(K.dtype, K.V.get(), K.n, j + 1, h, w, wk, s);
+    gram_dots(K.dtype, K.V.get(), K.n, j + 1, w, wk, s, h2);
+    double nrm2 = gram_update(K.dtype, K.V.get(), K.n, j + 1, h2, w, wk, s);
+    for (int i = 0; i <= j; i++) h[i] += h2[i];
+    return std::sqrt(nrm2);
+}
+
+bool better(int which, double a, double b) {
+    // true if Ritz value a is more wanted than b
+    switch (which) {
+    case QS_WHICH_SA: return a < b;
+    case QS_WHICH_LA: return a > b;
+    case QS_WHICH_SM: return std::fabs(a) < std::fabs(b);
+    case QS_WHICH_LM: return std::fabs(a) > std::fabs(b);
+    }
+    return a < b;
+}
+
+// indices of the `want` most wanted Ritz values (BE: alternate both ends,
+// one more from the high end when want is odd)
+std::vector<int> select(const std::vector<double> &theta, int which, int want) {
+    int m = (int)theta.size();
+    std::vector<int> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    if (which == QS_WHICH_BE) {
+        std::sort(idx.begin(), idx.end(), [&](int x, int y) { return theta[x] < theta[y]; });
+        std::vector<int> out;
+        int lo = 0, hi = m - 1;
+        bool take_hi = (want % 2 == 1);
+        while ((int)out.size() < want && lo <= hi) {
+            if (take_hi) out.push_back(idx[hi--]);
+            else out.push_back(idx[lo++]);
+            take_hi = !take_hi;
+        }
+        return out;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return better(which, theta[x], theta[y]); });
+    idx.resize(std::min(want, m));
+    return idx;
+}
+
+}  // namespace
+
+void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0_dev,
+                  EigshResult &res, void *vecs_dev) {
+    const int64_t n = op.basis->Ns;
+    hipStream_t s = op.stream;
+    if (!op.hermitian) fail(QS_EARG, "eigsh on the device needs a Hermitian operator");
+    if (tol <= 0) tol = kEps;
+    if (maxiter <= 0) maxiter = 300;
+    if (ncv <= 0) ncv = std::max(20, 2 * k + 1);
+    ncv = (int)std::min<int64_t>(ncv, n);
+    if (ncv <= k) fail(QS_EARG, "ncv must exceed k");
+    const double eps23 = std::pow(kEps, 2.0 / 3.0);
+    Krylov K(dtype, n, ncv + 1);
+    VecWork wk;
+    // starting vector
+    if (v0_dev) {
+        QS_HIP(hipMemcpyAsync(K.col(0), v0_dev, (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
+    } else {
+        vec_random(dtype, K.col(0), n, 0x51a7ULL, s);
+    }
+    double nrm = std::sqrt(vec_norm2(dtype, K.col(0), n, wk, s));
+    if (!(nrm > 0)) fail(QS_EARG, "initial vector must be nonzero");
+    vec_scale_copy(dtype, K.col(0), K.col(0), n, cd(1.0 / nrm, 0), s);
+
+    const int m = ncv;
+    std::vector<double> T((size_t)(m + 1) * (m + 1), 0.0);
+    auto t = [&](int i, int j) -> double & { return T[(size_t)i * (m + 1) + j]; };
+    int kept = 0;
+    double beta_last = 0;
+    std::vector<double> theta, Y;
+    std::vector<int> wanted;
+    int nconv = 0;
+    uint64_t reseed = 1;
+    for (int iter = 0; iter < maxiter; iter++) {
+        for (int j = kept; j < m; j++) {
+            void *w = K.col(j + 1);
+            matvec(op, dtype, K.col(j), w);
+            res.nmatvec++;
+            std::vector<cd> h;
+            double beta = orthogonalize(K, j, w, wk, h, s);
+            for (int i = kept; i <= j; i++) {
+                // the Rayleigh quotient entries of the new column; entries
+                // below `kept` are the arrow couplings set at restart
+                t(i, j) = h[i].real();
+                t(j, i) = h[i].real();
+            }
+            double tnorm = 0;
+            for (double v : T) tnorm = std::max(tnorm, std::fabs(v));
+            if (beta <= 1e3 * kEps * std::max(1.0, tnorm)) {
+                // invariant subspace: continue with a fresh direction
+                beta = 0;
+                if (j + 1 < m + 1) {
+                    vec_random(dtype, w, n, 0x9e37ULL + (reseed++), s);
+                    std::vector<cd> hh;
+                    double b2 = orthogonalize(K, j, w, wk, hh, s);
+                    vec_scale_copy(dtype, w, w, n, cd(1.0 / b2, 0), s);
+                }
+            } else {
+                vec_scale_copy(dtype, w, w, n, cd(1.0 / beta, 0), s);
+            }
+            if (j + 1 < m) {
+                t(j, j + 1) = beta;
+                t(j + 1, j) = beta;
+            }
+            beta_last = beta;
+        }
+        // Rayleigh-Ritz on the m x m projected matrix
+        std::vector<double> A((size_t)m * m);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) A[(size_t)i * m + j] = t(i, j);
+        sym_eig(m, A, theta, Y);
+        wanted = select(theta, which, k);
+        nconv = 0;
+        for (int idx : wanted) {
+            double r = std::fabs(beta_last * Y[(size_t)(m - 1) * m + idx]);
+            if (r <= tol * std::max(eps23, std::fabs(theta[idx]))) nconv++;
+        }
+        if (nconv >= k || iter == maxiter - 1) break;
+        // thick restart: keep the wanted Ritz vectors plus as many runners-up
+        int keep = std::min(m - 1, k + (m - k) / 2);
+        std::vector<int> sel = select(theta, which, keep);
+        std::vector<cd> C((size_t)m * keep);
+        for (int c = 0; c < keep; c++)
+            for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + sel[c]], 0);
+        Krylov tmp(dtype, n, keep);
+        basis_combine(dtype, K.V.get(), n, m, C, keep, tmp.V.get(), s);
+        QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * keep * K.esz, hipMemcpyDeviceToDevice, s));
+        QS_HIP(hipMemcpyAsync(K.col(keep), K.col(m), (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int c = 0; c < keep; c++) {
+            t(c, c) = theta[sel[c]];
+            double sc = beta_last * Y[(size_t)(m - 1) * m + sel[c]];
+            t(c, keep) = sc;
+            t(keep, c) = sc;
+        }
+        kept = keep;
+    }
+    // order the output the way operators.jl:3393-3404 does
+    std::vector<int> out = wanted;
+    if (which == QS_WHICH_SA) std::sort(out.begin(), out.end(), [&](int a, int b) { return theta[a] < theta[b]; });
+    else if (which == QS_WHICH_LA) std::sort(out.begin(), out.end(), [&](int a, int b) { return theta[a] > theta[b]; });
+    else if (which == QS_WHICH_SM)
+        std::stable_sort(out.begin(), out.end(), [&](int a, int b) { return std::fabs(theta[a]) < std::fabs(theta[b]); });
+    else if (which == QS_WHICH_LM)
+        std::stable_sort(out.begin(), out.end(), [&](int a, int b) { return std::fabs(theta[a]) > std::fabs(theta[b]); });
+    else std::sort(out.begin(), out.end(), [&](int a, int b) { return theta[a] < theta[b]; });
+    res.vals.resize(k);
+    for (int i = 0; i < k; i++) res.vals[i] = theta[out[i]];
+    res.nconv = nconv;
+    if (vecs_dev) {
+        std::vector<cd> C((size_t)m * k);
+        for (int c = 0; c < k; c++)
+            for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + out[c]], 0);
+        basis_combine(dtype, K.V.get(), n, m, C, k, vecs_dev, s);
+    }
+    QS_HIP(hipStreamSynchronize(s));
+}
+
+// -------------------------------------------------------------- expm ------
+namespace {
+
+struct Factorization {
+    std::unique_ptr<Krylov> K;
+    std::vector<cd> H;   // (steps+1) x steps, row-major
+    int steps = 0, completed = 0;
+    double beta = 0, terminal = 0;
+    int64_t dim = 0;
+    bool hermitian = true;
+    cd h(int i, int j) const { return H[(size_t)i * steps + j]; }
+};
+
+double frob(const std::vector<cd> &H) {
+    double s = 0;
+    for (auto &z : H) s += std::norm(z);
+    return std::sqrt(s);
+}
+
+// operators.jl:1655-1731
+void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorization &F, VecWork &wk, int &nmv) {
+    hipStream_t s = op.stream;
+    const int64_t n = op.basis->Ns;
+    F.dim = n;
+    F.hermitian = op.hermitian != 0;
+    F.steps = (int)std::min<int64_t>(krylov_dim, n);
+    F.K = std::make_unique<Krylov>(dtype, n, F.steps + 1);
+    Krylov &K = *F.K;
+    F.H.assign((size_t)(F.steps + 1) * F.steps, cd(0, 0));
+    auto H = [&](int i, int j) -> cd & { return F.H[(size_t)i * F.steps + j]; };
+    F.beta = std::sqrt(vec_norm2(dtype, v, n, wk, s));
+    vec_scale_copy(dtype, v, K.col(0), n, cd(1.0 / F.beta, 0), s);
+    F.completed = F.steps;
+    F.terminal = 0;
+    double prev = 0;
+    for (int col = 0; col < F.steps; col++) {
+        void *w = K.col(col + 1);
+        matvec(op, dtype, K.col(col), w);
+        nmv++;
+        double tn;
+        if (F.hermitian) {
+            // three-term Lanczos without reorthogonalisation
+            if (col > 0) vec_axpby(dtype, cd(-prev, 0), K.col(col - 1), cd(1, 0), w, n, s);
+            std::vector<cd> hh;
+            gram_dots(dtype, K.col(col), n, 1, w, wk, s, hh);
+            H(col, col) = hh[0];
+            std::vector<cd> one{hh[0]};
+            tn = std::sqrt(gram_update(dtype, K.col(col), n, 1, one, w, wk, s));
+            H(col + 1, col) = tn;
+            if (col + 1 < F.steps) H(col, col + 1) = tn;
+        } else {
+            // Arnoldi, two Gram-Schmidt passes
+            std::vector<cd> h1, h2;
+            gram_dots(dtype, K.V.get(), n, col + 1, w, wk, s, h1);
+            gram_update(dtype, K.V.get(), n, col + 1, h1, w, wk, s);
+            gram_dots(dtype, K.V.get(), n, col + 1, w, wk, s, h2);
+            double nr2 = gram_update(dtype, K.V.get(), n, col + 1, h2, w, wk, s);
+            for (int r = 0; r <= col; r++) H(r, col) = h1[r] + h2[r];
+            tn = std::sqrt(nr2);
+            H(col + 1, col) = tn;
+        }
+        F.terminal = tn;
+        if (tn <= 100 * kEps * std::max(1.0, frob(F.H))) {
+            F.completed = col + 1;
+            break;
+        } else if (col + 1 < F.steps) {
+            vec_scale_copy(dtype, w, w, n, cd(1.0 / tn, 0), s);
+        }
+        prev = tn;
+    }
+}
+
+// operators.jl:1733-1756: result = beta * V[:, :c] exp(scale H_c)[:, 0]
+double project(OpHost &op, int dtype, Factorization &F, cd scale, void *out, VecWork &wk) {
+    int c = F.completed;
+    std::vector<cd> A((size_t)c * c), E;
+    for (int i = 0; i < c; i++)
+        for (int j = 0; j < c; j++) A[(size_t)i * c + j] = scale * F.h(i, j);
+    expm_dense(c, A, E);
+    std::vector<cd> coef(c);
+    for (int i = 0; i < c; i++) coef[i] = F.beta * E[(size_t)i * c + 0];
+    basis_combine(dtype, F.K->V.get(), F.dim, c, coef, 1, out, op.stream);
+    bool full = F.completed == F.steps && F.completed < F.dim;
+    return full ? std::abs(scale) * F.terminal * F.beta * std::abs(E[(size_t)(c - 1) * c + 0]) : 0.0;
+}
+
+// operators.jl:1758-1803
+void expmv_vector(OpHost &op, int dtype, const void *v, cd scale, double tol, int krylov_dim, int depth, void *out,
+                  VecWork &wk, int &nmv) {
+    hipStream_t s = op.stream;
+    const int64_t n = op.basis->Ns;
+    if (tol <= 0) fail(QS_EARG, "tol must be positive");
+    if (scale == cd(0, 0) || vec_norm2(dtype, v, n, wk, s) == 0) {
+        QS_HIP(hipMemcpyAsync(out, v, (size_t)n * dtype_size(dtype), hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    Factorization F;
+    factorize(op, dtype, v, krylov_dim, F, wk, nmv);
+    double est = project(op, dtype, F, scale, out, wk);
+    double nr = std::sqrt(vec_norm2(dtype, out, n, wk, s));
+    if (est > tol * std::max(1.0, nr)) {
+        if (depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
+        F.K.reset();
+        DevBuf<char> mid((size_t)n * dtype_size(dtype));
+        expmv_vector(op, dtype, v, scale / 2.0, tol / 2, krylov_dim, depth + 1, mid.get(), wk, nmv);
+        expmv_vector(op, dtype, mid.get(), scale / 2.0, tol / 2, krylov_dim, depth + 1, out, wk, nmv);
+    }
+}
+
+}  // namespace
+
+void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<cd> &scales, double tol,
+                       int krylov_dim, void *out, int &nmatvec) {
+    hipStream_t s = op.stream;
+    const int64_t n = op.basis->Ns;
+    const size_t esz = dtype_size(dtype);
+    if (!dtype_complex(dtype)) fail(QS_EARG, "expm_multiply needs a complex vector dtype");
+    if (tol <= 0) fail(QS_EARG, "tol must be positive");
+    if (krylov_dim <= 0) fail(QS_EARG, "krylov_dim must be positive");
+    VecWork wk;
+    auto outcol = [&](size_t j) { return (void *)((char *)out + j * n * esz); };
+    bool all_zero = true;
+    for (auto &z : scales) all_zero &= z == cd(0, 0);
+    if (scales.empty()) return;
+    if (all_zero || vec_norm2(dtype, v, n, wk, s) == 0) {
+        for (size_t j = 0; j < scales.size(); j++)
+            QS_HIP(hipMemcpyAsync(outcol(j), v, n * esz, hipMemcpyDeviceToDevice, s));
+        QS_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    Factorization F;
+    factorize(op, dtype, v, krylov_dim, F, wk, nmatvec);
+    int c = F.completed;
+    bool full = F.completed == F.steps && F.completed < F.dim;
+    if (F.hermitian) {
+        // eigen(Hermitian(H_c)): upper triangle, real diagonal
+        std::vector<double> A((size_t)c * c), w, Vv;
+        for (int i = 0; i < c; i++)
+            for (int j = 0; j < c; j++) {
+                int a = std::min(i, j), b = std::max(i, j);
+                A[(size_t)i * c + j] = (i == j) ? F.h(i, i).real() : F.h(a, b).real();
+            }
+        bool real_t = true;
+        for (int i = 0; i + 1 < c; i++) real_t &= std::fabs(F.h(i, i + 1).imag()) == 0.0;
+        (void)real_t;
+        sym_eig(c, A, w, Vv);
+        std::vector<cd> coefs((size_t)c * scales.size(), cd(0, 0));
+        std::vector<double> ests(scales.size(), 0.0);
+        for (size_t j = 0; j < scales.size(); j++) {
+            if (scales[j] == cd(0, 0)) continue;
+            for (int i = 0; i < c; i++) {
+                cd acc = 0;
+                for (int q = 0; q < c; q++) acc += Vv[(size_t)i * c + q] * std::exp(scales[j] * w[q]) * Vv[(size_t)0 * c + q];
+                coefs[j * c + i] = F.beta * acc;
+            }
+            ests[j] = full ? std::abs(scales[j]) * F.terminal * std::abs(coefs[j * c + c - 1]) : 0.0;
+        }
+        basis_combine(dtype, F.K->V.get(), n, c, coefs, (int)scales.size(), out, s);
+        for (size_t j = 0; j < scales.size(); j++) {
+            if (scales[j] == cd(0, 0)) {
+                QS_HIP(hipMemcpyAsync(outcol(j), v, n * esz, hipMemcpyDeviceToDevice, s));
+                continue;
+            }
+            double nr = std::sqrt(vec_norm2(dtype, outcol(j), n, wk, s));
+            if (ests[j] > tol * std::max(1.0, nr))
+                expmv_vector(op, dtype, v, scales[j], tol, krylov_dim, 0, outcol(j), wk, nmatvec);
+        }
+    } else {
+        for (size_t j = 0; j < scales.size(); j++) {
+            if (scales[j] == cd(0, 0)) {
+                QS_HIP(hipMemcpyAsync(outcol(j), v, n * esz, hipMemcpyDeviceToDevice, s));
+                continue;
+            }
+            double est = project(op, dtype, F, scales[j], outcol(j), wk);
+            double nr = std::sqrt(vec_norm2(dtype, outcol(j), n, wk, s));
+            if (est > tol * std::max(1.0, nr))
+                expmv_vector(op, dtype, v, scales[j], tol, krylov_dim, 0, outcol(j), wk, nmatvec);
+        }
+    }
+    QS_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace qs

@@ -1,0 +1,351 @@
+// qs_vec.hip -- Krylov vector kernels with FP64 accumulation.
+//
+// Every reduction is deterministic: a fixed number of blocks, each owning a
+// fixed contiguous row range, a fixed-shape tree inside the block and a
+// second fixed-shape tree over the block partials.  Results therefore do not
+// depend on what else shares the device.  Vectors stored in FP32 (the
+// FP32-stream form) are widened to FP64 before every product.
+#include <algorithm>
+
+#include "qs_device.h"
+
+namespace qs {
+
+size_t dtype_size(int dtype) {
+    switch (dtype) {
+    case QS_F64: return 8;
+    case QS_C128: return 16;
+    case QS_F32: return 4;
+    case QS_C64: return 8;
+    }
+    fail(QS_EARG, "unknown dtype");
+}
+bool dtype_complex(int dtype) { return dtype == QS_C128 || dtype == QS_C64; }
+
+namespace {
+
+constexpr int kVB = 256;
+constexpr int kMaxBlocks = 1024;
+constexpr int kCols = 8;   // Gram columns per pass
+
+__device__ __forceinline__ void block_reduce2(double &re, double &im, double *sh) {
+    // fixed-shape tree over kVB lanes; result in lane 0
+    sh[threadIdx.x] = re;
+    sh[kVB + threadIdx.x] = im;
+    __syncthreads();
+    for (int s = kVB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sh[threadIdx.x] += sh[threadIdx.x + s];
+            sh[kVB + threadIdx.x] += sh[kVB + threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    re = sh[0];
+    im = sh[kVB];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void row_range(int64_t n, int64_t &r0, int64_t &r1) {
+    int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    r0 = (int64_t)blockIdx.x * chunk;
+    r1 = r0 + chunk < n ? r0 + chunk : n;
+}
+
+// partial[(j * nb + b) * 2 + {0,1}] = sum over block b rows of conj(V[r, j]) * w[r]
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_gram(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
+                                              double *partial) {
+    using T = VTraits<V>;
+    __shared__ double sh[2 * kVB];
+    int64_t r0, r1;
+    row_range(n, r0, r1);
+    for (int j0 = 0; j0 < m; j0 += kCols) {
+        double acc_r[kCols], acc_i[kCols];
+#pragma unroll
+        for (int c = 0; c < kCols; c++) { acc_r[c] = 0; acc_i[c] = 0; }
+        int nc = m - j0 < kCols ? m - j0 : kCols;
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
+            double wr, wi;
+            T::load(w, r, wr, wi);
+#pragma unroll
+            for (int c = 0; c < kCols; c++) {
+                if (c < nc) {
+                    double vr, vi;
+                    T::load(Vb, (int64_t)(j0 + c) * n + r, vr, vi);
+                    acc_r[c] += vr * wr + vi * wi;
+                    acc_i[c] += vr * wi - vi * wr;
+                }
+            }
+        }
+        for (int c = 0; c < nc; c++) {
+            double re = acc_r[c], im = acc_i[c];
+            block_reduce2(re, im, sh);
+            if (threadIdx.x == 0) {
+                partial[((int64_t)(j0 + c) * gridDim.x + blockIdx.x) * 2] = re;
+                partial[((int64_t)(j0 + c) * gridDim.x + blockIdx.x) * 2 + 1] = im;
+            }
+        }
+    }
+}
+
+// out[j] = sum_b partial[j][b] in a fixed tree (one block per j)
+__global__ __launch_bounds__(kVB) void k_reduce(const double *partial, int nb, double *out) {
+    __shared__ double sh[2 * kVB];
+    int j = blockIdx.x;
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nb; b += kVB) {
+        re += partial[((int64_t)j * nb + b) * 2];
+        im += partial[((int64_t)j * nb + b) * 2 + 1];
+    }
+    block_reduce2(re, im, sh);
+    if (threadIdx.x == 0) {
+        out[2 * j] = re;
+        out[2 * j + 1] = im;
+    }
+}
+
+// w[r] -= sum_j V[r, j] h[j]; partial[b] = sum |w[r]|^2 (of the stored values)
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_update(const V *__restrict__ Vb, int64_t n, int m, const double *h,
+                                                V *__restrict__ w, double *partial) {
+    using T = VTraits<V>;
+    __shared__ double sh[2 * kVB];
+    int64_t r0, r1;
+    row_range(n, r0, r1);
+    double nrm = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
+        double wr, wi;
+        T::load(w, r, wr, wi);
+        for (int j = 0; j < m; j++) {
+            double vr, vi;
+            T::load(Vb, (int64_t)j * n + r, vr, vi);
+            double hr = h[2 * j], hi = h[2 * j + 1];
+            wr -= vr * hr - vi * hi;
+            wi -= vr * hi + vi * hr;
+        }
+        T::store(w, r, wr, wi);
+        T::load(w, r, wr, wi);
+        nrm += wr * wr + wi * wi;
+    }
+    double dummy = 0;
+    block_reduce2(nrm, dummy, sh);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = nrm;
+        partial[2 * blockIdx.x + 1] = 0;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_norm2(const V *__restrict__ x, int64_t n, double *partial) {
+    using T = VTraits<V>;
+    __shared__ double sh[2 * kVB];
+    int64_t r0, r1;
+    row_range(n, r0, r1);
+    double nrm = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
+        double a, b;
+        T::load(x, r, a, b);
+        nrm += a * a + b * b;
+    }
+    double dummy = 0;
+    block_reduce2(nrm, dummy, sh);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = nrm;
+        partial[2 * blockIdx.x + 1] = 0;
+    }
+}
+
+template <typename V>
+__global__ void k_axpby(double ar, double ai, const V *__restrict__ x, double br, double bi, V *__restrict__ y,
+                        int64_t n, int b_zero) {
+    using T = VTraits<V>;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double xr, xi, yr = 0, yi = 0;
+        T::load(x, r, xr, xi);
+        if (!b_zero) T::load(y, r, yr, yi);
+        double orr = ar * xr - ai * xi + (b_zero ? 0.0 : br * yr - bi * yi);
+        double oi = ar * xi + ai * xr + (b_zero ? 0.0 : br * yi + bi * yr);
+        T::store(y, r, orr, oi);
+    }
+}
+
+// Y[:, i] = sum_j V[:, j] C[j + i*m]   (C interleaved complex, column-major m x k)
+template <typename V>
+__global__ void k_combine(const V *__restrict__ Vb, int64_t n, int m, const double *C, int k, V *__restrict__ Y) {
+    using T = VTraits<V>;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        for (int i = 0; i < k; i++) {
+            double ar = 0, ai = 0;
+            for (int j = 0; j < m; j++) {
+                double vr, vi;
+                T::load(Vb, (int64_t)j * n + r, vr, vi);
+                double cr = C[2 * ((int64_t)i * m + j)], ci = C[2 * ((int64_t)i * m + j) + 1];
+                ar += vr * cr - vi * ci;
+                ai += vr * ci + vi * cr;
+            }
+            T::store(Y, (int64_t)i * n + r, ar, ai);
+        }
+    }
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ x, B *__restrict__ y, int64_t n) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double re, im;
+        VTraits<A>::load(x, r, re, im);
+        VTraits<B>::store(y, r, re, im);
+    }
+}
+
+// uniform(-1, 1) entries from a counter-based hash (splitmix64) of (seed, index)
+__device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+template <typename V>
+__global__ void k_random(V *__restrict__ y, int64_t n, uint64_t seed) {
+    using T = VTraits<V>;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double re = hash_uniform(seed, 2 * (uint64_t)r);
+        double im = T::cplx ? hash_uniform(seed, 2 * (uint64_t)r + 1) : 0.0;
+        T::store(y, r, re, im);
+    }
+}
+
+int ew_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
+
+template <typename F>
+void by_dtype(int dtype, F &&f) {
+    switch (dtype) {
+    case QS_F64: f((double *)nullptr); break;
+    case QS_C128: f((cplx_d *)nullptr); break;
+    case QS_F32: f((float *)nullptr); break;
+    case QS_C64: f((cplx_f *)nullptr); break;
+    default: fail(QS_EARG, "unknown dtype");
+    }
+}
+
+void ensure(VecWork &wk, size_t partial, size_t result) {
+    if (wk.partial.n < partial) wk.partial.alloc(std::max(partial, (size_t)2 * kMaxBlocks * 64));
+    if (wk.result.n < result) wk.result.alloc(std::max(result, (size_t)2 * 512));
+}
+
+}  // namespace
+
+int vec_blocks(int64_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + 4 * kVB - 1) / (4 * kVB)));
+}
+
+void gram_dots(int dtype, const void *Vmat, int64_t n, int m, const void *w, VecWork &wk, hipStream_t s,
+               std::vector<cd> &h) {
+    int nb = vec_blocks(n);
+    ensure(wk, (size_t)2 * nb * m, (size_t)2 * m);
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_gram<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, (const V *)w,
+                           wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(m), dim3(kVB), 0, s, wk.partial.get(), nb, wk.result.get());
+    QS_HIP(hipGetLastError());
+    wk.host.resize((size_t)2 * m);
+    QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * 2 * m, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    h.resize(m);
+    for (int j = 0; j < m; j++) h[j] = cd(wk.host[2 * j], wk.host[2 * j + 1]);
+}
+
+double gram_update(int dtype, const void *Vmat, int64_t n, int m, const std::vector<cd> &h, void *w, VecWork &wk,
+                   hipStream_t s) {
+    int nb = vec_blocks(n);
+    ensure(wk, (size_t)2 * nb + 2 * m + 2, (size_t)2 * m + 2);
+    // h goes after the partials in the same buffer
+    std::vector<double> hh((size_t)2 * m);
+    for (int j = 0; j < m; j++) {
+        hh[2 * j] = h[j].real();
+        hh[2 * j + 1] = h[j].imag();
+    }
+    double *dh = wk.partial.get() + 2 * nb;
+    if (m > 0) QS_HIP(hipMemcpyAsync(dh, hh.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, s));
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_update<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, dh, (V *)w,
+                           wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, wk.result.get());
+    QS_HIP(hipGetLastError());
+    double out[2];
+    QS_HIP(hipMemcpyAsync(out, wk.result.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    return out[0];
+}
+
+double vec_norm2(int dtype, const void *x, int64_t n, VecWork &wk, hipStream_t s) {
+    int nb = vec_blocks(n);
+    ensure(wk, (size_t)2 * nb, 2);
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_norm2<V>), dim3(nb), dim3(kVB), 0, s, (const V *)x, n, wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, wk.result.get());
+    QS_HIP(hipGetLastError());
+    double out[2];
+    QS_HIP(hipMemcpyAsync(out, wk.result.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    return out[0];
+}
+
+void vec_scale_copy(int dtype, const void *x, void *y, int64_t n, cd scale, hipStream_t s) {
+    vec_axpby(dtype, scale, x, cd(0, 0), y, n, s);
+}
+
+void vec_axpby(int dtype, cd a, const void *x, cd b, void *y, int64_t n, hipStream_t s) {
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_axpby<V>), dim3(ew_grid(n)), dim3(256), 0, s, a.real(), a.imag(), (const V *)x,
+                           b.real(), b.imag(), (V *)y, n, (int)(b == cd(0, 0)));
+    });
+    QS_HIP(hipGetLastError());
+}
+
+void basis_combine(int dtype, const void *Vmat, int64_t n, int m, const std::vector<cd> &C, int k, void *Y,
+                   hipStream_t s) {
+    std::vector<double> cc((size_t)2 * m * k);
+    for (size_t i = 0; i < (size_t)m * k; i++) {
+        cc[2 * i] = C[i].real();
+        cc[2 * i + 1] = C[i].imag();
+    }
+    DevBuf<double> dc(cc.size());
+    QS_HIP(hipMemcpyAsync(dc.get(), cc.data(), sizeof(double) * cc.size(), hipMemcpyHostToDevice, s));
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_combine<V>), dim3(ew_grid(n)), dim3(256), 0, s, (const V *)Vmat, n, m, dc.get(), k,
+                           (V *)Y);
+    });
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipStreamSynchronize(s));
+}
+
+void vec_random(int dtype, void *y, int64_t n, uint64_t seed, hipStream_t s) {
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_random<V>), dim3(ew_grid(n)), dim3(256), 0, s, (V *)y, n, seed);
+    });
+    QS_HIP(hipGetLastError());
+}
+
+void vec_convert(int from, const void *x, int to, void *y, int64_t n, hipStream_t s) {
+    by_dtype(from, [&](auto *ta) {
+        using A = std::remove_pointer_t<decltype(ta)>;
+        by_dtype(to, [&](auto *tb) {
+            using B = std::remove_pointer_t<decltype(tb)>;
+            hipLaunchKernelGGL((k_convert<A, B>), dim3(ew_grid(n)), dim3(256), 0, s, (const A *)x, (B *)y, n);
+        });
+    });
+    QS_HIP(hipGetLastError());
+}
+
+}  // namespace qs

@@ -1,0 +1,70 @@
+"""ctypes binding of libqsgpu.so (the C ABI declared in include/qsgpu.h)."""
+import ctypes
+import os
+import subprocess
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "_build", "libqsgpu.so")
+CSRC = os.path.join(_PKG_ROOT, "csrc")
+
+QS_OK, QS_EARG, QS_EDIM, QS_ENOCONV, QS_EOOM, QS_EDEVICE = range(6)
+QS_UNSET = -1000000
+QS_F64, QS_C128, QS_F32, QS_C64 = range(4)
+WHICH = {"SA": 0, "LA": 1, "SM": 2, "LM": 3, "BE": 4}
+QS_CHECK_SYMM, QS_CHECK_HERM, QS_CHECK_PCON, QS_FORCE_COMPLEX = 1, 2, 4, 8
+
+EXPORTS = (
+    "qs_last_error", "qs_device_info", "qs_basis_create", "qs_basis_destroy", "qs_basis_size",
+    "qs_basis_states", "qs_basis_reps", "qs_basis_norms", "qs_basis_is_complex", "qs_basis_lookup",
+    "qs_op_create", "qs_op_destroy", "qs_op_info", "qs_op_set_diagonal", "qs_op_set_stream",
+    "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply",
+)
+
+_lib = None
+
+
+def build(jobs=8):
+    """Compile libqsgpu.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
+    subprocess.run(["make", "-s", "-j%d" % jobs, "-C", CSRC], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libqsgpu.so is not built (%s); run qsgpu.build() or __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    L.qs_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.qs_last_error.restype = ctypes.c_size_t
+    L.qs_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, vp, vp]
+    L.qs_basis_create.argtypes = [i32] * 11 + [pp]
+    L.qs_basis_destroy.argtypes = [vp]
+    L.qs_basis_size.argtypes = [vp, vp]
+    L.qs_basis_states.argtypes = [vp, vp]
+    L.qs_basis_reps.argtypes = [vp, vp]
+    L.qs_basis_norms.argtypes = [vp, vp, vp]
+    L.qs_basis_is_complex.argtypes = [vp, vp]
+    L.qs_basis_lookup.argtypes = [vp, vp, i64, vp, vp]
+    L.qs_op_create.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, pp]
+    L.qs_op_destroy.argtypes = [vp]
+    L.qs_op_info.argtypes = [vp, vp, vp, vp]
+    L.qs_op_set_diagonal.argtypes = [vp, vp]
+    L.qs_op_set_stream.argtypes = [vp, vp]
+    L.qs_op_stream.argtypes = [vp, pp]
+    L.qs_op_matvec.argtypes = [vp, i32, vp, vp, i64, dbl, dbl, dbl, dbl, i32, i32, i32]
+    L.qs_eigsh.argtypes = [vp, i32, i32, i32, dbl, i32, i32, vp, vp, vp, vp, vp]
+    L.qs_expm_multiply.argtypes = [vp, i32, vp, i64, vp, dbl, i32, vp, vp]
+    for name in EXPORTS:
+        if name != "qs_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().qs_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")

@@ -1,0 +1,130 @@
+"""SpinBasis1D on the GPU (basis.jl:622-777)."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .errors import ArgumentError, check
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _blk(v):
+    return _lib.QS_UNSET if v is None else int(v)
+
+
+class SpinBasis1D:
+    """Spin-1/2 chain basis in a particle-number and symmetry sector.
+
+    Arguments follow SpinBasis1D(L; nup, Nup, m, S, pauli, a, kblock, pblock,
+    zblock, pzblock, zAblock, zBblock) in basis.jl:622-639.  The sector is
+    enumerated on the GPU; ``states`` are the reference's representative
+    labels in column order and ``reps`` the orbit-minimum keys the kernels use.
+    """
+
+    def __init__(self, L, nup=None, Nup=None, m=None, S="1/2", pauli=True, a=1, kblock=None, pblock=None,
+                 zblock=None, pzblock=None, zAblock=None, zBblock=None):
+        if str(S) not in ("1/2", "0.5"):
+            raise ArgumentError("only spin-1/2 chains are on the GPU path")
+        if nup is not None and Nup is not None:
+            raise ArgumentError("specify only one of nup and Nup")
+        sel = Nup if nup is None else nup
+        if m is not None:
+            if sel is not None:
+                raise ArgumentError("m cannot be combined with nup or Nup")
+            sel = int(round((float(m) + 0.5) * L))
+        self.L = int(L)
+        self.nup = None if sel is None else int(sel)
+        self.pauli = bool(pauli)
+        self.a = int(a)
+        self._blocks = dict(kblock=kblock, pblock=pblock, zblock=zblock, pzblock=pzblock, zAblock=zAblock,
+                            zBblock=zBblock)
+        h = ctypes.c_void_p()
+        check(_lib.lib().qs_basis_create(self.L, _blk(self.nup), int(self.pauli), self.a, _blk(kblock), _blk(pblock),
+                                          _blk(zblock), _blk(pzblock), _blk(zAblock), _blk(zBblock), 0,
+                                          ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        check(_lib.lib().qs_basis_size(self._h, ctypes.byref(n)))
+        self.Ns = int(n.value)
+        c = ctypes.c_int()
+        check(_lib.lib().qs_basis_is_complex(self._h, ctypes.byref(c)))
+        self.is_complex = bool(c.value)
+        self._states = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.lib().qs_basis_destroy(h)
+            self._h = None
+
+    # basis.jl:779-812 -------------------------------------------------------
+    def __len__(self):
+        return self.Ns
+
+    @property
+    def N(self):
+        return self.L
+
+    @property
+    def blocks(self):
+        out = {"nup": self.nup, "pauli": self.pauli}
+        for k, v in self._blocks.items():
+            if v is not None:
+                out[k] = int(v)
+        if self._blocks["kblock"] is not None:
+            out["a"] = self.a
+            out["kblock"] = int(self._blocks["kblock"]) % (self.L // self.a)
+        return out
+
+    @property
+    def states(self):
+        if self._states is None:
+            out = np.empty(self.Ns, dtype=np.uint64)
+            check(_lib.lib().qs_basis_states(self._h, _ptr(out)))
+            self._states = out
+        return self._states.copy()
+
+    @property
+    def reps(self):
+        out = np.empty(self.Ns, dtype=np.uint64)
+        check(_lib.lib().qs_basis_reps(self._h, _ptr(out)))
+        return out
+
+    def norms(self):
+        """(orbit sizes, reference normalization at each label)."""
+        orb = np.empty(self.Ns, dtype=np.uint32)
+        ref = np.empty(self.Ns, dtype=np.float64)
+        check(_lib.lib().qs_basis_norms(self._h, _ptr(orb), _ptr(ref)))
+        return orb, ref
+
+    def normalization(self, state=None):
+        orb, ref = self.norms()
+        if state is None:
+            return ref
+        idx, _ = self.index(np.asarray([state], dtype=np.uint64))
+        return ref[idx[0]]
+
+    def index(self, states):
+        """Column of each state's orbit (-1 if outside the sector) and <state|column>."""
+        s = np.ascontiguousarray(np.asarray(states, dtype=np.uint64))
+        idx = np.empty(s.size, dtype=np.int64)
+        amp = np.empty(s.size, dtype=np.complex128)
+        check(_lib.lib().qs_basis_lookup(self._h, _ptr(s), s.size, _ptr(idx), _ptr(amp)))
+        return idx, amp
+
+    def state_index(self, state):
+        idx, _ = self.index([state])
+        if idx[0] < 0:
+            raise ArgumentError("state is outside this basis")
+        return int(idx[0])
+
+    def project_to(self, parent_vector, parent_states):
+        """P' v for a vector given on explicit parent states (used for initial states)."""
+        idx, amp = self.index(parent_states)
+        out = np.zeros(self.Ns, dtype=np.complex128)
+        ok = idx >= 0
+        np.add.at(out, idx[ok], np.conj(amp[ok]) * np.asarray(parent_vector)[ok])
+        return out

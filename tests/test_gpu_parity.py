@@ -1,0 +1,221 @@
+"""GPU parity: libqsgpu (through its C ABI) against the CPU oracle.
+
+Bars: integer basis work bit-exact (states, order, orbit sizes, reference
+labels and normalizations); FP64 matvec within 1e-12 relative; FP32-stream
+matvec within 2e-6 relative; eigenvalues within 1e-10 (FP64) of dense /
+oracle results; Krylov exponentials within 3e-11 (the reference's own
+evolve-vs-dense tolerance, completeness_gaps.jl:241-253).
+"""
+import numpy as np
+import pytest
+
+import _dense
+import oracle
+import qsgpu
+
+pytestmark = pytest.mark.gpu
+
+SECTORS = [
+    (8, 4, {}), (8, None, {}), (8, 4, dict(kblock=0)), (8, 4, dict(kblock=1)), (8, 4, dict(kblock=3)),
+    (8, 4, dict(kblock=4)), (8, 4, dict(pblock=1)), (8, 4, dict(pblock=-1)), (8, 4, dict(zblock=1)),
+    (8, 4, dict(zblock=-1)), (8, 4, dict(kblock=0, pblock=1)), (8, 4, dict(kblock=4, pblock=-1, zblock=1)),
+    (8, 4, dict(kblock=0, pblock=1, zblock=1)), (8, 4, dict(kblock=2, zblock=-1)), (8, 4, dict(pzblock=1)),
+    (8, 4, dict(kblock=0, pzblock=-1)), (8, None, dict(zAblock=1)), (8, None, dict(zBblock=-1, zAblock=-1)),
+    (10, 5, dict(kblock=0, pblock=1, zblock=1)), (10, 5, dict(kblock=3, zblock=1)), (12, 6, dict(kblock=0, pblock=-1, zblock=-1)),
+    (12, 6, dict(kblock=6, pblock=1, zblock=1)), (12, 6, dict(kblock=5)), (12, 4, dict(kblock=0, pblock=1)),
+    (12, 6, dict(zblock=1)), (12, 6, dict(kblock=0, a=2)), (10, None, dict(kblock=1)), (16, 8, {}),
+    (16, 8, dict(kblock=0, pblock=1, zblock=1)), (20, 10, dict(kblock=0, pblock=1, zblock=1)),
+    (18, 9, dict(kblock=3, zblock=1)),
+]
+
+
+def _id(c):
+    L, nup, b = c
+    return "L%d_n%s_%s" % (L, nup, "_".join("%s%s" % (k[0], v) for k, v in sorted(b.items())) or "plain")
+
+
+def _column_minima(sec):
+    colptr, row, _ = sec.projector()
+    par = sec.parent_states()
+    return np.array([par[row[colptr[c]:colptr[c + 1]]].min() for c in range(sec.Ns)], dtype=np.uint64)
+
+
+def _terms(L, periodic=True):
+    return _dense.xxz_terms(L, J=1.0, delta=0.7, J2=0.35, periodic=periodic)
+
+
+@pytest.mark.parametrize("case", SECTORS, ids=_id)
+def test_basis_bit_exact(gpu, case):
+    L, nup, blocks = case
+    print("basis", _id(case), flush=True)
+    ref = oracle.Sector(L, nup=nup, pauli=False, **blocks)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
+    assert b.Ns == ref.Ns
+    if ref.reduced:
+        assert b.reps.tolist() == _column_minima(ref).tolist()
+    else:
+        assert b.reps.tolist() == ref.parent_states().tolist()
+    assert b.states.tolist() == ref.states().tolist()
+    orb, nrm = b.norms()
+    orb_ref, nrm_ref = ref.norms()
+    assert orb.tolist() == orb_ref.tolist()
+    assert nrm.tobytes() == nrm_ref.tobytes()
+
+
+@pytest.mark.parametrize("case", SECTORS, ids=_id)
+def test_matvec_fp64_matches_oracle(gpu, case):
+    L, nup, blocks = case
+    print("matvec", _id(case), flush=True)
+    terms = _terms(L)
+    if nup is None:
+        terms = terms + [("x", [(0.3, i) for i in range(1, L + 1)])]
+    ref_s = oracle.Sector(L, nup=nup, pauli=True, **blocks)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=True, **blocks)
+    ref = oracle.Operator(ref_s, terms, check_symm=False)
+    op = qsgpu.QuantumLinearOperator(b, terms, check_symm=False, check_pcon=nup is not None)
+    rng = np.random.default_rng(L * 100 + b.Ns)
+    x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    want = ref.matvec(x)
+    got = op.matvec(x.astype(np.complex128))
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    if not b.is_complex:
+        xr = x.real.copy()
+        got_r = op.matvec(xr)
+        assert got_r.dtype == np.float64
+        assert np.linalg.norm(got_r - ref.matvec(xr).real) <= 1e-12 * np.linalg.norm(want)
+        got32 = op.matvec(xr.astype(np.float32), dtype=np.float32)
+        assert np.linalg.norm(got32 - ref.matvec(xr).real) <= 2e-6 * np.linalg.norm(want)
+    got64c = op.matvec(x.astype(np.complex64), dtype=np.complex64)
+    assert np.linalg.norm(got64c - want) <= 2e-6 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("blocks", [dict(kblock=1), dict(kblock=0), dict(kblock=2, zblock=-1)], ids=str)
+def test_matvec_nonsymmetric_terms_flags_and_diagonal(gpu, blocks):
+    # P'HP of terms that break the symmetry (check_symm=false), a complex
+    # non-Hermitian term list, a user diagonal, alpha/beta, transpose/conj/adjoint
+    L, nup = 10, 5
+    terms = [("+-", [(0.7 + 0.2j, 1, 2), (0.4, 3, 4)]), ("-+", [(0.3, 2, 5)]), ("z", [(0.9, 1), (-0.2, 7)]),
+             ("zz", [(0.55, 2, 9)]), ("+z-", [(0.25j, 3, 6, 8)])]
+    print("flags", blocks, flush=True)
+    ref_s = oracle.Sector(L, nup=nup, pauli=False, **blocks)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
+    rng = np.random.default_rng(11)
+    diag = rng.standard_normal(b.Ns) + 0.5j * rng.standard_normal(b.Ns)
+    ref = oracle.Operator(ref_s, terms, check_symm=False, diagonal=diag)
+    op = qsgpu.QuantumLinearOperator(b, terms, diagonal=diag, check_symm=False, check_herm=False)
+    x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    y0 = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    for t, c in [(False, False), (True, False), (False, True), (True, True)]:
+        want = ref.matvec(x, y=y0, alpha=1.25 - 0.5j, beta=-0.5 + 0.25j, transposed=t, conjugated=c)
+        opf = op if not (t or c) else (op.transpose() if t and not c else (op.conj() if c and not t else op.adjoint()))
+        out = y0.copy()
+        opf.mul(out, x, alpha=1.25 - 0.5j, beta=-0.5 + 0.25j)
+        assert np.linalg.norm(out - want) <= 1e-12 * np.linalg.norm(want), (t, c)
+    # a matrix right-hand side applies column by column (operators.jl:3185-3208)
+    X = np.asfortranarray(rng.standard_normal((b.Ns, 3)) + 1j * rng.standard_normal((b.Ns, 3)))
+    Y = np.zeros_like(X)
+    op.mul(Y, X)
+    for j in range(3):
+        assert np.linalg.norm(Y[:, j] - ref.matvec(X[:, j])) <= 1e-12 * np.linalg.norm(Y[:, j])
+
+
+@pytest.mark.parametrize("case", [(12, 6, dict(kblock=0, pblock=1, zblock=1)), (12, 6, dict(kblock=3)),
+                                  (12, 6, {}), (14, 7, dict(zblock=-1)), (12, 6, dict(kblock=6, pblock=-1))], ids=_id)
+@pytest.mark.parametrize("which", ["SA", "LA", "LM"])
+def test_eigsh_fp64_against_dense(gpu, case, which):
+    L, nup, blocks = case
+    print("eigsh", _id(case), which, flush=True)
+    terms = _terms(L)
+    ref = oracle.Operator(oracle.Sector(L, nup=nup, pauli=False, **blocks), terms, check_symm=False).dense()
+    ev = np.linalg.eigvalsh(ref)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
+    op = qsgpu.QuantumLinearOperator(b, terms)
+    k = 4
+    vals, vecs = qsgpu.eigsh(op, k=k, which=which)
+    if which == "SA":
+        want = ev[:k]
+    elif which == "LA":
+        want = ev[::-1][:k]
+    else:
+        want = ev[np.argsort(-np.abs(ev), kind="stable")][:k]
+    assert np.allclose(vals, want, atol=1e-10, rtol=0)
+    for j in range(k):
+        r = ref @ vecs[:, j] - vals[j] * vecs[:, j]
+        assert np.linalg.norm(r) < 1e-8
+
+
+def test_eigsh_c1_against_arpack(gpu):
+    # C1: L=16 nup=8 XXZ Delta=0.5, eigsh k=4 :SA against scipy's ARPACK on the oracle CSR
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    L = 16
+    terms = [("+-", [(0.5, i, i % L + 1) for i in range(1, L + 1)]), ("-+", [(0.5, i, i % L + 1) for i in range(1, L + 1)]),
+             ("zz", [(0.5, i, i % L + 1) for i in range(1, L + 1)])]
+    sec = oracle.Sector(L, nup=8, pauli=False)
+    rowptr, col, val = oracle.CSR(sec, terms).export()
+    A = sp.csr_matrix((val, col, rowptr), shape=(sec.Ns, sec.Ns))
+    want = np.sort(sla.eigsh(A, k=4, which="SA", tol=0)[0])
+    b = qsgpu.SpinBasis1D(L, nup=8, pauli=False)
+    op = qsgpu.QuantumLinearOperator(b, terms)
+    rng = np.random.default_rng(0x51a7)
+    v0 = rng.standard_normal(b.Ns)
+    vals, vecs = qsgpu.eigsh(op, k=4, which="SA", v0=v0 / np.linalg.norm(v0))
+    assert np.allclose(vals, want, atol=1e-10, rtol=0)
+    x = rng.standard_normal(b.Ns)
+    assert np.linalg.norm(op.matvec(x) - A @ x) <= 1e-12 * np.linalg.norm(A @ x)
+
+
+@pytest.mark.parametrize("case", [(10, 5, dict(kblock=0, pblock=1, zblock=1)), (10, 5, dict(kblock=2)), (8, 4, {})],
+                         ids=_id)
+def test_expm_against_dense(gpu, case):
+    import scipy.linalg as sl
+    L, nup, blocks = case
+    print("expm", _id(case), flush=True)
+    terms = _dense.xxz_terms(L, J=1.0, delta=0.9)
+    ref = oracle.Operator(oracle.Sector(L, nup=nup, pauli=False, **blocks), terms).dense()
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
+    op = qsgpu.QuantumLinearOperator(b, terms)
+    n = b.Ns
+    v = np.sin(np.arange(1, n + 1)) + 1j * np.cos(2 * np.arange(1, n + 1))
+    v /= np.linalg.norm(v)
+    times = [0.0, 0.17, 0.83, 3.0]
+    out = qsgpu.evolve(op, v, 0.0, times)
+    for j, t in enumerate(times):
+        want = sl.expm(-1j * t * ref) @ v
+        assert np.linalg.norm(out[:, j] - want) < 3e-11
+        assert abs(np.linalg.norm(out[:, j]) - 1) < 3e-12
+    e = qsgpu.ExpmMultiplyParallel(op, -0.37j)
+    assert np.linalg.norm(e.apply(v) - sl.expm(-0.37j * ref) @ v) < 3e-11
+    # imaginary time renormalises (operators.jl:2157-2161)
+    out = qsgpu.evolve(op, v, 0.0, [0.5], imag_time=True)
+    want = sl.expm(-0.5 * ref) @ v
+    assert np.linalg.norm(out[:, 0] - want / np.linalg.norm(want)) < 3e-11
+
+
+def test_fullsize_c2_basis_and_ground_state(gpu):
+    # C2: J1-J2 XXZ L=28, nup=14, k=0, p=+1, z=+1.  Size-independent properties:
+    # dimension from the character formula, orbit sizes summing to C(28,14),
+    # Hermiticity <x,Hy> = <Hx,y>, an FP64 ground state with a small residual,
+    # and the FP32-stream eigenvalue within 1e-8 of the FP64 one.
+    import math
+    L = 28
+    print("c2 basis", flush=True)
+    b = qsgpu.SpinBasis1D(L, nup=14, pauli=False, kblock=0, pblock=1, zblock=1)
+    assert b.Ns == 361270
+    orb, _ = b.norms()
+    assert int(orb.astype(np.int64).sum()) == math.comb(28, 14)
+    reps = b.reps
+    assert np.all(np.diff(reps.astype(np.int64)) > 0)
+    terms = _dense.xxz_terms(L, J=1.0, delta=1.0, J2=0.4)
+    op = qsgpu.QuantumLinearOperator(b, terms)
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal(b.Ns), rng.standard_normal(b.Ns)
+    assert abs(x @ op.matvec(y) - op.matvec(x) @ y) < 1e-10 * abs(x @ op.matvec(y)) + 1e-9
+    print("c2 eigsh fp64", flush=True)
+    v0 = rng.standard_normal(b.Ns)
+    e64, v64 = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0)
+    r = op.matvec(v64[:, 0]) - e64[0] * v64[:, 0]
+    assert np.linalg.norm(r) < 1e-6
+    print("c2 eigsh fp32", flush=True)
+    e32, _ = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0, dtype=np.float32)
+    assert abs(e32[0] - e64[0]) <= 1e-8
