@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-eigsh", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; the multi-GPU sharding is a design note only "
@@ -115,6 +117,14 @@ def main():
                      "kernel": "k_matvec<uint32,double,real,STORED>",
                      "note": "canonicalisation over the 112-element group dominates; integer-ALU bound at this N"},
     }
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        result["copy_calibration"] = {"bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8, "launches": 3}
+        del src, dst
     if not args.no_eigsh:
         print("bench: FP64 Lanczos ground state", file=sys.stderr, flush=True)
         v0 = np.random.default_rng(0x51a7).standard_normal(N)

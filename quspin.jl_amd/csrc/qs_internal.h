@@ -196,10 +196,17 @@ void herm_eig(int n, std::vector<cd> &A, std::vector<double> &w, std::vector<cd>
 void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E);
 
 // --------------------------------------------------------- kernels -------
+struct VecWork;
 // y = beta*y + alpha*K x on the sector; K = M (adjoint=false) or M^dagger.
 // conj_x / conj_y conjugate x on load and y on load and store.
+// dot_partial (optional): matvec_grid(Ns) x 2 doubles receiving per-block
+// FP64 partial sums of <x, y_new> computed before y is rounded to its dtype.
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim,
-                   double bre, double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s);
+                   double bre, double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s,
+                   double *dot_partial = nullptr);
+int matvec_grid(int64_t n);
+// <x, A x> in FP64 with x stored in `dtype` (fused into the matvec epilogue); y is scratch
+cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk);
 
 // vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
 struct VecWork {

@@ -193,11 +193,26 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     res.vals.resize(k);
     for (int i = 0; i < k; i++) res.vals[i] = theta[out[i]];
     res.nconv = nconv;
-    if (vecs_dev) {
+    const bool fp32 = dtype == QS_F32 || dtype == QS_C64;
+    if (vecs_dev || fp32) {
         std::vector<cd> C((size_t)m * k);
         for (int c = 0; c < k; c++)
             for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + out[c]], 0);
-        basis_combine(dtype, K.V.get(), n, m, C, k, vecs_dev, s);
+        Krylov tmp(dtype, n, vecs_dev ? 0 : k);
+        void *dst = vecs_dev ? vecs_dev : tmp.V.get();
+        basis_combine(dtype, K.V.get(), n, m, C, k, dst, s);
+        if (fp32) {
+            // FP32-stored Ritz vectors: report the FP64-accumulated Rayleigh
+            // quotient <v, Hv>/<v, v> (the product is reduced in FP64 before
+            // any rounding of Hv to FP32)
+            for (int c = 0; c < k; c++) {
+                const void *v = (const char *)dst + (size_t)c * n * K.esz;
+                cd num = rayleigh_numerator(op, dtype, v, K.col(m), wk);
+                double den = vec_norm2(dtype, v, n, wk, s);
+                res.vals[c] = num.real() / den;
+                res.nmatvec++;
+            }
+        }
     }
     QS_HIP(hipStreamSynchronize(s));
 }

@@ -13,6 +13,8 @@
 // where u runs over the states the adjoint couplings reach from s.  One lane
 // owns one row: no atomics, a fixed summation order, so results do not
 // depend on what else runs on the device.
+#include <algorithm>
+
 #include "qs_device.h"
 
 namespace qs {
@@ -43,6 +45,7 @@ struct MVArgs {
     double are, aim, bre, bim;
     int conj_x, conj_y;
     int beta_zero;
+    double *dot_partial;   // optional: per-block FP64 partials of <x, y>
 };
 
 // Amplitude of one compiled term on state s: A * (-1)^popcount(s & smask),
@@ -78,6 +81,8 @@ __device__ __forceinline__ int64_t search_rep(const S *reps, const S *lsamp, int
 template <typename S, typename V, bool CC, int MODE, bool TRACK>
 __global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restrict__ x, V *__restrict__ y) {
     using T = VTraits<V>;
+    __shared__ double s_red[2 * kMvBlock];
+    double dot_r = 0, dot_i = 0;
     __shared__ S s_samp[MODE == MODE_STORED ? kMaxSamples : 1];
     __shared__ uint32_t s_binom[MODE == MODE_STORED ? 1 : 33 * 65];
     if (MODE == MODE_STORED) {
@@ -177,7 +182,28 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restri
             oi += A.bre * yi + A.bim * yr;
         }
         if (A.conj_y) oi = -oi;
+        if (A.dot_partial) {
+            // FP64 <x, y_row> of the unrounded result (Rayleigh quotients of FP32-stored vectors)
+            dot_r += xr * orr + xi * oi;
+            dot_i += xr * oi - xi * orr;
+        }
         T::store(y, row, orr, oi);
+    }
+    if (A.dot_partial) {
+        s_red[threadIdx.x] = dot_r;
+        s_red[kMvBlock + threadIdx.x] = dot_i;
+        __syncthreads();
+        for (int st = kMvBlock / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                s_red[threadIdx.x] += s_red[threadIdx.x + st];
+                s_red[kMvBlock + threadIdx.x] += s_red[kMvBlock + threadIdx.x + st];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            A.dot_partial[2 * blockIdx.x] = s_red[0];
+            A.dot_partial[2 * blockIdx.x + 1] = s_red[kMvBlock];
+        }
     }
 }
 
@@ -239,8 +265,13 @@ GroupDev group_dev(const GroupHost &h) {
 
 }  // namespace
 
+int matvec_grid(int64_t n) {
+    int64_t blocks = (n + kMvBlock - 1) / kMvBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 65536));
+}
+
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
-                   double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s) {
+                   double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
     const BasisHost &b = *op.basis;
     MVArgs a;
     memset(&a, 0, sizeof a);
@@ -279,6 +310,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     a.beta_zero = (bre == 0.0 && bim == 0.0);
     a.conj_x = conj_x ? 1 : 0;
     a.conj_y = conj_y ? 1 : 0;
+    a.dot_partial = dot_partial;
     bool track = false;
     for (int p = 0; p < b.group.npoint; p++)
         if (b.group.pchar[p] != cd(1, 0)) track = true;
@@ -286,9 +318,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         if (b.group.lam[t] != cd(1, 0)) track = true;
     bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
-    int64_t blocks = (b.Ns + kMvBlock - 1) / kMvBlock;
-    int grid = (int)std::min<int64_t>(blocks, 65536);
-    if (grid < 1) grid = 1;
+    int grid = matvec_grid(b.Ns);
     if (b.use64) dispatch_v<uint64_t>(a, dtype, cc, b.mode, track, x, y, grid, s);
     else dispatch_v<uint32_t>(a, dtype, cc, b.mode, track, x, y, grid, s);
     QS_HIP(hipGetLastError());
