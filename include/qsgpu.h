@@ -128,6 +128,15 @@ int qs_op_set_diagonal(qs_op *op, const double *diag);
 int qs_op_set_stream(qs_op *op, void *stream);
 int qs_op_stream(const qs_op *op, void **stream);
 
+/* Matvec kernel variant: lanes cooperating on one row (1, 2, 4 or 8; 0 =
+ * default).  No reference counterpart: the reference has one CPU loop. */
+int qs_op_set_variant(qs_op *op, int lanes_per_row);
+
+/* Autotuner: times every variant for `dtype` on device-resident random
+ * vectors (`repeats` launches each), keeps the fastest on the operator and
+ * returns it with its time.  No reference counterpart. */
+int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *best_ms);
+
 /* mul!(Y, op, X, alpha, beta) for ncols column vectors --
  * operators.jl:3147-3208; transposed/conjugated select transpose(op),
  * conj(op) and adjoint(op) (operators.jl:3286-3296).

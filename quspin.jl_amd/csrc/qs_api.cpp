@@ -171,7 +171,7 @@ void upload_terms(const std::vector<CTerm> &src, DevBuf<TermDev> &dst) {
 
 void upload_groups(const std::vector<GroupOfTerms> &src, DevBuf<GroupOfTerms> &dst) {
     std::vector<GroupOfTerms> h(src);
-    if (h.empty()) h.push_back(GroupOfTerms{0, 0, 0});
+    if (h.empty()) h.push_back(GroupOfTerms{0, 0, 0, -1, -1});
     dst.alloc(h.size());
     QS_HIP(hipMemcpy(dst.get(), h.data(), sizeof(GroupOfTerms) * h.size(), hipMemcpyHostToDevice));
 }
@@ -486,6 +486,54 @@ int qs_op_stream(const qs_op *op, void **stream) {
         check_ptr(op, "op");
         check_ptr(stream, "stream");
         *stream = (void *)op->o.stream;
+    });
+}
+
+int qs_op_set_variant(qs_op *op, int lanes_per_row) {
+    return guard([&] {
+        check_ptr(op, "op");
+        if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8)
+            fail(QS_EARG, "lanes_per_row must be 0, 1, 2, 4 or 8");
+        op->o.lanes_per_row = lanes_per_row;
+    });
+}
+
+int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *best_ms) {
+    return guard([&] {
+        check_ptr(op, "op");
+        OpHost &o = op->o;
+        if (o.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+        hipStream_t s = ensure_stream(o);
+        const int64_t n = o.basis->Ns;
+        DevBuf<char> x((size_t)n * dtype_size(dtype)), y((size_t)n * dtype_size(dtype));
+        vec_random(dtype, x.get(), n, 0x7a11ULL, s);
+        hipEvent_t e0, e1;
+        QS_HIP(hipEventCreate(&e0));
+        QS_HIP(hipEventCreate(&e1));
+        int best = 0;
+        float best_t = 1e30f;
+        if (repeats < 1) repeats = 3;
+        for (int lpr : {1, 2, 4, 8}) {
+            o.lanes_per_row = lpr;
+            launch_matvec(o, dtype, x.get(), y.get(), 1, 0, 0, 0, false, false, false, s);
+            QS_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < repeats; r++)
+                launch_matvec(o, dtype, x.get(), y.get(), 1, 0, 0, 0, false, false, false, s);
+            QS_HIP(hipEventRecord(e1, s));
+            QS_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            QS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= repeats;
+            if (ms < best_t) {
+                best_t = ms;
+                best = lpr;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        o.lanes_per_row = best;
+        if (lanes_per_row) *lanes_per_row = best;
+        if (best_ms) *best_ms = best_t;
     });
 }
 

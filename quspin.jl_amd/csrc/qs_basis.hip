@@ -138,29 +138,9 @@ __global__ void k_samples(const S *reps, int64_t n, int shift, int64_t nsample, 
     if (i < nsample) samp[i] = reps[i << shift];
 }
 
-GroupDev to_dev(const GroupHost &h) {
-    GroupDev d;
-    memset(&d, 0, sizeof d);
-    d.L = h.L;
-    d.a = h.a;
-    d.nT = h.nT;
-    d.npoint = h.npoint;
-    for (int p = 0; p < h.npoint; p++) {
-        d.prev[p] = h.prev[p];
-        d.pflip[p] = h.pflip[p];
-        d.pch_re[p] = h.pchar[p].real();
-        d.pch_im[p] = h.pchar[p].imag();
-    }
-    for (int t = 0; t < h.nT; t++) {
-        d.lam_re[t] = h.lam[t].real();
-        d.lam_im[t] = h.lam[t].imag();
-    }
-    return d;
-}
-
 template <typename S>
 void build_stored(BasisHost &b, int64_t npar) {
-    GroupDev g = to_dev(b.group);
+    GroupDev g = group_to_dev(b.group);
     S mask = (S)full_mask(b.L);
     int64_t lanes = (npar + kChunk - 1) / kChunk;
     int64_t nblk = (lanes + kBlock - 1) / kBlock;

@@ -91,6 +91,34 @@ __device__ __forceinline__ S canonical(const GroupDev &g, S u, S mask, int &bt, 
     return best;
 }
 
+// L <= 32: the same minimum from 32-bit funnel shifts.  P is the state
+// repeated periodically over 64 bits; the window (P >> s)[31:0] carries a
+// rotation R of the state in its top L bits and, below it, bits that are a
+// function of R alone (periodicity).  Comparing windows therefore orders the
+// rotations, at one v_alignbit_b32 and one min per group element.
+template <bool TRACK>
+__device__ __forceinline__ uint32_t canonical32(const GroupDev &g, uint32_t u, int &bt, int &bp) {
+    uint32_t best = 0xFFFFFFFFu;
+    bt = 0;
+    bp = 0;
+    const int L = g.L;
+    for (int p = 0; p < g.npoint; p++) {
+        uint32_t v = point_apply(g, p, u);
+        uint64_t P = v;
+        for (int k = L; k < 64; k += L) P |= (uint64_t)v << k;
+        const uint32_t lo = (uint32_t)P, hi = (uint32_t)(P >> 32);
+        for (int t = 0; t < g.nT; t++) {
+            uint32_t w = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)g.win[t]);
+            if (TRACK) {
+                if (w < best) { best = w; bt = t; bp = p; }
+            } else {
+                best = w < best ? w : best;
+            }
+        }
+    }
+    return best >> (32 - L);
+}
+
 // Combinadic (colex) rank of a word with w set bits: sum_k C(p_k, k).
 // binom is laid out as C(n, k) at [n * 65 + k].
 template <typename S, typename B>

@@ -51,6 +51,33 @@ int64_t binom(int n, int k)
     return r;
 }
 
+GroupDev group_to_dev(const GroupHost &h)
+{
+    GroupDev d;
+    memset(&d, 0, sizeof d);
+    d.L = h.L;
+    d.a = h.a;
+    d.nT = h.nT;
+    d.npoint = h.npoint;
+    for (int p = 0; p < h.npoint; p++) {
+        d.prev[p] = h.prev[p];
+        d.pflip[p] = h.pflip[p];
+        d.pch_re[p] = h.pchar[p].real();
+        d.pch_im[p] = h.pchar[p].imag();
+    }
+    for (int t = 0; t < h.nT; t++) {
+        d.lam_re[t] = h.lam[t].real();
+        d.lam_im[t] = h.lam[t].imag();
+        // T^t = rotl by t*a = rotr by o; the 32-bit window at offset s holds
+        // that rotation in its top L bits when s + 32 - L = o (mod L)
+        if (h.L <= 32) {
+            int o = ((h.L - t * h.a) % h.L + h.L) % h.L;
+            d.win[t] = ((o + h.L - 32) % h.L + h.L) % h.L;
+        }
+    }
+    return d;
+}
+
 // --------------------------------------------------------------- groups --
 namespace {
 
@@ -443,6 +470,11 @@ void split_terms(const std::vector<CTerm> &all, std::vector<CTerm> &diag, std::v
     for (uint64_t f : order) {
         GroupOfTerms gr;
         gr.flip = f;
+        gr.lo = gr.hi = -1;
+        if (__builtin_popcountll(f) == 2) {
+            gr.lo = __builtin_ctzll(f);
+            gr.hi = 63 - __builtin_clzll(f);
+        }
         gr.t0 = (int)off.size();
         for (auto &x : by[f]) off.push_back(x);
         gr.t1 = (int)off.size();

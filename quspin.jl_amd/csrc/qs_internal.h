@@ -84,6 +84,9 @@ struct GroupHost {
 // Device-side copy passed by value in kernel arguments.
 struct GroupDev {
     int L, a, nT, npoint;
+    // L <= 32: funnel-shift window offset that reads the rotation T^t of a
+    // periodic 64-bit copy of the state (see canonical32)
+    int win[kMaxRot];
     int prev[kMaxPoint];
     uint64_t pflip[kMaxPoint];
     double pch_re[kMaxPoint], pch_im[kMaxPoint];
@@ -106,6 +109,7 @@ struct TermDev {
 struct GroupOfTerms {   // off-diagonal terms sharing one flip mask
     uint64_t flip;
     int t0, t1;
+    int lo, hi;          // the two flipped bits when the mask has exactly two, else -1
 };
 
 // Basis modes.
@@ -163,10 +167,13 @@ struct OpHost {
     DevBuf<double> d_user_diag;      // interleaved complex
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
+    int lanes_per_row = 0;       // matvec kernel variant (0 = default heuristic)
     // staging buffers for host vectors
     DevBuf<char> stage_x, stage_y;
     ~OpHost();
 };
+
+GroupDev group_to_dev(const GroupHost &h);
 
 // ---------------------------------------------------------- host logic ---
 uint64_t full_mask(int L);

@@ -17,7 +17,7 @@ EXPORTS = (
     "qs_last_error", "qs_device_info", "qs_basis_create", "qs_basis_destroy", "qs_basis_size",
     "qs_basis_states", "qs_basis_reps", "qs_basis_norms", "qs_basis_is_complex", "qs_basis_lookup",
     "qs_op_create", "qs_op_destroy", "qs_op_info", "qs_op_set_diagonal", "qs_op_set_stream",
-    "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply",
+    "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply", "qs_op_set_variant", "qs_op_tune",
 )
 
 _lib = None
@@ -57,6 +57,8 @@ def lib():
     L.qs_op_matvec.argtypes = [vp, i32, vp, vp, i64, dbl, dbl, dbl, dbl, i32, i32, i32]
     L.qs_eigsh.argtypes = [vp, i32, i32, i32, dbl, i32, i32, vp, vp, vp, vp, vp]
     L.qs_expm_multiply.argtypes = [vp, i32, vp, i64, vp, dbl, i32, vp, vp]
+    L.qs_op_set_variant.argtypes = [vp, i32]
+    L.qs_op_tune.argtypes = [vp, i32, i32, vp, vp]
     for name in EXPORTS:
         if name != "qs_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -229,6 +229,18 @@ class QuantumLinearOperator:
                                        int(self.transposed), int(self.conjugated), 1))
         return y
 
+    def set_variant(self, lanes_per_row):
+        """Fix the matvec kernel variant (lanes cooperating on one row; 0 = default)."""
+        check(_lib.lib().qs_op_set_variant(self._h, int(lanes_per_row)))
+        return self
+
+    def tune(self, dtype=None, repeats=5):
+        """Time every kernel variant for `dtype` and keep the fastest; returns (lanes_per_row, ms)."""
+        dt = np.dtype(dtype) if dtype is not None else self._vec_dtype(np.float64)
+        lpr, ms = ctypes.c_int(), ctypes.c_double()
+        check(_lib.lib().qs_op_tune(self._h, _DT[dt], int(repeats), ctypes.byref(lpr), ctypes.byref(ms)))
+        return lpr.value, ms.value
+
     def set_stream(self, stream_ptr):
         check(_lib.lib().qs_op_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
 

@@ -219,3 +219,31 @@ def test_fullsize_c2_basis_and_ground_state(gpu):
     print("c2 eigsh fp32", flush=True)
     e32, _ = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0, dtype=np.float32)
     assert abs(e32[0] - e64[0]) <= 1e-8
+
+
+@pytest.mark.parametrize("case", [(12, 6, {}), (12, None, {}), (12, 6, dict(zblock=-1)), (12, None, dict(zblock=1)),
+                                  (12, 6, dict(kblock=0, pblock=1, zblock=1)), (12, 6, dict(kblock=5)),
+                                  (12, 6, dict(kblock=6, pblock=-1, zblock=-1))], ids=_id)
+def test_every_kernel_variant_matches_oracle(gpu, case):
+    # lanes-per-row variants and the autotuner change only the summation grouping;
+    # long-range hops exercise the incremental combinadic rank with particles in between
+    L, nup, blocks = case
+    print("variants", _id(case), flush=True)
+    terms = _terms(L) + [("+-", [(0.3, i, (i + 4) % L + 1) for i in range(1, L + 1)]),
+                         ("-+", [(0.3, i, (i + 4) % L + 1) for i in range(1, L + 1)])]
+    if nup is None:
+        terms = terms + [("x", [(0.2, i) for i in range(1, L + 1)])]
+    ref = oracle.Operator(oracle.Sector(L, nup=nup, pauli=True, **blocks), terms, check_symm=False)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=True, **blocks)
+    op = qsgpu.QuantumLinearOperator(b, terms, check_symm=False, check_pcon=False)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    want = ref.matvec(x)
+    for lpr in (1, 2, 4, 8):
+        op.set_variant(lpr)
+        got = op.matvec(x)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), lpr
+    lpr, ms = op.tune(np.complex128, repeats=2)
+    assert lpr in (1, 2, 4, 8) and ms > 0
+    got = op.matvec(x)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
