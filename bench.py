@@ -93,7 +93,8 @@ def main():
     x /= x.norm()
     y = torch.zeros_like(x)
     bytes_alg = N * (8 + 8 + 4 + 8)
-    print("bench: timing %d matvecs (N=%d)" % (args.steps, N), file=sys.stderr, flush=True)
+    lpr, _ = op.tune(np.float64, repeats=5)
+    print("bench: timing %d matvecs (N=%d, lanes/row %d)" % (args.steps, N, lpr), file=sys.stderr, flush=True)
     t_mv, wall = time_matvec(op, x, y, args.steps, args.warmup)
     gbps = bytes_alg / t_mv / 1e9
     result = {
@@ -111,7 +112,8 @@ def main():
         "data": "synthetic (seeded Gaussian vector; Hamiltonian from closed-form couplings)",
         "config": {"workload": "C2: J1-J2 XXZ L=28 nup=14 k=0 p=+1 z=+1, N=%d, FP64 matrix-free matvec" % N,
                    "L": L, "nup": 14, "J1": 1.0, "J2": 0.4, "Delta": 1.0, "sector_dim": N,
-                   "bytes_per_matvec": bytes_alg, "basis_build_s": round(t_basis, 4), "device": dev["name"]},
+                   "bytes_per_matvec": bytes_alg, "basis_build_s": round(t_basis, 4), "device": dev["name"],
+                   "lanes_per_row": lpr},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 3), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(gbps / HBM_PEAK_GBPS, 5), "traffic": None,
                      "kernel": "k_matvec<uint32,double,real,STORED>",
@@ -149,8 +151,9 @@ def main():
         for dt, name, esz in ((torch.float32, "fp32", 4), (torch.float64, "fp64", 8)):
             x3 = torch.rand(n3, dtype=dt, device="cuda")
             y3 = torch.zeros_like(x3)
+            lpr3, _ = op3.tune(np.float32 if esz == 4 else np.float64, repeats=2)
             t3, _ = time_matvec(op3, x3, y3, max(3, args.steps // 10), 2)
-            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(n3 * 2 * esz / t3 / 1e9, 2),
+            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(n3 * 2 * esz / t3 / 1e9, 2), "lanes_per_row": lpr3,
                          "frac": round(n3 * 2 * esz / t3 / 1e9 / HBM_PEAK_GBPS, 4)}
             del x3, y3
             torch.cuda.empty_cache()
