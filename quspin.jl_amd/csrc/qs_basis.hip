@@ -20,6 +20,7 @@ namespace {
 
 constexpr int kChunk = 32;      // parent states per lane
 constexpr int kBlock = 256;
+constexpr int kSeg = 32;        // search segment (entries read by one scan)
 
 template <typename S>
 __device__ __forceinline__ S gosper_next(S s) {
@@ -161,15 +162,21 @@ void build_stored(BasisHost &b, int64_t npar) {
     b.Ns = (int64_t)total;
     DevBuf<uint64_t> d_off((size_t)nblk);
     QS_HIP(hipMemcpy(d_off.get(), off.data(), sizeof(uint64_t) * nblk, hipMemcpyHostToDevice));
-    DevBuf<S> reps((size_t)total);
+    // padded to whole 32-entry segments with the largest word, so the
+    // search can read any segment with full-width loads
+    const size_t padded = (total + kSeg - 1) / kSeg * kSeg;
+    DevBuf<S> reps(padded);
+    QS_HIP(hipMemset(reps.get(), 0xFF, padded * sizeof(S)));
     b.orbit.alloc((size_t)total);
     b.inv_nrm.alloc((size_t)total);
     hipLaunchKernelGGL((k_write<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
                        b.binom.get(), d_off.get(), reps.get(), b.orbit.get(), b.inv_nrm.get());
     QS_HIP(hipGetLastError());
-    // sampled search table: at most 4096 samples
-    int shift = 0;
-    while (((total + (1ULL << shift) - 1) >> shift) > 4096) shift++;
+    // sampled search table kept in LDS by the matvec: the first state of
+    // every 2^shift-entry segment, at most 48 KiB of samples
+    const uint64_t max_samples = 49152 / sizeof(S);
+    int shift = 5;
+    while (((total + (1ULL << shift) - 1) >> shift) > max_samples) shift++;
     b.sample_shift = shift;
     b.nsample = (int64_t)((total + (1ULL << shift) - 1) >> shift);
     DevBuf<S> samp((size_t)b.nsample);

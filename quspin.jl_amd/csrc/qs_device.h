@@ -99,23 +99,46 @@ __device__ __forceinline__ S canonical(const GroupDev &g, S u, S mask, int &bt, 
 template <bool TRACK>
 __device__ __forceinline__ uint32_t canonical32(const GroupDev &g, uint32_t u, int &bt, int &bp) {
     uint32_t best = 0xFFFFFFFFu;
-    bt = 0;
+    int bs = 0;
     bp = 0;
-    const int L = g.L;
+    const int L = g.L, a = g.a, nT = g.nT;
+    // window offset of T^0; each further translation moves it down by a
+    // (mod L).  Kept in scalar registers: no memory access inside the loops.
+    const int s0 = g.win[0];
     for (int p = 0; p < g.npoint; p++) {
         uint32_t v = point_apply(g, p, u);
         uint64_t P = v;
         for (int k = L; k < 64; k += L) P |= (uint64_t)v << k;
         const uint32_t lo = (uint32_t)P, hi = (uint32_t)(P >> 32);
-        for (int t = 0; t < g.nT; t++) {
-            uint32_t w = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)g.win[t]);
-            if (TRACK) {
-                if (w < best) { best = w; bt = t; bp = p; }
-            } else {
-                best = w < best ? w : best;
+        if (a == 1 && nT == L) {
+            // every offset 0..L-1 is some translation: scan them in order
+#pragma unroll 8
+            for (int s = 0; s < L; s++) {
+                const uint32_t w = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)s);
+                if (TRACK) {
+                    if (w < best) { best = w; bs = s; bp = p; }
+                } else {
+                    best = w < best ? w : best;
+                }
+            }
+        } else {
+            int s = s0;
+            for (int t = 0; t < nT; t++) {
+                const uint32_t w = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)s);
+                if (TRACK) {
+                    if (w < best) { best = w; bs = s; bp = p; }
+                } else {
+                    best = w < best ? w : best;
+                }
+                s -= a;
+                if (s < 0) s += L;
             }
         }
     }
+    // offset s belongs to T^t with s = s0 - t*a (mod L) and t*a < L
+    int d = s0 - bs;
+    if (d < 0) d += L;
+    bt = d / a;
     return best >> (32 - L);
 }
 

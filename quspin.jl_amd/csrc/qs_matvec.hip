@@ -31,7 +31,8 @@ namespace qs {
 
 namespace {
 
-constexpr int kMvBlock = 256;
+constexpr int kMvBlock = 256;   // implicit-basis kernel
+constexpr int kStBlock = 512;   // stored-basis kernel: larger blocks share one LDS search table
 constexpr int kMaxLdsGroups = 256;
 constexpr int kMaxLdsTerms = 512;
 
@@ -81,13 +82,43 @@ __device__ __forceinline__ int64_t search_rep(const S *reps, const S *lsamp, int
         int mid = (lo + hi) >> 1;
         if (lsamp[mid] <= r) lo = mid; else hi = mid;
     }
-    int64_t a = (int64_t)lo << shift, b = (int64_t)(lo + 1) << shift;
-    if (b > N) b = N;
-    while (b - a > 1) {
-        int64_t mid = (a + b) >> 1;
-        if (reps[mid] <= r) a = mid; else b = mid;
+    // narrow the sample's range to one 32-entry segment (steps only when
+    // the range holds more than one segment) ...
+    const int64_t nseg = (N + 31) >> 5;
+    int64_t sa = ((int64_t)lo << shift) >> 5, sb = (((int64_t)lo + 1) << shift) >> 5;
+    if (sb > nseg) sb = nseg;
+    while (sb - sa > 1) {
+        int64_t mid = (sa + sb) >> 1;
+        if (reps[mid << 5] <= r) sa = mid; else sb = mid;
     }
-    return reps[a] == r ? a : -1;
+    // ... and scan that segment with full-width loads (one round trip): the
+    // largest entry <= r sits at (count - 1).  The array is padded with the
+    // largest word to whole segments.
+    int cnt = 0;
+    S m = 0;
+    if constexpr (sizeof(S) == 4) {
+        const uint4 *p = (const uint4 *)(reps + (sa << 5));
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const uint4 v = p[q];
+            cnt += (v.x <= r) + (v.y <= r) + (v.z <= r) + (v.w <= r);
+            m = (v.x <= r && v.x > m) ? v.x : m;
+            m = (v.y <= r && v.y > m) ? v.y : m;
+            m = (v.z <= r && v.z > m) ? v.z : m;
+            m = (v.w <= r && v.w > m) ? v.w : m;
+        }
+    } else {
+        const ulonglong2 *p = (const ulonglong2 *)(reps + (sa << 5));
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const ulonglong2 v = p[q];
+            cnt += (v.x <= r) + (v.y <= r);
+            m = (v.x <= r && v.x > m) ? (S)v.x : m;
+            m = (v.y <= r && v.y > m) ? (S)v.y : m;
+        }
+    }
+    const int64_t idx = (sa << 5) + cnt - 1;
+    return (cnt > 0 && m == r && idx < N) ? idx : -1;
 }
 
 // Colex rank change when the one particle on bits {i, j} (i < j) hops to
@@ -117,33 +148,148 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-template <typename S, typename V, bool CC, int MODE, bool TRACK, int LPR>
-__global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restrict__ x, V *__restrict__ y) {
+// Column of state u for the implicit bases (no symmetry / spin inversion
+// only); row is the rank of s and u = s ^ (one flip group).  Returns -1 when
+// u leaves the particle sector.  cr receives the inversion character.
+template <typename S, int MODE, bool LDSB>
+__device__ __forceinline__ int64_t implicit_col(const MVArgs &A, S s, S u, int64_t row, int lo, int hi,
+                                                const uint32_t *s_binom, double &cr) {
+    cr = 1.0;
+    if (A.nup < 0) {
+        if (MODE == MODE_ZONLY && ((u >> (A.L - 1)) & 1)) {
+            cr = (double)A.zchar;
+            return (int64_t)(~u & (S)A.mask);
+        }
+        return (int64_t)u;
+    }
+    if (popc(u) != A.nup) return -1;
+    if (MODE == MODE_ZONLY && ((u >> (A.L - 1)) & 1)) {
+        cr = (double)A.zchar;
+        const S r = ~u & (S)A.mask;
+        return LDSB ? comb_rank<S, uint32_t>(r, s_binom) : comb_rank<S, uint64_t>(r, A.binom);
+    }
+    if (lo >= 0)
+        return row + (LDSB ? rank_delta<S, uint32_t>(s, lo, hi, s_binom) : rank_delta<S, uint64_t>(s, lo, hi, A.binom));
+    return LDSB ? comb_rank<S, uint32_t>(u, s_binom) : comb_rank<S, uint64_t>(u, A.binom);
+}
+
+// MODE_FULL / MODE_ZONLY: one lane per row and a wave-uniform loop over the
+// flip groups.  The group and term tables are read with uniform indices
+// (scalar loads), and for one bond consecutive rows reach nearby columns, so
+// the gathers of x coalesce.
+template <typename S, typename V, bool CC, int MODE>
+__global__ __launch_bounds__(kMvBlock) void k_matvec_implicit(MVArgs A, const GroupOfTerms *__restrict__ grp,
+                                                              const TermDev *__restrict__ off,
+                                                              const TermDev *__restrict__ diag,
+                                                              const V *__restrict__ x, V *__restrict__ y) {
+    using T = VTraits<V>;
+    extern __shared__ __align__(16) char smem[];
+    uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
+    double *s_red = (double *)(smem + A.off_red);
+    constexpr bool LDSB = sizeof(S) == 4;
+    if (LDSB && A.nup >= 0)
+        for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
+    __syncthreads();
+    const int vb = xcd_block(blockIdx.x, gridDim.x);
+    double dot_r = 0, dot_i = 0;
+    for (int64_t base = (int64_t)vb * kMvBlock; base < A.N; base += (int64_t)gridDim.x * kMvBlock) {
+        const int64_t row = base + threadIdx.x;
+        if (row >= A.N) continue;
+        S s;
+        if (A.nup >= 0) s = LDSB ? comb_unrank<S, uint32_t>(row, A.L, A.nup, s_binom)
+                                 : comb_unrank<S, uint64_t>(row, A.L, A.nup, A.binom);
+        else s = (S)row;
+        double xr, xi;
+        T::load(x, row, xr, xi);
+        if (A.conj_x) xi = -xi;
+        double dr = 0, di = 0;
+        for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(diag[k], s, dr, di);
+        di = -di;   // conj(amp): the terms are the adjoint couplings
+        if (A.user_diag) {
+            dr += A.user_diag[2 * row];
+            di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+        }
+        double ar = dr * xr - di * xi;
+        double ai = dr * xi + di * xr;
+        for (int gi = 0; gi < A.ngroups; gi++) {
+            const GroupOfTerms gr = grp[gi];
+            double mr = 0, mi = 0;
+            for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(off[k], s, mr, mi);
+            if (mr == 0.0 && mi == 0.0) continue;
+            mi = -mi;
+            double cr;
+            const int64_t col = implicit_col<S, MODE, LDSB>(A, s, s ^ (S)gr.flip, row, gr.lo, gr.hi, s_binom, cr);
+            if (col < 0) continue;
+            double vr, vi;
+            T::load(x, col, vr, vi);
+            if (A.conj_x) vi = -vi;
+            const double wr = mr * cr, wi = mi * cr;
+            ar += wr * vr - wi * vi;
+            ai += wr * vi + wi * vr;
+        }
+        double orr = A.are * ar - A.aim * ai;
+        double oi = A.are * ai + A.aim * ar;
+        if (!A.beta_zero) {
+            double yr, yi;
+            T::load(y, row, yr, yi);
+            if (A.conj_y) yi = -yi;
+            orr += A.bre * yr - A.bim * yi;
+            oi += A.bre * yi + A.bim * yr;
+        }
+        if (A.conj_y) oi = -oi;
+        if (A.dot_partial) {
+            dot_r += xr * orr + xi * oi;
+            dot_i += xr * oi - xi * orr;
+        }
+        T::store(y, row, orr, oi);
+    }
+    if (A.dot_partial) {
+        s_red[threadIdx.x] = dot_r;
+        s_red[kMvBlock + threadIdx.x] = dot_i;
+        __syncthreads();
+        for (int st = kMvBlock / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                s_red[threadIdx.x] += s_red[threadIdx.x + st];
+                s_red[kMvBlock + threadIdx.x] += s_red[kMvBlock + threadIdx.x + st];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            A.dot_partial[2 * blockIdx.x] = s_red[0];
+            A.dot_partial[2 * blockIdx.x + 1] = s_red[kMvBlock];
+        }
+    }
+}
+
+// MODE_STORED: LPR lanes per row, each walking only the live flip groups of
+// its share (the orbit canonicalisation dominates, so skipping dead bonds
+// matters more than gather coalescing); tables staged in LDS when they fit.
+template <typename S, typename V, bool CC, bool TRACK, int LPR, bool TLDS>
+__global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const GroupOfTerms *__restrict__ grp,
+                                                            const TermDev *__restrict__ off,
+                                                            const TermDev *__restrict__ diag,
+                                                            const V *__restrict__ x, V *__restrict__ y) {
     using T = VTraits<V>;
     extern __shared__ __align__(16) char smem[];
     S *s_samp = (S *)(smem + A.off_samp);
-    uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
     GroupOfTerms *s_grp = (GroupOfTerms *)(smem + A.off_grp);
     TermDev *s_term = (TermDev *)(smem + A.off_term);
     double *s_red = (double *)(smem + A.off_red);
-    constexpr bool kLdsBinom = (MODE != MODE_STORED) && sizeof(S) == 4;
-    if (MODE == MODE_STORED) {
+    {
         const S *samp = (const S *)A.samp;
-        for (int i = threadIdx.x; i < A.nsample; i += kMvBlock) s_samp[i] = samp[i];
+        for (int i = threadIdx.x; i < A.nsample; i += kStBlock) s_samp[i] = samp[i];
     }
-    if (kLdsBinom && A.nup >= 0)
-        for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
-    if (A.lds_groups)
-        for (int i = threadIdx.x; i < A.ngroups; i += kMvBlock) s_grp[i] = A.groups[i];
-    if (A.lds_terms)
-        for (int i = threadIdx.x; i < A.noff; i += kMvBlock) s_term[i] = A.off[i];
+    if (TLDS) {
+        for (int i = threadIdx.x; i < A.ngroups; i += kStBlock) s_grp[i] = grp[i];
+        for (int i = threadIdx.x; i < A.noff; i += kStBlock) s_term[i] = off[i];
+    }
     __syncthreads();
-    const GroupOfTerms *G = A.lds_groups ? s_grp : A.groups;
-    const TermDev *TT = A.lds_terms ? s_term : A.off;
+    const GroupOfTerms *G = TLDS ? s_grp : grp;
+    const TermDev *TT = TLDS ? s_term : off;
     const S mask = (S)A.mask;
     const S *reps = (const S *)A.reps;
     const int sub = threadIdx.x % LPR;
-    constexpr int kRows = kMvBlock / LPR;
+    constexpr int kRows = kStBlock / LPR;
     const int vb = xcd_block(blockIdx.x, gridDim.x);
     double dot_r = 0, dot_i = 0;
     for (int64_t base = (int64_t)vb * kRows; base < A.N; base += (int64_t)gridDim.x * kRows) {
@@ -151,23 +297,14 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restri
         const bool valid = row < A.N;
         double ar = 0, ai = 0, xr = 0, xi = 0;
         if (valid) {
-            S s;
-            if (MODE == MODE_STORED) {
-                s = reps[row];
-            } else if (A.nup >= 0) {
-                if (kLdsBinom) s = comb_unrank<S, uint32_t>(row, A.L, A.nup, s_binom);
-                else s = comb_unrank<S, uint64_t>(row, A.L, A.nup, A.binom);
-            } else {
-                s = (S)row;
-            }
-            double own_scale = 1.0;
-            if (MODE == MODE_STORED) own_scale = 1.0 / A.inv_nrm[row];
+            const S s = reps[row];
+            const double own_scale = 1.0 / A.inv_nrm[row];
             if (sub == 0) {
                 T::load(x, row, xr, xi);
                 if (A.conj_x) xi = -xi;
                 double dr = 0, di = 0;
-                for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(A.diag[k], s, dr, di);
-                di = -di;   // conj(amp): the terms are the adjoint couplings
+                for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(diag[k], s, dr, di);
+                di = -di;
                 if (A.user_diag) {
                     dr += A.user_diag[2 * row];
                     di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
@@ -176,7 +313,6 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restri
                 ai = dr * xi + di * xr;
             }
             for (int g0 = sub; g0 < A.ngroups; g0 += 64 * LPR) {
-                // live flip groups of this lane's share
                 uint64_t live = 0;
                 for (int jj = 0; jj < 64; jj++) {
                     const int gi = g0 + jj * LPR;
@@ -194,55 +330,22 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restri
                     for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr, mi);
                     mi = -mi;
                     const S u = s ^ (S)gr.flip;
-                    int64_t col;
+                    int bt, bp;
+                    S r;
+                    if constexpr (sizeof(S) == 4) r = canonical32<TRACK>(A.g, u, bt, bp);
+                    else r = canonical<S, TRACK>(A.g, u, mask, bt, bp);
+                    const int64_t col = search_rep<S>(reps, s_samp, A.nsample, A.shift, A.N, r);
+                    if (col < 0) continue;
                     double cr = 1.0, ci = 0.0;
-                    if (MODE == MODE_FULL) {
-                        if (A.nup >= 0) {
-                            if (popc(u) != A.nup) continue;
-                            if (gr.lo >= 0) {
-                                col = row + (kLdsBinom ? rank_delta<S, uint32_t>(s, gr.lo, gr.hi, s_binom)
-                                                       : rank_delta<S, uint64_t>(s, gr.lo, gr.hi, A.binom));
-                            } else {
-                                col = kLdsBinom ? comb_rank<S, uint32_t>(u, s_binom) : comb_rank<S, uint64_t>(u, A.binom);
-                            }
-                        } else {
-                            col = (int64_t)u;
-                        }
-                    } else if (MODE == MODE_ZONLY) {
-                        if (A.nup >= 0 && popc(u) != A.nup) continue;
-                        if ((u >> (A.L - 1)) & 1) {
-                            const S r = ~u & mask;
-                            cr = (double)A.zchar;
-                            if (A.nup >= 0)
-                                col = kLdsBinom ? comb_rank<S, uint32_t>(r, s_binom) : comb_rank<S, uint64_t>(r, A.binom);
-                            else
-                                col = (int64_t)r;
-                        } else if (A.nup >= 0) {
-                            if (gr.lo >= 0)
-                                col = row + (kLdsBinom ? rank_delta<S, uint32_t>(s, gr.lo, gr.hi, s_binom)
-                                                       : rank_delta<S, uint64_t>(s, gr.lo, gr.hi, A.binom));
-                            else
-                                col = kLdsBinom ? comb_rank<S, uint32_t>(u, s_binom) : comb_rank<S, uint64_t>(u, A.binom);
-                        } else {
-                            col = (int64_t)u;
-                        }
-                    } else {
-                        int bt, bp;
-                        S r;
-                        if constexpr (sizeof(S) == 4) r = canonical32<TRACK>(A.g, u, bt, bp);
-                        else r = canonical<S, TRACK>(A.g, u, mask, bt, bp);
-                        col = search_rep<S>(reps, s_samp, A.nsample, A.shift, A.N, r);
-                        if (col < 0) continue;
-                        if (TRACK) {
-                            const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
-                            const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
-                            cr = lr * pr - li * pi;
-                            ci = lr * pi + li * pr;
-                        }
-                        const double f = A.inv_nrm[col] * own_scale;
-                        cr *= f;
-                        ci *= f;
+                    if (TRACK) {
+                        const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
+                        const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
+                        cr = lr * pr - li * pi;
+                        ci = lr * pi + li * pr;
                     }
+                    const double f = A.inv_nrm[col] * own_scale;
+                    cr *= f;
+                    ci *= f;
                     const double wr = mr * cr - mi * ci;
                     const double wi = mr * ci + mi * cr;
                     double vr, vi;
@@ -280,18 +383,18 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec(MVArgs A, const V *__restri
     }
     if (A.dot_partial) {
         s_red[threadIdx.x] = dot_r;
-        s_red[kMvBlock + threadIdx.x] = dot_i;
+        s_red[kStBlock + threadIdx.x] = dot_i;
         __syncthreads();
-        for (int st = kMvBlock / 2; st > 0; st >>= 1) {
+        for (int st = kStBlock / 2; st > 0; st >>= 1) {
             if (threadIdx.x < st) {
                 s_red[threadIdx.x] += s_red[threadIdx.x + st];
-                s_red[kMvBlock + threadIdx.x] += s_red[kMvBlock + threadIdx.x + st];
+                s_red[kStBlock + threadIdx.x] += s_red[kStBlock + threadIdx.x + st];
             }
             __syncthreads();
         }
         if (threadIdx.x == 0) {
             A.dot_partial[2 * blockIdx.x] = s_red[0];
-            A.dot_partial[2 * blockIdx.x + 1] = s_red[kMvBlock];
+            A.dot_partial[2 * blockIdx.x + 1] = s_red[kStBlock];
         }
     }
 }
@@ -300,61 +403,81 @@ struct Launch {
     int grid;
     size_t lds;
     hipStream_t s;
+    const GroupOfTerms *grp;
+    const TermDev *off, *diag;
 };
 
-template <typename S, typename V, bool CC, int MODE, bool TRACK>
+template <typename S, typename V, bool CC, bool TRACK, bool TLDS>
 void dispatch_lpr(const MVArgs &a, int lpr, const void *x, void *y, const Launch &L) {
     const V *xv = (const V *)x;
     V *yv = (V *)y;
+#define QS_STORED(N)                                                                                            \
+    hipLaunchKernelGGL((k_matvec_stored<S, V, CC, TRACK, N, TLDS>), dim3(L.grid), dim3(kStBlock), L.lds, L.s, a, \
+                       L.grp, L.off, L.diag, xv, yv)
     switch (lpr) {
-    case 1: hipLaunchKernelGGL((k_matvec<S, V, CC, MODE, TRACK, 1>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, xv, yv); break;
-    case 2: hipLaunchKernelGGL((k_matvec<S, V, CC, MODE, TRACK, 2>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, xv, yv); break;
-    case 4: hipLaunchKernelGGL((k_matvec<S, V, CC, MODE, TRACK, 4>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, xv, yv); break;
-    default: hipLaunchKernelGGL((k_matvec<S, V, CC, MODE, TRACK, 8>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, xv, yv); break;
+    case 1: QS_STORED(1); break;
+    case 2: QS_STORED(2); break;
+    case 4: QS_STORED(4); break;
+    default: QS_STORED(8); break;
     }
+#undef QS_STORED
 }
 
 template <typename S, typename V, bool CC>
-void dispatch_mode(const MVArgs &a, int mode, bool track, int lpr, const void *x, void *y, const Launch &L) {
-    if (mode == MODE_FULL) dispatch_lpr<S, V, CC, MODE_FULL, false>(a, lpr, x, y, L);
-    else if (mode == MODE_ZONLY) dispatch_lpr<S, V, CC, MODE_ZONLY, false>(a, lpr, x, y, L);
-    else if (track) dispatch_lpr<S, V, CC, MODE_STORED, true>(a, lpr, x, y, L);
-    else dispatch_lpr<S, V, CC, MODE_STORED, false>(a, lpr, x, y, L);
+void dispatch_mode(const MVArgs &a, int mode, bool track, bool tlds, int lpr, const void *x, void *y,
+                   const Launch &L) {
+    const V *xv = (const V *)x;
+    V *yv = (V *)y;
+    if (mode == MODE_FULL) {
+        hipLaunchKernelGGL((k_matvec_implicit<S, V, CC, MODE_FULL>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a,
+                           L.grp, L.off, L.diag, xv, yv);
+    } else if (mode == MODE_ZONLY) {
+        hipLaunchKernelGGL((k_matvec_implicit<S, V, CC, MODE_ZONLY>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a,
+                           L.grp, L.off, L.diag, xv, yv);
+    } else if (track) {
+        if (tlds) dispatch_lpr<S, V, CC, true, true>(a, lpr, x, y, L);
+        else dispatch_lpr<S, V, CC, true, false>(a, lpr, x, y, L);
+    } else {
+        if (tlds) dispatch_lpr<S, V, CC, false, true>(a, lpr, x, y, L);
+        else dispatch_lpr<S, V, CC, false, false>(a, lpr, x, y, L);
+    }
 }
 
 template <typename S, typename V>
-void dispatch_cc(const MVArgs &a, bool cc, int mode, bool track, int lpr, const void *x, void *y, const Launch &L) {
+void dispatch_cc(const MVArgs &a, bool cc, int mode, bool track, bool tlds, int lpr, const void *x, void *y,
+                 const Launch &L) {
     if constexpr (VTraits<V>::cplx) {
-        if (cc) dispatch_mode<S, V, true>(a, mode, track, lpr, x, y, L);
-        else dispatch_mode<S, V, false>(a, mode, track, lpr, x, y, L);
+        if (cc) dispatch_mode<S, V, true>(a, mode, track, tlds, lpr, x, y, L);
+        else dispatch_mode<S, V, false>(a, mode, track, tlds, lpr, x, y, L);
     } else {
-        dispatch_mode<S, V, false>(a, mode, track, lpr, x, y, L);
+        dispatch_mode<S, V, false>(a, mode, track, tlds, lpr, x, y, L);
     }
 }
 
 template <typename S>
-void dispatch_v(const MVArgs &a, int dtype, bool cc, int mode, bool track, int lpr, const void *x, void *y,
-                const Launch &L) {
+void dispatch_v(const MVArgs &a, int dtype, bool cc, int mode, bool track, bool tlds, int lpr, const void *x,
+                void *y, const Launch &L) {
     switch (dtype) {
-    case QS_F64: dispatch_cc<S, double>(a, cc, mode, track, lpr, x, y, L); break;
-    case QS_C128: dispatch_cc<S, cplx_d>(a, cc, mode, track, lpr, x, y, L); break;
-    case QS_F32: dispatch_cc<S, float>(a, cc, mode, track, lpr, x, y, L); break;
-    case QS_C64: dispatch_cc<S, cplx_f>(a, cc, mode, track, lpr, x, y, L); break;
+    case QS_F64: dispatch_cc<S, double>(a, cc, mode, track, tlds, lpr, x, y, L); break;
+    case QS_C128: dispatch_cc<S, cplx_d>(a, cc, mode, track, tlds, lpr, x, y, L); break;
+    case QS_F32: dispatch_cc<S, float>(a, cc, mode, track, tlds, lpr, x, y, L); break;
+    case QS_C64: dispatch_cc<S, cplx_f>(a, cc, mode, track, tlds, lpr, x, y, L); break;
     default: fail(QS_EARG, "unknown dtype");
     }
 }
 
 }  // namespace
 
-int default_lanes_per_row(const BasisHost &b) { return b.mode == MODE_STORED ? 4 : 1; }
+int default_lanes_per_row(const BasisHost &b) { return b.mode == MODE_STORED ? 2 : 1; }
 
-int matvec_grid_lpr(int64_t n, int lpr) {
-    const int rows = kMvBlock / lpr;
+int matvec_grid_lpr(int64_t n, int lpr, int block = kMvBlock) {
+    const int rows = block / lpr;
     int64_t blocks = (n + rows - 1) / rows;
     return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)1 << 22));
 }
 
-int matvec_grid(int64_t n) { return matvec_grid_lpr(n, 8); }
+// blocks of a matvec launch that carries FP64 dot partials (one per block)
+int matvec_grid(int64_t n) { return matvec_grid_lpr(n, 8, kStBlock); }
 
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
                    double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
@@ -401,6 +524,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
     int lpr = op.lanes_per_row > 0 ? op.lanes_per_row : default_lanes_per_row(b);
+    if (b.mode != MODE_STORED) lpr = 1;
     if (dot_partial) lpr = 8;   // the partial-sum grid is sized by matvec_grid
     // dynamic LDS layout
     size_t off = 0;
@@ -412,14 +536,18 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     const size_t ssz = b.use64 ? 8 : 4;
     a.off_samp = place(b.mode == MODE_STORED ? ssz * (size_t)b.nsample : 0);
     a.off_binom = place((b.mode != MODE_STORED && !b.use64 && b.nup >= 0) ? sizeof(uint32_t) * 33 * 65 : 0);
-    a.lds_groups = a.ngroups <= kMaxLdsGroups;
-    a.lds_terms = a.noff <= kMaxLdsTerms;
-    a.off_grp = place(a.lds_groups ? sizeof(GroupOfTerms) * a.ngroups : 0);
-    a.off_term = place(a.lds_terms ? sizeof(TermDev) * a.noff : 0);
-    a.off_red = place(dot_partial ? sizeof(double) * 2 * kMvBlock : 0);
-    Launch L{matvec_grid_lpr(b.Ns, lpr), off, s};
-    if (b.use64) dispatch_v<uint64_t>(a, dtype, cc, b.mode, track, lpr, x, y, L);
-    else dispatch_v<uint32_t>(a, dtype, cc, b.mode, track, lpr, x, y, L);
+    const bool tlds = b.mode == MODE_STORED && a.ngroups <= kMaxLdsGroups && a.noff <= kMaxLdsTerms;
+    a.lds_groups = a.lds_terms = tlds ? 1 : 0;
+    a.off_grp = place(tlds ? sizeof(GroupOfTerms) * a.ngroups : 0);
+    a.off_term = place(tlds ? sizeof(TermDev) * a.noff : 0);
+    a.off_red = place(dot_partial ? sizeof(double) * 2 * std::max(kMvBlock, kStBlock) : 0);
+    // the dot partials are indexed by block: the implicit kernel uses one
+    // lane per row, so size its grid like the stored kernel's LPR = 8 grid
+    int grid = b.mode == MODE_STORED ? matvec_grid_lpr(b.Ns, lpr, kStBlock) : matvec_grid_lpr(b.Ns, 1);
+    if (dot_partial) grid = matvec_grid(b.Ns);
+    Launch L{grid, off, s, a.groups, a.off, a.diag};
+    if (b.use64) dispatch_v<uint64_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
+    else dispatch_v<uint32_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     QS_HIP(hipGetLastError());
 }
 

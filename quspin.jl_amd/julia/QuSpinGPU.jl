@@ -1,0 +1,190 @@
+"""
+    QuSpinGPU
+
+Julia binding of libqsgpu.so (include/qsgpu.h) for QuSpin.jl.  It adds a
+`GPUQuantumLinearOperator` that satisfies the same AbstractMatrix protocol as
+`QuantumLinearOperator` (operators.jl:3128-3245), so ARPACK, the Krylov
+exponential and BlockOps consume it unchanged, plus `eigsh` / `evolve` /
+`expm_multiply` methods that keep the Krylov basis resident on the GPU.
+
+Julia is not installed where this repository is built, so this file is
+exercised only through the same C ABI by the Python tests.
+"""
+module QuSpinGPU
+
+using LinearAlgebra
+import QuSpin: SpinBasis1D, OperatorTerm, eigsh, evolve
+
+const libqsgpu = get(ENV, "QSGPU_LIB", joinpath(@__DIR__, "..", "_build", "libqsgpu.so"))
+
+const QS_OK, QS_EARG, QS_EDIM, QS_ENOCONV, QS_EOOM, QS_EDEVICE = 0, 1, 2, 3, 4, 5
+const QS_UNSET = Cint(-1000000)
+const QS_F64, QS_C128, QS_F32, QS_C64 = Cint(0), Cint(1), Cint(2), Cint(3)
+const WHICH = Dict(:SA => 0, :LA => 1, :SM => 2, :LM => 3, :BE => 4)
+
+function last_error()
+    buf = Vector{UInt8}(undef, 1024)
+    ccall((:qs_last_error, libqsgpu), Csize_t, (Ptr{UInt8}, Csize_t), buf, length(buf))
+    return unsafe_string(pointer(buf))
+end
+
+# status code -> the exception the reference throws for the same condition
+function check(status::Integer)
+    status == QS_OK && return nothing
+    msg = last_error()
+    status == QS_EARG && throw(ArgumentError(msg))
+    status == QS_EDIM && throw(DimensionMismatch(msg))
+    status == QS_ENOCONV && throw(ErrorException(msg))
+    throw(ErrorException("libqsgpu status $status: $msg"))
+end
+
+_blk(v) = v === nothing ? QS_UNSET : Cint(v)
+_dtype(::Type{Float64}) = QS_F64
+_dtype(::Type{ComplexF64}) = QS_C128
+_dtype(::Type{Float32}) = QS_F32
+_dtype(::Type{ComplexF32}) = QS_C64
+
+"""GPU-resident sector of a SpinBasis1D (same keywords, basis.jl:622-639)."""
+mutable struct GPUBasis
+    handle::Ptr{Cvoid}
+    L::Int
+    Ns::Int
+end
+
+function GPUBasis(L::Integer; nup=nothing, pauli::Bool=true, a::Integer=1, kblock=nothing,
+                  pblock=nothing, zblock=nothing, pzblock=nothing, zAblock=nothing, zBblock=nothing)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:qs_basis_create, libqsgpu), Cint,
+                (Cint, Cint, Cint, Cint, Cint, Cint, Cint, Cint, Cint, Cint, Cint, Ref{Ptr{Cvoid}}),
+                L, _blk(nup), pauli, a, _blk(kblock), _blk(pblock), _blk(zblock), _blk(pzblock),
+                _blk(zAblock), _blk(zBblock), 0, h))
+    n = Ref{Int64}(0)
+    check(ccall((:qs_basis_size, libqsgpu), Cint, (Ptr{Cvoid}, Ref{Int64}), h[], n))
+    b = GPUBasis(h[], Int(L), Int(n[]))
+    finalizer(x -> ccall((:qs_basis_destroy, libqsgpu), Cint, (Ptr{Cvoid},), x.handle), b)
+    return b
+end
+
+function states(b::GPUBasis)
+    out = Vector{UInt64}(undef, b.Ns)
+    check(ccall((:qs_basis_states, libqsgpu), Cint, (Ptr{Cvoid}, Ptr{UInt64}), b.handle, out))
+    return out
+end
+
+"""
+    GPUQuantumLinearOperator(basis, static_list; diagonal, check_symm, check_herm, check_pcon)
+
+Drop-in for QuantumLinearOperator (operators.jl:2492-2552) on the GPU.
+"""
+mutable struct GPUQuantumLinearOperator{T<:Number} <: AbstractMatrix{T}
+    handle::Ptr{Cvoid}
+    basis::GPUBasis
+    hermitian::Bool
+    transposed::Bool
+    conjugated::Bool
+end
+
+function GPUQuantumLinearOperator(basis::GPUBasis, terms::AbstractVector{<:OperatorTerm};
+                                  diagonal=nothing, check_symm::Bool=true, check_herm::Bool=true,
+                                  check_pcon::Bool=true)
+    ops = [t.op for t in terms]
+    ncoup = Int32[length(t.couplings) for t in terms]
+    coef = Float64[]
+    sites = Int32[]
+    iscomplex = false
+    for t in terms, c in t.couplings
+        z = ComplexF64(first(c))
+        push!(coef, real(z), imag(z))
+        iscomplex |= first(c) isa Complex
+        append!(sites, Int32.(Base.tail(c)))
+    end
+    d = diagonal === nothing ? C_NULL : reinterpret(Float64, ComplexF64.(diagonal))
+    flags = (check_symm ? 1 : 0) | (check_herm ? 2 : 0) | (check_pcon ? 4 : 0) | (iscomplex ? 8 : 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:qs_op_create, libqsgpu), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cstring}, Ptr{Int32}, Ptr{Float64}, Ptr{Int32}, Ptr{Float64}, Cint,
+                 Ref{Ptr{Cvoid}}),
+                basis.handle, length(terms), ops, ncoup, coef, sites, d, flags, h))
+    c, herm, n = Ref{Cint}(0), Ref{Cint}(0), Ref{Int64}(0)
+    check(ccall((:qs_op_info, libqsgpu), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}, Ref{Int64}),
+                h[], c, herm, n))
+    T = c[] != 0 ? ComplexF64 : Float64
+    op = GPUQuantumLinearOperator{T}(h[], basis, herm[] != 0, false, false)
+    finalizer(x -> ccall((:qs_op_destroy, libqsgpu), Cint, (Ptr{Cvoid},), x.handle), op)
+    return op
+end
+
+# AbstractMatrix protocol (operators.jl:3228-3245)
+Base.size(op::GPUQuantumLinearOperator) = (op.basis.Ns, op.basis.Ns)
+Base.getindex(op::GPUQuantumLinearOperator, i::Int, j::Int) =
+    error("GPUQuantumLinearOperator is matrix-free; use mul!")
+LinearAlgebra.ishermitian(op::GPUQuantumLinearOperator) = op.hermitian
+LinearAlgebra.issymmetric(op::GPUQuantumLinearOperator{T}) where {T} = T <: Real && op.hermitian
+Base.transpose(op::GPUQuantumLinearOperator{T}) where {T} =
+    GPUQuantumLinearOperator{T}(op.handle, op.basis, op.hermitian, !op.transposed, op.conjugated)
+Base.conj(op::GPUQuantumLinearOperator{T}) where {T} =
+    GPUQuantumLinearOperator{T}(op.handle, op.basis, op.hermitian, op.transposed, !op.conjugated)
+Base.adjoint(op::GPUQuantumLinearOperator{T}) where {T} =
+    GPUQuantumLinearOperator{T}(op.handle, op.basis, op.hermitian, !op.transposed, !op.conjugated)
+
+# mul!(y, op, x, alpha, beta) -- operators.jl:3147-3208 (host arrays are staged)
+function LinearAlgebra.mul!(y::StridedVecOrMat{S}, op::GPUQuantumLinearOperator, x::StridedVecOrMat{S},
+                            alpha::Number, beta::Number) where {S<:Union{Float64,ComplexF64,Float32,ComplexF32}}
+    size(x, 1) == size(op, 2) || throw(DimensionMismatch("operator and value dimensions do not match"))
+    size(y) == size(x) || throw(DimensionMismatch("output and operator dimensions do not match"))
+    a, b = ComplexF64(alpha), ComplexF64(beta)
+    check(ccall((:qs_op_matvec, libqsgpu), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{S}, Ptr{S}, Int64, Cdouble, Cdouble, Cdouble, Cdouble, Cint, Cint, Cint),
+                op.handle, _dtype(S), x, y, size(x, 2), real(a), imag(a), real(b), imag(b),
+                op.transposed, op.conjugated, 0))
+    return y
+end
+LinearAlgebra.mul!(y::StridedVecOrMat, op::GPUQuantumLinearOperator, x::StridedVecOrMat) =
+    mul!(y, op, x, true, false)
+Base.:*(op::GPUQuantumLinearOperator{T}, x::AbstractVector{S}) where {T,S} =
+    mul!(zeros(promote_type(T, S), size(op, 1)), op, Vector{promote_type(T, S)}(x))
+
+# eigsh with the Krylov basis resident on the device (operators.jl:3367-3408)
+function eigsh(op::GPUQuantumLinearOperator{T}; k::Integer=min(6, size(op, 1)), which=:SA,
+               tol::Real=0.0, maxiter::Integer=300, ncv=nothing, v0=nothing,
+               return_eigenvectors::Bool=true, dtype::Type=T) where {T}
+    n = size(op, 1)
+    1 <= k <= n || throw(ArgumentError("k must lie in 1:Ns"))
+    w = Symbol(uppercase(String(which)))
+    haskey(WHICH, w) || throw(ArgumentError("which must be SA, LA, SM, LM, or BE"))
+    vals = zeros(Float64, k)
+    vecs = return_eigenvectors ? zeros(dtype, n, k) : nothing
+    v = v0 === nothing ? C_NULL : Vector{dtype}(v0)
+    nconv, nmv = Ref{Cint}(0), Ref{Cint}(0)
+    check(ccall((:qs_eigsh, libqsgpu), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Cint, Cint, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid},
+                 Ref{Cint}, Ref{Cint}),
+                op.handle, _dtype(dtype), k, WHICH[w], tol, maxiter, ncv === nothing ? 0 : ncv, v, vals,
+                vecs === nothing ? C_NULL : vecs, nconv, nmv))
+    return return_eigenvectors ? (vals, vecs) : vals
+end
+
+# exp(scale * op) v for every scale: _krylov_expmv_times (operators.jl:1805-1893)
+function expm_multiply(op::GPUQuantumLinearOperator, v::AbstractVector, scales::AbstractVector;
+                       tol::Real=1e-12, krylov_dim::Integer=30, dtype::Type=ComplexF64)
+    x = Vector{dtype}(v)
+    sc = ComplexF64.(scales)
+    out = zeros(dtype, length(x), length(sc))
+    nmv = Ref{Cint}(0)
+    check(ccall((:qs_expm_multiply, libqsgpu), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{dtype}, Int64, Ptr{ComplexF64}, Cdouble, Cint, Ptr{dtype}, Ref{Cint}),
+                op.handle, _dtype(dtype), x, length(sc), sc, tol, krylov_dim, out, nmv))
+    return out
+end
+
+# evolve for a static generator (operators.jl:2073-2192, :SE branch)
+function evolve(op::GPUQuantumLinearOperator, v0::AbstractVector, t0::Real, times;
+                imag_time::Bool=false, tol::Real=1e-12, krylov_dim::Integer=30)
+    targets = times isa Number ? [times] : collect(times)
+    scales = imag_time ? ComplexF64[-(t - t0) for t in targets] : ComplexF64[-im * (t - t0) for t in targets]
+    states = expm_multiply(op, v0, scales; tol, krylov_dim)
+    imag_time && foreach(s -> rmul!(s, inv(norm(s))), eachcol(states))
+    return times isa Number ? states[:, 1] : states
+end
+
+end # module
