@@ -133,10 +133,32 @@ __global__ __launch_bounds__(kBlock) void k_write(GroupDev g, int nup, int64_t n
     }
 }
 
+// Insert every canonical state into the open-addressing column table.  The
+// slot a state lands in depends on insertion order, but a lookup always
+// returns the one column holding that state, so results are deterministic.
 template <typename S>
-__global__ void k_samples(const S *reps, int64_t n, int shift, int64_t nsample, S *samp) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nsample) samp[i] = reps[i << shift];
+__global__ void k_hash_insert(const S *reps, int64_t n, uint64_t *keys, uint32_t *vals, uint64_t hmask,
+                              int *max_probe) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const S r = reps[i];
+    uint64_t h = state_hash((uint64_t)r) & hmask;
+    int probe = 1;
+    const unsigned long long empty = ~0ULL;
+    if constexpr (sizeof(S) == 4) {
+        const unsigned long long ent = ((unsigned long long)r << 32) | (uint32_t)i;
+        while (atomicCAS((unsigned long long *)&keys[h], empty, ent) != empty) {
+            h = (h + 1) & hmask;
+            probe++;
+        }
+    } else {
+        while (atomicCAS((unsigned long long *)&keys[h], empty, (unsigned long long)r) != empty) {
+            h = (h + 1) & hmask;
+            probe++;
+        }
+        vals[h] = (uint32_t)i;
+    }
+    atomicMax(max_probe, probe);
 }
 
 template <typename S>
@@ -172,25 +194,22 @@ void build_stored(BasisHost &b, int64_t npar) {
     hipLaunchKernelGGL((k_write<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
                        b.binom.get(), d_off.get(), reps.get(), b.orbit.get(), b.inv_nrm.get());
     QS_HIP(hipGetLastError());
-    // sampled search table kept in LDS by the matvec: the first state of
-    // every 2^shift-entry segment, at most 48 KiB of samples
-    const uint64_t max_samples = 49152 / sizeof(S);
-    int shift = 5;
-    while (((total + (1ULL << shift) - 1) >> shift) > max_samples) shift++;
-    b.sample_shift = shift;
-    b.nsample = (int64_t)((total + (1ULL << shift) - 1) >> shift);
-    DevBuf<S> samp((size_t)b.nsample);
-    hipLaunchKernelGGL((k_samples<S>), dim3((unsigned)((b.nsample + 255) / 256)), dim3(256), 0, 0,
-                       reps.get(), (int64_t)total, shift, b.nsample, samp.get());
+    // column hash table: power of two with load factor <= 1/2
+    uint64_t hsize = 1024;
+    while (hsize < 2 * total) hsize <<= 1;
+    b.hmask = hsize - 1;
+    b.hkeys.alloc((size_t)hsize);
+    QS_HIP(hipMemset(b.hkeys.get(), 0xFF, hsize * sizeof(uint64_t)));
+    if (sizeof(S) == 8) b.hvals.alloc((size_t)hsize);
+    DevBuf<int> maxp(1);
+    QS_HIP(hipMemset(maxp.get(), 0, sizeof(int)));
+    hipLaunchKernelGGL((k_hash_insert<S>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0, reps.get(),
+                       (int64_t)total, b.hkeys.get(), b.hvals.get(), b.hmask, maxp.get());
     QS_HIP(hipGetLastError());
+    QS_HIP(hipMemcpy(&b.hmax_probe, maxp.get(), sizeof(int), hipMemcpyDeviceToHost));
     QS_HIP(hipDeviceSynchronize());
-    if constexpr (sizeof(S) == 4) {
-        b.reps32 = std::move(reps);
-        b.samp32 = std::move(samp);
-    } else {
-        b.reps64 = std::move(reps);
-        b.samp64 = std::move(samp);
-    }
+    if constexpr (sizeof(S) == 4) b.reps32 = std::move(reps);
+    else b.reps64 = std::move(reps);
 }
 
 }  // namespace

@@ -142,6 +142,75 @@ __device__ __forceinline__ uint32_t canonical32(const GroupDev &g, uint32_t u, i
     return best >> (32 - L);
 }
 
+// canonical32 for NB states at once, point operations outermost, so each
+// point operation's parameters are read once per batch and the NB
+// independent min-chains interleave.  act[j] false leaves best[j] untouched.
+template <int NB, bool TRACK>
+__device__ __forceinline__ void canonical32_batch(const GroupDev &g, const uint32_t (&u)[NB], uint32_t (&best)[NB],
+                                                  int (&bt)[NB], int (&bp)[NB]) {
+    const int L = g.L, a = g.a, nT = g.nT;
+    const int s0 = g.win[0];
+    int bs[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        best[j] = 0xFFFFFFFFu;
+        bs[j] = 0;
+        bp[j] = 0;
+    }
+    for (int p = 0; p < g.npoint; p++) {
+        const int rev = g.prev[p];
+        const uint32_t flip = (uint32_t)g.pflip[p];
+        uint32_t lo[NB], hi[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const uint32_t v = (rev ? rev_bits(u[j], L) : u[j]) ^ flip;
+            uint64_t P = v;
+            for (int k = L; k < 64; k += L) P |= (uint64_t)v << k;
+            lo[j] = (uint32_t)P;
+            hi[j] = (uint32_t)(P >> 32);
+        }
+        if (a == 1 && nT == L) {
+#pragma unroll 4
+            for (int s = 0; s < L; s++) {
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const uint32_t w = __builtin_amdgcn_alignbit(hi[j], lo[j], (uint32_t)s);
+                    if (TRACK) {
+                        if (w < best[j]) { best[j] = w; bs[j] = s; bp[j] = p; }
+                    } else {
+                        best[j] = w < best[j] ? w : best[j];
+                    }
+                }
+            }
+        } else {
+            int s = s0;
+            for (int t = 0; t < nT; t++) {
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const uint32_t w = __builtin_amdgcn_alignbit(hi[j], lo[j], (uint32_t)s);
+                    if (TRACK) {
+                        if (w < best[j]) { best[j] = w; bs[j] = s; bp[j] = p; }
+                    } else {
+                        best[j] = w < best[j] ? w : best[j];
+                    }
+                }
+                s -= a;
+                if (s < 0) s += L;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        int d = s0 - bs[j];
+        if (d < 0) d += L;
+        bt[j] = d / a;
+        best[j] >>= (32 - L);
+    }
+}
+
+// Hash of a canonical state for the column table (Fibonacci hashing).
+__device__ __forceinline__ uint64_t state_hash(uint64_t r) { return (r * 0x9E3779B97F4A7C15ULL) >> 20; }
+
 // Combinadic (colex) rank of a word with w set bits: sum_k C(p_k, k).
 // binom is laid out as C(n, k) at [n * 65 + k].
 template <typename S, typename B>

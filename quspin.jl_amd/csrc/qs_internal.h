@@ -133,6 +133,13 @@ struct BasisHost {
     DevBuf<uint64_t> reps64;
     DevBuf<double> inv_nrm;      // 1/sqrt(orbit size)
     DevBuf<uint32_t> orbit;      // orbit size per column
+    // open-addressing hash table canonical state -> column (linear probing,
+    // load <= 1/2): for L <= 32 one u64 per slot, (state << 32) | column;
+    // for L > 32 a key array and a column array.  Empty slots are all ones.
+    DevBuf<uint64_t> hkeys;
+    DevBuf<uint32_t> hvals;
+    uint64_t hmask = 0;          // table size - 1
+    int hmax_probe = 1;          // longest probe sequence of any stored state
     // search acceleration: every 2^sample_shift-th representative
     int sample_shift = 0;
     int64_t nsample = 0;

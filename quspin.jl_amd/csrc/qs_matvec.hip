@@ -10,19 +10,25 @@
 // reference's columns are these up to rounding) and H replaced by its group
 // average (P'HP = P'H_avg P), one output row c' with canonical state s is
 //   y[c'] = sum_u <s|H|u> chi(h_u) sqrt(D_c'/D_c) x[c(u)],   h_u u = s_c(u),
-// where u runs over the states the adjoint couplings reach from s.
+// where u runs over the states the adjoint couplings reach from s.  Output
+// rows are owned by one lane (or LPR lanes combined by a fixed shuffle
+// tree): no atomics and a fixed summation order, so a result never depends
+// on what else runs on the device.
 //
-// Work decomposition: LPR lanes share one row (LPR in {1, 2, 4, 8}, picked
-// per operator); each lane evaluates the amplitude of its share of the
-// flip groups, keeps the live ones in a bit mask and walks only those, so
-// lanes do not idle on dead bonds.  The LPR partial sums are combined with a
-// fixed xor-shuffle tree and one lane writes y: no atomics and a fixed
-// summation order, so a result never depends on what else runs on the device.
-// Column lookup: combinadic rank (no symmetry / spin inversion only),
-// updated incrementally for two-site hops; otherwise the orbit minimum (32-bit
-// funnel-shift windows) located in the sorted canonical states through a
-// binary search whose top levels sit in LDS.  Blocks are renumbered so each
-// XCD owns a contiguous row range and keeps its own L2 warm.
+// Two kernels:
+//  * k_matvec_implicit -- no symmetry or spin inversion only.  The column is
+//    the combinadic rank of the state (updated incrementally for two-site
+//    hops), so there is no table at all.  One lane per row, a wave-uniform
+//    loop over the flip groups (couplings read with scalar loads; for one
+//    bond consecutive rows reach nearby columns, so gathers coalesce).
+//  * k_matvec_stored -- translations / parity / inversions.  Each lane
+//    collects the live flip groups of its share in a bit mask, then works on
+//    them NB at a time: orbit minima from 32-bit funnel shifts, one probe of
+//    the HBM hash table canonical state -> column, then the gathers of x and
+//    1/sqrt(D).
+// Both issue all loads of an NB-neighbour batch before consuming any, so
+// memory latency is paid once per batch, and both renumber blocks so each
+// XCD owns a contiguous row range (its own L2 stays warm).
 #include <algorithm>
 
 #include "qs_device.h"
@@ -32,9 +38,10 @@ namespace qs {
 namespace {
 
 constexpr int kMvBlock = 256;   // implicit-basis kernel
-constexpr int kStBlock = 512;   // stored-basis kernel: larger blocks share one LDS search table
+constexpr int kStBlock = 256;   // stored-basis kernel
 constexpr int kMaxLdsGroups = 256;
 constexpr int kMaxLdsTerms = 512;
+constexpr int kNB = 8;          // neighbours per batch
 
 struct MVArgs {
     int64_t N;
@@ -43,15 +50,13 @@ struct MVArgs {
     GroupDev g;
     const void *reps;
     const double *inv_nrm;
-    const void *samp;
-    int nsample;
-    int shift;
+    const uint64_t *hkeys;
+    const uint32_t *hvals;
+    uint64_t hmask;
+    int hmax_probe;
     const uint64_t *binom;
-    const TermDev *diag;
     int ndiag;
-    const TermDev *off;
     int noff;
-    const GroupOfTerms *groups;
     int ngroups;
     const double *user_diag;   // interleaved complex or null
     int user_diag_conj;
@@ -60,8 +65,7 @@ struct MVArgs {
     int beta_zero;
     double *dot_partial;   // optional: per-block FP64 partials of <x, y>
     // dynamic LDS layout (byte offsets)
-    int off_samp, off_binom, off_grp, off_term, off_red;
-    int lds_groups, lds_terms;
+    int off_binom, off_grp, off_term, off_red;
 };
 
 // Amplitude of one compiled term on state s: A * (-1)^popcount(s & smask),
@@ -74,51 +78,21 @@ __device__ __forceinline__ void term_amp(const TermDev &t, S s, double &re, doub
     if (CC) im += sg * t.aim;
 }
 
+// Column of canonical state r in the hash table, or -1.  `e` is the first
+// probe, already loaded by the caller.
 template <typename S>
-__device__ __forceinline__ int64_t search_rep(const S *reps, const S *lsamp, int nsample, int shift, int64_t N, S r) {
-    if (r < lsamp[0]) return -1;
-    int lo = 0, hi = nsample;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (lsamp[mid] <= r) lo = mid; else hi = mid;
-    }
-    // narrow the sample's range to one 32-entry segment (steps only when
-    // the range holds more than one segment) ...
-    const int64_t nseg = (N + 31) >> 5;
-    int64_t sa = ((int64_t)lo << shift) >> 5, sb = (((int64_t)lo + 1) << shift) >> 5;
-    if (sb > nseg) sb = nseg;
-    while (sb - sa > 1) {
-        int64_t mid = (sa + sb) >> 1;
-        if (reps[mid << 5] <= r) sa = mid; else sb = mid;
-    }
-    // ... and scan that segment with full-width loads (one round trip): the
-    // largest entry <= r sits at (count - 1).  The array is padded with the
-    // largest word to whole segments.
-    int cnt = 0;
-    S m = 0;
-    if constexpr (sizeof(S) == 4) {
-        const uint4 *p = (const uint4 *)(reps + (sa << 5));
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const uint4 v = p[q];
-            cnt += (v.x <= r) + (v.y <= r) + (v.z <= r) + (v.w <= r);
-            m = (v.x <= r && v.x > m) ? v.x : m;
-            m = (v.y <= r && v.y > m) ? v.y : m;
-            m = (v.z <= r && v.z > m) ? v.z : m;
-            m = (v.w <= r && v.w > m) ? v.w : m;
+__device__ __forceinline__ int64_t hash_resolve(const MVArgs &A, S r, uint64_t h, uint64_t e) {
+    for (int p = 0;; p++) {
+        if (e == ~0ULL) return -1;
+        if constexpr (sizeof(S) == 4) {
+            if ((uint32_t)(e >> 32) == r) return (int64_t)(uint32_t)e;
+        } else {
+            if (e == (uint64_t)r) return (int64_t)A.hvals[h];
         }
-    } else {
-        const ulonglong2 *p = (const ulonglong2 *)(reps + (sa << 5));
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const ulonglong2 v = p[q];
-            cnt += (v.x <= r) + (v.y <= r);
-            m = (v.x <= r && v.x > m) ? (S)v.x : m;
-            m = (v.y <= r && v.y > m) ? (S)v.y : m;
-        }
+        if (p + 1 >= A.hmax_probe) return -1;
+        h = (h + 1) & A.hmask;
+        e = A.hkeys[h];
     }
-    const int64_t idx = (sa << 5) + cnt - 1;
-    return (cnt > 0 && m == r && idx < N) ? idx : -1;
 }
 
 // Colex rank change when the one particle on bits {i, j} (i < j) hops to
@@ -148,9 +122,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-// Column of state u for the implicit bases (no symmetry / spin inversion
-// only); row is the rank of s and u = s ^ (one flip group).  Returns -1 when
-// u leaves the particle sector.  cr receives the inversion character.
+// Column of state u for the implicit bases; row is the rank of s and
+// u = s ^ (one flip group).  -1 when u leaves the particle sector; cr
+// receives the inversion character.
 template <typename S, int MODE, bool LDSB>
 __device__ __forceinline__ int64_t implicit_col(const MVArgs &A, S s, S u, int64_t row, int lo, int hi,
                                                 const uint32_t *s_binom, double &cr) {
@@ -173,10 +147,47 @@ __device__ __forceinline__ int64_t implicit_col(const MVArgs &A, S s, S u, int64
     return LDSB ? comb_rank<S, uint32_t>(u, s_binom) : comb_rank<S, uint64_t>(u, A.binom);
 }
 
-// MODE_FULL / MODE_ZONLY: one lane per row and a wave-uniform loop over the
-// flip groups.  The group and term tables are read with uniform indices
-// (scalar loads), and for one bond consecutive rows reach nearby columns, so
-// the gathers of x coalesce.
+template <int BLOCK>
+__device__ __forceinline__ void write_dot_partial(const MVArgs &A, double *s_red, double dot_r, double dot_i) {
+    s_red[threadIdx.x] = dot_r;
+    s_red[BLOCK + threadIdx.x] = dot_i;
+    __syncthreads();
+    for (int st = BLOCK / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            s_red[threadIdx.x] += s_red[threadIdx.x + st];
+            s_red[BLOCK + threadIdx.x] += s_red[BLOCK + threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        A.dot_partial[2 * blockIdx.x] = s_red[0];
+        A.dot_partial[2 * blockIdx.x + 1] = s_red[BLOCK];
+    }
+}
+
+// y[row] = beta*y + alpha*acc with optional conjugations; FP64 <x, y> partial
+template <typename V>
+__device__ __forceinline__ void finish_row(const MVArgs &A, V *y, int64_t row, double ar, double ai, double xr,
+                                           double xi, double &dot_r, double &dot_i) {
+    using T = VTraits<V>;
+    double orr = A.are * ar - A.aim * ai;
+    double oi = A.are * ai + A.aim * ar;
+    if (!A.beta_zero) {
+        double yr, yi;
+        T::load(y, row, yr, yi);
+        if (A.conj_y) yi = -yi;
+        orr += A.bre * yr - A.bim * yi;
+        oi += A.bre * yi + A.bim * yr;
+    }
+    if (A.conj_y) oi = -oi;
+    if (A.dot_partial) {
+        // FP64 <x, y_row> of the unrounded result (Rayleigh quotients of FP32-stored vectors)
+        dot_r += xr * orr + xi * oi;
+        dot_i += xr * oi - xi * orr;
+    }
+    T::store(y, row, orr, oi);
+}
+
 template <typename S, typename V, bool CC, int MODE>
 __global__ __launch_bounds__(kMvBlock) void k_matvec_implicit(MVArgs A, const GroupOfTerms *__restrict__ grp,
                                                               const TermDev *__restrict__ off,
@@ -211,59 +222,46 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec_implicit(MVArgs A, const Gr
         }
         double ar = dr * xr - di * xi;
         double ai = dr * xi + di * xr;
-        for (int gi = 0; gi < A.ngroups; gi++) {
-            const GroupOfTerms gr = grp[gi];
-            double mr = 0, mi = 0;
-            for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(off[k], s, mr, mi);
-            if (mr == 0.0 && mi == 0.0) continue;
-            mi = -mi;
-            double cr;
-            const int64_t col = implicit_col<S, MODE, LDSB>(A, s, s ^ (S)gr.flip, row, gr.lo, gr.hi, s_binom, cr);
-            if (col < 0) continue;
-            double vr, vi;
-            T::load(x, col, vr, vi);
-            if (A.conj_x) vi = -vi;
-            const double wr = mr * cr, wi = mi * cr;
-            ar += wr * vr - wi * vi;
-            ai += wr * vi + wi * vr;
-        }
-        double orr = A.are * ar - A.aim * ai;
-        double oi = A.are * ai + A.aim * ar;
-        if (!A.beta_zero) {
-            double yr, yi;
-            T::load(y, row, yr, yi);
-            if (A.conj_y) yi = -yi;
-            orr += A.bre * yr - A.bim * yi;
-            oi += A.bre * yi + A.bim * yr;
-        }
-        if (A.conj_y) oi = -oi;
-        if (A.dot_partial) {
-            dot_r += xr * orr + xi * oi;
-            dot_i += xr * oi - xi * orr;
-        }
-        T::store(y, row, orr, oi);
-    }
-    if (A.dot_partial) {
-        s_red[threadIdx.x] = dot_r;
-        s_red[kMvBlock + threadIdx.x] = dot_i;
-        __syncthreads();
-        for (int st = kMvBlock / 2; st > 0; st >>= 1) {
-            if (threadIdx.x < st) {
-                s_red[threadIdx.x] += s_red[threadIdx.x + st];
-                s_red[kMvBlock + threadIdx.x] += s_red[kMvBlock + threadIdx.x + st];
+        for (int g0 = 0; g0 < A.ngroups; g0 += kNB) {
+            int64_t col[kNB];
+            double wr[kNB], wi[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; j++) {
+                const int gi = g0 + j;
+                col[j] = row;
+                wr[j] = 0;
+                wi[j] = 0;
+                if (gi < A.ngroups) {
+                    const GroupOfTerms gr = grp[gi];
+                    double mr = 0, mi = 0;
+                    for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(off[k], s, mr, mi);
+                    if (mr != 0.0 || mi != 0.0) {
+                        double cr;
+                        const int64_t c = implicit_col<S, MODE, LDSB>(A, s, s ^ (S)gr.flip, row, gr.lo, gr.hi,
+                                                                      s_binom, cr);
+                        if (c >= 0) {
+                            col[j] = c;
+                            wr[j] = mr * cr;
+                            wi[j] = -mi * cr;
+                        }
+                    }
+                }
             }
-            __syncthreads();
+            double vr[kNB], vi[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; j++) T::load(x, col[j], vr[j], vi[j]);
+#pragma unroll
+            for (int j = 0; j < kNB; j++) {
+                if (A.conj_x) vi[j] = -vi[j];
+                ar += wr[j] * vr[j] - wi[j] * vi[j];
+                ai += wr[j] * vi[j] + wi[j] * vr[j];
+            }
         }
-        if (threadIdx.x == 0) {
-            A.dot_partial[2 * blockIdx.x] = s_red[0];
-            A.dot_partial[2 * blockIdx.x + 1] = s_red[kMvBlock];
-        }
+        finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
     }
+    if (A.dot_partial) write_dot_partial<kMvBlock>(A, s_red, dot_r, dot_i);
 }
 
-// MODE_STORED: LPR lanes per row, each walking only the live flip groups of
-// its share (the orbit canonicalisation dominates, so skipping dead bonds
-// matters more than gather coalescing); tables staged in LDS when they fit.
 template <typename S, typename V, bool CC, bool TRACK, int LPR, bool TLDS>
 __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const GroupOfTerms *__restrict__ grp,
                                                             const TermDev *__restrict__ off,
@@ -271,14 +269,9 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
                                                             const V *__restrict__ x, V *__restrict__ y) {
     using T = VTraits<V>;
     extern __shared__ __align__(16) char smem[];
-    S *s_samp = (S *)(smem + A.off_samp);
     GroupOfTerms *s_grp = (GroupOfTerms *)(smem + A.off_grp);
     TermDev *s_term = (TermDev *)(smem + A.off_term);
     double *s_red = (double *)(smem + A.off_red);
-    {
-        const S *samp = (const S *)A.samp;
-        for (int i = threadIdx.x; i < A.nsample; i += kStBlock) s_samp[i] = samp[i];
-    }
     if (TLDS) {
         for (int i = threadIdx.x; i < A.ngroups; i += kStBlock) s_grp[i] = grp[i];
         for (int i = threadIdx.x; i < A.noff; i += kStBlock) s_term[i] = off[i];
@@ -323,36 +316,73 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
                     if (mr != 0.0 || mi != 0.0) live |= 1ULL << jj;
                 }
                 while (live) {
-                    const int jj = __builtin_ctzll(live);
-                    live &= live - 1;
-                    const GroupOfTerms gr = G[g0 + jj * LPR];
-                    double mr = 0, mi = 0;
-                    for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr, mi);
-                    mi = -mi;
-                    const S u = s ^ (S)gr.flip;
-                    int bt, bp;
-                    S r;
-                    if constexpr (sizeof(S) == 4) r = canonical32<TRACK>(A.g, u, bt, bp);
-                    else r = canonical<S, TRACK>(A.g, u, mask, bt, bp);
-                    const int64_t col = search_rep<S>(reps, s_samp, A.nsample, A.shift, A.N, r);
-                    if (col < 0) continue;
-                    double cr = 1.0, ci = 0.0;
-                    if (TRACK) {
-                        const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
-                        const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
-                        cr = lr * pr - li * pi;
-                        ci = lr * pi + li * pr;
+                    // a batch of up to kNB live neighbours
+                    S u[kNB];
+                    double mr[kNB], mi[kNB];
+                    bool act[kNB];
+#pragma unroll
+                    for (int j = 0; j < kNB; j++) {
+                        act[j] = live != 0;
+                        mr[j] = 0;
+                        mi[j] = 0;
+                        u[j] = s;
+                        if (act[j]) {
+                            const int jj = __builtin_ctzll(live);
+                            live &= live - 1;
+                            const GroupOfTerms gr = G[g0 + jj * LPR];
+                            for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr[j], mi[j]);
+                            mi[j] = -mi[j];
+                            u[j] = s ^ (S)gr.flip;
+                        }
                     }
-                    const double f = A.inv_nrm[col] * own_scale;
-                    cr *= f;
-                    ci *= f;
-                    const double wr = mr * cr - mi * ci;
-                    const double wi = mr * ci + mi * cr;
-                    double vr, vi;
-                    T::load(x, col, vr, vi);
-                    if (A.conj_x) vi = -vi;
-                    ar += wr * vr - wi * vi;
-                    ai += wr * vi + wi * vr;
+                    S r[kNB];
+                    int bt[kNB], bp[kNB];
+                    if constexpr (sizeof(S) == 4) {
+                        canonical32_batch<kNB, TRACK>(A.g, u, r, bt, bp);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < kNB; j++) r[j] = canonical<S, TRACK>(A.g, u[j], mask, bt[j], bp[j]);
+                    }
+                    // first probe of every neighbour in flight together
+                    uint64_t h[kNB], e[kNB];
+#pragma unroll
+                    for (int j = 0; j < kNB; j++) {
+                        h[j] = state_hash((uint64_t)r[j]) & A.hmask;
+                        e[j] = A.hkeys[h[j]];
+                    }
+                    int64_t col[kNB];
+#pragma unroll
+                    for (int j = 0; j < kNB; j++) {
+                        col[j] = act[j] ? hash_resolve<S>(A, r[j], h[j], e[j]) : -1;
+                        if (col[j] < 0) act[j] = false;
+                    }
+                    double f[kNB], vr[kNB], vi[kNB];
+#pragma unroll
+                    for (int j = 0; j < kNB; j++) {
+                        const int64_t c = act[j] ? col[j] : row;
+                        f[j] = A.inv_nrm[c];
+                        T::load(x, c, vr[j], vi[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kNB; j++) {
+                        if (!act[j]) continue;
+                        double cr = 1.0, ci = 0.0;
+                        if (TRACK) {
+                            const double lr = A.g.lam_re[bt[j]], li = A.g.lam_im[bt[j]];
+                            const double pr = A.g.pch_re[bp[j]], pi = A.g.pch_im[bp[j]];
+                            cr = lr * pr - li * pi;
+                            ci = lr * pi + li * pr;
+                        }
+                        const double fac = f[j] * own_scale;
+                        cr *= fac;
+                        ci *= fac;
+                        const double wr = mr[j] * cr - mi[j] * ci;
+                        const double wi = mr[j] * ci + mi[j] * cr;
+                        double v_i = vi[j];
+                        if (A.conj_x) v_i = -v_i;
+                        ar += wr * vr[j] - wi * v_i;
+                        ai += wr * v_i + wi * vr[j];
+                    }
                 }
             }
         }
@@ -362,41 +392,9 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
             ar += __shfl_xor(ar, o);
             ai += __shfl_xor(ai, o);
         }
-        if (valid && sub == 0) {
-            double orr = A.are * ar - A.aim * ai;
-            double oi = A.are * ai + A.aim * ar;
-            if (!A.beta_zero) {
-                double yr, yi;
-                T::load(y, row, yr, yi);
-                if (A.conj_y) yi = -yi;
-                orr += A.bre * yr - A.bim * yi;
-                oi += A.bre * yi + A.bim * yr;
-            }
-            if (A.conj_y) oi = -oi;
-            if (A.dot_partial) {
-                // FP64 <x, y_row> of the unrounded result (Rayleigh quotients of FP32-stored vectors)
-                dot_r += xr * orr + xi * oi;
-                dot_i += xr * oi - xi * orr;
-            }
-            T::store(y, row, orr, oi);
-        }
+        if (valid && sub == 0) finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
     }
-    if (A.dot_partial) {
-        s_red[threadIdx.x] = dot_r;
-        s_red[kStBlock + threadIdx.x] = dot_i;
-        __syncthreads();
-        for (int st = kStBlock / 2; st > 0; st >>= 1) {
-            if (threadIdx.x < st) {
-                s_red[threadIdx.x] += s_red[threadIdx.x + st];
-                s_red[kStBlock + threadIdx.x] += s_red[kStBlock + threadIdx.x + st];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            A.dot_partial[2 * blockIdx.x] = s_red[0];
-            A.dot_partial[2 * blockIdx.x + 1] = s_red[kStBlock];
-        }
-    }
+    if (A.dot_partial) write_dot_partial<kStBlock>(A, s_red, dot_r, dot_i);
 }
 
 struct Launch {
@@ -466,18 +464,18 @@ void dispatch_v(const MVArgs &a, int dtype, bool cc, int mode, bool track, bool 
     }
 }
 
+int grid_for(int64_t n, int rows_per_block) {
+    int64_t blocks = (n + rows_per_block - 1) / rows_per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)1 << 22));
+}
+
 }  // namespace
 
 int default_lanes_per_row(const BasisHost &b) { return b.mode == MODE_STORED ? 2 : 1; }
 
-int matvec_grid_lpr(int64_t n, int lpr, int block = kMvBlock) {
-    const int rows = block / lpr;
-    int64_t blocks = (n + rows - 1) / rows;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)1 << 22));
-}
-
-// blocks of a matvec launch that carries FP64 dot partials (one per block)
-int matvec_grid(int64_t n) { return matvec_grid_lpr(n, 8, kStBlock); }
+// blocks of a matvec launch that carries FP64 dot partials (one per block):
+// the stored kernel at LPR = 8, and the implicit kernel sized the same way
+int matvec_grid(int64_t n) { return grid_for(n, kStBlock / 8); }
 
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
                    double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
@@ -492,19 +490,17 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     a.g = group_to_dev(b.group);
     a.reps = b.use64 ? (const void *)b.reps64.get() : (const void *)b.reps32.get();
     a.inv_nrm = b.inv_nrm.get();
-    a.samp = b.use64 ? (const void *)b.samp64.get() : (const void *)b.samp32.get();
-    a.nsample = (int)b.nsample;
-    a.shift = b.sample_shift;
+    a.hkeys = b.hkeys.get();
+    a.hvals = b.hvals.get();
+    a.hmask = b.hmask;
+    a.hmax_probe = b.hmax_probe;
     a.binom = b.binom.get();
     // plain products use the adjoint couplings (pull form), adjoint products the couplings
     const std::vector<CTerm> &dg = adjoint ? op.diag_terms_adj : op.diag_terms;
     const std::vector<CTerm> &of = adjoint ? op.off_terms_adj : op.off_terms;
     const std::vector<GroupOfTerms> &gr = adjoint ? op.groups_adj : op.groups;
-    a.diag = adjoint ? op.d_diag_adj.get() : op.d_diag.get();
     a.ndiag = (int)dg.size();
-    a.off = adjoint ? op.d_off_adj.get() : op.d_off.get();
     a.noff = (int)of.size();
-    a.groups = adjoint ? op.d_groups_adj.get() : op.d_groups.get();
     a.ngroups = (int)gr.size();
     a.user_diag = op.has_user_diag ? op.d_user_diag.get() : nullptr;
     a.user_diag_conj = adjoint ? 1 : 0;
@@ -533,19 +529,16 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         off = o + bytes;
         return (int)o;
     };
-    const size_t ssz = b.use64 ? 8 : 4;
-    a.off_samp = place(b.mode == MODE_STORED ? ssz * (size_t)b.nsample : 0);
-    a.off_binom = place((b.mode != MODE_STORED && !b.use64 && b.nup >= 0) ? sizeof(uint32_t) * 33 * 65 : 0);
     const bool tlds = b.mode == MODE_STORED && a.ngroups <= kMaxLdsGroups && a.noff <= kMaxLdsTerms;
-    a.lds_groups = a.lds_terms = tlds ? 1 : 0;
+    a.off_binom = place((b.mode != MODE_STORED && !b.use64 && b.nup >= 0) ? sizeof(uint32_t) * 33 * 65 : 0);
     a.off_grp = place(tlds ? sizeof(GroupOfTerms) * a.ngroups : 0);
     a.off_term = place(tlds ? sizeof(TermDev) * a.noff : 0);
     a.off_red = place(dot_partial ? sizeof(double) * 2 * std::max(kMvBlock, kStBlock) : 0);
-    // the dot partials are indexed by block: the implicit kernel uses one
-    // lane per row, so size its grid like the stored kernel's LPR = 8 grid
-    int grid = b.mode == MODE_STORED ? matvec_grid_lpr(b.Ns, lpr, kStBlock) : matvec_grid_lpr(b.Ns, 1);
+    int grid = b.mode == MODE_STORED ? grid_for(b.Ns, kStBlock / lpr) : grid_for(b.Ns, kMvBlock);
     if (dot_partial) grid = matvec_grid(b.Ns);
-    Launch L{grid, off, s, a.groups, a.off, a.diag};
+    const bool adj = adjoint;
+    Launch L{grid, off, s, adj ? op.d_groups_adj.get() : op.d_groups.get(), adj ? op.d_off_adj.get() : op.d_off.get(),
+             adj ? op.d_diag_adj.get() : op.d_diag.get()};
     if (b.use64) dispatch_v<uint64_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     else dispatch_v<uint32_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     QS_HIP(hipGetLastError());
