@@ -4,7 +4,7 @@ import os
 import subprocess
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_ROOT, "_build", "libqsgpu.so")
+LIB_PATH = os.environ.get("QSGPU_LIB") or os.path.join(_PKG_ROOT, "_build", "libqsgpu.so")
 CSRC = os.path.join(_PKG_ROOT, "csrc")
 
 QS_OK, QS_EARG, QS_EDIM, QS_ENOCONV, QS_EOOM, QS_EDEVICE = range(6)
