@@ -188,78 +188,204 @@ __device__ __forceinline__ void finish_row(const MVArgs &A, V *y, int64_t row, d
     T::store(y, row, orr, oi);
 }
 
+// One row of an implicit basis (no symmetry / spin inversion only).
+template <typename S, typename V, bool CC, int MODE>
+__device__ __forceinline__ void implicit_row(const MVArgs &A, const GroupOfTerms *__restrict__ grp,
+                                             const TermDev *__restrict__ off, const TermDev *__restrict__ diag,
+                                             const V *__restrict__ x, V *__restrict__ y, int64_t row,
+                                             const uint32_t *s_binom, double &dot_r, double &dot_i) {
+    using T = VTraits<V>;
+    constexpr bool LDSB = sizeof(S) == 4;
+    S s;
+    if (A.nup >= 0) s = LDSB ? comb_unrank<S, uint32_t>(row, A.L, A.nup, s_binom)
+                             : comb_unrank<S, uint64_t>(row, A.L, A.nup, A.binom);
+    else s = (S)row;
+    double xr, xi;
+    T::load(x, row, xr, xi);
+    if (A.conj_x) xi = -xi;
+    double dr = 0, di = 0;
+    for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(diag[k], s, dr, di);
+    di = -di;   // conj(amp): the terms are the adjoint couplings
+    if (A.user_diag) {
+        dr += A.user_diag[2 * row];
+        di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+    }
+    double ar = dr * xr - di * xi;
+    double ai = dr * xi + di * xr;
+    for (int g0 = 0; g0 < A.ngroups; g0 += kNB) {
+        int64_t col[kNB];
+        double wr[kNB], wi[kNB];
+#pragma unroll
+        for (int j = 0; j < kNB; j++) {
+            const int gi = g0 + j;
+            col[j] = row;
+            wr[j] = 0;
+            wi[j] = 0;
+            if (gi < A.ngroups) {
+                const GroupOfTerms gr = grp[gi];
+                double mr = 0, mi = 0;
+                for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(off[k], s, mr, mi);
+                if (mr != 0.0 || mi != 0.0) {
+                    double cr;
+                    const int64_t c = implicit_col<S, MODE, LDSB>(A, s, s ^ (S)gr.flip, row, gr.lo, gr.hi,
+                                                                  s_binom, cr);
+                    if (c >= 0) {
+                        col[j] = c;
+                        wr[j] = mr * cr;
+                        wi[j] = -mi * cr;
+                    }
+                }
+            }
+        }
+        double vr[kNB], vi[kNB];
+#pragma unroll
+        for (int j = 0; j < kNB; j++) T::load(x, col[j], vr[j], vi[j]);
+#pragma unroll
+        for (int j = 0; j < kNB; j++) {
+            if (A.conj_x) vi[j] = -vi[j];
+            ar += wr[j] * vr[j] - wi[j] * vi[j];
+            ai += wr[j] * vi[j] + wi[j] * vr[j];
+        }
+    }
+    finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
+}
+
 template <typename S, typename V, bool CC, int MODE>
 __global__ __launch_bounds__(kMvBlock) void k_matvec_implicit(MVArgs A, const GroupOfTerms *__restrict__ grp,
                                                               const TermDev *__restrict__ off,
                                                               const TermDev *__restrict__ diag,
                                                               const V *__restrict__ x, V *__restrict__ y) {
-    using T = VTraits<V>;
     extern __shared__ __align__(16) char smem[];
     uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
     double *s_red = (double *)(smem + A.off_red);
-    constexpr bool LDSB = sizeof(S) == 4;
-    if (LDSB && A.nup >= 0)
+    if (sizeof(S) == 4 && A.nup >= 0)
         for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
     __syncthreads();
     const int vb = xcd_block(blockIdx.x, gridDim.x);
     double dot_r = 0, dot_i = 0;
     for (int64_t base = (int64_t)vb * kMvBlock; base < A.N; base += (int64_t)gridDim.x * kMvBlock) {
         const int64_t row = base + threadIdx.x;
-        if (row >= A.N) continue;
-        S s;
-        if (A.nup >= 0) s = LDSB ? comb_unrank<S, uint32_t>(row, A.L, A.nup, s_binom)
-                                 : comb_unrank<S, uint64_t>(row, A.L, A.nup, A.binom);
-        else s = (S)row;
-        double xr, xi;
-        T::load(x, row, xr, xi);
-        if (A.conj_x) xi = -xi;
-        double dr = 0, di = 0;
-        for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(diag[k], s, dr, di);
-        di = -di;   // conj(amp): the terms are the adjoint couplings
-        if (A.user_diag) {
-            dr += A.user_diag[2 * row];
-            di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
-        }
-        double ar = dr * xr - di * xi;
-        double ai = dr * xi + di * xr;
-        for (int g0 = 0; g0 < A.ngroups; g0 += kNB) {
-            int64_t col[kNB];
-            double wr[kNB], wi[kNB];
-#pragma unroll
-            for (int j = 0; j < kNB; j++) {
-                const int gi = g0 + j;
-                col[j] = row;
-                wr[j] = 0;
-                wi[j] = 0;
-                if (gi < A.ngroups) {
-                    const GroupOfTerms gr = grp[gi];
-                    double mr = 0, mi = 0;
-                    for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(off[k], s, mr, mi);
-                    if (mr != 0.0 || mi != 0.0) {
-                        double cr;
-                        const int64_t c = implicit_col<S, MODE, LDSB>(A, s, s ^ (S)gr.flip, row, gr.lo, gr.hi,
-                                                                      s_binom, cr);
-                        if (c >= 0) {
-                            col[j] = c;
-                            wr[j] = mr * cr;
-                            wi[j] = -mi * cr;
-                        }
-                    }
-                }
-            }
-            double vr[kNB], vi[kNB];
-#pragma unroll
-            for (int j = 0; j < kNB; j++) T::load(x, col[j], vr[j], vi[j]);
-#pragma unroll
-            for (int j = 0; j < kNB; j++) {
-                if (A.conj_x) vi[j] = -vi[j];
-                ar += wr[j] * vr[j] - wi[j] * vi[j];
-                ai += wr[j] * vi[j] + wi[j] * vr[j];
-            }
-        }
-        finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
+        if (row < A.N) implicit_row<S, V, CC, MODE>(A, grp, off, diag, x, y, row, s_binom, dot_r, dot_i);
     }
     if (A.dot_partial) write_dot_partial<kMvBlock>(A, s_red, dot_r, dot_i);
+}
+
+// One tile of kStBlock / LPR rows starting at `base` (stored basis).
+template <typename S, typename V, bool CC, bool TRACK, int LPR>
+__device__ __forceinline__ void stored_tile(const MVArgs &A, const GroupOfTerms *G, const TermDev *TT,
+                                            const TermDev *__restrict__ diag, const V *__restrict__ x,
+                                            V *__restrict__ y, int64_t base, double &dot_r, double &dot_i) {
+    using T = VTraits<V>;
+    const S mask = (S)A.mask;
+    const S *reps = (const S *)A.reps;
+    const int sub = threadIdx.x % LPR;
+    const int64_t row = base + threadIdx.x / LPR;
+    const bool valid = row < A.N;
+    double ar = 0, ai = 0, xr = 0, xi = 0;
+    if (valid) {
+        const S s = reps[row];
+        const double own_scale = 1.0 / A.inv_nrm[row];
+        if (sub == 0) {
+            T::load(x, row, xr, xi);
+            if (A.conj_x) xi = -xi;
+            double dr = 0, di = 0;
+            for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(diag[k], s, dr, di);
+            di = -di;
+            if (A.user_diag) {
+                dr += A.user_diag[2 * row];
+                di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+            }
+            ar = dr * xr - di * xi;
+            ai = dr * xi + di * xr;
+        }
+        for (int g0 = sub; g0 < A.ngroups; g0 += 64 * LPR) {
+            uint64_t live = 0;
+            for (int jj = 0; jj < 64; jj++) {
+                const int gi = g0 + jj * LPR;
+                if (gi >= A.ngroups) break;
+                const GroupOfTerms gr = G[gi];
+                double mr = 0, mi = 0;
+                for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr, mi);
+                if (mr != 0.0 || mi != 0.0) live |= 1ULL << jj;
+            }
+            while (live) {
+                // a batch of up to kNB live neighbours
+                S u[kNB];
+                double mr[kNB], mi[kNB];
+                bool act[kNB];
+#pragma unroll
+                for (int j = 0; j < kNB; j++) {
+                    act[j] = live != 0;
+                    mr[j] = 0;
+                    mi[j] = 0;
+                    u[j] = s;
+                    if (act[j]) {
+                        const int jj = __builtin_ctzll(live);
+                        live &= live - 1;
+                        const GroupOfTerms gr = G[g0 + jj * LPR];
+                        for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr[j], mi[j]);
+                        mi[j] = -mi[j];
+                        u[j] = s ^ (S)gr.flip;
+                    }
+                }
+                S r[kNB];
+                int bt[kNB], bp[kNB];
+                if constexpr (sizeof(S) == 4) {
+                    canonical32_batch<kNB, TRACK>(A.g, u, r, bt, bp);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kNB; j++) r[j] = canonical<S, TRACK>(A.g, u[j], mask, bt[j], bp[j]);
+                }
+                // first probe of every neighbour in flight together
+                uint64_t h[kNB], e[kNB];
+#pragma unroll
+                for (int j = 0; j < kNB; j++) {
+                    h[j] = state_hash((uint64_t)r[j]) & A.hmask;
+                    e[j] = A.hkeys[h[j]];
+                }
+                int64_t col[kNB];
+#pragma unroll
+                for (int j = 0; j < kNB; j++) {
+                    col[j] = act[j] ? hash_resolve<S>(A, r[j], h[j], e[j]) : -1;
+                    if (col[j] < 0) act[j] = false;
+                }
+                double f[kNB], vr[kNB], vi[kNB];
+#pragma unroll
+                for (int j = 0; j < kNB; j++) {
+                    const int64_t c = act[j] ? col[j] : row;
+                    f[j] = A.inv_nrm[c];
+                    T::load(x, c, vr[j], vi[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < kNB; j++) {
+                    if (!act[j]) continue;
+                    double cr = 1.0, ci = 0.0;
+                    if (TRACK) {
+                        const double lr = A.g.lam_re[bt[j]], li = A.g.lam_im[bt[j]];
+                        const double pr = A.g.pch_re[bp[j]], pi = A.g.pch_im[bp[j]];
+                        cr = lr * pr - li * pi;
+                        ci = lr * pi + li * pr;
+                    }
+                    const double fac = f[j] * own_scale;
+                    cr *= fac;
+                    ci *= fac;
+                    const double wr = mr[j] * cr - mi[j] * ci;
+                    const double wi = mr[j] * ci + mi[j] * cr;
+                    double v_i = vi[j];
+                    if (A.conj_x) v_i = -v_i;
+                    ar += wr * vr[j] - wi * v_i;
+                    ai += wr * v_i + wi * vr[j];
+                }
+            }
+        }
+    }
+    // fixed-order combination of the LPR partial sums
+#pragma unroll
+    for (int o = LPR / 2; o >= 1; o >>= 1) {
+        ar += __shfl_xor(ar, o);
+        ai += __shfl_xor(ai, o);
+    }
+    if (valid && sub == 0) finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
 }
 
 template <typename S, typename V, bool CC, bool TRACK, int LPR, bool TLDS>
@@ -267,7 +393,6 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
                                                             const TermDev *__restrict__ off,
                                                             const TermDev *__restrict__ diag,
                                                             const V *__restrict__ x, V *__restrict__ y) {
-    using T = VTraits<V>;
     extern __shared__ __align__(16) char smem[];
     GroupOfTerms *s_grp = (GroupOfTerms *)(smem + A.off_grp);
     TermDev *s_term = (TermDev *)(smem + A.off_term);
@@ -279,121 +404,11 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
     __syncthreads();
     const GroupOfTerms *G = TLDS ? s_grp : grp;
     const TermDev *TT = TLDS ? s_term : off;
-    const S mask = (S)A.mask;
-    const S *reps = (const S *)A.reps;
-    const int sub = threadIdx.x % LPR;
     constexpr int kRows = kStBlock / LPR;
     const int vb = xcd_block(blockIdx.x, gridDim.x);
     double dot_r = 0, dot_i = 0;
-    for (int64_t base = (int64_t)vb * kRows; base < A.N; base += (int64_t)gridDim.x * kRows) {
-        const int64_t row = base + threadIdx.x / LPR;
-        const bool valid = row < A.N;
-        double ar = 0, ai = 0, xr = 0, xi = 0;
-        if (valid) {
-            const S s = reps[row];
-            const double own_scale = 1.0 / A.inv_nrm[row];
-            if (sub == 0) {
-                T::load(x, row, xr, xi);
-                if (A.conj_x) xi = -xi;
-                double dr = 0, di = 0;
-                for (int k = 0; k < A.ndiag; k++) term_amp<S, CC>(diag[k], s, dr, di);
-                di = -di;
-                if (A.user_diag) {
-                    dr += A.user_diag[2 * row];
-                    di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
-                }
-                ar = dr * xr - di * xi;
-                ai = dr * xi + di * xr;
-            }
-            for (int g0 = sub; g0 < A.ngroups; g0 += 64 * LPR) {
-                uint64_t live = 0;
-                for (int jj = 0; jj < 64; jj++) {
-                    const int gi = g0 + jj * LPR;
-                    if (gi >= A.ngroups) break;
-                    const GroupOfTerms gr = G[gi];
-                    double mr = 0, mi = 0;
-                    for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr, mi);
-                    if (mr != 0.0 || mi != 0.0) live |= 1ULL << jj;
-                }
-                while (live) {
-                    // a batch of up to kNB live neighbours
-                    S u[kNB];
-                    double mr[kNB], mi[kNB];
-                    bool act[kNB];
-#pragma unroll
-                    for (int j = 0; j < kNB; j++) {
-                        act[j] = live != 0;
-                        mr[j] = 0;
-                        mi[j] = 0;
-                        u[j] = s;
-                        if (act[j]) {
-                            const int jj = __builtin_ctzll(live);
-                            live &= live - 1;
-                            const GroupOfTerms gr = G[g0 + jj * LPR];
-                            for (int k = gr.t0; k < gr.t1; k++) term_amp<S, CC>(TT[k], s, mr[j], mi[j]);
-                            mi[j] = -mi[j];
-                            u[j] = s ^ (S)gr.flip;
-                        }
-                    }
-                    S r[kNB];
-                    int bt[kNB], bp[kNB];
-                    if constexpr (sizeof(S) == 4) {
-                        canonical32_batch<kNB, TRACK>(A.g, u, r, bt, bp);
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < kNB; j++) r[j] = canonical<S, TRACK>(A.g, u[j], mask, bt[j], bp[j]);
-                    }
-                    // first probe of every neighbour in flight together
-                    uint64_t h[kNB], e[kNB];
-#pragma unroll
-                    for (int j = 0; j < kNB; j++) {
-                        h[j] = state_hash((uint64_t)r[j]) & A.hmask;
-                        e[j] = A.hkeys[h[j]];
-                    }
-                    int64_t col[kNB];
-#pragma unroll
-                    for (int j = 0; j < kNB; j++) {
-                        col[j] = act[j] ? hash_resolve<S>(A, r[j], h[j], e[j]) : -1;
-                        if (col[j] < 0) act[j] = false;
-                    }
-                    double f[kNB], vr[kNB], vi[kNB];
-#pragma unroll
-                    for (int j = 0; j < kNB; j++) {
-                        const int64_t c = act[j] ? col[j] : row;
-                        f[j] = A.inv_nrm[c];
-                        T::load(x, c, vr[j], vi[j]);
-                    }
-#pragma unroll
-                    for (int j = 0; j < kNB; j++) {
-                        if (!act[j]) continue;
-                        double cr = 1.0, ci = 0.0;
-                        if (TRACK) {
-                            const double lr = A.g.lam_re[bt[j]], li = A.g.lam_im[bt[j]];
-                            const double pr = A.g.pch_re[bp[j]], pi = A.g.pch_im[bp[j]];
-                            cr = lr * pr - li * pi;
-                            ci = lr * pi + li * pr;
-                        }
-                        const double fac = f[j] * own_scale;
-                        cr *= fac;
-                        ci *= fac;
-                        const double wr = mr[j] * cr - mi[j] * ci;
-                        const double wi = mr[j] * ci + mi[j] * cr;
-                        double v_i = vi[j];
-                        if (A.conj_x) v_i = -v_i;
-                        ar += wr * vr[j] - wi * v_i;
-                        ai += wr * v_i + wi * vr[j];
-                    }
-                }
-            }
-        }
-        // fixed-order combination of the LPR partial sums
-#pragma unroll
-        for (int o = LPR / 2; o >= 1; o >>= 1) {
-            ar += __shfl_xor(ar, o);
-            ai += __shfl_xor(ai, o);
-        }
-        if (valid && sub == 0) finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
-    }
+    for (int64_t base = (int64_t)vb * kRows; base < A.N; base += (int64_t)gridDim.x * kRows)
+        stored_tile<S, V, CC, TRACK, LPR>(A, G, TT, diag, x, y, base, dot_r, dot_i);
     if (A.dot_partial) write_dot_partial<kStBlock>(A, s_red, dot_r, dot_i);
 }
 
