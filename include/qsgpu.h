@@ -128,8 +128,12 @@ int qs_op_set_diagonal(qs_op *op, const double *diag);
 int qs_op_set_stream(qs_op *op, void *stream);
 int qs_op_stream(const qs_op *op, void **stream);
 
-/* Matvec kernel variant: lanes cooperating on one row (1, 2, 4 or 8; 0 =
- * default).  No reference counterpart: the reference has one CPU loop. */
+/* Matvec kernel variant: 0 = automatic; 1, 2, 4 or 8 = the general kernel
+ * with that many lanes cooperating on one row; QS_VARIANT_HOP = the two-bit
+ * hop kernel (couplings that flip two sites and test only those two, L <= 32;
+ * symmetry sectors also need the parent lookup table).  No reference
+ * counterpart: the reference has one CPU loop. */
+#define QS_VARIANT_HOP 16
 int qs_op_set_variant(qs_op *op, int lanes_per_row);
 
 /* Autotuner: times every variant for `dtype` on device-resident random

@@ -176,6 +176,44 @@ void upload_groups(const std::vector<GroupOfTerms> &src, DevBuf<GroupOfTerms> &d
     QS_HIP(hipMemcpy(dst.get(), h.data(), sizeof(GroupOfTerms) * h.size(), hipMemcpyHostToDevice));
 }
 
+template <typename T>
+void upload_vec(const std::vector<T> &src, DevBuf<T> &dst) {
+    std::vector<T> h(src);
+    if (h.empty()) h.push_back(T{});
+    dst.alloc(h.size());
+    QS_HIP(hipMemcpy(dst.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+void upload_fast(const FastTerms &f, DevBuf<HopDev> &hops, DevBuf<DiagClass> &dcls, DevBuf<TermDev> &drest) {
+    upload_vec(f.hops, hops);
+    upload_vec(f.dcls, dcls);
+    upload_terms(f.drest, drest);
+}
+
+// Device term lists of an operator from its group-averaged couplings.
+void setup_terms(OpHost &o, const std::vector<CTerm> &Havg, const double *diag) {
+    const BasisHost &bb = *o.basis;
+    // pull-form kernels: plain products apply the adjoint couplings
+    std::vector<CTerm> Hadj;
+    for (auto &x : Havg) Hadj.push_back(adjoint_term(x));
+    split_terms(Hadj, o.diag_terms, o.off_terms, o.groups);
+    split_terms(Havg, o.diag_terms_adj, o.off_terms_adj, o.groups_adj);
+    upload_terms(o.diag_terms, o.d_diag);
+    upload_terms(o.off_terms, o.d_off);
+    upload_groups(o.groups, o.d_groups);
+    upload_terms(o.diag_terms_adj, o.d_diag_adj);
+    upload_terms(o.off_terms_adj, o.d_off_adj);
+    upload_groups(o.groups_adj, o.d_groups_adj);
+    o.fast = compile_fast(o.diag_terms, o.off_terms, o.groups, bb.L);
+    o.fast_adj = compile_fast(o.diag_terms_adj, o.off_terms_adj, o.groups_adj, bb.L);
+    upload_fast(o.fast, o.d_hops, o.d_dcls, o.d_drest);
+    upload_fast(o.fast_adj, o.d_hops_adj, o.d_dcls_adj, o.d_drest_adj);
+    if (o.has_user_diag) {
+        o.d_user_diag.alloc((size_t)2 * bb.Ns);
+        QS_HIP(hipMemcpy(o.d_user_diag.get(), diag, sizeof(double) * 2 * bb.Ns, hipMemcpyHostToDevice));
+    }
+}
+
 hipStream_t ensure_stream(OpHost &o) {
     if (o.stream) return o.stream;
     if (!o.own_stream) QS_HIP(hipStreamCreate(&o.own_stream));
@@ -425,20 +463,22 @@ int qs_op_create(const qs_basis *b, int n_terms, const char *const *opstr, const
         for (auto &x : Havg)
             if (x.A.imag() != 0) o.is_complex = 1;
         o.hermitian = check_herm ? 1 : ((herm && !diag_complex) ? 1 : 0);
-        // pull-form kernels: plain products apply the adjoint couplings
-        std::vector<CTerm> Hadj;
-        for (auto &x : Havg) Hadj.push_back(adjoint_term(x));
-        split_terms(Hadj, o.diag_terms, o.off_terms, o.groups);
-        split_terms(Havg, o.diag_terms_adj, o.off_terms_adj, o.groups_adj);
-        upload_terms(o.diag_terms, o.d_diag);
-        upload_terms(o.off_terms, o.d_off);
-        upload_groups(o.groups, o.d_groups);
-        upload_terms(o.diag_terms_adj, o.d_diag_adj);
-        upload_terms(o.off_terms_adj, o.d_off_adj);
-        upload_groups(o.groups_adj, o.d_groups_adj);
-        if (o.has_user_diag) {
-            o.d_user_diag.alloc((size_t)2 * bb.Ns);
-            QS_HIP(hipMemcpy(o.d_user_diag.get(), diag, sizeof(double) * 2 * bb.Ns, hipMemcpyHostToDevice));
+        setup_terms(o, Havg, diag);
+        if (bb.mode == MODE_FULL) {
+            // operators.jl:2372-2429 extracts the structurally diagonal terms
+            // of a basis without symmetry; mul! scales them (and the user
+            // diagonal) by alpha (:3097-3117) but conjugates alpha together
+            // with the other couplings' amplitudes (:2581)
+            std::vector<Coupling> dcs;
+            for (auto &c : cs)
+                if (c.op.find_first_not_of("Izn|") == std::string::npos) dcs.push_back(c);
+            o.extracted = std::make_unique<OpHost>();
+            OpHost &e = *o.extracted;
+            e.basis = &bb;
+            e.is_complex = o.is_complex;
+            e.has_user_diag = o.has_user_diag;
+            e.user_diag = o.user_diag;
+            setup_terms(e, consolidate_cterms(compile_couplings(dcs, bb.L, bb.pauli)), diag);
         }
         ensure_stream(o);
         *out = h.release();
@@ -463,13 +503,15 @@ int qs_op_set_diagonal(qs_op *op, const double *diag) {
     return guard([&] {
         check_ptr(op, "op");
         check_ptr(diag, "diag");
-        OpHost &o = op->o;
-        int64_t n = o.basis->Ns;
-        o.user_diag.resize((size_t)n);
-        for (int64_t i = 0; i < n; i++) o.user_diag[i] = cd(diag[2 * i], diag[2 * i + 1]);
-        o.has_user_diag = true;
-        o.d_user_diag.alloc((size_t)2 * n);
-        QS_HIP(hipMemcpy(o.d_user_diag.get(), diag, sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+        int64_t n = op->o.basis->Ns;
+        for (OpHost *o : {&op->o, op->o.extracted.get()}) {
+            if (!o) continue;
+            o->user_diag.resize((size_t)n);
+            for (int64_t i = 0; i < n; i++) o->user_diag[i] = cd(diag[2 * i], diag[2 * i + 1]);
+            o->has_user_diag = true;
+            o->d_user_diag.alloc((size_t)2 * n);
+            QS_HIP(hipMemcpy(o->d_user_diag.get(), diag, sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+        }
     });
 }
 
@@ -492,8 +534,11 @@ int qs_op_stream(const qs_op *op, void **stream) {
 int qs_op_set_variant(qs_op *op, int lanes_per_row) {
     return guard([&] {
         check_ptr(op, "op");
-        if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8)
-            fail(QS_EARG, "lanes_per_row must be 0, 1, 2, 4 or 8");
+        if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 &&
+            lanes_per_row != 8 && lanes_per_row != QS_VARIANT_HOP)
+            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8 or QS_VARIANT_HOP");
+        if (lanes_per_row == QS_VARIANT_HOP && !(hop_eligible(op->o, false) && hop_eligible(op->o, true)))
+            fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
         op->o.lanes_per_row = lanes_per_row;
     });
 }
@@ -513,7 +558,11 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
         int best = 0;
         float best_t = 1e30f;
         if (repeats < 1) repeats = 3;
-        for (int lpr : {1, 2, 4, 8}) {
+        std::vector<int> cands;
+        if (hop_eligible(o, false)) cands.push_back(QS_VARIANT_HOP);
+        if (o.basis->mode == MODE_STORED) cands.insert(cands.end(), {1, 2, 4, 8});
+        else cands.push_back(1);
+        for (int lpr : cands) {
             o.lanes_per_row = lpr;
             launch_matvec(o, dtype, x.get(), y.get(), 1, 0, 0, 0, false, false, false, s);
             QS_HIP(hipEventRecord(e0, s));
@@ -570,10 +619,20 @@ int qs_op_matvec(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, do
         bool adj = transposed != 0;
         bool cj = (transposed != 0) != (conjugated != 0);
         double ar = alpha_re, ai = cj ? -alpha_im : alpha_im, br = beta_re, bi = cj ? -beta_im : beta_im;
+        // without symmetry a conjugated product conjugates alpha with the
+        // coupling amplitudes but not on the extracted diagonal
+        // (operators.jl:2581 vs :3097-3117): apply everything with conj(alpha),
+        // then add (alpha - conj(alpha)) times the extracted diagonal
+        const bool split = conjugated && alpha_im != 0 && o.extracted;
         for (int64_t c = 0; c < ncols; c++) {
             const char *xc = (const char *)dx + (size_t)c * n * esz;
             char *yc = (char *)dy + (size_t)c * n * esz;
-            launch_matvec(o, dtype, xc, yc, ar, ai, br, bi, adj, cj, cj, s);
+            if (!split) {
+                launch_matvec(o, dtype, xc, yc, ar, ai, br, bi, adj, cj, cj, s);
+            } else {
+                launch_matvec(o, dtype, xc, yc, ar, -ai, br, bi, adj, cj, cj, s);
+                launch_matvec(*o.extracted, dtype, xc, yc, 0.0, 2 * ai, 1.0, 0.0, adj, cj, cj, s);
+            }
         }
         if (!on_device) {
             QS_HIP(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, s));

@@ -161,6 +161,94 @@ __global__ void k_hash_insert(const S *reps, int64_t n, uint64_t *keys, uint32_t
     atomicMax(max_probe, probe);
 }
 
+// Column states of an implicit basis (every nup-word of the parent range, in
+// ascending order): unrank the first state of a chunk, then Gosper.
+__global__ __launch_bounds__(kBlock) void k_states(int L, int nup, int64_t n, const uint64_t *binom,
+                                                   uint32_t *states) {
+    const int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    if (base >= n) return;
+    uint32_t u = comb_unrank<uint32_t, uint64_t>(base, L, nup, binom);
+    const int64_t end = base + kChunk < n ? base + kChunk : n;
+    for (int64_t i = base; i < end; i++) {
+        states[i] = u;
+        if (i + 1 < end) u = gosper_next(u);
+    }
+}
+
+// Parent lookup table: entry of parent rank i = column of its orbit minimum
+// | (group element reaching the minimum) << colbits, or all ones.  The
+// element is the one canonical32 reports, so the table path applies exactly
+// the characters of the canonicalising path.
+template <bool TRACK>
+__global__ __launch_bounds__(kBlock) void k_ptab(GroupDev g, int nup, int64_t npar, const uint64_t *binom,
+                                                 const uint64_t *hkeys, uint64_t hmask, int max_probe, int colbits,
+                                                 uint32_t *ptab) {
+    const int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    if (base >= npar) return;
+    uint32_t u = parent_state<uint32_t>(base, g.L, nup, binom);
+    const int64_t end = base + kChunk < npar ? base + kChunk : npar;
+    for (int64_t i = base; i < end; i++) {
+        int bt, bp;
+        const uint32_t r = canonical32<TRACK>(g, u, bt, bp);
+        const int64_t c = hash_find32(hkeys, hmask, max_probe, r);
+        uint32_t e = 0xFFFFFFFFu;
+        if (c >= 0) e = (uint32_t)c | (TRACK ? (uint32_t)(bt * g.npoint + bp) << colbits : 0u);
+        ptab[i] = e;
+        if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : u + 1;
+    }
+}
+
+bool group_tracks(const GroupHost &g) {
+    for (int p = 0; p < g.npoint; p++)
+        if (g.pchar[p] != cd(1, 0)) return true;
+    for (int t = 0; t < g.nT; t++)
+        if (g.lam[t] != cd(1, 0)) return true;
+    return false;
+}
+
+// Device memory a lookup structure may take: at most a quarter of what is free.
+bool fits_device(size_t bytes) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return bytes <= fr / 4;
+}
+
+void build_ptab(BasisHost &b) {
+    const bool track = group_tracks(b.group);
+    const int order = b.group.order();
+    int ebits = 0;
+    if (track)
+        while ((1 << ebits) < order) ebits++;
+    const int colbits = 32 - ebits;
+    if (b.Ns >= ((int64_t)1 << colbits) - 1) return;
+    if (b.npar > ((int64_t)1 << 31) || !fits_device((size_t)b.npar * 4)) return;
+    b.ptab.alloc((size_t)b.npar);
+    b.ptab_colbits = colbits;
+    GroupDev g = group_to_dev(b.group);
+    const int64_t lanes = (b.npar + kChunk - 1) / kChunk;
+    const unsigned nblk = (unsigned)((lanes + kBlock - 1) / kBlock);
+    if (track)
+        hipLaunchKernelGGL((k_ptab<true>), dim3(nblk), dim3(kBlock), 0, 0, g, b.nup, b.npar, b.binom.get(),
+                           b.hkeys.get(), b.hmask, b.hmax_probe, colbits, b.ptab.get());
+    else
+        hipLaunchKernelGGL((k_ptab<false>), dim3(nblk), dim3(kBlock), 0, 0, g, b.nup, b.npar, b.binom.get(),
+                           b.hkeys.get(), b.hmask, b.hmax_probe, colbits, b.ptab.get());
+    QS_HIP(hipGetLastError());
+}
+
+void build_implicit_states(BasisHost &b) {
+    if (b.use64 || b.nup < 0 || !fits_device((size_t)b.Ns * 4)) return;
+    b.reps32.alloc((size_t)b.Ns);
+    const int64_t lanes = (b.Ns + kChunk - 1) / kChunk;
+    const unsigned nblk = (unsigned)((lanes + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_states, dim3(nblk), dim3(kBlock), 0, 0, b.L, b.nup, b.Ns, b.binom.get(), b.reps32.get());
+    QS_HIP(hipGetLastError());
+    b.states_stored = true;
+}
+
 template <typename S>
 void build_stored(BasisHost &b, int64_t npar) {
     GroupDev g = group_to_dev(b.group);
@@ -225,18 +313,28 @@ void build_basis_device(BasisHost &b) {
     QS_HIP(hipMemcpy(b.binom.get(), tab.data(), sizeof(uint64_t) * tab.size(), hipMemcpyHostToDevice));
     b.use64 = b.L > 32;
     int64_t npar = b.nup >= 0 ? binom(b.L, b.nup) : (int64_t)1 << b.L;
+    b.npar = npar;
     if (b.mode == MODE_FULL) {
         b.Ns = npar;
+        build_implicit_states(b);
+        QS_HIP(hipDeviceSynchronize());
         return;
     }
     if (b.mode == MODE_ZONLY) {
         // canonical states are the ones with bit L-1 clear: C(L-1, nup) of them
         b.Ns = b.nup >= 0 ? binom(b.L - 1, b.nup) : (int64_t)1 << (b.L - 1);
         if (b.Ns == 0) fail(QS_EARG, "the requested symmetry sector is empty");
+        build_implicit_states(b);
+        QS_HIP(hipDeviceSynchronize());
         return;
     }
-    if (b.use64) build_stored<uint64_t>(b, npar);
-    else build_stored<uint32_t>(b, npar);
+    if (b.use64) {
+        build_stored<uint64_t>(b, npar);
+    } else {
+        build_stored<uint32_t>(b, npar);
+        build_ptab(b);
+        QS_HIP(hipDeviceSynchronize());
+    }
 }
 
 }  // namespace qs

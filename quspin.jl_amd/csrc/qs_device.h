@@ -211,6 +211,50 @@ __device__ __forceinline__ void canonical32_batch(const GroupDev &g, const uint3
 // Hash of a canonical state for the column table (Fibonacci hashing).
 __device__ __forceinline__ uint64_t state_hash(uint64_t r) { return (r * 0x9E3779B97F4A7C15ULL) >> 20; }
 
+// Column of canonical state r (L <= 32) in the (state << 32 | column) table,
+// or -1.
+__device__ __forceinline__ int64_t hash_find32(const uint64_t *keys, uint64_t hmask, int max_probe, uint32_t r) {
+    uint64_t h = state_hash((uint64_t)r) & hmask;
+    for (int p = 0; p < max_probe; p++) {
+        const uint64_t e = keys[h];
+        if (e == ~0ULL) return -1;
+        if ((uint32_t)(e >> 32) == r) return (int64_t)(uint32_t)e;
+        h = (h + 1) & hmask;
+    }
+    return -1;
+}
+
+// Colex rank from a 32-bit binomial table C(n, k) at [n * 65 + k].
+__device__ __forceinline__ uint32_t comb_rank32(uint32_t u, const uint32_t *binom) {
+    uint32_t r = 0;
+    int k = 1;
+    while (u) {
+        r += binom[ctz(u) * 65 + k];
+        k++;
+        u &= u - 1;
+    }
+    return r;
+}
+
+// Colex rank change (mod 2^32) when the particle on bit i or j (i < j, the
+// two bits differ) hops to the other one: the moved particle changes index
+// and the particles strictly between the two bits shift by one.
+__device__ __forceinline__ uint32_t rank_delta32(uint32_t s, int i, int j, const uint32_t *binom) {
+    const int c = popc(s & ((1u << i) - 1));
+    const uint32_t between = s & ((j < 32 ? (1u << j) : 0u) - (2u << i));
+    const int m = popc(between);
+    uint32_t d = binom[j * 65 + c + m + 1] - binom[i * 65 + c + 1];
+    uint32_t b = between;
+    int l = 1;
+    while (b) {
+        const int q = ctz(b);
+        d += binom[q * 65 + c + l] - binom[q * 65 + c + l + 1];
+        l++;
+        b &= b - 1;
+    }
+    return ((s >> i) & 1) ? d : (uint32_t)0 - d;
+}
+
 // Combinadic (colex) rank of a word with w set bits: sum_k C(p_k, k).
 // binom is laid out as C(n, k) at [n * 65 + k].
 template <typename S, typename B>

@@ -482,6 +482,65 @@ void split_terms(const std::vector<CTerm> &all, std::vector<CTerm> &diag, std::v
     }
 }
 
+FastTerms compile_fast(const std::vector<CTerm> &diag, const std::vector<CTerm> &off,
+                       const std::vector<GroupOfTerms> &groups, int L)
+{
+    FastTerms f;
+    // two-bit hop tables: the summed amplitude of a group as a function of
+    // its two bits, accumulated in term order as the generic kernel does
+    f.hop_ok = true;
+    for (const auto &g : groups) {
+        if (g.lo < 0) { f.hop_ok = false; break; }
+        HopDev h{};
+        h.lo = g.lo;
+        h.hi = g.hi;
+        for (int k = g.t0; k < g.t1; k++)
+            if ((off[k].cmask | off[k].smask) & ~g.flip) f.hop_ok = false;
+        if (!f.hop_ok) break;
+        for (int pat = 0; pat < 4; pat++) {
+            const uint64_t s = ((uint64_t)(pat & 1) << g.lo) | ((uint64_t)(pat >> 1) << g.hi);
+            double re = 0, im = 0;
+            for (int k = g.t0; k < g.t1; k++) {
+                const CTerm &t = off[k];
+                if ((s & t.cmask) != t.cval) continue;
+                const double sg = (__builtin_popcountll(s & t.smask) & 1) ? -1.0 : 1.0;
+                re += sg * t.A.real();
+                im += sg * t.A.imag();
+            }
+            h.re[pat] = re;
+            h.im[pat] = im;
+        }
+        f.hops.push_back(h);
+    }
+    if (!f.hop_ok) f.hops.clear();
+    // sign terms on one site or on a pair at cyclic distance d, grouped by (d, A)
+    for (const auto &t : diag) {
+        const int nb = __builtin_popcountll(t.smask);
+        if (t.cmask != 0 || nb < 1 || nb > 2) {
+            f.drest.push_back(t);
+            continue;
+        }
+        int d = 0, base = __builtin_ctzll(t.smask);
+        if (nb == 2) {
+            const int i = base, j = 63 - __builtin_clzll(t.smask);
+            const int d1 = j - i;
+            if (d1 <= L - d1) { d = d1; base = i; }
+            else { d = L - d1; base = j; }
+        }
+        bool placed = false;
+        for (auto &c : f.dcls) {
+            if (c.d == d && c.re == t.A.real() && c.im == t.A.imag() && !((c.mask >> base) & 1)) {
+                c.mask |= 1ULL << base;
+                c.n++;
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) f.dcls.push_back(DiagClass{1ULL << base, d, 1, t.A.real(), t.A.imag()});
+    }
+    return f;
+}
+
 // ---------------------------------------------------- dense eigensolvers --
 // Cyclic Jacobi for a real symmetric n x n matrix (row-major A, destroyed).
 // Eigenvalues ascending in w, eigenvectors in the columns of V (row-major).

@@ -111,6 +111,28 @@ struct GroupOfTerms {   // off-diagonal terms sharing one flip mask
     int t0, t1;
     int lo, hi;          // the two flipped bits when the mask has exactly two, else -1
 };
+// A flip group whose terms test and sign only its own two bits: the summed
+// amplitude is a function of the 2-bit pattern (bit lo) | (bit hi) << 1.
+struct HopDev {
+    int lo, hi;
+    double re[4], im[4];
+};
+// Diagonal terms A * (-1)^(s_i ^ s_{(i+d) mod L}) for every i in `mask`
+// (d = 0: A * (-1)^s_i), summed as A * (n - 2 popcount(mask & (s ^ rotr(s, d)))).
+struct DiagClass {
+    uint64_t mask;
+    int d, n;
+    double re, im;
+};
+// Fast forms of one compiled term list (split_terms output).
+struct FastTerms {
+    bool hop_ok = false;           // every flip group is a two-bit hop
+    std::vector<HopDev> hops;      // one per flip group, same order
+    std::vector<DiagClass> dcls;
+    std::vector<CTerm> drest;      // diagonal terms outside the classes
+};
+FastTerms compile_fast(const std::vector<CTerm> &diag, const std::vector<CTerm> &off,
+                       const std::vector<GroupOfTerms> &groups, int L);
 
 // Basis modes.
 enum BasisMode : int {
@@ -147,6 +169,16 @@ struct BasisHost {
     DevBuf<uint64_t> samp64;
     // binomial table C(n, k) for n, k < 64 (device), combinadic ranking
     DevBuf<uint64_t> binom;
+    // implicit modes, L <= 32: the column states themselves (u32), so rows
+    // need no unranking
+    bool states_stored = false;
+    // stored mode, L <= 32: parent lookup table indexed by the parent rank
+    // (combinadic rank, or the state itself without nup).  Entry = column |
+    // (group element t * npoint + p mapping the state to its orbit minimum)
+    // << ptab_colbits; all ones = the state's orbit is not in the sector.
+    DevBuf<uint32_t> ptab;
+    int ptab_colbits = 32;
+    int64_t npar = 0;            // parent sector size
     // reference labels and normalizations (host, computed on demand)
     std::vector<uint64_t> labels;
     std::vector<double> ref_norm;
@@ -168,6 +200,11 @@ struct OpHost {
     std::vector<GroupOfTerms> groups_adj;
     DevBuf<TermDev> d_diag_adj, d_off_adj;
     DevBuf<GroupOfTerms> d_groups_adj;
+    // fast forms (hop amplitude tables, diagonal classes) of both term lists
+    FastTerms fast, fast_adj;
+    DevBuf<HopDev> d_hops, d_hops_adj;
+    DevBuf<DiagClass> d_dcls, d_dcls_adj;
+    DevBuf<TermDev> d_drest, d_drest_adj;
     // user diagonal
     std::vector<cd> user_diag;
     bool has_user_diag = false;
@@ -177,6 +214,10 @@ struct OpHost {
     int lanes_per_row = 0;       // matvec kernel variant (0 = default heuristic)
     // staging buffers for host vectors
     DevBuf<char> stage_x, stage_y;
+    // basis without symmetry: the extracted diagonal alone (structurally
+    // diagonal terms + user diagonal), which a conjugated product scales by
+    // alpha while the other terms take conj(alpha)
+    std::unique_ptr<OpHost> extracted;
     ~OpHost();
 };
 
@@ -219,6 +260,8 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
                    double bre, double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s,
                    double *dot_partial = nullptr);
 int matvec_grid(int64_t n);
+// the two-bit hop kernel applies to this operator (plain or adjoint list)
+bool hop_eligible(const OpHost &op, bool adjoint);
 // <x, A x> in FP64 with x stored in `dtype` (fused into the matvec epilogue); y is scratch
 cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk);
 

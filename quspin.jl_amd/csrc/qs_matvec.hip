@@ -66,6 +66,11 @@ struct MVArgs {
     double *dot_partial;   // optional: per-block FP64 partials of <x, y>
     // dynamic LDS layout (byte offsets)
     int off_binom, off_grp, off_term, off_red;
+    // hop kernel
+    const uint32_t *ptab;  // parent lookup table (stored mode)
+    int colbits;
+    uint32_t cfull;        // C(L, nup): spin inversion reverses the colex order
+    int ndcls, ndrest;
 };
 
 // Amplitude of one compiled term on state s: A * (-1)^popcount(s & smask),
@@ -412,6 +417,191 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
     if (A.dot_partial) write_dot_partial<kStBlock>(A, s_red, dot_r, dot_i);
 }
 
+// ------------------------------------------------------------ hop kernel --
+// Every flip group is a two-bit hop whose amplitude is a function of its two
+// bits (HopDev), L <= 32 and all indices fit 31 bits.  One lane per row and a
+// wave-uniform loop over the groups (group data in scalar registers).  The
+// neighbour's column comes from 32-bit colex arithmetic alone:
+//  * CM_FULL  -- the column is the rank itself (or the state without nup);
+//  * CM_ZONLY -- canonical = top bit clear; an inverted neighbour's column
+//                is C(L, nup) - 1 - rank, since inversion reverses the
+//                colex order of the half-filled words;
+//  * CM_PTAB  -- the rank indexes the parent lookup table, which holds the
+//                column and the group element reaching the orbit minimum.
+enum { CM_FULL = 0, CM_ZONLY = 1, CM_PTAB = 2 };
+
+// diagonal of state s from the sign classes and the remaining terms
+template <bool CC>
+__device__ __forceinline__ void diag_value(const MVArgs &A, const DiagClass *__restrict__ dcls,
+                                           const TermDev *__restrict__ drest, uint32_t s, double &dr, double &di) {
+    const uint32_t mask = (uint32_t)A.mask;
+    for (int k = 0; k < A.ndcls; k++) {
+        const DiagClass c = dcls[k];
+        const uint32_t xs = c.d ? (s ^ (((s >> c.d) | (s << (A.L - c.d))) & mask)) : s;
+        const double f = (double)(c.n - 2 * popc(xs & (uint32_t)c.mask));
+        dr += f * c.re;
+        if (CC) di += f * c.im;
+    }
+    for (int k = 0; k < A.ndrest; k++) term_amp<uint32_t, CC>(drest[k], s, dr, di);
+}
+
+template <typename V, bool CC, int CM, bool TRACK>
+__device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restrict__ hops,
+                                        const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
+                                        const V *__restrict__ x, V *__restrict__ y, uint32_t row,
+                                        const uint32_t *s_binom, double &dot_r, double &dot_i) {
+    using T = VTraits<V>;
+    constexpr bool VC = T::cplx;
+    const uint32_t *reps = (const uint32_t *)A.reps;
+    const uint32_t mask = (uint32_t)A.mask;
+    const bool pc = A.nup >= 0;
+    uint32_t s, rs;
+    if (CM == CM_PTAB) {
+        s = reps[row];
+        rs = pc ? comb_rank32(s, s_binom) : s;
+    } else {
+        s = reps ? reps[row] : (pc ? comb_unrank<uint32_t, uint32_t>(row, A.L, A.nup, s_binom) : row);
+        rs = row;
+    }
+    double xr, xi;
+    T::load(x, row, xr, xi);
+    if (A.conj_x) xi = -xi;
+    double dr = 0, di = 0;
+    diag_value<CC>(A, dcls, drest, s, dr, di);
+    di = -di;   // conj(amp): the terms are the adjoint couplings
+    if (A.user_diag) {
+        dr += A.user_diag[2 * row];
+        di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+    }
+    double ar, ai = 0;
+    if (VC) {
+        ar = dr * xr - di * xi;
+        ai = dr * xi + di * xr;
+    } else {
+        ar = dr * xr;
+    }
+    double own_scale = 1.0;
+    if (CM == CM_PTAB) own_scale = 1.0 / A.inv_nrm[row];
+    const uint32_t colmask = A.colbits >= 32 ? 0xFFFFFFFFu : ((1u << A.colbits) - 1);
+    for (int g0 = 0; g0 < A.ngroups; g0 += kNB) {
+        uint32_t col[kNB];
+        double mr[kNB], mi[kNB];
+        int elem[kNB];
+#pragma unroll
+        for (int j = 0; j < kNB; j++) {
+            const int gi = g0 + j;
+            col[j] = CM == CM_PTAB ? rs : row;
+            mr[j] = 0;
+            mi[j] = 0;
+            elem[j] = 0;
+            if (gi < A.ngroups) {
+                const HopDev h = hops[gi];
+                const uint32_t blo = (s >> h.lo) & 1, bhi = (s >> h.hi) & 1;
+                double m_r, m_i = 0;
+                bool live;
+                if (pc) {
+                    live = blo != bhi;
+                    m_r = blo ? h.re[1] : h.re[2];
+                    if (CC) m_i = blo ? h.im[1] : h.im[2];
+                } else {
+                    live = true;
+                    m_r = bhi ? (blo ? h.re[3] : h.re[2]) : (blo ? h.re[1] : h.re[0]);
+                    if (CC) m_i = bhi ? (blo ? h.im[3] : h.im[2]) : (blo ? h.im[1] : h.im[0]);
+                }
+                live = live && (m_r != 0.0 || m_i != 0.0);
+                const uint32_t u = s ^ ((1u << h.lo) | (1u << h.hi));
+                uint32_t c;
+                double cr = 1.0;
+                if (pc) {
+                    c = rs + rank_delta32(s, h.lo, h.hi, s_binom);
+                } else {
+                    c = u;
+                }
+                if (CM == CM_ZONLY && ((u >> (A.L - 1)) & 1)) {
+                    c = pc ? A.cfull - 1 - c : (~u & mask);
+                    cr = (double)A.zchar;
+                }
+                if (live) {
+                    col[j] = c;
+                    mr[j] = m_r * cr;
+                    mi[j] = -m_i * cr;
+                }
+            }
+        }
+        double f[kNB];
+        if (CM == CM_PTAB) {
+            uint32_t e[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; j++) e[j] = A.ptab[col[j]];
+#pragma unroll
+            for (int j = 0; j < kNB; j++) {
+                if (e[j] == 0xFFFFFFFFu) {
+                    mr[j] = 0;
+                    mi[j] = 0;
+                    col[j] = row;
+                } else {
+                    col[j] = e[j] & colmask;
+                    elem[j] = TRACK ? (int)(e[j] >> A.colbits) : 0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kNB; j++) f[j] = A.inv_nrm[col[j]];
+        }
+        double vr[kNB], vi[kNB];
+#pragma unroll
+        for (int j = 0; j < kNB; j++) T::load(x, col[j], vr[j], vi[j]);
+#pragma unroll
+        for (int j = 0; j < kNB; j++) {
+            double wr = mr[j], wi = mi[j];
+            if (CM == CM_PTAB) {
+                double cr = 1.0, ci = 0.0;
+                if (TRACK) {
+                    const int bt = elem[j] / A.g.npoint, bp = elem[j] - bt * A.g.npoint;
+                    const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
+                    const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
+                    cr = lr * pr - li * pi;
+                    ci = lr * pi + li * pr;
+                }
+                const double fac = f[j] * own_scale;
+                cr *= fac;
+                ci *= fac;
+                wr = mr[j] * cr - mi[j] * ci;
+                wi = mr[j] * ci + mi[j] * cr;
+            }
+            if (VC) {
+                double v_i = vi[j];
+                if (A.conj_x) v_i = -v_i;
+                ar += wr * vr[j] - wi * v_i;
+                ai += wr * v_i + wi * vr[j];
+            } else {
+                ar += wr * vr[j];
+            }
+        }
+    }
+    finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
+}
+
+template <typename V, bool CC, int CM, bool TRACK>
+__global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev *__restrict__ hops,
+                                                         const DiagClass *__restrict__ dcls,
+                                                         const TermDev *__restrict__ drest, const V *__restrict__ x,
+                                                         V *__restrict__ y) {
+    extern __shared__ __align__(16) char smem[];
+    uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
+    double *s_red = (double *)(smem + A.off_red);
+    if (A.nup >= 0)
+        for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
+    __syncthreads();
+    const int vb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t n = (uint32_t)A.N;
+    double dot_r = 0, dot_i = 0;
+    for (uint32_t base = (uint32_t)vb * kMvBlock; base < n; base += (uint32_t)gridDim.x * kMvBlock) {
+        const uint32_t row = base + threadIdx.x;
+        if (row < n) hop_row<V, CC, CM, TRACK>(A, hops, dcls, drest, x, y, row, s_binom, dot_r, dot_i);
+    }
+    if (A.dot_partial) write_dot_partial<kMvBlock>(A, s_red, dot_r, dot_i);
+}
+
 struct Launch {
     int grid;
     size_t lds;
@@ -419,6 +609,44 @@ struct Launch {
     const GroupOfTerms *grp;
     const TermDev *off, *diag;
 };
+
+struct HopLaunch {
+    const HopDev *hops;
+    const DiagClass *dcls;
+    const TermDev *drest;
+};
+
+template <typename V, bool CC>
+void dispatch_hop_cm(const MVArgs &a, int cm, bool track, const void *x, void *y, const Launch &L,
+                     const HopLaunch &H) {
+    const V *xv = (const V *)x;
+    V *yv = (V *)y;
+#define QS_HOP(CMV, TR)                                                                                            \
+    hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, H.hops, H.dcls, \
+                       H.drest, xv, yv)
+    if (cm == CM_FULL) QS_HOP(CM_FULL, false);
+    else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false);
+    else if (track) QS_HOP(CM_PTAB, true);
+    else QS_HOP(CM_PTAB, false);
+#undef QS_HOP
+}
+
+void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, const void *x, void *y, const Launch &L,
+                  const HopLaunch &H) {
+    switch (dtype) {
+    case QS_F64: dispatch_hop_cm<double, false>(a, cm, track, x, y, L, H); break;
+    case QS_F32: dispatch_hop_cm<float, false>(a, cm, track, x, y, L, H); break;
+    case QS_C128:
+        if (cc) dispatch_hop_cm<cplx_d, true>(a, cm, track, x, y, L, H);
+        else dispatch_hop_cm<cplx_d, false>(a, cm, track, x, y, L, H);
+        break;
+    case QS_C64:
+        if (cc) dispatch_hop_cm<cplx_f, true>(a, cm, track, x, y, L, H);
+        else dispatch_hop_cm<cplx_f, false>(a, cm, track, x, y, L, H);
+        break;
+    default: fail(QS_EARG, "unknown dtype");
+    }
+}
 
 template <typename S, typename V, bool CC, bool TRACK, bool TLDS>
 void dispatch_lpr(const MVArgs &a, int lpr, const void *x, void *y, const Launch &L) {
@@ -488,6 +716,14 @@ int grid_for(int64_t n, int rows_per_block) {
 
 int default_lanes_per_row(const BasisHost &b) { return b.mode == MODE_STORED ? 2 : 1; }
 
+bool hop_eligible(const OpHost &op, bool adjoint) {
+    const BasisHost &b = *op.basis;
+    const FastTerms &f = adjoint ? op.fast_adj : op.fast;
+    if (!f.hop_ok || b.use64 || b.Ns >= ((int64_t)1 << 31) || b.npar > ((int64_t)1 << 31)) return false;
+    if (b.mode == MODE_STORED) return b.ptab.get() != nullptr;
+    return true;
+}
+
 // blocks of a matvec launch that carries FP64 dot partials (one per block):
 // the stored kernel at LPR = 8, and the implicit kernel sized the same way
 int matvec_grid(int64_t n) { return grid_for(n, kStBlock / 8); }
@@ -534,6 +770,34 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         if (b.group.lam[t] != cd(1, 0)) track = true;
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_HOP);
+    if (op.lanes_per_row == QS_VARIANT_HOP && !hop)
+        fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
+    if (hop) {
+        const FastTerms &ft = adjoint ? op.fast_adj : op.fast;
+        a.reps = b.mode == MODE_STORED || b.states_stored ? (const void *)b.reps32.get() : nullptr;
+        a.ptab = b.ptab.get();
+        a.colbits = b.ptab_colbits;
+        a.cfull = (b.nup >= 0) ? (uint32_t)binom(b.L, b.nup) : 0u;
+        a.ndcls = (int)ft.dcls.size();
+        a.ndrest = (int)ft.drest.size();
+        size_t off = 0;
+        auto place = [&](size_t bytes) {
+            size_t o = (off + 15) & ~(size_t)15;
+            off = o + bytes;
+            return (int)o;
+        };
+        a.off_binom = place(b.nup >= 0 ? sizeof(uint32_t) * 33 * 65 : 0);
+        a.off_red = place(dot_partial ? sizeof(double) * 2 * kMvBlock : 0);
+        const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock);
+        const int cm = b.mode == MODE_STORED ? CM_PTAB : (b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL);
+        Launch L{grid, off, s, nullptr, nullptr, nullptr};
+        HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
+                    adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
+        dispatch_hop(a, dtype, cc, cm, track, x, y, L, H);
+        QS_HIP(hipGetLastError());
+        return;
+    }
     int lpr = op.lanes_per_row > 0 ? op.lanes_per_row : default_lanes_per_row(b);
     if (b.mode != MODE_STORED) lpr = 1;
     if (dot_partial) lpr = 8;   // the partial-sum grid is sized by matvec_grid

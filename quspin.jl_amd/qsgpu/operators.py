@@ -230,7 +230,8 @@ class QuantumLinearOperator:
         return y
 
     def set_variant(self, lanes_per_row):
-        """Fix the matvec kernel variant (lanes cooperating on one row; 0 = default)."""
+        """Fix the matvec kernel variant: 0 = automatic, 1/2/4/8 = the general kernel
+        with that many lanes per row, VARIANT_HOP = the two-bit hop kernel."""
         check(_lib.lib().qs_op_set_variant(self._h, int(lanes_per_row)))
         return self
 

@@ -89,20 +89,28 @@ def test_matvec_fp64_matches_oracle(gpu, case):
     assert np.linalg.norm(got64c - want) <= 2e-6 * np.linalg.norm(want)
 
 
-@pytest.mark.parametrize("blocks", [dict(kblock=1), dict(kblock=0), dict(kblock=2, zblock=-1)], ids=str)
-def test_matvec_nonsymmetric_terms_flags_and_diagonal(gpu, blocks):
+@pytest.mark.parametrize("hop", [False, True], ids=["general", "hop"])
+@pytest.mark.parametrize("blocks", [dict(kblock=1), dict(kblock=0), dict(kblock=2, zblock=-1), {},
+                                    dict(zblock=1)], ids=str)
+def test_matvec_nonsymmetric_terms_flags_and_diagonal(gpu, blocks, hop):
     # P'HP of terms that break the symmetry (check_symm=false), a complex
-    # non-Hermitian term list, a user diagonal, alpha/beta, transpose/conj/adjoint
+    # non-Hermitian term list, a user diagonal, alpha/beta, transpose/conj/adjoint.
+    # Without the three-site term every coupling is a two-bit hop, so the
+    # default variant is the hop kernel.
     L, nup = 10, 5
     terms = [("+-", [(0.7 + 0.2j, 1, 2), (0.4, 3, 4)]), ("-+", [(0.3, 2, 5)]), ("z", [(0.9, 1), (-0.2, 7)]),
-             ("zz", [(0.55, 2, 9)]), ("+z-", [(0.25j, 3, 6, 8)])]
-    print("flags", blocks, flush=True)
+             ("zz", [(0.55, 2, 9)])]
+    if not hop:
+        terms.append(("+z-", [(0.25j, 3, 6, 8)]))
+    print("flags", blocks, hop, flush=True)
     ref_s = oracle.Sector(L, nup=nup, pauli=False, **blocks)
     b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
     rng = np.random.default_rng(11)
     diag = rng.standard_normal(b.Ns) + 0.5j * rng.standard_normal(b.Ns)
     ref = oracle.Operator(ref_s, terms, check_symm=False, diagonal=diag)
     op = qsgpu.QuantumLinearOperator(b, terms, diagonal=diag, check_symm=False, check_herm=False)
+    if hop:
+        op.set_variant(qsgpu.VARIANT_HOP)
     x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     y0 = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     for t, c in [(False, False), (True, False), (False, True), (True, True)]:
@@ -232,18 +240,33 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
     terms = _terms(L) + [("+-", [(0.3, i, (i + 4) % L + 1) for i in range(1, L + 1)]),
                          ("-+", [(0.3, i, (i + 4) % L + 1) for i in range(1, L + 1)])]
     if nup is None:
-        terms = terms + [("x", [(0.2, i) for i in range(1, L + 1)])]
+        # pair creation / annihilation: the hop tables' equal-bit patterns
+        terms = terms + [("++", [(0.2, i, i % L + 1) for i in range(1, L + 1)]),
+                         ("--", [(0.2, i, i % L + 1) for i in range(1, L + 1)])]
     ref = oracle.Operator(oracle.Sector(L, nup=nup, pauli=True, **blocks), terms, check_symm=False)
     b = qsgpu.SpinBasis1D(L, nup=nup, pauli=True, **blocks)
     op = qsgpu.QuantumLinearOperator(b, terms, check_symm=False, check_pcon=False)
     rng = np.random.default_rng(5)
     x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     want = ref.matvec(x)
-    for lpr in (1, 2, 4, 8):
+    for lpr in (1, 2, 4, 8, qsgpu.VARIANT_HOP):
         op.set_variant(lpr)
         got = op.matvec(x)
         assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), lpr
+        if not b.is_complex:
+            got_r = op.matvec(x.real.copy())
+            assert np.linalg.norm(got_r - ref.matvec(x.real.copy()).real) <= 1e-12 * np.linalg.norm(want), lpr
+            got32 = op.matvec(x.real.astype(np.float32), dtype=np.float32)
+            assert np.linalg.norm(got32 - ref.matvec(x.real.copy()).real) <= 2e-6 * np.linalg.norm(want), lpr
     lpr, ms = op.tune(np.complex128, repeats=2)
-    assert lpr in (1, 2, 4, 8) and ms > 0
+    assert lpr in (1, 2, 4, 8, qsgpu.VARIANT_HOP) and ms > 0
     got = op.matvec(x)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_hop_variant_rejected_for_other_couplings(gpu):
+    # a three-site coupling is not a two-bit hop: the hop kernel refuses it
+    b = qsgpu.SpinBasis1D(8, nup=4, pauli=False)
+    op = qsgpu.QuantumLinearOperator(b, [("+z-", [(0.5, 1, 2, 3)])], check_symm=False, check_herm=False)
+    with pytest.raises(qsgpu.ArgumentError, match="hop"):
+        op.set_variant(qsgpu.VARIANT_HOP)
