@@ -204,8 +204,8 @@ void setup_terms(OpHost &o, const std::vector<CTerm> &Havg, const double *diag) 
     upload_terms(o.diag_terms_adj, o.d_diag_adj);
     upload_terms(o.off_terms_adj, o.d_off_adj);
     upload_groups(o.groups_adj, o.d_groups_adj);
-    o.fast = compile_fast(o.diag_terms, o.off_terms, o.groups, bb.L);
-    o.fast_adj = compile_fast(o.diag_terms_adj, o.off_terms_adj, o.groups_adj, bb.L);
+    o.fast = compile_fast(o.diag_terms, o.off_terms, o.groups, bb.L, bb.nup);
+    o.fast_adj = compile_fast(o.diag_terms_adj, o.off_terms_adj, o.groups_adj, bb.L, bb.nup);
     upload_fast(o.fast, o.d_hops, o.d_dcls, o.d_drest);
     upload_fast(o.fast_adj, o.d_hops_adj, o.d_dcls_adj, o.d_drest_adj);
     if (o.has_user_diag) {

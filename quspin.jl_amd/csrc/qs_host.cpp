@@ -483,17 +483,22 @@ void split_terms(const std::vector<CTerm> &all, std::vector<CTerm> &diag, std::v
 }
 
 FastTerms compile_fast(const std::vector<CTerm> &diag, const std::vector<CTerm> &off,
-                       const std::vector<GroupOfTerms> &groups, int L)
+                       const std::vector<GroupOfTerms> &groups, int L, int nup)
 {
     FastTerms f;
     // two-bit hop tables: the summed amplitude of a group as a function of
     // its two bits, accumulated in term order as the generic kernel does
-    f.hop_ok = true;
+    f.hop_ok = L <= 32;
     for (const auto &g : groups) {
-        if (g.lo < 0) { f.hop_ok = false; break; }
+        if (g.lo < 0 || !f.hop_ok) { f.hop_ok = false; break; }
         HopDev h{};
         h.lo = g.lo;
         h.hi = g.hi;
+        h.lowmask = (uint32_t)((1ULL << g.lo) - 1);
+        h.between = (uint32_t)(((1ULL << g.hi) - 1) & ~((2ULL << g.lo) - 1));
+        h.lo65 = g.lo * 65;
+        h.hi65 = g.hi * 65;
+        h.top = g.hi == L - 1;
         for (int k = g.t0; k < g.t1; k++)
             if ((off[k].cmask | off[k].smask) & ~g.flip) f.hop_ok = false;
         if (!f.hop_ok) break;
@@ -509,7 +514,11 @@ FastTerms compile_fast(const std::vector<CTerm> &diag, const std::vector<CTerm> 
             }
             h.re[pat] = re;
             h.im[pat] = im;
+            // with a particle number, equal bits leave the sector
+            const bool in_sector = nup < 0 || pat == 1 || pat == 2;
+            if (in_sector && (re != 0.0 || im != 0.0)) h.nzmask |= 1 << pat;
         }
+        h.sym = h.re[1] == h.re[2] && h.im[1] == h.im[2];
         f.hops.push_back(h);
     }
     if (!f.hop_ok) f.hops.clear();

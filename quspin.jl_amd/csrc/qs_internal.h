@@ -115,6 +115,13 @@ struct GroupOfTerms {   // off-diagonal terms sharing one flip mask
 // amplitude is a function of the 2-bit pattern (bit lo) | (bit hi) << 1.
 struct HopDev {
     int lo, hi;
+    uint32_t lowmask;    // bits below lo
+    uint32_t between;    // bits strictly between lo and hi
+    int lo65, hi65;      // binomial-table rows of lo and hi
+    int nzmask;          // patterns with a nonzero amplitude (inside the particle sector)
+    int sym;             // patterns 1 and 2 share one amplitude
+    int top;             // hi is bit L-1
+    int pad;
     double re[4], im[4];
 };
 // Diagonal terms A * (-1)^(s_i ^ s_{(i+d) mod L}) for every i in `mask`
@@ -132,7 +139,7 @@ struct FastTerms {
     std::vector<CTerm> drest;      // diagonal terms outside the classes
 };
 FastTerms compile_fast(const std::vector<CTerm> &diag, const std::vector<CTerm> &off,
-                       const std::vector<GroupOfTerms> &groups, int L);
+                       const std::vector<GroupOfTerms> &groups, int L, int nup);
 
 // Basis modes.
 enum BasisMode : int {

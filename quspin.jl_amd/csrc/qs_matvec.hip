@@ -445,7 +445,7 @@ __device__ __forceinline__ void diag_value(const MVArgs &A, const DiagClass *__r
     for (int k = 0; k < A.ndrest; k++) term_amp<uint32_t, CC>(drest[k], s, dr, di);
 }
 
-template <typename V, bool CC, int CM, bool TRACK>
+template <typename V, bool CC, int CM, bool TRACK, bool PC>
 __device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restrict__ hops,
                                         const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
                                         const V *__restrict__ x, V *__restrict__ y, uint32_t row,
@@ -454,13 +454,12 @@ __device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restric
     constexpr bool VC = T::cplx;
     const uint32_t *reps = (const uint32_t *)A.reps;
     const uint32_t mask = (uint32_t)A.mask;
-    const bool pc = A.nup >= 0;
     uint32_t s, rs;
     if (CM == CM_PTAB) {
         s = reps[row];
-        rs = pc ? comb_rank32(s, s_binom) : s;
+        rs = PC ? comb_rank32(s, s_binom) : s;
     } else {
-        s = reps ? reps[row] : (pc ? comb_unrank<uint32_t, uint32_t>(row, A.L, A.nup, s_binom) : row);
+        s = reps ? reps[row] : (PC ? comb_unrank<uint32_t, uint32_t>(row, A.L, A.nup, s_binom) : row);
         rs = row;
     }
     double xr, xi;
@@ -495,37 +494,55 @@ __device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restric
             mi[j] = 0;
             elem[j] = 0;
             if (gi < A.ngroups) {
+                // group constants are wave-uniform: scalar loads and branches
                 const HopDev h = hops[gi];
                 const uint32_t blo = (s >> h.lo) & 1, bhi = (s >> h.hi) & 1;
+                const bool live = (h.nzmask >> (blo | (bhi << 1))) & 1;
                 double m_r, m_i = 0;
-                bool live;
-                if (pc) {
-                    live = blo != bhi;
-                    m_r = blo ? h.re[1] : h.re[2];
-                    if (CC) m_i = blo ? h.im[1] : h.im[2];
+                if (PC) {
+                    if (h.sym) {
+                        m_r = h.re[1];
+                        if (CC) m_i = h.im[1];
+                    } else {
+                        m_r = blo ? h.re[1] : h.re[2];
+                        if (CC) m_i = blo ? h.im[1] : h.im[2];
+                    }
                 } else {
-                    live = true;
                     m_r = bhi ? (blo ? h.re[3] : h.re[2]) : (blo ? h.re[1] : h.re[0]);
                     if (CC) m_i = bhi ? (blo ? h.im[3] : h.im[2]) : (blo ? h.im[1] : h.im[0]);
                 }
-                live = live && (m_r != 0.0 || m_i != 0.0);
-                const uint32_t u = s ^ ((1u << h.lo) | (1u << h.hi));
                 uint32_t c;
-                double cr = 1.0;
-                if (pc) {
-                    c = rs + rank_delta32(s, h.lo, h.hi, s_binom);
+                if (PC) {
+                    // colex rank change of the hop (bits lo and hi differ)
+                    const int cl = popc(s & h.lowmask);
+                    uint32_t bw = s & h.between;
+                    uint32_t d = s_binom[h.hi65 + cl + popc(bw) + 1] - s_binom[h.lo65 + cl + 1];
+                    if (h.between) {
+                        int l = 1;
+                        while (bw) {
+                            const int q = ctz(bw);
+                            d += s_binom[q * 65 + cl + l] - s_binom[q * 65 + cl + l + 1];
+                            l++;
+                            bw &= bw - 1;
+                        }
+                    }
+                    c = rs + (blo ? d : 0u - d);
                 } else {
-                    c = u;
+                    c = s ^ ((1u << h.lo) | (1u << h.hi));
                 }
-                if (CM == CM_ZONLY && ((u >> (A.L - 1)) & 1)) {
-                    c = pc ? A.cfull - 1 - c : (~u & mask);
-                    cr = (double)A.zchar;
+                if (CM == CM_ZONLY && h.top) {
+                    // s has bit L-1 clear, so the hop sets it: the column is
+                    // the inverted state's, with the inversion character
+                    c = PC ? A.cfull - 1 - c : (~c & mask);
+                    if (A.zchar < 0) {
+                        m_r = -m_r;
+                        m_i = -m_i;
+                    }
                 }
-                if (live) {
-                    col[j] = c;
-                    mr[j] = m_r * cr;
-                    mi[j] = -m_i * cr;
-                }
+                // branch-free: dead neighbours keep col = own row, weight 0
+                col[j] = live ? c : col[j];
+                mr[j] = live ? m_r : 0.0;
+                mi[j] = live ? -m_i : 0.0;
             }
         }
         double f[kNB];
@@ -581,7 +598,7 @@ __device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restric
     finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
 }
 
-template <typename V, bool CC, int CM, bool TRACK>
+template <typename V, bool CC, int CM, bool TRACK, bool PC>
 __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev *__restrict__ hops,
                                                          const DiagClass *__restrict__ dcls,
                                                          const TermDev *__restrict__ drest, const V *__restrict__ x,
@@ -589,7 +606,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev 
     extern __shared__ __align__(16) char smem[];
     uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
     double *s_red = (double *)(smem + A.off_red);
-    if (A.nup >= 0)
+    if (PC)
         for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
     __syncthreads();
     const int vb = xcd_block(blockIdx.x, gridDim.x);
@@ -597,7 +614,7 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev 
     double dot_r = 0, dot_i = 0;
     for (uint32_t base = (uint32_t)vb * kMvBlock; base < n; base += (uint32_t)gridDim.x * kMvBlock) {
         const uint32_t row = base + threadIdx.x;
-        if (row < n) hop_row<V, CC, CM, TRACK>(A, hops, dcls, drest, x, y, row, s_binom, dot_r, dot_i);
+        if (row < n) hop_row<V, CC, CM, TRACK, PC>(A, hops, dcls, drest, x, y, row, s_binom, dot_r, dot_i);
     }
     if (A.dot_partial) write_dot_partial<kMvBlock>(A, s_red, dot_r, dot_i);
 }
@@ -621,13 +638,20 @@ void dispatch_hop_cm(const MVArgs &a, int cm, bool track, const void *x, void *y
                      const HopLaunch &H) {
     const V *xv = (const V *)x;
     V *yv = (V *)y;
-#define QS_HOP(CMV, TR)                                                                                            \
-    hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, H.hops, H.dcls, \
-                       H.drest, xv, yv)
-    if (cm == CM_FULL) QS_HOP(CM_FULL, false);
-    else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false);
-    else if (track) QS_HOP(CM_PTAB, true);
-    else QS_HOP(CM_PTAB, false);
+#define QS_HOP(CMV, TR, P)                                                                                      \
+    hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, H.hops,     \
+                       H.dcls, H.drest, xv, yv)
+    if (a.nup >= 0) {
+        if (cm == CM_FULL) QS_HOP(CM_FULL, false, true);
+        else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false, true);
+        else if (track) QS_HOP(CM_PTAB, true, true);
+        else QS_HOP(CM_PTAB, false, true);
+    } else {
+        if (cm == CM_FULL) QS_HOP(CM_FULL, false, false);
+        else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false, false);
+        else if (track) QS_HOP(CM_PTAB, true, false);
+        else QS_HOP(CM_PTAB, false, false);
+    }
 #undef QS_HOP
 }
 
