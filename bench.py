@@ -8,7 +8,7 @@ algorithmic bytes per matvec / time, with
     bytes = N * (8 read x + 8 write y + 4 stored state + 8 stored 1/sqrt(D)).
 Beside it: the FP64 Lanczos ground state (k=1, tol 1e-10) of the same
 sector, the L=32 z+ FP32-stream / FP64 matvec pair (N = 300,540,195,
-implicit basis) and the CPU baseline: the oracle's restatement of QuSpin.jl's
+implicit basis; bytes = N * (2 * element + 4 stored state)) and the CPU baseline: the oracle's restatement of QuSpin.jl's
 parent-space apply loop (operators.jl:2554-2589), single-threaded, timed on a
 bounded sample of parent states and extrapolated to the whole sector.
 
@@ -37,6 +37,17 @@ def xxz_terms(L, J=1.0, delta=1.0, J2=0.0):
         terms += [("+-", [(J2 / 2, i, j) for i, j in b2]), ("-+", [(J2 / 2, i, j) for i, j in b2]),
                   ("zz", [(J2 * delta, i, j) for i, j in b2])]
     return terms
+
+
+def profiled_traffic(key):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE
+    and WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[key]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def time_matvec(op, x, y, steps, warmup):
@@ -115,9 +126,11 @@ def main():
                    "bytes_per_matvec": bytes_alg, "basis_build_s": round(t_basis, 4), "device": dev["name"],
                    "lanes_per_row": lpr},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 3), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbps / HBM_PEAK_GBPS, 5), "traffic": None,
-                     "kernel": "k_matvec<uint32,double,real,STORED>",
-                     "note": "canonicalisation over the 112-element group dominates; integer-ALU bound at this N"},
+                     "frac": round(gbps / HBM_PEAK_GBPS, 5), "traffic": profiled_traffic("c2_fp64"),
+                     "kernel": "k_matvec_hop<double, real, parent table, variant %d>" % lpr,
+                     "note": "per neighbour: colex rank update, parent-table gather, x and 1/sqrt(D) gathers; "
+                             "the whole working set sits in L2/MALL at this N, so the bound is issue and "
+                             "gather latency, not HBM"},
     }
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -145,7 +158,11 @@ def main():
     if not args.no_c3:
         print("bench: C3 L=32 z+ matvec pair", file=sys.stderr, flush=True)
         b3 = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
-        op3 = qsgpu.QuantumLinearOperator(b3, xxz_terms(32, J=1.0, delta=1.0))
+        # disordered fields h_i ~ U[-2, 2]: odd under spin inversion, so the
+        # sector's group average P'HP cancels them exactly (check_symm=false)
+        h = np.random.default_rng(0xC3).uniform(-2.0, 2.0, 32)
+        terms3 = xxz_terms(32, J=1.0, delta=1.0) + [("z", [(float(h[i]), i + 1) for i in range(32)])]
+        op3 = qsgpu.QuantumLinearOperator(b3, terms3, check_symm=False)
         n3 = b3.Ns
         out = {"sector_dim": n3}
         for dt, name, esz in ((torch.float32, "fp32", 4), (torch.float64, "fp64", 8)):
@@ -153,8 +170,10 @@ def main():
             y3 = torch.zeros_like(x3)
             lpr3, _ = op3.tune(np.float32 if esz == 4 else np.float64, repeats=2)
             t3, _ = time_matvec(op3, x3, y3, max(3, args.steps // 10), 2)
-            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(n3 * 2 * esz / t3 / 1e9, 2), "lanes_per_row": lpr3,
-                         "frac": round(n3 * 2 * esz / t3 / 1e9 / HBM_PEAK_GBPS, 4)}
+            b3_alg = n3 * (2 * esz + 4)   # x, y and the stored u32 state per row
+            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(b3_alg / t3 / 1e9, 2), "lanes_per_row": lpr3,
+                         "frac": round(b3_alg / t3 / 1e9 / HBM_PEAK_GBPS, 4), "bytes_per_matvec": b3_alg,
+                         "traffic": profiled_traffic("c3_" + name)}
             del x3, y3
             torch.cuda.empty_cache()
         out["fp32_over_fp64"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)

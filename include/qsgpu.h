@@ -129,9 +129,10 @@ int qs_op_set_stream(qs_op *op, void *stream);
 int qs_op_stream(const qs_op *op, void **stream);
 
 /* Matvec kernel variant: 0 = automatic; 1, 2, 4 or 8 = the general kernel
- * with that many lanes cooperating on one row; QS_VARIANT_HOP = the two-bit
- * hop kernel (couplings that flip two sites and test only those two, L <= 32;
- * symmetry sectors also need the parent lookup table).  No reference
+ * with that many lanes cooperating on one row; QS_VARIANT_HOP + k (k = 0, 1,
+ * 2) = the two-bit hop kernel (couplings that flip two sites and test only
+ * those two, L <= 32; symmetry sectors also need the parent lookup table)
+ * with 2^k wavefronts sharing each row tile's couplings.  No reference
  * counterpart: the reference has one CPU loop. */
 #define QS_VARIANT_HOP 16
 int qs_op_set_variant(qs_op *op, int lanes_per_row);

@@ -445,16 +445,11 @@ __device__ __forceinline__ void diag_value(const MVArgs &A, const DiagClass *__r
     for (int k = 0; k < A.ndrest; k++) term_amp<uint32_t, CC>(drest[k], s, dr, di);
 }
 
-template <typename V, bool CC, int CM, bool TRACK, bool PC>
-__device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restrict__ hops,
-                                        const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
-                                        const V *__restrict__ x, V *__restrict__ y, uint32_t row,
-                                        const uint32_t *s_binom, double &dot_r, double &dot_i) {
-    using T = VTraits<V>;
-    constexpr bool VC = T::cplx;
+// Own state of a row: the state, its parent rank (CM_PTAB) or row index.
+template <int CM, bool PC>
+__device__ __forceinline__ void hop_state(const MVArgs &A, uint32_t row, const uint32_t *s_binom, uint32_t &s,
+                                          uint32_t &rs) {
     const uint32_t *reps = (const uint32_t *)A.reps;
-    const uint32_t mask = (uint32_t)A.mask;
-    uint32_t s, rs;
     if (CM == CM_PTAB) {
         s = reps[row];
         rs = PC ? comb_rank32(s, s_binom) : s;
@@ -462,33 +457,27 @@ __device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restric
         s = reps ? reps[row] : (PC ? comb_unrank<uint32_t, uint32_t>(row, A.L, A.nup, s_binom) : row);
         rs = row;
     }
-    double xr, xi;
-    T::load(x, row, xr, xi);
-    if (A.conj_x) xi = -xi;
-    double dr = 0, di = 0;
-    diag_value<CC>(A, dcls, drest, s, dr, di);
-    di = -di;   // conj(amp): the terms are the adjoint couplings
-    if (A.user_diag) {
-        dr += A.user_diag[2 * row];
-        di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
-    }
-    double ar, ai = 0;
-    if (VC) {
-        ar = dr * xr - di * xi;
-        ai = dr * xi + di * xr;
-    } else {
-        ar = dr * xr;
-    }
+}
+
+// Adds the flip groups g = g_first, g_first + WS, ... of row `row` (own
+// state s, rank rs) to (ar, ai), kNB neighbours per batch.
+template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
+__device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__restrict__ hops,
+                                           const V *__restrict__ x, uint32_t row, uint32_t s, uint32_t rs,
+                                           int g_first, const uint32_t *s_binom, double &ar, double &ai) {
+    using T = VTraits<V>;
+    constexpr bool VC = T::cplx;
+    const uint32_t mask = (uint32_t)A.mask;
     double own_scale = 1.0;
     if (CM == CM_PTAB) own_scale = 1.0 / A.inv_nrm[row];
     const uint32_t colmask = A.colbits >= 32 ? 0xFFFFFFFFu : ((1u << A.colbits) - 1);
-    for (int g0 = 0; g0 < A.ngroups; g0 += kNB) {
+    for (int g0 = g_first; g0 < A.ngroups; g0 += kNB * WS) {
         uint32_t col[kNB];
         double mr[kNB], mi[kNB];
         int elem[kNB];
 #pragma unroll
         for (int j = 0; j < kNB; j++) {
-            const int gi = g0 + j;
+            const int gi = g0 + j * WS;
             col[j] = CM == CM_PTAB ? rs : row;
             mr[j] = 0;
             mi[j] = 0;
@@ -595,26 +584,70 @@ __device__ __forceinline__ void hop_row(const MVArgs &A, const HopDev *__restric
             }
         }
     }
-    finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
 }
 
-template <typename V, bool CC, int CM, bool TRACK, bool PC>
+// Block of kMvBlock threads = kMvBlock / 64 waves.  WS waves share each
+// 64-row tile: wave `sub` takes the flip groups g = sub (mod WS) (so group
+// data stays wave-uniform) and the partial sums meet in LDS in a fixed order.
+template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
 __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev *__restrict__ hops,
                                                          const DiagClass *__restrict__ dcls,
                                                          const TermDev *__restrict__ drest, const V *__restrict__ x,
                                                          V *__restrict__ y) {
+    using T = VTraits<V>;
+    constexpr bool VC = T::cplx;
+    constexpr int kRows = kMvBlock / WS;
     extern __shared__ __align__(16) char smem[];
     uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
     double *s_red = (double *)(smem + A.off_red);
+    __shared__ double s_part[WS > 1 ? 2 * kMvBlock : 1];
     if (PC)
         for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
     __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sub = wave % WS, rw = wave / WS;
     const int vb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t n = (uint32_t)A.N;
     double dot_r = 0, dot_i = 0;
-    for (uint32_t base = (uint32_t)vb * kMvBlock; base < n; base += (uint32_t)gridDim.x * kMvBlock) {
-        const uint32_t row = base + threadIdx.x;
-        if (row < n) hop_row<V, CC, CM, TRACK, PC>(A, hops, dcls, drest, x, y, row, s_binom, dot_r, dot_i);
+    for (uint32_t base = (uint32_t)vb * kRows; base < n; base += (uint32_t)gridDim.x * kRows) {
+        const uint32_t row = base + rw * 64 + lane;
+        double ar = 0, ai = 0, xr = 0, xi = 0;
+        if (row < n) {
+            uint32_t s, rs;
+            hop_state<CM, PC>(A, row, s_binom, s, rs);
+            if (sub == 0) {
+                T::load(x, row, xr, xi);
+                if (A.conj_x) xi = -xi;
+                double dr = 0, di = 0;
+                diag_value<CC>(A, dcls, drest, s, dr, di);
+                di = -di;   // conj(amp): the terms are the adjoint couplings
+                if (A.user_diag) {
+                    dr += A.user_diag[2 * row];
+                    di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+                }
+                if (VC) {
+                    ar = dr * xr - di * xi;
+                    ai = dr * xi + di * xr;
+                } else {
+                    ar = dr * xr;
+                }
+            }
+            hop_groups<V, CC, CM, TRACK, PC, WS>(A, hops, x, row, s, rs, sub, s_binom, ar, ai);
+        }
+        if (WS > 1) {
+            s_part[threadIdx.x] = ar;
+            s_part[kMvBlock + threadIdx.x] = ai;
+            __syncthreads();
+            if (sub == 0) {
+#pragma unroll
+                for (int k = 1; k < WS; k++) {
+                    ar += s_part[threadIdx.x + 64 * k];
+                    ai += s_part[kMvBlock + threadIdx.x + 64 * k];
+                }
+            }
+            __syncthreads();
+        }
+        if (sub == 0 && row < n) finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
     }
     if (A.dot_partial) write_dot_partial<kMvBlock>(A, s_red, dot_r, dot_i);
 }
@@ -634,39 +667,45 @@ struct HopLaunch {
 };
 
 template <typename V, bool CC>
-void dispatch_hop_cm(const MVArgs &a, int cm, bool track, const void *x, void *y, const Launch &L,
+void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x, void *y, const Launch &L,
                      const HopLaunch &H) {
     const V *xv = (const V *)x;
     V *yv = (V *)y;
 #define QS_HOP(CMV, TR, P)                                                                                      \
-    hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, H.hops,     \
-                       H.dcls, H.drest, xv, yv)
+    switch (ws) {                                                                                                  \
+    case 1: hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, 1>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a,  \
+                               H.hops, H.dcls, H.drest, xv, yv); break;                                            \
+    case 2: hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, 2>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a,  \
+                               H.hops, H.dcls, H.drest, xv, yv); break;                                            \
+    default: hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, 4>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, \
+                                H.hops, H.dcls, H.drest, xv, yv); break;                                           \
+    }
     if (a.nup >= 0) {
-        if (cm == CM_FULL) QS_HOP(CM_FULL, false, true);
-        else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false, true);
-        else if (track) QS_HOP(CM_PTAB, true, true);
-        else QS_HOP(CM_PTAB, false, true);
+        if (cm == CM_FULL) QS_HOP(CM_FULL, false, true)
+        else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false, true)
+        else if (track) QS_HOP(CM_PTAB, true, true)
+        else QS_HOP(CM_PTAB, false, true)
     } else {
-        if (cm == CM_FULL) QS_HOP(CM_FULL, false, false);
-        else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false, false);
-        else if (track) QS_HOP(CM_PTAB, true, false);
-        else QS_HOP(CM_PTAB, false, false);
+        if (cm == CM_FULL) QS_HOP(CM_FULL, false, false)
+        else if (cm == CM_ZONLY) QS_HOP(CM_ZONLY, false, false)
+        else if (track) QS_HOP(CM_PTAB, true, false)
+        else QS_HOP(CM_PTAB, false, false)
     }
 #undef QS_HOP
 }
 
-void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, const void *x, void *y, const Launch &L,
-                  const HopLaunch &H) {
+void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, int ws, const void *x, void *y,
+                  const Launch &L, const HopLaunch &H) {
     switch (dtype) {
-    case QS_F64: dispatch_hop_cm<double, false>(a, cm, track, x, y, L, H); break;
-    case QS_F32: dispatch_hop_cm<float, false>(a, cm, track, x, y, L, H); break;
+    case QS_F64: dispatch_hop_cm<double, false>(a, cm, track, ws, x, y, L, H); break;
+    case QS_F32: dispatch_hop_cm<float, false>(a, cm, track, ws, x, y, L, H); break;
     case QS_C128:
-        if (cc) dispatch_hop_cm<cplx_d, true>(a, cm, track, x, y, L, H);
-        else dispatch_hop_cm<cplx_d, false>(a, cm, track, x, y, L, H);
+        if (cc) dispatch_hop_cm<cplx_d, true>(a, cm, track, ws, x, y, L, H);
+        else dispatch_hop_cm<cplx_d, false>(a, cm, track, ws, x, y, L, H);
         break;
     case QS_C64:
-        if (cc) dispatch_hop_cm<cplx_f, true>(a, cm, track, x, y, L, H);
-        else dispatch_hop_cm<cplx_f, false>(a, cm, track, x, y, L, H);
+        if (cc) dispatch_hop_cm<cplx_f, true>(a, cm, track, ws, x, y, L, H);
+        else dispatch_hop_cm<cplx_f, false>(a, cm, track, ws, x, y, L, H);
         break;
     default: fail(QS_EARG, "unknown dtype");
     }
@@ -794,8 +833,9 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         if (b.group.lam[t] != cd(1, 0)) track = true;
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
-    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_HOP);
-    if (op.lanes_per_row == QS_VARIANT_HOP && !hop)
+    const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP;
+    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || want_hop);
+    if (want_hop && !hop)
         fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
     if (hop) {
         const FastTerms &ft = adjoint ? op.fast_adj : op.fast;
@@ -813,12 +853,14 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         };
         a.off_binom = place(b.nup >= 0 ? sizeof(uint32_t) * 33 * 65 : 0);
         a.off_red = place(dot_partial ? sizeof(double) * 2 * kMvBlock : 0);
-        const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock);
+        // waves sharing one row tile: QS_VARIANT_HOP + k selects 2^k
+        const int ws = want_hop ? (1 << (op.lanes_per_row - QS_VARIANT_HOP)) : 1;
+        const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock / ws);
         const int cm = b.mode == MODE_STORED ? CM_PTAB : (b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL);
         Launch L{grid, off, s, nullptr, nullptr, nullptr};
         HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                     adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
-        dispatch_hop(a, dtype, cc, cm, track, x, y, L, H);
+        dispatch_hop(a, dtype, cc, cm, track, ws, x, y, L, H);
         QS_HIP(hipGetLastError());
         return;
     }

@@ -110,7 +110,7 @@ def test_matvec_nonsymmetric_terms_flags_and_diagonal(gpu, blocks, hop):
     ref = oracle.Operator(ref_s, terms, check_symm=False, diagonal=diag)
     op = qsgpu.QuantumLinearOperator(b, terms, diagonal=diag, check_symm=False, check_herm=False)
     if hop:
-        op.set_variant(qsgpu.VARIANT_HOP)
+        op.set_variant(qsgpu.VARIANT_HOP + 1)
     x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     y0 = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     for t, c in [(False, False), (True, False), (False, True), (True, True)]:
@@ -249,7 +249,8 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
     rng = np.random.default_rng(5)
     x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     want = ref.matvec(x)
-    for lpr in (1, 2, 4, 8, qsgpu.VARIANT_HOP):
+    hops = (qsgpu.VARIANT_HOP, qsgpu.VARIANT_HOP + 1, qsgpu.VARIANT_HOP + 2)
+    for lpr in (1, 2, 4, 8) + hops:
         op.set_variant(lpr)
         got = op.matvec(x)
         assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), lpr
@@ -259,7 +260,7 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
             got32 = op.matvec(x.real.astype(np.float32), dtype=np.float32)
             assert np.linalg.norm(got32 - ref.matvec(x.real.copy()).real) <= 2e-6 * np.linalg.norm(want), lpr
     lpr, ms = op.tune(np.complex128, repeats=2)
-    assert lpr in (1, 2, 4, 8, qsgpu.VARIANT_HOP) and ms > 0
+    assert lpr in (1, 2, 4, 8) + hops and ms > 0
     got = op.matvec(x)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
