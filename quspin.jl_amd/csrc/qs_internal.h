@@ -272,6 +272,44 @@ bool hop_eligible(const OpHost &op, bool adjoint);
 // <x, A x> in FP64 with x stored in `dtype` (fused into the matvec epilogue); y is scratch
 cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk);
 
+// ------------------------------------------------------ batched launches --
+// One launch carrying several jobs: each job owns a contiguous range of the
+// grid's blocks and sees exactly the block partition of its own single
+// launch, so its results are bitwise those of the single launch.
+enum VecOp { VOP_GRAM = 0, VOP_REDUCE, VOP_UPDATE, VOP_NORM2, VOP_AXPBY, VOP_COMBINE };
+struct VecJob {
+    const void *V;      // basis block (column-major, ld n)
+    const void *x;      // input vector
+    void *w;            // in/out vector (update, axpby, combine)
+    double *partial;    // per-block partial sums
+    double *aux;        // reduce: results; update: coefficients h; combine: C
+    int64_t n;
+    int m, nb, b0, k, b_zero;
+    double ar, ai, br, bi;
+};
+constexpr int kMaxVecJobs = 24;
+struct VecJobs {
+    int njobs;
+    VecJob job[kMaxVecJobs];
+};
+// Launch `op` over jobs (b0 and the grid are filled in here); jobs beyond
+// kMaxVecJobs go to further launches on the same stream.
+void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s);
+int ew_blocks(int64_t n);
+
+// Batched matvec: y_j = M_j x_j for several operators in one launch (all
+// must take the hop kernel with one wave per row tile).
+struct MatvecJob {
+    const OpHost *op;
+    const void *x;
+    void *y;
+};
+struct BatchMatvec;   // device copies of the operators' kernel arguments
+std::unique_ptr<BatchMatvec, void (*)(BatchMatvec *)> make_batch_matvec(const std::vector<const OpHost *> &ops,
+                                                                         int dtype);
+void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hipStream_t s);
+bool batch_matvec_ok(const std::vector<const OpHost *> &ops, int dtype);
+
 // vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
 struct VecWork {
     DevBuf<double> partial;   // per-block partial sums

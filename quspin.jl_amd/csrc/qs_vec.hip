@@ -45,20 +45,22 @@ __device__ __forceinline__ void block_reduce2(double &re, double &im, double *sh
     __syncthreads();
 }
 
-__device__ __forceinline__ void row_range(int64_t n, int64_t &r0, int64_t &r1) {
-    int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    r0 = (int64_t)blockIdx.x * chunk;
+// Block b of nb owns one contiguous row range.  Kernel bodies take (b, nb)
+// explicitly so a batched launch (many jobs, one grid) gives every job the
+// partition -- and therefore the sums -- of its own single launch.
+__device__ __forceinline__ void row_range(int64_t n, int b, int nb, int64_t &r0, int64_t &r1) {
+    int64_t chunk = (n + nb - 1) / nb;
+    r0 = (int64_t)b * chunk;
     r1 = r0 + chunk < n ? r0 + chunk : n;
 }
 
 // partial[(j * nb + b) * 2 + {0,1}] = sum over block b rows of conj(V[r, j]) * w[r]
 template <typename V>
-__global__ __launch_bounds__(kVB) void k_gram(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
-                                              double *partial) {
+__device__ __forceinline__ void gram_body(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
+                                          double *partial, int b, int nb, double *sh) {
     using T = VTraits<V>;
-    __shared__ double sh[2 * kVB];
     int64_t r0, r1;
-    row_range(n, r0, r1);
+    row_range(n, b, nb, r0, r1);
     for (int j0 = 0; j0 < m; j0 += kCols) {
         double acc_r[kCols], acc_i[kCols];
 #pragma unroll
@@ -81,17 +83,15 @@ __global__ __launch_bounds__(kVB) void k_gram(const V *__restrict__ Vb, int64_t 
             double re = acc_r[c], im = acc_i[c];
             block_reduce2(re, im, sh);
             if (threadIdx.x == 0) {
-                partial[((int64_t)(j0 + c) * gridDim.x + blockIdx.x) * 2] = re;
-                partial[((int64_t)(j0 + c) * gridDim.x + blockIdx.x) * 2 + 1] = im;
+                partial[((int64_t)(j0 + c) * nb + b) * 2] = re;
+                partial[((int64_t)(j0 + c) * nb + b) * 2 + 1] = im;
             }
         }
     }
 }
 
 // out[j] = sum_b partial[j][b] in a fixed tree (one block per j)
-__global__ __launch_bounds__(kVB) void k_reduce(const double *partial, int nb, double *out) {
-    __shared__ double sh[2 * kVB];
-    int j = blockIdx.x;
+__device__ __forceinline__ void reduce_body(const double *partial, int nb, double *out, int j, double *sh) {
     double re = 0, im = 0;
     for (int b = threadIdx.x; b < nb; b += kVB) {
         re += partial[((int64_t)j * nb + b) * 2];
@@ -106,12 +106,11 @@ __global__ __launch_bounds__(kVB) void k_reduce(const double *partial, int nb, d
 
 // w[r] -= sum_j V[r, j] h[j]; partial[b] = sum |w[r]|^2 (of the stored values)
 template <typename V>
-__global__ __launch_bounds__(kVB) void k_update(const V *__restrict__ Vb, int64_t n, int m, const double *h,
-                                                V *__restrict__ w, double *partial) {
+__device__ __forceinline__ void update_body(const V *__restrict__ Vb, int64_t n, int m, const double *h,
+                                            V *__restrict__ w, double *partial, int b, int nb, double *sh) {
     using T = VTraits<V>;
-    __shared__ double sh[2 * kVB];
     int64_t r0, r1;
-    row_range(n, r0, r1);
+    row_range(n, b, nb, r0, r1);
     double nrm = 0;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
         double wr, wi;
@@ -130,36 +129,36 @@ __global__ __launch_bounds__(kVB) void k_update(const V *__restrict__ Vb, int64_
     double dummy = 0;
     block_reduce2(nrm, dummy, sh);
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = nrm;
-        partial[2 * blockIdx.x + 1] = 0;
+        partial[2 * b] = nrm;
+        partial[2 * b + 1] = 0;
     }
 }
 
 template <typename V>
-__global__ __launch_bounds__(kVB) void k_norm2(const V *__restrict__ x, int64_t n, double *partial) {
+__device__ __forceinline__ void norm2_body(const V *__restrict__ x, int64_t n, double *partial, int b, int nb,
+                                           double *sh) {
     using T = VTraits<V>;
-    __shared__ double sh[2 * kVB];
     int64_t r0, r1;
-    row_range(n, r0, r1);
+    row_range(n, b, nb, r0, r1);
     double nrm = 0;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
-        double a, b;
-        T::load(x, r, a, b);
-        nrm += a * a + b * b;
+        double a, c;
+        T::load(x, r, a, c);
+        nrm += a * a + c * c;
     }
     double dummy = 0;
     block_reduce2(nrm, dummy, sh);
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = nrm;
-        partial[2 * blockIdx.x + 1] = 0;
+        partial[2 * b] = nrm;
+        partial[2 * b + 1] = 0;
     }
 }
 
 template <typename V>
-__global__ void k_axpby(double ar, double ai, const V *__restrict__ x, double br, double bi, V *__restrict__ y,
-                        int64_t n, int b_zero) {
+__device__ __forceinline__ void axpby_body(double ar, double ai, const V *__restrict__ x, double br, double bi,
+                                           V *__restrict__ y, int64_t n, int b_zero, int b, int nb) {
     using T = VTraits<V>;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t r = (int64_t)b * blockDim.x + threadIdx.x; r < n; r += (int64_t)nb * blockDim.x) {
         double xr, xi, yr = 0, yi = 0;
         T::load(x, r, xr, xi);
         if (!b_zero) T::load(y, r, yr, yi);
@@ -171,9 +170,10 @@ __global__ void k_axpby(double ar, double ai, const V *__restrict__ x, double br
 
 // Y[:, i] = sum_j V[:, j] C[j + i*m]   (C interleaved complex, column-major m x k)
 template <typename V>
-__global__ void k_combine(const V *__restrict__ Vb, int64_t n, int m, const double *C, int k, V *__restrict__ Y) {
+__device__ __forceinline__ void combine_body(const V *__restrict__ Vb, int64_t n, int m, const double *C, int k,
+                                             V *__restrict__ Y, int b, int nb) {
     using T = VTraits<V>;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t r = (int64_t)b * blockDim.x + threadIdx.x; r < n; r += (int64_t)nb * blockDim.x) {
         for (int i = 0; i < k; i++) {
             double ar = 0, ai = 0;
             for (int j = 0; j < m; j++) {
@@ -185,6 +185,72 @@ __global__ void k_combine(const V *__restrict__ Vb, int64_t n, int m, const doub
             }
             T::store(Y, (int64_t)i * n + r, ar, ai);
         }
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_gram(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
+                                              double *partial) {
+    __shared__ double sh[2 * kVB];
+    gram_body<V>(Vb, n, m, w, partial, blockIdx.x, gridDim.x, sh);
+}
+
+__global__ __launch_bounds__(kVB) void k_reduce(const double *partial, int nb, double *out) {
+    __shared__ double sh[2 * kVB];
+    reduce_body(partial, nb, out, blockIdx.x, sh);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_update(const V *__restrict__ Vb, int64_t n, int m, const double *h,
+                                                V *__restrict__ w, double *partial) {
+    __shared__ double sh[2 * kVB];
+    update_body<V>(Vb, n, m, h, w, partial, blockIdx.x, gridDim.x, sh);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_norm2(const V *__restrict__ x, int64_t n, double *partial) {
+    __shared__ double sh[2 * kVB];
+    norm2_body<V>(x, n, partial, blockIdx.x, gridDim.x, sh);
+}
+
+template <typename V>
+__global__ void k_axpby(double ar, double ai, const V *__restrict__ x, double br, double bi, V *__restrict__ y,
+                        int64_t n, int b_zero) {
+    axpby_body<V>(ar, ai, x, br, bi, y, n, b_zero, blockIdx.x, gridDim.x);
+}
+
+template <typename V>
+__global__ void k_combine(const V *__restrict__ Vb, int64_t n, int m, const double *C, int k, V *__restrict__ Y) {
+    combine_body<V>(Vb, n, m, C, k, Y, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------ batched launches --
+// Block -> (job, local block) by the jobs' first-block offsets (ascending).
+__device__ __forceinline__ int find_job(const VecJobs &J, int &b) {
+    int lo = 0, hi = J.njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)blockIdx.x >= J.job[mid].b0) lo = mid;
+        else hi = mid - 1;
+    }
+    b = (int)blockIdx.x - J.job[lo].b0;
+    return lo;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void kb_vec(VecJobs J, int op) {
+    __shared__ double sh[2 * kVB];
+    int b;
+    const VecJob &q = J.job[find_job(J, b)];
+    switch (op) {
+    case VOP_GRAM: gram_body<V>((const V *)q.V, q.n, q.m, (const V *)q.x, q.partial, b, q.nb, sh); break;
+    case VOP_REDUCE: reduce_body(q.partial, q.nb, q.aux, b, sh); break;
+    case VOP_UPDATE: update_body<V>((const V *)q.V, q.n, q.m, q.aux, (V *)q.w, q.partial, b, q.nb, sh); break;
+    case VOP_NORM2: norm2_body<V>((const V *)q.x, q.n, q.partial, b, q.nb, sh); break;
+    case VOP_AXPBY:
+        axpby_body<V>(q.ar, q.ai, (const V *)q.x, q.br, q.bi, (V *)q.w, q.n, q.b_zero, b, q.nb);
+        break;
+    case VOP_COMBINE: combine_body<V>((const V *)q.V, q.n, q.m, q.aux, q.k, (V *)q.w, b, q.nb); break;
     }
 }
 
@@ -218,6 +284,7 @@ __global__ void k_random(V *__restrict__ y, int64_t n, uint64_t seed) {
 
 int ew_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
 
+
 template <typename F>
 void by_dtype(int dtype, F &&f) {
     switch (dtype) {
@@ -235,6 +302,27 @@ void ensure(VecWork &wk, size_t partial, size_t result) {
 }
 
 }  // namespace
+
+int ew_blocks(int64_t n) { return ew_grid(n); }
+
+void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s) {
+    for (size_t first = 0; first < jobs.size(); first += kMaxVecJobs) {
+        VecJobs J;
+        J.njobs = (int)std::min<size_t>(kMaxVecJobs, jobs.size() - first);
+        int total = 0;
+        for (int i = 0; i < J.njobs; i++) {
+            J.job[i] = jobs[first + i];
+            J.job[i].b0 = total;
+            total += blocks[first + i];
+        }
+        if (total == 0) continue;
+        by_dtype(dtype, [&](auto *tag) {
+            using V = std::remove_pointer_t<decltype(tag)>;
+            hipLaunchKernelGGL((kb_vec<V>), dim3(total), dim3(kVB), 0, s, J, op);
+        });
+        QS_HIP(hipGetLastError());
+    }
+}
 
 int vec_blocks(int64_t n) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + 4 * kVB - 1) / (4 * kVB)));
