@@ -172,6 +172,48 @@ int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, in
 int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const double *scales,
                      double tol, int krylov_dim, void *out, int *n_matvec);
 
+/* ------------------------------------------------ blocks of one parent space --
+ * BlockOps' psi_0 lives in the basis built without the block keywords
+ * (tools.jl:2447-2456).  kind 0: that parent is the block basis' own particle
+ * sector, indexed by combinadic rank in ascending state order (all 2^L
+ * states, indexed by the state, without nup); kind 1: the parent has no
+ * particle number while the block has (blocks over nup), indexed by the
+ * state.  Device pointers. */
+int qs_basis_parent_size(const qs_basis *b, int kind, int64_t *n);
+
+/* projector' * psi -- tools.jl:2453 (projection_matrix, symmetry_basis.jl:234-255):
+ * out (Ns) from parent (parent size). */
+int qs_basis_project(const qs_basis *b, int dtype, int kind, const void *parent, void *out);
+
+/* parent (+)= projector * x for ncols columns -- _accumulate_projection!,
+ * tools.jl:2488-2495 (x: Ns x ncols, parent: parent size x ncols, both
+ * column-major).  Symmetry sectors need the parent lookup table (L <= 32). */
+int qs_basis_expand(const qs_basis *b, int dtype, int kind, const void *x, void *parent, int64_t ncols,
+                    int accumulate);
+
+/* ------------------------------------------------------------ batches --------
+ * Many operators (sectors) advanced in lockstep: one launch per Krylov step
+ * for all of them.  Every job's result is bitwise the same whatever else is
+ * in the batch.  Operators must take the two-bit hop kernel (see
+ * qs_op_set_variant).  Vectors are device pointers; calls are synchronous. */
+typedef struct qs_batch qs_batch;
+
+/* No reference counterpart (the reference loops over blocks,
+ * tools.jl:1864-1882); the operators must outlive the batch. */
+int qs_batch_create(qs_op *const *ops, int n_ops, int dtype, qs_batch **out);
+int qs_batch_destroy(qs_batch *b);
+
+/* y_j = op[op_index[j]] x_j for n_jobs jobs in one launch per kernel class. */
+int qs_batch_matvec(qs_batch *b, int n_jobs, const int32_t *op_index, const void *const *x, void *const *y);
+
+/* _krylov_expmv_times (operators.jl:1805-1893) of n_jobs (operator, v,
+ * time grid) jobs, as _block_batch_states runs them per block
+ * (tools.jl:2500-2520): job j's grid is n_times[j] complex scales taken in
+ * order from `scales` (interleaved re, im); out[j] is Ns_j x n_times[j]. */
+int qs_batch_expm_multiply(qs_batch *b, int n_jobs, const int32_t *op_index, const void *const *v,
+                           const int64_t *n_times, const double *scales, double tol, int krylov_dim,
+                           void *const *out, int32_t *n_matvec);
+
 #ifdef __cplusplus
 }
 #endif

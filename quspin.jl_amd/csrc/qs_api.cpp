@@ -754,3 +754,114 @@ int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const
 }
 
 }  // extern "C"
+
+struct qs_batch {
+    std::vector<const qs::OpHost *> ops;
+    int dtype = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<qs::BatchMatvec, void (*)(qs::BatchMatvec *)> bm{nullptr, [](qs::BatchMatvec *) {}};
+    ~qs_batch() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+int qs_basis_parent_size(const qs_basis *b, int kind, int64_t *n) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(n, "n");
+        if (kind != 0 && kind != 1) fail(QS_EARG, "parent kind must be 0 (own parent) or 1 (all states)");
+        *n = parent_length(b->b, kind);
+    });
+}
+
+int qs_basis_project(const qs_basis *b, int dtype, int kind, const void *parent, void *out) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(parent, "parent");
+        check_ptr(out, "out");
+        basis_project(b->b, dtype, kind, parent, out, nullptr);
+        QS_HIP(hipStreamSynchronize(nullptr));
+    });
+}
+
+int qs_basis_expand(const qs_basis *b, int dtype, int kind, const void *x, void *parent, int64_t ncols,
+                    int accumulate) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(x, "x");
+        check_ptr(parent, "parent");
+        if (ncols < 0 || ncols > 0x7fffffff) fail(QS_EARG, "ncols out of range");
+        basis_expand(b->b, dtype, kind, x, parent, (int)ncols, accumulate != 0, nullptr);
+        QS_HIP(hipStreamSynchronize(nullptr));
+    });
+}
+
+int qs_batch_create(qs_op *const *ops, int n_ops, int dtype, qs_batch **out) {
+    return guard([&] {
+        check_ptr(out, "out");
+        *out = nullptr;
+        if (n_ops <= 0) fail(QS_EARG, "a batch needs at least one operator");
+        check_ptr(ops, "ops");
+        auto h = std::make_unique<qs_batch>();
+        for (int i = 0; i < n_ops; i++) {
+            check_ptr(ops[i], "ops[i]");
+            h->ops.push_back(&ops[i]->o);
+        }
+        dtype_size(dtype);
+        h->dtype = dtype;
+        h->bm = make_batch_matvec(h->ops, dtype);
+        QS_HIP(hipStreamCreate(&h->stream));
+        *out = h.release();
+    });
+}
+
+int qs_batch_destroy(qs_batch *b) {
+    return guard([&] { delete b; });
+}
+
+int qs_batch_matvec(qs_batch *b, int n_jobs, const int32_t *op_index, const void *const *x, void *const *y) {
+    return guard([&] {
+        check_ptr(b, "batch");
+        if (n_jobs < 0) fail(QS_EARG, "n_jobs must be non-negative");
+        std::vector<MatvecJob> jobs;
+        for (int j = 0; j < n_jobs; j++) {
+            if (op_index[j] < 0 || op_index[j] >= (int)b->ops.size()) fail(QS_EARG, "op_index out of range");
+            check_ptr(x[j], "x[j]");
+            check_ptr(y[j], "y[j]");
+            jobs.push_back({op_index[j], x[j], y[j]});
+        }
+        if (!jobs.empty()) launch_batch_matvec(*b->bm, jobs, b->stream);
+        QS_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+int qs_batch_expm_multiply(qs_batch *b, int n_jobs, const int32_t *op_index, const void *const *v,
+                           const int64_t *n_times, const double *scales, double tol, int krylov_dim,
+                           void *const *out, int32_t *n_matvec) {
+    return guard([&] {
+        check_ptr(b, "batch");
+        if (n_jobs < 0) fail(QS_EARG, "n_jobs must be non-negative");
+        std::vector<ExpmTask> tasks;
+        size_t k = 0;
+        for (int j = 0; j < n_jobs; j++) {
+            if (op_index[j] < 0 || op_index[j] >= (int)b->ops.size()) fail(QS_EARG, "op_index out of range");
+            if (n_times[j] < 0) fail(QS_EARG, "n_times must be non-negative");
+            check_ptr(v[j], "v[j]");
+            check_ptr(out[j], "out[j]");
+            ExpmTask t;
+            t.op = op_index[j];
+            t.v = v[j];
+            t.out = out[j];
+            for (int64_t i = 0; i < n_times[j]; i++, k++) t.scales.push_back(cd(scales[2 * k], scales[2 * k + 1]));
+            tasks.push_back(std::move(t));
+        }
+        std::vector<int> nmv;
+        expm_times_batch(b->ops, b->dtype, tasks, tol, krylov_dim, b->stream, nmv);
+        if (n_matvec)
+            for (int j = 0; j < n_jobs; j++) n_matvec[j] = nmv[j];
+    });
+}
+
+}  // extern "C"

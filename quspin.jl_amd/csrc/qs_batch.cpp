@@ -1,0 +1,515 @@
+// qs_batch.cpp -- many sectors / initial states / time grids in lockstep.
+//
+// The reference evolves the blocks of a BlockOps one after another (or on
+// host threads): per block P'psi, _krylov_expmv_times, then P * states
+// (tools.jl:2450-2560, operators.jl:1655-1893).  Here every block's Krylov
+// recursion advances together: one step of all blocks is one batched matvec
+// launch plus batched Gram / update / norm launches and a single host round
+// trip for all their scalars.  Each job inside a batched launch sees the
+// block partition of its own single launch, and the host arithmetic is the
+// single driver's (KrylovProj), so a job's result does not depend on which
+// other jobs share the batch: alone and in a mix it is bitwise the same.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "qs_internal.h"
+
+namespace qs {
+
+namespace {
+
+// ---------------------------------------------------- batched vector ops --
+struct BWork {
+    DevBuf<double> partial, result, coef;
+    std::vector<double> host;
+    void need(size_t p, size_t r, size_t c) {
+        if (partial.n < p) partial.alloc(std::max<size_t>(p, 4096));
+        if (result.n < r) result.alloc(std::max<size_t>(r, 1024));
+        if (coef.n < c) coef.alloc(std::max<size_t>(c, 1024));
+    }
+};
+
+struct DotReq {            // h[i] = <V[:, i], w>, i < m
+    const void *V;
+    int64_t n;
+    int m;
+    const void *w;
+};
+
+void bdots(int dtype, const std::vector<DotReq> &rq, BWork &wk, hipStream_t s, std::vector<std::vector<cd>> &out) {
+    size_t np = 0, nr = 0;
+    std::vector<size_t> po, ro;
+    for (auto &q : rq) {
+        po.push_back(np);
+        ro.push_back(nr);
+        np += (size_t)2 * vec_blocks(q.n) * q.m;
+        nr += (size_t)2 * q.m;
+    }
+    wk.need(np, nr, 0);
+    std::vector<VecJob> g, r;
+    std::vector<int> gb, rb;
+    for (size_t i = 0; i < rq.size(); i++) {
+        VecJob j{};
+        j.V = rq[i].V;
+        j.x = rq[i].w;
+        j.n = rq[i].n;
+        j.m = rq[i].m;
+        j.nb = vec_blocks(rq[i].n);
+        j.partial = wk.partial.get() + po[i];
+        j.aux = wk.result.get() + ro[i];
+        g.push_back(j);
+        gb.push_back(j.nb);
+        r.push_back(j);
+        rb.push_back(j.m);
+    }
+    launch_vec_jobs(dtype, VOP_GRAM, g, gb, s);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    wk.host.resize(nr);
+    QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    out.assign(rq.size(), {});
+    for (size_t i = 0; i < rq.size(); i++) {
+        out[i].resize(rq[i].m);
+        for (int k = 0; k < rq[i].m; k++) out[i][k] = cd(wk.host[ro[i] + 2 * k], wk.host[ro[i] + 2 * k + 1]);
+    }
+}
+
+struct UpdReq {            // w -= V[:, :m] h; returns ||w||^2
+    const void *V;
+    int64_t n;
+    int m;
+    std::vector<cd> h;
+    void *w;
+};
+
+void bupdate(int dtype, const std::vector<UpdReq> &rq, BWork &wk, hipStream_t s, std::vector<double> &nrm2) {
+    size_t np = 0, nc = 0;
+    std::vector<size_t> po, co;
+    for (auto &q : rq) {
+        po.push_back(np);
+        co.push_back(nc);
+        np += (size_t)2 * vec_blocks(q.n);
+        nc += (size_t)2 * q.m;
+    }
+    wk.need(np, (size_t)2 * rq.size(), nc);
+    std::vector<double> hh(nc);
+    for (size_t i = 0; i < rq.size(); i++)
+        for (int k = 0; k < rq[i].m; k++) {
+            hh[co[i] + 2 * k] = rq[i].h[k].real();
+            hh[co[i] + 2 * k + 1] = rq[i].h[k].imag();
+        }
+    if (nc) QS_HIP(hipMemcpyAsync(wk.coef.get(), hh.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    std::vector<VecJob> u, r;
+    std::vector<int> ub, rb;
+    for (size_t i = 0; i < rq.size(); i++) {
+        VecJob j{};
+        j.V = rq[i].V;
+        j.w = rq[i].w;
+        j.n = rq[i].n;
+        j.m = rq[i].m;
+        j.nb = vec_blocks(rq[i].n);
+        j.partial = wk.partial.get() + po[i];
+        j.aux = wk.coef.get() + co[i];
+        u.push_back(j);
+        ub.push_back(j.nb);
+        VecJob q = j;
+        q.aux = wk.result.get() + 2 * i;
+        r.push_back(q);
+        rb.push_back(1);
+    }
+    launch_vec_jobs(dtype, VOP_UPDATE, u, ub, s);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    wk.host.resize(2 * rq.size());
+    QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * 2 * rq.size(), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    nrm2.resize(rq.size());
+    for (size_t i = 0; i < rq.size(); i++) nrm2[i] = wk.host[2 * i];
+}
+
+struct NormReq {
+    const void *x;
+    int64_t n;
+};
+
+void bnorm2(int dtype, const std::vector<NormReq> &rq, BWork &wk, hipStream_t s, std::vector<double> &nrm2) {
+    size_t np = 0;
+    std::vector<size_t> po;
+    for (auto &q : rq) {
+        po.push_back(np);
+        np += (size_t)2 * vec_blocks(q.n);
+    }
+    wk.need(np, (size_t)2 * rq.size(), 0);
+    std::vector<VecJob> a, r;
+    std::vector<int> ab, rb;
+    for (size_t i = 0; i < rq.size(); i++) {
+        VecJob j{};
+        j.x = rq[i].x;
+        j.n = rq[i].n;
+        j.m = 1;
+        j.nb = vec_blocks(rq[i].n);
+        j.partial = wk.partial.get() + po[i];
+        a.push_back(j);
+        ab.push_back(j.nb);
+        j.aux = wk.result.get() + 2 * i;
+        r.push_back(j);
+        rb.push_back(1);
+    }
+    launch_vec_jobs(dtype, VOP_NORM2, a, ab, s);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    wk.host.resize(2 * rq.size());
+    QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * 2 * rq.size(), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    nrm2.resize(rq.size());
+    for (size_t i = 0; i < rq.size(); i++) nrm2[i] = wk.host[2 * i];
+}
+
+struct AxpbyReq {          // y = a x + b y
+    cd a;
+    const void *x;
+    cd b;
+    void *y;
+    int64_t n;
+};
+
+void baxpby(int dtype, const std::vector<AxpbyReq> &rq, hipStream_t s) {
+    std::vector<VecJob> a;
+    std::vector<int> ab;
+    for (auto &q : rq) {
+        VecJob j{};
+        j.x = q.x;
+        j.w = q.y;
+        j.n = q.n;
+        j.ar = q.a.real();
+        j.ai = q.a.imag();
+        j.br = q.b.real();
+        j.bi = q.b.imag();
+        j.b_zero = q.b == cd(0, 0);
+        j.nb = ew_blocks(q.n);
+        a.push_back(j);
+        ab.push_back(j.nb);
+    }
+    launch_vec_jobs(dtype, VOP_AXPBY, a, ab, s);
+}
+
+struct CombReq {           // Y[:, :k] = V[:, :m] C (C column-major m x k)
+    const void *V;
+    int64_t n;
+    int m;
+    std::vector<cd> C;
+    int k;
+    void *Y;
+};
+
+void bcombine(int dtype, const std::vector<CombReq> &rq, BWork &wk, hipStream_t s) {
+    size_t nc = 0;
+    std::vector<size_t> co;
+    for (auto &q : rq) {
+        co.push_back(nc);
+        nc += (size_t)2 * q.m * q.k;
+    }
+    wk.need(0, 0, nc);
+    std::vector<double> cc(nc);
+    for (size_t i = 0; i < rq.size(); i++)
+        for (size_t t = 0; t < (size_t)rq[i].m * rq[i].k; t++) {
+            cc[co[i] + 2 * t] = rq[i].C[t].real();
+            cc[co[i] + 2 * t + 1] = rq[i].C[t].imag();
+        }
+    if (nc) QS_HIP(hipMemcpyAsync(wk.coef.get(), cc.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    std::vector<VecJob> a;
+    std::vector<int> ab;
+    for (size_t i = 0; i < rq.size(); i++) {
+        VecJob j{};
+        j.V = rq[i].V;
+        j.n = rq[i].n;
+        j.m = rq[i].m;
+        j.k = rq[i].k;
+        j.w = rq[i].Y;
+        j.aux = wk.coef.get() + co[i];
+        j.nb = ew_blocks(rq[i].n);
+        a.push_back(j);
+        ab.push_back(j.nb);
+    }
+    launch_vec_jobs(dtype, VOP_COMBINE, a, ab, s);
+    // the coefficient buffer is reused by the next call
+    QS_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------ lockstep factorization --
+struct BFact {
+    int op;
+    int64_t n;
+    DevBuf<char> V;     // n x (steps + 1)
+    KrylovProj P;
+    void *col(int j, size_t esz) const { return (void *)(V.get() + (size_t)j * n * esz); }
+};
+
+// operators.jl:1655-1731 for every request at once; identical per-request
+// operation order to the single driver.
+void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dtype,
+                const std::vector<std::pair<int, const void *>> &req, int krylov_dim, BWork &wk, hipStream_t s,
+                std::vector<std::unique_ptr<BFact>> &F, std::vector<int> &nmv) {
+    const size_t esz = dtype_size(dtype);
+    const size_t R = req.size();
+    F.clear();
+    std::vector<NormReq> nq;
+    for (size_t i = 0; i < R; i++) {
+        auto f = std::make_unique<BFact>();
+        const OpHost &op = *ops[req[i].first];
+        f->op = req[i].first;
+        f->n = op.basis->Ns;
+        f->P.dim = f->n;
+        f->P.hermitian = op.hermitian != 0;
+        f->P.steps = (int)std::min<int64_t>(krylov_dim, f->n);
+        f->V.alloc((size_t)f->n * (f->P.steps + 1) * esz);
+        f->P.H.assign((size_t)(f->P.steps + 1) * f->P.steps, cd(0, 0));
+        f->P.completed = f->P.steps;
+        f->P.terminal = 0;
+        nq.push_back({req[i].second, f->n});
+        F.push_back(std::move(f));
+    }
+    std::vector<double> b2;
+    bnorm2(dtype, nq, wk, s, b2);
+    std::vector<AxpbyReq> sc;
+    for (size_t i = 0; i < R; i++) {
+        F[i]->P.beta = std::sqrt(b2[i]);
+        sc.push_back({cd(1.0 / F[i]->P.beta, 0), req[i].second, cd(0, 0), F[i]->col(0, esz), F[i]->n});
+    }
+    baxpby(dtype, sc, s);
+    std::vector<char> active(R, 1);
+    std::vector<double> prev(R, 0.0);
+    int max_steps = 0;
+    for (auto &f : F) max_steps = std::max(max_steps, f->P.steps);
+    for (int col = 0; col < max_steps; col++) {
+        std::vector<size_t> act;
+        for (size_t i = 0; i < R; i++)
+            if (active[i] && col < F[i]->P.steps) act.push_back(i);
+        if (act.empty()) break;
+        std::vector<MatvecJob> mj;
+        for (size_t i : act) mj.push_back({F[i]->op, F[i]->col(col, esz), F[i]->col(col + 1, esz)});
+        launch_batch_matvec(bm, mj, s);
+        for (size_t i : act) nmv[i]++;
+        std::vector<size_t> herm, arn;
+        for (size_t i : act) (F[i]->P.hermitian ? herm : arn).push_back(i);
+        std::vector<double> tn(R, 0.0);
+        if (!herm.empty()) {
+            // three-term Lanczos without reorthogonalisation
+            std::vector<AxpbyReq> ax;
+            for (size_t i : herm)
+                if (col > 0)
+                    ax.push_back({cd(-prev[i], 0), F[i]->col(col - 1, esz), cd(1, 0), F[i]->col(col + 1, esz), F[i]->n});
+            if (!ax.empty()) baxpby(dtype, ax, s);
+            std::vector<DotReq> dq;
+            for (size_t i : herm) dq.push_back({F[i]->col(col, esz), F[i]->n, 1, F[i]->col(col + 1, esz)});
+            std::vector<std::vector<cd>> hh;
+            bdots(dtype, dq, wk, s, hh);
+            std::vector<UpdReq> uq;
+            for (size_t t = 0; t < herm.size(); t++) {
+                const size_t i = herm[t];
+                uq.push_back({F[i]->col(col, esz), F[i]->n, 1, {hh[t][0]}, F[i]->col(col + 1, esz)});
+            }
+            std::vector<double> n2;
+            bupdate(dtype, uq, wk, s, n2);
+            for (size_t t = 0; t < herm.size(); t++) {
+                const size_t i = herm[t];
+                tn[i] = std::sqrt(n2[t]);
+                F[i]->P.lanczos_entry(col, hh[t][0], tn[i]);
+            }
+        }
+        if (!arn.empty()) {
+            // Arnoldi, two Gram-Schmidt passes
+            std::vector<DotReq> dq;
+            for (size_t i : arn) dq.push_back({F[i]->V.get(), F[i]->n, col + 1, F[i]->col(col + 1, esz)});
+            std::vector<std::vector<cd>> h1, h2;
+            bdots(dtype, dq, wk, s, h1);
+            std::vector<UpdReq> uq;
+            for (size_t t = 0; t < arn.size(); t++)
+                uq.push_back({F[arn[t]]->V.get(), F[arn[t]]->n, col + 1, h1[t], F[arn[t]]->col(col + 1, esz)});
+            std::vector<double> n2;
+            bupdate(dtype, uq, wk, s, n2);
+            bdots(dtype, dq, wk, s, h2);
+            for (size_t t = 0; t < arn.size(); t++) uq[t].h = h2[t];
+            bupdate(dtype, uq, wk, s, n2);
+            for (size_t t = 0; t < arn.size(); t++) {
+                const size_t i = arn[t];
+                tn[i] = std::sqrt(n2[t]);
+                F[i]->P.arnoldi_column(col, h1[t], h2[t], tn[i]);
+            }
+        }
+        std::vector<AxpbyReq> sc2;
+        for (size_t i : act) {
+            if (F[i]->P.breakdown(col, tn[i])) {
+                active[i] = 0;
+                continue;
+            }
+            if (col + 1 < F[i]->P.steps)
+                sc2.push_back({cd(1.0 / tn[i], 0), F[i]->col(col + 1, esz), cd(0, 0), F[i]->col(col + 1, esz), F[i]->n});
+            prev[i] = tn[i];
+        }
+        if (!sc2.empty()) baxpby(dtype, sc2, s);
+    }
+}
+
+// One pending _krylov_expmv_vector call (operators.jl:1758-1803).
+struct Call {
+    const void *v;
+    cd scale;
+    double tol;
+    int depth;
+    void *out;
+    std::shared_ptr<DevBuf<char>> mid;   // keeps a split's midpoint alive
+};
+struct Chain {              // the calls of one (task, time point), run in order
+    int task;
+    std::vector<Call> stack;  // back() runs next
+};
+
+}  // namespace
+
+void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const std::vector<ExpmTask> &tasks,
+                      double tol, int krylov_dim, hipStream_t s, std::vector<int> &nmatvec) {
+    if (!dtype_complex(dtype)) fail(QS_EARG, "expm_multiply needs a complex vector dtype");
+    if (tol <= 0) fail(QS_EARG, "tol must be positive");
+    if (krylov_dim <= 0) fail(QS_EARG, "krylov_dim must be positive");
+    const size_t esz = dtype_size(dtype);
+    auto bm = make_batch_matvec(ops, dtype);
+    BWork wk;
+    nmatvec.assign(tasks.size(), 0);
+    auto outcol = [&](size_t t, size_t j) {
+        return (void *)((char *)tasks[t].out + j * ops[tasks[t].op]->basis->Ns * esz);
+    };
+    auto nrows = [&](size_t t) { return ops[tasks[t].op]->basis->Ns; };
+    // tasks with no work: copies of v
+    std::vector<NormReq> nq;
+    for (size_t t = 0; t < tasks.size(); t++) nq.push_back({tasks[t].v, nrows(t)});
+    std::vector<double> v2;
+    if (!nq.empty()) bnorm2(dtype, nq, wk, s, v2);
+    std::vector<size_t> live;
+    for (size_t t = 0; t < tasks.size(); t++) {
+        bool all_zero = true;
+        for (auto &z : tasks[t].scales) all_zero &= z == cd(0, 0);
+        if (tasks[t].scales.empty()) continue;
+        if (all_zero || v2[t] == 0) {
+            for (size_t j = 0; j < tasks[t].scales.size(); j++)
+                QS_HIP(hipMemcpyAsync(outcol(t, j), tasks[t].v, nrows(t) * esz, hipMemcpyDeviceToDevice, s));
+            continue;
+        }
+        live.push_back(t);
+    }
+    // one factorization per task, all in lockstep
+    std::vector<std::pair<int, const void *>> req;
+    for (size_t t : live) req.push_back({tasks[t].op, tasks[t].v});
+    std::vector<std::unique_ptr<BFact>> F;
+    std::vector<int> nmv(req.size(), 0);
+    bfactorize(*bm, ops, dtype, req, krylov_dim, wk, s, F, nmv);
+    for (size_t i = 0; i < live.size(); i++) nmatvec[live[i]] += nmv[i];
+    // every time point from its task's factorization; failures become chains
+    std::vector<Chain> chains;
+    std::vector<CombReq> cq;
+    std::vector<std::vector<double>> ests(live.size());
+    for (size_t i = 0; i < live.size(); i++) {
+        const size_t t = live[i];
+        const ExpmTask &T = tasks[t];
+        const KrylovProj &P = F[i]->P;
+        const int c = P.completed;
+        if (P.hermitian) {
+            std::vector<cd> coefs;
+            P.lanczos_times(T.scales, coefs, ests[i]);
+            cq.push_back({F[i]->V.get(), F[i]->n, c, coefs, (int)T.scales.size(), T.out});
+        } else {
+            ests[i].assign(T.scales.size(), 0.0);
+            for (size_t j = 0; j < T.scales.size(); j++) {
+                if (T.scales[j] == cd(0, 0)) continue;
+                std::vector<cd> coef;
+                ests[i][j] = P.expm_column(T.scales[j], coef);
+                cq.push_back({F[i]->V.get(), F[i]->n, c, coef, 1, outcol(t, j)});
+            }
+        }
+    }
+    bcombine(dtype, cq, wk, s);
+    std::vector<NormReq> oq;
+    std::vector<std::pair<size_t, size_t>> idx;
+    for (size_t i = 0; i < live.size(); i++) {
+        const size_t t = live[i];
+        for (size_t j = 0; j < tasks[t].scales.size(); j++) {
+            if (tasks[t].scales[j] == cd(0, 0)) {
+                QS_HIP(hipMemcpyAsync(outcol(t, j), tasks[t].v, nrows(t) * esz, hipMemcpyDeviceToDevice, s));
+                continue;
+            }
+            oq.push_back({outcol(t, j), nrows(t)});
+            idx.push_back({i, j});
+        }
+    }
+    std::vector<double> o2;
+    if (!oq.empty()) bnorm2(dtype, oq, wk, s, o2);
+    for (size_t q = 0; q < idx.size(); q++) {
+        const size_t i = idx[q].first, j = idx[q].second, t = live[i];
+        if (ests[i][j] > tol * std::max(1.0, std::sqrt(o2[q])))
+            chains.push_back({(int)t, {Call{tasks[t].v, tasks[t].scales[j], tol, 0, outcol(t, j), nullptr}}});
+    }
+    F.clear();
+    // halving recursion, the next call of as many chains as device memory
+    // holds Krylov bases for, in lockstep
+    int64_t nmax = 1;
+    for (auto *op : ops) nmax = std::max<int64_t>(nmax, op->basis->Ns);
+    size_t fr = 0, tot = 0;
+    QS_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t per_chain = (size_t)(krylov_dim + 3) * nmax * esz;
+    const size_t cap = std::max<size_t>(1, std::min<size_t>(4096, fr / 3 / per_chain));
+    while (true) {
+        std::vector<size_t> run;
+        for (size_t c = 0; c < chains.size() && run.size() < cap; c++)
+            if (!chains[c].stack.empty()) run.push_back(c);
+        if (run.empty()) break;
+        std::vector<NormReq> vq;
+        for (size_t c : run) vq.push_back({chains[c].stack.back().v, nrows(chains[c].task)});
+        std::vector<double> vn;
+        bnorm2(dtype, vq, wk, s, vn);
+        std::vector<size_t> fac;
+        for (size_t r = 0; r < run.size(); r++) {
+            Chain &ch = chains[run[r]];
+            Call &cl = ch.stack.back();
+            if (cl.scale == cd(0, 0) || vn[r] == 0) {
+                QS_HIP(hipMemcpyAsync(cl.out, cl.v, nrows(ch.task) * esz, hipMemcpyDeviceToDevice, s));
+                ch.stack.pop_back();
+                continue;
+            }
+            fac.push_back(run[r]);
+        }
+        if (fac.empty()) continue;
+        std::vector<std::pair<int, const void *>> rq;
+        for (size_t c : fac) rq.push_back({tasks[chains[c].task].op, chains[c].stack.back().v});
+        std::vector<int> m2(rq.size(), 0);
+        bfactorize(*bm, ops, dtype, rq, krylov_dim, wk, s, F, m2);
+        std::vector<CombReq> pq;
+        std::vector<double> est(fac.size());
+        for (size_t r = 0; r < fac.size(); r++) {
+            nmatvec[chains[fac[r]].task] += m2[r];
+            const Call &cl = chains[fac[r]].stack.back();
+            std::vector<cd> coef;
+            est[r] = F[r]->P.expm_column(cl.scale, coef);
+            pq.push_back({F[r]->V.get(), F[r]->n, F[r]->P.completed, coef, 1, cl.out});
+        }
+        bcombine(dtype, pq, wk, s);
+        std::vector<NormReq> on;
+        for (size_t r = 0; r < fac.size(); r++) on.push_back({chains[fac[r]].stack.back().out, F[r]->n});
+        std::vector<double> onr;
+        bnorm2(dtype, on, wk, s, onr);
+        F.clear();
+        for (size_t r = 0; r < fac.size(); r++) {
+            Chain &ch = chains[fac[r]];
+            Call cl = ch.stack.back();
+            ch.stack.pop_back();
+            if (est[r] > cl.tol * std::max(1.0, std::sqrt(onr[r]))) {
+                if (cl.depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
+                auto mid = std::make_shared<DevBuf<char>>((size_t)nrows(ch.task) * esz);
+                // second half runs after the first: pushed first
+                ch.stack.push_back(Call{mid->get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid});
+                ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid->get(), mid});
+            }
+        }
+    }
+    QS_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace qs

@@ -300,7 +300,7 @@ int ew_blocks(int64_t n);
 // Batched matvec: y_j = M_j x_j for several operators in one launch (all
 // must take the hop kernel with one wave per row tile).
 struct MatvecJob {
-    const OpHost *op;
+    int op;            // index into the batch's operator list
     const void *x;
     void *y;
 };
@@ -309,6 +309,26 @@ std::unique_ptr<BatchMatvec, void (*)(BatchMatvec *)> make_batch_matvec(const st
                                                                          int dtype);
 void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hipStream_t s);
 bool batch_matvec_ok(const std::vector<const OpHost *> &ops, int dtype);
+
+// Sector <-> parent space (qs_project.hip).  kind 0: the parent vector is
+// indexed like the basis' own parent (colex rank, or the state without nup);
+// kind 1: by the state (2^L entries).  out = P' parent;  parent (+)= P x for
+// ncols columns (column-major, leading dimensions parent_length and Ns).
+int64_t parent_length(const BasisHost &b, int kind);
+void basis_project(const BasisHost &b, int dtype, int kind, const void *parent, void *out, hipStream_t s);
+void basis_expand(const BasisHost &b, int dtype, int kind, const void *x, void *parent, int ncols, bool accumulate,
+                  hipStream_t s);
+
+// _krylov_expmv_times of many (operator, v, time grid) tasks in lockstep
+// (qs_batch.cpp); out: Ns x scales.size() of the complex dtype per task.
+struct ExpmTask {
+    int op;
+    const void *v;
+    std::vector<cd> scales;
+    void *out;
+};
+void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const std::vector<ExpmTask> &tasks,
+                      double tol, int krylov_dim, hipStream_t s, std::vector<int> &nmatvec);
 
 // vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
 struct VecWork {
@@ -333,6 +353,27 @@ void vec_random(int dtype, void *y, int64_t n, uint64_t seed, hipStream_t s);
 void vec_convert(int from_dtype, const void *x, int to_dtype, void *y, int64_t n, hipStream_t s);
 size_t dtype_size(int dtype);
 bool dtype_complex(int dtype);
+
+// Host side of an expm Krylov factorization (operators.jl:1655-1756): the
+// (steps+1) x steps projected matrix and the quantities the reference's
+// a-posteriori estimate needs.  Shared by the single and batched drivers so
+// both do the same host arithmetic.
+struct KrylovProj {
+    std::vector<cd> H;   // row-major (steps+1) x steps
+    int steps = 0, completed = 0;
+    double beta = 0, terminal = 0;
+    int64_t dim = 0;
+    bool hermitian = true;
+    cd h(int i, int j) const { return H[(size_t)i * steps + j]; }
+    void lanczos_entry(int col, cd alpha, double tn);
+    void arnoldi_column(int col, const std::vector<cd> &h1, const std::vector<cd> &h2, double tn);
+    // records tn; true (and completed = col + 1) on breakdown
+    bool breakdown(int col, double tn);
+    // beta exp(scale H_c)[:, 0] and the error estimate (_krylov_project)
+    double expm_column(cd scale, std::vector<cd> &coef) const;
+    // Hermitian: coefficients for every scale from one eigen-decomposition
+    void lanczos_times(const std::vector<cd> &scales, std::vector<cd> &coefs, std::vector<double> &ests) const;
+};
 
 // Krylov solvers (qs_krylov.cpp); vectors are device buffers of `dtype`
 struct EigshResult {

@@ -222,51 +222,38 @@ namespace {
 
 struct Factorization {
     std::unique_ptr<Krylov> K;
-    std::vector<cd> H;   // (steps+1) x steps, row-major
-    int steps = 0, completed = 0;
-    double beta = 0, terminal = 0;
-    int64_t dim = 0;
-    bool hermitian = true;
-    cd h(int i, int j) const { return H[(size_t)i * steps + j]; }
+    KrylovProj P;
 };
-
-double frob(const std::vector<cd> &H) {
-    double s = 0;
-    for (auto &z : H) s += std::norm(z);
-    return std::sqrt(s);
-}
 
 // operators.jl:1655-1731
 void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorization &F, VecWork &wk, int &nmv) {
     hipStream_t s = op.stream;
     const int64_t n = op.basis->Ns;
-    F.dim = n;
-    F.hermitian = op.hermitian != 0;
-    F.steps = (int)std::min<int64_t>(krylov_dim, n);
-    F.K = std::make_unique<Krylov>(dtype, n, F.steps + 1);
+    KrylovProj &P = F.P;
+    P.dim = n;
+    P.hermitian = op.hermitian != 0;
+    P.steps = (int)std::min<int64_t>(krylov_dim, n);
+    F.K = std::make_unique<Krylov>(dtype, n, P.steps + 1);
     Krylov &K = *F.K;
-    F.H.assign((size_t)(F.steps + 1) * F.steps, cd(0, 0));
-    auto H = [&](int i, int j) -> cd & { return F.H[(size_t)i * F.steps + j]; };
-    F.beta = std::sqrt(vec_norm2(dtype, v, n, wk, s));
-    vec_scale_copy(dtype, v, K.col(0), n, cd(1.0 / F.beta, 0), s);
-    F.completed = F.steps;
-    F.terminal = 0;
+    P.H.assign((size_t)(P.steps + 1) * P.steps, cd(0, 0));
+    P.beta = std::sqrt(vec_norm2(dtype, v, n, wk, s));
+    vec_scale_copy(dtype, v, K.col(0), n, cd(1.0 / P.beta, 0), s);
+    P.completed = P.steps;
+    P.terminal = 0;
     double prev = 0;
-    for (int col = 0; col < F.steps; col++) {
+    for (int col = 0; col < P.steps; col++) {
         void *w = K.col(col + 1);
         matvec(op, dtype, K.col(col), w);
         nmv++;
         double tn;
-        if (F.hermitian) {
+        if (P.hermitian) {
             // three-term Lanczos without reorthogonalisation
             if (col > 0) vec_axpby(dtype, cd(-prev, 0), K.col(col - 1), cd(1, 0), w, n, s);
             std::vector<cd> hh;
             gram_dots(dtype, K.col(col), n, 1, w, wk, s, hh);
-            H(col, col) = hh[0];
             std::vector<cd> one{hh[0]};
             tn = std::sqrt(gram_update(dtype, K.col(col), n, 1, one, w, wk, s));
-            H(col + 1, col) = tn;
-            if (col + 1 < F.steps) H(col, col + 1) = tn;
+            P.lanczos_entry(col, hh[0], tn);
         } else {
             // Arnoldi, two Gram-Schmidt passes
             std::vector<cd> h1, h2;
@@ -274,33 +261,21 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
             gram_update(dtype, K.V.get(), n, col + 1, h1, w, wk, s);
             gram_dots(dtype, K.V.get(), n, col + 1, w, wk, s, h2);
             double nr2 = gram_update(dtype, K.V.get(), n, col + 1, h2, w, wk, s);
-            for (int r = 0; r <= col; r++) H(r, col) = h1[r] + h2[r];
             tn = std::sqrt(nr2);
-            H(col + 1, col) = tn;
+            P.arnoldi_column(col, h1, h2, tn);
         }
-        F.terminal = tn;
-        if (tn <= 100 * kEps * std::max(1.0, frob(F.H))) {
-            F.completed = col + 1;
-            break;
-        } else if (col + 1 < F.steps) {
-            vec_scale_copy(dtype, w, w, n, cd(1.0 / tn, 0), s);
-        }
+        if (P.breakdown(col, tn)) break;
+        if (col + 1 < P.steps) vec_scale_copy(dtype, w, w, n, cd(1.0 / tn, 0), s);
         prev = tn;
     }
 }
 
 // operators.jl:1733-1756: result = beta * V[:, :c] exp(scale H_c)[:, 0]
 double project(OpHost &op, int dtype, Factorization &F, cd scale, void *out, VecWork &wk) {
-    int c = F.completed;
-    std::vector<cd> A((size_t)c * c), E;
-    for (int i = 0; i < c; i++)
-        for (int j = 0; j < c; j++) A[(size_t)i * c + j] = scale * F.h(i, j);
-    expm_dense(c, A, E);
-    std::vector<cd> coef(c);
-    for (int i = 0; i < c; i++) coef[i] = F.beta * E[(size_t)i * c + 0];
-    basis_combine(dtype, F.K->V.get(), F.dim, c, coef, 1, out, op.stream);
-    bool full = F.completed == F.steps && F.completed < F.dim;
-    return full ? std::abs(scale) * F.terminal * F.beta * std::abs(E[(size_t)(c - 1) * c + 0]) : 0.0;
+    std::vector<cd> coef;
+    const double est = F.P.expm_column(scale, coef);
+    basis_combine(dtype, F.K->V.get(), F.P.dim, F.P.completed, coef, 1, out, op.stream);
+    return est;
 }
 
 // operators.jl:1758-1803
@@ -349,31 +324,11 @@ void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<c
     }
     Factorization F;
     factorize(op, dtype, v, krylov_dim, F, wk, nmatvec);
-    int c = F.completed;
-    bool full = F.completed == F.steps && F.completed < F.dim;
-    if (F.hermitian) {
-        // eigen(Hermitian(H_c)): upper triangle, real diagonal
-        std::vector<double> A((size_t)c * c), w, Vv;
-        for (int i = 0; i < c; i++)
-            for (int j = 0; j < c; j++) {
-                int a = std::min(i, j), b = std::max(i, j);
-                A[(size_t)i * c + j] = (i == j) ? F.h(i, i).real() : F.h(a, b).real();
-            }
-        bool real_t = true;
-        for (int i = 0; i + 1 < c; i++) real_t &= std::fabs(F.h(i, i + 1).imag()) == 0.0;
-        (void)real_t;
-        sym_eig(c, A, w, Vv);
-        std::vector<cd> coefs((size_t)c * scales.size(), cd(0, 0));
-        std::vector<double> ests(scales.size(), 0.0);
-        for (size_t j = 0; j < scales.size(); j++) {
-            if (scales[j] == cd(0, 0)) continue;
-            for (int i = 0; i < c; i++) {
-                cd acc = 0;
-                for (int q = 0; q < c; q++) acc += Vv[(size_t)i * c + q] * std::exp(scales[j] * w[q]) * Vv[(size_t)0 * c + q];
-                coefs[j * c + i] = F.beta * acc;
-            }
-            ests[j] = full ? std::abs(scales[j]) * F.terminal * std::abs(coefs[j * c + c - 1]) : 0.0;
-        }
+    const int c = F.P.completed;
+    if (F.P.hermitian) {
+        std::vector<cd> coefs;
+        std::vector<double> ests;
+        F.P.lanczos_times(scales, coefs, ests);
         basis_combine(dtype, F.K->V.get(), n, c, coefs, (int)scales.size(), out, s);
         for (size_t j = 0; j < scales.size(); j++) {
             if (scales[j] == cd(0, 0)) {
@@ -397,6 +352,66 @@ void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<c
         }
     }
     QS_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------- small projected problem --
+void KrylovProj::lanczos_entry(int col, cd alpha, double tn) {
+    H[(size_t)col * steps + col] = alpha;
+    H[(size_t)(col + 1) * steps + col] = tn;
+    if (col + 1 < steps) H[(size_t)col * steps + col + 1] = tn;
+}
+
+void KrylovProj::arnoldi_column(int col, const std::vector<cd> &h1, const std::vector<cd> &h2, double tn) {
+    for (int r = 0; r <= col; r++) H[(size_t)r * steps + col] = h1[r] + h2[r];
+    H[(size_t)(col + 1) * steps + col] = tn;
+}
+
+bool KrylovProj::breakdown(int col, double tn) {
+    terminal = tn;
+    double f = 0;
+    for (auto &z : H) f += std::norm(z);
+    if (tn <= 100 * kEps * std::max(1.0, std::sqrt(f))) {
+        completed = col + 1;
+        return true;
+    }
+    return false;
+}
+
+double KrylovProj::expm_column(cd scale, std::vector<cd> &coef) const {
+    const int c = completed;
+    std::vector<cd> A((size_t)c * c), E;
+    for (int i = 0; i < c; i++)
+        for (int j = 0; j < c; j++) A[(size_t)i * c + j] = scale * h(i, j);
+    expm_dense(c, A, E);
+    coef.assign(c, cd(0, 0));
+    for (int i = 0; i < c; i++) coef[i] = beta * E[(size_t)i * c + 0];
+    const bool full = completed == steps && completed < dim;
+    return full ? std::abs(scale) * terminal * beta * std::abs(E[(size_t)(c - 1) * c + 0]) : 0.0;
+}
+
+void KrylovProj::lanczos_times(const std::vector<cd> &scales, std::vector<cd> &coefs,
+                               std::vector<double> &ests) const {
+    // eigen(Hermitian(H_c)): upper triangle, real diagonal
+    const int c = completed;
+    const bool full = completed == steps && completed < dim;
+    std::vector<double> A((size_t)c * c), w, Vv;
+    for (int i = 0; i < c; i++)
+        for (int j = 0; j < c; j++) {
+            int a = std::min(i, j), b = std::max(i, j);
+            A[(size_t)i * c + j] = (i == j) ? h(i, i).real() : h(a, b).real();
+        }
+    sym_eig(c, A, w, Vv);
+    coefs.assign((size_t)c * scales.size(), cd(0, 0));
+    ests.assign(scales.size(), 0.0);
+    for (size_t j = 0; j < scales.size(); j++) {
+        if (scales[j] == cd(0, 0)) continue;
+        for (int i = 0; i < c; i++) {
+            cd acc = 0;
+            for (int q = 0; q < c; q++) acc += Vv[(size_t)i * c + q] * std::exp(scales[j] * w[q]) * Vv[(size_t)0 * c + q];
+            coefs[j * c + i] = beta * acc;
+        }
+        ests[j] = full ? std::abs(scales[j]) * terminal * std::abs(coefs[j * c + c - 1]) : 0.0;
+    }
 }
 
 }  // namespace qs

@@ -547,7 +547,9 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
                     col[j] = row;
                 } else {
                     col[j] = e[j] & colmask;
-                    elem[j] = TRACK ? (int)(e[j] >> A.colbits) : 0;
+                    // a batched launch may run a basis without characters
+                    // (no element bits, colbits = 32) in the tracking kernel
+                    elem[j] = (TRACK && A.colbits < 32) ? (int)(e[j] >> A.colbits) : 0;
                 }
             }
 #pragma unroll
@@ -586,18 +588,18 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
     }
 }
 
-// Block of kMvBlock threads = kMvBlock / 64 waves.  WS waves share each
-// 64-row tile: wave `sub` takes the flip groups g = sub (mod WS) (so group
-// data stays wave-uniform) and the partial sums meet in LDS in a fixed order.
+// Block b of nb (kMvBlock threads = kMvBlock / 64 waves).  WS waves share
+// each 64-row tile: wave `sub` takes the flip groups g = sub (mod WS) (so
+// group data stays wave-uniform) and the partial sums meet in LDS in a fixed
+// order.  (b, nb) are explicit so a batched launch gives every job the
+// partition of its own single launch.
 template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
-__global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev *__restrict__ hops,
-                                                         const DiagClass *__restrict__ dcls,
-                                                         const TermDev *__restrict__ drest, const V *__restrict__ x,
-                                                         V *__restrict__ y) {
+__device__ __forceinline__ void hop_body(const MVArgs &A, const HopDev *__restrict__ hops,
+                                         const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
+                                         const V *__restrict__ x, V *__restrict__ y, int b, int nb, char *smem) {
     using T = VTraits<V>;
     constexpr bool VC = T::cplx;
     constexpr int kRows = kMvBlock / WS;
-    extern __shared__ __align__(16) char smem[];
     uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
     double *s_red = (double *)(smem + A.off_red);
     __shared__ double s_part[WS > 1 ? 2 * kMvBlock : 1];
@@ -606,10 +608,10 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int sub = wave % WS, rw = wave / WS;
-    const int vb = xcd_block(blockIdx.x, gridDim.x);
+    const int vb = xcd_block(b, nb);
     const uint32_t n = (uint32_t)A.N;
     double dot_r = 0, dot_i = 0;
-    for (uint32_t base = (uint32_t)vb * kRows; base < n; base += (uint32_t)gridDim.x * kRows) {
+    for (uint32_t base = (uint32_t)vb * kRows; base < n; base += (uint32_t)nb * kRows) {
         const uint32_t row = base + rw * 64 + lane;
         double ar = 0, ai = 0, xr = 0, xi = 0;
         if (row < n) {
@@ -649,7 +651,62 @@ __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev 
         }
         if (sub == 0 && row < n) finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
     }
-    if (A.dot_partial) write_dot_partial<kMvBlock>(A, s_red, dot_r, dot_i);
+    if (A.dot_partial) {
+        s_red[threadIdx.x] = dot_r;
+        s_red[kMvBlock + threadIdx.x] = dot_i;
+        __syncthreads();
+        for (int st = kMvBlock / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                s_red[threadIdx.x] += s_red[threadIdx.x + st];
+                s_red[kMvBlock + threadIdx.x] += s_red[kMvBlock + threadIdx.x + st];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            A.dot_partial[2 * b] = s_red[0];
+            A.dot_partial[2 * b + 1] = s_red[kMvBlock];
+        }
+    }
+}
+
+template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
+__global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev *__restrict__ hops,
+                                                         const DiagClass *__restrict__ dcls,
+                                                         const TermDev *__restrict__ drest, const V *__restrict__ x,
+                                                         V *__restrict__ y) {
+    extern __shared__ __align__(16) char smem[];
+    hop_body<V, CC, CM, TRACK, PC, WS>(A, hops, dcls, drest, x, y, blockIdx.x, gridDim.x, smem);
+}
+
+// Batched hop matvec: job j owns blocks [b0_j, b0_j + nb_j) of the grid and
+// runs them exactly as its own launch of nb_j blocks would.
+struct MVJob {
+    const MVArgs *A;   // device copy
+    const HopDev *hops;
+    const DiagClass *dcls;
+    const TermDev *drest;
+    const void *x;
+    void *y;
+    int b0, nb;
+};
+constexpr int kMaxMVJobs = 32;
+struct MVJobs {
+    int njobs;
+    MVJob job[kMaxMVJobs];
+};
+
+template <typename V, bool CC, int CM, bool TRACK, bool PC>
+__global__ __launch_bounds__(kMvBlock) void kb_matvec_hop(MVJobs J) {
+    extern __shared__ __align__(16) char smem[];
+    int lo = 0, hi = J.njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)blockIdx.x >= J.job[mid].b0) lo = mid;
+        else hi = mid - 1;
+    }
+    const MVJob &q = J.job[lo];
+    hop_body<V, CC, CM, TRACK, PC, 1>(*q.A, q.hops, q.dcls, q.drest, (const V *)q.x, (V *)q.y,
+                                      (int)blockIdx.x - q.b0, q.nb, smem);
 }
 
 struct Launch {
@@ -791,8 +848,10 @@ bool hop_eligible(const OpHost &op, bool adjoint) {
 // the stored kernel at LPR = 8, and the implicit kernel sized the same way
 int matvec_grid(int64_t n) { return grid_for(n, kStBlock / 8); }
 
-void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
-                   double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
+namespace {
+
+MVArgs base_args(const OpHost &op, bool adjoint, double are, double aim, double bre, double bim, bool conj_x,
+                 bool conj_y, double *dot_partial) {
     const BasisHost &b = *op.basis;
     MVArgs a;
     memset(&a, 0, sizeof a);
@@ -826,11 +885,49 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     a.conj_x = conj_x ? 1 : 0;
     a.conj_y = conj_y ? 1 : 0;
     a.dot_partial = dot_partial;
-    bool track = false;
+    return a;
+}
+
+bool basis_tracks(const BasisHost &b) {
     for (int p = 0; p < b.group.npoint; p++)
-        if (b.group.pchar[p] != cd(1, 0)) track = true;
+        if (b.group.pchar[p] != cd(1, 0)) return true;
     for (int t = 0; t < b.group.nT; t++)
-        if (b.group.lam[t] != cd(1, 0)) track = true;
+        if (b.group.lam[t] != cd(1, 0)) return true;
+    return false;
+}
+
+int hop_mode(const BasisHost &b) {
+    return b.mode == MODE_STORED ? CM_PTAB : (b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL);
+}
+
+// hop-kernel fields and LDS layout; returns the dynamic LDS bytes
+size_t hop_args(MVArgs &a, const OpHost &op, bool adjoint, bool dot_partial) {
+    const BasisHost &b = *op.basis;
+    const FastTerms &ft = adjoint ? op.fast_adj : op.fast;
+    a.reps = b.mode == MODE_STORED || b.states_stored ? (const void *)b.reps32.get() : nullptr;
+    a.ptab = b.ptab.get();
+    a.colbits = b.ptab_colbits;
+    a.cfull = (b.nup >= 0) ? (uint32_t)binom(b.L, b.nup) : 0u;
+    a.ndcls = (int)ft.dcls.size();
+    a.ndrest = (int)ft.drest.size();
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        size_t o = (off + 15) & ~(size_t)15;
+        off = o + bytes;
+        return (int)o;
+    };
+    a.off_binom = place(b.nup >= 0 ? sizeof(uint32_t) * 33 * 65 : 0);
+    a.off_red = place(dot_partial ? sizeof(double) * 2 * kMvBlock : 0);
+    return off;
+}
+
+}  // namespace
+
+void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
+                   double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
+    const BasisHost &b = *op.basis;
+    MVArgs a = base_args(op, adjoint, are, aim, bre, bim, conj_x, conj_y, dot_partial);
+    const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
     const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP;
@@ -838,26 +935,12 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     if (want_hop && !hop)
         fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
     if (hop) {
-        const FastTerms &ft = adjoint ? op.fast_adj : op.fast;
-        a.reps = b.mode == MODE_STORED || b.states_stored ? (const void *)b.reps32.get() : nullptr;
-        a.ptab = b.ptab.get();
-        a.colbits = b.ptab_colbits;
-        a.cfull = (b.nup >= 0) ? (uint32_t)binom(b.L, b.nup) : 0u;
-        a.ndcls = (int)ft.dcls.size();
-        a.ndrest = (int)ft.drest.size();
-        size_t off = 0;
-        auto place = [&](size_t bytes) {
-            size_t o = (off + 15) & ~(size_t)15;
-            off = o + bytes;
-            return (int)o;
-        };
-        a.off_binom = place(b.nup >= 0 ? sizeof(uint32_t) * 33 * 65 : 0);
-        a.off_red = place(dot_partial ? sizeof(double) * 2 * kMvBlock : 0);
+        const size_t lds = hop_args(a, op, adjoint, dot_partial != nullptr);
         // waves sharing one row tile: QS_VARIANT_HOP + k selects 2^k
         const int ws = want_hop ? (1 << (op.lanes_per_row - QS_VARIANT_HOP)) : 1;
         const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock / ws);
-        const int cm = b.mode == MODE_STORED ? CM_PTAB : (b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL);
-        Launch L{grid, off, s, nullptr, nullptr, nullptr};
+        const int cm = hop_mode(b);
+        Launch L{grid, lds, s, nullptr, nullptr, nullptr};
         HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                     adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
         dispatch_hop(a, dtype, cc, cm, track, ws, x, y, L, H);
@@ -887,6 +970,112 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     if (b.use64) dispatch_v<uint64_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     else dispatch_v<uint32_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     QS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------ batched matvec --
+struct BatchMatvec {
+    int dtype = 0;
+    std::vector<const OpHost *> ops;
+    DevBuf<MVArgs> args;          // device copies, one per operator
+    std::vector<int> nb, cm, pc;
+    std::vector<size_t> lds;
+    std::vector<char> track, cc;
+};
+
+bool batch_matvec_ok(const std::vector<const OpHost *> &ops, int dtype) {
+    for (const OpHost *op : ops)
+        if (!hop_eligible(*op, false) || (op->is_complex && !dtype_complex(dtype))) return false;
+    return !ops.empty();
+}
+
+std::unique_ptr<BatchMatvec, void (*)(BatchMatvec *)> make_batch_matvec(const std::vector<const OpHost *> &ops,
+                                                                         int dtype) {
+    if (!batch_matvec_ok(ops, dtype))
+        fail(QS_EARG, "batched matvec needs two-bit hop couplings (L <= 32, parent lookup tables) for every operator");
+    std::unique_ptr<BatchMatvec, void (*)(BatchMatvec *)> bm(new BatchMatvec, [](BatchMatvec *p) { delete p; });
+    bm->dtype = dtype;
+    bm->ops = ops;
+    std::vector<MVArgs> host(ops.size());
+    for (size_t i = 0; i < ops.size(); i++) {
+        const OpHost &op = *ops[i];
+        const BasisHost &b = *op.basis;
+        host[i] = base_args(op, false, 1.0, 0.0, 0.0, 0.0, false, false, nullptr);
+        bm->lds.push_back(hop_args(host[i], op, false, false));
+        bm->nb.push_back(grid_for(b.Ns, kMvBlock));
+        bm->cm.push_back(hop_mode(b));
+        bm->pc.push_back(b.nup >= 0);
+        bm->track.push_back(basis_tracks(b));
+        bm->cc.push_back(op.is_complex != 0);
+    }
+    bm->args.alloc(host.size());
+    QS_HIP(hipMemcpy(bm->args.get(), host.data(), sizeof(MVArgs) * host.size(), hipMemcpyHostToDevice));
+    return bm;
+}
+
+namespace {
+
+template <typename V, bool CC>
+void batch_launch_cm(int cm, bool pc, bool track, int total, size_t lds, hipStream_t s, const MVJobs &J) {
+#define QS_BHOP(CMV, TR, P) \
+    hipLaunchKernelGGL((kb_matvec_hop<V, CC, CMV, TR, P>), dim3(total), dim3(kMvBlock), lds, s, J)
+    if (pc) {
+        if (cm == CM_FULL) QS_BHOP(CM_FULL, false, true);
+        else if (cm == CM_ZONLY) QS_BHOP(CM_ZONLY, false, true);
+        else if (track) QS_BHOP(CM_PTAB, true, true);
+        else QS_BHOP(CM_PTAB, false, true);
+    } else {
+        if (cm == CM_FULL) QS_BHOP(CM_FULL, false, false);
+        else if (cm == CM_ZONLY) QS_BHOP(CM_ZONLY, false, false);
+        else if (track) QS_BHOP(CM_PTAB, true, false);
+        else QS_BHOP(CM_PTAB, false, false);
+    }
+#undef QS_BHOP
+}
+
+}  // namespace
+
+void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hipStream_t s) {
+    // one launch per (column mode, particle number) class, jobs in order
+    std::vector<char> done(jobs.size(), 0);
+    for (size_t first = 0; first < jobs.size(); first++) {
+        if (done[first]) continue;
+        const int key_cm = bm.cm[jobs[first].op], key_pc = bm.pc[jobs[first].op];
+        std::vector<size_t> group;
+        for (size_t j = first; j < jobs.size(); j++)
+            if (!done[j] && bm.cm[jobs[j].op] == key_cm && bm.pc[jobs[j].op] == key_pc) group.push_back(j);
+        for (size_t c0 = 0; c0 < group.size(); c0 += kMaxMVJobs) {
+            MVJobs J;
+            J.njobs = (int)std::min<size_t>(kMaxMVJobs, group.size() - c0);
+            int total = 0;
+            size_t lds = 0;
+            bool track = false, cc = false;
+            for (int i = 0; i < J.njobs; i++) {
+                const MatvecJob &mj = jobs[group[c0 + i]];
+                const OpHost &op = *bm.ops[mj.op];
+                J.job[i] = MVJob{bm.args.get() + mj.op, op.d_hops.get(), op.d_dcls.get(), op.d_drest.get(), mj.x, mj.y,
+                                 total, bm.nb[mj.op]};
+                total += bm.nb[mj.op];
+                lds = std::max(lds, bm.lds[mj.op]);
+                track = track || bm.track[mj.op];
+                cc = cc || bm.cc[mj.op];
+                done[group[c0 + i]] = 1;
+            }
+            switch (bm.dtype) {
+            case QS_F64: batch_launch_cm<double, false>(key_cm, key_pc, track, total, lds, s, J); break;
+            case QS_F32: batch_launch_cm<float, false>(key_cm, key_pc, track, total, lds, s, J); break;
+            case QS_C128:
+                if (cc) batch_launch_cm<cplx_d, true>(key_cm, key_pc, track, total, lds, s, J);
+                else batch_launch_cm<cplx_d, false>(key_cm, key_pc, track, total, lds, s, J);
+                break;
+            case QS_C64:
+                if (cc) batch_launch_cm<cplx_f, true>(key_cm, key_pc, track, total, lds, s, J);
+                else batch_launch_cm<cplx_f, false>(key_cm, key_pc, track, total, lds, s, J);
+                break;
+            default: fail(QS_EARG, "unknown dtype");
+            }
+            QS_HIP(hipGetLastError());
+        }
+    }
 }
 
 }  // namespace qs

@@ -22,3 +22,4 @@ from .operators import (  # noqa: F401
     mul,
     device_info,
 )
+from .blocks import BlockOps, Batch, block_expm  # noqa: F401

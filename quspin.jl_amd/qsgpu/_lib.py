@@ -19,6 +19,8 @@ EXPORTS = (
     "qs_basis_states", "qs_basis_reps", "qs_basis_norms", "qs_basis_is_complex", "qs_basis_lookup",
     "qs_op_create", "qs_op_destroy", "qs_op_info", "qs_op_set_diagonal", "qs_op_set_stream",
     "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply", "qs_op_set_variant", "qs_op_tune",
+    "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
+    "qs_batch_matvec", "qs_batch_expm_multiply",
 )
 
 _lib = None
@@ -60,6 +62,13 @@ def lib():
     L.qs_expm_multiply.argtypes = [vp, i32, vp, i64, vp, dbl, i32, vp, vp]
     L.qs_op_set_variant.argtypes = [vp, i32]
     L.qs_op_tune.argtypes = [vp, i32, i32, vp, vp]
+    L.qs_basis_parent_size.argtypes = [vp, i32, vp]
+    L.qs_basis_project.argtypes = [vp, i32, i32, vp, vp]
+    L.qs_basis_expand.argtypes = [vp, i32, i32, vp, vp, i64, i32]
+    L.qs_batch_create.argtypes = [vp, i32, i32, pp]
+    L.qs_batch_destroy.argtypes = [vp]
+    L.qs_batch_matvec.argtypes = [vp, i32, vp, vp, vp]
+    L.qs_batch_expm_multiply.argtypes = [vp, i32, vp, vp, vp, vp, dbl, i32, vp, vp]
     for name in EXPORTS:
         if name != "qs_last_error":
             getattr(L, name).restype = ctypes.c_int
