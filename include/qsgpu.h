@@ -214,6 +214,42 @@ int qs_batch_expm_multiply(qs_batch *b, int n_jobs, const int32_t *op_index, con
                            const int64_t *n_times, const double *scales, double tol, int krylov_dim,
                            void *const *out, int32_t *n_matvec);
 
+/* ------------------------------------------------------------- serving -----
+ * A request loop on one GPU: n_workers workers, each with its own HIP stream,
+ * take eigsh / expm jobs first-in first-out; a job starts when a worker is
+ * free and its Krylov working set fits the HBM budget beside the running
+ * jobs', so requests that arrive while others run are admitted as slots and
+ * memory free up and their kernels overlap on the device.  A job's result is
+ * bitwise the same alone and in any mix.  No reference counterpart: the
+ * reference answers one call at a time (eigsh operators.jl:3367-3408,
+ * _krylov_expmv_times :1805-1893); every job runs exactly those. */
+#define QS_JOB_EIGSH 0
+#define QS_JOB_EXPM 1
+typedef struct qs_job {
+    int kind;               /* QS_JOB_EIGSH or QS_JOB_EXPM */
+    qs_op *op;              /* must outlive the job */
+    int dtype;
+    int k, which, maxiter, ncv;   /* eigsh arguments as in qs_eigsh */
+    double tol;             /* eigsh tol, or the expm tolerance */
+    const void *v0;         /* host, Ns elements: eigsh start (may be NULL) or expm v (required); copied */
+    int64_t n_times;        /* expm: number of complex scales */
+    const double *scales;   /* expm: interleaved (re, im); copied */
+    int krylov_dim;         /* expm */
+    int want_vectors;       /* eigsh: keep the Ritz vectors (Ns x k) */
+} qs_job;
+typedef struct qs_server qs_server;
+
+int qs_server_create(int n_workers, int64_t hbm_budget_bytes, qs_server **out);
+/* Runs every queued job, then stops the workers. */
+int qs_server_destroy(qs_server *s);
+int qs_server_submit(qs_server *s, const qs_job *job, int64_t *ticket);
+/* Blocks until the job is done and hands back its results once: eigsh vals
+ * (k, or 2k interleaved when not Hermitian) and vectors in `out` (Ns x k) if
+ * asked for; expm states in `out` (Ns x n_times).  seconds[3] = time queued,
+ * time running, total.  Returns the job's own status. */
+int qs_server_wait(qs_server *s, int64_t ticket, double *vals, void *out, int *nconv, int *n_matvec,
+                   double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -228,6 +228,112 @@ void ensure_stage(DevBuf<char> &buf, size_t bytes) {
 }  // namespace
 }  // namespace qs
 
+namespace qs {
+// eigsh on stream s (see qs_eigsh); host v0 / vals / vecs
+void eigsh_host(OpHost &o, hipStream_t s, int dtype, int k, int which, double tol, int maxiter, int ncv,
+                const void *v0, double *vals, void *vecs, int *nconv, int *n_matvec) {
+    const int64_t n = o.basis->Ns;
+    if (k < 1 || k > n) fail(QS_EARG, "k must lie in 1:Ns");
+    if (which < QS_WHICH_SA || which > QS_WHICH_BE) fail(QS_EARG, "which must be SA, LA, SM, LM, or BE");
+    if (o.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+    StreamScope scope(s);
+    const size_t esz = dtype_size(dtype);
+    DevBuf<char> dv0, dvec;
+    if (v0) {
+        dv0.alloc((size_t)n * esz);
+        QS_HIP(hipMemcpyAsync(dv0.get(), v0, (size_t)n * esz, hipMemcpyHostToDevice, s));
+    }
+    if (vecs) dvec.alloc((size_t)n * k * esz);
+    EigshResult res;
+    if (k >= n - 1) {
+        // operators.jl:3379-3382: dense fallback for tiny problems
+        int nn = (int)n;
+        int cdt = dtype_complex(dtype) ? QS_C128 : QS_F64;
+        DevBuf<char> E((size_t)nn * nn * dtype_size(cdt)), Y((size_t)nn * nn * dtype_size(cdt));
+        std::vector<char> eh((size_t)nn * nn * dtype_size(cdt), 0);
+        for (int j = 0; j < nn; j++) {
+            if (cdt == QS_F64) ((double *)eh.data())[(size_t)j * nn + j] = 1.0;
+            else ((double *)eh.data())[2 * ((size_t)j * nn + j)] = 1.0;
+        }
+        QS_HIP(hipMemcpyAsync(E.get(), eh.data(), eh.size(), hipMemcpyHostToDevice, s));
+        for (int j = 0; j < nn; j++)
+            launch_matvec(o, cdt, E.get() + (size_t)j * nn * dtype_size(cdt), Y.get() + (size_t)j * nn * dtype_size(cdt),
+                          1, 0, 0, 0, false, false, false, s);
+        QS_HIP(hipMemcpyAsync(eh.data(), Y.get(), eh.size(), hipMemcpyDeviceToHost, s));
+        QS_HIP(hipStreamSynchronize(s));
+        std::vector<cd> A((size_t)nn * nn);
+        for (int i = 0; i < nn; i++)
+            for (int j = 0; j < nn; j++)
+                A[(size_t)i * nn + j] = cdt == QS_F64 ? cd(((double *)eh.data())[(size_t)j * nn + i], 0)
+                                                      : cd(((double *)eh.data())[2 * ((size_t)j * nn + i)],
+                                                           ((double *)eh.data())[2 * ((size_t)j * nn + i) + 1]);
+        std::vector<double> w;
+        std::vector<cd> Vv;
+        herm_eig(nn, A, w, Vv);
+        std::vector<int> idx(nn);
+        for (int i = 0; i < nn; i++) idx[i] = i;
+        auto key = [&](int i) {
+            switch (which) {
+            case QS_WHICH_LA: return -w[i];
+            case QS_WHICH_SM: return std::fabs(w[i]);
+            case QS_WHICH_LM: return -std::fabs(w[i]);
+            default: return w[i];
+            }
+        };
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b2) { return key(a) < key(b2); });
+        if (which == QS_WHICH_BE) {
+            std::vector<int> be;
+            for (int i = 0; i < k / 2; i++) be.push_back(i);
+            for (int i = 0; i < (k + 1) / 2; i++) be.push_back(nn - 1 - i);
+            idx = be;
+        }
+        for (int i = 0; i < k; i++) vals[i] = w[idx[i]];
+        if (vecs) {
+            std::vector<char> vh((size_t)n * k * esz);
+            for (int c = 0; c < k; c++)
+                for (int r = 0; r < nn; r++) {
+                    cd z = Vv[(size_t)r * nn + idx[c]];
+                    size_t off = (size_t)c * n + r;
+                    switch (dtype) {
+                    case QS_F64: ((double *)vh.data())[off] = z.real(); break;
+                    case QS_F32: ((float *)vh.data())[off] = (float)z.real(); break;
+                    case QS_C128: ((double *)vh.data())[2 * off] = z.real(); ((double *)vh.data())[2 * off + 1] = z.imag(); break;
+                    case QS_C64: ((float *)vh.data())[2 * off] = (float)z.real(); ((float *)vh.data())[2 * off + 1] = (float)z.imag(); break;
+                    }
+                }
+            memcpy(vecs, vh.data(), vh.size());
+        }
+        if (nconv) *nconv = k;
+        if (n_matvec) *n_matvec = nn;
+        return;
+    }
+    eigsh_device(o, dtype, k, which, tol, maxiter, ncv, v0 ? dv0.get() : nullptr, res, vecs ? dvec.get() : nullptr);
+    for (int i = 0; i < k; i++) vals[i] = res.vals[i];
+    if (vecs) QS_HIP(hipMemcpyAsync(vecs, dvec.get(), (size_t)n * k * esz, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    if (nconv) *nconv = res.nconv;
+    if (n_matvec) *n_matvec = res.nmatvec;
+    if (res.nconv < k) fail(QS_ENOCONV, "eigsh: only " + std::to_string(res.nconv) + " of " + std::to_string(k) +
+                                            " Ritz values converged within maxiter");
+}
+// _krylov_expmv_times on stream s (see qs_expm_multiply); host v / out
+void expm_host(OpHost &o, hipStream_t s, int dtype, const void *v, int64_t n_times, const double *scales, double tol,
+               int krylov_dim, void *out, int *n_matvec) {
+    StreamScope scope(s);
+    const int64_t n = o.basis->Ns;
+    const size_t esz = dtype_size(dtype);
+    std::vector<cd> sc((size_t)n_times);
+    for (int64_t j = 0; j < n_times; j++) sc[j] = cd(scales[2 * j], scales[2 * j + 1]);
+    DevBuf<char> dv((size_t)n * esz), dout((size_t)n * std::max<int64_t>(1, n_times) * esz);
+    QS_HIP(hipMemcpyAsync(dv.get(), v, (size_t)n * esz, hipMemcpyHostToDevice, s));
+    int nmv = 0;
+    expm_times_device(o, dtype, dv.get(), sc, tol, krylov_dim, dout.get(), nmv);
+    QS_HIP(hipMemcpyAsync(out, dout.get(), (size_t)n * n_times * esz, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    if (n_matvec) *n_matvec = nmv;
+}
+}  // namespace qs
+
 using namespace qs;
 
 extern "C" {
@@ -647,109 +753,16 @@ int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, in
     return guard([&] {
         check_ptr(op, "op");
         check_ptr(vals, "vals");
-        OpHost &o = op->o;
-        const int64_t n = o.basis->Ns;
-        if (k < 1 || k > n) fail(QS_EARG, "k must lie in 1:Ns");
-        if (which < QS_WHICH_SA || which > QS_WHICH_BE) fail(QS_EARG, "which must be SA, LA, SM, LM, or BE");
-        if (o.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
-        hipStream_t s = ensure_stream(o);
-        const size_t esz = dtype_size(dtype);
-        DevBuf<char> dv0, dvec;
-        if (v0) {
-            dv0.alloc((size_t)n * esz);
-            QS_HIP(hipMemcpyAsync(dv0.get(), v0, (size_t)n * esz, hipMemcpyHostToDevice, s));
-        }
-        if (vecs) dvec.alloc((size_t)n * k * esz);
-        EigshResult res;
-        if (k >= n - 1) {
-            // operators.jl:3379-3382: dense fallback for tiny problems
-            int nn = (int)n;
-            int cdt = dtype_complex(dtype) ? QS_C128 : QS_F64;
-            DevBuf<char> E((size_t)nn * nn * dtype_size(cdt)), Y((size_t)nn * nn * dtype_size(cdt));
-            std::vector<char> eh((size_t)nn * nn * dtype_size(cdt), 0);
-            for (int j = 0; j < nn; j++) {
-                if (cdt == QS_F64) ((double *)eh.data())[(size_t)j * nn + j] = 1.0;
-                else ((double *)eh.data())[2 * ((size_t)j * nn + j)] = 1.0;
-            }
-            QS_HIP(hipMemcpy(E.get(), eh.data(), eh.size(), hipMemcpyHostToDevice));
-            for (int j = 0; j < nn; j++)
-                launch_matvec(o, cdt, E.get() + (size_t)j * nn * dtype_size(cdt), Y.get() + (size_t)j * nn * dtype_size(cdt),
-                              1, 0, 0, 0, false, false, false, s);
-            QS_HIP(hipMemcpy(eh.data(), Y.get(), eh.size(), hipMemcpyDeviceToHost));
-            std::vector<cd> A((size_t)nn * nn);
-            for (int i = 0; i < nn; i++)
-                for (int j = 0; j < nn; j++)
-                    A[(size_t)i * nn + j] = cdt == QS_F64 ? cd(((double *)eh.data())[(size_t)j * nn + i], 0)
-                                                          : cd(((double *)eh.data())[2 * ((size_t)j * nn + i)],
-                                                               ((double *)eh.data())[2 * ((size_t)j * nn + i) + 1]);
-            std::vector<double> w;
-            std::vector<cd> Vv;
-            herm_eig(nn, A, w, Vv);
-            std::vector<int> idx(nn);
-            for (int i = 0; i < nn; i++) idx[i] = i;
-            auto key = [&](int i) {
-                switch (which) {
-                case QS_WHICH_LA: return -w[i];
-                case QS_WHICH_SM: return std::fabs(w[i]);
-                case QS_WHICH_LM: return -std::fabs(w[i]);
-                default: return w[i];
-                }
-            };
-            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b2) { return key(a) < key(b2); });
-            if (which == QS_WHICH_BE) {
-                std::vector<int> be;
-                for (int i = 0; i < k / 2; i++) be.push_back(i);
-                for (int i = 0; i < (k + 1) / 2; i++) be.push_back(nn - 1 - i);
-                idx = be;
-            }
-            for (int i = 0; i < k; i++) vals[i] = w[idx[i]];
-            if (vecs) {
-                std::vector<char> vh((size_t)n * k * esz);
-                for (int c = 0; c < k; c++)
-                    for (int r = 0; r < nn; r++) {
-                        cd z = Vv[(size_t)r * nn + idx[c]];
-                        size_t off = (size_t)c * n + r;
-                        switch (dtype) {
-                        case QS_F64: ((double *)vh.data())[off] = z.real(); break;
-                        case QS_F32: ((float *)vh.data())[off] = (float)z.real(); break;
-                        case QS_C128: ((double *)vh.data())[2 * off] = z.real(); ((double *)vh.data())[2 * off + 1] = z.imag(); break;
-                        case QS_C64: ((float *)vh.data())[2 * off] = (float)z.real(); ((float *)vh.data())[2 * off + 1] = (float)z.imag(); break;
-                        }
-                    }
-                memcpy(vecs, vh.data(), vh.size());
-            }
-            if (nconv) *nconv = k;
-            if (n_matvec) *n_matvec = nn;
-            return;
-        }
-        eigsh_device(o, dtype, k, which, tol, maxiter, ncv, v0 ? dv0.get() : nullptr, res, vecs ? dvec.get() : nullptr);
-        for (int i = 0; i < k; i++) vals[i] = res.vals[i];
-        if (vecs) QS_HIP(hipMemcpy(vecs, dvec.get(), (size_t)n * k * esz, hipMemcpyDeviceToHost));
-        if (nconv) *nconv = res.nconv;
-        if (n_matvec) *n_matvec = res.nmatvec;
-        if (res.nconv < k) fail(QS_ENOCONV, "eigsh: only " + std::to_string(res.nconv) + " of " + std::to_string(k) +
-                                                " Ritz values converged within maxiter");
+        eigsh_host(op->o, ensure_stream(op->o), dtype, k, which, tol, maxiter, ncv, v0, vals, vecs, nconv, n_matvec);
     });
 }
-
 int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const double *scales, double tol,
                      int krylov_dim, void *out, int *n_matvec) {
     return guard([&] {
         check_ptr(op, "op");
         check_ptr(v, "v");
         check_ptr(out, "out");
-        OpHost &o = op->o;
-        hipStream_t s = ensure_stream(o);
-        const int64_t n = o.basis->Ns;
-        const size_t esz = dtype_size(dtype);
-        std::vector<cd> sc((size_t)n_times);
-        for (int64_t j = 0; j < n_times; j++) sc[j] = cd(scales[2 * j], scales[2 * j + 1]);
-        DevBuf<char> dv((size_t)n * esz), dout((size_t)n * std::max<int64_t>(1, n_times) * esz);
-        QS_HIP(hipMemcpyAsync(dv.get(), v, (size_t)n * esz, hipMemcpyHostToDevice, s));
-        int nmv = 0;
-        expm_times_device(o, dtype, dv.get(), sc, tol, krylov_dim, dout.get(), nmv);
-        QS_HIP(hipMemcpy(out, dout.get(), (size_t)n * n_times * esz, hipMemcpyDeviceToHost));
-        if (n_matvec) *n_matvec = nmv;
+        expm_host(op->o, ensure_stream(op->o), dtype, v, n_times, scales, tol, krylov_dim, out, n_matvec);
     });
 }
 

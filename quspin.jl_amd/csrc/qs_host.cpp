@@ -723,4 +723,12 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
     for (int i = 0; i < s; i++) E = mm(E, E);
 }
 
+namespace {
+thread_local hipStream_t t_stream = nullptr;
+}
+
+hipStream_t op_stream(const OpHost &op) { return t_stream ? t_stream : op.stream; }
+StreamScope::StreamScope(hipStream_t s) : prev(t_stream) { t_stream = s; }
+StreamScope::~StreamScope() { t_stream = prev; }
+
 }  // namespace qs

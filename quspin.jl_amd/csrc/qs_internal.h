@@ -230,6 +230,16 @@ struct OpHost {
 
 GroupDev group_to_dev(const GroupHost &h);
 
+// Stream the solvers launch on for `op` from the calling thread: a worker
+// stream installed with StreamScope (the serving loop runs several jobs on
+// one operator at once), else the operator's own stream.
+hipStream_t op_stream(const OpHost &op);
+struct StreamScope {
+    explicit StreamScope(hipStream_t s);
+    ~StreamScope();
+    hipStream_t prev;
+};
+
 // ---------------------------------------------------------- host logic ---
 uint64_t full_mask(int L);
 uint64_t bit_reverse(uint64_t u, int L);
@@ -329,6 +339,14 @@ struct ExpmTask {
 };
 void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const std::vector<ExpmTask> &tasks,
                       double tol, int krylov_dim, hipStream_t s, std::vector<int> &nmatvec);
+
+// eigsh / _krylov_expmv_times of one operator on stream s with host
+// vectors (the bodies of qs_eigsh and qs_expm_multiply; the serving loop's
+// workers call them with their own streams).
+void eigsh_host(OpHost &o, hipStream_t s, int dtype, int k, int which, double tol, int maxiter, int ncv,
+                const void *v0, double *vals, void *vecs, int *nconv, int *n_matvec);
+void expm_host(OpHost &o, hipStream_t s, int dtype, const void *v, int64_t n_times, const double *scales, double tol,
+               int krylov_dim, void *out, int *n_matvec);
 
 // vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
 struct VecWork {

@@ -34,7 +34,7 @@ struct Krylov {
 };
 
 void matvec(OpHost &op, int dtype, const void *x, void *y) {
-    launch_matvec(op, dtype, x, y, 1.0, 0.0, 0.0, 0.0, false, false, false, op.stream);
+    launch_matvec(op, dtype, x, y, 1.0, 0.0, 0.0, 0.0, false, false, false, op_stream(op));
 }
 
 // w (already holding A v_j) orthogonalised against V[:, 0..j] twice;
@@ -88,7 +88,7 @@ std::vector<int> select(const std::vector<double> &theta, int which, int want) {
 void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0_dev,
                   EigshResult &res, void *vecs_dev) {
     const int64_t n = op.basis->Ns;
-    hipStream_t s = op.stream;
+    hipStream_t s = op_stream(op);
     if (!op.hermitian) fail(QS_EARG, "eigsh on the device needs a Hermitian operator");
     if (tol <= 0) tol = kEps;
     if (maxiter <= 0) maxiter = 300;
@@ -227,7 +227,7 @@ struct Factorization {
 
 // operators.jl:1655-1731
 void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorization &F, VecWork &wk, int &nmv) {
-    hipStream_t s = op.stream;
+    hipStream_t s = op_stream(op);
     const int64_t n = op.basis->Ns;
     KrylovProj &P = F.P;
     P.dim = n;
@@ -274,14 +274,14 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
 double project(OpHost &op, int dtype, Factorization &F, cd scale, void *out, VecWork &wk) {
     std::vector<cd> coef;
     const double est = F.P.expm_column(scale, coef);
-    basis_combine(dtype, F.K->V.get(), F.P.dim, F.P.completed, coef, 1, out, op.stream);
+    basis_combine(dtype, F.K->V.get(), F.P.dim, F.P.completed, coef, 1, out, op_stream(op));
     return est;
 }
 
 // operators.jl:1758-1803
 void expmv_vector(OpHost &op, int dtype, const void *v, cd scale, double tol, int krylov_dim, int depth, void *out,
                   VecWork &wk, int &nmv) {
-    hipStream_t s = op.stream;
+    hipStream_t s = op_stream(op);
     const int64_t n = op.basis->Ns;
     if (tol <= 0) fail(QS_EARG, "tol must be positive");
     if (scale == cd(0, 0) || vec_norm2(dtype, v, n, wk, s) == 0) {
@@ -305,7 +305,7 @@ void expmv_vector(OpHost &op, int dtype, const void *v, cd scale, double tol, in
 
 void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<cd> &scales, double tol,
                        int krylov_dim, void *out, int &nmatvec) {
-    hipStream_t s = op.stream;
+    hipStream_t s = op_stream(op);
     const int64_t n = op.basis->Ns;
     const size_t esz = dtype_size(dtype);
     if (!dtype_complex(dtype)) fail(QS_EARG, "expm_multiply needs a complex vector dtype");

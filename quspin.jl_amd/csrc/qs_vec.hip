@@ -420,12 +420,12 @@ void basis_combine(int dtype, const void *Vmat, int64_t n, int m, const std::vec
 cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk) {
     const int g = matvec_grid(op.basis->Ns);
     ensure(wk, (size_t)2 * g, 2);
-    launch_matvec(op, dtype, x, y, 1.0, 0.0, 0.0, 0.0, false, false, false, op.stream, wk.partial.get());
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, op.stream, wk.partial.get(), g, wk.result.get());
+    launch_matvec(op, dtype, x, y, 1.0, 0.0, 0.0, 0.0, false, false, false, op_stream(op), wk.partial.get());
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, op_stream(op), wk.partial.get(), g, wk.result.get());
     QS_HIP(hipGetLastError());
     double out[2];
-    QS_HIP(hipMemcpyAsync(out, wk.result.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, op.stream));
-    QS_HIP(hipStreamSynchronize(op.stream));
+    QS_HIP(hipMemcpyAsync(out, wk.result.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, op_stream(op)));
+    QS_HIP(hipStreamSynchronize(op_stream(op)));
     return cd(out[0], out[1]);
 }
 

@@ -23,3 +23,4 @@ from .operators import (  # noqa: F401
     device_info,
 )
 from .blocks import BlockOps, Batch, block_expm  # noqa: F401
+from .serve import Server  # noqa: F401

@@ -20,7 +20,8 @@ EXPORTS = (
     "qs_op_create", "qs_op_destroy", "qs_op_info", "qs_op_set_diagonal", "qs_op_set_stream",
     "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply", "qs_op_set_variant", "qs_op_tune",
     "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
-    "qs_batch_matvec", "qs_batch_expm_multiply",
+    "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
+    "qs_server_wait",
 )
 
 _lib = None

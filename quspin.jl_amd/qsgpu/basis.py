@@ -57,7 +57,10 @@ class SpinBasis1D:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _lib.lib().qs_basis_destroy(h)
+            try:
+                _lib.lib().qs_basis_destroy(h)
+            except (TypeError, AttributeError):
+                pass   # interpreter shutdown: module globals are already gone
             self._h = None
 
     # basis.jl:779-812 -------------------------------------------------------

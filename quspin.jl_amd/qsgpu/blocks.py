@@ -54,11 +54,11 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _lib is not None and _lib.lib is not None:
+        if h is not None and h.value:
             try:
                 _lib.lib().qs_batch_destroy(h)
-            except Exception:
-                pass
+            except (TypeError, AttributeError):
+                pass   # interpreter shutdown: module globals are already gone
             self._h = None
 
     def matvec(self, jobs):

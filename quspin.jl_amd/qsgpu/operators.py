@@ -258,7 +258,10 @@ class _Handle:
 
     def __del__(self):
         if self.h is not None and self.h.value:
-            _lib.lib().qs_op_destroy(self.h)
+            try:
+                _lib.lib().qs_op_destroy(self.h)
+            except (TypeError, AttributeError):
+                pass   # interpreter shutdown: module globals are already gone
             self.h = None
 
 
