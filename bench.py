@@ -8,11 +8,14 @@ algorithmic bytes per matvec / time, with
     bytes = N * (8 read x + 8 write y + 4 stored state + 8 stored 1/sqrt(D)).
 Beside it: the FP64 Lanczos ground state (k=1, tol 1e-10) of the same
 sector, the L=32 z+ FP32-stream / FP64 matvec pair (N = 300,540,195,
-implicit basis; bytes = N * (2 * element + 4 stored state)) and the CPU baseline: the oracle's restatement of QuSpin.jl's
+implicit basis; bytes = N * (2 * element + 4 stored state)), the batched block evolution
+(L=24, all momentum blocks, Neel quench), the serving stream (Poisson, mixed eigsh /
+evolve, solves/s and p50/p99 latency) and the CPU baseline: the oracle's restatement of QuSpin.jl's
 parent-space apply loop (operators.jl:2554-2589), single-threaded, timed on a
 bounded sample of parent states and extrapolated to the whole sector.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-c3] [--no-eigsh]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-c3] [--no-c4] [--no-c5]
+                       [--no-eigsh]
 """
 import argparse
 import json
@@ -69,6 +72,98 @@ def time_matvec(op, x, y, steps, warmup):
     return start.elapsed_time(end) * 1e-3 / steps, wall
 
 
+def bench_c4(n_times):
+    """Batched form: quench from the Neel state in every momentum block (p-blocks at k = 0, pi) of the
+    L=24 nup=12 XXZ chain (J=1, Delta=2), exp(-iHt) on a grid of t in [0, 20], all blocks in one
+    lockstep engine; ComplexF64 and ComplexF32-stream forms."""
+    import numpy as np
+    import torch
+    import qsgpu
+    from qsgpu import serve
+    L, nup = 24, 12
+    print("bench: C4 blocks", file=sys.stderr, flush=True)
+    blocks = []
+    for k in range(L):
+        blocks += [dict(kblock=k, pblock=1), dict(kblock=k, pblock=-1)] if k in (0, L // 2) else [dict(kblock=k)]
+    tb = time.perf_counter()
+    ops = qsgpu.BlockOps(blocks, serve.chain_terms(L, 0.0, 2.0), [], qsgpu.SpinBasis1D, (L,), np.complex128,
+                         basis_kwargs=dict(nup=nup, pauli=False), compute_all_blocks=True)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - tb
+    neel = sum(1 << i for i in range(0, L, 2))
+    rank, kk, p, st = 0, 1, 0, neel
+    while st:
+        if st & 1:
+            rank += math.comb(p, kk)
+            kk += 1
+        st >>= 1
+        p += 1
+    psi = np.zeros(math.comb(L, nup), dtype=np.complex128)
+    psi[rank] = 1.0
+    times = np.linspace(0.0, 20.0, n_times)
+    out = {"blocks": len(ops.basis_dict), "parent_dim": int(psi.size), "n_times": int(n_times), "t_max": 20.0,
+           "build_s": round(t_build, 3)}
+    ref = None
+    for dt, name, tol in ((np.complex128, "c128", 1e-12), (np.complex64, "c64", 1e-6)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = ops.evolve(psi, 0.0, times, dtype=dt, tol=tol)
+        el = time.perf_counter() - t0
+        mv = int(sum(ops.last_n_matvec))
+        rows = sum(int(ops.basis_dict[k].Ns) for k in ops.basis_dict)
+        out[name] = {"seconds": round(el, 3), "block_matvecs": mv, "tol": tol}
+        if ref is None:
+            ref = st
+        else:
+            out[name]["max_abs_err_vs_c128"] = float(np.abs(st - ref).max())
+    out["rows_all_blocks"] = rows
+    return out
+
+
+def bench_c5(n_requests, n_workers):
+    """Serving form: Poisson stream (mean 200 req/s, seeded) of 60% eigsh k=1..4 / 40% evolve to
+    t ~ U[1,10] on XXZ / J1-J2 chains, L in {16,20,24,28}, nup=L/2, random sectors, HBM budget 200 GB.
+    Sector bases are built (cached) before the clock starts; each request's operator is built at
+    arrival.  Latency = completion - scheduled arrival."""
+    import numpy as np
+    import qsgpu
+    from qsgpu import serve
+    print("bench: C5 serving", file=sys.stderr, flush=True)
+    reqs = serve.poisson_workload(n_requests, rate=200.0, seed=2026)
+    builder = serve.RequestBuilder()
+    for r in reqs:
+        try:
+            builder.operator(r)
+        except qsgpu.ArgumentError:
+            r["skip"] = True
+    done = []
+    with serve.Server(n_workers=n_workers, hbm_budget=200e9) as srv:
+        start = time.perf_counter()
+        tickets = []
+        for r in reqs:
+            if r.get("skip"):
+                continue
+            d = start + r["arrival"] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            op = builder.operator(r)
+            if r["kind"] == "eigsh":
+                t = srv.submit_eigsh(op, k=min(r["k"], op.Ns), which="SA", tol=1e-10)
+            else:
+                t = srv.submit_evolve(op, builder.initial_state(r, op), 0.0, [r["t"]])
+            tickets.append((r, t))
+        for r, t in tickets:
+            res = srv.wait(t)
+            done.append((r, res, time.perf_counter() - start))
+    lat = np.array([c - r["arrival"] for r, _, c in done])
+    span = max(c for _, _, c in done)
+    return {"requests": len(done), "rate_per_s": 200.0, "workers": n_workers, "hbm_budget_bytes": 200e9,
+            "solves_per_s": round(len(done) / span, 2), "p50_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_s": round(float(np.percentile(lat, 99)), 4),
+            "mean_run_s": round(float(np.mean([x["seconds"][1] for _, x, _ in done])), 4),
+            "eigsh_tol": 1e-10, "evolve_tol": 1e-12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +172,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-eigsh", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c4-times", type=int, default=200)
+    ap.add_argument("--c5-requests", type=int, default=200)
+    ap.add_argument("--c5-workers", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for PMC byte counters)")
@@ -178,6 +278,10 @@ def main():
             torch.cuda.empty_cache()
         out["fp32_over_fp64"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)
         result["c3_matvec"] = out
+    if not args.no_c4:
+        result["c4_block_evolve"] = bench_c4(args.c4_times)
+    if not args.no_c5:
+        result["c5_serving"] = bench_c5(args.c5_requests, args.c5_workers)
     if not args.no_cpu:
         import oracle
         print("bench: CPU baseline (reference parent-apply loop, 1 thread, %d-state sample)" % args.cpu_sample,

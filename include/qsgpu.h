@@ -172,6 +172,17 @@ int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, in
 int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const double *scales,
                      double tol, int krylov_dim, void *out, int *n_matvec);
 
+/* lanczos_full(op, v0, m; full_ortho, eps) and the coefficients of
+ * lanczos_iter -- tools.jl:2742-2850: the reference's three-term recurrence
+ * on the device.  v0: host Ns elements of dtype; eps <= 0 selects the machine
+ * epsilon of dtype.  Outputs (host): alphas[m], betas[m-1], the
+ * eigen(SymTridiagonal) values E[used] (ascending) and vectors V (used x used,
+ * column-major), and, when Q_T is not NULL, the used Lanczos vectors as the
+ * rows of Q_T (m x Ns row-major).  *used = rows actually built (early
+ * breakdown at beta <= eps). */
+int qs_lanczos(qs_op *op, int dtype, const void *v0, int m, int full_ortho, double eps, double *alphas,
+               double *betas, double *E, double *V, void *Q_T, int *used);
+
 /* ------------------------------------------------ blocks of one parent space --
  * BlockOps' psi_0 lives in the basis built without the block keywords
  * (tools.jl:2447-2456).  kind 0: that parent is the block basis' own particle

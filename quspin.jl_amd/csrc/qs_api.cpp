@@ -766,6 +766,45 @@ int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const
     });
 }
 
+int qs_lanczos(qs_op *op, int dtype, const void *v0, int m, int full_ortho, double eps, double *alphas,
+               double *betas, double *E, double *V, void *Q_T, int *used) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(v0, "v0");
+        check_ptr(used, "used");
+        OpHost &o = op->o;
+        if (o.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+        hipStream_t s = ensure_stream(o);
+        StreamScope scope(s);
+        const int64_t n = o.basis->Ns;
+        const size_t esz = dtype_size(dtype);
+        if (!(1 <= m && m < n)) fail(QS_EARG, "Lanczos dimension m must satisfy 1 <= m < length(v0)");
+        DevBuf<char> dv((size_t)n * esz), Q((size_t)n * m * esz);
+        QS_HIP(hipMemcpyAsync(dv.get(), v0, (size_t)n * esz, hipMemcpyHostToDevice, s));
+        std::vector<double> a, b;
+        int u = 0;
+        lanczos_device(o, dtype, dv.get(), m, full_ortho != 0, eps, Q.get(), a, b, u);
+        *used = u;
+        if (alphas) std::copy(a.begin(), a.end(), alphas);
+        if (betas) std::copy(b.begin(), b.end(), betas);
+        // eigen(SymTridiagonal(alphas, betas))
+        std::vector<double> T((size_t)u * u, 0.0), w, Vv;
+        for (int i = 0; i < u; i++) {
+            T[(size_t)i * u + i] = a[i];
+            if (i + 1 < u) T[(size_t)i * u + i + 1] = T[(size_t)(i + 1) * u + i] = b[i];
+        }
+        sym_eig(u, T, w, Vv);
+        if (E) std::copy(w.begin(), w.end(), E);
+        if (V)
+            for (int i = 0; i < u; i++)
+                for (int j = 0; j < u; j++) V[(size_t)j * u + i] = Vv[(size_t)i * u + j];
+        if (Q_T) {
+            QS_HIP(hipMemcpyAsync(Q_T, Q.get(), (size_t)n * u * esz, hipMemcpyDeviceToHost, s));
+            QS_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
 }  // extern "C"
 
 struct qs_batch {

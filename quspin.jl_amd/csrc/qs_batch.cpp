@@ -259,7 +259,7 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         f->op = req[i].first;
         f->n = op.basis->Ns;
         f->P.dim = f->n;
-        f->P.hermitian = op.hermitian != 0;
+        f->P.hermitian = lanczos_ok(op, dtype);
         f->P.steps = (int)std::min<int64_t>(krylov_dim, f->n);
         f->V.alloc((size_t)f->n * (f->P.steps + 1) * esz);
         f->P.H.assign((size_t)(f->P.steps + 1) * f->P.steps, cd(0, 0));
@@ -350,14 +350,17 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
     }
 }
 
-// One pending _krylov_expmv_vector call (operators.jl:1758-1803).
+// One pending _krylov_expmv_vector call (operators.jl:1758-1803).  A call
+// owns the split midpoints it reads and writes (vbuf / obuf; null for the
+// task's own v and output column) so they live until every call using them
+// has run.
 struct Call {
     const void *v;
     cd scale;
     double tol;
     int depth;
     void *out;
-    std::shared_ptr<DevBuf<char>> mid;   // keeps a split's midpoint alive
+    std::shared_ptr<DevBuf<char>> vbuf, obuf;
 };
 struct Chain {              // the calls of one (task, time point), run in order
     int task;
@@ -445,7 +448,7 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
     for (size_t q = 0; q < idx.size(); q++) {
         const size_t i = idx[q].first, j = idx[q].second, t = live[i];
         if (ests[i][j] > tol * std::max(1.0, std::sqrt(o2[q])))
-            chains.push_back({(int)t, {Call{tasks[t].v, tasks[t].scales[j], tol, 0, outcol(t, j), nullptr}}});
+            chains.push_back({(int)t, {Call{tasks[t].v, tasks[t].scales[j], tol, 0, outcol(t, j), nullptr, nullptr}}});
     }
     F.clear();
     // halving recursion, the next call of as many chains as device memory
@@ -503,9 +506,9 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
             if (est[r] > cl.tol * std::max(1.0, std::sqrt(onr[r]))) {
                 if (cl.depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
                 auto mid = std::make_shared<DevBuf<char>>((size_t)nrows(ch.task) * esz);
-                // second half runs after the first: pushed first
-                ch.stack.push_back(Call{mid->get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid});
-                ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid->get(), mid});
+                // second half (mid -> out) runs after the first (v -> mid): pushed first
+                ch.stack.push_back(Call{mid->get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid, cl.obuf});
+                ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid->get(), cl.vbuf, mid});
             }
         }
     }

@@ -393,7 +393,15 @@ struct KrylovProj {
     void lanczos_times(const std::vector<cd> &scales, std::vector<cd> &coefs, std::vector<double> &ests) const;
 };
 
+// lanczos_full / lanczos_iter recurrence (qs_krylov.cpp): Q gets the used
+// rows (Ns x m column-major); alphas / betas the SymTridiagonal entries.
+void lanczos_device(OpHost &op, int dtype, const void *v0, int m, bool full_ortho, double eps, void *Q,
+                    std::vector<double> &alphas, std::vector<double> &betas, int &used);
+
 // Krylov solvers (qs_krylov.cpp); vectors are device buffers of `dtype`
+// expm factorizations use three-term Lanczos (the reference's Hermitian
+// path) only for Hermitian operators in FP64 storage
+bool lanczos_ok(const OpHost &op, int dtype);
 struct EigshResult {
     std::vector<double> vals;    // k (re) or 2k (re, im)
     int nconv = 0, nmatvec = 0;

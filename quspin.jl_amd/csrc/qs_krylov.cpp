@@ -220,6 +220,17 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
 // -------------------------------------------------------------- expm ------
 namespace {
 
+}  // namespace
+
+// The reference's Hermitian path is Lanczos without reorthogonalisation
+// (operators.jl:1686-1700).  With FP32-stored vectors orthogonality is lost
+// within a few steps and long-time results drift by O(1e-1); the FP32 form
+// therefore always takes the two-pass Gram-Schmidt (Arnoldi) path, whose
+// reorthogonalisation runs in FP64.
+bool lanczos_ok(const OpHost &op, int dtype) { return op.hermitian != 0 && (dtype == QS_F64 || dtype == QS_C128); }
+
+namespace {
+
 struct Factorization {
     std::unique_ptr<Krylov> K;
     KrylovProj P;
@@ -231,7 +242,7 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
     const int64_t n = op.basis->Ns;
     KrylovProj &P = F.P;
     P.dim = n;
-    P.hermitian = op.hermitian != 0;
+    P.hermitian = lanczos_ok(op, dtype);
     P.steps = (int)std::min<int64_t>(krylov_dim, n);
     F.K = std::make_unique<Krylov>(dtype, n, P.steps + 1);
     Krylov &K = *F.K;
@@ -412,6 +423,66 @@ void KrylovProj::lanczos_times(const std::vector<cd> &scales, std::vector<cd> &c
         }
         ests[j] = full ? std::abs(scales[j]) * terminal * std::abs(coefs[j * c + c - 1]) : 0.0;
     }
+}
+
+}  // namespace qs
+
+namespace qs {
+
+// lanczos_full / _lanczos_coefficients (tools.jl:2742-2850), step for step:
+// q = v0 / |v0|, residual = (A v0) / |v0|; for each index: store q, residual
+// = A q - beta_prev q_prev (index > 1), alpha = Re <q, residual>, residual -=
+// alpha q, optional sequential projection against every stored row, then
+// beta = |residual| and q = residual / beta.  Q (Ns x m column-major, i.e.
+// the reference's m x Ns Q_T row-major) is device memory of `dtype`.
+void lanczos_device(OpHost &op, int dtype, const void *v0, int m, bool full_ortho, double eps, void *Q,
+                    std::vector<double> &alphas, std::vector<double> &betas, int &used) {
+    hipStream_t s = op_stream(op);
+    const int64_t n = op.basis->Ns;
+    const size_t esz = dtype_size(dtype);
+    if (!(1 <= m && m < n)) fail(QS_EARG, "Lanczos dimension m must satisfy 1 <= m < length(v0)");
+    VecWork wk;
+    const double nrm = std::sqrt(vec_norm2(dtype, v0, n, wk, s));
+    if (!(nrm > 0)) fail(QS_EARG, "initial Lanczos vector must be nonzero");
+    const double tol = eps > 0 ? eps : ((dtype == QS_F32 || dtype == QS_C64) ? 1.1920928955078125e-07 : kEps);
+    auto col = [&](int j) { return (void *)((char *)Q + (size_t)j * n * esz); };
+    DevBuf<char> res((size_t)n * esz), qprev((size_t)n * esz);
+    matvec(op, dtype, v0, res.get());
+    vec_scale_copy(dtype, res.get(), res.get(), n, cd(1.0 / nrm, 0), s);
+    vec_scale_copy(dtype, v0, col(0), n, cd(1.0 / nrm, 0), s);
+    alphas.assign(m, 0.0);
+    betas.assign(std::max(0, m - 1), 0.0);
+    double beta_prev = 0;
+    used = 0;
+    for (int index = 0; index < m; index++) {
+        used = index + 1;
+        void *q = col(index);
+        if (index > 0) {
+            matvec(op, dtype, q, res.get());
+            vec_axpby(dtype, cd(-beta_prev, 0), col(index - 1), cd(1, 0), res.get(), n, s);
+        }
+        std::vector<cd> h;
+        gram_dots(dtype, q, n, 1, res.get(), wk, s, h);
+        const double alpha = h[0].real();
+        vec_axpby(dtype, cd(-alpha, 0), q, cd(1, 0), res.get(), n, s);
+        if (full_ortho) {
+            for (int p = 0; p <= index; p++) {
+                std::vector<cd> hp;
+                gram_dots(dtype, col(p), n, 1, res.get(), wk, s, hp);
+                vec_axpby(dtype, -hp[0], col(p), cd(1, 0), res.get(), n, s);
+            }
+        }
+        alphas[index] = alpha;
+        if (index + 1 == m) break;
+        const double beta = std::sqrt(vec_norm2(dtype, res.get(), n, wk, s));
+        if (beta <= tol) break;
+        betas[index] = beta;
+        vec_scale_copy(dtype, res.get(), col(index + 1), n, cd(1.0 / beta, 0), s);
+        beta_prev = beta;
+    }
+    alphas.resize(used);
+    betas.resize(std::max(0, used - 1));
+    QS_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace qs

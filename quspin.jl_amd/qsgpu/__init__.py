@@ -24,3 +24,4 @@ from .operators import (  # noqa: F401
 )
 from .blocks import BlockOps, Batch, block_expm  # noqa: F401
 from .serve import Server  # noqa: F401
+from .lanczos import lanczos_full, lanczos_iter, lin_comb_Q_T, expm_lanczos  # noqa: F401

@@ -167,3 +167,34 @@ def test_batch_rejects_general_couplings(gpu):
     op = qsgpu.QuantumLinearOperator(b, [("+z-", [(0.5, 1, 2, 3)]), ("-z+", [(0.5, 1, 2, 3)])], check_symm=False)
     with pytest.raises(qsgpu.ArgumentError, match="hop"):
         qsgpu.Batch([op])
+
+
+@pytest.mark.parametrize("dtype,tol,bound", [(np.complex128, 1e-12, 1e-11), (np.complex64, 1e-6, 2e-5)],
+                         ids=["c128", "c64"])
+def test_blockops_long_times_deep_recursion(gpu, dtype, tol, bound):
+    # t up to 20 at L=14 drives the halving recursion past depth 3 (split midpoints
+    # feeding further splits) in every block; exact per-block evolution as the bar
+    print("blockops long times", np.dtype(dtype).name, flush=True)
+    L, nup = 14, 7
+    terms = _xxz(L)
+    ops = qsgpu.BlockOps([dict(kblock=k) for k in range(L)], terms, [], qsgpu.SpinBasis1D, (L,), np.complex128,
+                         basis_kwargs=dict(nup=nup, pauli=False))
+    neel = sum(1 << i for i in range(0, L, 2))
+    n = math.comb(L, nup)
+    psi = np.zeros(n, dtype=np.complex128)
+    psi[_colex_rank(neel)] = 1.0
+    times = np.array([0.0, 5.0, 10.0, 20.0])
+    got = ops.evolve(psi, 0.0, times, dtype=dtype, tol=tol)
+    exact = torch.zeros((len(times), n), dtype=torch.complex128, device="cuda")
+    psig = torch.tensor(psi, device="cuda")
+    for k in range(L):
+        b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, kblock=k)
+        x = qsgpu.blocks.project(b, psig).cpu().numpy()
+        if np.linalg.norm(x) < 1e-12:
+            continue
+        w, V = np.linalg.eigh(qsgpu.QuantumLinearOperator(b, terms).toarray())
+        st = np.stack([V @ (np.exp(-1j * w * t) * (V.conj().T @ x)) for t in times])
+        qsgpu.blocks.expand(b, torch.tensor(st, device="cuda").contiguous(), exact, accumulate=True)
+    err = np.abs(got - exact.cpu().numpy().T).max(axis=0)
+    assert err.max() <= bound, err
+    assert np.abs(np.linalg.norm(got, axis=0) - 1).max() <= bound
