@@ -225,6 +225,16 @@ int qs_batch_expm_multiply(qs_batch *b, int n_jobs, const int32_t *op_index, con
                            const int64_t *n_times, const double *scales, double tol, int krylov_dim,
                            void *const *out, int32_t *n_matvec);
 
+/* evolve(::BlockOps, psi_0, t0, times) / block_expm with host vectors --
+ * tools.jl:2445-2646: psi (host, parent size of kind[j] for every block)
+ * is projected into each block (blocks whose projection has norm <= 1000 eps
+ * are skipped, as in _active_block_entries), every active block runs
+ * _krylov_expmv_times in one lockstep batch, and out (host, parent size x
+ * n_times, column-major) receives sum_j P_j states_j.  dtype QS_C128 or QS_C64. */
+int qs_blocks_evolve(qs_op *const *ops, const int32_t *kind, int n_ops, int dtype, const void *psi,
+                     int64_t n_times, const double *scales, double tol, int krylov_dim, void *out,
+                     int32_t *n_matvec);
+
 /* ------------------------------------------------------------- serving -----
  * A request loop on one GPU: n_workers workers, each with its own HIP stream,
  * take eigsh / expm jobs first-in first-out; a job starts when a worker is

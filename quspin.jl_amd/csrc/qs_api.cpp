@@ -869,6 +869,69 @@ int qs_batch_create(qs_op *const *ops, int n_ops, int dtype, qs_batch **out) {
     });
 }
 
+int qs_blocks_evolve(qs_op *const *ops, const int32_t *kind, int n_ops, int dtype, const void *psi, int64_t n_times,
+                     const double *scales, double tol, int krylov_dim, void *out, int32_t *n_matvec) {
+    return guard([&] {
+        if (n_ops <= 0) fail(QS_EARG, "BlockOps needs at least one block");
+        check_ptr(ops, "ops");
+        check_ptr(kind, "kind");
+        check_ptr(psi, "psi");
+        check_ptr(out, "out");
+        if (!dtype_complex(dtype)) fail(QS_EARG, "block evolution needs a complex vector dtype");
+        if (n_times < 0 || (n_times > 0 && !scales)) fail(QS_EARG, "bad time grid");
+        const size_t esz = dtype_size(dtype);
+        int64_t npar = -1;
+        for (int j = 0; j < n_ops; j++) {
+            check_ptr(ops[j], "ops[j]");
+            const int64_t p = parent_length(*ops[j]->o.basis, kind[j]);
+            if (npar >= 0 && p != npar) fail(QS_EDIM, "blocks of different parent spaces");
+            npar = p;
+        }
+        hipStream_t s;
+        QS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        DevBuf<char> dpsi((size_t)npar * esz), dout((size_t)npar * std::max<int64_t>(1, n_times) * esz);
+        QS_HIP(hipMemcpyAsync(dpsi.get(), psi, (size_t)npar * esz, hipMemcpyHostToDevice, s));
+        QS_HIP(hipMemsetAsync(dout.get(), 0, (size_t)npar * std::max<int64_t>(1, n_times) * esz, s));
+        // _active_block_entries: project, keep blocks with |P' psi| > 1000 eps
+        std::vector<DevBuf<char>> x(n_ops), st;
+        std::vector<const OpHost *> act_ops;
+        std::vector<int> act;
+        VecWork wk;
+        for (int j = 0; j < n_ops; j++) {
+            const BasisHost &b = *ops[j]->o.basis;
+            x[j].alloc((size_t)b.Ns * esz);
+            basis_project(b, dtype, kind[j], dpsi.get(), x[j].get(), s);
+            if (std::sqrt(vec_norm2(dtype, x[j].get(), b.Ns, wk, s)) > 1000 * 2.220446049250313e-16) {
+                act.push_back(j);
+                act_ops.push_back(&ops[j]->o);
+            }
+        }
+        if (act.empty()) fail(QS_EARG, "initial state has no projection onto the selected blocks");
+        std::vector<cd> sc((size_t)n_times);
+        for (int64_t t = 0; t < n_times; t++) sc[t] = cd(scales[2 * t], scales[2 * t + 1]);
+        std::vector<ExpmTask> tasks;
+        for (size_t i = 0; i < act.size(); i++) {
+            const int64_t ns = act_ops[i]->basis->Ns;
+            st.emplace_back((size_t)ns * std::max<int64_t>(1, n_times) * esz);
+            tasks.push_back(ExpmTask{(int)i, x[act[i]].get(), sc, st.back().get()});
+        }
+        std::vector<int> nmv;
+        expm_times_batch(act_ops, dtype, tasks, tol, krylov_dim, s, nmv);
+        for (size_t i = 0; i < act.size(); i++)
+            basis_expand(*act_ops[i]->basis, dtype, kind[act[i]], st[i].get(), dout.get(), (int)n_times, true, s);
+        QS_HIP(hipMemcpyAsync(out, dout.get(), (size_t)npar * n_times * esz, hipMemcpyDeviceToHost, s));
+        QS_HIP(hipStreamSynchronize(s));
+        if (n_matvec) {
+            for (int j = 0; j < n_ops; j++) n_matvec[j] = 0;
+            for (size_t i = 0; i < act.size(); i++) n_matvec[act[i]] = nmv[i];
+        }
+    });
+}
+
 int qs_batch_destroy(qs_batch *b) {
     return guard([&] { delete b; });
 }

@@ -187,4 +187,86 @@ function evolve(op::GPUQuantumLinearOperator, v0::AbstractVector, t0::Real, time
     return times isa Number ? states[:, 1] : states
 end
 
+# lanczos_full(A, v0, m; full_ortho, eps) -- tools.jl:2742-2809, recurrence on the device
+function lanczos_full(op::GPUQuantumLinearOperator{T}, v0::AbstractVector, m::Integer;
+                      full_ortho::Bool=false, eps=nothing) where {T}
+    n = size(op, 1)
+    1 <= m < n || throw(ArgumentError("Lanczos dimension m must satisfy 1 <= m < length(v0)"))
+    S = promote_type(T, eltype(v0))
+    x = Vector{S}(v0)
+    alphas, betas = zeros(Float64, m), zeros(Float64, max(1, m - 1))
+    E, V = zeros(Float64, m), zeros(Float64, m * m)
+    QT = zeros(S, n, m)                   # column j = Lanczos vector j (row j of Q_T)
+    used = Ref{Cint}(0)
+    check(ccall((:qs_lanczos, libqsgpu), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{S}, Cint, Cint, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{S}, Ref{Cint}),
+                op.handle, _dtype(S), x, m, full_ortho, eps === nothing ? 0.0 : eps, alphas, betas, E, V, QT,
+                used))
+    u = Int(used[])
+    return E[1:u], reshape(V[1:u*u], u, u), permutedims(QT[:, 1:u])
+end
+
+"""
+    GPUBlockOps(blocks, static, basis_args; basis_kwargs...)
+
+BlockOps (tools.jl:2303-2353) over GPU sectors: one `GPUBasis` and one
+`GPUQuantumLinearOperator` per non-empty block.  `evolve` and `block_expm`
+run every active block in one lockstep batch (`qs_blocks_evolve`).
+"""
+struct GPUBlockOps
+    bases::Vector{GPUBasis}
+    ops::Vector{GPUQuantumLinearOperator}
+    kinds::Vector{Cint}
+end
+
+function GPUBlockOps(blocks, static::AbstractVector{<:OperatorTerm}, basis_args::Tuple; basis_kwargs...)
+    bases, ops, kinds = GPUBasis[], GPUQuantumLinearOperator[], Cint[]
+    parent_nup = get(basis_kwargs, :nup, nothing)
+    for block in blocks
+        b = try
+            GPUBasis(basis_args...; basis_kwargs..., block...)
+        catch e
+            e isa ArgumentError && occursin("empty", e.msg) && continue
+            rethrow()
+        end
+        push!(bases, b)
+        push!(ops, GPUQuantumLinearOperator(b, static))
+        # blocks over nup inside a parent without one: the parent vector is indexed by the state
+        push!(kinds, (haskey(block, :nup) && parent_nup === nothing) ? Cint(1) : Cint(0))
+    end
+    return GPUBlockOps(bases, ops, kinds)
+end
+
+function _blocks_evolve(bo::GPUBlockOps, psi::AbstractVector, scales::AbstractVector; tol=1e-12, krylov_dim=30)
+    x = Vector{ComplexF64}(psi)
+    sc = ComplexF64.(scales)
+    out = zeros(ComplexF64, length(x), length(sc))
+    check(ccall((:qs_blocks_evolve, libqsgpu), Cint,
+                (Ptr{Ptr{Cvoid}}, Ptr{Cint}, Cint, Cint, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Cdouble, Cint,
+                 Ptr{ComplexF64}, Ptr{Int32}),
+                [o.handle for o in bo.ops], bo.kinds, length(bo.ops), QS_C128, x, length(sc), sc, tol,
+                krylov_dim, out, C_NULL))
+    return out
+end
+
+# evolve(::BlockOps, psi_0, t0, times) -- tools.jl:2588-2646 (static generator)
+function evolve(bo::GPUBlockOps, psi_0::AbstractVector, t0::Real, times; imag_time::Bool=false,
+                tol::Real=1e-12, krylov_dim::Integer=30)
+    targets = times isa Number ? [times] : collect(times)
+    scales = imag_time ? ComplexF64[-(t - t0) for t in targets] : ComplexF64[-im * (t - t0) for t in targets]
+    out = _blocks_evolve(bo, psi_0, scales; tol, krylov_dim)
+    return times isa Number ? out[:, 1] : out
+end
+
+# block_expm(::BlockOps, psi_0; a, start, stop, num, endpoint) -- tools.jl:2656-2702
+function block_expm(bo::GPUBlockOps, psi_0::AbstractVector; a::Number=-im, start=nothing, stop=nothing,
+                    num::Integer=50, endpoint::Bool=true)
+    grid = start === nothing && stop === nothing ? [1.0] :
+        collect(range(start, stop; length=num + (!endpoint)))
+    !endpoint && pop!(grid)
+    out = _blocks_evolve(bo, psi_0, ComplexF64.(grid .* a))
+    return length(grid) == 1 ? vec(out) : out
+end
+
 end # module

@@ -21,7 +21,7 @@ EXPORTS = (
     "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply", "qs_op_set_variant", "qs_op_tune",
     "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
-    "qs_server_wait", "qs_lanczos",
+    "qs_server_wait", "qs_lanczos", "qs_blocks_evolve",
 )
 
 _lib = None
@@ -71,6 +71,7 @@ def lib():
     L.qs_batch_matvec.argtypes = [vp, i32, vp, vp, vp]
     L.qs_batch_expm_multiply.argtypes = [vp, i32, vp, vp, vp, vp, dbl, i32, vp, vp]
     L.qs_lanczos.argtypes = [vp, i32, vp, i32, i32, dbl, vp, vp, vp, vp, vp, vp]
+    L.qs_blocks_evolve.argtypes = [vp, vp, i32, i32, vp, i64, vp, dbl, i32, vp, vp]
     for name in EXPORTS:
         if name != "qs_last_error":
             getattr(L, name).restype = ctypes.c_int

@@ -275,3 +275,23 @@ def block_expm(operator, psi_0, H_time_eval=0.0, iterate=False, n_jobs=1, block_
     if iterate:
         return (out[:, j].copy() for j in range(out.shape[1]))
     return out[:, 0].copy() if grid.size == 1 else out
+
+
+def blocks_evolve(ops, kinds, psi_0, scales, tol=1e-12, krylov_dim=30, dtype=np.complex128):
+    """One call of qs_blocks_evolve (host vectors): project psi_0 into every block, evolve the active
+    blocks in one lockstep batch, recombine -- the C-ABI form of evolve(::BlockOps) for callers without
+    device arrays (tools.jl:2445-2646).  Returns parent size x len(scales)."""
+    dt = np.dtype(dtype)
+    if dt not in (np.dtype(np.complex128), np.dtype(np.complex64)):
+        raise ArgumentError("block evolution needs a complex dtype (complex128 or complex64)")
+    psi = np.ascontiguousarray(np.asarray(psi_0).astype(dt))
+    sc = np.ascontiguousarray(np.atleast_1d(np.asarray(scales, dtype=np.complex128)))
+    out = np.zeros((psi.size, sc.size), dtype=dt, order="F")
+    n = len(ops)
+    nmv = np.zeros(max(1, n), dtype=np.int32)
+    check(_lib.lib().qs_blocks_evolve((ctypes.c_void_p * n)(*[op._h for op in ops]),
+                                      np.asarray(kinds, dtype=np.int32).ctypes.data, n, _tdtype(dt)[1],
+                                      psi.ctypes.data, sc.size, sc.ctypes.data, float(tol), int(krylov_dim),
+                                      out.ctypes.data, nmv.ctypes.data))
+    blocks_evolve.last_n_matvec = nmv[:n].tolist()
+    return out

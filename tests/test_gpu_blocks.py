@@ -198,3 +198,19 @@ def test_blockops_long_times_deep_recursion(gpu, dtype, tol, bound):
     err = np.abs(got - exact.cpu().numpy().T).max(axis=0)
     assert err.max() <= bound, err
     assert np.abs(np.linalg.norm(got, axis=0) - 1).max() <= bound
+
+
+def test_host_blocks_evolve_matches_blockops(gpu):
+    # qs_blocks_evolve (host vectors, the Julia binding's entry point) == BlockOps.evolve, bitwise
+    L, nup = 10, 5
+    terms = _xxz(L)
+    ops = qsgpu.BlockOps([dict(kblock=k) for k in range(L)], terms, [], qsgpu.SpinBasis1D, (L,), np.complex128,
+                         basis_kwargs=dict(nup=nup, pauli=False), compute_all_blocks=True)
+    rng = np.random.default_rng(8)
+    psi = rng.standard_normal(math.comb(L, nup)) + 1j * rng.standard_normal(math.comb(L, nup))
+    psi /= np.linalg.norm(psi)
+    times = np.array([0.0, 0.7, 3.0])
+    want = ops.evolve(psi, 0.0, times)
+    keys = list(ops.basis_dict)
+    got = qsgpu.blocks.blocks_evolve([ops.H_dict[k] for k in keys], [0] * len(keys), psi, -1j * times)
+    assert np.array_equal(got, want)
