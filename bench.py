@@ -8,7 +8,7 @@ algorithmic bytes per matvec / time, with
     bytes = N * (8 read x + 8 write y + 4 stored state + 8 stored 1/sqrt(D)).
 Beside it: the FP64 Lanczos ground state (k=1, tol 1e-10) of the same
 sector, the L=32 z+ FP32-stream / FP64 matvec pair (N = 300,540,195,
-implicit basis; bytes = N * (2 * element + 4 stored state)), the batched block evolution
+implicit basis; bytes = N * 2 * element), the batched block evolution
 (L=24, all momentum blocks, Neel quench), the serving stream (Poisson, mixed eigsh /
 evolve, solves/s and p50/p99 latency) and the CPU baseline: the oracle's restatement of QuSpin.jl's
 parent-space apply loop (operators.jl:2554-2589), single-threaded, timed on a
@@ -51,6 +51,15 @@ def profiled_traffic(key):
             return json.load(f)[key]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def variant_name(v):
+    import qsgpu
+    if v == qsgpu.VARIANT_SPLIT:
+        return "k_matvec_split (high-word blocks, low word in LDS)"
+    if v >= qsgpu.VARIANT_HOP:
+        return "k_matvec_hop (%d wave(s) per row tile)" % (1 << (v - qsgpu.VARIANT_HOP))
+    return "general kernel, %d lane(s) per row" % v
 
 
 def time_matvec(op, x, y, steps, warmup):
@@ -270,8 +279,9 @@ def main():
             y3 = torch.zeros_like(x3)
             lpr3, _ = op3.tune(np.float32 if esz == 4 else np.float64, repeats=2)
             t3, _ = time_matvec(op3, x3, y3, max(3, args.steps // 10), 2)
-            b3_alg = n3 * (2 * esz + 4)   # x, y and the stored u32 state per row
-            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(b3_alg / t3 / 1e9, 2), "lanes_per_row": lpr3,
+            b3_alg = n3 * 2 * esz   # read x once, write y once; the z+ basis is implicit (bit 31 clear)
+            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(b3_alg / t3 / 1e9, 2), "variant": lpr3,
+                         "kernel": variant_name(lpr3),
                          "frac": round(b3_alg / t3 / 1e9 / HBM_PEAK_GBPS, 4), "bytes_per_matvec": b3_alg,
                          "traffic": profiled_traffic("c3_" + name)}
             del x3, y3

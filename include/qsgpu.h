@@ -135,6 +135,12 @@ int qs_op_stream(const qs_op *op, void **stream);
  * with 2^k wavefronts sharing each row tile's couplings.  No reference
  * counterpart: the reference has one CPU loop. */
 #define QS_VARIANT_HOP 16
+/* The split kernel: two-bit hop couplings on a basis without symmetry or
+ * with spin inversion only, with a particle number, L <= 32.  Rows sharing
+ * their high bits form a contiguous block, so hops among the high bits read
+ * whole neighbour blocks and hops among the low bits stay in LDS.  No
+ * reference counterpart. */
+#define QS_VARIANT_SPLIT 32
 int qs_op_set_variant(qs_op *op, int lanes_per_row);
 
 /* Autotuner: times every variant for `dtype` on device-resident random

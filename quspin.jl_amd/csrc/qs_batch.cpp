@@ -30,103 +30,6 @@ struct BWork {
     }
 };
 
-struct DotReq {            // h[i] = <V[:, i], w>, i < m
-    const void *V;
-    int64_t n;
-    int m;
-    const void *w;
-};
-
-void bdots(int dtype, const std::vector<DotReq> &rq, BWork &wk, hipStream_t s, std::vector<std::vector<cd>> &out) {
-    size_t np = 0, nr = 0;
-    std::vector<size_t> po, ro;
-    for (auto &q : rq) {
-        po.push_back(np);
-        ro.push_back(nr);
-        np += (size_t)2 * vec_blocks(q.n) * q.m;
-        nr += (size_t)2 * q.m;
-    }
-    wk.need(np, nr, 0);
-    std::vector<VecJob> g, r;
-    std::vector<int> gb, rb;
-    for (size_t i = 0; i < rq.size(); i++) {
-        VecJob j{};
-        j.V = rq[i].V;
-        j.x = rq[i].w;
-        j.n = rq[i].n;
-        j.m = rq[i].m;
-        j.nb = vec_blocks(rq[i].n);
-        j.partial = wk.partial.get() + po[i];
-        j.aux = wk.result.get() + ro[i];
-        g.push_back(j);
-        gb.push_back(j.nb);
-        r.push_back(j);
-        rb.push_back(j.m);
-    }
-    launch_vec_jobs(dtype, VOP_GRAM, g, gb, s);
-    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
-    wk.host.resize(nr);
-    QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * nr, hipMemcpyDeviceToHost, s));
-    QS_HIP(hipStreamSynchronize(s));
-    out.assign(rq.size(), {});
-    for (size_t i = 0; i < rq.size(); i++) {
-        out[i].resize(rq[i].m);
-        for (int k = 0; k < rq[i].m; k++) out[i][k] = cd(wk.host[ro[i] + 2 * k], wk.host[ro[i] + 2 * k + 1]);
-    }
-}
-
-struct UpdReq {            // w -= V[:, :m] h; returns ||w||^2
-    const void *V;
-    int64_t n;
-    int m;
-    std::vector<cd> h;
-    void *w;
-};
-
-void bupdate(int dtype, const std::vector<UpdReq> &rq, BWork &wk, hipStream_t s, std::vector<double> &nrm2) {
-    size_t np = 0, nc = 0;
-    std::vector<size_t> po, co;
-    for (auto &q : rq) {
-        po.push_back(np);
-        co.push_back(nc);
-        np += (size_t)2 * vec_blocks(q.n);
-        nc += (size_t)2 * q.m;
-    }
-    wk.need(np, (size_t)2 * rq.size(), nc);
-    std::vector<double> hh(nc);
-    for (size_t i = 0; i < rq.size(); i++)
-        for (int k = 0; k < rq[i].m; k++) {
-            hh[co[i] + 2 * k] = rq[i].h[k].real();
-            hh[co[i] + 2 * k + 1] = rq[i].h[k].imag();
-        }
-    if (nc) QS_HIP(hipMemcpyAsync(wk.coef.get(), hh.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    std::vector<VecJob> u, r;
-    std::vector<int> ub, rb;
-    for (size_t i = 0; i < rq.size(); i++) {
-        VecJob j{};
-        j.V = rq[i].V;
-        j.w = rq[i].w;
-        j.n = rq[i].n;
-        j.m = rq[i].m;
-        j.nb = vec_blocks(rq[i].n);
-        j.partial = wk.partial.get() + po[i];
-        j.aux = wk.coef.get() + co[i];
-        u.push_back(j);
-        ub.push_back(j.nb);
-        VecJob q = j;
-        q.aux = wk.result.get() + 2 * i;
-        r.push_back(q);
-        rb.push_back(1);
-    }
-    launch_vec_jobs(dtype, VOP_UPDATE, u, ub, s);
-    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
-    wk.host.resize(2 * rq.size());
-    QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * 2 * rq.size(), hipMemcpyDeviceToHost, s));
-    QS_HIP(hipStreamSynchronize(s));
-    nrm2.resize(rq.size());
-    for (size_t i = 0; i < rq.size(); i++) nrm2[i] = wk.host[2 * i];
-}
-
 struct NormReq {
     const void *x;
     int64_t n;
@@ -235,6 +138,138 @@ void bcombine(int dtype, const std::vector<CombReq> &rq, BWork &wk, hipStream_t 
     QS_HIP(hipStreamSynchronize(s));
 }
 
+// Device-scalar forms: results land in device memory at the caller's
+// addresses and coefficients are read from there, so a whole lockstep
+// factorization is queued without a host round trip.  `wk.partial` must
+// already hold every job's partials (bwork_reserve).
+void bwork_reserve(BWork &wk, const std::vector<int64_t> &ns, int mmax) {
+    size_t np = 0;
+    for (int64_t n : ns) np += (size_t)2 * vec_blocks(n) * std::max(1, mmax);
+    wk.need(np, 0, 0);
+}
+
+struct DotDev {            // out[2i..2i+1] = <V[:, i], w>, i < m
+    const void *V;
+    int64_t n;
+    int m;
+    const void *w;
+    double *out;
+};
+
+void bdots_dev(int dtype, const std::vector<DotDev> &rq, BWork &wk, hipStream_t s) {
+    std::vector<VecJob> g, r;
+    std::vector<int> gb, rb;
+    size_t np = 0;
+    for (auto &q : rq) {
+        VecJob j{};
+        j.V = q.V;
+        j.x = q.w;
+        j.n = q.n;
+        j.m = q.m;
+        j.nb = vec_blocks(q.n);
+        j.partial = wk.partial.get() + np;
+        j.aux = q.out;
+        np += (size_t)2 * j.nb * q.m;
+        g.push_back(j);
+        gb.push_back(j.nb);
+        r.push_back(j);
+        rb.push_back(j.m);
+    }
+    if (np > wk.partial.n) fail(QS_EARG, "internal: batched workspace not reserved");
+    launch_vec_jobs(dtype, VOP_GRAM, g, gb, s);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+}
+
+struct UpdDev {            // w -= V[:, :m] h (h device); nrm2 (device, optional) = ||w||^2
+    const void *V;
+    int64_t n;
+    int m;
+    const double *h;
+    void *w;
+    double *nrm2;
+};
+
+void bupdate_dev(int dtype, const std::vector<UpdDev> &rq, BWork &wk, hipStream_t s) {
+    std::vector<VecJob> u, r;
+    std::vector<int> ub, rb;
+    size_t np = 0;
+    for (auto &q : rq) {
+        VecJob j{};
+        j.V = q.V;
+        j.w = q.w;
+        j.n = q.n;
+        j.m = q.m;
+        j.nb = vec_blocks(q.n);
+        j.partial = wk.partial.get() + np;
+        j.aux = const_cast<double *>(q.h);
+        np += (size_t)2 * j.nb;
+        u.push_back(j);
+        ub.push_back(j.nb);
+        if (q.nrm2) {
+            VecJob t = j;
+            t.aux = q.nrm2;
+            r.push_back(t);
+            rb.push_back(1);
+        }
+    }
+    if (np > wk.partial.n) fail(QS_EARG, "internal: batched workspace not reserved");
+    launch_vec_jobs(dtype, VOP_UPDATE, u, ub, s);
+    if (!r.empty()) launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+}
+
+void bnorm2_dev(int dtype, const std::vector<std::pair<const void *, int64_t>> &xs, const std::vector<double *> &out,
+                BWork &wk, hipStream_t s) {
+    std::vector<VecJob> a, r;
+    std::vector<int> ab, rb;
+    size_t np = 0;
+    for (size_t i = 0; i < xs.size(); i++) {
+        VecJob j{};
+        j.x = xs[i].first;
+        j.n = xs[i].second;
+        j.m = 1;
+        j.nb = vec_blocks(j.n);
+        j.partial = wk.partial.get() + np;
+        np += (size_t)2 * j.nb;
+        a.push_back(j);
+        ab.push_back(j.nb);
+        j.aux = out[i];
+        r.push_back(j);
+        rb.push_back(1);
+    }
+    if (np > wk.partial.n) fail(QS_EARG, "internal: batched workspace not reserved");
+    launch_vec_jobs(dtype, VOP_NORM2, a, ab, s);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+}
+
+struct AxpbyDev {          // y = a x + b y with a from device memory (CoefMode)
+    const double *coef;
+    int cmode;
+    const void *x;
+    cd b;
+    void *y;
+    int64_t n;
+};
+
+void baxpby_dev(int dtype, const std::vector<AxpbyDev> &rq, hipStream_t s) {
+    std::vector<VecJob> a;
+    std::vector<int> ab;
+    for (auto &q : rq) {
+        VecJob j{};
+        j.x = q.x;
+        j.w = q.y;
+        j.n = q.n;
+        j.br = q.b.real();
+        j.bi = q.b.imag();
+        j.b_zero = q.b == cd(0, 0);
+        j.acoef = q.coef;
+        j.cmode = q.cmode;
+        j.nb = ew_blocks(q.n);
+        a.push_back(j);
+        ab.push_back(j.nb);
+    }
+    launch_vec_jobs(dtype, VOP_AXPBY, a, ab, s);
+}
+
 // ------------------------------------------------ lockstep factorization --
 struct BFact {
     int op;
@@ -245,14 +280,20 @@ struct BFact {
 };
 
 // operators.jl:1655-1731 for every request at once; identical per-request
-// operation order to the single driver.
+// operation order to the single driver.  All steps of all requests are
+// queued with their scalars in device memory (FactScalars layout per
+// request); one copy at the end feeds KrylovProj::replay, which applies the
+// reference's bookkeeping and breakdown test in step order.
 void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dtype,
                 const std::vector<std::pair<int, const void *>> &req, int krylov_dim, BWork &wk, hipStream_t s,
                 std::vector<std::unique_ptr<BFact>> &F, std::vector<int> &nmv) {
     const size_t esz = dtype_size(dtype);
     const size_t R = req.size();
     F.clear();
-    std::vector<NormReq> nq;
+    std::vector<int64_t> ns;
+    std::vector<size_t> off;
+    size_t tot = 0;
+    int max_steps = 0;
     for (size_t i = 0; i < R; i++) {
         auto f = std::make_unique<BFact>();
         const OpHost &op = *ops[req[i].first];
@@ -265,88 +306,85 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         f->P.H.assign((size_t)(f->P.steps + 1) * f->P.steps, cd(0, 0));
         f->P.completed = f->P.steps;
         f->P.terminal = 0;
-        nq.push_back({req[i].second, f->n});
+        ns.push_back(f->n);
+        off.push_back(tot);
+        tot += FactScalars(f->P.steps).total();
+        max_steps = std::max(max_steps, f->P.steps);
         F.push_back(std::move(f));
     }
-    std::vector<double> b2;
-    bnorm2(dtype, nq, wk, s, b2);
-    std::vector<AxpbyReq> sc;
-    for (size_t i = 0; i < R; i++) {
-        F[i]->P.beta = std::sqrt(b2[i]);
-        sc.push_back({cd(1.0 / F[i]->P.beta, 0), req[i].second, cd(0, 0), F[i]->col(0, esz), F[i]->n});
+    bwork_reserve(wk, ns, max_steps + 1);
+    DevBuf<double> d(std::max<size_t>(tot, 1));
+    auto sc = [&](size_t i) { return FactScalars(F[i]->P.steps); };
+    {
+        std::vector<std::pair<const void *, int64_t>> xs;
+        std::vector<double *> outs;
+        std::vector<AxpbyDev> ax;
+        for (size_t i = 0; i < R; i++) {
+            xs.push_back({req[i].second, F[i]->n});
+            outs.push_back(d.get() + off[i] + sc(i).beta2());
+            ax.push_back({outs.back(), CO_INV_SQRT, req[i].second, cd(0, 0), F[i]->col(0, esz), F[i]->n});
+        }
+        bnorm2_dev(dtype, xs, outs, wk, s);
+        baxpby_dev(dtype, ax, s);
     }
-    baxpby(dtype, sc, s);
-    std::vector<char> active(R, 1);
-    std::vector<double> prev(R, 0.0);
-    int max_steps = 0;
-    for (auto &f : F) max_steps = std::max(max_steps, f->P.steps);
     for (int col = 0; col < max_steps; col++) {
         std::vector<size_t> act;
         for (size_t i = 0; i < R; i++)
-            if (active[i] && col < F[i]->P.steps) act.push_back(i);
+            if (col < F[i]->P.steps) act.push_back(i);
         if (act.empty()) break;
         std::vector<MatvecJob> mj;
         for (size_t i : act) mj.push_back({F[i]->op, F[i]->col(col, esz), F[i]->col(col + 1, esz)});
         launch_batch_matvec(bm, mj, s);
-        for (size_t i : act) nmv[i]++;
         std::vector<size_t> herm, arn;
         for (size_t i : act) (F[i]->P.hermitian ? herm : arn).push_back(i);
-        std::vector<double> tn(R, 0.0);
+        auto at = [&](size_t i, size_t o) { return d.get() + off[i] + o; };
         if (!herm.empty()) {
             // three-term Lanczos without reorthogonalisation
-            std::vector<AxpbyReq> ax;
+            std::vector<AxpbyDev> ax;
             for (size_t i : herm)
                 if (col > 0)
-                    ax.push_back({cd(-prev[i], 0), F[i]->col(col - 1, esz), cd(1, 0), F[i]->col(col + 1, esz), F[i]->n});
-            if (!ax.empty()) baxpby(dtype, ax, s);
-            std::vector<DotReq> dq;
-            for (size_t i : herm) dq.push_back({F[i]->col(col, esz), F[i]->n, 1, F[i]->col(col + 1, esz)});
-            std::vector<std::vector<cd>> hh;
-            bdots(dtype, dq, wk, s, hh);
-            std::vector<UpdReq> uq;
-            for (size_t t = 0; t < herm.size(); t++) {
-                const size_t i = herm[t];
-                uq.push_back({F[i]->col(col, esz), F[i]->n, 1, {hh[t][0]}, F[i]->col(col + 1, esz)});
+                    ax.push_back({at(i, sc(i).nrm2(col - 1)), CO_NEG_SQRT, F[i]->col(col - 1, esz), cd(1, 0),
+                                  F[i]->col(col + 1, esz), F[i]->n});
+            if (!ax.empty()) baxpby_dev(dtype, ax, s);
+            std::vector<DotDev> dq;
+            std::vector<UpdDev> uq;
+            for (size_t i : herm) {
+                dq.push_back({F[i]->col(col, esz), F[i]->n, 1, F[i]->col(col + 1, esz), at(i, sc(i).h1(col))});
+                uq.push_back({F[i]->col(col, esz), F[i]->n, 1, at(i, sc(i).h1(col)), F[i]->col(col + 1, esz),
+                              at(i, sc(i).nrm2(col))});
             }
-            std::vector<double> n2;
-            bupdate(dtype, uq, wk, s, n2);
-            for (size_t t = 0; t < herm.size(); t++) {
-                const size_t i = herm[t];
-                tn[i] = std::sqrt(n2[t]);
-                F[i]->P.lanczos_entry(col, hh[t][0], tn[i]);
-            }
+            bdots_dev(dtype, dq, wk, s);
+            bupdate_dev(dtype, uq, wk, s);
         }
         if (!arn.empty()) {
             // Arnoldi, two Gram-Schmidt passes
-            std::vector<DotReq> dq;
-            for (size_t i : arn) dq.push_back({F[i]->V.get(), F[i]->n, col + 1, F[i]->col(col + 1, esz)});
-            std::vector<std::vector<cd>> h1, h2;
-            bdots(dtype, dq, wk, s, h1);
-            std::vector<UpdReq> uq;
-            for (size_t t = 0; t < arn.size(); t++)
-                uq.push_back({F[arn[t]]->V.get(), F[arn[t]]->n, col + 1, h1[t], F[arn[t]]->col(col + 1, esz)});
-            std::vector<double> n2;
-            bupdate(dtype, uq, wk, s, n2);
-            bdots(dtype, dq, wk, s, h2);
-            for (size_t t = 0; t < arn.size(); t++) uq[t].h = h2[t];
-            bupdate(dtype, uq, wk, s, n2);
-            for (size_t t = 0; t < arn.size(); t++) {
-                const size_t i = arn[t];
-                tn[i] = std::sqrt(n2[t]);
-                F[i]->P.arnoldi_column(col, h1[t], h2[t], tn[i]);
+            std::vector<DotDev> d1, d2;
+            std::vector<UpdDev> u1, u2;
+            for (size_t i : arn) {
+                void *w = F[i]->col(col + 1, esz);
+                d1.push_back({F[i]->V.get(), F[i]->n, col + 1, w, at(i, sc(i).h1(col))});
+                d2.push_back({F[i]->V.get(), F[i]->n, col + 1, w, at(i, sc(i).h2(col))});
+                u1.push_back({F[i]->V.get(), F[i]->n, col + 1, at(i, sc(i).h1(col)), w, nullptr});
+                u2.push_back({F[i]->V.get(), F[i]->n, col + 1, at(i, sc(i).h2(col)), w, at(i, sc(i).nrm2(col))});
             }
+            bdots_dev(dtype, d1, wk, s);
+            bupdate_dev(dtype, u1, wk, s);
+            bdots_dev(dtype, d2, wk, s);
+            bupdate_dev(dtype, u2, wk, s);
         }
-        std::vector<AxpbyReq> sc2;
-        for (size_t i : act) {
-            if (F[i]->P.breakdown(col, tn[i])) {
-                active[i] = 0;
-                continue;
-            }
+        std::vector<AxpbyDev> sc2;
+        for (size_t i : act)
             if (col + 1 < F[i]->P.steps)
-                sc2.push_back({cd(1.0 / tn[i], 0), F[i]->col(col + 1, esz), cd(0, 0), F[i]->col(col + 1, esz), F[i]->n});
-            prev[i] = tn[i];
-        }
-        if (!sc2.empty()) baxpby(dtype, sc2, s);
+                sc2.push_back({at(i, sc(i).nrm2(col)), CO_INV_SQRT, F[i]->col(col + 1, esz), cd(0, 0),
+                               F[i]->col(col + 1, esz), F[i]->n});
+        if (!sc2.empty()) baxpby_dev(dtype, sc2, s);
+    }
+    std::vector<double> host(tot);
+    if (tot) QS_HIP(hipMemcpyAsync(host.data(), d.get(), sizeof(double) * tot, hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < R; i++) {
+        std::vector<double> mine(host.begin() + off[i], host.begin() + off[i] + sc(i).total());
+        nmv[i] += F[i]->P.replay(sc(i), mine);
     }
 }
 

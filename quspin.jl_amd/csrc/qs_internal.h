@@ -6,6 +6,7 @@
 #include <complex>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -148,6 +149,24 @@ enum BasisMode : int {
     MODE_STORED = 2  // general group: sorted canonical states stored on the device
 };
 
+// Tables of the split matvec kernel for an implicit basis with a particle
+// number (built on first use): H low bits, W = L - H high bits.
+struct SplitBasis {
+    int H = 0, W = 0;
+    int nblocks = 0, maxblock = 0;
+    int cls_off[34] = {}, cls_n[34] = {};
+    DevBuf<uint32_t> blist;    // high words with a nonempty block, ascending
+    DevBuf<uint32_t> boff;     // first row of the block of each high word (2^W; ~0 if none)
+    DevBuf<uint16_t> rank;     // rank of every H-bit word among the words of its popcount
+    DevBuf<uint16_t> words;    // H-bit words by (popcount, value)
+};
+// Hop groups of one term list classified for the split kernel: inside the
+// low word, inside the high word, across the split (in this order).
+struct SplitHops {
+    int nlow = 0, nhigh = 0, ncross = 0;
+    DevBuf<HopDev> d;
+};
+
 struct BasisHost {
     int L = 0, nup = -1, pauli = 1, a = 1;
     int blocks[6];               // k, p, z, pz, zA, zB (QS_UNSET if absent)
@@ -190,6 +209,9 @@ struct BasisHost {
     std::vector<uint64_t> labels;
     std::vector<double> ref_norm;
     bool labels_ready = false;
+    // split-kernel tables (built once, on first use)
+    mutable std::once_flag split_once;
+    mutable std::unique_ptr<SplitBasis> split;
 };
 
 struct OpHost {
@@ -212,6 +234,9 @@ struct OpHost {
     DevBuf<HopDev> d_hops, d_hops_adj;
     DevBuf<DiagClass> d_dcls, d_dcls_adj;
     DevBuf<TermDev> d_drest, d_drest_adj;
+    // hop groups classified for the split kernel (built once, on first use)
+    mutable std::once_flag split_once;
+    mutable SplitHops split_hops, split_hops_adj;
     // user diagonal
     std::vector<cd> user_diag;
     bool has_user_diag = false;
@@ -279,6 +304,8 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
 int matvec_grid(int64_t n);
 // the two-bit hop kernel applies to this operator (plain or adjoint list)
 bool hop_eligible(const OpHost &op, bool adjoint);
+// the split kernel applies (hop couplings, implicit basis with a particle number)
+bool split_eligible(const OpHost &op, bool adjoint);
 // <x, A x> in FP64 with x stored in `dtype` (fused into the matvec epilogue); y is scratch
 cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk);
 
@@ -287,6 +314,10 @@ cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk
 // grid's blocks and sees exactly the block partition of its own single
 // launch, so its results are bitwise those of the single launch.
 enum VecOp { VOP_GRAM = 0, VOP_REDUCE, VOP_UPDATE, VOP_NORM2, VOP_AXPBY, VOP_COMBINE };
+// Axpby coefficient a taken from device memory c (a reduction result):
+// CO_NEG a = -(c0 + i c1), CO_NEG_RE a = -c0, CO_NEG_SQRT a = -sqrt(c0),
+// CO_INV_SQRT a = 1 / sqrt(c0); CO_HOST: a is the job's (ar, ai).
+enum CoefMode { CO_HOST = 0, CO_NEG, CO_NEG_RE, CO_NEG_SQRT, CO_INV_SQRT };
 struct VecJob {
     const void *V;      // basis block (column-major, ld n)
     const void *x;      // input vector
@@ -296,6 +327,8 @@ struct VecJob {
     int64_t n;
     int m, nb, b0, k, b_zero;
     double ar, ai, br, bi;
+    const double *acoef;   // axpby: device coefficient source (CoefMode cmode)
+    int cmode;
 };
 constexpr int kMaxVecJobs = 24;
 struct VecJobs {
@@ -362,6 +395,18 @@ void gram_dots(int dtype, const void *V, int64_t n, int m, const void *w, VecWor
 double gram_update(int dtype, const void *V, int64_t n, int m, const std::vector<cd> &h, void *w,
                    VecWork &wk, hipStream_t s);
 double vec_norm2(int dtype, const void *x, int64_t n, VecWork &wk, hipStream_t s);
+// Device-scalar forms: results stay in device memory (out / nrm2_out, two
+// doubles per value), coefficients are read from there; nothing waits for
+// the host.  vec_reserve sizes wk for up to mmax Gram columns beforehand
+// (the workspace must not be reallocated while launches are queued).
+void vec_reserve(VecWork &wk, int64_t n, int mmax);
+void gram_dots_dev(int dtype, const void *V, int64_t n, int m, const void *w, VecWork &wk, double *out,
+                   hipStream_t s);
+void gram_update_dev(int dtype, const void *V, int64_t n, int m, const double *h, void *w, VecWork &wk,
+                     double *nrm2_out, hipStream_t s);
+void vec_norm2_dev(int dtype, const void *x, int64_t n, VecWork &wk, double *out, hipStream_t s);
+void vec_axpby_dev(int dtype, const double *acoef, int cmode, const void *x, cd b, void *y, int64_t n,
+                   hipStream_t s);
 void vec_scale_copy(int dtype, const void *x, void *y, int64_t n, cd scale, hipStream_t s);
 void vec_axpby(int dtype, cd a, const void *x, cd b, void *y, int64_t n, hipStream_t s);
 // Y[:, :k] = V[:, :m] C  (C is m x k column-major, complex or real)
@@ -376,6 +421,20 @@ bool dtype_complex(int dtype);
 // (steps+1) x steps projected matrix and the quantities the reference's
 // a-posteriori estimate needs.  Shared by the single and batched drivers so
 // both do the same host arithmetic.
+// Device layout of one factorization's scalars (doubles): |v|^2, then per
+// column the two Gram passes' coefficients (steps + 1 complex each) and the
+// squared norm of the new residual.
+struct FactScalars {
+    int steps;
+    explicit FactScalars(int st) : steps(st) {}
+    size_t per() const { return (size_t)4 * (steps + 1) + 2; }
+    size_t beta2() const { return 0; }
+    size_t h1(int col) const { return 2 + (size_t)col * per(); }
+    size_t h2(int col) const { return h1(col) + (size_t)2 * (steps + 1); }
+    size_t nrm2(int col) const { return h2(col) + (size_t)2 * (steps + 1); }
+    size_t total() const { return 2 + (size_t)steps * per(); }
+};
+
 struct KrylovProj {
     std::vector<cd> H;   // row-major (steps+1) x steps
     int steps = 0, completed = 0;
@@ -387,6 +446,10 @@ struct KrylovProj {
     void arnoldi_column(int col, const std::vector<cd> &h1, const std::vector<cd> &h2, double tn);
     // records tn; true (and completed = col + 1) on breakdown
     bool breakdown(int col, double tn);
+    // the per-step bookkeeping of a factorization queued with its scalars on
+    // the device (beta, H entries, breakdown test in step order); returns the
+    // number of steps the reference performs (= matvecs)
+    int replay(const FactScalars &L, const std::vector<double> &d);
     // beta exp(scale H_c)[:, 0] and the error estimate (_krylov_project)
     double expm_column(cd scale, std::vector<cd> &coef) const;
     // Hermitian: coefficients for every scale from one eigen-decomposition

@@ -117,38 +117,67 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     std::vector<int> wanted;
     int nconv = 0;
     uint64_t reseed = 1;
+    // per-step scalars of one Lanczos cycle, device side: two Gram passes
+    // (m + 1 complex each) and |w|^2
+    const size_t per = (size_t)4 * (m + 1) + 2;
+    DevBuf<double> d(per * m);
+    std::vector<double> host(per * m);
+    vec_reserve(wk, n, m + 1);
     for (int iter = 0; iter < maxiter; iter++) {
-        for (int j = kept; j < m; j++) {
-            void *w = K.col(j + 1);
-            matvec(op, dtype, K.col(j), w);
-            res.nmatvec++;
-            std::vector<cd> h;
-            double beta = orthogonalize(K, j, w, wk, h, s);
-            for (int i = kept; i <= j; i++) {
-                // the Rayleigh quotient entries of the new column; entries
-                // below `kept` are the arrow couplings set at restart
-                t(i, j) = h[i].real();
-                t(j, i) = h[i].real();
+        int j0 = kept;
+        while (j0 < m) {
+            // queue steps j0 .. m-1 with no host round trip
+            for (int j = j0; j < m; j++) {
+                void *w = K.col(j + 1);
+                double *h1 = d.get() + per * j, *h2 = h1 + 2 * (m + 1), *n2 = h2 + 2 * (m + 1);
+                matvec(op, dtype, K.col(j), w);
+                gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h1, s);
+                gram_update_dev(dtype, K.V.get(), n, j + 1, h1, w, wk, nullptr, s);
+                gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h2, s);
+                gram_update_dev(dtype, K.V.get(), n, j + 1, h2, w, wk, n2, s);
+                vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
             }
-            double tnorm = 0;
-            for (double v : T) tnorm = std::max(tnorm, std::fabs(v));
-            if (beta <= 1e3 * kEps * std::max(1.0, tnorm)) {
-                // invariant subspace: continue with a fresh direction
-                beta = 0;
-                if (j + 1 < m + 1) {
+            QS_HIP(hipMemcpyAsync(host.data() + per * j0, d.get() + per * j0, sizeof(double) * per * (m - j0),
+                                  hipMemcpyDeviceToHost, s));
+            QS_HIP(hipStreamSynchronize(s));
+            // the step-by-step bookkeeping, in order; a breakdown discards the
+            // steps queued after it
+            int next = m;
+            for (int j = j0; j < m; j++) {
+                res.nmatvec++;
+                const double *h1 = host.data() + per * j, *h2 = h1 + 2 * (m + 1);
+                std::vector<cd> h(j + 1);
+                for (int i = 0; i <= j; i++) h[i] = cd(h1[2 * i], h1[2 * i + 1]) + cd(h2[2 * i], h2[2 * i + 1]);
+                double beta = std::sqrt(h2[2 * (m + 1)]);
+                for (int i = kept; i <= j; i++) {
+                    // the Rayleigh quotient entries of the new column; entries
+                    // below `kept` are the arrow couplings set at restart
+                    t(i, j) = h[i].real();
+                    t(j, i) = h[i].real();
+                }
+                double tnorm = 0;
+                for (double v : T) tnorm = std::max(tnorm, std::fabs(v));
+                const bool brk = beta <= 1e3 * kEps * std::max(1.0, tnorm);
+                if (brk) {
+                    // invariant subspace: continue with a fresh direction
+                    beta = 0;
+                    void *w = K.col(j + 1);
                     vec_random(dtype, w, n, 0x9e37ULL + (reseed++), s);
                     std::vector<cd> hh;
                     double b2 = orthogonalize(K, j, w, wk, hh, s);
                     vec_scale_copy(dtype, w, w, n, cd(1.0 / b2, 0), s);
                 }
-            } else {
-                vec_scale_copy(dtype, w, w, n, cd(1.0 / beta, 0), s);
+                if (j + 1 < m) {
+                    t(j, j + 1) = beta;
+                    t(j + 1, j) = beta;
+                }
+                beta_last = beta;
+                if (brk) {
+                    next = j + 1;
+                    break;
+                }
             }
-            if (j + 1 < m) {
-                t(j, j + 1) = beta;
-                t(j + 1, j) = beta;
-            }
-            beta_last = beta;
+            j0 = next;
         }
         // Rayleigh-Ritz on the m x m projected matrix
         std::vector<double> A((size_t)m * m);
@@ -236,7 +265,12 @@ struct Factorization {
     KrylovProj P;
 };
 
-// operators.jl:1655-1731
+// operators.jl:1655-1731.  Every step is queued on the stream with its
+// scalars (h, |w|^2) kept in device memory; one copy brings them to the host
+// at the end, where the reference's per-step bookkeeping and breakdown test
+// run in the original order (KrylovProj::replay).  The test at step c only
+// reads quantities of steps <= c, so testing afterwards stops at the same
+// step; columns queued past a breakdown are never used.
 void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorization &F, VecWork &wk, int &nmv) {
     hipStream_t s = op_stream(op);
     const int64_t n = op.basis->Ns;
@@ -247,38 +281,36 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
     F.K = std::make_unique<Krylov>(dtype, n, P.steps + 1);
     Krylov &K = *F.K;
     P.H.assign((size_t)(P.steps + 1) * P.steps, cd(0, 0));
-    P.beta = std::sqrt(vec_norm2(dtype, v, n, wk, s));
-    vec_scale_copy(dtype, v, K.col(0), n, cd(1.0 / P.beta, 0), s);
     P.completed = P.steps;
     P.terminal = 0;
-    double prev = 0;
-    for (int col = 0; col < P.steps; col++) {
+    const int steps = P.steps;
+    vec_reserve(wk, n, steps + 1);
+    FactScalars L(steps);
+    DevBuf<double> d(L.total());
+    vec_norm2_dev(dtype, v, n, wk, d.get() + L.beta2(), s);
+    vec_axpby_dev(dtype, d.get() + L.beta2(), CO_INV_SQRT, v, cd(0, 0), K.col(0), n, s);
+    for (int col = 0; col < steps; col++) {
         void *w = K.col(col + 1);
+        double *h1 = d.get() + L.h1(col), *h2 = d.get() + L.h2(col), *n2 = d.get() + L.nrm2(col);
         matvec(op, dtype, K.col(col), w);
-        nmv++;
-        double tn;
         if (P.hermitian) {
             // three-term Lanczos without reorthogonalisation
-            if (col > 0) vec_axpby(dtype, cd(-prev, 0), K.col(col - 1), cd(1, 0), w, n, s);
-            std::vector<cd> hh;
-            gram_dots(dtype, K.col(col), n, 1, w, wk, s, hh);
-            std::vector<cd> one{hh[0]};
-            tn = std::sqrt(gram_update(dtype, K.col(col), n, 1, one, w, wk, s));
-            P.lanczos_entry(col, hh[0], tn);
+            if (col > 0) vec_axpby_dev(dtype, d.get() + L.nrm2(col - 1), CO_NEG_SQRT, K.col(col - 1), cd(1, 0), w, n, s);
+            gram_dots_dev(dtype, K.col(col), n, 1, w, wk, h1, s);
+            gram_update_dev(dtype, K.col(col), n, 1, h1, w, wk, n2, s);
         } else {
             // Arnoldi, two Gram-Schmidt passes
-            std::vector<cd> h1, h2;
-            gram_dots(dtype, K.V.get(), n, col + 1, w, wk, s, h1);
-            gram_update(dtype, K.V.get(), n, col + 1, h1, w, wk, s);
-            gram_dots(dtype, K.V.get(), n, col + 1, w, wk, s, h2);
-            double nr2 = gram_update(dtype, K.V.get(), n, col + 1, h2, w, wk, s);
-            tn = std::sqrt(nr2);
-            P.arnoldi_column(col, h1, h2, tn);
+            gram_dots_dev(dtype, K.V.get(), n, col + 1, w, wk, h1, s);
+            gram_update_dev(dtype, K.V.get(), n, col + 1, h1, w, wk, nullptr, s);
+            gram_dots_dev(dtype, K.V.get(), n, col + 1, w, wk, h2, s);
+            gram_update_dev(dtype, K.V.get(), n, col + 1, h2, w, wk, n2, s);
         }
-        if (P.breakdown(col, tn)) break;
-        if (col + 1 < P.steps) vec_scale_copy(dtype, w, w, n, cd(1.0 / tn, 0), s);
-        prev = tn;
+        if (col + 1 < steps) vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
     }
+    std::vector<double> host(L.total());
+    QS_HIP(hipMemcpyAsync(host.data(), d.get(), sizeof(double) * host.size(), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    nmv += P.replay(L, host);
 }
 
 // operators.jl:1733-1756: result = beta * V[:, :c] exp(scale H_c)[:, 0]
@@ -388,6 +420,22 @@ bool KrylovProj::breakdown(int col, double tn) {
     return false;
 }
 
+int KrylovProj::replay(const FactScalars &L, const std::vector<double> &d) {
+    beta = std::sqrt(d[L.beta2()]);
+    for (int col = 0; col < steps; col++) {
+        const double tn = std::sqrt(d[L.nrm2(col)]);
+        auto hv = [&](size_t off, int count) {
+            std::vector<cd> h(count);
+            for (int i = 0; i < count; i++) h[i] = cd(d[off + 2 * i], d[off + 2 * i + 1]);
+            return h;
+        };
+        if (hermitian) lanczos_entry(col, cd(d[L.h1(col)], d[L.h1(col) + 1]), tn);
+        else arnoldi_column(col, hv(L.h1(col), col + 1), hv(L.h2(col), col + 1), tn);
+        if (breakdown(col, tn)) return col + 1;
+    }
+    return steps;
+}
+
 double KrylovProj::expm_column(cd scale, std::vector<cd> &coef) const {
     const int c = completed;
     std::vector<cd> A((size_t)c * c), E;
@@ -429,7 +477,9 @@ void KrylovProj::lanczos_times(const std::vector<cd> &scales, std::vector<cd> &c
 
 namespace qs {
 
-// lanczos_full / _lanczos_coefficients (tools.jl:2742-2850), step for step:
+// lanczos_full / _lanczos_coefficients (tools.jl:2742-2850), step for step
+// (queued with the scalars on the device; the stopping rule is applied
+// afterwards in step order, and columns of Q past `used` are scratch):
 // q = v0 / |v0|, residual = (A v0) / |v0|; for each index: store q, residual
 // = A q - beta_prev q_prev (index > 1), alpha = Re <q, residual>, residual -=
 // alpha q, optional sequential projection against every stored row, then
@@ -452,37 +502,46 @@ void lanczos_device(OpHost &op, int dtype, const void *v0, int m, bool full_orth
     vec_scale_copy(dtype, v0, col(0), n, cd(1.0 / nrm, 0), s);
     alphas.assign(m, 0.0);
     betas.assign(std::max(0, m - 1), 0.0);
-    double beta_prev = 0;
     used = 0;
+    // every step queued with its scalars in device memory: per index the
+    // alpha dot, |res|^2 and (full_ortho) the m projection dots
+    const size_t per = (size_t)2 * (m + 2);
+    DevBuf<double> d(per * m);
+    vec_reserve(wk, n, 1);
     for (int index = 0; index < m; index++) {
-        used = index + 1;
+        double *ha = d.get() + per * index, *n2 = ha + 2, *hp = ha + 4;
         void *q = col(index);
         if (index > 0) {
             matvec(op, dtype, q, res.get());
-            vec_axpby(dtype, cd(-beta_prev, 0), col(index - 1), cd(1, 0), res.get(), n, s);
+            vec_axpby_dev(dtype, d.get() + per * (index - 1) + 2, CO_NEG_SQRT, col(index - 1), cd(1, 0), res.get(),
+                          n, s);
         }
-        std::vector<cd> h;
-        gram_dots(dtype, q, n, 1, res.get(), wk, s, h);
-        const double alpha = h[0].real();
-        vec_axpby(dtype, cd(-alpha, 0), q, cd(1, 0), res.get(), n, s);
+        gram_dots_dev(dtype, q, n, 1, res.get(), wk, ha, s);
+        vec_axpby_dev(dtype, ha, CO_NEG_RE, q, cd(1, 0), res.get(), n, s);
         if (full_ortho) {
             for (int p = 0; p <= index; p++) {
-                std::vector<cd> hp;
-                gram_dots(dtype, col(p), n, 1, res.get(), wk, s, hp);
-                vec_axpby(dtype, -hp[0], col(p), cd(1, 0), res.get(), n, s);
+                gram_dots_dev(dtype, col(p), n, 1, res.get(), wk, hp + 2 * p, s);
+                vec_axpby_dev(dtype, hp + 2 * p, CO_NEG, col(p), cd(1, 0), res.get(), n, s);
             }
         }
-        alphas[index] = alpha;
         if (index + 1 == m) break;
-        const double beta = std::sqrt(vec_norm2(dtype, res.get(), n, wk, s));
+        vec_norm2_dev(dtype, res.get(), n, wk, n2, s);
+        vec_axpby_dev(dtype, n2, CO_INV_SQRT, res.get(), cd(0, 0), col(index + 1), n, s);
+    }
+    std::vector<double> host(per * m);
+    QS_HIP(hipMemcpyAsync(host.data(), d.get(), sizeof(double) * host.size(), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    // the reference's stopping rule, in step order
+    for (int index = 0; index < m; index++) {
+        used = index + 1;
+        alphas[index] = host[per * index];
+        if (index + 1 == m) break;
+        const double beta = std::sqrt(host[per * index + 2]);
         if (beta <= tol) break;
         betas[index] = beta;
-        vec_scale_copy(dtype, res.get(), col(index + 1), n, cd(1.0 / beta, 0), s);
-        beta_prev = beta;
     }
     alphas.resize(used);
     betas.resize(std::max(0, used - 1));
-    QS_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace qs

@@ -709,6 +709,253 @@ __global__ __launch_bounds__(kMvBlock) void kb_matvec_hop(MVJobs J) {
                                       (int)blockIdx.x - q.b0, q.nb, smem);
 }
 
+// ------------------------------------------------------- split kernel --
+// Implicit bases with a particle number (no symmetry, or spin inversion at
+// half filling) and two-bit hop couplings.  A state splits into its high
+// word h = s >> H and low word l = s & (2^H - 1).  In colex order the rows
+// sharing h form one contiguous block of C(H, p) rows (p = nup - popcount h)
+// ordered by the rank of l among the p-bit words, starting at boff[h]:
+//  * a hop inside h moves whole blocks: row r of block h pulls row r of
+//    block h ^ flip (row C(H,p)-1-r of the inverted block when the hop sets
+//    bit L-1 under spin inversion).  Liveness and amplitude are the same for
+//    the whole block and the reads are contiguous; a row issues these loads
+//    kSpNB at a time;
+//  * a hop inside l stays in the block, staged in LDS: the neighbour's row
+//    is rank[l ^ flip] from an LDS table over every H-bit word;
+//  * a hop across the split changes p and reaches one whole block boff[h']
+//    at rows rank[l'] in every order: that block is staged in LDS beside the
+//    own one (up to the LDS budget; further ones are gathered from HBM).
+// One workgroup owns one block at a time; each XCD walks a contiguous range
+// of blocks, so the blocks its low high-bit hops reach are still in its L2.
+constexpr int kSpBlock = 512;
+constexpr int kSpMaxHops = 64;   // live hops of one block kept in LDS
+
+struct SplitDev {
+    int H;
+    uint32_t lmask, wmask;
+    const uint32_t *blist;
+    int nblocks;
+    const uint32_t *boff;
+    const uint16_t *rank;
+    const uint16_t *words;
+    int cls_off[34], cls_n[34];
+    int nlow, nhigh, ncross;
+    int nstage;                  // cross blocks staged in LDS
+    int xstride;                 // elements per staged block (max block + 1)
+    int off_rank, off_words, off_x, off_list;   // dynamic LDS layout (bytes)
+};
+
+// per-block list of live high / cross hops (LDS)
+struct SpHop {
+    int64_t off;       // first row of the neighbour block
+    double re, im;     // weight (conjugated amplitude, inversion character applied)
+    int rev;           // high: read row n-1-r; cross: the neighbour's low word is inverted
+    int lo, bhi;       // cross: the low bit and the high bit's value in h
+    int slot;          // cross: staged block index (1 + k), or -1 for HBM gathers
+    int n2;            // rows of the neighbour block
+};
+
+template <typename V>
+__device__ __forceinline__ void acc_nb(double wr, double wi, double vr, double vi, double &ar, double &ai) {
+    if (VTraits<V>::cplx) {
+        ar += wr * vr - wi * vi;
+        ai += wr * vi + wi * vr;
+    } else {
+        ar += wr * vr;
+    }
+}
+
+template <typename V, bool CC, int CM>
+__global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D, const HopDev *__restrict__ hops,
+                                                           const DiagClass *__restrict__ dcls,
+                                                           const TermDev *__restrict__ drest,
+                                                           const V *__restrict__ x, V *__restrict__ y) {
+    using T = VTraits<V>;
+    constexpr int kNBH = T::cplx ? 4 : 8;   // high-hop loads in flight per row
+    extern __shared__ __align__(16) char smem[];
+    uint16_t *s_rank = (uint16_t *)(smem + D.off_rank);
+    uint16_t *s_words = (uint16_t *)(smem + D.off_words);
+    V *s_x = (V *)(smem + D.off_x);
+    SpHop *s_list = (SpHop *)(smem + D.off_list);
+    __shared__ int s_nh, s_nc;
+    const int nw = 1 << D.H;
+    for (int i = threadIdx.x; i < nw; i += kSpBlock) {
+        s_rank[i] = D.rank[i];
+        s_words[i] = D.words[i];
+    }
+    // the grid is a multiple of 8 and hardware places block b on XCD b % 8
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+    const int chunk = (D.nblocks + 7) >> 3;
+    const int b0 = xcd * chunk;
+    const int b1 = min(D.nblocks, b0 + chunk);
+    double dot_r = 0, dot_i = 0;
+    for (int vb = b0 + (int)(blockIdx.x >> 3); vb < b1; vb += per) {
+        const uint32_t h = D.blist[vb];
+        const uint32_t off = D.boff[h];
+        const int p = A.nup - __popc(h);
+        const int n = D.cls_n[p];
+        const int cw = D.cls_off[p];
+        __syncthreads();   // the previous block is done with s_x and s_list
+        // the block's live hop lists, one hop group per lane (wave 0: high
+        // hops, wave 1: cross hops), compacted in group order by ballot
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (wave == 0) {
+            bool live = false;
+            SpHop e{};
+            if (lane < D.nhigh) {
+                const HopDev hp = hops[D.nlow + lane];
+                const int lo = hp.lo - D.H, hi = hp.hi - D.H;
+                const uint32_t blo = (h >> lo) & 1, bhi = (h >> hi) & 1;
+                live = (hp.nzmask >> (blo | (bhi << 1))) & 1;
+                double m_r = hp.sym ? hp.re[1] : (blo ? hp.re[1] : hp.re[2]);
+                double m_i = CC ? (hp.sym ? hp.im[1] : (blo ? hp.im[1] : hp.im[2])) : 0.0;
+                uint32_t h2 = h ^ ((1u << lo) | (1u << hi));
+                int rev = 0;
+                if (CM == CM_ZONLY && hp.top) {
+                    // the hop sets bit L-1: the column is the inverted state's
+                    h2 = ~h2 & D.wmask;
+                    rev = 1;
+                    if (A.zchar < 0) {
+                        m_r = -m_r;
+                        m_i = -m_i;
+                    }
+                }
+                if (live) e.off = D.boff[h2];
+                e.re = m_r;
+                e.im = -m_i;   // conj(amp): the terms are the adjoint couplings
+                e.rev = rev;
+                e.slot = -1;
+                e.n2 = n;
+            }
+            const uint64_t m = __ballot(live);
+            if (live) s_list[__popcll(m & ((1ULL << lane) - 1))] = e;
+            if (lane == 0) s_nh = __popcll(m);
+        } else if (wave == 1) {
+            // cross hops: the rows whose bit lo differs from h's bit hi move
+            bool live = false;
+            SpHop e{};
+            if (lane < D.ncross) {
+                const HopDev hp = hops[D.nlow + D.nhigh + lane];
+                const int hi = hp.hi - D.H;
+                const uint32_t bhi = (h >> hi) & 1;
+                const int pat = bhi ? 2 : 1;
+                double m_r = hp.re[pat];
+                double m_i = CC ? hp.im[pat] : 0.0;
+                uint32_t h2 = h ^ (1u << hi);
+                const int inv = CM == CM_ZONLY && hp.top;
+                if (inv) {
+                    h2 = ~h2 & D.wmask;
+                    if (A.zchar < 0) {
+                        m_r = -m_r;
+                        m_i = -m_i;
+                    }
+                }
+                const int p2 = A.nup - __popc(h2);
+                live = ((hp.nzmask >> pat) & 1) && p2 >= 0 && p2 <= D.H;
+                const uint32_t o2 = live ? D.boff[h2] : 0xFFFFFFFFu;
+                live = live && o2 != 0xFFFFFFFFu;   // else no row of this block moves that way
+                e.off = o2;
+                e.re = m_r;
+                e.im = -m_i;
+                e.rev = inv;
+                e.lo = hp.lo;
+                e.bhi = (int)bhi;
+                e.n2 = live ? D.cls_n[p2] : 0;
+            }
+            const uint64_t m = __ballot(live);
+            const int pos = __popcll(m & ((1ULL << lane) - 1));
+            e.slot = pos < D.nstage ? 1 + pos : -1;
+            if (live) s_list[kSpMaxHops + pos] = e;
+            if (lane == 0) s_nc = __popcll(m);
+        }
+        __syncthreads();
+        const int nh = s_nh, nc = s_nc;
+        // stage the own block and the staged cross blocks (all loads issued together)
+        for (int i = threadIdx.x; i < D.xstride; i += kSpBlock) {
+            double a = 0, b = 0;
+            if (i < n) T::load(x, (int64_t)off + i, a, b);
+            if (A.conj_x) b = -b;
+            T::store(s_x, i, a, b);   // s_x[n] = 0: dead neighbours read it
+            for (int c = 0; c < nc; c++) {
+                const SpHop &e = s_list[kSpMaxHops + c];
+                if (e.slot < 0) continue;
+                double u = 0, v = 0;
+                if (i < e.n2) T::load(x, e.off + i, u, v);
+                if (A.conj_x) v = -v;
+                T::store(s_x, (int64_t)e.slot * D.xstride + i, u, v);
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += kSpBlock) {
+            const uint32_t l = s_words[cw + r];
+            double ar = 0, ai = 0;
+            {
+                double dr = 0, di = 0;
+                diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
+                di = -di;   // conj(amp): the terms are the adjoint couplings
+                if (A.user_diag) {
+                    const int64_t row = (int64_t)off + r;
+                    dr += A.user_diag[2 * row];
+                    di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+                }
+                double xr, xi;
+                T::load(s_x, r, xr, xi);
+                acc_nb<V>(dr, di, xr, xi, ar, ai);
+            }
+            // high hops: whole blocks from HBM, kNBH loads in flight
+            for (int k0 = 0; k0 < nh; k0 += kNBH) {
+                double vr[kNBH], vi[kNBH];
+#pragma unroll
+                for (int kk = 0; kk < kNBH; kk++) {
+                    vr[kk] = 0;
+                    vi[kk] = 0;
+                    if (k0 + kk < nh) {
+                        const SpHop &e = s_list[k0 + kk];
+                        T::load(x, e.off + (e.rev ? n - 1 - r : r), vr[kk], vi[kk]);
+                    }
+                }
+#pragma unroll
+                for (int kk = 0; kk < kNBH; kk++) {
+                    if (k0 + kk < nh) {
+                        const SpHop &e = s_list[k0 + kk];
+                        acc_nb<V>(e.re, e.im, vr[kk], A.conj_x ? -vi[kk] : vi[kk], ar, ai);
+                    }
+                }
+            }
+            // low hops: neighbours in the staged own block
+            for (int g = 0; g < D.nlow; g++) {
+                const HopDev hp = hops[g];
+                const uint32_t blo = (l >> hp.lo) & 1, bhi = (l >> hp.hi) & 1;
+                const bool live = (hp.nzmask >> (blo | (bhi << 1))) & 1;
+                // with a particle number only patterns 1 and 2 are live
+                const double m_r = hp.sym ? hp.re[1] : (blo ? hp.re[1] : hp.re[2]);
+                const double m_i = CC ? (hp.sym ? hp.im[1] : (blo ? hp.im[1] : hp.im[2])) : 0.0;
+                const int nid = live ? (int)s_rank[l ^ ((1u << hp.lo) | (1u << hp.hi))] : n;
+                double vr, vi;
+                T::load(s_x, nid, vr, vi);
+                acc_nb<V>(m_r, -m_i, vr, vi, ar, ai);
+            }
+            // cross hops: staged neighbour blocks (or HBM gathers past the budget)
+            for (int c = 0; c < nc; c++) {
+                const SpHop &e = s_list[kSpMaxHops + c];
+                if ((int)((l >> e.lo) & 1) == e.bhi) continue;
+                uint32_t l2 = l ^ (1u << e.lo);
+                if (e.rev) l2 = ~l2 & D.lmask;
+                const int idx = s_rank[l2];
+                double vr, vi;
+                if (e.slot >= 0) {
+                    T::load(s_x, (int64_t)e.slot * D.xstride + idx, vr, vi);
+                } else {
+                    T::load(x, e.off + idx, vr, vi);
+                    if (A.conj_x) vi = -vi;
+                }
+                acc_nb<V>(e.re, e.im, vr, vi, ar, ai);
+            }
+            finish_row<V>(A, y, (int64_t)off + r, ar, ai, 0.0, 0.0, dot_r, dot_i);
+        }
+    }
+}
+
 struct Launch {
     int grid;
     size_t lds;
@@ -844,6 +1091,13 @@ bool hop_eligible(const OpHost &op, bool adjoint) {
     return true;
 }
 
+bool split_eligible(const OpHost &op, bool adjoint) {
+    const BasisHost &b = *op.basis;
+    const FastTerms &f = adjoint ? op.fast_adj : op.fast;
+    return f.hop_ok && !b.use64 && b.nup >= 0 && b.L >= 8 && (b.mode == MODE_FULL || b.mode == MODE_ZONLY) &&
+           b.Ns < ((int64_t)1 << 31);
+}
+
 // blocks of a matvec launch that carries FP64 dot partials (one per block):
 // the stored kernel at LPR = 8, and the implicit kernel sized the same way
 int matvec_grid(int64_t n) { return grid_for(n, kStBlock / 8); }
@@ -921,6 +1175,164 @@ size_t hop_args(MVArgs &a, const OpHost &op, bool adjoint, bool dot_partial) {
     return off;
 }
 
+// split-kernel tables of the basis (once per basis)
+const SplitBasis &split_basis(const BasisHost &b) {
+    std::call_once(b.split_once, [&] {
+        auto t = std::make_unique<SplitBasis>();
+        const int L = b.L, nup = b.nup;
+        const int H = L >= 24 ? 12 : std::max(2, L / 2);
+        const int W = L - H;
+        t->H = H;
+        t->W = W;
+        std::vector<uint16_t> words, rank((size_t)1 << H);
+        for (int p = 0; p <= H; p++) {
+            t->cls_off[p] = (int)words.size();
+            for (uint32_t w = 0; w < (1u << H); w++)
+                if (__builtin_popcount(w) == p) {
+                    rank[w] = (uint16_t)(words.size() - t->cls_off[p]);
+                    words.push_back((uint16_t)w);
+                }
+            t->cls_n[p] = (int)words.size() - t->cls_off[p];
+            t->maxblock = std::max(t->maxblock, t->cls_n[p]);
+        }
+        std::vector<uint32_t> boff((size_t)1 << W, 0xFFFFFFFFu), blist;
+        int64_t next = 0;
+        for (uint32_t h = 0; h < (1u << W); h++) {
+            const int p = nup - __builtin_popcount(h);
+            if (p < 0 || p > H) continue;
+            if (b.mode == MODE_ZONLY && ((h >> (W - 1)) & 1)) continue;
+            // colex rank of the block's first state: h's particles sit above
+            // the p lowest ones
+            int64_t off = 0;
+            int k = p;
+            for (int i = 0; i < W; i++)
+                if ((h >> i) & 1) off += binom(H + i, ++k);
+            if (off != next) fail(QS_EARG, "internal: split blocks are not contiguous");
+            boff[h] = (uint32_t)off;
+            blist.push_back(h);
+            next += t->cls_n[p];
+        }
+        if (next != b.Ns) fail(QS_EARG, "internal: split blocks do not cover the basis");
+        t->nblocks = (int)blist.size();
+        t->blist.alloc(blist.size());
+        t->boff.alloc(boff.size());
+        t->rank.alloc(rank.size());
+        t->words.alloc(words.size());
+        QS_HIP(hipMemcpy(t->blist.get(), blist.data(), blist.size() * 4, hipMemcpyHostToDevice));
+        QS_HIP(hipMemcpy(t->boff.get(), boff.data(), boff.size() * 4, hipMemcpyHostToDevice));
+        QS_HIP(hipMemcpy(t->rank.get(), rank.data(), rank.size() * 2, hipMemcpyHostToDevice));
+        QS_HIP(hipMemcpy(t->words.get(), words.data(), words.size() * 2, hipMemcpyHostToDevice));
+        b.split = std::move(t);
+    });
+    return *b.split;
+}
+
+// hop groups of the operator classified by where they sit (once per operator)
+void split_classify(const OpHost &op) {
+    std::call_once(op.split_once, [&] {
+        const int H = split_basis(*op.basis).H;
+        auto build = [&](const FastTerms &f, SplitHops &out) {
+            std::vector<HopDev> low, high, cross;
+            for (const HopDev &h : f.hops) (h.hi < H ? low : (h.lo >= H ? high : cross)).push_back(h);
+            out.nlow = (int)low.size();
+            out.nhigh = (int)high.size();
+            out.ncross = (int)cross.size();
+            low.insert(low.end(), high.begin(), high.end());
+            low.insert(low.end(), cross.begin(), cross.end());
+            if (!low.empty()) {
+                out.d.alloc(low.size());
+                QS_HIP(hipMemcpy(out.d.get(), low.data(), low.size() * sizeof(HopDev), hipMemcpyHostToDevice));
+            }
+        };
+        build(op.fast, op.split_hops);
+        build(op.fast_adj, op.split_hops_adj);
+    });
+}
+
+template <typename V, bool CC>
+void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, size_t lds, hipStream_t s, const HopDev *hops,
+                       const HopLaunch &H, const void *x, void *y) {
+    const V *xv = (const V *)x;
+    V *yv = (V *)y;
+    auto go = [&](auto kern) {
+        // persistent workgroups: as many as stay resident, a multiple of 8
+        static int cus = 0;
+        if (!cus) {
+            int dev = 0;
+            QS_HIP(hipGetDevice(&dev));
+            QS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        }
+        int occ = 0;
+        QS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kSpBlock, lds));
+        const int64_t want = (int64_t)std::max(1, occ) * cus;
+        const int grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, ((int64_t)D.nblocks + 7) / 8 * 8) / 8 * 8);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kSpBlock), lds, s, a, D, hops, H.dcls, H.drest, xv, yv);
+    };
+    if (cm == CM_ZONLY) go(k_matvec_split<V, CC, CM_ZONLY>);
+    else go(k_matvec_split<V, CC, CM_FULL>);
+}
+
+void launch_split(const OpHost &op, MVArgs &a, int dtype, bool cc, bool adjoint, const void *x, void *y,
+                  hipStream_t s) {
+    const BasisHost &b = *op.basis;
+    const SplitBasis &t = split_basis(b);
+    split_classify(op);
+    const SplitHops &sh = adjoint ? op.split_hops_adj : op.split_hops;
+    SplitDev D;
+    memset(&D, 0, sizeof D);
+    D.H = t.H;
+    D.lmask = (1u << t.H) - 1;
+    D.wmask = t.W >= 32 ? 0xFFFFFFFFu : ((1u << t.W) - 1);
+    D.blist = t.blist.get();
+    D.nblocks = t.nblocks;
+    D.boff = t.boff.get();
+    D.rank = t.rank.get();
+    D.words = t.words.get();
+    for (int p = 0; p <= t.H; p++) {
+        D.cls_off[p] = t.cls_off[p];
+        D.cls_n[p] = t.cls_n[p];
+    }
+    D.nlow = sh.nlow;
+    D.nhigh = sh.nhigh;
+    D.ncross = sh.ncross;
+    a.ndcls = (int)(adjoint ? op.fast_adj : op.fast).dcls.size();
+    a.ndrest = (int)(adjoint ? op.fast_adj : op.fast).drest.size();
+    const size_t nw = (size_t)1 << t.H;
+    const size_t esz = dtype_size(dtype);
+    D.xstride = t.maxblock + 1;
+    // LDS: rank and word tables, the own block plus staged cross blocks
+    // within a 64 KB budget, the per-block hop lists
+    const size_t list_bytes = sizeof(SpHop) * 2 * kSpMaxHops;
+    const size_t tables = (nw * 4 + 15) & ~(size_t)15;
+    const size_t blk = ((size_t)D.xstride * esz + 15) & ~(size_t)15;
+    const size_t budget = 64 * 1024;
+    const int fit = (int)((budget > tables + list_bytes + blk) ? (budget - tables - list_bytes - blk) / blk : 0);
+    D.nstage = std::min(sh.ncross, fit);
+    D.off_rank = 0;
+    D.off_words = (int)(nw * 2);
+    D.off_x = (int)tables;
+    D.off_list = (int)(tables + (size_t)(1 + D.nstage) * D.xstride * esz + 15) & ~15;
+    const size_t lds = (size_t)D.off_list + list_bytes;
+    if (sh.nhigh > kSpMaxHops || sh.ncross > kSpMaxHops) fail(QS_EARG, "internal: too many hop groups for the split kernel");
+    const int cm = b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL;
+    HopLaunch H{nullptr, adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
+                adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
+    const HopDev *hops = sh.d.get();
+    switch (dtype) {
+    case QS_F64: dispatch_split_cm<double, false>(a, D, cm, lds, s, hops, H, x, y); break;
+    case QS_F32: dispatch_split_cm<float, false>(a, D, cm, lds, s, hops, H, x, y); break;
+    case QS_C128:
+        if (cc) dispatch_split_cm<cplx_d, true>(a, D, cm, lds, s, hops, H, x, y);
+        else dispatch_split_cm<cplx_d, false>(a, D, cm, lds, s, hops, H, x, y);
+        break;
+    case QS_C64:
+        if (cc) dispatch_split_cm<cplx_f, true>(a, D, cm, lds, s, hops, H, x, y);
+        else dispatch_split_cm<cplx_f, false>(a, D, cm, lds, s, hops, H, x, y);
+        break;
+    default: fail(QS_EARG, "unknown dtype");
+    }
+}
+
 }  // namespace
 
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
@@ -930,8 +1342,17 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
-    const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP;
-    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || want_hop);
+    if (op.lanes_per_row == QS_VARIANT_SPLIT && !dot_partial) {
+        if (!split_eligible(op, adjoint))
+            fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
+                          "and no symmetry or spin inversion only, L <= 32");
+        launch_split(op, a, dtype, cc, adjoint, x, y, s);
+        QS_HIP(hipGetLastError());
+        return;
+    }
+    const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP && op.lanes_per_row <= QS_VARIANT_HOP + 2;
+    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SPLIT ||
+                                                   want_hop);
     if (want_hop && !hop)
         fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
     if (hop) {

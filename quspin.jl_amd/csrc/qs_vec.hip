@@ -154,10 +154,24 @@ __device__ __forceinline__ void norm2_body(const V *__restrict__ x, int64_t n, d
     }
 }
 
+// Coefficient of an axpby read from device memory (a reduction result still
+// on the device), computed with the same IEEE operations the host would use.
+__device__ __forceinline__ void dev_coef(const double *c, int mode, double &ar, double &ai) {
+    switch (mode) {
+    case CO_NEG: ar = -c[0]; ai = -c[1]; break;
+    case CO_NEG_RE: ar = -c[0]; ai = 0.0; break;
+    case CO_NEG_SQRT: ar = -sqrt(c[0]); ai = 0.0; break;
+    case CO_INV_SQRT: ar = 1.0 / sqrt(c[0]); ai = 0.0; break;
+    default: break;
+    }
+}
+
 template <typename V>
 __device__ __forceinline__ void axpby_body(double ar, double ai, const V *__restrict__ x, double br, double bi,
-                                           V *__restrict__ y, int64_t n, int b_zero, int b, int nb) {
+                                           V *__restrict__ y, int64_t n, int b_zero, int b, int nb,
+                                           const double *acoef = nullptr, int cmode = CO_HOST) {
     using T = VTraits<V>;
+    if (acoef) dev_coef(acoef, cmode, ar, ai);
     for (int64_t r = (int64_t)b * blockDim.x + threadIdx.x; r < n; r += (int64_t)nb * blockDim.x) {
         double xr, xi, yr = 0, yi = 0;
         T::load(x, r, xr, xi);
@@ -215,8 +229,8 @@ __global__ __launch_bounds__(kVB) void k_norm2(const V *__restrict__ x, int64_t 
 
 template <typename V>
 __global__ void k_axpby(double ar, double ai, const V *__restrict__ x, double br, double bi, V *__restrict__ y,
-                        int64_t n, int b_zero) {
-    axpby_body<V>(ar, ai, x, br, bi, y, n, b_zero, blockIdx.x, gridDim.x);
+                        int64_t n, int b_zero, const double *acoef, int cmode) {
+    axpby_body<V>(ar, ai, x, br, bi, y, n, b_zero, blockIdx.x, gridDim.x, acoef, cmode);
 }
 
 template <typename V>
@@ -248,7 +262,7 @@ __global__ __launch_bounds__(kVB) void kb_vec(VecJobs J, int op) {
     case VOP_UPDATE: update_body<V>((const V *)q.V, q.n, q.m, q.aux, (V *)q.w, q.partial, b, q.nb, sh); break;
     case VOP_NORM2: norm2_body<V>((const V *)q.x, q.n, q.partial, b, q.nb, sh); break;
     case VOP_AXPBY:
-        axpby_body<V>(q.ar, q.ai, (const V *)q.x, q.br, q.bi, (V *)q.w, q.n, q.b_zero, b, q.nb);
+        axpby_body<V>(q.ar, q.ai, (const V *)q.x, q.br, q.bi, (V *)q.w, q.n, q.b_zero, b, q.nb, q.acoef, q.cmode);
         break;
     case VOP_COMBINE: combine_body<V>((const V *)q.V, q.n, q.m, q.aux, q.k, (V *)q.w, b, q.nb); break;
     }
@@ -394,7 +408,64 @@ void vec_axpby(int dtype, cd a, const void *x, cd b, void *y, int64_t n, hipStre
     by_dtype(dtype, [&](auto *tag) {
         using V = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL((k_axpby<V>), dim3(ew_grid(n)), dim3(256), 0, s, a.real(), a.imag(), (const V *)x,
-                           b.real(), b.imag(), (V *)y, n, (int)(b == cd(0, 0)));
+                           b.real(), b.imag(), (V *)y, n, (int)(b == cd(0, 0)), (const double *)nullptr, (int)CO_HOST);
+    });
+    QS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------- device-scalar variants -------
+// The same kernels with every scalar left in device memory: no host round
+// trip, so a whole Krylov factorization is queued on the stream at once.
+
+void vec_reserve(VecWork &wk, int64_t n, int mmax) {
+    const int nb = vec_blocks(n);
+    ensure(wk, (size_t)2 * nb * std::max(1, mmax) + 2 * (size_t)std::max(1, mmax) + 2,
+           (size_t)2 * std::max(1, mmax) + 2);
+}
+
+void gram_dots_dev(int dtype, const void *Vmat, int64_t n, int m, const void *w, VecWork &wk, double *out,
+                   hipStream_t s) {
+    const int nb = vec_blocks(n);
+    if (wk.partial.n < (size_t)2 * nb * m) fail(QS_EARG, "internal: vector workspace not reserved");
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_gram<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, (const V *)w,
+                           wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(m), dim3(kVB), 0, s, wk.partial.get(), nb, out);
+    QS_HIP(hipGetLastError());
+}
+
+void gram_update_dev(int dtype, const void *Vmat, int64_t n, int m, const double *h, void *w, VecWork &wk,
+                     double *nrm2_out, hipStream_t s) {
+    const int nb = vec_blocks(n);
+    if (wk.partial.n < (size_t)2 * nb) fail(QS_EARG, "internal: vector workspace not reserved");
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_update<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, h, (V *)w,
+                           wk.partial.get());
+    });
+    if (nrm2_out) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, nrm2_out);
+    QS_HIP(hipGetLastError());
+}
+
+void vec_norm2_dev(int dtype, const void *x, int64_t n, VecWork &wk, double *out, hipStream_t s) {
+    const int nb = vec_blocks(n);
+    if (wk.partial.n < (size_t)2 * nb) fail(QS_EARG, "internal: vector workspace not reserved");
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_norm2<V>), dim3(nb), dim3(kVB), 0, s, (const V *)x, n, wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, out);
+    QS_HIP(hipGetLastError());
+}
+
+void vec_axpby_dev(int dtype, const double *acoef, int cmode, const void *x, cd b, void *y, int64_t n,
+                   hipStream_t s) {
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_axpby<V>), dim3(ew_grid(n)), dim3(256), 0, s, 0.0, 0.0, (const V *)x, b.real(),
+                           b.imag(), (V *)y, n, (int)(b == cd(0, 0)), acoef, cmode);
     });
     QS_HIP(hipGetLastError());
 }

@@ -13,6 +13,7 @@ QS_F64, QS_C128, QS_F32, QS_C64 = range(4)
 WHICH = {"SA": 0, "LA": 1, "SM": 2, "LM": 3, "BE": 4}
 QS_CHECK_SYMM, QS_CHECK_HERM, QS_CHECK_PCON, QS_FORCE_COMPLEX = 1, 2, 4, 8
 VARIANT_HOP = 16   # QS_VARIANT_HOP: the two-bit hop matvec kernel
+VARIANT_SPLIT = 32   # QS_VARIANT_SPLIT: the split (high word / low word) hop kernel
 
 EXPORTS = (
     "qs_last_error", "qs_device_info", "qs_basis_create", "qs_basis_destroy", "qs_basis_size",

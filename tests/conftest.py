@@ -16,6 +16,12 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def gpu():
     """Fail loudly (not skip) when a GPU test cannot reach the native library."""
+    try:
+        # torch's ROCm runtime must be loaded before libqsgpu initialises HIP,
+        # or torch sees no device in the same process
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     import qsgpu
     qsgpu.lib()
     info = qsgpu.device_info()

@@ -112,6 +112,14 @@ def test_matvec_nonsymmetric_terms_flags_and_diagonal(gpu, blocks, hop):
     if hop:
         op.set_variant(qsgpu.VARIANT_HOP + 1)
     x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    variants = [None] + ([qsgpu.VARIANT_SPLIT] if hop and set(blocks) <= {"zblock"} else [])
+    for v in variants:
+        if v is not None:
+            op.set_variant(v)
+        _flags_and_rhs(op, ref, b, x, rng)
+
+
+def _flags_and_rhs(op, ref, b, x, rng):
     y0 = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     for t, c in [(False, False), (True, False), (False, True), (True, True)]:
         want = ref.matvec(x, y=y0, alpha=1.25 - 0.5j, beta=-0.5 + 0.25j, transposed=t, conjugated=c)
@@ -259,8 +267,21 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
             assert np.linalg.norm(got_r - ref.matvec(x.real.copy()).real) <= 1e-12 * np.linalg.norm(want), lpr
             got32 = op.matvec(x.real.astype(np.float32), dtype=np.float32)
             assert np.linalg.norm(got32 - ref.matvec(x.real.copy()).real) <= 2e-6 * np.linalg.norm(want), lpr
+    split = nup is not None and set(blocks) <= {"zblock"}
+    if split:
+        # the split kernel (high word / low word blocks) on the same operator
+        op.set_variant(qsgpu.VARIANT_SPLIT)
+        got = op.matvec(x)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), "split"
+        got32 = op.matvec(x.real.astype(np.float32), dtype=np.float32)
+        assert np.linalg.norm(got32 - ref.matvec(x.real.copy()).real) <= 2e-6 * np.linalg.norm(want), "split"
+        got64c = op.matvec(x.astype(np.complex64), dtype=np.complex64)
+        assert np.linalg.norm(got64c - want) <= 2e-6 * np.linalg.norm(want), "split"
+    else:
+        with pytest.raises(qsgpu.ArgumentError, match="split"):
+            op.set_variant(qsgpu.VARIANT_SPLIT)
     lpr, ms = op.tune(np.complex128, repeats=2)
-    assert lpr in (1, 2, 4, 8) + hops and ms > 0
+    assert lpr in (1, 2, 4, 8) + hops + ((qsgpu.VARIANT_SPLIT,) if split else ()) and ms > 0
     got = op.matvec(x)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
@@ -271,3 +292,61 @@ def test_hop_variant_rejected_for_other_couplings(gpu):
     op = qsgpu.QuantumLinearOperator(b, [("+z-", [(0.5, 1, 2, 3)])], check_symm=False, check_herm=False)
     with pytest.raises(qsgpu.ArgumentError, match="hop"):
         op.set_variant(qsgpu.VARIANT_HOP)
+
+
+@pytest.mark.parametrize("case", [(24, 12, dict(zblock=1)), (24, 10, {}), (20, 10, dict(zblock=-1)), (16, 7, {})],
+                         ids=_id)
+def test_split_kernel_against_oracle(gpu, case):
+    # split point 12 for L >= 24 (rows of one block share bits 12..L-1), L/2
+    # below; next-nearest and fourth-neighbour bonds cross the split, and
+    # under spin inversion the bonds onto bit L-1 read inverted blocks
+    L, nup, blocks = case
+    print("split", _id(case), flush=True)
+    terms = _terms(L) + [("+-", [(0.3, i, (i + 3) % L + 1) for i in range(1, L + 1)]),
+                         ("-+", [(0.3, i, (i + 3) % L + 1) for i in range(1, L + 1)])]
+    ref = oracle.Operator(oracle.Sector(L, nup=nup, pauli=False, **blocks), terms)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
+    op = qsgpu.QuantumLinearOperator(b, terms)
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal(b.Ns)
+    want = ref.matvec(x).real
+    op.set_variant(qsgpu.VARIANT_SPLIT)
+    for dt, tol in ((np.float64, 1e-12), (np.float32, 2e-6)):
+        got = op.matvec(x.astype(dt), dtype=dt)
+        assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want), dt
+    y0 = rng.standard_normal(b.Ns)
+    out = y0.copy()
+    op.mul(out, x, alpha=0.75, beta=-1.5)
+    assert np.linalg.norm(out - (0.75 * want - 1.5 * y0)) <= 1e-12 * np.linalg.norm(out)
+
+
+def test_split_kernel_fullsize_c3(gpu):
+    # L=32, nup=16, z=+1 (N = 300,540,195): the split kernel equals the hop
+    # kernel (validated against the oracle at small L) to 1e-12 in FP64, and
+    # the FP32 stream is within 2e-6; the random fields cancel in z+.
+    import torch
+    print("split fullsize C3", flush=True)
+    b = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
+    h = np.random.default_rng(0xC3).uniform(-2.0, 2.0, 32)
+    terms = _dense.xxz_terms(32, J=1.0, delta=1.0, J2=0.0, periodic=True) + \
+        [("z", [(float(h[i]), i + 1) for i in range(32)])]
+    op = qsgpu.QuantumLinearOperator(b, terms, check_symm=False)
+    assert b.Ns == 300540195
+    g = torch.Generator(device="cpu").manual_seed(7)
+    x = torch.randn(b.Ns, generator=g, dtype=torch.float64).cuda()
+    y_hop = torch.empty_like(x)
+    y_split = torch.empty_like(x)
+    op.set_variant(qsgpu.VARIANT_HOP)
+    op.matvec_device(x, y_hop)
+    op.set_variant(qsgpu.VARIANT_SPLIT)
+    op.matvec_device(x, y_split)
+    torch.cuda.synchronize()
+    ref = torch.linalg.vector_norm(y_hop).item()
+    assert torch.linalg.vector_norm(y_split - y_hop).item() <= 1e-12 * ref
+    x32 = x.float()
+    y32 = torch.empty_like(x32)
+    op.matvec_device(x32, y32)
+    torch.cuda.synchronize()
+    assert torch.linalg.vector_norm(y32.double() - y_hop).item() <= 2e-6 * ref
+    # <x, Hx> is real and the product is symmetric: <x, H x32> == <x32, H x> in FP64
+    del y32, x32
