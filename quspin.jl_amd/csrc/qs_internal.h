@@ -156,6 +156,7 @@ struct SplitBasis {
     int nblocks = 0, maxblock = 0;
     int cls_off[34] = {}, cls_n[34] = {};
     DevBuf<uint32_t> blist;    // high words with a nonempty block, ascending
+    DevBuf<uint32_t> bstart;   // first row of each of those blocks
     DevBuf<uint32_t> boff;     // first row of the block of each high word (2^W; ~0 if none)
     DevBuf<uint16_t> rank;     // rank of every H-bit word among the words of its popcount
     DevBuf<uint16_t> words;    // H-bit words by (popcount, value)

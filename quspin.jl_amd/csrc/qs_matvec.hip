@@ -714,45 +714,49 @@ __global__ __launch_bounds__(kMvBlock) void kb_matvec_hop(MVJobs J) {
 // half filling) and two-bit hop couplings.  A state splits into its high
 // word h = s >> H and low word l = s & (2^H - 1).  In colex order the rows
 // sharing h form one contiguous block of C(H, p) rows (p = nup - popcount h)
-// ordered by the rank of l among the p-bit words, starting at boff[h]:
+// ordered by the rank of l among the p-bit words, starting at bstart:
 //  * a hop inside h moves whole blocks: row r of block h pulls row r of
 //    block h ^ flip (row C(H,p)-1-r of the inverted block when the hop sets
-//    bit L-1 under spin inversion).  Liveness and amplitude are the same for
-//    the whole block and the reads are contiguous; a row issues these loads
-//    kSpNB at a time;
+//    bit L-1 under spin inversion).  Liveness, weight and the neighbour
+//    block's offset are wavefront-uniform (scalar registers) and the reads
+//    are contiguous;
 //  * a hop inside l stays in the block, staged in LDS: the neighbour's row
 //    is rank[l ^ flip] from an LDS table over every H-bit word;
-//  * a hop across the split changes p and reaches one whole block boff[h']
-//    at rows rank[l'] in every order: that block is staged in LDS beside the
-//    own one (up to the LDS budget; further ones are gathered from HBM).
-// One workgroup owns one block at a time; each XCD walks a contiguous range
-// of blocks, so the blocks its low high-bit hops reach are still in its L2.
-constexpr int kSpBlock = 512;
-constexpr int kSpMaxHops = 64;   // live hops of one block kept in LDS
+//  * a hop across the split changes p and reaches one whole block at rows
+//    rank[l'] in any order: that block is staged in LDS beside the own one
+//    (up to the LDS budget; further ones are gathered from HBM).
+// Each wavefront owns one block at a time (at most 252 rows, four per lane)
+// and needs no workgroup barrier, so every wavefront of a CU has its own
+// block's loads in flight.  The grid sweeps the blocks in ascending order:
+// blocks a hop reaches behind the sweep front were read recently and sit in
+// the shared MALL.
+constexpr int kSpBlock = 256;
+constexpr int kSpRows = 4;       // rows per lane: blocks of at most 256 rows
+constexpr int kSpMaxHops = 64;   // high + cross hop groups, one lane each
 
 struct SplitDev {
     int H;
     uint32_t lmask, wmask;
-    const uint32_t *blist;
+    const uint32_t *blist;       // high word of each block, ascending
+    const uint32_t *bstart;      // first row of each block
     int nblocks;
-    const uint32_t *boff;
+    const uint32_t *boff;        // first row of the block of each high word (2^W)
     const uint16_t *rank;
     const uint16_t *words;
     int cls_off[34], cls_n[34];
     int nlow, nhigh, ncross;
-    int nstage;                  // cross blocks staged in LDS
-    int xstride;                 // elements per staged block (max block + 1)
-    int off_rank, off_words, off_x, off_list;   // dynamic LDS layout (bytes)
+    int nstage;                  // cross blocks staged in LDS per wavefront
+    int xstride;                 // elements per staged block
+    int wave_lds;                // bytes of one wavefront's LDS region
+    int off_rank, off_words, off_wave, off_xw;   // LDS layout (bytes)
 };
 
-// per-block list of live high / cross hops (LDS)
+// one live high / cross hop of the current block (wavefront-private LDS)
 struct SpHop {
-    int64_t off;       // first row of the neighbour block
-    double re, im;     // weight (conjugated amplitude, inversion character applied)
-    int rev;           // high: read row n-1-r; cross: the neighbour's low word is inverted
-    int lo, bhi;       // cross: the low bit and the high bit's value in h
-    int slot;          // cross: staged block index (1 + k), or -1 for HBM gathers
-    int n2;            // rows of the neighbour block
+    uint32_t off;      // first row of the neighbour block
+    uint32_t flags;    // bit 0: reversed rows (high) / inverted low word (cross); bit 1: h's bit hi
+                       // (cross); bits 8..15: the low bit (cross); bits 16..31: rows of the block
+    double re, im;     // weight: conjugated amplitude with the inversion character
 };
 
 template <typename V>
@@ -765,130 +769,185 @@ __device__ __forceinline__ void acc_nb(double wr, double wi, double vr, double v
     }
 }
 
+// LDS written by some lanes of a wavefront, read by others
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename V, bool CC, int CM>
 __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D, const HopDev *__restrict__ hops,
                                                            const DiagClass *__restrict__ dcls,
                                                            const TermDev *__restrict__ drest,
                                                            const V *__restrict__ x, V *__restrict__ y) {
     using T = VTraits<V>;
-    constexpr int kNBH = T::cplx ? 4 : 8;   // high-hop loads in flight per row
+    constexpr int kNB = sizeof(V) <= 8 ? 4 : 2;   // high hops per batch (x kSpRows loads in flight)
     extern __shared__ __align__(16) char smem[];
     uint16_t *s_rank = (uint16_t *)(smem + D.off_rank);
     uint16_t *s_words = (uint16_t *)(smem + D.off_words);
-    V *s_x = (V *)(smem + D.off_x);
-    SpHop *s_list = (SpHop *)(smem + D.off_list);
-    __shared__ int s_nh, s_nc;
     const int nw = 1 << D.H;
     for (int i = threadIdx.x; i < nw; i += kSpBlock) {
         s_rank[i] = D.rank[i];
         s_words[i] = D.words[i];
     }
-    // the grid is a multiple of 8 and hardware places block b on XCD b % 8
-    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
-    const int chunk = (D.nblocks + 7) >> 3;
-    const int b0 = xcd * chunk;
-    const int b1 = min(D.nblocks, b0 + chunk);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    SpHop *s_list = (SpHop *)(smem + D.off_wave + wave * D.wave_lds);
+    V *s_x = (V *)(smem + D.off_wave + wave * D.wave_lds + D.off_xw);
+    // lane k < nhigh owns high hop group k, lane nhigh + k cross group k,
+    // for every block: their constants stay in registers
+    HopDev mine{};
+    const bool is_high = lane < D.nhigh;
+    const bool is_cross = !is_high && lane < D.nhigh + D.ncross;
+    if (is_high || is_cross) mine = hops[D.nlow + lane];
     double dot_r = 0, dot_i = 0;
-    for (int vb = b0 + (int)(blockIdx.x >> 3); vb < b1; vb += per) {
-        const uint32_t h = D.blist[vb];
-        const uint32_t off = D.boff[h];
+    const int gw = blockIdx.x * (kSpBlock / 64) + wave, nwaves = gridDim.x * (kSpBlock / 64);
+    int vb = gw;
+    uint32_t hn = 0, on = 0;
+    if (vb < D.nblocks) {
+        hn = D.blist[vb];
+        on = D.bstart[vb];
+    }
+    for (; vb < D.nblocks; vb += nwaves) {
+        const uint32_t h = hn, off = on;
+        if (vb + nwaves < D.nblocks) {   // the next block's header, in flight meanwhile
+            hn = D.blist[vb + nwaves];
+            on = D.bstart[vb + nwaves];
+        }
         const int p = A.nup - __popc(h);
         const int n = D.cls_n[p];
         const int cw = D.cls_off[p];
-        __syncthreads();   // the previous block is done with s_x and s_list
-        // the block's live hop lists, one hop group per lane (wave 0: high
-        // hops, wave 1: cross hops), compacted in group order by ballot
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        if (wave == 0) {
-            bool live = false;
-            SpHop e{};
-            if (lane < D.nhigh) {
-                const HopDev hp = hops[D.nlow + lane];
-                const int lo = hp.lo - D.H, hi = hp.hi - D.H;
-                const uint32_t blo = (h >> lo) & 1, bhi = (h >> hi) & 1;
-                live = (hp.nzmask >> (blo | (bhi << 1))) & 1;
-                double m_r = hp.sym ? hp.re[1] : (blo ? hp.re[1] : hp.re[2]);
-                double m_i = CC ? (hp.sym ? hp.im[1] : (blo ? hp.im[1] : hp.im[2])) : 0.0;
-                uint32_t h2 = h ^ ((1u << lo) | (1u << hi));
-                int rev = 0;
-                if (CM == CM_ZONLY && hp.top) {
-                    // the hop sets bit L-1: the column is the inverted state's
-                    h2 = ~h2 & D.wmask;
-                    rev = 1;
-                    if (A.zchar < 0) {
-                        m_r = -m_r;
-                        m_i = -m_i;
-                    }
-                }
-                if (live) e.off = D.boff[h2];
-                e.re = m_r;
-                e.im = -m_i;   // conj(amp): the terms are the adjoint couplings
-                e.rev = rev;
-                e.slot = -1;
-                e.n2 = n;
-            }
-            const uint64_t m = __ballot(live);
-            if (live) s_list[__popcll(m & ((1ULL << lane) - 1))] = e;
-            if (lane == 0) s_nh = __popcll(m);
-        } else if (wave == 1) {
-            // cross hops: the rows whose bit lo differs from h's bit hi move
-            bool live = false;
-            SpHop e{};
-            if (lane < D.ncross) {
-                const HopDev hp = hops[D.nlow + D.nhigh + lane];
-                const int hi = hp.hi - D.H;
-                const uint32_t bhi = (h >> hi) & 1;
-                const int pat = bhi ? 2 : 1;
-                double m_r = hp.re[pat];
-                double m_i = CC ? hp.im[pat] : 0.0;
-                uint32_t h2 = h ^ (1u << hi);
-                const int inv = CM == CM_ZONLY && hp.top;
-                if (inv) {
-                    h2 = ~h2 & D.wmask;
-                    if (A.zchar < 0) {
-                        m_r = -m_r;
-                        m_i = -m_i;
-                    }
-                }
-                const int p2 = A.nup - __popc(h2);
-                live = ((hp.nzmask >> pat) & 1) && p2 >= 0 && p2 <= D.H;
-                const uint32_t o2 = live ? D.boff[h2] : 0xFFFFFFFFu;
-                live = live && o2 != 0xFFFFFFFFu;   // else no row of this block moves that way
-                e.off = o2;
-                e.re = m_r;
-                e.im = -m_i;
-                e.rev = inv;
-                e.lo = hp.lo;
-                e.bhi = (int)bhi;
-                e.n2 = live ? D.cls_n[p2] : 0;
-            }
-            const uint64_t m = __ballot(live);
-            const int pos = __popcll(m & ((1ULL << lane) - 1));
-            e.slot = pos < D.nstage ? 1 + pos : -1;
-            if (live) s_list[kSpMaxHops + pos] = e;
-            if (lane == 0) s_nc = __popcll(m);
+        // own block
+        double xo_r[kSpRows], xo_i[kSpRows];
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            const int r = lane + 64 * j;
+            xo_r[j] = 0;
+            xo_i[j] = 0;
+            if (r < n) T::load(x, (int64_t)off + r, xo_r[j], xo_i[j]);
+            if (A.conj_x) xo_i[j] = -xo_i[j];
         }
-        __syncthreads();
-        const int nh = s_nh, nc = s_nc;
-        // stage the own block and the staged cross blocks (all loads issued together)
-        for (int i = threadIdx.x; i < D.xstride; i += kSpBlock) {
-            double a = 0, b = 0;
-            if (i < n) T::load(x, (int64_t)off + i, a, b);
-            if (A.conj_x) b = -b;
-            T::store(s_x, i, a, b);   // s_x[n] = 0: dead neighbours read it
-            for (int c = 0; c < nc; c++) {
-                const SpHop &e = s_list[kSpMaxHops + c];
-                if (e.slot < 0) continue;
-                double u = 0, v = 0;
-                if (i < e.n2) T::load(x, e.off + i, u, v);
-                if (A.conj_x) v = -v;
-                T::store(s_x, (int64_t)e.slot * D.xstride + i, u, v);
+        // this block's entry of the lane's hop group
+        bool live = false;
+        SpHop e{};
+        if (is_high) {
+            const int lo = mine.lo - D.H, hi = mine.hi - D.H;
+            const uint32_t blo = (h >> lo) & 1, bhi = (h >> hi) & 1;
+            live = (mine.nzmask >> (blo | (bhi << 1))) & 1;
+            double m_r = mine.sym ? mine.re[1] : (blo ? mine.re[1] : mine.re[2]);
+            double m_i = CC ? (mine.sym ? mine.im[1] : (blo ? mine.im[1] : mine.im[2])) : 0.0;
+            uint32_t h2 = h ^ ((1u << lo) | (1u << hi));
+            uint32_t fl = 0;
+            if (CM == CM_ZONLY && mine.top) {
+                // the hop sets bit L-1: the column is the inverted state's
+                h2 = ~h2 & D.wmask;
+                fl = 1;
+                if (A.zchar < 0) {
+                    m_r = -m_r;
+                    m_i = -m_i;
+                }
+            }
+            e.off = live ? D.boff[h2] : 0;
+            e.flags = fl;
+            e.re = m_r;
+            e.im = -m_i;   // conj(amp): the terms are the adjoint couplings
+        } else if (is_cross) {
+            // the rows whose bit lo differs from h's bit hi move
+            const int hi = mine.hi - D.H;
+            const uint32_t bhi = (h >> hi) & 1;
+            const int pat = bhi ? 2 : 1;
+            double m_r = mine.re[pat];
+            double m_i = CC ? mine.im[pat] : 0.0;
+            uint32_t h2 = h ^ (1u << hi);
+            const int inv = CM == CM_ZONLY && mine.top;
+            if (inv) {
+                h2 = ~h2 & D.wmask;
+                if (A.zchar < 0) {
+                    m_r = -m_r;
+                    m_i = -m_i;
+                }
+            }
+            const int p2 = A.nup - __popc(h2);
+            live = ((mine.nzmask >> pat) & 1) && p2 >= 0 && p2 <= D.H;
+            const uint32_t o2 = live ? D.boff[h2] : 0xFFFFFFFFu;
+            live = live && o2 != 0xFFFFFFFFu;   // else no row of this block moves that way
+            e.off = o2;
+            e.flags = (uint32_t)inv | (bhi << 1) | ((uint32_t)mine.lo << 8) |
+                      ((uint32_t)(live ? D.cls_n[p2] : 0) << 16);
+            e.re = m_r;
+            e.im = -m_i;
+        }
+        const uint64_t mh = __ballot(is_high && live), mc = __ballot(is_cross && live);
+        const uint64_t below = (1ULL << lane) - 1;
+        if (is_high && live) s_list[__popcll(mh & below)] = e;
+        if (is_cross && live) s_list[D.nhigh + __popcll(mc & below)] = e;
+        const int nh = __popcll(mh), nc = __popcll(mc);
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            const int r = lane + 64 * j;
+            if (r < D.xstride) T::store(s_x, r, xo_r[j], xo_i[j]);   // s_x[n] = 0: dead neighbours read it
+        }
+        wave_lds_sync();
+        // staged cross blocks: loads of all of them first
+        const int ns = min(nc, D.nstage);
+        for (int c = 0; c < ns; c++) {
+            const SpHop ec = s_list[D.nhigh + c];
+            const uint32_t n2 = ec.flags >> 16;
+            double u_r[kSpRows], u_i[kSpRows];
+#pragma unroll
+            for (int j = 0; j < kSpRows; j++) {
+                const int i = lane + 64 * j;
+                u_r[j] = 0;
+                u_i[j] = 0;
+                if (i < (int)n2) T::load(x, (int64_t)ec.off + i, u_r[j], u_i[j]);
+                if (A.conj_x) u_i[j] = -u_i[j];
+            }
+#pragma unroll
+            for (int j = 0; j < kSpRows; j++) {
+                const int i = lane + 64 * j;
+                if (i < D.xstride) T::store(s_x, (int64_t)(1 + c) * D.xstride + i, u_r[j], u_i[j]);
             }
         }
-        __syncthreads();
-        for (int r = threadIdx.x; r < n; r += kSpBlock) {
+        // high hops: whole neighbour blocks, kNB x kSpRows loads in flight
+        double ar[kSpRows], ai[kSpRows];
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            ar[j] = 0;
+            ai[j] = 0;
+        }
+        for (int k0 = 0; k0 < nh; k0 += kNB) {
+            double vr[kNB][kSpRows], vi[kNB][kSpRows];
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk++) {
+                const int k = min(k0 + kk, nh - 1);
+                const uint32_t eoff = __builtin_amdgcn_readfirstlane(s_list[k].off);
+                const uint32_t rev = __builtin_amdgcn_readfirstlane(s_list[k].flags) & 1;
+                const V *xb = x + eoff;
+#pragma unroll
+                for (int j = 0; j < kSpRows; j++) {
+                    const int r = lane + 64 * j;
+                    vr[kk][j] = 0;
+                    vi[kk][j] = 0;
+                    if (k0 + kk < nh && r < n) T::load(xb, rev ? n - 1 - r : r, vr[kk][j], vi[kk][j]);
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk++) {
+                if (k0 + kk < nh) {
+                    const double wr = s_list[k0 + kk].re, wi = s_list[k0 + kk].im;
+#pragma unroll
+                    for (int j = 0; j < kSpRows; j++)
+                        acc_nb<V>(wr, wi, vr[kk][j], A.conj_x ? -vi[kk][j] : vi[kk][j], ar[j], ai[j]);
+                }
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            const int r = lane + 64 * j;
+            if (r >= n) continue;
             const uint32_t l = s_words[cw + r];
-            double ar = 0, ai = 0;
             {
                 double dr = 0, di = 0;
                 diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
@@ -898,61 +957,41 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
                     dr += A.user_diag[2 * row];
                     di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
                 }
-                double xr, xi;
-                T::load(s_x, r, xr, xi);
-                acc_nb<V>(dr, di, xr, xi, ar, ai);
-            }
-            // high hops: whole blocks from HBM, kNBH loads in flight
-            for (int k0 = 0; k0 < nh; k0 += kNBH) {
-                double vr[kNBH], vi[kNBH];
-#pragma unroll
-                for (int kk = 0; kk < kNBH; kk++) {
-                    vr[kk] = 0;
-                    vi[kk] = 0;
-                    if (k0 + kk < nh) {
-                        const SpHop &e = s_list[k0 + kk];
-                        T::load(x, e.off + (e.rev ? n - 1 - r : r), vr[kk], vi[kk]);
-                    }
-                }
-#pragma unroll
-                for (int kk = 0; kk < kNBH; kk++) {
-                    if (k0 + kk < nh) {
-                        const SpHop &e = s_list[k0 + kk];
-                        acc_nb<V>(e.re, e.im, vr[kk], A.conj_x ? -vi[kk] : vi[kk], ar, ai);
-                    }
-                }
+                acc_nb<V>(dr, di, xo_r[j], xo_i[j], ar[j], ai[j]);
             }
             // low hops: neighbours in the staged own block
             for (int g = 0; g < D.nlow; g++) {
                 const HopDev hp = hops[g];
                 const uint32_t blo = (l >> hp.lo) & 1, bhi = (l >> hp.hi) & 1;
-                const bool live = (hp.nzmask >> (blo | (bhi << 1))) & 1;
+                const bool lv = (hp.nzmask >> (blo | (bhi << 1))) & 1;
                 // with a particle number only patterns 1 and 2 are live
                 const double m_r = hp.sym ? hp.re[1] : (blo ? hp.re[1] : hp.re[2]);
                 const double m_i = CC ? (hp.sym ? hp.im[1] : (blo ? hp.im[1] : hp.im[2])) : 0.0;
-                const int nid = live ? (int)s_rank[l ^ ((1u << hp.lo) | (1u << hp.hi))] : n;
+                const int nid = lv ? (int)s_rank[l ^ ((1u << hp.lo) | (1u << hp.hi))] : n;
                 double vr, vi;
                 T::load(s_x, nid, vr, vi);
-                acc_nb<V>(m_r, -m_i, vr, vi, ar, ai);
+                acc_nb<V>(m_r, -m_i, vr, vi, ar[j], ai[j]);
             }
             // cross hops: staged neighbour blocks (or HBM gathers past the budget)
             for (int c = 0; c < nc; c++) {
-                const SpHop &e = s_list[kSpMaxHops + c];
-                if ((int)((l >> e.lo) & 1) == e.bhi) continue;
-                uint32_t l2 = l ^ (1u << e.lo);
-                if (e.rev) l2 = ~l2 & D.lmask;
+                const SpHop ec = s_list[D.nhigh + c];
+                const uint32_t lo = (ec.flags >> 8) & 0xFF, bhi = (ec.flags >> 1) & 1;
+                if (((l >> lo) & 1) == bhi) continue;
+                uint32_t l2 = l ^ (1u << lo);
+                if (ec.flags & 1) l2 = ~l2 & D.lmask;
                 const int idx = s_rank[l2];
                 double vr, vi;
-                if (e.slot >= 0) {
-                    T::load(s_x, (int64_t)e.slot * D.xstride + idx, vr, vi);
+                if (c < D.nstage) {
+                    T::load(s_x, (int64_t)(1 + c) * D.xstride + idx, vr, vi);
                 } else {
-                    T::load(x, e.off + idx, vr, vi);
+                    T::load(x, (int64_t)ec.off + idx, vr, vi);
                     if (A.conj_x) vi = -vi;
                 }
-                acc_nb<V>(e.re, e.im, vr, vi, ar, ai);
+                acc_nb<V>(ec.re, ec.im, vr, vi, ar[j], ai[j]);
             }
-            finish_row<V>(A, y, (int64_t)off + r, ar, ai, 0.0, 0.0, dot_r, dot_i);
+            finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
         }
+        wave_lds_sync();   // the block's LDS reads are done before the next block's writes
     }
 }
 
@@ -1180,7 +1219,7 @@ const SplitBasis &split_basis(const BasisHost &b) {
     std::call_once(b.split_once, [&] {
         auto t = std::make_unique<SplitBasis>();
         const int L = b.L, nup = b.nup;
-        const int H = L >= 24 ? 12 : std::max(2, L / 2);
+        const int H = L >= 20 ? 10 : std::max(2, L / 2);
         const int W = L - H;
         t->H = H;
         t->W = W;
@@ -1195,7 +1234,7 @@ const SplitBasis &split_basis(const BasisHost &b) {
             t->cls_n[p] = (int)words.size() - t->cls_off[p];
             t->maxblock = std::max(t->maxblock, t->cls_n[p]);
         }
-        std::vector<uint32_t> boff((size_t)1 << W, 0xFFFFFFFFu), blist;
+        std::vector<uint32_t> boff((size_t)1 << W, 0xFFFFFFFFu), blist, bstart;
         int64_t next = 0;
         for (uint32_t h = 0; h < (1u << W); h++) {
             const int p = nup - __builtin_popcount(h);
@@ -1210,11 +1249,14 @@ const SplitBasis &split_basis(const BasisHost &b) {
             if (off != next) fail(QS_EARG, "internal: split blocks are not contiguous");
             boff[h] = (uint32_t)off;
             blist.push_back(h);
+            bstart.push_back((uint32_t)off);
             next += t->cls_n[p];
         }
         if (next != b.Ns) fail(QS_EARG, "internal: split blocks do not cover the basis");
         t->nblocks = (int)blist.size();
         t->blist.alloc(blist.size());
+        t->bstart.alloc(bstart.size());
+        QS_HIP(hipMemcpy(t->bstart.get(), bstart.data(), bstart.size() * 4, hipMemcpyHostToDevice));
         t->boff.alloc(boff.size());
         t->rank.alloc(rank.size());
         t->words.alloc(words.size());
@@ -1265,7 +1307,8 @@ void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, size_t lds, h
         int occ = 0;
         QS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kSpBlock, lds));
         const int64_t want = (int64_t)std::max(1, occ) * cus;
-        const int grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, ((int64_t)D.nblocks + 7) / 8 * 8) / 8 * 8);
+        const int64_t need = ((int64_t)D.nblocks + kSpBlock / 64 - 1) / (kSpBlock / 64);
+        const int grid = (int)std::max<int64_t>(1, std::min(want, need));
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kSpBlock), lds, s, a, D, hops, H.dcls, H.drest, xv, yv);
     };
     if (cm == CM_ZONLY) go(k_matvec_split<V, CC, CM_ZONLY>);
@@ -1299,21 +1342,23 @@ void launch_split(const OpHost &op, MVArgs &a, int dtype, bool cc, bool adjoint,
     a.ndrest = (int)(adjoint ? op.fast_adj : op.fast).drest.size();
     const size_t nw = (size_t)1 << t.H;
     const size_t esz = dtype_size(dtype);
+    if (sh.nhigh + sh.ncross > kSpMaxHops) fail(QS_EARG, "internal: too many hop groups for the split kernel");
+    if (t.maxblock > 64 * kSpRows) fail(QS_EARG, "internal: split block larger than the wavefront's rows");
+    D.bstart = t.bstart.get();
     D.xstride = t.maxblock + 1;
-    // LDS: rank and word tables, the own block plus staged cross blocks
-    // within a 64 KB budget, the per-block hop lists
-    const size_t list_bytes = sizeof(SpHop) * 2 * kSpMaxHops;
-    const size_t tables = (nw * 4 + 15) & ~(size_t)15;
-    const size_t blk = ((size_t)D.xstride * esz + 15) & ~(size_t)15;
-    const size_t budget = 64 * 1024;
-    const int fit = (int)((budget > tables + list_bytes + blk) ? (budget - tables - list_bytes - blk) / blk : 0);
+    // LDS: shared rank and word tables; per wavefront the hop list, the own
+    // block and as many staged cross blocks as fit a 12 KB region
+    const size_t list_bytes = ((sizeof(SpHop) * (sh.nhigh + sh.ncross)) + 15) & ~(size_t)15;
+    const size_t blk = (size_t)D.xstride * esz;
+    const size_t region = 12 * 1024;
+    const int fit = (int)(region > list_bytes + blk ? (region - list_bytes - blk) / blk : 0);
     D.nstage = std::min(sh.ncross, fit);
     D.off_rank = 0;
     D.off_words = (int)(nw * 2);
-    D.off_x = (int)tables;
-    D.off_list = (int)(tables + (size_t)(1 + D.nstage) * D.xstride * esz + 15) & ~15;
-    const size_t lds = (size_t)D.off_list + list_bytes;
-    if (sh.nhigh > kSpMaxHops || sh.ncross > kSpMaxHops) fail(QS_EARG, "internal: too many hop groups for the split kernel");
+    D.off_wave = (int)((nw * 4 + 15) & ~(size_t)15);
+    D.off_xw = (int)list_bytes;
+    D.wave_lds = (int)((list_bytes + (size_t)(1 + D.nstage) * blk + 15) & ~(size_t)15);
+    const size_t lds = (size_t)D.off_wave + (size_t)(kSpBlock / 64) * D.wave_lds;
     const int cm = b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL;
     HopLaunch H{nullptr, adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                 adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
