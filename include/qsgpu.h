@@ -138,8 +138,9 @@ int qs_op_stream(const qs_op *op, void **stream);
 /* The split kernel: two-bit hop couplings on a basis without symmetry or
  * with spin inversion only, with a particle number, L <= 32.  Rows sharing
  * their high bits form a contiguous block, so hops among the high bits read
- * whole neighbour blocks and hops among the low bits stay in LDS.  No
- * reference counterpart. */
+ * whole neighbour blocks and hops among the low bits stay in LDS;
+ * QS_VARIANT_SPLIT + k (k = 0, 1, 2) keeps k fewer workgroups per CU
+ * resident than fit.  No reference counterpart. */
 #define QS_VARIANT_SPLIT 32
 int qs_op_set_variant(qs_op *op, int lanes_per_row);
 

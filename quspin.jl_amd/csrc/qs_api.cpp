@@ -641,10 +641,10 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
     return guard([&] {
         check_ptr(op, "op");
         if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 &&
-            lanes_per_row != 8 && lanes_per_row != QS_VARIANT_SPLIT &&
+            lanes_per_row != 8 && (lanes_per_row < QS_VARIANT_SPLIT || lanes_per_row > QS_VARIANT_SPLIT + 2) &&
             (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 2))
-            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2 or QS_VARIANT_SPLIT");
-        if (lanes_per_row == QS_VARIANT_SPLIT && !(split_eligible(op->o, false) && split_eligible(op->o, true)))
+            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2 or QS_VARIANT_SPLIT + 0, 1, 2");
+        if (lanes_per_row >= QS_VARIANT_SPLIT && !(split_eligible(op->o, false) && split_eligible(op->o, true)))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
         if (lanes_per_row >= QS_VARIANT_HOP && lanes_per_row <= QS_VARIANT_HOP + 2 &&
@@ -672,7 +672,8 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
         std::vector<int> cands;
         if (hop_eligible(o, false))
             cands.insert(cands.end(), {QS_VARIANT_HOP, QS_VARIANT_HOP + 1, QS_VARIANT_HOP + 2});
-        if (split_eligible(o, false) && split_eligible(o, true)) cands.push_back(QS_VARIANT_SPLIT);
+        if (split_eligible(o, false) && split_eligible(o, true))
+            cands.insert(cands.end(), {QS_VARIANT_SPLIT, QS_VARIANT_SPLIT + 1, QS_VARIANT_SPLIT + 2});
         if (o.basis->mode == MODE_STORED) cands.insert(cands.end(), {1, 2, 4, 8});
         else cands.push_back(1);
         for (int lpr : cands) {

@@ -745,6 +745,7 @@ struct SplitDev {
     const uint16_t *words;
     int cls_off[34], cls_n[34];
     int nlow, nhigh, ncross;
+    int chunk;                   // blocks per XCD chunk
     int nstage;                  // cross blocks staged in LDS per wavefront
     int xstride;                 // elements per staged block
     int wave_lds;                // bytes of one wavefront's LDS region
@@ -782,7 +783,8 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
                                                            const TermDev *__restrict__ drest,
                                                            const V *__restrict__ x, V *__restrict__ y) {
     using T = VTraits<V>;
-    constexpr int kNB = sizeof(V) <= 8 ? 4 : 2;   // high hops per batch (x kSpRows loads in flight)
+    // high hops per batch (x kSpRows loads in flight)
+    constexpr int kNB = sizeof(V) <= 4 ? 10 : (sizeof(V) <= 8 ? 4 : 2);
     extern __shared__ __align__(16) char smem[];
     uint16_t *s_rank = (uint16_t *)(smem + D.off_rank);
     uint16_t *s_words = (uint16_t *)(smem + D.off_words);
@@ -797,23 +799,47 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
     V *s_x = (V *)(smem + D.off_wave + wave * D.wave_lds + D.off_xw);
     // lane k < nhigh owns high hop group k, lane nhigh + k cross group k,
     // for every block: their constants stay in registers
-    HopDev mine{};
     const bool is_high = lane < D.nhigh;
     const bool is_cross = !is_high && lane < D.nhigh + D.ncross;
-    if (is_high || is_cross) mine = hops[D.nlow + lane];
+    int g_lo = 0, g_hi = 0, g_nz = 0, g_sym = 0, g_top = 0;
+    double g_re1 = 0, g_re2 = 0, g_im1 = 0, g_im2 = 0;
+    if (is_high || is_cross) {
+        const HopDev &hg = hops[D.nlow + lane];
+        g_lo = hg.lo;
+        g_hi = hg.hi;
+        g_nz = hg.nzmask;
+        g_sym = hg.sym;
+        g_top = hg.top;
+        g_re1 = hg.re[1];
+        g_re2 = hg.re[2];
+        g_im1 = hg.im[1];
+        g_im2 = hg.im[2];
+    }
     double dot_r = 0, dot_i = 0;
-    const int gw = blockIdx.x * (kSpBlock / 64) + wave, nwaves = gridDim.x * (kSpBlock / 64);
-    int vb = gw;
+    // Blocks come in chunks of D.chunk; chunk c belongs to XCD c % 8 (the
+    // grid is a multiple of 8 and hardware places workgroup b on XCD b % 8),
+    // and the wavefronts of one XCD deal its chunks out in order.  The eight
+    // XCDs advance side by side, so the chip sweeps the blocks in ascending
+    // order (far hops find their blocks in the MALL) while a block's near
+    // neighbours were read on the same XCD (its L2).
+    const int xcd = blockIdx.x & 7;
+    const int nwx = (int)(gridDim.x >> 3) * (kSpBlock / 64);
+    const int64_t C = D.chunk;
+    auto block_of = [&](int64_t q) { return (8 * (q / C) + xcd) * C + q % C; };
+    int64_t q = (int64_t)(blockIdx.x >> 3) * (kSpBlock / 64) + wave;
+    int64_t vb = block_of(q);
     uint32_t hn = 0, on = 0;
     if (vb < D.nblocks) {
         hn = D.blist[vb];
         on = D.bstart[vb];
     }
-    for (; vb < D.nblocks; vb += nwaves) {
+    while (vb < D.nblocks) {
         const uint32_t h = hn, off = on;
-        if (vb + nwaves < D.nblocks) {   // the next block's header, in flight meanwhile
-            hn = D.blist[vb + nwaves];
-            on = D.bstart[vb + nwaves];
+        q += nwx;
+        vb = block_of(q);
+        if (vb < D.nblocks) {   // the next block's header, in flight meanwhile
+            hn = D.blist[vb];
+            on = D.bstart[vb];
         }
         const int p = A.nup - __popc(h);
         const int n = D.cls_n[p];
@@ -832,14 +858,15 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
         bool live = false;
         SpHop e{};
         if (is_high) {
-            const int lo = mine.lo - D.H, hi = mine.hi - D.H;
+            const int lo = g_lo - D.H, hi = g_hi - D.H;
             const uint32_t blo = (h >> lo) & 1, bhi = (h >> hi) & 1;
-            live = (mine.nzmask >> (blo | (bhi << 1))) & 1;
-            double m_r = mine.sym ? mine.re[1] : (blo ? mine.re[1] : mine.re[2]);
-            double m_i = CC ? (mine.sym ? mine.im[1] : (blo ? mine.im[1] : mine.im[2])) : 0.0;
+            live = (g_nz >> (blo | (bhi << 1))) & 1;
+            // with a particle number only patterns 1 (bit lo set) and 2 are live
+            double m_r = (g_sym || blo) ? g_re1 : g_re2;
+            double m_i = CC ? ((g_sym || blo) ? g_im1 : g_im2) : 0.0;
             uint32_t h2 = h ^ ((1u << lo) | (1u << hi));
             uint32_t fl = 0;
-            if (CM == CM_ZONLY && mine.top) {
+            if (CM == CM_ZONLY && g_top) {
                 // the hop sets bit L-1: the column is the inverted state's
                 h2 = ~h2 & D.wmask;
                 fl = 1;
@@ -854,13 +881,13 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
             e.im = -m_i;   // conj(amp): the terms are the adjoint couplings
         } else if (is_cross) {
             // the rows whose bit lo differs from h's bit hi move
-            const int hi = mine.hi - D.H;
+            const int hi = g_hi - D.H;
             const uint32_t bhi = (h >> hi) & 1;
             const int pat = bhi ? 2 : 1;
-            double m_r = mine.re[pat];
-            double m_i = CC ? mine.im[pat] : 0.0;
+            double m_r = bhi ? g_re2 : g_re1;
+            double m_i = CC ? (bhi ? g_im2 : g_im1) : 0.0;
             uint32_t h2 = h ^ (1u << hi);
-            const int inv = CM == CM_ZONLY && mine.top;
+            const int inv = CM == CM_ZONLY && g_top;
             if (inv) {
                 h2 = ~h2 & D.wmask;
                 if (A.zchar < 0) {
@@ -869,11 +896,11 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
                 }
             }
             const int p2 = A.nup - __popc(h2);
-            live = ((mine.nzmask >> pat) & 1) && p2 >= 0 && p2 <= D.H;
+            live = ((g_nz >> pat) & 1) && p2 >= 0 && p2 <= D.H;
             const uint32_t o2 = live ? D.boff[h2] : 0xFFFFFFFFu;
             live = live && o2 != 0xFFFFFFFFu;   // else no row of this block moves that way
             e.off = o2;
-            e.flags = (uint32_t)inv | (bhi << 1) | ((uint32_t)mine.lo << 8) |
+            e.flags = (uint32_t)inv | (bhi << 1) | ((uint32_t)g_lo << 8) |
                       ((uint32_t)(live ? D.cls_n[p2] : 0) << 16);
             e.re = m_r;
             e.im = -m_i;
@@ -889,56 +916,94 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
             if (r < D.xstride) T::store(s_x, r, xo_r[j], xo_i[j]);   // s_x[n] = 0: dead neighbours read it
         }
         wave_lds_sync();
-        // staged cross blocks: loads of all of them first
-        const int ns = min(nc, D.nstage);
-        for (int c = 0; c < ns; c++) {
-            const SpHop ec = s_list[D.nhigh + c];
-            const uint32_t n2 = ec.flags >> 16;
-            double u_r[kSpRows], u_i[kSpRows];
-#pragma unroll
-            for (int j = 0; j < kSpRows; j++) {
-                const int i = lane + 64 * j;
-                u_r[j] = 0;
-                u_i[j] = 0;
-                if (i < (int)n2) T::load(x, (int64_t)ec.off + i, u_r[j], u_i[j]);
-                if (A.conj_x) u_i[j] = -u_i[j];
-            }
-#pragma unroll
-            for (int j = 0; j < kSpRows; j++) {
-                const int i = lane + 64 * j;
-                if (i < D.xstride) T::store(s_x, (int64_t)(1 + c) * D.xstride + i, u_r[j], u_i[j]);
-            }
-        }
-        // high hops: whole neighbour blocks, kNB x kSpRows loads in flight
+        // One round of loads for the first kNB high hops and the staged cross
+        // blocks together; later batches of high hops (if any) follow.
         double ar[kSpRows], ai[kSpRows];
 #pragma unroll
         for (int j = 0; j < kSpRows; j++) {
             ar[j] = 0;
             ai[j] = 0;
         }
-        for (int k0 = 0; k0 < nh; k0 += kNB) {
-            double vr[kNB][kSpRows], vi[kNB][kSpRows];
-#pragma unroll
-            for (int kk = 0; kk < kNB; kk++) {
-                const int k = min(k0 + kk, nh - 1);
-                const uint32_t eoff = __builtin_amdgcn_readfirstlane(s_list[k].off);
-                const uint32_t rev = __builtin_amdgcn_readfirstlane(s_list[k].flags) & 1;
-                const V *xb = x + eoff;
+        const int ns = min(nc, D.nstage);
+        auto stage_cross = [&]() {
+            for (int c = 0; c < ns; c++) {
+                const SpHop ec = s_list[D.nhigh + c];
+                const uint32_t n2 = ec.flags >> 16;
+                V u[kSpRows];
 #pragma unroll
                 for (int j = 0; j < kSpRows; j++) {
-                    const int r = lane + 64 * j;
-                    vr[kk][j] = 0;
-                    vi[kk][j] = 0;
-                    if (k0 + kk < nh && r < n) T::load(xb, rev ? n - 1 - r : r, vr[kk][j], vi[kk][j]);
+                    const int i = lane + 64 * j;
+                    u[j] = V{};
+                    if (i < (int)n2) u[j] = x[(int64_t)ec.off + i];
+                }
+#pragma unroll
+                for (int j = 0; j < kSpRows; j++) {
+                    const int i = lane + 64 * j;
+                    if (i < D.xstride) {
+                        double ur, ui;
+                        T::load(&u[j], 0, ur, ui);
+                        T::store(s_x, (int64_t)(1 + c) * D.xstride + i, ur, A.conj_x ? -ui : ui);
+                    }
                 }
             }
+        };
+        for (int k0 = 0; k0 == 0 || k0 < nh; k0 += kNB) {
+            if constexpr (sizeof(V) > 4) {
+                // 8- and 16-byte elements: widened to FP64 as they land
+                double vr[kNB][kSpRows], vi[kNB][kSpRows];
 #pragma unroll
-            for (int kk = 0; kk < kNB; kk++) {
-                if (k0 + kk < nh) {
-                    const double wr = s_list[k0 + kk].re, wi = s_list[k0 + kk].im;
+                for (int kk = 0; kk < kNB; kk++) {
+                    const int k = max(0, min(k0 + kk, nh - 1));
+                    const uint32_t eoff = __builtin_amdgcn_readfirstlane(s_list[k].off);
+                    const uint32_t rev = __builtin_amdgcn_readfirstlane(s_list[k].flags) & 1;
+                    const V *xb = x + eoff;
 #pragma unroll
-                    for (int j = 0; j < kSpRows; j++)
-                        acc_nb<V>(wr, wi, vr[kk][j], A.conj_x ? -vi[kk][j] : vi[kk][j], ar[j], ai[j]);
+                    for (int j = 0; j < kSpRows; j++) {
+                        const int r = lane + 64 * j;
+                        vr[kk][j] = 0;
+                        vi[kk][j] = 0;
+                        if (k0 + kk < nh && r < n) T::load(xb, rev ? n - 1 - r : r, vr[kk][j], vi[kk][j]);
+                    }
+                }
+                if (k0 == 0) stage_cross();
+#pragma unroll
+                for (int kk = 0; kk < kNB; kk++) {
+                    if (k0 + kk < nh) {
+                        const double wr = s_list[k0 + kk].re, wi = s_list[k0 + kk].im;
+#pragma unroll
+                        for (int j = 0; j < kSpRows; j++)
+                            acc_nb<V>(wr, wi, vr[kk][j], A.conj_x ? -vi[kk][j] : vi[kk][j], ar[j], ai[j]);
+                    }
+                }
+            } else {
+                // 4-byte elements: raw values, one register each
+                V raw[kNB][kSpRows];
+#pragma unroll
+                for (int kk = 0; kk < kNB; kk++) {
+                    const int k = max(0, min(k0 + kk, nh - 1));
+                    const uint32_t eoff = __builtin_amdgcn_readfirstlane(s_list[k].off);
+                    const uint32_t rev = __builtin_amdgcn_readfirstlane(s_list[k].flags) & 1;
+                    const V *xb = x + eoff;
+#pragma unroll
+                    for (int j = 0; j < kSpRows; j++) {
+                        const int r = lane + 64 * j;
+                        if (k0 + kk < nh && r < n) raw[kk][j] = xb[rev ? n - 1 - r : r];
+                    }
+                }
+                if (k0 == 0) stage_cross();
+#pragma unroll
+                for (int kk = 0; kk < kNB; kk++) {
+                    if (k0 + kk < nh) {
+                        const double wr = s_list[k0 + kk].re, wi = s_list[k0 + kk].im;
+#pragma unroll
+                        for (int j = 0; j < kSpRows; j++) {
+                            if (lane + 64 * j < n) {
+                                double vr, vi;
+                                T::load(&raw[kk][j], 0, vr, vi);
+                                acc_nb<V>(wr, wi, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -1292,8 +1357,8 @@ void split_classify(const OpHost &op) {
 }
 
 template <typename V, bool CC>
-void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, size_t lds, hipStream_t s, const HopDev *hops,
-                       const HopLaunch &H, const void *x, void *y) {
+void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, int fewer, size_t lds, hipStream_t s,
+                       const HopDev *hops, const HopLaunch &H, const void *x, void *y) {
     const V *xv = (const V *)x;
     V *yv = (V *)y;
     auto go = [&](auto kern) {
@@ -1306,10 +1371,15 @@ void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, size_t lds, h
         }
         int occ = 0;
         QS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kSpBlock, lds));
-        const int64_t want = (int64_t)std::max(1, occ) * cus;
+        // `fewer` resident workgroups per CU than fit: fewer blocks in flight
+        // keep more of the recently read ones in the MALL
+        const int64_t want = (int64_t)std::max(1, occ - fewer) * cus;
         const int64_t need = ((int64_t)D.nblocks + kSpBlock / 64 - 1) / (kSpBlock / 64);
-        const int grid = (int)std::max<int64_t>(1, std::min(want, need));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(kSpBlock), lds, s, a, D, hops, H.dcls, H.drest, xv, yv);
+        const int grid = (int)std::max<int64_t>(8, std::min(want, need) / 8 * 8);
+        // four blocks per wavefront per chunk
+        SplitDev Dc = D;
+        Dc.chunk = std::max(1, (grid / 8) * (kSpBlock / 64) * 4);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kSpBlock), lds, s, a, Dc, hops, H.dcls, H.drest, xv, yv);
     };
     if (cm == CM_ZONLY) go(k_matvec_split<V, CC, CM_ZONLY>);
     else go(k_matvec_split<V, CC, CM_FULL>);
@@ -1360,19 +1430,20 @@ void launch_split(const OpHost &op, MVArgs &a, int dtype, bool cc, bool adjoint,
     D.wave_lds = (int)((list_bytes + (size_t)(1 + D.nstage) * blk + 15) & ~(size_t)15);
     const size_t lds = (size_t)D.off_wave + (size_t)(kSpBlock / 64) * D.wave_lds;
     const int cm = b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL;
+    const int fewer = std::max(0, op.lanes_per_row - QS_VARIANT_SPLIT);
     HopLaunch H{nullptr, adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                 adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
     const HopDev *hops = sh.d.get();
     switch (dtype) {
-    case QS_F64: dispatch_split_cm<double, false>(a, D, cm, lds, s, hops, H, x, y); break;
-    case QS_F32: dispatch_split_cm<float, false>(a, D, cm, lds, s, hops, H, x, y); break;
+    case QS_F64: dispatch_split_cm<double, false>(a, D, cm, fewer, lds, s, hops, H, x, y); break;
+    case QS_F32: dispatch_split_cm<float, false>(a, D, cm, fewer, lds, s, hops, H, x, y); break;
     case QS_C128:
-        if (cc) dispatch_split_cm<cplx_d, true>(a, D, cm, lds, s, hops, H, x, y);
-        else dispatch_split_cm<cplx_d, false>(a, D, cm, lds, s, hops, H, x, y);
+        if (cc) dispatch_split_cm<cplx_d, true>(a, D, cm, fewer, lds, s, hops, H, x, y);
+        else dispatch_split_cm<cplx_d, false>(a, D, cm, fewer, lds, s, hops, H, x, y);
         break;
     case QS_C64:
-        if (cc) dispatch_split_cm<cplx_f, true>(a, D, cm, lds, s, hops, H, x, y);
-        else dispatch_split_cm<cplx_f, false>(a, D, cm, lds, s, hops, H, x, y);
+        if (cc) dispatch_split_cm<cplx_f, true>(a, D, cm, fewer, lds, s, hops, H, x, y);
+        else dispatch_split_cm<cplx_f, false>(a, D, cm, fewer, lds, s, hops, H, x, y);
         break;
     default: fail(QS_EARG, "unknown dtype");
     }
@@ -1387,7 +1458,9 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
-    if (op.lanes_per_row == QS_VARIANT_SPLIT && !dot_partial) {
+    // automatic choice: the split kernel for large implicit sectors
+    const bool auto_split = op.lanes_per_row == 0 && b.Ns >= ((int64_t)1 << 20) && split_eligible(op, adjoint);
+    if ((op.lanes_per_row >= QS_VARIANT_SPLIT || auto_split) && !dot_partial) {
         if (!split_eligible(op, adjoint))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
@@ -1396,7 +1469,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         return;
     }
     const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP && op.lanes_per_row <= QS_VARIANT_HOP + 2;
-    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SPLIT ||
+    const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row >= QS_VARIANT_SPLIT ||
                                                    want_hop);
     if (want_hop && !hop)
         fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");

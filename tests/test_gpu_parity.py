@@ -281,7 +281,7 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
         with pytest.raises(qsgpu.ArgumentError, match="split"):
             op.set_variant(qsgpu.VARIANT_SPLIT)
     lpr, ms = op.tune(np.complex128, repeats=2)
-    assert lpr in (1, 2, 4, 8) + hops + ((qsgpu.VARIANT_SPLIT,) if split else ()) and ms > 0
+    assert lpr in (1, 2, 4, 8) + hops + (tuple(qsgpu.VARIANT_SPLIT + k for k in range(3)) if split else ()) and ms > 0
     got = op.matvec(x)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
