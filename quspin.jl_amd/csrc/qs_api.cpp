@@ -907,7 +907,7 @@ int qs_blocks_evolve(qs_op *const *ops, const int32_t *kind, int n_ops, int dtyp
         std::vector<DevBuf<char>> x(n_ops), st;
         std::vector<const OpHost *> act_ops;
         std::vector<int> act;
-        VecWork wk;
+        VecWork wk(s);
         for (int j = 0; j < n_ops; j++) {
             const BasisHost &b = *ops[j]->o.basis;
             x[j].alloc((size_t)b.Ns * esz);

@@ -21,12 +21,14 @@ namespace {
 
 // ---------------------------------------------------- batched vector ops --
 struct BWork {
-    DevBuf<double> partial, result, coef;
+    explicit BWork(hipStream_t st) : s(st) {}
+    hipStream_t s;   // every launch using this workspace runs on it
+    SBuf<double> partial, result, coef;
     std::vector<double> host;
     void need(size_t p, size_t r, size_t c) {
-        if (partial.n < p) partial.alloc(std::max<size_t>(p, 4096));
-        if (result.n < r) result.alloc(std::max<size_t>(r, 1024));
-        if (coef.n < c) coef.alloc(std::max<size_t>(c, 1024));
+        if (partial.n < p) partial.alloc(std::max<size_t>(p, 4096), s);
+        if (result.n < r) result.alloc(std::max<size_t>(r, 1024), s);
+        if (coef.n < c) coef.alloc(std::max<size_t>(c, 1024), s);
     }
 };
 
@@ -274,7 +276,7 @@ void baxpby_dev(int dtype, const std::vector<AxpbyDev> &rq, hipStream_t s) {
 struct BFact {
     int op;
     int64_t n;
-    DevBuf<char> V;     // n x (steps + 1)
+    SBuf<char> V;       // n x (steps + 1)
     KrylovProj P;
     void *col(int j, size_t esz) const { return (void *)(V.get() + (size_t)j * n * esz); }
 };
@@ -302,7 +304,7 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         f->P.dim = f->n;
         f->P.hermitian = lanczos_ok(op, dtype);
         f->P.steps = (int)std::min<int64_t>(krylov_dim, f->n);
-        f->V.alloc((size_t)f->n * (f->P.steps + 1) * esz);
+        f->V.alloc((size_t)f->n * (f->P.steps + 1) * esz, s);
         f->P.H.assign((size_t)(f->P.steps + 1) * f->P.steps, cd(0, 0));
         f->P.completed = f->P.steps;
         f->P.terminal = 0;
@@ -313,7 +315,7 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         F.push_back(std::move(f));
     }
     bwork_reserve(wk, ns, max_steps + 1);
-    DevBuf<double> d(std::max<size_t>(tot, 1));
+    SBuf<double> d(std::max<size_t>(tot, 1), s);
     auto sc = [&](size_t i) { return FactScalars(F[i]->P.steps); };
     {
         std::vector<std::pair<const void *, int64_t>> xs;
@@ -398,7 +400,7 @@ struct Call {
     double tol;
     int depth;
     void *out;
-    std::shared_ptr<DevBuf<char>> vbuf, obuf;
+    std::shared_ptr<SBuf<char>> vbuf, obuf;
 };
 struct Chain {              // the calls of one (task, time point), run in order
     int task;
@@ -414,7 +416,7 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
     if (krylov_dim <= 0) fail(QS_EARG, "krylov_dim must be positive");
     const size_t esz = dtype_size(dtype);
     auto bm = make_batch_matvec(ops, dtype);
-    BWork wk;
+    BWork wk(s);
     nmatvec.assign(tasks.size(), 0);
     auto outcol = [&](size_t t, size_t j) {
         return (void *)((char *)tasks[t].out + j * ops[tasks[t].op]->basis->Ns * esz);
@@ -543,7 +545,7 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
             ch.stack.pop_back();
             if (est[r] > cl.tol * std::max(1.0, std::sqrt(onr[r]))) {
                 if (cl.depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
-                auto mid = std::make_shared<DevBuf<char>>((size_t)nrows(ch.task) * esz);
+                auto mid = std::make_shared<SBuf<char>>((size_t)nrows(ch.task) * esz, s);
                 // second half (mid -> out) runs after the first (v -> mid): pushed first
                 ch.stack.push_back(Call{mid->get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid, cl.obuf});
                 ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid->get(), cl.vbuf, mid});

@@ -8,6 +8,8 @@
 #include <set>
 #include <tuple>
 
+#include <mutex>
+
 #include "qs_internal.h"
 
 namespace qs {
@@ -721,6 +723,22 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
     }
     E = Q;
     for (int i = 0; i < s; i++) E = mm(E, E);
+}
+
+void stream_pool_setup() {
+    // the default pool of the device keeps up to 8 GB of memory freed in
+    // stream order for reuse before handing it back at a synchronisation
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        uint64_t keep = 8ULL << 30;
+        if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) (void)hipGetLastError();
+    });
 }
 
 namespace {

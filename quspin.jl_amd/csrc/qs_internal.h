@@ -65,6 +65,42 @@ struct DevBuf {
     T *get() const { return p; }
 };
 
+// Device scratch allocated and freed in stream order (hipMallocAsync /
+// hipFreeAsync on the solver's stream).  Unlike hipFree, releasing it does
+// not wait for the whole device, so solvers on different streams (the
+// serving loop's workers) keep overlapping; the default pool keeps up to a
+// few GB of freed memory for reuse (stream_pool_setup).
+void stream_pool_setup();
+template <typename T>
+struct SBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipStream_t s = nullptr;
+    SBuf() = default;
+    SBuf(size_t count, hipStream_t st) { alloc(count, st); }
+    SBuf(const SBuf &) = delete;
+    SBuf &operator=(const SBuf &) = delete;
+    ~SBuf() { release(); }
+    void alloc(size_t count, hipStream_t st) {
+        release();
+        s = st;
+        if (count == 0) return;
+        stream_pool_setup();
+        if (hipMallocAsync((void **)&p, count * sizeof(T), s) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(QS_EOOM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        n = 0;
+    }
+    T *get() const { return p; }
+};
+
 // -------------------------------------------------------------- groups ----
 // A symmetry group element acts on an L-bit state as
 //   g(u) = rotl_L( rev_L^r(u) ^ f , t * a )
@@ -384,8 +420,10 @@ void expm_host(OpHost &o, hipStream_t s, int dtype, const void *v, int64_t n_tim
 
 // vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
 struct VecWork {
-    DevBuf<double> partial;   // per-block partial sums
-    DevBuf<double> result;    // reduced values (device)
+    explicit VecWork(hipStream_t st) : s(st) {}
+    hipStream_t s;            // the stream every launch using this workspace runs on
+    SBuf<double> partial;     // per-block partial sums
+    SBuf<double> result;      // reduced values (device)
     std::vector<double> host;
 };
 int vec_blocks(int64_t n);

@@ -25,10 +25,10 @@ struct Krylov {
     int dtype;
     int64_t n;
     size_t esz;
-    DevBuf<char> V;   // n x cols
+    SBuf<char> V;   // n x cols
     int cols = 0;
-    Krylov(int dt, int64_t n_, int c) : dtype(dt), n(n_), esz(dtype_size(dt)), cols(c) {
-        V.alloc((size_t)n * c * esz);
+    Krylov(int dt, int64_t n_, int c, hipStream_t s) : dtype(dt), n(n_), esz(dtype_size(dt)), cols(c) {
+        V.alloc((size_t)n * c * esz, s);
     }
     void *col(int j) const { return (void *)(V.get() + (size_t)j * n * esz); }
 };
@@ -96,8 +96,8 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     ncv = (int)std::min<int64_t>(ncv, n);
     if (ncv <= k) fail(QS_EARG, "ncv must exceed k");
     const double eps23 = std::pow(kEps, 2.0 / 3.0);
-    Krylov K(dtype, n, ncv + 1);
-    VecWork wk;
+    Krylov K(dtype, n, ncv + 1, s);
+    VecWork wk(s);
     // starting vector
     if (v0_dev) {
         QS_HIP(hipMemcpyAsync(K.col(0), v0_dev, (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
@@ -120,7 +120,7 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     // per-step scalars of one Lanczos cycle, device side: two Gram passes
     // (m + 1 complex each) and |w|^2
     const size_t per = (size_t)4 * (m + 1) + 2;
-    DevBuf<double> d(per * m);
+    SBuf<double> d(per * m, s);
     std::vector<double> host(per * m);
     vec_reserve(wk, n, m + 1);
     for (int iter = 0; iter < maxiter; iter++) {
@@ -197,7 +197,7 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
         std::vector<cd> C((size_t)m * keep);
         for (int c = 0; c < keep; c++)
             for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + sel[c]], 0);
-        Krylov tmp(dtype, n, keep);
+        Krylov tmp(dtype, n, keep, s);
         basis_combine(dtype, K.V.get(), n, m, C, keep, tmp.V.get(), s);
         QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * keep * K.esz, hipMemcpyDeviceToDevice, s));
         QS_HIP(hipMemcpyAsync(K.col(keep), K.col(m), (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
@@ -227,7 +227,7 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
         std::vector<cd> C((size_t)m * k);
         for (int c = 0; c < k; c++)
             for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + out[c]], 0);
-        Krylov tmp(dtype, n, vecs_dev ? 0 : k);
+        Krylov tmp(dtype, n, vecs_dev ? 0 : k, s);
         void *dst = vecs_dev ? vecs_dev : tmp.V.get();
         basis_combine(dtype, K.V.get(), n, m, C, k, dst, s);
         if (fp32) {
@@ -278,7 +278,7 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
     P.dim = n;
     P.hermitian = lanczos_ok(op, dtype);
     P.steps = (int)std::min<int64_t>(krylov_dim, n);
-    F.K = std::make_unique<Krylov>(dtype, n, P.steps + 1);
+    F.K = std::make_unique<Krylov>(dtype, n, P.steps + 1, s);
     Krylov &K = *F.K;
     P.H.assign((size_t)(P.steps + 1) * P.steps, cd(0, 0));
     P.completed = P.steps;
@@ -286,7 +286,7 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
     const int steps = P.steps;
     vec_reserve(wk, n, steps + 1);
     FactScalars L(steps);
-    DevBuf<double> d(L.total());
+    SBuf<double> d(L.total(), s);
     vec_norm2_dev(dtype, v, n, wk, d.get() + L.beta2(), s);
     vec_axpby_dev(dtype, d.get() + L.beta2(), CO_INV_SQRT, v, cd(0, 0), K.col(0), n, s);
     for (int col = 0; col < steps; col++) {
@@ -338,7 +338,7 @@ void expmv_vector(OpHost &op, int dtype, const void *v, cd scale, double tol, in
     if (est > tol * std::max(1.0, nr)) {
         if (depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
         F.K.reset();
-        DevBuf<char> mid((size_t)n * dtype_size(dtype));
+        SBuf<char> mid((size_t)n * dtype_size(dtype), s);
         expmv_vector(op, dtype, v, scale / 2.0, tol / 2, krylov_dim, depth + 1, mid.get(), wk, nmv);
         expmv_vector(op, dtype, mid.get(), scale / 2.0, tol / 2, krylov_dim, depth + 1, out, wk, nmv);
     }
@@ -354,7 +354,7 @@ void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<c
     if (!dtype_complex(dtype)) fail(QS_EARG, "expm_multiply needs a complex vector dtype");
     if (tol <= 0) fail(QS_EARG, "tol must be positive");
     if (krylov_dim <= 0) fail(QS_EARG, "krylov_dim must be positive");
-    VecWork wk;
+    VecWork wk(s);
     auto outcol = [&](size_t j) { return (void *)((char *)out + j * n * esz); };
     bool all_zero = true;
     for (auto &z : scales) all_zero &= z == cd(0, 0);
@@ -491,12 +491,12 @@ void lanczos_device(OpHost &op, int dtype, const void *v0, int m, bool full_orth
     const int64_t n = op.basis->Ns;
     const size_t esz = dtype_size(dtype);
     if (!(1 <= m && m < n)) fail(QS_EARG, "Lanczos dimension m must satisfy 1 <= m < length(v0)");
-    VecWork wk;
+    VecWork wk(s);
     const double nrm = std::sqrt(vec_norm2(dtype, v0, n, wk, s));
     if (!(nrm > 0)) fail(QS_EARG, "initial Lanczos vector must be nonzero");
     const double tol = eps > 0 ? eps : ((dtype == QS_F32 || dtype == QS_C64) ? 1.1920928955078125e-07 : kEps);
     auto col = [&](int j) { return (void *)((char *)Q + (size_t)j * n * esz); };
-    DevBuf<char> res((size_t)n * esz), qprev((size_t)n * esz);
+    SBuf<char> res((size_t)n * esz, s);
     matvec(op, dtype, v0, res.get());
     vec_scale_copy(dtype, res.get(), res.get(), n, cd(1.0 / nrm, 0), s);
     vec_scale_copy(dtype, v0, col(0), n, cd(1.0 / nrm, 0), s);
@@ -506,7 +506,7 @@ void lanczos_device(OpHost &op, int dtype, const void *v0, int m, bool full_orth
     // every step queued with its scalars in device memory: per index the
     // alpha dot, |res|^2 and (full_ortho) the m projection dots
     const size_t per = (size_t)2 * (m + 2);
-    DevBuf<double> d(per * m);
+    SBuf<double> d(per * m, s);
     vec_reserve(wk, n, 1);
     for (int index = 0; index < m; index++) {
         double *ha = d.get() + per * index, *n2 = ha + 2, *hp = ha + 4;

@@ -311,8 +311,8 @@ void by_dtype(int dtype, F &&f) {
 }
 
 void ensure(VecWork &wk, size_t partial, size_t result) {
-    if (wk.partial.n < partial) wk.partial.alloc(std::max(partial, (size_t)2 * kMaxBlocks * 64));
-    if (wk.result.n < result) wk.result.alloc(std::max(result, (size_t)2 * 512));
+    if (wk.partial.n < partial) wk.partial.alloc(std::max(partial, (size_t)2 * kMaxBlocks * 64), wk.s);
+    if (wk.result.n < result) wk.result.alloc(std::max(result, (size_t)2 * 512), wk.s);
 }
 
 }  // namespace
@@ -477,7 +477,7 @@ void basis_combine(int dtype, const void *Vmat, int64_t n, int m, const std::vec
         cc[2 * i] = C[i].real();
         cc[2 * i + 1] = C[i].imag();
     }
-    DevBuf<double> dc(cc.size());
+    SBuf<double> dc(cc.size(), s);
     QS_HIP(hipMemcpyAsync(dc.get(), cc.data(), sizeof(double) * cc.size(), hipMemcpyHostToDevice, s));
     by_dtype(dtype, [&](auto *tag) {
         using V = std::remove_pointer_t<decltype(tag)>;
