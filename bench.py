@@ -241,6 +241,19 @@ def main():
                              "the whole working set sits in L2/MALL at this N, so the bound is issue and "
                              "gather latency, not HBM"},
     }
+    # the same sector matrix assembled once on the device (the reference's
+    # static_fmt=:csr path): plain products read the stored rows
+    ta = time.perf_counter()
+    op_a = qsgpu.QuantumLinearOperator(basis, terms).assemble()
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - ta
+    op_a.set_variant(qsgpu.VARIANT_SELL)
+    t_sell, _ = time_matvec(op_a, x, y, args.steps, args.warmup)
+    result["c2_assembled"] = {"ms": round(t_sell * 1e3, 5), "assemble_s": round(t_asm, 4),
+                              "GBps_algorithmic": round(bytes_alg / t_sell / 1e9, 3),
+                              "speedup_vs_matrix_free": round(t_mv / t_sell, 3),
+                              "kernel": "k_sell_spmv (SELL-64 rows assembled by the pull kernel's enumeration)"}
+    del op_a
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)

@@ -142,9 +142,28 @@ int qs_op_stream(const qs_op *op, void **stream);
  * QS_VARIANT_SPLIT + k (k = 0, 1, 2) keeps k fewer workgroups per CU
  * resident than fit.  No reference counterpart. */
 #define QS_VARIANT_SPLIT 32
+/* Plain products from the assembled sector matrix (qs_op_assemble).  The
+ * reference's counterpart is the assembled Hamiltonian, static_fmt = :csr. */
+#define QS_VARIANT_SELL 48
 int qs_op_set_variant(qs_op *op, int lanes_per_row);
 
-/* Autotuner: times every variant for `dtype` on device-resident random
+/* Hamiltonian(...; static_fmt=:csr) assembly -- operators.jl:461-576
+ * (_assemble: P'.H_par.P for a sector, sparse_formats.jl:21-50 storage):
+ * builds the sector matrix of the plain product on the device (sliced ELL,
+ * 64-row slices) from the same neighbour enumeration as the matrix-free
+ * kernel.  Afterwards plain products read the stored rows (variant 0 or
+ * QS_VARIANT_SELL); transposed / adjoint products stay matrix-free.  Needs
+ * two-bit hop couplings (QS_VARIANT_HOP's condition). */
+int qs_op_assemble(qs_op *op);
+
+/* The assembled matrix as CSR (SparseMatrixCSR, sparse_formats.jl:21-50):
+ * duplicate columns merged and exact zeros dropped (operators.jl:567-572),
+ * columns ascending; val interleaved complex (2*nnz doubles).  With rowptr,
+ * col and val all NULL only *nnz is written; otherwise *nnz must hold the
+ * capacity on entry. */
+int qs_op_csr(const qs_op *op, int64_t *nnz, int64_t *rowptr, int64_t *col, double *val);
+
+/* Autotuner: times every variant for `dtype` on device-resident random/* Autotuner: times every variant for `dtype` on device-resident random
  * vectors (`repeats` launches each), keeps the fastest on the operator and
  * returns it with its time.  No reference counterpart. */
 int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *best_ms);

@@ -642,9 +642,13 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
         check_ptr(op, "op");
         if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 &&
             lanes_per_row != 8 && (lanes_per_row < QS_VARIANT_SPLIT || lanes_per_row > QS_VARIANT_SPLIT + 2) &&
-            (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 2))
-            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2 or QS_VARIANT_SPLIT + 0, 1, 2");
-        if (lanes_per_row >= QS_VARIANT_SPLIT && !(split_eligible(op->o, false) && split_eligible(op->o, true)))
+            (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 2) && lanes_per_row != QS_VARIANT_SELL)
+            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2, QS_VARIANT_SPLIT + 0, 1, 2 or "
+                          "QS_VARIANT_SELL");
+        if (lanes_per_row == QS_VARIANT_SELL && !op->o.sell)
+            fail(QS_EARG, "the assembled variant needs qs_op_assemble first");
+        if (lanes_per_row >= QS_VARIANT_SPLIT && lanes_per_row <= QS_VARIANT_SPLIT + 2 &&
+            !(split_eligible(op->o, false) && split_eligible(op->o, true)))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
         if (lanes_per_row >= QS_VARIANT_HOP && lanes_per_row <= QS_VARIANT_HOP + 2 &&
@@ -674,6 +678,7 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
             cands.insert(cands.end(), {QS_VARIANT_HOP, QS_VARIANT_HOP + 1, QS_VARIANT_HOP + 2});
         if (split_eligible(o, false) && split_eligible(o, true))
             cands.insert(cands.end(), {QS_VARIANT_SPLIT, QS_VARIANT_SPLIT + 1, QS_VARIANT_SPLIT + 2});
+        if (o.sell) cands.push_back(QS_VARIANT_SELL);
         if (o.basis->mode == MODE_STORED) cands.insert(cands.end(), {1, 2, 4, 8});
         else cands.push_back(1);
         for (int lpr : cands) {
@@ -697,6 +702,34 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
         o.lanes_per_row = best;
         if (lanes_per_row) *lanes_per_row = best;
         if (best_ms) *best_ms = best_t;
+    });
+}
+
+int qs_op_assemble(qs_op *op) {
+    return guard([&] {
+        check_ptr(op, "op");
+        OpHost &o = op->o;
+        hipStream_t s = ensure_stream(o);
+        sell_assemble(o, s);
+    });
+}
+
+int qs_op_csr(const qs_op *op, int64_t *nnz, int64_t *rowptr, int64_t *col, double *val) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(nnz, "nnz");
+        const OpHost &o = op->o;
+        if (!o.sell) fail(QS_EARG, "the operator is not assembled (qs_op_assemble)");
+        std::vector<int64_t> rp, c;
+        std::vector<double> v;
+        sell_to_csr(o, rp, c, v);
+        const int64_t have = *nnz;
+        *nnz = (int64_t)c.size();
+        if (!rowptr && !col && !val) return;   // size query
+        if (have < (int64_t)c.size()) fail(QS_EDIM, "nnz smaller than the assembled matrix");
+        if (rowptr) std::copy(rp.begin(), rp.end(), rowptr);
+        if (col) std::copy(c.begin(), c.end(), col);
+        if (val) std::copy(v.begin(), v.end(), val);
     });
 }
 

@@ -124,6 +124,8 @@ void bcombine(int dtype, const std::vector<CombReq> &rq, BWork &wk, hipStream_t 
     std::vector<VecJob> a;
     std::vector<int> ab;
     for (size_t i = 0; i < rq.size(); i++) {
+        // a job with many output columns is a GEMM: its own launch on the matrix cores
+        if (combine_on_mfma(dtype, rq[i].V, rq[i].n, rq[i].m, wk.coef.get() + co[i], rq[i].k, rq[i].Y, s)) continue;
         VecJob j{};
         j.V = rq[i].V;
         j.n = rq[i].n;
@@ -135,7 +137,7 @@ void bcombine(int dtype, const std::vector<CombReq> &rq, BWork &wk, hipStream_t 
         a.push_back(j);
         ab.push_back(j.nb);
     }
-    launch_vec_jobs(dtype, VOP_COMBINE, a, ab, s);
+    if (!a.empty()) launch_vec_jobs(dtype, VOP_COMBINE, a, ab, s);
     // the coefficient buffer is reused by the next call
     QS_HIP(hipStreamSynchronize(s));
 }

@@ -204,6 +204,19 @@ struct SplitHops {
     DevBuf<HopDev> d;
 };
 
+// The assembled sector matrix of an operator (plain products), SELL-64:
+// rows in slices of 64, slice q padded to its longest row, column-major.
+struct SellHost {
+    int64_t nslices = 0;
+    uint64_t stored = 0;          // entries including padding
+    uint64_t entries = 0;         // entries without padding
+    bool cplx = false;            // complex weights
+    DevBuf<uint64_t> sptr;        // first entry of each slice
+    DevBuf<uint32_t> width;       // entries per row of each slice
+    DevBuf<uint32_t> cols;
+    DevBuf<double> wre, wim;
+};
+
 struct BasisHost {
     int L = 0, nup = -1, pauli = 1, a = 1;
     int blocks[6];               // k, p, z, pz, zA, zB (QS_UNSET if absent)
@@ -271,6 +284,8 @@ struct OpHost {
     DevBuf<HopDev> d_hops, d_hops_adj;
     DevBuf<DiagClass> d_dcls, d_dcls_adj;
     DevBuf<TermDev> d_drest, d_drest_adj;
+    // assembled sector matrix (qs_op_assemble), used for plain products
+    std::unique_ptr<SellHost> sell;
     // hop groups classified for the split kernel (built once, on first use)
     mutable std::once_flag split_once;
     mutable SplitHops split_hops, split_hops_adj;
@@ -343,6 +358,12 @@ int matvec_grid(int64_t n);
 bool hop_eligible(const OpHost &op, bool adjoint);
 // the split kernel applies (hop couplings, implicit basis with a particle number)
 bool split_eligible(const OpHost &op, bool adjoint);
+// Assemble the sector matrix of the plain product (hop-eligible operators),
+// and read it back as CSR with duplicate columns merged, in ascending column
+// order (rowptr n+1, col / val nnz; val interleaved complex).
+void sell_assemble(OpHost &op, hipStream_t s);
+void sell_to_csr(const OpHost &op, std::vector<int64_t> &rowptr, std::vector<int64_t> &col,
+                 std::vector<double> &val);
 // <x, A x> in FP64 with x stored in `dtype` (fused into the matvec epilogue); y is scratch
 cd rayleigh_numerator(OpHost &op, int dtype, const void *x, void *y, VecWork &wk);
 
@@ -452,6 +473,11 @@ void vec_axpby(int dtype, cd a, const void *x, cd b, void *y, int64_t n, hipStre
 void basis_combine(int dtype, const void *V, int64_t n, int m, const std::vector<cd> &C, int k,
                    void *Y, hipStream_t s);
 void vec_random(int dtype, void *y, int64_t n, uint64_t seed, hipStream_t s);
+// Y = V C on the matrix cores when there are at least kMfmaMinCols output
+// columns and the storage is FP64 (real or complex); false (nothing
+// launched) otherwise.  C is the device copy, interleaved complex m x k.
+constexpr int kMfmaMinCols = 16;
+bool combine_on_mfma(int dtype, const void *V, int64_t n, int m, const double *C, int k, void *Y, hipStream_t s);
 void vec_convert(int from_dtype, const void *x, int to_dtype, void *y, int64_t n, hipStream_t s);
 size_t dtype_size(int dtype);
 bool dtype_complex(int dtype);

@@ -459,6 +459,60 @@ __device__ __forceinline__ void hop_state(const MVArgs &A, uint32_t row, const u
     }
 }
 
+// Neighbour of state s (parent rank / column rs) under hop group h: whether
+// the group acts on s, the target (column for the implicit modes, parent
+// rank for the parent table) and the amplitude with the inversion character.
+// The same enumeration as hop_groups, which keeps its copy inline: sharing
+// this function there costs the matvec registers and occupancy.
+template <bool CC, int CM, bool PC>
+__device__ __forceinline__ void hop_target(const MVArgs &A, const HopDev &h, uint32_t s, uint32_t rs,
+                                           const uint32_t *s_binom, bool &live, uint32_t &c, double &m_r,
+                                           double &m_i) {
+    const uint32_t mask = (uint32_t)A.mask;
+    const uint32_t blo = (s >> h.lo) & 1, bhi = (s >> h.hi) & 1;
+    live = (h.nzmask >> (blo | (bhi << 1))) & 1;
+    m_i = 0;
+    if (PC) {
+        if (h.sym) {
+            m_r = h.re[1];
+            if (CC) m_i = h.im[1];
+        } else {
+            m_r = blo ? h.re[1] : h.re[2];
+            if (CC) m_i = blo ? h.im[1] : h.im[2];
+        }
+    } else {
+        m_r = bhi ? (blo ? h.re[3] : h.re[2]) : (blo ? h.re[1] : h.re[0]);
+        if (CC) m_i = bhi ? (blo ? h.im[3] : h.im[2]) : (blo ? h.im[1] : h.im[0]);
+    }
+    if (PC) {
+        // colex rank change of the hop (bits lo and hi differ)
+        const int cl = popc(s & h.lowmask);
+        uint32_t bw = s & h.between;
+        uint32_t d = s_binom[h.hi65 + cl + popc(bw) + 1] - s_binom[h.lo65 + cl + 1];
+        if (h.between) {
+            int l = 1;
+            while (bw) {
+                const int q = ctz(bw);
+                d += s_binom[q * 65 + cl + l] - s_binom[q * 65 + cl + l + 1];
+                l++;
+                bw &= bw - 1;
+            }
+        }
+        c = rs + (blo ? d : 0u - d);
+    } else {
+        c = s ^ ((1u << h.lo) | (1u << h.hi));
+    }
+    if (CM == CM_ZONLY && h.top) {
+        // s has bit L-1 clear, so the hop sets it: the column is
+        // the inverted state's, with the inversion character
+        c = PC ? A.cfull - 1 - c : (~c & mask);
+        if (A.zchar < 0) {
+            m_r = -m_r;
+            m_i = -m_i;
+        }
+    }
+}
+
 // Adds the flip groups g = g_first, g_first + WS, ... of row `row` (own
 // state s, rank rs) to (ar, ai), kNB neighbours per batch.
 template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
@@ -707,6 +761,142 @@ __global__ __launch_bounds__(kMvBlock) void kb_matvec_hop(MVJobs J) {
     const MVJob &q = J.job[lo];
     hop_body<V, CC, CM, TRACK, PC, 1>(*q.A, q.hops, q.dcls, q.drest, (const V *)q.x, (V *)q.y,
                                       (int)blockIdx.x - q.b0, q.nb, smem);
+}
+
+// ------------------------------------------------ assembled sector matrix --
+// The sector matrix M = P'HP of a hop-eligible operator, assembled once in
+// sliced ELL form (SELL-64): rows in slices of 64; slice q is padded to the
+// length w_q of its longest row and stored column-major (entry k of row
+// 64q + i at sptr[q] + 64k + i), so a wavefront's loads coalesce.  A row's
+// entries are its diagonal, then every live hop group's neighbour in group
+// order, with the weights the pull kernel applies; padding points at the row
+// itself with weight 0.  Duplicate columns stay separate entries (the
+// product sums them); qs_op_csr merges them for export.
+constexpr int kSellBlock = 256;
+
+// one row's entries: emit(col, w_re, w_im)
+template <bool CC, int CM, bool TRACK, bool PC, typename F>
+__device__ __forceinline__ void row_entries(const MVArgs &A, const HopDev *__restrict__ hops,
+                                            const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
+                                            uint32_t row, const uint32_t *s_binom, F &&emit) {
+    uint32_t s, rs;
+    hop_state<CM, PC>(A, row, s_binom, s, rs);
+    double dr = 0, di = 0;
+    diag_value<CC>(A, dcls, drest, s, dr, di);
+    di = -di;   // conj(amp): the terms are the adjoint couplings
+    if (A.user_diag) {
+        dr += A.user_diag[2 * row];
+        di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+    }
+    emit(row, dr, di);
+    double own_scale = 1.0;
+    if (CM == CM_PTAB) own_scale = 1.0 / A.inv_nrm[row];
+    const uint32_t colmask = A.colbits >= 32 ? 0xFFFFFFFFu : ((1u << A.colbits) - 1);
+    for (int g = 0; g < A.ngroups; g++) {
+        const HopDev h = hops[g];
+        bool live;
+        uint32_t c;
+        double m_r, m_i;
+        hop_target<CC, CM, PC>(A, h, s, rs, s_binom, live, c, m_r, m_i);
+        if (!live) continue;
+        double wr = m_r, wi = -m_i;
+        uint32_t col = c;
+        if (CM == CM_PTAB) {
+            const uint32_t e = A.ptab[c];
+            if (e == 0xFFFFFFFFu) continue;
+            col = e & colmask;
+            double cr = 1.0, ci = 0.0;
+            if (TRACK && A.colbits < 32) {
+                const int elem = (int)(e >> A.colbits);
+                const int bt = elem / A.g.npoint, bp = elem - bt * A.g.npoint;
+                const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
+                const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
+                cr = lr * pr - li * pi;
+                ci = lr * pi + li * pr;
+            }
+            const double fac = A.inv_nrm[col] * own_scale;
+            cr *= fac;
+            ci *= fac;
+            wr = m_r * cr + m_i * ci;
+            wi = m_r * ci - m_i * cr;
+        }
+        emit(col, wr, wi);
+    }
+}
+
+template <bool CC, int CM, bool TRACK, bool PC>
+__global__ __launch_bounds__(kSellBlock) void k_sell_count(MVArgs A, const HopDev *__restrict__ hops,
+                                                           const DiagClass *__restrict__ dcls,
+                                                           const TermDev *__restrict__ drest, uint32_t *cnt) {
+    extern __shared__ __align__(16) char smem[];
+    uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
+    if (PC)
+        for (int i = threadIdx.x; i < 33 * 65; i += kSellBlock) s_binom[i] = (uint32_t)A.binom[i];
+    __syncthreads();
+    const uint32_t row = blockIdx.x * kSellBlock + threadIdx.x;
+    if (row >= (uint32_t)A.N) return;
+    uint32_t n = 0;
+    row_entries<CC, CM, TRACK, PC>(A, hops, dcls, drest, row, s_binom, [&](uint32_t, double, double) { n++; });
+    cnt[row] = n;
+}
+
+template <bool CC, int CM, bool TRACK, bool PC>
+__global__ __launch_bounds__(kSellBlock) void k_sell_fill(MVArgs A, const HopDev *__restrict__ hops,
+                                                          const DiagClass *__restrict__ dcls,
+                                                          const TermDev *__restrict__ drest,
+                                                          const uint64_t *__restrict__ sptr,
+                                                          const uint32_t *__restrict__ width, uint32_t *cols,
+                                                          double *wre, double *wim) {
+    extern __shared__ __align__(16) char smem[];
+    uint32_t *s_binom = (uint32_t *)(smem + A.off_binom);
+    if (PC)
+        for (int i = threadIdx.x; i < 33 * 65; i += kSellBlock) s_binom[i] = (uint32_t)A.binom[i];
+    __syncthreads();
+    const uint32_t row = blockIdx.x * kSellBlock + threadIdx.x;
+    if (row >= (uint32_t)A.N) return;
+    const uint32_t q = row >> 6, i = row & 63;
+    const uint64_t base = sptr[q] + i;
+    uint32_t k = 0;
+    row_entries<CC, CM, TRACK, PC>(A, hops, dcls, drest, row, s_binom, [&](uint32_t col, double wr, double wi) {
+        cols[base + 64 * (uint64_t)k] = col;
+        wre[base + 64 * (uint64_t)k] = wr;
+        if (wim) wim[base + 64 * (uint64_t)k] = wi;
+        k++;
+    });
+    for (; k < width[q]; k++) {
+        cols[base + 64 * (uint64_t)k] = row;
+        wre[base + 64 * (uint64_t)k] = 0.0;
+        if (wim) wim[base + 64 * (uint64_t)k] = 0.0;
+    }
+}
+
+// y = beta*y + alpha*M x from the assembled rows (WC: complex weights)
+template <typename V, bool WC>
+__global__ __launch_bounds__(kSellBlock) void k_sell_spmv(MVArgs A, const uint64_t *__restrict__ sptr,
+                                                          const uint32_t *__restrict__ width,
+                                                          const uint32_t *__restrict__ cols,
+                                                          const double *__restrict__ wre,
+                                                          const double *__restrict__ wim,
+                                                          const V *__restrict__ x, V *__restrict__ y) {
+    using T = VTraits<V>;
+    const int64_t row = (int64_t)blockIdx.x * kSellBlock + threadIdx.x;
+    if (row >= A.N) return;
+    const uint64_t q = (uint64_t)row >> 6;
+    const uint64_t base = sptr[q] + (row & 63);
+    const uint32_t w = width[q];
+    double ar = 0, ai = 0;
+    for (uint32_t k = 0; k < w; k++) {
+        const uint64_t at = base + 64 * (uint64_t)k;
+        const uint32_t c = cols[at];
+        const double er = wre[at], ei = WC ? wim[at] : 0.0;
+        double vr, vi;
+        T::load(x, c, vr, vi);
+        if (A.conj_x) vi = -vi;
+        ar += er * vr - ei * vi;
+        ai += er * vi + ei * vr;
+    }
+    double dot_r = 0, dot_i = 0;
+    finish_row<V>(A, y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
 }
 
 // ------------------------------------------------------- split kernel --
@@ -1449,7 +1639,134 @@ void launch_split(const OpHost &op, MVArgs &a, int dtype, bool cc, bool adjoint,
     }
 }
 
+template <bool CC, int CM, bool TRACK, bool PC>
+void sell_kernels(const MVArgs &a, const HopLaunch &H, size_t lds, hipStream_t s, int grid, uint32_t *cnt,
+                  const SellHost *t) {
+    if (!t) {
+        hipLaunchKernelGGL((k_sell_count<CC, CM, TRACK, PC>), dim3(grid), dim3(kSellBlock), lds, s, a, H.hops,
+                           H.dcls, H.drest, cnt);
+    } else {
+        hipLaunchKernelGGL((k_sell_fill<CC, CM, TRACK, PC>), dim3(grid), dim3(kSellBlock), lds, s, a, H.hops,
+                           H.dcls, H.drest, t->sptr.get(), t->width.get(), t->cols.get(), t->wre.get(),
+                           t->cplx ? t->wim.get() : nullptr);
+    }
+}
+
+void sell_pass(const MVArgs &a, bool cc, int cm, bool track, bool pc, const HopLaunch &H, size_t lds, hipStream_t s,
+               int grid, uint32_t *cnt, const SellHost *t) {
+#define QS_SELL(CCV, CMV, TR, P) sell_kernels<CCV, CMV, TR, P>(a, H, lds, s, grid, cnt, t)
+#define QS_SELL_CM(CCV, TR, P)                                        \
+    do {                                                              \
+        if (cm == CM_FULL) QS_SELL(CCV, CM_FULL, TR, P);              \
+        else if (cm == CM_ZONLY) QS_SELL(CCV, CM_ZONLY, TR, P);       \
+        else QS_SELL(CCV, CM_PTAB, TR, P);                            \
+    } while (0)
+    if (cc) {
+        if (pc) { if (track) QS_SELL_CM(true, true, true); else QS_SELL_CM(true, false, true); }
+        else { if (track) QS_SELL_CM(true, true, false); else QS_SELL_CM(true, false, false); }
+    } else {
+        if (pc) { if (track) QS_SELL_CM(false, true, true); else QS_SELL_CM(false, false, true); }
+        else { if (track) QS_SELL_CM(false, true, false); else QS_SELL_CM(false, false, false); }
+    }
+#undef QS_SELL_CM
+#undef QS_SELL
+}
+
 }  // namespace
+
+// operators.jl:461-576 (_assemble) builds P'.H_par.P as a sparse matrix; here
+// the rows of the plain product come straight from the pull kernel's
+// neighbour enumeration (two passes: lengths, then entries).
+void sell_assemble(OpHost &op, hipStream_t s) {
+    const BasisHost &b = *op.basis;
+    if (!hop_eligible(op, false))
+        fail(QS_EARG, "assembly needs two-bit hop couplings, L <= 32 and a parent lookup table");
+    MVArgs a = base_args(op, false, 1.0, 0.0, 0.0, 0.0, false, false, nullptr);
+    const size_t lds = hop_args(a, op, false, false);
+    const bool track = basis_tracks(b);
+    const bool cc = op.is_complex != 0;
+    const int cm = hop_mode(b);
+    const bool pc = b.nup >= 0;
+    HopLaunch H{op.d_hops.get(), op.d_dcls.get(), op.d_drest.get()};
+    const int64_t n = b.Ns;
+    const int grid = (int)((n + kSellBlock - 1) / kSellBlock);
+    auto t = std::make_unique<SellHost>();
+    t->cplx = cc || b.is_complex;
+    t->nslices = (n + 63) / 64;
+    {
+        DevBuf<uint32_t> cnt((size_t)n);
+        sell_pass(a, cc, cm, track, pc, H, lds, s, grid, cnt.get(), nullptr);
+        QS_HIP(hipGetLastError());
+        std::vector<uint32_t> hc((size_t)n);
+        QS_HIP(hipMemcpyAsync(hc.data(), cnt.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        QS_HIP(hipStreamSynchronize(s));
+        std::vector<uint64_t> sp((size_t)t->nslices);
+        std::vector<uint32_t> w((size_t)t->nslices);
+        uint64_t at = 0;
+        for (int64_t q = 0; q < t->nslices; q++) {
+            uint32_t m = 0;
+            for (int64_t r = q * 64; r < std::min<int64_t>(n, q * 64 + 64); r++) {
+                m = std::max(m, hc[(size_t)r]);
+                t->entries += hc[(size_t)r];
+            }
+            sp[(size_t)q] = at;
+            w[(size_t)q] = m;
+            at += (uint64_t)64 * m;
+        }
+        t->stored = at;
+        t->sptr.alloc(sp.size());
+        t->width.alloc(w.size());
+        QS_HIP(hipMemcpy(t->sptr.get(), sp.data(), sp.size() * 8, hipMemcpyHostToDevice));
+        QS_HIP(hipMemcpy(t->width.get(), w.data(), w.size() * 4, hipMemcpyHostToDevice));
+    }
+    t->cols.alloc(std::max<uint64_t>(1, t->stored));
+    t->wre.alloc(std::max<uint64_t>(1, t->stored));
+    if (t->cplx) t->wim.alloc(std::max<uint64_t>(1, t->stored));
+    sell_pass(a, cc, cm, track, pc, H, lds, s, grid, nullptr, t.get());
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipStreamSynchronize(s));
+    op.sell = std::move(t);
+}
+
+void sell_to_csr(const OpHost &op, std::vector<int64_t> &rowptr, std::vector<int64_t> &col,
+                 std::vector<double> &val) {
+    const SellHost &t = *op.sell;
+    const int64_t n = op.basis->Ns;
+    std::vector<uint64_t> sp((size_t)t.nslices);
+    std::vector<uint32_t> w((size_t)t.nslices), c((size_t)t.stored);
+    std::vector<double> re((size_t)t.stored), im(t.cplx ? (size_t)t.stored : 0);
+    QS_HIP(hipMemcpy(sp.data(), t.sptr.get(), sp.size() * 8, hipMemcpyDeviceToHost));
+    QS_HIP(hipMemcpy(w.data(), t.width.get(), w.size() * 4, hipMemcpyDeviceToHost));
+    QS_HIP(hipMemcpy(c.data(), t.cols.get(), c.size() * 4, hipMemcpyDeviceToHost));
+    QS_HIP(hipMemcpy(re.data(), t.wre.get(), re.size() * 8, hipMemcpyDeviceToHost));
+    if (t.cplx) QS_HIP(hipMemcpy(im.data(), t.wim.get(), im.size() * 8, hipMemcpyDeviceToHost));
+    rowptr.assign((size_t)n + 1, 0);
+    col.clear();
+    val.clear();
+    std::vector<std::pair<uint32_t, cd>> ent;
+    for (int64_t r = 0; r < n; r++) {
+        const uint64_t q = (uint64_t)r >> 6, base = sp[q] + (r & 63);
+        ent.clear();
+        for (uint32_t k = 0; k < w[q]; k++) {
+            const uint64_t at = base + 64 * (uint64_t)k;
+            ent.push_back({c[at], cd(re[at], t.cplx ? im[at] : 0.0)});
+        }
+        std::stable_sort(ent.begin(), ent.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (size_t i = 0; i < ent.size();) {
+            cd sum = 0;
+            size_t j = i;
+            for (; j < ent.size() && ent[j].first == ent[i].first; j++) sum += ent[j].second;
+            // exact zeros are not stored (operators.jl:567-572 drops zero diagonals)
+            if (sum != cd(0, 0)) {
+                col.push_back(ent[i].first);
+                val.push_back(sum.real());
+                val.push_back(sum.imag());
+            }
+            i = j;
+        }
+        rowptr[(size_t)r + 1] = (int64_t)col.size();
+    }
+}
 
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
                    double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
@@ -1458,9 +1775,37 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+    // an assembled operator takes its plain products from the stored rows
+    if (op.sell && !adjoint && !dot_partial && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SELL)) {
+        const SellHost &t = *op.sell;
+        if (t.cplx && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
+        const int grid = (int)((b.Ns + kSellBlock - 1) / kSellBlock);
+        auto go = [&](auto *tag) {
+            using V = std::remove_pointer_t<decltype(tag)>;
+            if (t.cplx)
+                hipLaunchKernelGGL((k_sell_spmv<V, true>), dim3(grid), dim3(kSellBlock), 0, s, a, t.sptr.get(),
+                                   t.width.get(), t.cols.get(), t.wre.get(), t.wim.get(), (const V *)x, (V *)y);
+            else
+                hipLaunchKernelGGL((k_sell_spmv<V, false>), dim3(grid), dim3(kSellBlock), 0, s, a, t.sptr.get(),
+                                   t.width.get(), t.cols.get(), t.wre.get(), (const double *)nullptr,
+                                   (const V *)x, (V *)y);
+        };
+        switch (dtype) {
+        case QS_F64: go((double *)nullptr); break;
+        case QS_F32: go((float *)nullptr); break;
+        case QS_C128: go((cplx_d *)nullptr); break;
+        case QS_C64: go((cplx_f *)nullptr); break;
+        default: fail(QS_EARG, "unknown dtype");
+        }
+        QS_HIP(hipGetLastError());
+        return;
+    }
+    if (op.lanes_per_row == QS_VARIANT_SELL && !op.sell)
+        fail(QS_EARG, "the assembled variant needs qs_op_assemble first");
     // automatic choice: the split kernel for large implicit sectors
+    const bool want_split = op.lanes_per_row >= QS_VARIANT_SPLIT && op.lanes_per_row <= QS_VARIANT_SPLIT + 2;
     const bool auto_split = op.lanes_per_row == 0 && b.Ns >= ((int64_t)1 << 20) && split_eligible(op, adjoint);
-    if ((op.lanes_per_row >= QS_VARIANT_SPLIT || auto_split) && !dot_partial) {
+    if ((want_split || auto_split) && !dot_partial) {
         if (!split_eligible(op, adjoint))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
@@ -1470,7 +1815,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     }
     const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP && op.lanes_per_row <= QS_VARIANT_HOP + 2;
     const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row >= QS_VARIANT_SPLIT ||
-                                                   want_hop);
+                                                   want_hop);   // split / assembled variants fall back here
     if (want_hop && !hop)
         fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
     if (hop) {

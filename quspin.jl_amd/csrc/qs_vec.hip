@@ -202,6 +202,78 @@ __device__ __forceinline__ void combine_body(const V *__restrict__ Vb, int64_t n
     }
 }
 
+// Y[:, :k] = V[:, :m] C on the matrix cores (v_mfma_f64_16x16x4f64): one
+// wavefront computes a 16-row x 64-column tile of Y as four 16 x 16 MFMA
+// accumulators, stepping through m four basis columns at a time.  Operand
+// layout of the instruction (checked on gfx950): lane l holds A[l % 16][l / 16]
+// of the 16 x 4 block of V and B[l / 16][l % 16] of the 4 x 16 block of C;
+// accumulator register i of lane l is D[l / 16 + 4 i][l % 16].  Complex values run as
+// four real products (re.re - im.im, re.im + im.re).  FP64 accumulation.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kMfmaCols = 64;
+
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_combine_mfma(const double *__restrict__ Vb, int64_t n, int m,
+                                                      const double *__restrict__ C, int k, double *__restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int col0 = blockIdx.y * kMfmaCols;
+    if (row0 >= n) return;   // whole wavefront: no MFMA issued
+    const int ar = lane & 15, ak = lane >> 4;            // A operand: row ar, basis column ak
+    const int bk = lane >> 4, bc = lane & 15;            // B operand: basis column bk, output column bc
+    const int64_t arow = row0 + ar;
+    v4d accr[4], acci[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        accr[t] = v4d{0, 0, 0, 0};
+        acci[t] = v4d{0, 0, 0, 0};
+    }
+    for (int j0 = 0; j0 < m; j0 += 4) {
+        const int j = j0 + ak;
+        double vr = 0, vi = 0;
+        if (j < m && arow < n) {
+            if (CPLX) {
+                vr = Vb[2 * ((int64_t)j * n + arow)];
+                vi = Vb[2 * ((int64_t)j * n + arow) + 1];
+            } else {
+                vr = Vb[(int64_t)j * n + arow];
+            }
+        }
+        const int jb = j0 + bk;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int col = col0 + 16 * t + bc;
+            double cr = 0, ci = 0;
+            if (jb < m && col < k) {
+                cr = C[2 * ((int64_t)col * m + jb)];
+                ci = C[2 * ((int64_t)col * m + jb) + 1];
+            }
+            accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vr, cr, accr[t], 0, 0, 0);
+            if (CPLX) {
+                accr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vi, -ci, accr[t], 0, 0, 0);
+                acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vr, ci, acci[t], 0, 0, 0);
+                acci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vi, cr, acci[t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int col = col0 + 16 * t + (lane & 15);
+        if (col >= k) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t r = row0 + (lane >> 4) + 4 * i;
+            if (r >= n) continue;
+            if (CPLX) {
+                Y[2 * ((int64_t)col * n + r)] = accr[t][i];
+                Y[2 * ((int64_t)col * n + r) + 1] = acci[t][i];
+            } else {
+                Y[(int64_t)col * n + r] = accr[t][i];
+            }
+        }
+    }
+}
+
 template <typename V>
 __global__ __launch_bounds__(kVB) void k_gram(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
                                               double *partial) {
@@ -318,6 +390,22 @@ void ensure(VecWork &wk, size_t partial, size_t result) {
 }  // namespace
 
 int ew_blocks(int64_t n) { return ew_grid(n); }
+
+bool combine_on_mfma(int dtype, const void *Vmat, int64_t n, int m, const double *C, int k, void *Y,
+                     hipStream_t s) {
+    // many output columns (a Krylov basis against a time grid or a block of
+    // Ritz vectors): a GEMM for the matrix cores; FP64 storage only
+    if (k < kMfmaMinCols || n < 16 || (dtype != QS_F64 && dtype != QS_C128)) return false;
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((k + kMfmaCols - 1) / kMfmaCols));
+    if (dtype == QS_C128)
+        hipLaunchKernelGGL((k_combine_mfma<true>), grid, dim3(256), 0, s, (const double *)Vmat, n, m, C, k,
+                           (double *)Y);
+    else
+        hipLaunchKernelGGL((k_combine_mfma<false>), grid, dim3(256), 0, s, (const double *)Vmat, n, m, C, k,
+                           (double *)Y);
+    QS_HIP(hipGetLastError());
+    return true;
+}
 
 void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s) {
     for (size_t first = 0; first < jobs.size(); first += kMaxVecJobs) {
@@ -479,11 +567,13 @@ void basis_combine(int dtype, const void *Vmat, int64_t n, int m, const std::vec
     }
     SBuf<double> dc(cc.size(), s);
     QS_HIP(hipMemcpyAsync(dc.get(), cc.data(), sizeof(double) * cc.size(), hipMemcpyHostToDevice, s));
-    by_dtype(dtype, [&](auto *tag) {
-        using V = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL((k_combine<V>), dim3(ew_grid(n)), dim3(256), 0, s, (const V *)Vmat, n, m, dc.get(), k,
-                           (V *)Y);
-    });
+    if (!combine_on_mfma(dtype, Vmat, n, m, dc.get(), k, Y, s)) {
+        by_dtype(dtype, [&](auto *tag) {
+            using V = std::remove_pointer_t<decltype(tag)>;
+            hipLaunchKernelGGL((k_combine<V>), dim3(ew_grid(n)), dim3(256), 0, s, (const V *)Vmat, n, m, dc.get(),
+                               k, (V *)Y);
+        });
+    }
     QS_HIP(hipGetLastError());
     QS_HIP(hipStreamSynchronize(s));
 }

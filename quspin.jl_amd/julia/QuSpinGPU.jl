@@ -13,6 +13,7 @@ exercised only through the same C ABI by the Python tests.
 module QuSpinGPU
 
 using LinearAlgebra
+using SparseArrays
 import QuSpin: SpinBasis1D, OperatorTerm, eigsh, evolve
 
 const libqsgpu = get(ENV, "QSGPU_LIB", joinpath(@__DIR__, "..", "_build", "libqsgpu.so"))
@@ -205,6 +206,28 @@ function lanczos_full(op::GPUQuantumLinearOperator{T}, v0::AbstractVector, m::In
                 used))
     u = Int(used[])
     return E[1:u], reshape(V[1:u*u], u, u), permutedims(QT[:, 1:u])
+end
+
+# Hamiltonian(...; static_fmt=:csr) -- operators.jl:461-576: the sector matrix
+# assembled on the device; plain products then read the stored rows
+function assemble!(op::GPUQuantumLinearOperator)
+    check(ccall((:qs_op_assemble, libqsgpu), Cint, (Ptr{Cvoid},), op.handle))
+    return op
+end
+
+# the assembled matrix with duplicates merged and exact zeros dropped
+# (sparse_formats.jl:21-50 stores the same entries row by row)
+function SparseArrays.SparseMatrixCSC(op::GPUQuantumLinearOperator{T}) where {T}
+    n = size(op, 1)
+    nnz = Ref{Int64}(0)
+    check(ccall((:qs_op_csr, libqsgpu), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                op.handle, nnz, C_NULL, C_NULL, C_NULL))
+    rowptr, col, val = Vector{Int64}(undef, n + 1), Vector{Int64}(undef, nnz[]), Vector{Float64}(undef, 2nnz[])
+    check(ccall((:qs_op_csr, libqsgpu), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                op.handle, nnz, rowptr, col, val))
+    z = reinterpret(ComplexF64, val)
+    rows = [r for r in 1:n for _ in 1:(rowptr[r+1] - rowptr[r])]
+    return sparse(rows, col .+ 1, T <: Real ? real.(z) : T.(z), n, n)
 end
 
 """

@@ -8,7 +8,7 @@ meanings and exception classes, and forwards every call through the C ABI of
 CPU fallback: without the built library, importing the operator classes
 works but every call raises ImportError.
 """
-from ._lib import build, lib, LIB_PATH, VARIANT_HOP, VARIANT_SPLIT  # noqa: F401
+from ._lib import build, lib, LIB_PATH, VARIANT_HOP, VARIANT_SPLIT, VARIANT_SELL  # noqa: F401
 from .errors import ArgumentError, DimensionMismatch, ErrorException  # noqa: F401
 from .basis import SpinBasis1D  # noqa: F401
 from .operators import (  # noqa: F401

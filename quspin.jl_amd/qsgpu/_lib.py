@@ -14,6 +14,7 @@ WHICH = {"SA": 0, "LA": 1, "SM": 2, "LM": 3, "BE": 4}
 QS_CHECK_SYMM, QS_CHECK_HERM, QS_CHECK_PCON, QS_FORCE_COMPLEX = 1, 2, 4, 8
 VARIANT_HOP = 16   # QS_VARIANT_HOP: the two-bit hop matvec kernel
 VARIANT_SPLIT = 32   # QS_VARIANT_SPLIT: the split (high word / low word) hop kernel
+VARIANT_SELL = 48    # QS_VARIANT_SELL: plain products from the assembled sector matrix
 
 EXPORTS = (
     "qs_last_error", "qs_device_info", "qs_basis_create", "qs_basis_destroy", "qs_basis_size",
@@ -22,7 +23,7 @@ EXPORTS = (
     "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply", "qs_op_set_variant", "qs_op_tune",
     "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
-    "qs_server_wait", "qs_lanczos", "qs_blocks_evolve",
+    "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr",
 )
 
 _lib = None
@@ -64,6 +65,8 @@ def lib():
     L.qs_expm_multiply.argtypes = [vp, i32, vp, i64, vp, dbl, i32, vp, vp]
     L.qs_op_set_variant.argtypes = [vp, i32]
     L.qs_op_tune.argtypes = [vp, i32, i32, vp, vp]
+    L.qs_op_assemble.argtypes = [vp]
+    L.qs_op_csr.argtypes = [vp, vp, vp, vp, vp]
     L.qs_basis_parent_size.argtypes = [vp, i32, vp]
     L.qs_basis_project.argtypes = [vp, i32, i32, vp, vp]
     L.qs_basis_expand.argtypes = [vp, i32, i32, vp, vp, i64, i32]

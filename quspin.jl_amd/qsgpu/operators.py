@@ -236,6 +236,31 @@ class QuantumLinearOperator:
         check(_lib.lib().qs_op_set_variant(self._h, int(lanes_per_row)))
         return self
 
+    def assemble(self):
+        """Assemble the sector matrix on the device (the reference's
+        Hamiltonian(...; static_fmt=:csr), operators.jl:461-576); plain products then
+        read the stored rows.  Needs two-bit hop couplings."""
+        check(_lib.lib().qs_op_assemble(self._h))
+        self.assembled = True
+        return self
+
+    def tocsr(self):
+        """The assembled matrix as scipy.sparse.csr_matrix (duplicates merged,
+        exact zeros dropped; SparseMatrixCSR, sparse_formats.jl:21-50)."""
+        import scipy.sparse as sp
+        L = _lib.lib()
+        nnz = ctypes.c_int64(0)
+        check(L.qs_op_csr(self._h, ctypes.byref(nnz), None, None, None))
+        n = self.Ns
+        rowptr = np.empty(n + 1, dtype=np.int64)
+        col = np.empty(max(1, nnz.value), dtype=np.int64)
+        val = np.empty(2 * max(1, nnz.value), dtype=np.float64)
+        check(L.qs_op_csr(self._h, ctypes.byref(nnz), rowptr.ctypes.data, col.ctypes.data, val.ctypes.data))
+        v = val[:2 * nnz.value].view(np.complex128)
+        if not np.any(v.imag):
+            v = v.real.copy()
+        return sp.csr_matrix((v, col[:nnz.value], rowptr), shape=(n, n))
+
     def tune(self, dtype=None, repeats=5):
         """Time every kernel variant for `dtype` and keep the fastest; returns (lanes_per_row, ms)."""
         dt = np.dtype(dtype) if dtype is not None else self._vec_dtype(np.float64)

@@ -214,3 +214,25 @@ def test_host_blocks_evolve_matches_blockops(gpu):
     keys = list(ops.basis_dict)
     got = qsgpu.blocks.blocks_evolve([ops.H_dict[k] for k in keys], [0] * len(keys), psi, -1j * times)
     assert np.array_equal(got, want)
+
+
+def test_block_evolve_many_times_on_matrix_cores(gpu):
+    # 24 time points per block from the first factorization: the batched
+    # recombination of every block runs on the FP64 matrix cores
+    L = 12   # blocks of about 77 rows: whole 16-row tiles for the matrix cores
+    print("block evolve 24 times", flush=True)
+    nup = L // 2
+    terms = _xxz(L)
+    blocks = [dict(kblock=k) for k in range(L)]
+    ops = qsgpu.BlockOps(blocks, terms, [], qsgpu.SpinBasis1D, (L,), np.complex128,
+                         basis_kwargs=dict(nup=nup, pauli=False))
+    par = oracle.Sector(L, nup=nup, pauli=False)
+    H = oracle.Operator(par, terms, check_symm=False).dense()
+    neel = sum(1 << i for i in range(0, L, 2))
+    psi = np.zeros(math.comb(L, nup), dtype=np.complex128)
+    psi[_colex_rank(neel)] = 1.0
+    times = np.linspace(0.0, 0.5, 24)
+    got = ops.evolve(psi, 0.0, times)
+    w, V = np.linalg.eigh(H)
+    want = np.stack([V @ (np.exp(-1j * w * t) * (V.conj().T @ psi)) for t in times], axis=1)
+    assert np.abs(got - want).max() <= 1e-10
