@@ -899,6 +899,54 @@ __global__ __launch_bounds__(kSellBlock) void k_sell_spmv(MVArgs A, const uint64
     finish_row<V>(A, y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
 }
 
+// Batched products from assembled operators: job j owns blocks [b0_j, b0_j +
+// nb_j) and runs exactly its single launch's rows (one lane per row), so its
+// result is bitwise the same alone and in any batch.
+struct SellJob {
+    const MVArgs *A;   // device copy
+    const uint64_t *sptr;
+    const uint32_t *width, *cols;
+    const double *wre, *wim;   // wim null: real weights
+    const void *x;
+    void *y;
+    int b0, nb;
+};
+struct SellJobs {
+    int njobs;
+    SellJob job[32];
+};
+
+template <typename V>
+__global__ __launch_bounds__(kSellBlock) void kb_sell_spmv(SellJobs J) {
+    using T = VTraits<V>;
+    int lo = 0, hi = J.njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)blockIdx.x >= J.job[mid].b0) lo = mid;
+        else hi = mid - 1;
+    }
+    const SellJob &q = J.job[lo];
+    const MVArgs &A = *q.A;
+    const int64_t row = (int64_t)((int)blockIdx.x - q.b0) * kSellBlock + threadIdx.x;
+    if (row >= A.N) return;
+    const uint64_t base = q.sptr[(uint64_t)row >> 6] + (row & 63);
+    const uint32_t w = q.width[(uint64_t)row >> 6];
+    const V *x = (const V *)q.x;
+    double ar = 0, ai = 0;
+    for (uint32_t k = 0; k < w; k++) {
+        const uint64_t at = base + 64 * (uint64_t)k;
+        const uint32_t c = q.cols[at];
+        const double er = q.wre[at], ei = q.wim ? q.wim[at] : 0.0;
+        double vr, vi;
+        T::load(x, c, vr, vi);
+        if (A.conj_x) vi = -vi;
+        ar += er * vr - ei * vi;
+        ai += er * vi + ei * vr;
+    }
+    double dot_r = 0, dot_i = 0;
+    finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
+}
+
 // ------------------------------------------------------- split kernel --
 // Implicit bases with a particle number (no symmetry, or spin inversion at
 // half filling) and two-bit hop couplings.  A state splits into its high
@@ -1919,8 +1967,37 @@ void batch_launch_cm(int cm, bool pc, bool track, int total, size_t lds, hipStre
 }  // namespace
 
 void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hipStream_t s) {
-    // one launch per (column mode, particle number) class, jobs in order
+    // jobs of assembled operators: one launch over their stored rows
     std::vector<char> done(jobs.size(), 0);
+    {
+        std::vector<size_t> asm_jobs;
+        for (size_t j = 0; j < jobs.size(); j++)
+            if (bm.ops[jobs[j].op]->sell) asm_jobs.push_back(j);
+        for (size_t c0 = 0; c0 < asm_jobs.size(); c0 += 32) {
+            SellJobs J;
+            J.njobs = (int)std::min<size_t>(32, asm_jobs.size() - c0);
+            int total = 0;
+            for (int i = 0; i < J.njobs; i++) {
+                const MatvecJob &mj = jobs[asm_jobs[c0 + i]];
+                const OpHost &op = *bm.ops[mj.op];
+                const SellHost &t = *op.sell;
+                const int nb = (int)((op.basis->Ns + kSellBlock - 1) / kSellBlock);
+                J.job[i] = SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(),
+                                   t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, total, nb};
+                total += nb;
+                done[asm_jobs[c0 + i]] = 1;
+            }
+            switch (bm.dtype) {
+            case QS_F64: hipLaunchKernelGGL(kb_sell_spmv<double>, dim3(total), dim3(kSellBlock), 0, s, J); break;
+            case QS_F32: hipLaunchKernelGGL(kb_sell_spmv<float>, dim3(total), dim3(kSellBlock), 0, s, J); break;
+            case QS_C128: hipLaunchKernelGGL(kb_sell_spmv<cplx_d>, dim3(total), dim3(kSellBlock), 0, s, J); break;
+            case QS_C64: hipLaunchKernelGGL(kb_sell_spmv<cplx_f>, dim3(total), dim3(kSellBlock), 0, s, J); break;
+            default: fail(QS_EARG, "unknown dtype");
+            }
+            QS_HIP(hipGetLastError());
+        }
+    }
+    // the others: one launch per (column mode, particle number) class, jobs in order
     for (size_t first = 0; first < jobs.size(); first++) {
         if (done[first]) continue;
         const int key_cm = bm.cm[jobs[first].op], key_pc = bm.pc[jobs[first].op];

@@ -192,8 +192,16 @@ class BlockOps:
         return self
 
     def _hamiltonian(self, basis):
-        return QuantumLinearOperator(basis, self.static, check_symm=self.check_symm, check_herm=self.check_herm,
-                                     check_pcon=self.check_pcon)
+        # the reference assembles every block (Hamiltonian(...; static_fmt=:csc),
+        # tools.jl:2147-2168); the device assembly serves the lockstep engine's
+        # products, and couplings it cannot assemble stay matrix-free
+        op = QuantumLinearOperator(basis, self.static, check_symm=self.check_symm, check_herm=self.check_herm,
+                                   check_pcon=self.check_pcon)
+        try:
+            op.assemble()
+        except ArgumentError:
+            pass
+        return op
 
     def _block_data(self, key, basis):
         if key in self.H_dict:
