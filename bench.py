@@ -129,7 +129,7 @@ def bench_c4(n_times):
     return out
 
 
-def bench_c5(n_requests, n_workers):
+def bench_c5(n_requests, n_workers, assemble=True):
     """Serving form: Poisson stream (mean 200 req/s, seeded) of 60% eigsh k=1..4 / 40% evolve to
     t ~ U[1,10] on XXZ / J1-J2 chains, L in {16,20,24,28}, nup=L/2, random sectors, HBM budget 200 GB.
     Sector bases are built (cached) before the clock starts; each request's operator is built at
@@ -139,12 +139,15 @@ def bench_c5(n_requests, n_workers):
     from qsgpu import serve
     print("bench: C5 serving", file=sys.stderr, flush=True)
     reqs = serve.poisson_workload(n_requests, rate=200.0, seed=2026)
-    builder = serve.RequestBuilder()
+    warm = serve.RequestBuilder()
     for r in reqs:
         try:
-            builder.operator(r)
+            warm.operator(r)
         except qsgpu.ArgumentError:
             r["skip"] = True
+    # operators are built (and, for sectors up to 4M rows, assembled) at arrival, on the clock
+    builder = serve.RequestBuilder(assemble=assemble)
+    builder.bases = warm.bases
     done = []
     with serve.Server(n_workers=n_workers, hbm_budget=200e9) as srv:
         start = time.perf_counter()
@@ -170,7 +173,7 @@ def bench_c5(n_requests, n_workers):
             "solves_per_s": round(len(done) / span, 2), "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4),
             "mean_run_s": round(float(np.mean([x["seconds"][1] for _, x, _ in done])), 4),
-            "eigsh_tol": 1e-10, "evolve_tol": 1e-12}
+            "eigsh_tol": 1e-10, "evolve_tol": 1e-12, "assembled_operators": bool(assemble)}
 
 
 def main():
@@ -186,6 +189,7 @@ def main():
     ap.add_argument("--c4-times", type=int, default=200)
     ap.add_argument("--c5-requests", type=int, default=200)
     ap.add_argument("--c5-workers", type=int, default=8)
+    ap.add_argument("--c5-matrix-free", action="store_true", help="serve without assembling the operators")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for PMC byte counters)")
@@ -304,7 +308,7 @@ def main():
     if not args.no_c4:
         result["c4_block_evolve"] = bench_c4(args.c4_times)
     if not args.no_c5:
-        result["c5_serving"] = bench_c5(args.c5_requests, args.c5_workers)
+        result["c5_serving"] = bench_c5(args.c5_requests, args.c5_workers, assemble=not args.c5_matrix_free)
     if not args.no_cpu:
         import oracle
         print("bench: CPU baseline (reference parent-apply loop, 1 thread, %d-state sample)" % args.cpu_sample,

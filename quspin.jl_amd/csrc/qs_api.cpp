@@ -645,8 +645,12 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
             (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 2) && lanes_per_row != QS_VARIANT_SELL)
             fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2, QS_VARIANT_SPLIT + 0, 1, 2 or "
                           "QS_VARIANT_SELL");
-        if (lanes_per_row == QS_VARIANT_SELL && !op->o.sell)
-            fail(QS_EARG, "the assembled variant needs qs_op_assemble first");
+        if (lanes_per_row == QS_VARIANT_SELL && !op->o.sell) {
+            // assembled by the first plain product, on that product's stream
+            if (!hop_eligible(op->o, false))
+                fail(QS_EARG, "assembly needs two-bit hop couplings, L <= 32 and a parent lookup table");
+            op->o.want_sell = true;
+        }
         if (lanes_per_row >= QS_VARIANT_SPLIT && lanes_per_row <= QS_VARIANT_SPLIT + 2 &&
             !(split_eligible(op->o, false) && split_eligible(op->o, true)))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "

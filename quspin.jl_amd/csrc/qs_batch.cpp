@@ -386,10 +386,12 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
     std::vector<double> host(tot);
     if (tot) QS_HIP(hipMemcpyAsync(host.data(), d.get(), sizeof(double) * tot, hipMemcpyDeviceToHost, s));
     QS_HIP(hipStreamSynchronize(s));
-    for (size_t i = 0; i < R; i++) {
+    std::vector<int> done(R, 0);
+    parallel_for(R, [&](size_t i) {
         std::vector<double> mine(host.begin() + off[i], host.begin() + off[i] + sc(i).total());
-        nmv[i] += F[i]->P.replay(sc(i), mine);
-    }
+        done[i] = F[i]->P.replay(sc(i), mine);
+    });
+    for (size_t i = 0; i < R; i++) nmv[i] += done[i];
 }
 
 // One pending _krylov_expmv_vector call (operators.jl:1758-1803).  A call
@@ -452,22 +454,32 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
     std::vector<Chain> chains;
     std::vector<CombReq> cq;
     std::vector<std::vector<double>> ests(live.size());
+    // the tasks' projected problems on host threads, then the combines in task order
+    std::vector<std::vector<cd>> tcoefs(live.size());
+    std::vector<std::vector<std::vector<cd>>> jcoefs(live.size());
+    parallel_for(live.size(), [&](size_t i) {
+        const ExpmTask &T = tasks[live[i]];
+        const KrylovProj &P = F[i]->P;
+        if (P.hermitian) {
+            P.lanczos_times(T.scales, tcoefs[i], ests[i]);
+        } else {
+            ests[i].assign(T.scales.size(), 0.0);
+            jcoefs[i].resize(T.scales.size());
+            for (size_t j = 0; j < T.scales.size(); j++)
+                if (T.scales[j] != cd(0, 0)) ests[i][j] = P.expm_column(T.scales[j], jcoefs[i][j]);
+        }
+    });
     for (size_t i = 0; i < live.size(); i++) {
         const size_t t = live[i];
         const ExpmTask &T = tasks[t];
         const KrylovProj &P = F[i]->P;
         const int c = P.completed;
         if (P.hermitian) {
-            std::vector<cd> coefs;
-            P.lanczos_times(T.scales, coefs, ests[i]);
-            cq.push_back({F[i]->V.get(), F[i]->n, c, coefs, (int)T.scales.size(), T.out});
+            cq.push_back({F[i]->V.get(), F[i]->n, c, std::move(tcoefs[i]), (int)T.scales.size(), T.out});
         } else {
-            ests[i].assign(T.scales.size(), 0.0);
             for (size_t j = 0; j < T.scales.size(); j++) {
                 if (T.scales[j] == cd(0, 0)) continue;
-                std::vector<cd> coef;
-                ests[i][j] = P.expm_column(T.scales[j], coef);
-                cq.push_back({F[i]->V.get(), F[i]->n, c, coef, 1, outcol(t, j)});
+                cq.push_back({F[i]->V.get(), F[i]->n, c, std::move(jcoefs[i][j]), 1, outcol(t, j)});
             }
         }
     }
@@ -528,12 +540,15 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
         bfactorize(*bm, ops, dtype, rq, krylov_dim, wk, s, F, m2);
         std::vector<CombReq> pq;
         std::vector<double> est(fac.size());
+        std::vector<std::vector<cd>> coefs(fac.size());
+        // the calls' small exponentials are independent: host threads
+        parallel_for(fac.size(), [&](size_t r) {
+            est[r] = F[r]->P.expm_column(chains[fac[r]].stack.back().scale, coefs[r]);
+        });
         for (size_t r = 0; r < fac.size(); r++) {
             nmatvec[chains[fac[r]].task] += m2[r];
             const Call &cl = chains[fac[r]].stack.back();
-            std::vector<cd> coef;
-            est[r] = F[r]->P.expm_column(cl.scale, coef);
-            pq.push_back({F[r]->V.get(), F[r]->n, F[r]->P.completed, coef, 1, cl.out});
+            pq.push_back({F[r]->V.get(), F[r]->n, F[r]->P.completed, std::move(coefs[r]), 1, cl.out});
         }
         bcombine(dtype, pq, wk, s);
         std::vector<NormReq> on;
@@ -555,6 +570,7 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
         }
     }
     QS_HIP(hipStreamSynchronize(s));
+    stream_pool_trim();
 }
 
 }  // namespace qs

@@ -8,7 +8,10 @@
 #include <set>
 #include <tuple>
 
+#include <atomic>
+#include <exception>
 #include <mutex>
+#include <thread>
 
 #include "qs_internal.h"
 
@@ -725,9 +728,36 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
     for (int i = 0; i < s; i++) E = mm(E, E);
 }
 
+void parallel_for(size_t n, const std::function<void(size_t)> &fn) {
+    const size_t nt = std::min<size_t>({n, 16, std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex em;
+    auto work = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < n;) {
+            try {
+                fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(em);
+                if (!err) err = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
 void stream_pool_setup() {
-    // the default pool of the device keeps up to 8 GB of memory freed in
-    // stream order for reuse before handing it back at a synchronisation
+    // the default pool of the device keeps the memory freed in stream order
+    // for reuse (a lockstep batch round re-allocates tens of GB of Krylov
+    // bases); stream_pool_trim hands the excess back after large batches
     static std::once_flag once;
     std::call_once(once, [] {
         int dev = 0;
@@ -736,9 +766,19 @@ void stream_pool_setup() {
             (void)hipGetLastError();
             return;
         }
-        uint64_t keep = 8ULL << 30;
+        uint64_t keep = UINT64_MAX;
         if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) (void)hipGetLastError();
     });
+}
+
+void stream_pool_trim() {
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    if (hipMemPoolTrimTo(pool, (size_t)8 << 30) != hipSuccess) (void)hipGetLastError();
 }
 
 namespace {

@@ -5,6 +5,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -65,12 +66,17 @@ struct DevBuf {
     T *get() const { return p; }
 };
 
+// fn(i) for i in [0, n) on up to 16 host threads (independent iterations:
+// the small projected problems of many Krylov factorizations)
+void parallel_for(size_t n, const std::function<void(size_t)> &fn);
+
 // Device scratch allocated and freed in stream order (hipMallocAsync /
 // hipFreeAsync on the solver's stream).  Unlike hipFree, releasing it does
 // not wait for the whole device, so solvers on different streams (the
 // serving loop's workers) keep overlapping; the default pool keeps up to a
 // few GB of freed memory for reuse (stream_pool_setup).
 void stream_pool_setup();
+void stream_pool_trim();   // after a large batch: keep at most 8 GB cached
 template <typename T>
 struct SBuf {
     T *p = nullptr;
@@ -284,8 +290,11 @@ struct OpHost {
     DevBuf<HopDev> d_hops, d_hops_adj;
     DevBuf<DiagClass> d_dcls, d_dcls_adj;
     DevBuf<TermDev> d_drest, d_drest_adj;
-    // assembled sector matrix (qs_op_assemble), used for plain products
+    // assembled sector matrix (qs_op_assemble), used for plain products;
+    // want_sell defers the assembly to the first plain product (its stream)
     std::unique_ptr<SellHost> sell;
+    mutable bool want_sell = false;
+    mutable std::mutex sell_mutex;
     // hop groups classified for the split kernel (built once, on first use)
     mutable std::once_flag split_once;
     mutable SplitHops split_hops, split_hops_adj;

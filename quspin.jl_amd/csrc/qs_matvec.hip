@@ -1742,7 +1742,8 @@ void sell_assemble(OpHost &op, hipStream_t s) {
     t->cplx = cc || b.is_complex;
     t->nslices = (n + 63) / 64;
     {
-        DevBuf<uint32_t> cnt((size_t)n);
+        // stream-ordered scratch and copies: other streams keep running
+        SBuf<uint32_t> cnt((size_t)n, s);
         sell_pass(a, cc, cm, track, pc, H, lds, s, grid, cnt.get(), nullptr);
         QS_HIP(hipGetLastError());
         std::vector<uint32_t> hc((size_t)n);
@@ -1764,8 +1765,9 @@ void sell_assemble(OpHost &op, hipStream_t s) {
         t->stored = at;
         t->sptr.alloc(sp.size());
         t->width.alloc(w.size());
-        QS_HIP(hipMemcpy(t->sptr.get(), sp.data(), sp.size() * 8, hipMemcpyHostToDevice));
-        QS_HIP(hipMemcpy(t->width.get(), w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        QS_HIP(hipMemcpyAsync(t->sptr.get(), sp.data(), sp.size() * 8, hipMemcpyHostToDevice, s));
+        QS_HIP(hipMemcpyAsync(t->width.get(), w.data(), w.size() * 4, hipMemcpyHostToDevice, s));
+        QS_HIP(hipStreamSynchronize(s));
     }
     t->cols.alloc(std::max<uint64_t>(1, t->stored));
     t->wre.alloc(std::max<uint64_t>(1, t->stored));
@@ -1823,6 +1825,18 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+    // an operator marked for assembly is assembled by its first plain product,
+    // on the stream that product runs on
+    if (op.want_sell && !adjoint) {
+        std::lock_guard<std::mutex> g(op.sell_mutex);
+        if (!op.sell) {
+            try {
+                sell_assemble(const_cast<OpHost &>(op), s);
+            } catch (const Error &) {
+                op.want_sell = false;   // couplings it cannot assemble: stay matrix-free
+            }
+        }
+    }
     // an assembled operator takes its plain products from the stored rows
     if (op.sell && !adjoint && !dot_partial && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SELL)) {
         const SellHost &t = *op.sell;
@@ -1848,8 +1862,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         QS_HIP(hipGetLastError());
         return;
     }
-    if (op.lanes_per_row == QS_VARIANT_SELL && !op.sell)
-        fail(QS_EARG, "the assembled variant needs qs_op_assemble first");
+
     // automatic choice: the split kernel for large implicit sectors
     const bool want_split = op.lanes_per_row >= QS_VARIANT_SPLIT && op.lanes_per_row <= QS_VARIANT_SPLIT + 2;
     const bool auto_split = op.lanes_per_row == 0 && b.Ns >= ((int64_t)1 << 20) && split_eligible(op, adjoint);

@@ -170,19 +170,30 @@ def chain_terms(L, J2, delta, J=1.0):
 
 
 class RequestBuilder:
-    """Bases cached per sector; one operator per request (its couplings are the request's)."""
+    """Bases cached per sector; one operator per request (its couplings are the request's).
+    With `assemble`, sectors of at most `assemble_rows` rows are assembled on the device
+    (the reference's static_fmt path), so the request's products read stored rows."""
 
-    def __init__(self):
+    def __init__(self, assemble=False, assemble_rows=4_000_000):
         from .basis import SpinBasis1D
         self._SB = SpinBasis1D
         self.bases = {}
+        self.assemble = bool(assemble)
+        self.assemble_rows = int(assemble_rows)
 
     def operator(self, req):
         key = (req["L"], tuple(sorted(req["blocks"].items())))
         if key not in self.bases:
             self.bases[key] = self._SB(req["L"], nup=req["nup"], pauli=False, **req["blocks"])
         b = self.bases[key]
-        return QuantumLinearOperator(b, chain_terms(req["L"], req["J2"], req["delta"]))
+        op = QuantumLinearOperator(b, chain_terms(req["L"], req["J2"], req["delta"]))
+        if self.assemble and op.Ns <= self.assemble_rows:
+            try:
+                # assembled by the first product, on the worker stream that runs it
+                op.set_variant(_lib.VARIANT_SELL)
+            except ArgumentError:
+                pass
+        return op
 
     def initial_state(self, req, op):
         rng = np.random.default_rng(req["seed"])

@@ -388,6 +388,10 @@ def test_assembled_matrix_matches_oracle(gpu, case):
     assert np.linalg.norm(got32 - want) <= 2e-6 * np.linalg.norm(want)
     adj = op.adjoint().matvec(x)
     assert np.linalg.norm(adj - D.conj().T @ x) <= 1e-12 * np.linalg.norm(adj)
+    # selecting the assembled variant defers the assembly to the first plain product
+    lazy = qsgpu.QuantumLinearOperator(b, terms).set_variant(qsgpu.VARIANT_SELL)
+    assert np.linalg.norm(lazy.matvec(x) - want) <= 1e-12 * np.linalg.norm(want)
+    assert (lazy.tocsr() != M).nnz == 0
 
 
 def test_assembled_csr_c1_equals_reference_csr(gpu):
