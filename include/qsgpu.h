@@ -147,6 +147,15 @@ int qs_op_stream(const qs_op *op, void **stream);
 #define QS_VARIANT_SELL 48
 int qs_op_set_variant(qs_op *op, int lanes_per_row);
 
+/* Opt in to HIP-graph replay of the Krylov exponential's halving
+ * recursion: on a non-blocking stream (qs_op_set_stream) its second and later
+ * factorizations replay one captured graph (every step queued with device
+ * scalars, operators.jl:1655-1731).  While a capture is open the HIP runtime
+ * refuses work on the legacy default stream from every thread of the
+ * process, so enable it only where no other thread uses that stream during
+ * the solve.  Off by default.  No reference counterpart. */
+int qs_op_set_graphs(qs_op *op, int enable);
+
 /* Hamiltonian(...; static_fmt=:csr) assembly -- operators.jl:461-576
  * (_assemble: P'.H_par.P for a sector, sparse_formats.jl:21-50 storage):
  * builds the sector matrix of the plain product on the device (sliced ELL,

@@ -709,6 +709,13 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
     });
 }
 
+int qs_op_set_graphs(qs_op *op, int enable) {
+    return guard([&] {
+        check_ptr(op, "op");
+        op->o.graphs = enable ? 1 : 0;
+    });
+}
+
 int qs_op_assemble(qs_op *op) {
     return guard([&] {
         check_ptr(op, "op");

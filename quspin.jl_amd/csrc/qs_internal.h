@@ -305,6 +305,7 @@ struct OpHost {
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     int lanes_per_row = 0;       // matvec kernel variant (0 = default heuristic)
+    int graphs = 0;              // replay the expm recursion's factorizations from a HIP graph
     // staging buffers for host vectors
     DevBuf<char> stage_x, stage_y;
     // basis without symmetry: the extracted diagonal alone (structurally

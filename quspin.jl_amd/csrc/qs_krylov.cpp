@@ -261,9 +261,55 @@ bool lanczos_ok(const OpHost &op, int dtype) { return op.hermitian != 0 && (dtyp
 namespace {
 
 struct Factorization {
-    std::unique_ptr<Krylov> K;
+    std::unique_ptr<Krylov> K;   // own basis, or null when a FactPlan holds it
+    const void *basis = nullptr;
     KrylovProj P;
 };
+
+// A factorization whose buffers stay fixed across the calls of one halving
+// recursion, so its launch sequence (every step, scalars on the device) is
+// captured once in a HIP graph and replayed by each later call; the input is
+// copied into `vin` first.  Falls back to direct launches if capture fails.
+struct FactPlan {
+    int64_t n = 0;
+    int steps = 0;
+    SBuf<char> vin, V;
+    SBuf<double> d;
+    hipGraphExec_t exec = nullptr;
+    bool no_graph = false;
+    int uses = 0;
+    ~FactPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+// every step of the factorization of vin into V (n x (steps + 1)), scalars in d
+void queue_factorization(OpHost &op, int dtype, int64_t n, int steps, bool hermitian, const void *vin, char *V,
+                         double *dd, VecWork &wk, hipStream_t s) {
+    const size_t esz = dtype_size(dtype);
+    auto col = [&](int j) { return (void *)(V + (size_t)j * n * esz); };
+    FactScalars L(steps);
+    vec_norm2_dev(dtype, vin, n, wk, dd + L.beta2(), s);
+    vec_axpby_dev(dtype, dd + L.beta2(), CO_INV_SQRT, vin, cd(0, 0), col(0), n, s);
+    for (int c = 0; c < steps; c++) {
+        void *w = col(c + 1);
+        double *h1 = dd + L.h1(c), *h2 = dd + L.h2(c), *n2 = dd + L.nrm2(c);
+        matvec(op, dtype, col(c), w);
+        if (hermitian) {
+            // three-term Lanczos without reorthogonalisation
+            if (c > 0) vec_axpby_dev(dtype, dd + L.nrm2(c - 1), CO_NEG_SQRT, col(c - 1), cd(1, 0), w, n, s);
+            gram_dots_dev(dtype, col(c), n, 1, w, wk, h1, s);
+            gram_update_dev(dtype, col(c), n, 1, h1, w, wk, n2, s);
+        } else {
+            // Arnoldi, two Gram-Schmidt passes
+            gram_dots_dev(dtype, V, n, c + 1, w, wk, h1, s);
+            gram_update_dev(dtype, V, n, c + 1, h1, w, wk, nullptr, s);
+            gram_dots_dev(dtype, V, n, c + 1, w, wk, h2, s);
+            gram_update_dev(dtype, V, n, c + 1, h2, w, wk, n2, s);
+        }
+        if (c + 1 < steps) vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
+    }
+}
 
 // operators.jl:1655-1731.  Every step is queued on the stream with its
 // scalars (h, |w|^2) kept in device memory; one copy brings them to the host
@@ -279,7 +325,7 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
     P.hermitian = lanczos_ok(op, dtype);
     P.steps = (int)std::min<int64_t>(krylov_dim, n);
     F.K = std::make_unique<Krylov>(dtype, n, P.steps + 1, s);
-    Krylov &K = *F.K;
+    F.basis = F.K->V.get();
     P.H.assign((size_t)(P.steps + 1) * P.steps, cd(0, 0));
     P.completed = P.steps;
     P.terminal = 0;
@@ -287,28 +333,77 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
     vec_reserve(wk, n, steps + 1);
     FactScalars L(steps);
     SBuf<double> d(L.total(), s);
-    vec_norm2_dev(dtype, v, n, wk, d.get() + L.beta2(), s);
-    vec_axpby_dev(dtype, d.get() + L.beta2(), CO_INV_SQRT, v, cd(0, 0), K.col(0), n, s);
-    for (int col = 0; col < steps; col++) {
-        void *w = K.col(col + 1);
-        double *h1 = d.get() + L.h1(col), *h2 = d.get() + L.h2(col), *n2 = d.get() + L.nrm2(col);
-        matvec(op, dtype, K.col(col), w);
-        if (P.hermitian) {
-            // three-term Lanczos without reorthogonalisation
-            if (col > 0) vec_axpby_dev(dtype, d.get() + L.nrm2(col - 1), CO_NEG_SQRT, K.col(col - 1), cd(1, 0), w, n, s);
-            gram_dots_dev(dtype, K.col(col), n, 1, w, wk, h1, s);
-            gram_update_dev(dtype, K.col(col), n, 1, h1, w, wk, n2, s);
-        } else {
-            // Arnoldi, two Gram-Schmidt passes
-            gram_dots_dev(dtype, K.V.get(), n, col + 1, w, wk, h1, s);
-            gram_update_dev(dtype, K.V.get(), n, col + 1, h1, w, wk, nullptr, s);
-            gram_dots_dev(dtype, K.V.get(), n, col + 1, w, wk, h2, s);
-            gram_update_dev(dtype, K.V.get(), n, col + 1, h2, w, wk, n2, s);
-        }
-        if (col + 1 < steps) vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
-    }
+    queue_factorization(op, dtype, n, steps, P.hermitian, v, F.K->V.get(), d.get(), wk, s);
     std::vector<double> host(L.total());
     QS_HIP(hipMemcpyAsync(host.data(), d.get(), sizeof(double) * host.size(), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    nmv += P.replay(L, host);
+}
+
+// The same through a FactPlan: the second and later calls replay one graph.
+void factorize_plan(OpHost &op, int dtype, const void *v, int krylov_dim, FactPlan &plan, Factorization &F,
+                    VecWork &wk, int &nmv) {
+    hipStream_t s = op_stream(op);
+    const int64_t n = op.basis->Ns;
+    const size_t esz = dtype_size(dtype);
+    KrylovProj &P = F.P;
+    P.dim = n;
+    P.hermitian = lanczos_ok(op, dtype);
+    P.steps = (int)std::min<int64_t>(krylov_dim, n);
+    P.H.assign((size_t)(P.steps + 1) * P.steps, cd(0, 0));
+    P.completed = P.steps;
+    P.terminal = 0;
+    const int steps = P.steps;
+    FactScalars L(steps);
+    if (!plan.V.get()) {
+        plan.n = n;
+        plan.steps = steps;
+        plan.vin.alloc((size_t)n * esz, s);
+        plan.V.alloc((size_t)n * (steps + 1) * esz, s);
+        plan.d.alloc(L.total(), s);
+        vec_reserve(wk, n, steps + 1);
+    }
+    F.basis = plan.V.get();
+    QS_HIP(hipMemcpyAsync(plan.vin.get(), v, (size_t)n * esz, hipMemcpyDeviceToDevice, s));
+    auto queue = [&] {
+        queue_factorization(op, dtype, n, steps, P.hermitian, plan.vin.get(), plan.V.get(), plan.d.get(), wk, s);
+    };
+    if (!plan.exec && !plan.no_graph && plan.uses >= 1) {
+        // only when the operator opted in (qs_op_set_graphs), and only on a
+        // non-blocking stream: while a capture is open the runtime refuses
+        // legacy-default-stream work from every thread
+        unsigned flags = 0;
+        if (!op.graphs || hipStreamGetFlags(s, &flags) != hipSuccess || !(flags & hipStreamNonBlocking)) {
+            (void)hipGetLastError();
+            plan.no_graph = true;
+        }
+    }
+    if (!plan.exec && !plan.no_graph && plan.uses >= 1) {
+        // the first call ran directly (lazy tables, workspaces and assembly are
+        // in place); capture this call's launches
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            try {
+                queue();
+            } catch (const Error &) {
+                ok = false;
+            }
+            ok = (hipStreamEndCapture(s, &g) == hipSuccess) && ok && g;
+            ok = ok && hipGraphInstantiate(&plan.exec, g, nullptr, nullptr, 0) == hipSuccess;
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (!ok) {
+            (void)hipGetLastError();
+            plan.exec = nullptr;
+            plan.no_graph = true;
+        }
+    }
+    if (plan.exec) QS_HIP(hipGraphLaunch(plan.exec, s));
+    else queue();
+    plan.uses++;
+    std::vector<double> host(L.total());
+    QS_HIP(hipMemcpyAsync(host.data(), plan.d.get(), sizeof(double) * host.size(), hipMemcpyDeviceToHost, s));
     QS_HIP(hipStreamSynchronize(s));
     nmv += P.replay(L, host);
 }
@@ -317,13 +412,13 @@ void factorize(OpHost &op, int dtype, const void *v, int krylov_dim, Factorizati
 double project(OpHost &op, int dtype, Factorization &F, cd scale, void *out, VecWork &wk) {
     std::vector<cd> coef;
     const double est = F.P.expm_column(scale, coef);
-    basis_combine(dtype, F.K->V.get(), F.P.dim, F.P.completed, coef, 1, out, op_stream(op));
+    basis_combine(dtype, F.basis, F.P.dim, F.P.completed, coef, 1, out, op_stream(op));
     return est;
 }
 
 // operators.jl:1758-1803
 void expmv_vector(OpHost &op, int dtype, const void *v, cd scale, double tol, int krylov_dim, int depth, void *out,
-                  VecWork &wk, int &nmv) {
+                  VecWork &wk, FactPlan &plan, int &nmv) {
     hipStream_t s = op_stream(op);
     const int64_t n = op.basis->Ns;
     if (tol <= 0) fail(QS_EARG, "tol must be positive");
@@ -331,16 +426,17 @@ void expmv_vector(OpHost &op, int dtype, const void *v, cd scale, double tol, in
         QS_HIP(hipMemcpyAsync(out, v, (size_t)n * dtype_size(dtype), hipMemcpyDeviceToDevice, s));
         return;
     }
+    // the plan's basis is reused by the nested calls: this call is done with
+    // it once `out` holds the projection
     Factorization F;
-    factorize(op, dtype, v, krylov_dim, F, wk, nmv);
+    factorize_plan(op, dtype, v, krylov_dim, plan, F, wk, nmv);
     double est = project(op, dtype, F, scale, out, wk);
     double nr = std::sqrt(vec_norm2(dtype, out, n, wk, s));
     if (est > tol * std::max(1.0, nr)) {
         if (depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
-        F.K.reset();
         SBuf<char> mid((size_t)n * dtype_size(dtype), s);
-        expmv_vector(op, dtype, v, scale / 2.0, tol / 2, krylov_dim, depth + 1, mid.get(), wk, nmv);
-        expmv_vector(op, dtype, mid.get(), scale / 2.0, tol / 2, krylov_dim, depth + 1, out, wk, nmv);
+        expmv_vector(op, dtype, v, scale / 2.0, tol / 2, krylov_dim, depth + 1, mid.get(), wk, plan, nmv);
+        expmv_vector(op, dtype, mid.get(), scale / 2.0, tol / 2, krylov_dim, depth + 1, out, wk, plan, nmv);
     }
 }
 
@@ -367,6 +463,7 @@ void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<c
     }
     Factorization F;
     factorize(op, dtype, v, krylov_dim, F, wk, nmatvec);
+    FactPlan plan;   // the halving recursions' factorizations (graph-replayed)
     const int c = F.P.completed;
     if (F.P.hermitian) {
         std::vector<cd> coefs;
@@ -380,7 +477,7 @@ void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<c
             }
             double nr = std::sqrt(vec_norm2(dtype, outcol(j), n, wk, s));
             if (ests[j] > tol * std::max(1.0, nr))
-                expmv_vector(op, dtype, v, scales[j], tol, krylov_dim, 0, outcol(j), wk, nmatvec);
+                expmv_vector(op, dtype, v, scales[j], tol, krylov_dim, 0, outcol(j), wk, plan, nmatvec);
         }
     } else {
         for (size_t j = 0; j < scales.size(); j++) {
@@ -391,7 +488,7 @@ void expm_times_device(OpHost &op, int dtype, const void *v, const std::vector<c
             double est = project(op, dtype, F, scales[j], outcol(j), wk);
             double nr = std::sqrt(vec_norm2(dtype, outcol(j), n, wk, s));
             if (est > tol * std::max(1.0, nr))
-                expmv_vector(op, dtype, v, scales[j], tol, krylov_dim, 0, outcol(j), wk, nmatvec);
+                expmv_vector(op, dtype, v, scales[j], tol, krylov_dim, 0, outcol(j), wk, plan, nmatvec);
         }
     }
     QS_HIP(hipStreamSynchronize(s));

@@ -23,7 +23,7 @@ EXPORTS = (
     "qs_op_stream", "qs_op_matvec", "qs_eigsh", "qs_expm_multiply", "qs_op_set_variant", "qs_op_tune",
     "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
-    "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr",
+    "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr", "qs_op_set_graphs",
 )
 
 _lib = None
@@ -66,6 +66,7 @@ def lib():
     L.qs_op_set_variant.argtypes = [vp, i32]
     L.qs_op_tune.argtypes = [vp, i32, i32, vp, vp]
     L.qs_op_assemble.argtypes = [vp]
+    L.qs_op_set_graphs.argtypes = [vp, i32]
     L.qs_op_csr.argtypes = [vp, vp, vp, vp, vp]
     L.qs_basis_parent_size.argtypes = [vp, i32, vp]
     L.qs_basis_project.argtypes = [vp, i32, i32, vp, vp]

@@ -237,6 +237,13 @@ class QuantumLinearOperator:
         check(_lib.lib().qs_op_set_variant(self._h, int(lanes_per_row)))
         return self
 
+    def set_graphs(self, enable=True):
+        """Replay the Krylov exponential's halving recursion from a HIP graph (on a
+        non-blocking stream set with set_stream; no legacy-default-stream work in
+        other threads meanwhile)."""
+        check(_lib.lib().qs_op_set_graphs(self._h, int(bool(enable))))
+        return self
+
     def assemble(self):
         """Assemble the sector matrix on the device (the reference's
         Hamiltonian(...; static_fmt=:csr), operators.jl:461-576); plain products then
