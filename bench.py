@@ -55,8 +55,14 @@ def profiled_traffic(key):
 
 def variant_name(v):
     import qsgpu
-    if v == qsgpu.VARIANT_SPLIT:
-        return "k_matvec_split (high-word blocks, low word in LDS)"
+    if v == qsgpu.VARIANT_SELL:
+        return "k_sell_spmv (assembled SELL-64 rows)"
+    if v >= qsgpu.VARIANT_SPLIT:
+        fewer = v - qsgpu.VARIANT_SPLIT
+        return "k_matvec_split (high-word blocks, low word in LDS%s)" % (
+            "; %d fewer resident workgroups" % fewer if fewer else "")
+    if v == qsgpu.VARIANT_HOP + 3:
+        return "k_matvec_hop (1 wave per row tile, pre-scaled x)"
     if v >= qsgpu.VARIANT_HOP:
         return "k_matvec_hop (%d wave(s) per row tile)" % (1 << (v - qsgpu.VARIANT_HOP))
     return "general kernel, %d lane(s) per row" % v
@@ -240,10 +246,12 @@ def main():
                    "lanes_per_row": lpr},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 3), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(gbps / HBM_PEAK_GBPS, 5), "traffic": profiled_traffic("c2_fp64"),
-                     "kernel": "k_matvec_hop<double, real, parent table, variant %d>" % lpr,
-                     "note": "per neighbour: colex rank update, parent-table gather, x and 1/sqrt(D) gathers; "
-                             "the whole working set sits in L2/MALL at this N, so the bound is issue and "
-                             "gather latency, not HBM"},
+                     "kernel": "k_prescale + k_matvec_hop<double, real, parent table, pre-scaled x> "
+                               "(variant %d)" % lpr,
+                     "note": "one step = k_prescale (x/sqrt(D) to an FP64 buffer) + the hop kernel; per "
+                             "neighbour: colex rank update, parent-table gather, one gather of the pre-scaled "
+                             "x; achieved counts the whole step; the working set sits in L2/MALL at this N, "
+                             "so the bound is gather issue and latency, not HBM"},
     }
     # the same sector matrix assembled once on the device (the reference's
     # static_fmt=:csr path): plain products read the stored rows

@@ -132,8 +132,11 @@ int qs_op_stream(const qs_op *op, void **stream);
  * with that many lanes cooperating on one row; QS_VARIANT_HOP + k (k = 0, 1,
  * 2) = the two-bit hop kernel (couplings that flip two sites and test only
  * those two, L <= 32; symmetry sectors also need the parent lookup table)
- * with 2^k wavefronts sharing each row tile's couplings.  No reference
- * counterpart: the reference has one CPU loop. */
+ * with 2^k wavefronts sharing each row tile's couplings; QS_VARIANT_HOP + 3 =
+ * the hop kernel with one wavefront per tile that, in a symmetry sector,
+ * first writes x / sqrt(norm) to a stream-ordered FP64 buffer and then
+ * gathers one value per neighbour.  No reference counterpart: the reference
+ * has one CPU loop. */
 #define QS_VARIANT_HOP 16
 /* The split kernel: two-bit hop couplings on a basis without symmetry or
  * with spin inversion only, with a particle number, L <= 32.  Rows sharing

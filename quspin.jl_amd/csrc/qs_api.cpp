@@ -642,8 +642,8 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
         check_ptr(op, "op");
         if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 &&
             lanes_per_row != 8 && (lanes_per_row < QS_VARIANT_SPLIT || lanes_per_row > QS_VARIANT_SPLIT + 2) &&
-            (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 2) && lanes_per_row != QS_VARIANT_SELL)
-            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2, QS_VARIANT_SPLIT + 0, 1, 2 or "
+            (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 3) && lanes_per_row != QS_VARIANT_SELL)
+            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2, 3, QS_VARIANT_SPLIT + 0, 1, 2 or "
                           "QS_VARIANT_SELL");
         if (lanes_per_row == QS_VARIANT_SELL && !op->o.sell) {
             // assembled by the first plain product, on that product's stream
@@ -655,7 +655,7 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
             !(split_eligible(op->o, false) && split_eligible(op->o, true)))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
-        if (lanes_per_row >= QS_VARIANT_HOP && lanes_per_row <= QS_VARIANT_HOP + 2 &&
+        if (lanes_per_row >= QS_VARIANT_HOP && lanes_per_row <= QS_VARIANT_HOP + 3 &&
             !(hop_eligible(op->o, false) && hop_eligible(op->o, true)))
             fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
         op->o.lanes_per_row = lanes_per_row;
@@ -679,7 +679,7 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
         if (repeats < 1) repeats = 3;
         std::vector<int> cands;
         if (hop_eligible(o, false))
-            cands.insert(cands.end(), {QS_VARIANT_HOP, QS_VARIANT_HOP + 1, QS_VARIANT_HOP + 2});
+            cands.insert(cands.end(), {QS_VARIANT_HOP, QS_VARIANT_HOP + 1, QS_VARIANT_HOP + 2, QS_VARIANT_HOP + 3});
         if (split_eligible(o, false) && split_eligible(o, true))
             cands.insert(cands.end(), {QS_VARIANT_SPLIT, QS_VARIANT_SPLIT + 1, QS_VARIANT_SPLIT + 2});
         if (o.sell) cands.push_back(QS_VARIANT_SELL);

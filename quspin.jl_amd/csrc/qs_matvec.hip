@@ -513,24 +513,46 @@ __device__ __forceinline__ void hop_target(const MVArgs &A, const HopDev &h, uin
     }
 }
 
+// FP64 form of a vector type (the pre-scaled x of the parent-table kernel)
+template <typename V> struct Wide { using type = double; };
+template <> struct Wide<cplx_f> { using type = cplx_d; };
+template <> struct Wide<cplx_d> { using type = cplx_d; };
+
+// xs = x * (1/sqrt(D)) column by column, in FP64: the parent-table hop
+// kernel then gathers one value per neighbour instead of x and 1/sqrt(D).
+template <typename V>
+__global__ __launch_bounds__(256) void k_prescale(const V *__restrict__ x, const double *__restrict__ inv_nrm,
+                                                  typename Wide<V>::type *__restrict__ xs, int64_t n) {
+    using W = typename Wide<V>::type;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        double re, im;
+        VTraits<V>::load(x, i, re, im);
+        const double f = inv_nrm[i];
+        VTraits<W>::store(xs, i, re * f, im * f);
+    }
+}
+
 // Adds the flip groups g = g_first, g_first + WS, ... of row `row` (own
-// state s, rank rs) to (ar, ai), kNB neighbours per batch.
-template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
+// state s, rank rs) to (ar, ai), kNB neighbours per batch.  PS: neighbours
+// are gathered from the pre-scaled xs (parent-table mode only).
+template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS, bool PS>
 __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__restrict__ hops,
-                                           const V *__restrict__ x, uint32_t row, uint32_t s, uint32_t rs,
-                                           int g_first, const uint32_t *s_binom, double &ar, double &ai) {
+                                           const V *__restrict__ x, const typename Wide<V>::type *__restrict__ xs,
+                                           uint32_t row, uint32_t s, uint32_t rs, int g_first,
+                                           const uint32_t *s_binom, double &ar, double &ai) {
     using T = VTraits<V>;
     constexpr bool VC = T::cplx;
+    constexpr int NB = kNB;
     const uint32_t mask = (uint32_t)A.mask;
     double own_scale = 1.0;
     if (CM == CM_PTAB) own_scale = 1.0 / A.inv_nrm[row];
     const uint32_t colmask = A.colbits >= 32 ? 0xFFFFFFFFu : ((1u << A.colbits) - 1);
-    for (int g0 = g_first; g0 < A.ngroups; g0 += kNB * WS) {
-        uint32_t col[kNB];
-        double mr[kNB], mi[kNB];
-        int elem[kNB];
+    for (int g0 = g_first; g0 < A.ngroups; g0 += NB * WS) {
+        uint32_t col[NB];
+        double mr[NB], mi[NB];
+        int elem[NB];
 #pragma unroll
-        for (int j = 0; j < kNB; j++) {
+        for (int j = 0; j < NB; j++) {
             const int gi = g0 + j * WS;
             col[j] = CM == CM_PTAB ? rs : row;
             mr[j] = 0;
@@ -588,13 +610,13 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
                 mi[j] = live ? -m_i : 0.0;
             }
         }
-        double f[kNB];
+        double f[NB];
         if (CM == CM_PTAB) {
-            uint32_t e[kNB];
+            uint32_t e[NB];
 #pragma unroll
-            for (int j = 0; j < kNB; j++) e[j] = A.ptab[col[j]];
+            for (int j = 0; j < NB; j++) e[j] = A.ptab[col[j]];
 #pragma unroll
-            for (int j = 0; j < kNB; j++) {
+            for (int j = 0; j < NB; j++) {
                 if (e[j] == 0xFFFFFFFFu) {
                     mr[j] = 0;
                     mi[j] = 0;
@@ -607,13 +629,16 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
                 }
             }
 #pragma unroll
-            for (int j = 0; j < kNB; j++) f[j] = A.inv_nrm[col[j]];
+            for (int j = 0; j < NB; j++) f[j] = PS ? 1.0 : A.inv_nrm[col[j]];
         }
-        double vr[kNB], vi[kNB];
+        double vr[NB], vi[NB];
 #pragma unroll
-        for (int j = 0; j < kNB; j++) T::load(x, col[j], vr[j], vi[j]);
+        for (int j = 0; j < NB; j++) {
+            if (PS) VTraits<typename Wide<V>::type>::load(xs, col[j], vr[j], vi[j]);
+            else T::load(x, col[j], vr[j], vi[j]);
+        }
 #pragma unroll
-        for (int j = 0; j < kNB; j++) {
+        for (int j = 0; j < NB; j++) {
             double wr = mr[j], wi = mi[j];
             if (CM == CM_PTAB) {
                 double cr = 1.0, ci = 0.0;
@@ -647,10 +672,11 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
 // group data stays wave-uniform) and the partial sums meet in LDS in a fixed
 // order.  (b, nb) are explicit so a batched launch gives every job the
 // partition of its own single launch.
-template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
+template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS, bool PS = false>
 __device__ __forceinline__ void hop_body(const MVArgs &A, const HopDev *__restrict__ hops,
                                          const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
-                                         const V *__restrict__ x, V *__restrict__ y, int b, int nb, char *smem) {
+                                         const V *__restrict__ x, const typename Wide<V>::type *__restrict__ xs,
+                                         V *__restrict__ y, int b, int nb, char *smem) {
     using T = VTraits<V>;
     constexpr bool VC = T::cplx;
     constexpr int kRows = kMvBlock / WS;
@@ -661,7 +687,9 @@ __device__ __forceinline__ void hop_body(const MVArgs &A, const HopDev *__restri
         for (int i = threadIdx.x; i < 33 * 65; i += kMvBlock) s_binom[i] = (uint32_t)A.binom[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int sub = wave % WS, rw = wave / WS;
+    // wave-uniform as far as the compiler can tell, so the flip-group
+    // constants of hop_groups stay scalar loads
+    const int sub = __builtin_amdgcn_readfirstlane(wave % WS), rw = wave / WS;
     const int vb = xcd_block(b, nb);
     const uint32_t n = (uint32_t)A.N;
     double dot_r = 0, dot_i = 0;
@@ -688,7 +716,7 @@ __device__ __forceinline__ void hop_body(const MVArgs &A, const HopDev *__restri
                     ar = dr * xr;
                 }
             }
-            hop_groups<V, CC, CM, TRACK, PC, WS>(A, hops, x, row, s, rs, sub, s_binom, ar, ai);
+            hop_groups<V, CC, CM, TRACK, PC, WS, PS>(A, hops, x, xs, row, s, rs, sub, s_binom, ar, ai);
         }
         if (WS > 1) {
             s_part[threadIdx.x] = ar;
@@ -723,13 +751,14 @@ __device__ __forceinline__ void hop_body(const MVArgs &A, const HopDev *__restri
     }
 }
 
-template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS>
+template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS, bool PS>
 __global__ __launch_bounds__(kMvBlock) void k_matvec_hop(MVArgs A, const HopDev *__restrict__ hops,
                                                          const DiagClass *__restrict__ dcls,
                                                          const TermDev *__restrict__ drest, const V *__restrict__ x,
+                                                         const typename Wide<V>::type *__restrict__ xs,
                                                          V *__restrict__ y) {
     extern __shared__ __align__(16) char smem[];
-    hop_body<V, CC, CM, TRACK, PC, WS>(A, hops, dcls, drest, x, y, blockIdx.x, gridDim.x, smem);
+    hop_body<V, CC, CM, TRACK, PC, WS, PS>(A, hops, dcls, drest, x, xs, y, blockIdx.x, gridDim.x, smem);
 }
 
 // Batched hop matvec: job j owns blocks [b0_j, b0_j + nb_j) of the grid and
@@ -759,7 +788,7 @@ __global__ __launch_bounds__(kMvBlock) void kb_matvec_hop(MVJobs J) {
         else hi = mid - 1;
     }
     const MVJob &q = J.job[lo];
-    hop_body<V, CC, CM, TRACK, PC, 1>(*q.A, q.hops, q.dcls, q.drest, (const V *)q.x, (V *)q.y,
+    hop_body<V, CC, CM, TRACK, PC, 1>(*q.A, q.hops, q.dcls, q.drest, (const V *)q.x, nullptr, (V *)q.y,
                                       (int)blockIdx.x - q.b0, q.nb, smem);
 }
 
@@ -1313,18 +1342,31 @@ struct HopLaunch {
 };
 
 template <typename V, bool CC>
-void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x, void *y, const Launch &L,
-                     const HopLaunch &H) {
+void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x, const void *xs, void *y,
+                     const Launch &L, const HopLaunch &H) {
+    using W = typename Wide<V>::type;
     const V *xv = (const V *)x;
+    const W *xw = (const W *)xs;
     V *yv = (V *)y;
+    // xs given (parent-table mode): neighbours gathered from the pre-scaled x
+#define QS_HOP_WS(CMV, TR, P, W)                                                                                \
+    {                                                                                                              \
+        if constexpr (CMV == CM_PTAB) {                                                                            \
+            if (xw) {                                                                                              \
+                hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, W, true>), dim3(L.grid), dim3(kMvBlock),       \
+                                   L.lds, L.s, a, H.hops, H.dcls, H.drest, xv, xw, yv);                            \
+                break;                                                                                             \
+            }                                                                                                      \
+        }                                                                                                          \
+        hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, W, false>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s,  \
+                           a, H.hops, H.dcls, H.drest, xv, nullptr, yv);                                           \
+        break;                                                                                                     \
+    }
 #define QS_HOP(CMV, TR, P)                                                                                      \
     switch (ws) {                                                                                                  \
-    case 1: hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, 1>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a,  \
-                               H.hops, H.dcls, H.drest, xv, yv); break;                                            \
-    case 2: hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, 2>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a,  \
-                               H.hops, H.dcls, H.drest, xv, yv); break;                                            \
-    default: hipLaunchKernelGGL((k_matvec_hop<V, CC, CMV, TR, P, 4>), dim3(L.grid), dim3(kMvBlock), L.lds, L.s, a, \
-                                H.hops, H.dcls, H.drest, xv, yv); break;                                           \
+    case 1: QS_HOP_WS(CMV, TR, P, 1)                                                                               \
+    case 2: QS_HOP_WS(CMV, TR, P, 2)                                                                               \
+    default: QS_HOP_WS(CMV, TR, P, 4)                                                                              \
     }
     if (a.nup >= 0) {
         if (cm == CM_FULL) QS_HOP(CM_FULL, false, true)
@@ -1338,20 +1380,40 @@ void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x,
         else QS_HOP(CM_PTAB, false, false)
     }
 #undef QS_HOP
+#undef QS_HOP_WS
 }
 
-void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, int ws, const void *x, void *y,
+// ps in the parent-table mode: x * (1/sqrt(D)) is written once to a
+// stream-ordered FP64 buffer and the hop kernel gathers from it
+template <typename V, bool CC>
+void hop_cm(const MVArgs &a, int cm, bool track, int ws, bool ps, const void *x, void *y, const Launch &L,
+            const HopLaunch &H) {
+    if (ps && cm == CM_PTAB) {
+        using W = typename Wide<V>::type;
+        const int64_t n = a.N;
+        W *xs = nullptr;
+        QS_HIP(hipMallocAsync((void **)&xs, sizeof(W) * (size_t)std::max<int64_t>(n, 1), L.s));
+        const int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL((k_prescale<V>), dim3(std::max(g, 1)), dim3(256), 0, L.s, (const V *)x, a.inv_nrm, xs, n);
+        dispatch_hop_cm<V, CC>(a, cm, track, ws, x, xs, y, L, H);
+        QS_HIP(hipFreeAsync(xs, L.s));
+        return;
+    }
+    dispatch_hop_cm<V, CC>(a, cm, track, ws, x, nullptr, y, L, H);
+}
+
+void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, int ws, bool ps, const void *x, void *y,
                   const Launch &L, const HopLaunch &H) {
     switch (dtype) {
-    case QS_F64: dispatch_hop_cm<double, false>(a, cm, track, ws, x, y, L, H); break;
-    case QS_F32: dispatch_hop_cm<float, false>(a, cm, track, ws, x, y, L, H); break;
+    case QS_F64: hop_cm<double, false>(a, cm, track, ws, ps, x, y, L, H); break;
+    case QS_F32: hop_cm<float, false>(a, cm, track, ws, ps, x, y, L, H); break;
     case QS_C128:
-        if (cc) dispatch_hop_cm<cplx_d, true>(a, cm, track, ws, x, y, L, H);
-        else dispatch_hop_cm<cplx_d, false>(a, cm, track, ws, x, y, L, H);
+        if (cc) hop_cm<cplx_d, true>(a, cm, track, ws, ps, x, y, L, H);
+        else hop_cm<cplx_d, false>(a, cm, track, ws, ps, x, y, L, H);
         break;
     case QS_C64:
-        if (cc) dispatch_hop_cm<cplx_f, true>(a, cm, track, ws, x, y, L, H);
-        else dispatch_hop_cm<cplx_f, false>(a, cm, track, ws, x, y, L, H);
+        if (cc) hop_cm<cplx_f, true>(a, cm, track, ws, ps, x, y, L, H);
+        else hop_cm<cplx_f, false>(a, cm, track, ws, ps, x, y, L, H);
         break;
     default: fail(QS_EARG, "unknown dtype");
     }
@@ -1874,21 +1936,28 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         QS_HIP(hipGetLastError());
         return;
     }
-    const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP && op.lanes_per_row <= QS_VARIANT_HOP + 2;
+    const bool want_hop = op.lanes_per_row >= QS_VARIANT_HOP && op.lanes_per_row <= QS_VARIANT_HOP + 3;
     const bool hop = hop_eligible(op, adjoint) && (op.lanes_per_row == 0 || op.lanes_per_row >= QS_VARIANT_SPLIT ||
                                                    want_hop);   // split / assembled variants fall back here
     if (want_hop && !hop)
         fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
     if (hop) {
         const size_t lds = hop_args(a, op, adjoint, dot_partial != nullptr);
-        // waves sharing one row tile: QS_VARIANT_HOP + k selects 2^k
-        const int ws = want_hop ? (1 << (op.lanes_per_row - QS_VARIANT_HOP)) : 1;
-        const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock / ws);
+        // waves sharing one row tile: QS_VARIANT_HOP + k selects 2^k (k <= 2),
+        // + 3 one wave.  Parent-table sectors of 2^17 rows or more (and + 3
+        // at any size) gather a pre-scaled x: one gather per neighbour
+        // instead of x and 1/sqrt(D), the kernel being bound by the issue of
+        // divergent gathers (C2, L=28: 58 us against 73 us per product,
+        // the pre-scale pass included)
         const int cm = hop_mode(b);
+        const int k = want_hop ? op.lanes_per_row - QS_VARIANT_HOP : 0;
+        const int ws = k == 3 ? 1 : 1 << k;
+        const bool ps = cm == CM_PTAB && (k == 3 || b.Ns >= ((int64_t)1 << 17));
+        const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock / ws);
         Launch L{grid, lds, s, nullptr, nullptr, nullptr};
         HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                     adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
-        dispatch_hop(a, dtype, cc, cm, track, ws, x, y, L, H);
+        dispatch_hop(a, dtype, cc, cm, track, ws, ps, x, y, L, H);
         QS_HIP(hipGetLastError());
         return;
     }

@@ -231,7 +231,7 @@ class QuantumLinearOperator:
 
     def set_variant(self, lanes_per_row):
         """Fix the matvec kernel variant: 0 = automatic, 1/2/4/8 = the general kernel
-        with that many lanes per row, VARIANT_HOP (+1, +2) = the two-bit hop kernel,
+        with that many lanes per row, VARIANT_HOP (+1, +2, +3) = the two-bit hop kernel,
         VARIANT_SPLIT = the split kernel (hop couplings, no symmetry or spin inversion only),
         VARIANT_SELL = plain products from the assembled matrix (assembled by the first product)."""
         check(_lib.lib().qs_op_set_variant(self._h, int(lanes_per_row)))

@@ -227,6 +227,14 @@ def test_fullsize_c2_basis_and_ground_state(gpu):
     rng = np.random.default_rng(2)
     x, y = rng.standard_normal(b.Ns), rng.standard_normal(b.Ns)
     assert abs(x @ op.matvec(y) - op.matvec(x) @ y) < 1e-10 * abs(x @ op.matvec(y)) + 1e-9
+    # at this size the matrix-free product gathers a pre-scaled x; the
+    # assembled rows carry the unscaled weights of the pull enumeration
+    print("c2 assembled vs matrix-free", flush=True)
+    op_a = qsgpu.QuantumLinearOperator(b, terms).assemble()
+    op_a.set_variant(qsgpu.VARIANT_SELL)
+    hy = op.matvec(y)
+    assert np.linalg.norm(op_a.matvec(y) - hy) <= 1e-13 * np.linalg.norm(hy)
+    del op_a
     print("c2 eigsh fp64", flush=True)
     v0 = rng.standard_normal(b.Ns)
     e64, v64 = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0)
@@ -257,7 +265,7 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
     rng = np.random.default_rng(5)
     x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
     want = ref.matvec(x)
-    hops = (qsgpu.VARIANT_HOP, qsgpu.VARIANT_HOP + 1, qsgpu.VARIANT_HOP + 2)
+    hops = (qsgpu.VARIANT_HOP, qsgpu.VARIANT_HOP + 1, qsgpu.VARIANT_HOP + 2, qsgpu.VARIANT_HOP + 3)
     for lpr in (1, 2, 4, 8) + hops:
         op.set_variant(lpr)
         got = op.matvec(x)
