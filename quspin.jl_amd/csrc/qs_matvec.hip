@@ -513,6 +513,14 @@ __device__ __forceinline__ void hop_target(const MVArgs &A, const HopDev &h, uin
     }
 }
 
+// Element i of base through a 32-bit byte offset (the caller guarantees
+// i * sizeof(T) < 2^32): the load takes the scalar-base + vector-offset form
+// and needs no 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ const T *at32(const T *base, uint32_t i) {
+    return (const T *)((const char *)base + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+
 // FP64 form of a vector type (the pre-scaled x of the parent-table kernel)
 template <typename V> struct Wide { using type = double; };
 template <> struct Wide<cplx_f> { using type = cplx_d; };
@@ -534,7 +542,8 @@ __global__ __launch_bounds__(256) void k_prescale(const V *__restrict__ x, const
 
 // Adds the flip groups g = g_first, g_first + WS, ... of row `row` (own
 // state s, rank rs) to (ar, ai), kNB neighbours per batch.  PS: neighbours
-// are gathered from the pre-scaled xs (parent-table mode only).
+// are gathered from the pre-scaled xs (parent-table mode only), the parent
+// table and xs through 32-bit byte offsets (the host checks they fit).
 template <typename V, bool CC, int CM, bool TRACK, bool PC, int WS, bool PS>
 __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__restrict__ hops,
                                            const V *__restrict__ x, const typename Wide<V>::type *__restrict__ xs,
@@ -551,6 +560,7 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
         uint32_t col[NB];
         double mr[NB], mi[NB];
         int elem[NB];
+        uint32_t livebits = 0;   // neighbour j acts on this row
 #pragma unroll
         for (int j = 0; j < NB; j++) {
             const int gi = g0 + j * WS;
@@ -608,13 +618,17 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
                 col[j] = live ? c : col[j];
                 mr[j] = live ? m_r : 0.0;
                 mi[j] = live ? -m_i : 0.0;
+                livebits |= (uint32_t)live << j;
             }
         }
         double f[NB];
         if (CM == CM_PTAB) {
+            // dead neighbours issue no parent-table request (their lanes are
+            // masked off the gather) and fall back to the own row below
             uint32_t e[NB];
 #pragma unroll
-            for (int j = 0; j < NB; j++) e[j] = A.ptab[col[j]];
+            for (int j = 0; j < NB; j++)
+                e[j] = ((livebits >> j) & 1) ? (PS ? *at32(A.ptab, col[j]) : A.ptab[col[j]]) : 0xFFFFFFFFu;
 #pragma unroll
             for (int j = 0; j < NB; j++) {
                 if (e[j] == 0xFFFFFFFFu) {
@@ -634,28 +648,32 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
         double vr[NB], vi[NB];
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-            if (PS) VTraits<typename Wide<V>::type>::load(xs, col[j], vr[j], vi[j]);
+            if (PS) VTraits<typename Wide<V>::type>::load(at32(xs, col[j]), 0, vr[j], vi[j]);
             else T::load(x, col[j], vr[j], vi[j]);
         }
+        // real couplings without characters: the weight is mr * fac, the
+        // same value the complex form gives
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-            double wr = mr[j], wi = mi[j];
+            double wr = mr[j], wi = CC ? mi[j] : 0.0;
             if (CM == CM_PTAB) {
-                double cr = 1.0, ci = 0.0;
+                const double fac = f[j] * own_scale;
                 if (TRACK) {
                     const int bt = elem[j] / A.g.npoint, bp = elem[j] - bt * A.g.npoint;
                     const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
                     const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
-                    cr = lr * pr - li * pi;
-                    ci = lr * pi + li * pr;
+                    const double cr = (lr * pr - li * pi) * fac;
+                    const double ci = (lr * pi + li * pr) * fac;
+                    wr = CC ? mr[j] * cr - mi[j] * ci : mr[j] * cr;
+                    wi = CC ? mr[j] * ci + mi[j] * cr : mr[j] * ci;
+                } else {
+                    wr = mr[j] * fac;
+                    if (CC) wi = mi[j] * fac;
                 }
-                const double fac = f[j] * own_scale;
-                cr *= fac;
-                ci *= fac;
-                wr = mr[j] * cr - mi[j] * ci;
-                wi = mr[j] * ci + mi[j] * cr;
             }
             if (VC) {
+                // one form for every weight: a job gives the same bits in the
+                // character-tracking kernel a batch may run it in
                 double v_i = vi[j];
                 if (A.conj_x) v_i = -v_i;
                 ar += wr * vr[j] - wi * v_i;
@@ -1952,7 +1970,9 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         const int cm = hop_mode(b);
         const int k = want_hop ? op.lanes_per_row - QS_VARIANT_HOP : 0;
         const int ws = k == 3 ? 1 : 1 << k;
-        const bool ps = cm == CM_PTAB && (k == 3 || b.Ns >= ((int64_t)1 << 17));
+        const size_t wsz = dtype_complex(dtype) ? 16 : 8;
+        const bool ps = cm == CM_PTAB && (k == 3 || b.Ns >= ((int64_t)1 << 17)) &&
+                        (uint64_t)b.Ns * wsz < ((uint64_t)1 << 32) && (uint64_t)b.ptab.n * 4 < ((uint64_t)1 << 32);
         const int grid = dot_partial ? matvec_grid(b.Ns) : grid_for(b.Ns, kMvBlock / ws);
         Launch L{grid, lds, s, nullptr, nullptr, nullptr};
         HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
