@@ -864,6 +864,7 @@ struct qs_batch {
     hipStream_t stream = nullptr;
     std::unique_ptr<qs::BatchMatvec, void (*)(qs::BatchMatvec *)> bm{nullptr, [](qs::BatchMatvec *) {}};
     ~qs_batch() {
+        bm.reset();   // its job table waits on the stream before freeing
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

@@ -25,6 +25,7 @@ struct BWork {
     hipStream_t s;   // every launch using this workspace runs on it
     SBuf<double> partial, result, coef;
     std::vector<double> host;
+    JobTable jt;     // job tables of launches over many requests
     void need(size_t p, size_t r, size_t c) {
         if (partial.n < p) partial.alloc(std::max<size_t>(p, 4096), s);
         if (result.n < r) result.alloc(std::max<size_t>(r, 1024), s);
@@ -60,8 +61,8 @@ void bnorm2(int dtype, const std::vector<NormReq> &rq, BWork &wk, hipStream_t s,
         r.push_back(j);
         rb.push_back(1);
     }
-    launch_vec_jobs(dtype, VOP_NORM2, a, ab, s);
-    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    launch_vec_jobs(dtype, VOP_NORM2, a, ab, s, &wk.jt);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s, &wk.jt);
     wk.host.resize(2 * rq.size());
     QS_HIP(hipMemcpyAsync(wk.host.data(), wk.result.get(), sizeof(double) * 2 * rq.size(), hipMemcpyDeviceToHost, s));
     QS_HIP(hipStreamSynchronize(s));
@@ -77,7 +78,7 @@ struct AxpbyReq {          // y = a x + b y
     int64_t n;
 };
 
-void baxpby(int dtype, const std::vector<AxpbyReq> &rq, hipStream_t s) {
+void baxpby(int dtype, const std::vector<AxpbyReq> &rq, hipStream_t s, JobTable *jt = nullptr) {
     std::vector<VecJob> a;
     std::vector<int> ab;
     for (auto &q : rq) {
@@ -94,7 +95,7 @@ void baxpby(int dtype, const std::vector<AxpbyReq> &rq, hipStream_t s) {
         a.push_back(j);
         ab.push_back(j.nb);
     }
-    launch_vec_jobs(dtype, VOP_AXPBY, a, ab, s);
+    launch_vec_jobs(dtype, VOP_AXPBY, a, ab, s, jt);
 }
 
 struct CombReq {           // Y[:, :k] = V[:, :m] C (C column-major m x k)
@@ -137,7 +138,7 @@ void bcombine(int dtype, const std::vector<CombReq> &rq, BWork &wk, hipStream_t 
         a.push_back(j);
         ab.push_back(j.nb);
     }
-    if (!a.empty()) launch_vec_jobs(dtype, VOP_COMBINE, a, ab, s);
+    if (!a.empty()) launch_vec_jobs(dtype, VOP_COMBINE, a, ab, s, &wk.jt);
     // the coefficient buffer is reused by the next call
     QS_HIP(hipStreamSynchronize(s));
 }
@@ -180,8 +181,8 @@ void bdots_dev(int dtype, const std::vector<DotDev> &rq, BWork &wk, hipStream_t 
         rb.push_back(j.m);
     }
     if (np > wk.partial.n) fail(QS_EARG, "internal: batched workspace not reserved");
-    launch_vec_jobs(dtype, VOP_GRAM, g, gb, s);
-    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    launch_vec_jobs(dtype, VOP_GRAM, g, gb, s, &wk.jt);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s, &wk.jt);
 }
 
 struct UpdDev {            // w -= V[:, :m] h (h device); nrm2 (device, optional) = ||w||^2
@@ -217,8 +218,8 @@ void bupdate_dev(int dtype, const std::vector<UpdDev> &rq, BWork &wk, hipStream_
         }
     }
     if (np > wk.partial.n) fail(QS_EARG, "internal: batched workspace not reserved");
-    launch_vec_jobs(dtype, VOP_UPDATE, u, ub, s);
-    if (!r.empty()) launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    launch_vec_jobs(dtype, VOP_UPDATE, u, ub, s, &wk.jt);
+    if (!r.empty()) launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s, &wk.jt);
 }
 
 void bnorm2_dev(int dtype, const std::vector<std::pair<const void *, int64_t>> &xs, const std::vector<double *> &out,
@@ -241,8 +242,8 @@ void bnorm2_dev(int dtype, const std::vector<std::pair<const void *, int64_t>> &
         rb.push_back(1);
     }
     if (np > wk.partial.n) fail(QS_EARG, "internal: batched workspace not reserved");
-    launch_vec_jobs(dtype, VOP_NORM2, a, ab, s);
-    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s);
+    launch_vec_jobs(dtype, VOP_NORM2, a, ab, s, &wk.jt);
+    launch_vec_jobs(dtype, VOP_REDUCE, r, rb, s, &wk.jt);
 }
 
 struct AxpbyDev {          // y = a x + b y with a from device memory (CoefMode)
@@ -254,7 +255,7 @@ struct AxpbyDev {          // y = a x + b y with a from device memory (CoefMode)
     int64_t n;
 };
 
-void baxpby_dev(int dtype, const std::vector<AxpbyDev> &rq, hipStream_t s) {
+void baxpby_dev(int dtype, const std::vector<AxpbyDev> &rq, hipStream_t s, JobTable *jt) {
     std::vector<VecJob> a;
     std::vector<int> ab;
     for (auto &q : rq) {
@@ -271,7 +272,7 @@ void baxpby_dev(int dtype, const std::vector<AxpbyDev> &rq, hipStream_t s) {
         a.push_back(j);
         ab.push_back(j.nb);
     }
-    launch_vec_jobs(dtype, VOP_AXPBY, a, ab, s);
+    launch_vec_jobs(dtype, VOP_AXPBY, a, ab, s, jt);
 }
 
 // ------------------------------------------------ lockstep factorization --
@@ -329,7 +330,7 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
             ax.push_back({outs.back(), CO_INV_SQRT, req[i].second, cd(0, 0), F[i]->col(0, esz), F[i]->n});
         }
         bnorm2_dev(dtype, xs, outs, wk, s);
-        baxpby_dev(dtype, ax, s);
+        baxpby_dev(dtype, ax, s, &wk.jt);
     }
     for (int col = 0; col < max_steps; col++) {
         std::vector<size_t> act;
@@ -349,7 +350,7 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
                 if (col > 0)
                     ax.push_back({at(i, sc(i).nrm2(col - 1)), CO_NEG_SQRT, F[i]->col(col - 1, esz), cd(1, 0),
                                   F[i]->col(col + 1, esz), F[i]->n});
-            if (!ax.empty()) baxpby_dev(dtype, ax, s);
+            if (!ax.empty()) baxpby_dev(dtype, ax, s, &wk.jt);
             std::vector<DotDev> dq;
             std::vector<UpdDev> uq;
             for (size_t i : herm) {
@@ -381,7 +382,7 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
             if (col + 1 < F[i]->P.steps)
                 sc2.push_back({at(i, sc(i).nrm2(col)), CO_INV_SQRT, F[i]->col(col + 1, esz), cd(0, 0),
                                F[i]->col(col + 1, esz), F[i]->n});
-        if (!sc2.empty()) baxpby_dev(dtype, sc2, s);
+        if (!sc2.empty()) baxpby_dev(dtype, sc2, s, &wk.jt);
     }
     std::vector<double> host(tot);
     if (tot) QS_HIP(hipMemcpyAsync(host.data(), d.get(), sizeof(double) * tot, hipMemcpyDeviceToHost, s));

@@ -781,6 +781,51 @@ void stream_pool_trim() {
     if (hipMemPoolTrimTo(pool, (size_t)8 << 30) != hipSuccess) (void)hipGetLastError();
 }
 
+JobTable::~JobTable() {
+    if (s_) (void)hipStreamSynchronize(s_);
+    if (dev_) (void)hipFreeAsync(dev_, s_);
+    if (host_) (void)hipHostFree(host_);
+    (void)hipGetLastError();
+}
+
+const void *JobTable::stage(const void *src, size_t bytes, hipStream_t s) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (s_ && s != s_) {
+        // copies staged on another stream may still read the host buffer
+        QS_HIP(hipStreamSynchronize(s_));
+        used_ = 0;
+    }
+    if (used_ + need > cap_) {
+        if (s_) QS_HIP(hipStreamSynchronize(s_));
+        used_ = 0;
+        if (need > cap_) {
+            if (dev_) QS_HIP(hipFreeAsync(dev_, s_));
+            if (host_) QS_HIP(hipHostFree(host_));
+            dev_ = host_ = nullptr;
+            cap_ = 0;
+            const size_t cap = std::max<size_t>(need, (size_t)16 << 20);
+            stream_pool_setup();
+            if (hipHostMalloc((void **)&host_, cap, hipHostMallocDefault) != hipSuccess) {
+                host_ = nullptr;
+                (void)hipGetLastError();
+                fail(QS_EOOM, "pinned allocation of " + std::to_string(cap) + " bytes failed");
+            }
+            if (hipMallocAsync((void **)&dev_, cap, s) != hipSuccess) {
+                dev_ = nullptr;
+                (void)hipGetLastError();
+                fail(QS_EOOM, "device allocation of " + std::to_string(cap) + " bytes failed");
+            }
+            cap_ = cap;
+        }
+    }
+    s_ = s;
+    std::memcpy(host_ + used_, src, bytes);
+    QS_HIP(hipMemcpyAsync(dev_ + used_, host_ + used_, bytes, hipMemcpyHostToDevice, s));
+    const void *at = dev_ + used_;
+    used_ += need;
+    return at;
+}
+
 namespace {
 thread_local hipStream_t t_stream = nullptr;
 }

@@ -107,6 +107,25 @@ struct SBuf {
     T *get() const { return p; }
 };
 
+// Job descriptors of batched launches too many for the kernel arguments:
+// written to pinned host memory and copied to a device table on the
+// launch's stream.  Space is handed out in order and reused once that stream
+// has drained (a full table waits for it, then starts over).
+class JobTable {
+public:
+    JobTable() = default;
+    JobTable(const JobTable &) = delete;
+    JobTable &operator=(const JobTable &) = delete;
+    ~JobTable();
+    // stream-ordered copy of `bytes` from src; returns the device address
+    const void *stage(const void *src, size_t bytes, hipStream_t s);
+
+private:
+    char *host_ = nullptr, *dev_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    hipStream_t s_ = nullptr;   // the stream every staged copy ran on
+};
+
 // -------------------------------------------------------------- groups ----
 // A symmetry group element acts on an L-bit state as
 //   g(u) = rotl_L( rev_L^r(u) ^ f , t * a )
@@ -403,9 +422,12 @@ struct VecJobs {
     int njobs;
     VecJob job[kMaxVecJobs];
 };
-// Launch `op` over jobs (b0 and the grid are filled in here); jobs beyond
-// kMaxVecJobs go to further launches on the same stream.
-void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s);
+// Launch `op` over jobs (b0 and the grid are filled in here).  Up to
+// kMaxVecJobs jobs travel in the kernel arguments; more go through `table`
+// (one launch, each job on max(blocks) blocks) or, without one, to further
+// launches on the same stream.  A job's blocks do the same work either way.
+void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s,
+                     JobTable *table = nullptr);
 int ew_blocks(int64_t n);
 
 // Batched matvec: y_j = M_j x_j for several operators in one launch (all

@@ -963,18 +963,12 @@ struct SellJobs {
     SellJob job[32];
 };
 
+// block b of job q: its kSellBlock rows
 template <typename V>
-__global__ __launch_bounds__(kSellBlock) void kb_sell_spmv(SellJobs J) {
+__device__ __forceinline__ void sell_job_block(const SellJob &q, int b) {
     using T = VTraits<V>;
-    int lo = 0, hi = J.njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((int)blockIdx.x >= J.job[mid].b0) lo = mid;
-        else hi = mid - 1;
-    }
-    const SellJob &q = J.job[lo];
     const MVArgs &A = *q.A;
-    const int64_t row = (int64_t)((int)blockIdx.x - q.b0) * kSellBlock + threadIdx.x;
+    const int64_t row = (int64_t)b * kSellBlock + threadIdx.x;
     if (row >= A.N) return;
     const uint64_t base = q.sptr[(uint64_t)row >> 6] + (row & 63);
     const uint32_t w = q.width[(uint64_t)row >> 6];
@@ -992,6 +986,27 @@ __global__ __launch_bounds__(kSellBlock) void kb_sell_spmv(SellJobs J) {
     }
     double dot_r = 0, dot_i = 0;
     finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kSellBlock) void kb_sell_spmv(SellJobs J) {
+    int lo = 0, hi = J.njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)blockIdx.x >= J.job[mid].b0) lo = mid;
+        else hi = mid - 1;
+    }
+    sell_job_block<V>(J.job[lo], (int)blockIdx.x - J.job[lo].b0);
+}
+
+// The same over a device job table: job j owns blocks [j nbmax, (j+1) nbmax)
+// and uses the first tab[j].nb of them.
+template <typename V>
+__global__ __launch_bounds__(kSellBlock) void kb_sell_tab(const SellJob *__restrict__ tab, int nbmax) {
+    const int j = (int)blockIdx.x / nbmax, b = (int)blockIdx.x - j * nbmax;
+    const SellJob q = tab[j];
+    if (b >= q.nb) return;
+    sell_job_block<V>(q, b);
 }
 
 // ------------------------------------------------------- split kernel --
@@ -2014,6 +2029,7 @@ struct BatchMatvec {
     std::vector<int> nb, cm, pc;
     std::vector<size_t> lds;
     std::vector<char> track, cc;
+    JobTable jt;                  // job tables of launches over many assembled operators
 };
 
 bool batch_matvec_ok(const std::vector<const OpHost *> &ops, int dtype) {
@@ -2075,6 +2091,33 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
         std::vector<size_t> asm_jobs;
         for (size_t j = 0; j < jobs.size(); j++)
             if (bm.ops[jobs[j].op]->sell) asm_jobs.push_back(j);
+        if (asm_jobs.size() > 32) {
+            // one launch through a device job table, each job on nbmax blocks
+            std::vector<SellJob> tab;
+            int nbmax = 0;
+            for (size_t j : asm_jobs) {
+                const MatvecJob &mj = jobs[j];
+                const OpHost &op = *bm.ops[mj.op];
+                const SellHost &t = *op.sell;
+                const int nb = (int)((op.basis->Ns + kSellBlock - 1) / kSellBlock);
+                tab.push_back(SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(),
+                                      t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, 0, nb});
+                nbmax = std::max(nbmax, nb);
+                done[j] = 1;
+            }
+            if ((int64_t)nbmax * (int64_t)tab.size() > INT32_MAX) fail(QS_EARG, "internal: batched grid too large");
+            const SellJob *d = (const SellJob *)bm.jt.stage(tab.data(), tab.size() * sizeof(SellJob), s);
+            const int grid = nbmax * (int)tab.size();
+            switch (bm.dtype) {
+            case QS_F64: hipLaunchKernelGGL(kb_sell_tab<double>, dim3(grid), dim3(kSellBlock), 0, s, d, nbmax); break;
+            case QS_F32: hipLaunchKernelGGL(kb_sell_tab<float>, dim3(grid), dim3(kSellBlock), 0, s, d, nbmax); break;
+            case QS_C128: hipLaunchKernelGGL(kb_sell_tab<cplx_d>, dim3(grid), dim3(kSellBlock), 0, s, d, nbmax); break;
+            case QS_C64: hipLaunchKernelGGL(kb_sell_tab<cplx_f>, dim3(grid), dim3(kSellBlock), 0, s, d, nbmax); break;
+            default: fail(QS_EARG, "unknown dtype");
+            }
+            QS_HIP(hipGetLastError());
+            asm_jobs.clear();
+        }
         for (size_t c0 = 0; c0 < asm_jobs.size(); c0 += 32) {
             SellJobs J;
             J.njobs = (int)std::min<size_t>(32, asm_jobs.size() - c0);

@@ -324,10 +324,7 @@ __device__ __forceinline__ int find_job(const VecJobs &J, int &b) {
 }
 
 template <typename V>
-__global__ __launch_bounds__(kVB) void kb_vec(VecJobs J, int op) {
-    __shared__ double sh[2 * kVB];
-    int b;
-    const VecJob &q = J.job[find_job(J, b)];
+__device__ __forceinline__ void vec_job_block(const VecJob &q, int b, int op, double *sh) {
     switch (op) {
     case VOP_GRAM: gram_body<V>((const V *)q.V, q.n, q.m, (const V *)q.x, q.partial, b, q.nb, sh); break;
     case VOP_REDUCE: reduce_body(q.partial, q.nb, q.aux, b, sh); break;
@@ -338,6 +335,25 @@ __global__ __launch_bounds__(kVB) void kb_vec(VecJobs J, int op) {
         break;
     case VOP_COMBINE: combine_body<V>((const V *)q.V, q.n, q.m, q.aux, q.k, (V *)q.w, b, q.nb); break;
     }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void kb_vec(VecJobs J, int op) {
+    __shared__ double sh[2 * kVB];
+    int b;
+    const VecJob &q = J.job[find_job(J, b)];
+    vec_job_block<V>(q, b, op, sh);
+}
+
+// The same over a device job table: job j owns blocks [j nbmax, (j+1) nbmax)
+// and uses the first tab[j].b0 of them (b0 holds its block count here).
+template <typename V>
+__global__ __launch_bounds__(kVB) void kb_vec_tab(const VecJob *__restrict__ tab, int nbmax, int op) {
+    __shared__ double sh[2 * kVB];
+    const int j = (int)blockIdx.x / nbmax, b = (int)blockIdx.x - j * nbmax;
+    const VecJob q = tab[j];
+    if (b >= q.b0) return;
+    vec_job_block<V>(q, b, op, sh);
 }
 
 template <typename A, typename B>
@@ -407,7 +423,23 @@ bool combine_on_mfma(int dtype, const void *Vmat, int64_t n, int m, const double
     return true;
 }
 
-void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s) {
+void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s,
+                     JobTable *table) {
+    if (table && jobs.size() > (size_t)kMaxVecJobs) {
+        int nbmax = 0;
+        for (int nb : blocks) nbmax = std::max(nbmax, nb);
+        if (nbmax == 0) return;
+        if ((int64_t)nbmax * (int64_t)jobs.size() > INT32_MAX) fail(QS_EARG, "internal: batched grid too large");
+        for (size_t i = 0; i < jobs.size(); i++) jobs[i].b0 = blocks[i];
+        const VecJob *tab = (const VecJob *)table->stage(jobs.data(), jobs.size() * sizeof(VecJob), s);
+        const int grid = nbmax * (int)jobs.size();
+        by_dtype(dtype, [&](auto *tag) {
+            using V = std::remove_pointer_t<decltype(tag)>;
+            hipLaunchKernelGGL((kb_vec_tab<V>), dim3(grid), dim3(kVB), 0, s, tab, nbmax, op);
+        });
+        QS_HIP(hipGetLastError());
+        return;
+    }
     for (size_t first = 0; first < jobs.size(); first += kMaxVecJobs) {
         VecJobs J;
         J.njobs = (int)std::min<size_t>(kMaxVecJobs, jobs.size() - first);
