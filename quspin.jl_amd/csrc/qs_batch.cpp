@@ -279,9 +279,13 @@ void baxpby_dev(int dtype, const std::vector<AxpbyDev> &rq, hipStream_t s, JobTa
 struct BFact {
     int op;
     int64_t n;
-    SBuf<char> V;       // n x (steps + 1)
+    // n x (steps + 1), a slice of one allocation shared by every request of
+    // the call (thousands of separate stream-ordered frees cost more host
+    // time than the factorizations take on the device)
+    std::shared_ptr<SBuf<char>> store;
+    char *V = nullptr;
     KrylovProj P;
-    void *col(int j, size_t esz) const { return (void *)(V.get() + (size_t)j * n * esz); }
+    void *col(int j, size_t esz) const { return (void *)(V + (size_t)j * n * esz); }
 };
 
 // operators.jl:1655-1731 for every request at once; identical per-request
@@ -307,7 +311,6 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         f->P.dim = f->n;
         f->P.hermitian = lanczos_ok(op, dtype);
         f->P.steps = (int)std::min<int64_t>(krylov_dim, f->n);
-        f->V.alloc((size_t)f->n * (f->P.steps + 1) * esz, s);
         f->P.H.assign((size_t)(f->P.steps + 1) * f->P.steps, cd(0, 0));
         f->P.completed = f->P.steps;
         f->P.terminal = 0;
@@ -316,6 +319,17 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         tot += FactScalars(f->P.steps).total();
         max_steps = std::max(max_steps, f->P.steps);
         F.push_back(std::move(f));
+    }
+    {
+        size_t bytes = 0;
+        for (auto &f : F) bytes += ((size_t)f->n * (f->P.steps + 1) * esz + 255) & ~(size_t)255;
+        auto store = std::make_shared<SBuf<char>>(std::max<size_t>(bytes, 1), s);
+        size_t at = 0;
+        for (auto &f : F) {
+            f->store = store;
+            f->V = store->get() + at;
+            at += ((size_t)f->n * (f->P.steps + 1) * esz + 255) & ~(size_t)255;
+        }
     }
     bwork_reserve(wk, ns, max_steps + 1);
     SBuf<double> d(std::max<size_t>(tot, 1), s);
@@ -367,10 +381,10 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
             std::vector<UpdDev> u1, u2;
             for (size_t i : arn) {
                 void *w = F[i]->col(col + 1, esz);
-                d1.push_back({F[i]->V.get(), F[i]->n, col + 1, w, at(i, sc(i).h1(col))});
-                d2.push_back({F[i]->V.get(), F[i]->n, col + 1, w, at(i, sc(i).h2(col))});
-                u1.push_back({F[i]->V.get(), F[i]->n, col + 1, at(i, sc(i).h1(col)), w, nullptr});
-                u2.push_back({F[i]->V.get(), F[i]->n, col + 1, at(i, sc(i).h2(col)), w, at(i, sc(i).nrm2(col))});
+                d1.push_back({F[i]->V, F[i]->n, col + 1, w, at(i, sc(i).h1(col))});
+                d2.push_back({F[i]->V, F[i]->n, col + 1, w, at(i, sc(i).h2(col))});
+                u1.push_back({F[i]->V, F[i]->n, col + 1, at(i, sc(i).h1(col)), w, nullptr});
+                u2.push_back({F[i]->V, F[i]->n, col + 1, at(i, sc(i).h2(col)), w, at(i, sc(i).nrm2(col))});
             }
             bdots_dev(dtype, d1, wk, s);
             bupdate_dev(dtype, u1, wk, s);
@@ -476,11 +490,11 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
         const KrylovProj &P = F[i]->P;
         const int c = P.completed;
         if (P.hermitian) {
-            cq.push_back({F[i]->V.get(), F[i]->n, c, std::move(tcoefs[i]), (int)T.scales.size(), T.out});
+            cq.push_back({F[i]->V, F[i]->n, c, std::move(tcoefs[i]), (int)T.scales.size(), T.out});
         } else {
             for (size_t j = 0; j < T.scales.size(); j++) {
                 if (T.scales[j] == cd(0, 0)) continue;
-                cq.push_back({F[i]->V.get(), F[i]->n, c, std::move(jcoefs[i][j]), 1, outcol(t, j)});
+                cq.push_back({F[i]->V, F[i]->n, c, std::move(jcoefs[i][j]), 1, outcol(t, j)});
             }
         }
     }
@@ -549,7 +563,7 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
         for (size_t r = 0; r < fac.size(); r++) {
             nmatvec[chains[fac[r]].task] += m2[r];
             const Call &cl = chains[fac[r]].stack.back();
-            pq.push_back({F[r]->V.get(), F[r]->n, F[r]->P.completed, std::move(coefs[r]), 1, cl.out});
+            pq.push_back({F[r]->V, F[r]->n, F[r]->P.completed, std::move(coefs[r]), 1, cl.out});
         }
         bcombine(dtype, pq, wk, s);
         std::vector<NormReq> on;
