@@ -331,6 +331,19 @@ def main():
                       % (args.cpu_sample, npar, ts, t_full),
             "speedup_gpu_over_cpu": round(t_full / t_mv, 1),
         }
+        # the same loop on the host's cores (at most 16, the share of one GPU)
+        nth = max(1, min(os.cpu_count() or 1, 16))
+        sample_mt = min(npar, args.cpu_sample * 4)
+        print("bench: CPU baseline, %d threads, %d-state sample" % (nth, sample_mt), file=sys.stderr, flush=True)
+        ts_mt = oracle.bench_parent_apply_mt(L, 14, False, terms, sample_mt, nth)
+        t_full_mt = ts_mt * npar / sample_mt
+        result["cpu_baseline_threads"] = {
+            "value": round(bytes_alg / t_full_mt / 1e9, 9), "unit": "GB/s", "cores": nth, "kind": "port",
+            "sample": "the same loop on %d OpenMP threads (columns split, atomic scatter) over the first %d "
+                      "parent states (%.1f s), extrapolated to one full matvec (%.1f s)"
+                      % (nth, sample_mt, ts_mt, t_full_mt),
+            "speedup_gpu_over_cpu": round(t_full_mt / t_mv, 1),
+        }
     print(json.dumps(result))
 
 

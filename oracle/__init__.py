@@ -71,6 +71,8 @@ def lib():
     L.qo_csr_free.restype = None
     L.qo_bench_parent_apply.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, c_i64, c_int]
     L.qo_bench_parent_apply.restype = c_dbl
+    L.qo_bench_parent_apply_mt.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, c_i64, c_int, c_int]
+    L.qo_bench_parent_apply_mt.restype = c_dbl
     _lib = L
     return L
 
@@ -248,3 +250,10 @@ def bench_parent_apply(L, nup, pauli, terms, n_sample, repeats=1):
     ops, ncoup, coef, sites = _flatten_terms(terms)
     return lib().qo_bench_parent_apply(L, nup, int(bool(pauli)), len(terms), ops, _ptr(ncoup), _ptr(coef),
                                        _ptr(sites), int(n_sample), int(repeats))
+
+
+def bench_parent_apply_mt(L, nup, pauli, terms, n_sample, threads, repeats=1):
+    """The same loop on `threads` OpenMP threads (columns split, atomic scatter)."""
+    ops, ncoup, coef, sites = _flatten_terms(terms)
+    return lib().qo_bench_parent_apply_mt(L, nup, int(bool(pauli)), len(terms), ops, _ptr(ncoup), _ptr(coef),
+                                          _ptr(sites), int(n_sample), int(repeats), int(threads))

@@ -1030,8 +1030,8 @@ void qo_csr_export(const qo_csr *m, int64_t *rowptr, int64_t *col, double *val)
  * extrapolated full-sector time is a lower bound on the reference's cost.
  * Returns seconds for the sample (best of `repeats`). */
 #include <time.h>
-double qo_bench_parent_apply(int L, int nup, int pauli, int n_terms, const char **ops, const int *ncoup,
-                             const double *coef, const int *sites, int64_t n_sample, int repeats)
+static double bench_parent_apply(int L, int nup, int pauli, int n_terms, const char **ops, const int *ncoup,
+                                 const double *coef, const int *sites, int64_t n_sample, int repeats, int threads)
 {
     int64_t npar = qo_fixed_weight_states(L, nup, NULL);
     uint64_t *states = malloc(sizeof(uint64_t) * (size_t)npar);
@@ -1059,16 +1059,38 @@ double qo_bench_parent_apply(int L, int nup, int pauli, int n_terms, const char 
     for (int r = 0; r < repeats; r++) {
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
-        for (int i = 0; i < total; i++) {
+        if (threads <= 1) {
+            for (int i = 0; i < total; i++) {
+                for (int64_t col = 0; col < n_sample; col++) {
+                    uint64_t out;
+                    cplx amp;
+                    if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
+                    int64_t row = lookup_state(states, npar, out);
+                    if (row < 0) continue;
+                    cplx in = x[col];
+                    if (in.re == 0 && in.im == 0) continue;
+                    y[row] = c_add(y[row], c_mul(amp, in));
+                }
+            }
+        } else {
+            /* the same push loop over columns split across threads; the
+               scattered additions into y are atomic */
+#pragma omp parallel for num_threads(threads) schedule(static)
             for (int64_t col = 0; col < n_sample; col++) {
-                uint64_t out;
-                cplx amp;
-                if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
-                int64_t row = lookup_state(states, npar, out);
-                if (row < 0) continue;
                 cplx in = x[col];
                 if (in.re == 0 && in.im == 0) continue;
-                y[row] = c_add(y[row], c_mul(amp, in));
+                for (int i = 0; i < total; i++) {
+                    uint64_t out;
+                    cplx amp;
+                    if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
+                    int64_t row = lookup_state(states, npar, out);
+                    if (row < 0) continue;
+                    cplx p = c_mul(amp, in);
+#pragma omp atomic
+                    y[row].re += p.re;
+#pragma omp atomic
+                    y[row].im += p.im;
+                }
             }
         }
         clock_gettime(CLOCK_MONOTONIC, &t1);
@@ -1079,4 +1101,19 @@ double qo_bench_parent_apply(int L, int nup, int pauli, int n_terms, const char 
     (void)sink;
     free(states); free(x); free(y); free(cs);
     return best;
+}
+
+/* Seconds of the reference's parent-space apply loop (operators.jl:2554-2589)
+   over the first n_sample parent states, best of `repeats`: one thread. */
+double qo_bench_parent_apply(int L, int nup, int pauli, int n_terms, const char **ops, const int *ncoup,
+                             const double *coef, const int *sites, int64_t n_sample, int repeats)
+{
+    return bench_parent_apply(L, nup, pauli, n_terms, ops, ncoup, coef, sites, n_sample, repeats, 1);
+}
+
+/* The same on `threads` OpenMP threads (columns split, atomic scatter). */
+double qo_bench_parent_apply_mt(int L, int nup, int pauli, int n_terms, const char **ops, const int *ncoup,
+                                const double *coef, const int *sites, int64_t n_sample, int repeats, int threads)
+{
+    return bench_parent_apply(L, nup, pauli, n_terms, ops, ncoup, coef, sites, n_sample, repeats, threads);
 }
