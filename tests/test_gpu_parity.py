@@ -245,6 +245,30 @@ def test_fullsize_c2_basis_and_ground_state(gpu):
     assert abs(e32[0] - e64[0]) <= 1e-8
 
 
+def test_prescaled_gather_complex_sector_fullsize(gpu):
+    # L=26 nup=13 k=1 (about 4.0e5 rows, complex characters): the matrix-free
+    # product gathers a pre-scaled x through the character-tracking kernel;
+    # the assembled rows carry the unscaled weights.  ComplexF64 and the
+    # ComplexF32 stream (FP64 pre-scaled copy) against them.
+    print("prescaled complex sector", flush=True)
+    L = 26
+    b = qsgpu.SpinBasis1D(L, nup=13, pauli=False, kblock=1)
+    assert b.Ns >= 1 << 17
+    terms = _dense.xxz_terms(L, J=1.0, delta=0.7, J2=0.3)
+    op = qsgpu.QuantumLinearOperator(b, terms)
+    op_a = qsgpu.QuantumLinearOperator(b, terms).assemble()
+    op_a.set_variant(qsgpu.VARIANT_SELL)
+    rng = np.random.default_rng(13)
+    x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    want = op_a.matvec(x)
+    got = op.matvec(x)
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    got32 = op.matvec(x.astype(np.complex64))
+    assert np.linalg.norm(got32 - want) <= 1e-6 * np.linalg.norm(want)
+    # Hermitian: <x, Hx> is real
+    assert abs(np.vdot(x, got).imag) <= 1e-10 * abs(np.vdot(x, got))
+
+
 @pytest.mark.parametrize("case", [(12, 6, {}), (12, None, {}), (12, 6, dict(zblock=-1)), (12, None, dict(zblock=1)),
                                   (12, 6, dict(kblock=0, pblock=1, zblock=1)), (12, 6, dict(kblock=5)),
                                   (12, 6, dict(kblock=6, pblock=-1, zblock=-1))], ids=_id)
