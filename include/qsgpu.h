@@ -134,7 +134,7 @@ int qs_op_stream(const qs_op *op, void **stream);
  * those two, L <= 32; symmetry sectors also need the parent lookup table)
  * with 2^k wavefronts sharing each row tile's couplings; QS_VARIANT_HOP + 3 =
  * the hop kernel with one wavefront per tile that, in a symmetry sector,
- * first writes x / sqrt(norm) to a stream-ordered FP64 buffer and then
+ * first writes x / sqrt(norm) to an FP64 buffer held by the operator and then
  * gathers one value per neighbour.  No reference counterpart: the reference
  * has one CPU loop. */
 #define QS_VARIANT_HOP 16

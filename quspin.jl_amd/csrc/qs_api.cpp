@@ -18,6 +18,8 @@ namespace qs {
 const std::string &last_error();
 
 OpHost::~OpHost() {
+    xs_buf.release();   // hipFree: waits for the products that read it
+    if (xs_ev) (void)hipEventDestroy(xs_ev);
     if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 

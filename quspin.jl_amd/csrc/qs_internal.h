@@ -325,6 +325,12 @@ struct OpHost {
     hipStream_t own_stream = nullptr;
     int lanes_per_row = 0;       // matvec kernel variant (0 = default heuristic)
     int graphs = 0;              // replay the expm recursion's factorizations from a HIP graph
+    // the hop kernel's pre-scaled x (parent-table sectors), kept between
+    // products; xs_ev marks its last use on xs_stream, so a product on
+    // another stream first waits for it
+    mutable DevBuf<char> xs_buf;
+    mutable hipEvent_t xs_ev = nullptr;
+    mutable hipStream_t xs_stream = nullptr;
     // staging buffers for host vectors
     DevBuf<char> stage_x, stage_y;
     // basis without symmetry: the extracted diagonal alone (structurally

@@ -1416,37 +1416,43 @@ void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x,
 #undef QS_HOP_WS
 }
 
-// ps in the parent-table mode: x * (1/sqrt(D)) is written once to a
-// stream-ordered FP64 buffer and the hop kernel gathers from it
+// ps in the parent-table mode: x * (1/sqrt(D)) is written once to the
+// operator's FP64 buffer (kept between products: a stream-ordered
+// allocation per product cost more host time than the product itself) and
+// the hop kernel gathers from it
 template <typename V, bool CC>
 void hop_cm(const MVArgs &a, int cm, bool track, int ws, bool ps, const void *x, void *y, const Launch &L,
-            const HopLaunch &H) {
+            const HopLaunch &H, const OpHost &op) {
     if (ps && cm == CM_PTAB) {
         using W = typename Wide<V>::type;
         const int64_t n = a.N;
-        W *xs = nullptr;
-        QS_HIP(hipMallocAsync((void **)&xs, sizeof(W) * (size_t)std::max<int64_t>(n, 1), L.s));
+        const size_t bytes = sizeof(W) * (size_t)std::max<int64_t>(n, 1);
+        if (op.xs_buf.n < bytes) op.xs_buf.alloc(bytes);   // hipFree of the old one waits for its readers
+        if (!op.xs_ev) QS_HIP(hipEventCreateWithFlags(&op.xs_ev, hipEventDisableTiming));
+        if (op.xs_stream && op.xs_stream != L.s) QS_HIP(hipStreamWaitEvent(L.s, op.xs_ev, 0));
+        W *xs = (W *)op.xs_buf.get();
         const int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
         hipLaunchKernelGGL((k_prescale<V>), dim3(std::max(g, 1)), dim3(256), 0, L.s, (const V *)x, a.inv_nrm, xs, n);
         dispatch_hop_cm<V, CC>(a, cm, track, ws, x, xs, y, L, H);
-        QS_HIP(hipFreeAsync(xs, L.s));
+        QS_HIP(hipEventRecord(op.xs_ev, L.s));
+        op.xs_stream = L.s;
         return;
     }
     dispatch_hop_cm<V, CC>(a, cm, track, ws, x, nullptr, y, L, H);
 }
 
 void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, int ws, bool ps, const void *x, void *y,
-                  const Launch &L, const HopLaunch &H) {
+                  const Launch &L, const HopLaunch &H, const OpHost &op) {
     switch (dtype) {
-    case QS_F64: hop_cm<double, false>(a, cm, track, ws, ps, x, y, L, H); break;
-    case QS_F32: hop_cm<float, false>(a, cm, track, ws, ps, x, y, L, H); break;
+    case QS_F64: hop_cm<double, false>(a, cm, track, ws, ps, x, y, L, H, op); break;
+    case QS_F32: hop_cm<float, false>(a, cm, track, ws, ps, x, y, L, H, op); break;
     case QS_C128:
-        if (cc) hop_cm<cplx_d, true>(a, cm, track, ws, ps, x, y, L, H);
-        else hop_cm<cplx_d, false>(a, cm, track, ws, ps, x, y, L, H);
+        if (cc) hop_cm<cplx_d, true>(a, cm, track, ws, ps, x, y, L, H, op);
+        else hop_cm<cplx_d, false>(a, cm, track, ws, ps, x, y, L, H, op);
         break;
     case QS_C64:
-        if (cc) hop_cm<cplx_f, true>(a, cm, track, ws, ps, x, y, L, H);
-        else hop_cm<cplx_f, false>(a, cm, track, ws, ps, x, y, L, H);
+        if (cc) hop_cm<cplx_f, true>(a, cm, track, ws, ps, x, y, L, H, op);
+        else hop_cm<cplx_f, false>(a, cm, track, ws, ps, x, y, L, H, op);
         break;
     default: fail(QS_EARG, "unknown dtype");
     }
@@ -1992,7 +1998,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         Launch L{grid, lds, s, nullptr, nullptr, nullptr};
         HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                     adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
-        dispatch_hop(a, dtype, cc, cm, track, ws, ps, x, y, L, H);
+        dispatch_hop(a, dtype, cc, cm, track, ws, ps, x, y, L, H, op);
         QS_HIP(hipGetLastError());
         return;
     }
