@@ -157,3 +157,13 @@ def test_burnside_dimensions_match_survey_configs():
     assert _dense.burnside_count(12, 6, kblock=0, pblock=1, zblock=1) == \
         oracle.Sector(12, nup=6, kblock=0, pblock=1, zblock=1).Ns
     assert math.comb(16, 8) == 12870
+
+
+def test_cpu_baseline_loops_run_on_one_and_several_threads():
+    # the bench's CPU baseline (the parent-space apply loop of
+    # operators.jl:2554-2589) on one thread and on several: both time a
+    # bounded sample and return a positive duration
+    terms = _dense.xxz_terms(12, J=1.0, delta=1.0, J2=0.4)
+    t1 = oracle.bench_parent_apply(12, 6, False, terms, 500, 1)
+    t4 = oracle.bench_parent_apply_mt(12, 6, False, terms, 500, 4)
+    assert t1 > 0 and t4 > 0
