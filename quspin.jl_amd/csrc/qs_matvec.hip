@@ -1308,25 +1308,34 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
             }
         }
         wave_lds_sync();
+        // every row: diagonal, low hops, cross hops, in that order; the loops
+        // run group by group over the lane's rows, so each group's constants
+        // are loaded once per block
+        uint32_t lw[kSpRows];
 #pragma unroll
         for (int j = 0; j < kSpRows; j++) {
             const int r = lane + 64 * j;
+            lw[j] = 0;
             if (r >= n) continue;
             const uint32_t l = s_words[cw + r];
-            {
-                double dr = 0, di = 0;
-                diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
-                di = -di;   // conj(amp): the terms are the adjoint couplings
-                if (A.user_diag) {
-                    const int64_t row = (int64_t)off + r;
-                    dr += A.user_diag[2 * row];
-                    di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
-                }
-                acc_nb<V>(dr, di, xo_r[j], xo_i[j], ar[j], ai[j]);
+            lw[j] = l;
+            double dr = 0, di = 0;
+            diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
+            di = -di;   // conj(amp): the terms are the adjoint couplings
+            if (A.user_diag) {
+                const int64_t row = (int64_t)off + r;
+                dr += A.user_diag[2 * row];
+                di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
             }
-            // low hops: neighbours in the staged own block
-            for (int g = 0; g < D.nlow; g++) {
-                const HopDev hp = hops[g];
+            acc_nb<V>(dr, di, xo_r[j], xo_i[j], ar[j], ai[j]);
+        }
+        // low hops: neighbours in the staged own block
+        for (int g = 0; g < D.nlow; g++) {
+            const HopDev hp = hops[g];
+#pragma unroll
+            for (int j = 0; j < kSpRows; j++) {
+                if (lane + 64 * j >= n) continue;
+                const uint32_t l = lw[j];
                 const uint32_t blo = (l >> hp.lo) & 1, bhi = (l >> hp.hi) & 1;
                 const bool lv = (hp.nzmask >> (blo | (bhi << 1))) & 1;
                 // with a particle number only patterns 1 and 2 are live
@@ -1337,10 +1346,15 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
                 T::load(s_x, nid, vr, vi);
                 acc_nb<V>(m_r, -m_i, vr, vi, ar[j], ai[j]);
             }
-            // cross hops: staged neighbour blocks (or HBM gathers past the budget)
-            for (int c = 0; c < nc; c++) {
-                const SpHop ec = s_list[D.nhigh + c];
-                const uint32_t lo = (ec.flags >> 8) & 0xFF, bhi = (ec.flags >> 1) & 1;
+        }
+        // cross hops: staged neighbour blocks (or HBM gathers past the budget)
+        for (int c = 0; c < nc; c++) {
+            const SpHop ec = s_list[D.nhigh + c];
+            const uint32_t lo = (ec.flags >> 8) & 0xFF, bhi = (ec.flags >> 1) & 1;
+#pragma unroll
+            for (int j = 0; j < kSpRows; j++) {
+                if (lane + 64 * j >= n) continue;
+                const uint32_t l = lw[j];
                 if (((l >> lo) & 1) == bhi) continue;
                 uint32_t l2 = l ^ (1u << lo);
                 if (ec.flags & 1) l2 = ~l2 & D.lmask;
@@ -1354,7 +1368,11 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
                 }
                 acc_nb<V>(ec.re, ec.im, vr, vi, ar[j], ai[j]);
             }
-            finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
+        }
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            const int r = lane + 64 * j;
+            if (r < n) finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
         }
         wave_lds_sync();   // the block's LDS reads are done before the next block's writes
     }
