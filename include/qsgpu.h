@@ -7,8 +7,8 @@
  * protocol (size / eltype / ishermitian / mul!) that QuantumLinearOperator
  * implements (operators.jl:3128-3245) and that Arpack.eigs (operators.jl:3384)
  * and the Krylov exponential (operators.jl:1642-1653) consume.  Each entry
- * point below names the reference function it stands in for; a Julia
- * `ccall` binding of every one is shown in INTEGRATION.md.
+ * point below names the reference function it stands in for; the Julia
+ * `ccall` shim in quspin.jl_amd/julia/QuSpinGPU.jl binds them.
  *
  * Conventions
  *  - Every function returns a status code (QS_OK on success).  The message of

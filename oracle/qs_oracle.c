@@ -907,7 +907,7 @@ int qo_op_ncouplings(const qo_op *o) { return o->ncoup; }
 typedef struct qo_csr {
     int64_t n, nnz;
     int64_t *rowptr, *col;
-    double *val;   /* real FP64 (the C1 case is real) */
+    double *val;   /* real FP64 (the L=16 XXZ benchmark case is real) */
 } qo_csr;
 
 typedef struct { int64_t row, col; double v; } rtrip_t;

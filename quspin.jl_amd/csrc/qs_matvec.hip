@@ -2004,8 +2004,8 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         // + 3 one wave.  Parent-table sectors of 2^17 rows or more (and + 3
         // at any size) gather a pre-scaled x: one gather per neighbour
         // instead of x and 1/sqrt(D), the kernel being bound by the issue of
-        // divergent gathers (C2, L=28: 58 us against 73 us per product,
-        // the pre-scale pass included)
+        // divergent gathers (J1-J2 chain, L=28 k=0 p=+1 z=+1, 361,270 rows:
+        // 58 us against 73 us per product, the pre-scale pass included)
         const int cm = hop_mode(b);
         const int k = want_hop ? op.lanes_per_row - QS_VARIANT_HOP : 0;
         const int ws = k == 3 ? 1 : 1 << k;

@@ -64,7 +64,7 @@ def test_blockops_particle_blocks_match_full_evolve(gpu):
 
 @pytest.mark.parametrize("L", [8, 10])
 def test_blockops_momentum_blocks_against_dense(gpu, L):
-    # C4 in miniature: every momentum block of the half-filled XXZ chain (Delta = 2),
+    # the L=24 block-evolution benchmark in miniature: every momentum block of the half-filled XXZ chain (Delta = 2),
     # the Neel state projected into each, exp(-iHt) recombined in the parent space
     print("blockops momentum L=%d" % L, flush=True)
     nup = L // 2

@@ -161,7 +161,7 @@ def test_eigsh_fp64_against_dense(gpu, case, which):
 
 
 def test_eigsh_c1_against_arpack(gpu):
-    # C1: L=16 nup=8 XXZ Delta=0.5, eigsh k=4 :SA against scipy's ARPACK on the oracle CSR
+    # L=16 nup=8 XXZ Delta=0.5, eigsh k=4 :SA against scipy's ARPACK on the oracle CSR
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
     L = 16
@@ -209,7 +209,7 @@ def test_expm_against_dense(gpu, case):
 
 
 def test_fullsize_c2_basis_and_ground_state(gpu):
-    # C2: J1-J2 XXZ L=28, nup=14, k=0, p=+1, z=+1.  Size-independent properties:
+    # J1-J2 XXZ L=28, nup=14, k=0, p=+1, z=+1 (361,270 rows).  Size-independent properties:
     # dimension from the character formula, orbit sizes summing to C(28,14),
     # Hermiticity <x,Hy> = <Hx,y>, an FP64 ground state with a small residual,
     # and the FP32-stream eigenvalue within 1e-8 of the FP64 one.
@@ -357,7 +357,7 @@ def test_split_kernel_fullsize_c3(gpu):
     # kernel (validated against the oracle at small L) to 1e-12 in FP64, and
     # the FP32 stream is within 2e-6; the random fields cancel in z+.
     import torch
-    print("split fullsize C3", flush=True)
+    print("split fullsize L=32 z+", flush=True)
     b = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
     h = np.random.default_rng(0xC3).uniform(-2.0, 2.0, 32)
     terms = _dense.xxz_terms(32, J=1.0, delta=1.0, J2=0.0, periodic=True) + \
@@ -427,7 +427,7 @@ def test_assembled_matrix_matches_oracle(gpu, case):
 
 
 def test_assembled_csr_c1_equals_reference_csr(gpu):
-    # C1: XXZ L=16, nup=8, pauli=false, J=1, Delta=0.5 -- the reference's CSR
+    # XXZ L=16, nup=8, pauli=false, J=1, Delta=0.5 -- the reference's CSR
     # (zero diagonals dropped) has 117,794 stored entries; the device assembly
     # exported as CSR has the same structure and values
     L = 16

@@ -137,7 +137,7 @@ def test_labels_where_ties_are_exact():
 
 
 def test_c1_csr_structure_and_product():
-    # C1: XXZ L=16, nup=8, pauli=false, J=1, Delta=0.5; nnz = 109,824 off-diagonal + 7,970 nonzero diagonal
+    # XXZ L=16, nup=8, pauli=false, J=1, Delta=0.5; nnz = 109,824 off-diagonal + 7,970 nonzero diagonal
     L = 16
     sec = oracle.Sector(L, nup=8, pauli=False)
     terms = [("+-", [(0.5, i, i % L + 1) for i in range(1, L + 1)]), ("-+", [(0.5, i, i % L + 1) for i in range(1, L + 1)]),
