@@ -201,7 +201,7 @@ def main():
                     help="also run a plain 1 GiB device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on exactly one GPU; the multi-GPU sharding is a design note only "
+        print("bench.py: the library runs on exactly one GPU; sharding across GPUs is not implemented "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np
