@@ -57,6 +57,11 @@ def variant_name(v):
     import qsgpu
     if v == qsgpu.VARIANT_SELL:
         return "k_sell_spmv (assembled SELL-64 rows)"
+    if v >= qsgpu.VARIANT_SPLIT + 4:
+        return "k_matvec_split2 x 2 passes (lean split kernel, tiled two-pass form, tile budget %d MB)" % (
+            1 << (v - qsgpu.VARIANT_SPLIT - 4))
+    if v == qsgpu.VARIANT_SPLIT + 3:
+        return "k_matvec_split2 (lean split kernel: table-driven low / cross hops, scalar high hops)"
     if v >= qsgpu.VARIANT_SPLIT:
         fewer = v - qsgpu.VARIANT_SPLIT
         return "k_matvec_split (high-word blocks, low word in LDS%s)" % (
@@ -135,6 +140,41 @@ def bench_c4(n_times):
     return out
 
 
+def bench_c3_eigsh(op3, n3):
+    """Precision form: the k=4 lowest eigenvalues of the L=32 z+ sector in the FP32-stream form
+    (FP32 Krylov basis, FP64 dots / norms / tridiagonal solve, FP64 Rayleigh quotients) and in
+    FP64; each eigenvalue's error against the FP64 GPU run, and each FP64 Ritz vector's residual."""
+    import numpy as np
+    import torch
+    import qsgpu
+    print("bench: C3 eigsh k=4 FP64 / FP32 stream", file=sys.stderr, flush=True)
+    v0 = np.random.default_rng(0x5EED).standard_normal(n3)
+    out = {"k": 4, "which": "SA", "tol": 1e-10}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e64, vec = qsgpu.eigsh(op3, k=4, which="SA", tol=1e-10, v0=v0)
+    out["seconds_fp64"] = round(time.perf_counter() - t0, 3)
+    out["matvecs_fp64"] = qsgpu.eigsh.last_n_matvec
+    y = torch.empty(n3, dtype=torch.float64, device="cuda")
+    res = []
+    for i in range(4):
+        v = torch.from_numpy(np.ascontiguousarray(vec[:, i])).cuda()
+        op3.matvec_device(v, y)
+        res.append((torch.linalg.vector_norm(y - float(e64[i]) * v) / torch.linalg.vector_norm(v)).item())
+        del v
+    del vec, y
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    e32 = qsgpu.eigsh(op3, k=4, which="SA", tol=1e-10, v0=v0, dtype=np.float32, return_eigenvectors=False)
+    out["seconds_fp32"] = round(time.perf_counter() - t0, 3)
+    out["matvecs_fp32"] = qsgpu.eigsh.last_n_matvec
+    out["E_fp64"] = [float(e) for e in e64]
+    out["E_fp32_stream"] = [float(e) for e in e32]
+    out["abs_err_fp32_vs_fp64"] = [abs(float(a) - float(b)) for a, b in zip(e32, e64)]
+    out["residual_fp64"] = res
+    return out
+
+
 def bench_c5(n_requests, n_workers, assemble=True):
     """Serving form: Poisson stream (mean 200 req/s, seeded) of 60% eigsh k=1..4 / 40% evolve to
     t ~ U[1,10] on XXZ / J1-J2 chains, L in {16,20,24,28}, nup=L/2, random sectors, HBM budget 200 GB.
@@ -189,6 +229,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c3-eigsh", action="store_true")
     ap.add_argument("--no-eigsh", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
@@ -313,6 +354,10 @@ def main():
             torch.cuda.empty_cache()
         out["fp32_over_fp64"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)
         result["c3_matvec"] = out
+        if not args.no_c3_eigsh:
+            result["c3_eigsh"] = bench_c3_eigsh(op3, n3)
+        del op3, b3
+        torch.cuda.empty_cache()
     if not args.no_c4:
         result["c4_block_evolve"] = bench_c4(args.c4_times)
     if not args.no_c5:

@@ -60,6 +60,12 @@ extern "C" {
 #define QS_CHECK_SYMM 1
 #define QS_CHECK_HERM 2
 #define QS_CHECK_PCON 4
+/* Complex element type even when every matrix element is real: the
+ * caller's coefficients or `diagonal` are complex-typed, which makes the
+ * reference's element type T complex (_coefficient_type, operators.jl:329-341;
+ * promote_type with eltype(diagonal), operators.jl:2523-2527), so products
+ * and solvers take complex vectors. */
+#define QS_FORCE_COMPLEX 8
 
 typedef struct qs_basis qs_basis;
 typedef struct qs_op qs_op;
@@ -142,8 +148,13 @@ int qs_op_stream(const qs_op *op, void **stream);
  * with spin inversion only, with a particle number, L <= 32.  Rows sharing
  * their high bits form a contiguous block, so hops among the high bits read
  * whole neighbour blocks and hops among the low bits stay in LDS;
- * QS_VARIANT_SPLIT + k (k = 0, 1, 2) keeps k fewer workgroups per CU
- * resident than fit.  No reference counterpart. */
+ * QS_VARIANT_SPLIT + k (k = 0, 1, 2) is one pass that keeps k fewer
+ * workgroups per CU resident than fit; QS_VARIANT_SPLIT + 3 is the lean
+ * form (neighbour rows from per-class tables, high hops in scalar
+ * registers); QS_VARIANT_SPLIT + 4 + d (d = 0..3) is the lean form in two
+ * passes, each over tiles of blocks whose hops stay inside the tile (at
+ * most 2^(20+d) bytes of x per tile, one tile per XCD at a time), the second
+ * adding to the first's y.  No reference counterpart. */
 #define QS_VARIANT_SPLIT 32
 /* Plain products from the assembled sector matrix (qs_op_assemble).  The
  * reference's counterpart is the assembled Hamiltonian, static_fmt = :csr. */
@@ -175,7 +186,7 @@ int qs_op_assemble(qs_op *op);
  * capacity on entry. */
 int qs_op_csr(const qs_op *op, int64_t *nnz, int64_t *rowptr, int64_t *col, double *val);
 
-/* Autotuner: times every variant for `dtype` on device-resident random/* Autotuner: times every variant for `dtype` on device-resident random
+/* Autotuner: times every variant for `dtype` on device-resident random
  * vectors (`repeats` launches each), keeps the fastest on the operator and
  * returns it with its time.  No reference counterpart. */
 int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *best_ms);

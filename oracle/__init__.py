@@ -5,7 +5,7 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 The product library (``libqsgpu.so``) never loads it.
 
 ``qs_oracle.c`` restates QuSpin.jl's CPU algorithms (cited per function there);
-``krylov.py`` restates the Krylov exponential and Lanczos helpers in numpy.
+``krylov.py`` restates the Krylov exponential action (operators.jl:1655-1893) in numpy.
 """
 import ctypes
 import os
@@ -73,8 +73,32 @@ def lib():
     L.qo_bench_parent_apply.restype = c_dbl
     L.qo_bench_parent_apply_mt.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, c_i64, c_int, c_int]
     L.qo_bench_parent_apply_mt.restype = c_dbl
+    L.qo_prim_hypot.argtypes = [c_dbl, c_dbl]
+    L.qo_prim_hypot.restype = c_dbl
+    L.qo_prim_cdiv.argtypes = [c_dbl, c_dbl, c_dbl, c_dbl, vp]
+    L.qo_prim_cdiv.restype = None
+    L.qo_prim_norm2.argtypes = [vp, c_int]
+    L.qo_prim_norm2.restype = c_dbl
     _lib = L
     return L
+
+
+def prim_hypot(x, y):
+    """The oracle's |z| (Julia Base.Math.hypot)."""
+    return lib().qo_prim_hypot(float(x), float(y))
+
+
+def prim_cdiv(a, b, c, d):
+    """The oracle's complex division (a + ib) / (c + id) (Julia Base, Baudin & Smith 2012)."""
+    out = np.zeros(2)
+    lib().qo_prim_cdiv(float(a), float(b), float(c), float(d), _ptr(out))
+    return complex(out[0], out[1])
+
+
+def prim_norm2(v):
+    """The oracle's LinearAlgebra.generic_norm2 of a complex vector."""
+    z = np.ascontiguousarray(np.asarray(v, dtype=np.complex128))
+    return lib().qo_prim_norm2(_ptr(z.view(np.float64)), z.size)
 
 
 def _ptr(a):

@@ -151,6 +151,17 @@ static double generic_norm2(const cplx *v, int n)
     return sqrt(sum);
 }
 
+/* Test hooks: the floating-point primitives above, exported so the tests
+ * can pin them against exact rational arithmetic. */
+double qo_prim_hypot(double x, double y) { return jl_hypot(x, y); }
+void qo_prim_cdiv(double a, double b, double c, double d, double *out)
+{
+    cplx q = c_div(c_make(a, b), c_make(c, d));
+    out[0] = q.re;
+    out[1] = q.im;
+}
+double qo_prim_norm2(const double *v, int n) { return generic_norm2((const cplx *)v, n); }
+
 /* ---------------------------------------------------------------- errors -- */
 static __thread char g_err[512];
 static void set_err(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }

@@ -18,9 +18,23 @@ namespace qs {
 const std::string &last_error();
 
 OpHost::~OpHost() {
-    xs_buf.release();   // hipFree: waits for the products that read it
-    if (xs_ev) (void)hipEventDestroy(xs_ev);
+    xs_bufs.clear();   // hipFree: waits for the products that read them
     if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+char *OpHost::xs_for(hipStream_t s, size_t bytes) const {
+    std::lock_guard<std::mutex> g(xs_mutex);
+    for (auto &e : xs_bufs) {
+        if (e->s != s) continue;
+        if (e->buf.n < bytes) e->buf.alloc(bytes);
+        return e->buf.get();
+    }
+    auto e = std::make_unique<XsBuf>();
+    e->s = s;
+    e->buf.alloc(bytes);
+    char *p = e->buf.get();
+    xs_bufs.push_back(std::move(e));
+    return p;
 }
 
 namespace {
@@ -643,9 +657,9 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
     return guard([&] {
         check_ptr(op, "op");
         if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 &&
-            lanes_per_row != 8 && (lanes_per_row < QS_VARIANT_SPLIT || lanes_per_row > QS_VARIANT_SPLIT + 2) &&
+            lanes_per_row != 8 && (lanes_per_row < QS_VARIANT_SPLIT || lanes_per_row > QS_VARIANT_SPLIT_MAX) &&
             (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 3) && lanes_per_row != QS_VARIANT_SELL)
-            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0, 1, 2, 3, QS_VARIANT_SPLIT + 0, 1, 2 or "
+            fail(QS_EARG, "variant must be 0, 1, 2, 4, 8, QS_VARIANT_HOP + 0..3, QS_VARIANT_SPLIT + 0..7 or "
                           "QS_VARIANT_SELL");
         if (lanes_per_row == QS_VARIANT_SELL && !op->o.sell) {
             // assembled by the first plain product, on that product's stream
@@ -653,10 +667,14 @@ int qs_op_set_variant(qs_op *op, int lanes_per_row) {
                 fail(QS_EARG, "assembly needs two-bit hop couplings, L <= 32 and a parent lookup table");
             op->o.want_sell = true;
         }
-        if (lanes_per_row >= QS_VARIANT_SPLIT && lanes_per_row <= QS_VARIANT_SPLIT + 2 &&
+        if (lanes_per_row >= QS_VARIANT_SPLIT && lanes_per_row <= QS_VARIANT_SPLIT_MAX &&
             !(split_eligible(op->o, false) && split_eligible(op->o, true)))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
+        if (lanes_per_row >= QS_VARIANT_SPLIT + QS_SPLIT_LEAN && lanes_per_row <= QS_VARIANT_SPLIT_MAX &&
+            !(split_lean_ok(op->o, false) && split_lean_ok(op->o, true)))
+            fail(QS_EARG, "the lean split kernel does not fit this coupling list (at most 32 hop groups inside "
+                          "the low word, 16 across the split and 64 in the high word)");
         if (lanes_per_row >= QS_VARIANT_HOP && lanes_per_row <= QS_VARIANT_HOP + 3 &&
             !(hop_eligible(op->o, false) && hop_eligible(op->o, true)))
             fail(QS_EARG, "the hop matvec variant needs two-bit hop couplings, L <= 32 and a parent lookup table");
@@ -683,7 +701,9 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
         if (hop_eligible(o, false))
             cands.insert(cands.end(), {QS_VARIANT_HOP, QS_VARIANT_HOP + 1, QS_VARIANT_HOP + 2, QS_VARIANT_HOP + 3});
         if (split_eligible(o, false) && split_eligible(o, true))
-            cands.insert(cands.end(), {QS_VARIANT_SPLIT, QS_VARIANT_SPLIT + 1, QS_VARIANT_SPLIT + 2});
+            for (int k = QS_VARIANT_SPLIT; k <= QS_VARIANT_SPLIT_MAX; k++)
+                if (k < QS_VARIANT_SPLIT + QS_SPLIT_LEAN || (split_lean_ok(o, false) && split_lean_ok(o, true)))
+                    cands.push_back(k);
         if (o.sell) cands.push_back(QS_VARIANT_SELL);
         if (o.basis->mode == MODE_STORED) cands.insert(cands.end(), {1, 2, 4, 8});
         else cands.push_back(1);

@@ -221,12 +221,41 @@ struct SplitBasis {
     DevBuf<uint32_t> boff;     // first row of the block of each high word (2^W; ~0 if none)
     DevBuf<uint16_t> rank;     // rank of every H-bit word among the words of its popcount
     DevBuf<uint16_t> words;    // H-bit words by (popcount, value)
+    std::vector<uint16_t> rank_h, words_h;   // host copies (lean-kernel tables)
+    // block orders of the two-pass tiled form, one per (kA, kB), built on
+    // first use: pass A tiles share the high-word bits >= kA, pass B tiles
+    // share the high-word bits < kB (with the inverted tile under spin
+    // inversion); each tile is padded to the same length with 0xFFFFFFFF
+    struct Order {
+        int kA = 0, kB = 0;
+        int CA = 0, CB = 0;                   // entries per tile
+        int nA = 0, nB = 0;                   // list lengths
+        DevBuf<uint32_t> blA, bsA, blB, bsB;  // high word, first row
+    };
+    mutable std::mutex order_mutex;
+    mutable std::vector<std::unique_ptr<Order>> orders;
 };
 // Hop groups of one term list classified for the split kernel: inside the
 // low word, inside the high word, across the split (in this order).
 struct SplitHops {
     int nlow = 0, nhigh = 0, ncross = 0;
     DevBuf<HopDev> d;
+    // the lean kernel's neighbour tables (lean = they fit its limits): per
+    // class position w, the low hops' neighbour rows (u8, 255 = none, stride
+    // sl) and the cross hops' rows in the neighbour block with the word's bit
+    // lo in bit 15 (u16, stride sc)
+    bool lean = false;
+    int sl = 0, sc = 0;
+    DevBuf<uint8_t> lnb;
+    DevBuf<uint16_t> cnb;
+};
+// The same groups dealt to the two passes of the tiled form for one (kA,
+// kB): pass A takes every group whose upper bit lies below H + kA, pass B the
+// rest, which must leave the high-word bits [0, kB) alone.
+struct SplitPassHops {
+    int kA = 0, kB = 0;
+    bool ok = false, ok_adj = false;
+    SplitHops a, b, a_adj, b_adj;
 };
 
 // The assembled sector matrix of an operator (plain products), SELL-64:
@@ -317,6 +346,9 @@ struct OpHost {
     // hop groups classified for the split kernel (built once, on first use)
     mutable std::once_flag split_once;
     mutable SplitHops split_hops, split_hops_adj;
+    // and for the passes of its tiled form, per (kA, kB)
+    mutable std::mutex pass_mutex;
+    mutable std::vector<std::unique_ptr<SplitPassHops>> pass_hops;
     // user diagonal
     std::vector<cd> user_diag;
     bool has_user_diag = false;
@@ -326,11 +358,18 @@ struct OpHost {
     int lanes_per_row = 0;       // matvec kernel variant (0 = default heuristic)
     int graphs = 0;              // replay the expm recursion's factorizations from a HIP graph
     // the hop kernel's pre-scaled x (parent-table sectors), kept between
-    // products; xs_ev marks its last use on xs_stream, so a product on
-    // another stream first waits for it
-    mutable DevBuf<char> xs_buf;
-    mutable hipEvent_t xs_ev = nullptr;
-    mutable hipStream_t xs_stream = nullptr;
+    // products: one buffer per stream, so products of concurrent jobs on
+    // different streams never share one (each stream orders its own writes
+    // and reads).  A buffer only grows; growing it frees the old one with
+    // hipFree, which waits for the products still reading it.
+    struct XsBuf {
+        hipStream_t s;
+        DevBuf<char> buf;
+    };
+    mutable std::mutex xs_mutex;
+    mutable std::vector<std::unique_ptr<XsBuf>> xs_bufs;
+    // the pre-scale buffer of stream s, at least `bytes` long
+    char *xs_for(hipStream_t s, size_t bytes) const;
     // staging buffers for host vectors
     DevBuf<char> stage_x, stage_y;
     // basis without symmetry: the extracted diagonal alone (structurally
@@ -393,6 +432,14 @@ int matvec_grid(int64_t n);
 bool hop_eligible(const OpHost &op, bool adjoint);
 // the split kernel applies (hop couplings, implicit basis with a particle number)
 bool split_eligible(const OpHost &op, bool adjoint);
+bool split_lean_ok(const OpHost &op, bool adjoint);
+// split variants: QS_VARIANT_SPLIT + k, k < QS_SPLIT_LEAN single pass
+// (k fewer resident workgroups), k = QS_SPLIT_LEAN the lean kernel in one
+// pass, k = QS_SPLIT_TILED + d the lean kernel's tiled two-pass form with
+// tile budget 2^(20 + d) bytes
+constexpr int QS_SPLIT_LEAN = 3;
+constexpr int QS_SPLIT_TILED = 4;
+constexpr int QS_VARIANT_SPLIT_MAX = QS_VARIANT_SPLIT + QS_SPLIT_TILED + 3;
 // Assemble the sector matrix of the plain product (hop-eligible operators),
 // and read it back as CSR with duplicate columns merged, in ascending column
 // order (rowptr n+1, col / val nnz; val interleaved complex).

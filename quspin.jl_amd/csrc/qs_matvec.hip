@@ -521,6 +521,13 @@ __device__ __forceinline__ const T *at32(const T *base, uint32_t i) {
     return (const T *)((const char *)base + (uint32_t)(i * (uint32_t)sizeof(T)));
 }
 
+// The element at byte offset `b` from base (b < 2^32): scalar base plus a
+// 32-bit vector offset, as at32 but with the offset already in bytes.
+template <typename T>
+__device__ __forceinline__ const T *atb(const T *base, uint32_t b) {
+    return (const T *)((const char *)base + b);
+}
+
 // FP64 form of a vector type (the pre-scaled x of the parent-table kernel)
 template <typename V> struct Wide { using type = double; };
 template <> struct Wide<cplx_f> { using type = cplx_d; };
@@ -1046,6 +1053,7 @@ struct SplitDev {
     int cls_off[34], cls_n[34];
     int nlow, nhigh, ncross;
     int chunk;                   // blocks per XCD chunk
+    int own;                     // the block's own terms: diagonal, low hops, user diagonal
     int nstage;                  // cross blocks staged in LDS per wavefront
     int xstride;                 // elements per staged block
     int wave_lds;                // bytes of one wavefront's LDS region
@@ -1141,17 +1149,18 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
             hn = D.blist[vb];
             on = D.bstart[vb];
         }
+        if (h == 0xFFFFFFFFu) continue;   // padding of a tile list
         const int p = A.nup - __popc(h);
         const int n = D.cls_n[p];
         const int cw = D.cls_off[p];
-        // own block
+        // own block (only the pass that applies the block's own terms reads it)
         double xo_r[kSpRows], xo_i[kSpRows];
 #pragma unroll
         for (int j = 0; j < kSpRows; j++) {
             const int r = lane + 64 * j;
             xo_r[j] = 0;
             xo_i[j] = 0;
-            if (r < n) T::load(x, (int64_t)off + r, xo_r[j], xo_i[j]);
+            if (D.own && r < n) T::load(x, (int64_t)off + r, xo_r[j], xo_i[j]);
             if (A.conj_x) xo_i[j] = -xo_i[j];
         }
         // this block's entry of the lane's hop group
@@ -1210,10 +1219,12 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
         if (is_high && live) s_list[__popcll(mh & below)] = e;
         if (is_cross && live) s_list[D.nhigh + __popcll(mc & below)] = e;
         const int nh = __popcll(mh), nc = __popcll(mc);
+        if (D.own) {
 #pragma unroll
-        for (int j = 0; j < kSpRows; j++) {
-            const int r = lane + 64 * j;
-            if (r < D.xstride) T::store(s_x, r, xo_r[j], xo_i[j]);   // s_x[n] = 0: dead neighbours read it
+            for (int j = 0; j < kSpRows; j++) {
+                const int r = lane + 64 * j;
+                if (r < D.xstride) T::store(s_x, r, xo_r[j], xo_i[j]);   // s_x[n] = 0: dead neighbours read it
+            }
         }
         wave_lds_sync();
         // One round of loads for the first kNB high hops and the staged cross
@@ -1319,6 +1330,7 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
             if (r >= n) continue;
             const uint32_t l = s_words[cw + r];
             lw[j] = l;
+            if (!D.own) continue;
             double dr = 0, di = 0;
             diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
             di = -di;   // conj(amp): the terms are the adjoint couplings
@@ -1373,6 +1385,305 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split(MVArgs A, SplitDev D,
         for (int j = 0; j < kSpRows; j++) {
             const int r = lane + 64 * j;
             if (r < n) finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
+        }
+        wave_lds_sync();   // the block's LDS reads are done before the next block's writes
+    }
+}
+
+// ------------------------------------------------- lean split kernel --
+// The same block decomposition with the per-row bookkeeping moved into
+// tables and scalar registers:
+//  * the low hops' neighbour rows come from a per-basis, per-coupling-list
+//    LDS table indexed by the row's position in its popcount class (one byte
+//    per hop, 255 = no neighbour: row 255 of the staged block is always 0);
+//  * the cross hops' rows in the neighbour block come from a second table
+//    (the row's bit lo in bit 15), liveness from that bit and h's bit hi;
+//  * the high hops are found by one ballot over lanes that hold one hop each;
+//    each live hop's neighbour-block offset and weight move to scalar
+//    registers (readlane), so a neighbour row is one load with a scalar base
+//    and the lane's fixed byte offset, and one FMA;
+//  * row slots past the block's last row are skipped by wave-uniform branches.
+// All high-hop loads of a batch are issued before the own block's diagonal
+// and low hops are computed, so their latency overlaps that arithmetic.
+constexpr int kS2MaxLow = 32;    // low hop groups
+constexpr int kS2MaxCross = 16;  // cross hop groups
+
+struct Split2Dev {
+    int H;
+    uint32_t lmask, wmask;
+    const uint32_t *blist, *bstart;
+    int nblocks, chunk, own;
+    const uint32_t *boff;
+    int cls_off[34], cls_n[34];
+    int nlow, nhigh, ncross;
+    int sl, sc;              // table strides: nlow rounded up to 4, ncross rounded up to 2
+    const uint16_t *words;   // H-bit words by (popcount, value)
+    const uint8_t *lnb;      // [w * sl + g]: low hop g's neighbour row of class position w (255: none)
+    const uint16_t *cnb;     // [w * sc + c]: cross hop c's row in the neighbour block | bit lo << 15
+    int off_words, off_lnb, off_cnb, off_wave, wave_bytes;
+};
+
+template <typename V, bool CC, int CM>
+__global__ __launch_bounds__(kSpBlock) void k_matvec_split2(MVArgs A, Split2Dev D, const HopDev *__restrict__ hops,
+                                                            const DiagClass *__restrict__ dcls,
+                                                            const TermDev *__restrict__ drest,
+                                                            const V *__restrict__ x, V *__restrict__ y) {
+    using T = VTraits<V>;
+    constexpr int kNB = sizeof(V) <= 4 ? 8 : (sizeof(V) <= 8 ? 6 : 4);
+    constexpr uint32_t ES = sizeof(V);
+    extern __shared__ __align__(16) char smem[];
+    uint16_t *s_words = (uint16_t *)(smem + D.off_words);
+    uint8_t *s_lnb = (uint8_t *)(smem + D.off_lnb);
+    uint16_t *s_cnb = (uint16_t *)(smem + D.off_cnb);
+    const int nw = 1 << D.H;
+    for (int i = threadIdx.x; i < nw; i += kSpBlock) s_words[i] = D.words[i];
+    if (D.own && D.nlow)
+        for (int i = threadIdx.x; i < nw * D.sl / 4; i += kSpBlock)
+            ((uint32_t *)s_lnb)[i] = ((const uint32_t *)D.lnb)[i];
+    if (D.ncross)
+        for (int i = threadIdx.x; i < nw * D.sc / 2; i += kSpBlock)
+            ((uint32_t *)s_cnb)[i] = ((const uint32_t *)D.cnb)[i];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    V *s_x = (V *)(smem + D.off_wave + wave * D.wave_bytes);
+    if (lane == 0) T::store(s_x, 255, 0.0, 0.0);   // the row dead low-hop neighbours read
+    __syncthreads();
+    // lane k < nhigh holds high hop group k
+    const bool is_high = lane < D.nhigh;
+    int g_lo = 0, g_hi = 0, g_nz = 0, g_sym = 0, g_top = 0;
+    double g_re1 = 0, g_re2 = 0, g_im1 = 0, g_im2 = 0;
+    if (is_high) {
+        const HopDev &hg = hops[D.nlow + lane];
+        g_lo = hg.lo - D.H;
+        g_hi = hg.hi - D.H;
+        g_nz = hg.nzmask;
+        g_sym = hg.sym;
+        g_top = hg.top;
+        g_re1 = hg.re[1];
+        g_re2 = hg.re[2];
+        g_im1 = hg.im[1];
+        g_im2 = hg.im[2];
+    }
+    double dot_r = 0, dot_i = 0;
+    const int xcd = blockIdx.x & 7;
+    const int nwx = (int)(gridDim.x >> 3) * (kSpBlock / 64);
+    const int64_t C = D.chunk;
+    auto block_of = [&](int64_t q) { return (8 * (q / C) + xcd) * C + q % C; };
+    int64_t q = (int64_t)(blockIdx.x >> 3) * (kSpBlock / 64) + wave;
+    int64_t vb = block_of(q);
+    uint32_t hn = 0, on = 0;
+    if (vb < D.nblocks) {
+        hn = D.blist[vb];
+        on = D.bstart[vb];
+    }
+    while (vb < D.nblocks) {
+        const uint32_t h = __builtin_amdgcn_readfirstlane(hn), off = __builtin_amdgcn_readfirstlane(on);
+        q += nwx;
+        vb = block_of(q);
+        if (vb < D.nblocks) {   // the next block's header, in flight meanwhile
+            hn = D.blist[vb];
+            on = D.bstart[vb];
+        }
+        if (h == 0xFFFFFFFFu) continue;   // padding of a tile list
+        const int p = A.nup - __popc(h);
+        const int n = D.cls_n[p];
+        const int cw = D.cls_off[p];
+        const int ns = (n + 63) >> 6;       // row slots in use (wave-uniform)
+        // this block's high hops: liveness by ballot, neighbour offset and
+        // weight per lane, moved to scalar registers below
+        bool live = false;
+        uint32_t e_off = 0;
+        double e_re = 0, e_im = 0;
+        int e_rev = 0;
+        if (is_high) {
+            const uint32_t blo = (h >> g_lo) & 1, bhi = (h >> g_hi) & 1;
+            live = (g_nz >> (blo | (bhi << 1))) & 1;
+            e_re = (g_sym || blo) ? g_re1 : g_re2;
+            e_im = CC ? -((g_sym || blo) ? g_im1 : g_im2) : 0.0;   // conj(amp): adjoint couplings
+            uint32_t h2 = h ^ ((1u << g_lo) | (1u << g_hi));
+            if (CM == CM_ZONLY && g_top) {
+                // the hop sets bit L-1: the inverted state's column, rows reversed
+                h2 = ~h2 & D.wmask;
+                e_rev = 1;
+                if (A.zchar < 0) {
+                    e_re = -e_re;
+                    e_im = -e_im;
+                }
+            }
+            e_off = live ? D.boff[h2] : 0u;
+        }
+        uint64_t lm = __ballot(live);
+        // per-slot byte offsets of the lane's rows (and of the reversed rows)
+        uint32_t roff[kSpRows], rrev[kSpRows];
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            const int r = lane + 64 * j;
+            roff[j] = (uint32_t)r * ES;
+            rrev[j] = (uint32_t)(n - 1 - r) * ES;
+        }
+        // own rows: loads issued now, consumed after the first high batch's loads
+        V xo[kSpRows];
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            xo[j] = V{};
+            if (D.own && j < ns && lane + 64 * j < n) xo[j] = *atb(x + off, roff[j]);
+        }
+        double ar[kSpRows], ai[kSpRows];
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            ar[j] = 0;
+            ai[j] = 0;
+        }
+        bool first = true;
+        do {
+            // one batch of up to kNB live high hops: scalar offsets and weights
+            uint32_t b_off[kNB];
+            double b_re[kNB], b_im[kNB];
+            int b_rev[kNB];
+            int nb = 0;
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk++) {
+                b_off[kk] = 0;
+                b_re[kk] = 0;
+                b_im[kk] = 0;
+                b_rev[kk] = 0;
+                if (lm) {
+                    const int k = __builtin_ctzll(lm);
+                    lm &= lm - 1;
+                    b_off[kk] = __builtin_amdgcn_readlane(e_off, k);
+                    b_rev[kk] = __builtin_amdgcn_readlane(e_rev, k);
+                    {
+                        const uint64_t bits = __builtin_bit_cast(uint64_t, e_re);
+                        const uint32_t lo32 = __builtin_amdgcn_readlane((uint32_t)bits, k);
+                        const uint32_t hi32 = __builtin_amdgcn_readlane((uint32_t)(bits >> 32), k);
+                        b_re[kk] = __builtin_bit_cast(double, ((uint64_t)hi32 << 32) | lo32);
+                    }
+                    if (CC) {
+                        const uint64_t bits = __builtin_bit_cast(uint64_t, e_im);
+                        const uint32_t lo32 = __builtin_amdgcn_readlane((uint32_t)bits, k);
+                        const uint32_t hi32 = __builtin_amdgcn_readlane((uint32_t)(bits >> 32), k);
+                        b_im[kk] = __builtin_bit_cast(double, ((uint64_t)hi32 << 32) | lo32);
+                    }
+                    nb = kk + 1;
+                }
+            }
+            V v[kNB][kSpRows];
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk++) {
+#pragma unroll
+                for (int j = 0; j < kSpRows; j++) {
+                    v[kk][j] = V{};
+                    if (kk < nb && j < ns && lane + 64 * j < n)
+                        v[kk][j] = *atb(x + b_off[kk], b_rev[kk] ? rrev[j] : roff[j]);
+                }
+            }
+            if (first) {
+                first = false;
+                if (D.own) {
+                    // own block: stage in LDS, diagonal, low hops
+#pragma unroll
+                    for (int j = 0; j < kSpRows; j++)
+                        if (j < ns && lane + 64 * j < n) s_x[lane + 64 * j] = xo[j];
+                    wave_lds_sync();
+#pragma unroll
+                    for (int j = 0; j < kSpRows; j++) {
+                        const int r = lane + 64 * j;
+                        if (j < ns && r < n) {
+                            double xr, xi;
+                            T::load(&xo[j], 0, xr, xi);
+                            if (A.conj_x) xi = -xi;
+                            const uint32_t l = s_words[cw + r];
+                            double dr = 0, di = 0;
+                            diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
+                            di = -di;   // conj(amp): the terms are the adjoint couplings
+                            if (A.user_diag) {
+                                const int64_t row = (int64_t)off + r;
+                                dr += A.user_diag[2 * row];
+                                di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+                            }
+                            acc_nb<V>(dr, di, xr, xi, ar[j], ai[j]);
+                        }
+                    }
+#pragma unroll 4
+                    for (int g = 0; g < D.nlow; g++) {
+                        const HopDev &hp = hops[g];
+                        const double r1 = hp.re[1], r2 = hp.re[2];
+                        const double i1 = CC ? hp.im[1] : 0.0, i2 = CC ? hp.im[2] : 0.0;
+                        const int sym = hp.sym, lo = hp.lo;
+#pragma unroll
+                        for (int j = 0; j < kSpRows; j++) {
+                            const int r = lane + 64 * j;
+                            if (j < ns && r < n) {
+                                const uint32_t nid = s_lnb[(cw + r) * D.sl + g];
+                                double vr, vi;
+                                T::load(s_x, nid, vr, vi);
+                                if (A.conj_x) vi = -vi;
+                                double mr = r1, mi = i1;
+                                if (!sym && !((s_words[cw + r] >> lo) & 1)) {
+                                    mr = r2;
+                                    mi = i2;
+                                }
+                                acc_nb<V>(mr, -mi, vr, vi, ar[j], ai[j]);
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk++) {
+                if (kk < nb) {
+#pragma unroll
+                    for (int j = 0; j < kSpRows; j++) {
+                        if (j < ns) {
+                            double vr, vi;
+                            T::load(&v[kk][j], 0, vr, vi);
+                            acc_nb<V>(b_re[kk], b_im[kk], vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+                        }
+                    }
+                }
+            }
+        } while (lm);
+        // cross hops: the neighbour block's rows from the table, gathered
+        for (int c = 0; c < D.ncross; c++) {
+            const HopDev &hc = hops[D.nlow + D.nhigh + c];
+            const int hi = hc.hi - D.H;
+            const uint32_t bhi = (h >> hi) & 1;
+            double m_r = bhi ? hc.re[2] : hc.re[1];
+            double m_i = CC ? (bhi ? hc.im[2] : hc.im[1]) : 0.0;
+            uint32_t h2 = h ^ (1u << hi);
+            if (CM == CM_ZONLY && hc.top) {
+                h2 = ~h2 & D.wmask;
+                if (A.zchar < 0) {
+                    m_r = -m_r;
+                    m_i = -m_i;
+                }
+            }
+            const int p2 = A.nup - __popc(h2);
+            if (!((hc.nzmask >> (bhi ? 2 : 1)) & 1) || p2 < 0 || p2 > D.H) continue;
+            const uint32_t o2 = D.boff[h2];
+            if (o2 == 0xFFFFFFFFu) continue;
+            V u[kSpRows];
+#pragma unroll
+            for (int j = 0; j < kSpRows; j++) {
+                const int r = lane + 64 * j;
+                u[j] = V{};
+                if (j < ns && r < n) {
+                    const uint32_t e = s_cnb[(cw + r) * D.sc + c];
+                    if ((e >> 15) != bhi) u[j] = x[(int64_t)o2 + (e & 0x7FFFu)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kSpRows; j++) {
+                if (j < ns) {
+                    double vr, vi;
+                    T::load(&u[j], 0, vr, vi);
+                    acc_nb<V>(m_r, -m_i, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kSpRows; j++) {
+            const int r = lane + 64 * j;
+            if (j < ns && r < n) finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
         }
         wave_lds_sync();   // the block's LDS reads are done before the next block's writes
     }
@@ -1434,26 +1745,24 @@ void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x,
 #undef QS_HOP_WS
 }
 
-// ps in the parent-table mode: x * (1/sqrt(D)) is written once to the
-// operator's FP64 buffer (kept between products: a stream-ordered
-// allocation per product cost more host time than the product itself) and
-// the hop kernel gathers from it
+// ps in the parent-table mode: x * (1/sqrt(D)) is written once to an FP64
+// buffer the operator keeps for the launching stream (kept between products:
+// a stream-ordered allocation per product cost more host time than the
+// product itself) and the hop kernel gathers from it.  The buffer belongs to
+// the stream, so the pre-scale pass and the gathers of one product are
+// ordered by that stream alone and products on other streams use their own.
 template <typename V, bool CC>
 void hop_cm(const MVArgs &a, int cm, bool track, int ws, bool ps, const void *x, void *y, const Launch &L,
             const HopLaunch &H, const OpHost &op) {
     if (ps && cm == CM_PTAB) {
         using W = typename Wide<V>::type;
         const int64_t n = a.N;
-        const size_t bytes = sizeof(W) * (size_t)std::max<int64_t>(n, 1);
-        if (op.xs_buf.n < bytes) op.xs_buf.alloc(bytes);   // hipFree of the old one waits for its readers
-        if (!op.xs_ev) QS_HIP(hipEventCreateWithFlags(&op.xs_ev, hipEventDisableTiming));
-        if (op.xs_stream && op.xs_stream != L.s) QS_HIP(hipStreamWaitEvent(L.s, op.xs_ev, 0));
-        W *xs = (W *)op.xs_buf.get();
+        // sized for the widest element so one buffer serves every dtype
+        const size_t bytes = sizeof(cplx_d) * (size_t)std::max<int64_t>(n, 1);
+        W *xs = (W *)op.xs_for(L.s, bytes);
         const int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
         hipLaunchKernelGGL((k_prescale<V>), dim3(std::max(g, 1)), dim3(256), 0, L.s, (const V *)x, a.inv_nrm, xs, n);
         dispatch_hop_cm<V, CC>(a, cm, track, ws, x, xs, y, L, H);
-        QS_HIP(hipEventRecord(op.xs_ev, L.s));
-        op.xs_stream = L.s;
         return;
     }
     dispatch_hop_cm<V, CC>(a, cm, track, ws, x, nullptr, y, L, H);
@@ -1686,21 +1995,72 @@ const SplitBasis &split_basis(const BasisHost &b) {
         QS_HIP(hipMemcpy(t->boff.get(), boff.data(), boff.size() * 4, hipMemcpyHostToDevice));
         QS_HIP(hipMemcpy(t->rank.get(), rank.data(), rank.size() * 2, hipMemcpyHostToDevice));
         QS_HIP(hipMemcpy(t->words.get(), words.data(), words.size() * 2, hipMemcpyHostToDevice));
+        t->rank_h = std::move(rank);
+        t->words_h = std::move(words);
         b.split = std::move(t);
     });
     return *b.split;
 }
 
+// The lean kernel's tables for hop list `o` (low groups first, cross
+// groups last) on basis tables t.
+void lean_tables(SplitHops &o, const std::vector<HopDev> &low, const std::vector<HopDev> &cross,
+                 const SplitBasis &t, bool zonly) {
+    o.lean = o.nlow <= kS2MaxLow && o.ncross <= kS2MaxCross && o.nhigh <= 64 && t.maxblock <= 255;
+    if (!o.lean) return;
+    const int nw = 1 << t.H;
+    const uint32_t lmask = (1u << t.H) - 1;
+    o.sl = std::max(4, (o.nlow + 3) & ~3);
+    o.sc = std::max(2, (o.ncross + 1) & ~1);
+    std::vector<uint8_t> lnb((size_t)nw * o.sl, 255);
+    std::vector<uint16_t> cnb((size_t)nw * o.sc, 0);
+    for (int w = 0; w < nw; w++) {
+        const uint32_t l = t.words_h[w];
+        for (int g = 0; g < o.nlow; g++) {
+            const HopDev &h = low[g];
+            const uint32_t blo = (l >> h.lo) & 1, bhi = (l >> h.hi) & 1;
+            if ((h.nzmask >> (blo | (bhi << 1))) & 1)
+                lnb[(size_t)w * o.sl + g] = (uint8_t)t.rank_h[l ^ ((1u << h.lo) | (1u << h.hi))];
+        }
+        for (int c = 0; c < o.ncross; c++) {
+            const HopDev &h = cross[c];
+            uint32_t l2 = l ^ (1u << h.lo);
+            if (zonly && h.top) l2 = ~l2 & lmask;
+            cnb[(size_t)w * o.sc + c] = (uint16_t)(t.rank_h[l2] | (((l >> h.lo) & 1) << 15));
+        }
+    }
+    o.lnb.alloc(lnb.size());
+    QS_HIP(hipMemcpy(o.lnb.get(), lnb.data(), lnb.size(), hipMemcpyHostToDevice));
+    o.cnb.alloc(cnb.size());
+    QS_HIP(hipMemcpy(o.cnb.get(), cnb.data(), cnb.size() * 2, hipMemcpyHostToDevice));
+}
+
+void split_classify(const OpHost &op);
+
+}  // namespace
+
+// the lean split kernel applies (split kernel conditions, and the hop lists
+// fit its tables)
+bool split_lean_ok(const OpHost &op, bool adjoint) {
+    if (!split_eligible(op, adjoint)) return false;
+    split_classify(op);
+    return (adjoint ? op.split_hops_adj : op.split_hops).lean;
+}
+
+namespace {
+
 // hop groups of the operator classified by where they sit (once per operator)
 void split_classify(const OpHost &op) {
     std::call_once(op.split_once, [&] {
-        const int H = split_basis(*op.basis).H;
+        const SplitBasis &t = split_basis(*op.basis);
+        const int H = t.H;
         auto build = [&](const FastTerms &f, SplitHops &out) {
             std::vector<HopDev> low, high, cross;
             for (const HopDev &h : f.hops) (h.hi < H ? low : (h.lo >= H ? high : cross)).push_back(h);
             out.nlow = (int)low.size();
             out.nhigh = (int)high.size();
             out.ncross = (int)cross.size();
+            lean_tables(out, low, cross, t, op.basis->mode == MODE_ZONLY);
             low.insert(low.end(), high.begin(), high.end());
             low.insert(low.end(), cross.begin(), cross.end());
             if (!low.empty()) {
@@ -1715,7 +2075,7 @@ void split_classify(const OpHost &op) {
 
 template <typename V, bool CC>
 void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, int fewer, size_t lds, hipStream_t s,
-                       const HopDev *hops, const HopLaunch &H, const void *x, void *y) {
+                       const HopDev *hops, const HopLaunch &H, const void *x, void *y, int chunk) {
     const V *xv = (const V *)x;
     V *yv = (V *)y;
     auto go = [&](auto kern) {
@@ -1733,77 +2093,310 @@ void dispatch_split_cm(const MVArgs &a, const SplitDev &D, int cm, int fewer, si
         const int64_t want = (int64_t)std::max(1, occ - fewer) * cus;
         const int64_t need = ((int64_t)D.nblocks + kSpBlock / 64 - 1) / (kSpBlock / 64);
         const int grid = (int)std::max<int64_t>(8, std::min(want, need) / 8 * 8);
-        // four blocks per wavefront per chunk
         SplitDev Dc = D;
-        Dc.chunk = std::max(1, (grid / 8) * (kSpBlock / 64) * 4);
+        // a tiled order brings its tile length; else four blocks per
+        // wavefront per chunk
+        Dc.chunk = chunk > 0 ? chunk : std::max(1, (grid / 8) * (kSpBlock / 64) * 4);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kSpBlock), lds, s, a, Dc, hops, H.dcls, H.drest, xv, yv);
     };
     if (cm == CM_ZONLY) go(k_matvec_split<V, CC, CM_ZONLY>);
     else go(k_matvec_split<V, CC, CM_FULL>);
 }
 
+// Block orders of the tiled form (see SplitBasis::Order): pass A deals out
+// runs of 2^kA consecutive high words, pass B the high words sharing their
+// bits below kB (and, under spin inversion, the tile of the inverted bits,
+// where the hops that set bit L-1 land).  Tile c goes to XCD c % 8.
+const SplitBasis::Order &split_order(const BasisHost &b, const SplitBasis &t, int kA, int kB) {
+    std::lock_guard<std::mutex> g(t.order_mutex);
+    for (auto &o : t.orders)
+        if (o->kA == kA && o->kB == kB) return *o;
+    auto o = std::make_unique<SplitBasis::Order>();
+    o->kA = kA;
+    o->kB = kB;
+    const int W = t.W;
+    std::vector<uint32_t> blist(t.nblocks), bstart(t.nblocks);
+    QS_HIP(hipMemcpy(blist.data(), t.blist.get(), blist.size() * 4, hipMemcpyDeviceToHost));
+    QS_HIP(hipMemcpy(bstart.data(), t.bstart.get(), bstart.size() * 4, hipMemcpyDeviceToHost));
+    auto emit = [](const std::vector<std::vector<int>> &tiles, const std::vector<uint32_t> &bl,
+                   const std::vector<uint32_t> &bs, int &C, int &n, DevBuf<uint32_t> &dl, DevBuf<uint32_t> &ds) {
+        C = 1;
+        for (auto &v : tiles) C = std::max<int>(C, (int)v.size());
+        std::vector<uint32_t> l, st;
+        for (auto &v : tiles) {
+            if (v.empty()) continue;
+            for (int i : v) {
+                l.push_back(bl[i]);
+                st.push_back(bs[i]);
+            }
+            for (size_t k = v.size(); k < (size_t)C; k++) {
+                l.push_back(0xFFFFFFFFu);
+                st.push_back(0);
+            }
+        }
+        n = (int)l.size();
+        dl.alloc(l.size());
+        ds.alloc(st.size());
+        QS_HIP(hipMemcpy(dl.get(), l.data(), l.size() * 4, hipMemcpyHostToDevice));
+        QS_HIP(hipMemcpy(ds.get(), st.data(), st.size() * 4, hipMemcpyHostToDevice));
+    };
+    // pass A: consecutive runs (the block list is ascending in h)
+    std::vector<std::vector<int>> ta((size_t)1 << (W - kA));
+    for (int i = 0; i < t.nblocks; i++) ta[blist[i] >> kA].push_back(i);
+    emit(ta, blist, bstart, o->CA, o->nA, o->blA, o->bsA);
+    // pass B: equal low bits m (paired with the inverted bits under spin inversion)
+    const uint32_t mm = (1u << kB) - 1;
+    std::vector<std::vector<int>> bym((size_t)1 << kB);
+    for (int i = 0; i < t.nblocks; i++) bym[blist[i] & mm].push_back(i);
+    std::vector<std::vector<int>> tb;
+    for (uint32_t m = 0; m <= mm; m++) {
+        if (b.mode == MODE_ZONLY) {
+            const uint32_t m2 = ~m & mm;
+            if (m2 < m) continue;
+            std::vector<int> v = bym[m];
+            if (m2 != m) v.insert(v.end(), bym[m2].begin(), bym[m2].end());
+            tb.push_back(std::move(v));
+        } else {
+            tb.push_back(bym[m]);
+        }
+    }
+    emit(tb, blist, bstart, o->CB, o->nB, o->blB, o->bsB);
+    t.orders.push_back(std::move(o));
+    return *t.orders.back();
+}
+
+// Hop groups dealt to the two passes for (kA, kB); ok = every group found a pass.
+const SplitPassHops &split_pass_hops(const OpHost &op, const SplitBasis &t, int kA, int kB) {
+    const int H = t.H;
+    std::lock_guard<std::mutex> g(op.pass_mutex);
+    for (auto &p : op.pass_hops)
+        if (p->kA == kA && p->kB == kB) return *p;
+    auto p = std::make_unique<SplitPassHops>();
+    p->kA = kA;
+    p->kB = kB;
+    auto build = [&](const FastTerms &f, SplitHops &A, SplitHops &B) {
+        std::vector<HopDev> lowA, highA, crossA, highB, crossB;
+        for (const HopDev &h : f.hops) {
+            if (h.hi < H + kA) {
+                (h.hi < H ? lowA : (h.lo >= H ? highA : crossA)).push_back(h);
+            } else if (h.lo < H) {
+                crossB.push_back(h);
+            } else if (h.lo >= H + kB) {
+                highB.push_back(h);
+            } else {
+                return false;
+            }
+        }
+        auto put = [&](SplitHops &o, std::vector<HopDev> low, const std::vector<HopDev> &high,
+                       const std::vector<HopDev> &cross) {
+            o.nlow = (int)low.size();
+            o.nhigh = (int)high.size();
+            o.ncross = (int)cross.size();
+            lean_tables(o, low, cross, t, op.basis->mode == MODE_ZONLY);
+            low.insert(low.end(), high.begin(), high.end());
+            low.insert(low.end(), cross.begin(), cross.end());
+            if (!low.empty()) {
+                o.d.alloc(low.size());
+                QS_HIP(hipMemcpy(o.d.get(), low.data(), low.size() * sizeof(HopDev), hipMemcpyHostToDevice));
+            }
+        };
+        put(A, lowA, highA, crossA);
+        put(B, {}, highB, crossB);
+        return A.lean && B.lean;
+    };
+    p->ok = build(op.fast, p->a, p->b);
+    p->ok_adj = build(op.fast_adj, p->a_adj, p->b_adj);
+    op.pass_hops.push_back(std::move(p));
+    return *op.pass_hops.back();
+}
+
+// (kA, kB) of the tiled form for tile budget index d: pass A tiles of at
+// most 2^(20 + d) bytes of x (1, 2, 4 or 8 MB for d = 0..3, against one
+// XCD's 4 MB L2, which also sees y), pass B split as far up as the couplings
+// allow.  False when no split fits.
+bool split_tiles(const OpHost &op, const SplitBasis &t, size_t esz, int d, int &kA, int &kB) {
+    const BasisHost &b = *op.basis;
+    const int W = t.W, H = t.H;
+    const double rows = (double)b.Ns / std::max(1, t.nblocks);
+    const double budget = std::ldexp(1.0, 20 + d);
+    kA = 1;
+    while (kA + 1 < W && std::ldexp(rows * (double)esz, kA + 1) <= budget) kA++;
+    // pass B may start at the lowest high-word bit of a group pass A leaves
+    kB = kA;
+    for (const FastTerms *f : {&op.fast, &op.fast_adj})
+        for (const HopDev &h : f->hops)
+            if (h.hi >= H + kA && h.lo >= H) kB = std::min(kB, h.lo - H);
+    return kB >= 1 && kA < W;
+}
+
+template <typename V, bool CC>
+void dispatch_split2_cm(const MVArgs &a, const Split2Dev &D, int cm, size_t lds, hipStream_t s, const HopDev *hops,
+                        const HopLaunch &H, const void *x, void *y, int chunk) {
+    auto go = [&](auto kern) {
+        static int cus = 0;
+        if (!cus) {
+            int dev = 0;
+            QS_HIP(hipGetDevice(&dev));
+            QS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        }
+        int occ = 0;
+        QS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kSpBlock, lds));
+        const int64_t want = (int64_t)std::max(1, occ) * cus;
+        const int64_t need = ((int64_t)D.nblocks + kSpBlock / 64 - 1) / (kSpBlock / 64);
+        const int grid = (int)std::max<int64_t>(8, std::min(want, need) / 8 * 8);
+        Split2Dev Dc = D;
+        Dc.chunk = chunk > 0 ? chunk : std::max(1, (grid / 8) * (kSpBlock / 64) * 4);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kSpBlock), lds, s, a, Dc, hops, H.dcls, H.drest, (const V *)x,
+                           (V *)y);
+    };
+    if (cm == CM_ZONLY) go(k_matvec_split2<V, CC, CM_ZONLY>);
+    else go(k_matvec_split2<V, CC, CM_FULL>);
+}
+
+// One product of the split kernel: the first kernel in one pass (fewer =
+// 0..2 resident workgroups held back), or the lean kernel in one pass
+// (tiled < 0) or in the two-pass tiled form with tile budget index `tiled`.
 void launch_split(const OpHost &op, MVArgs &a, int dtype, bool cc, bool adjoint, const void *x, void *y,
-                  hipStream_t s) {
+                  hipStream_t s, int fewer, int tiled, bool lean) {
     const BasisHost &b = *op.basis;
     const SplitBasis &t = split_basis(b);
     split_classify(op);
-    const SplitHops &sh = adjoint ? op.split_hops_adj : op.split_hops;
-    SplitDev D;
-    memset(&D, 0, sizeof D);
-    D.H = t.H;
-    D.lmask = (1u << t.H) - 1;
-    D.wmask = t.W >= 32 ? 0xFFFFFFFFu : ((1u << t.W) - 1);
-    D.blist = t.blist.get();
-    D.nblocks = t.nblocks;
-    D.boff = t.boff.get();
-    D.rank = t.rank.get();
-    D.words = t.words.get();
-    for (int p = 0; p <= t.H; p++) {
-        D.cls_off[p] = t.cls_off[p];
-        D.cls_n[p] = t.cls_n[p];
-    }
-    D.nlow = sh.nlow;
-    D.nhigh = sh.nhigh;
-    D.ncross = sh.ncross;
-    a.ndcls = (int)(adjoint ? op.fast_adj : op.fast).dcls.size();
-    a.ndrest = (int)(adjoint ? op.fast_adj : op.fast).drest.size();
     const size_t nw = (size_t)1 << t.H;
     const size_t esz = dtype_size(dtype);
-    if (sh.nhigh + sh.ncross > kSpMaxHops) fail(QS_EARG, "internal: too many hop groups for the split kernel");
     if (t.maxblock > 64 * kSpRows) fail(QS_EARG, "internal: split block larger than the wavefront's rows");
-    D.bstart = t.bstart.get();
-    D.xstride = t.maxblock + 1;
-    // LDS: shared rank and word tables; per wavefront the hop list, the own
-    // block and as many staged cross blocks as fit a 12 KB region
-    const size_t list_bytes = ((sizeof(SpHop) * (sh.nhigh + sh.ncross)) + 15) & ~(size_t)15;
-    const size_t blk = (size_t)D.xstride * esz;
-    const size_t region = 12 * 1024;
-    const int fit = (int)(region > list_bytes + blk ? (region - list_bytes - blk) / blk : 0);
-    D.nstage = std::min(sh.ncross, fit);
-    D.off_rank = 0;
-    D.off_words = (int)(nw * 2);
-    D.off_wave = (int)((nw * 4 + 15) & ~(size_t)15);
-    D.off_xw = (int)list_bytes;
-    D.wave_lds = (int)((list_bytes + (size_t)(1 + D.nstage) * blk + 15) & ~(size_t)15);
-    const size_t lds = (size_t)D.off_wave + (size_t)(kSpBlock / 64) * D.wave_lds;
     const int cm = b.mode == MODE_ZONLY ? CM_ZONLY : CM_FULL;
-    const int fewer = std::max(0, op.lanes_per_row - QS_VARIANT_SPLIT);
-    HopLaunch H{nullptr, adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
-                adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
-    const HopDev *hops = sh.d.get();
-    switch (dtype) {
-    case QS_F64: dispatch_split_cm<double, false>(a, D, cm, fewer, lds, s, hops, H, x, y); break;
-    case QS_F32: dispatch_split_cm<float, false>(a, D, cm, fewer, lds, s, hops, H, x, y); break;
-    case QS_C128:
-        if (cc) dispatch_split_cm<cplx_d, true>(a, D, cm, fewer, lds, s, hops, H, x, y);
-        else dispatch_split_cm<cplx_d, false>(a, D, cm, fewer, lds, s, hops, H, x, y);
-        break;
-    case QS_C64:
-        if (cc) dispatch_split_cm<cplx_f, true>(a, D, cm, fewer, lds, s, hops, H, x, y);
-        else dispatch_split_cm<cplx_f, false>(a, D, cm, fewer, lds, s, hops, H, x, y);
-        break;
-    default: fail(QS_EARG, "unknown dtype");
+    HopLaunch HL{nullptr, adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
+                 adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
+    a.ndcls = (int)(adjoint ? op.fast_adj : op.fast).dcls.size();
+    a.ndrest = (int)(adjoint ? op.fast_adj : op.fast).drest.size();
+    // the first kernel: one launch over the ascending block list
+    auto pass1 = [&](const SplitHops &sh) {
+        if (sh.nhigh + sh.ncross > kSpMaxHops) fail(QS_EARG, "internal: too many hop groups for the split kernel");
+        SplitDev D;
+        memset(&D, 0, sizeof D);
+        D.H = t.H;
+        D.lmask = (1u << t.H) - 1;
+        D.wmask = t.W >= 32 ? 0xFFFFFFFFu : ((1u << t.W) - 1);
+        D.blist = t.blist.get();
+        D.bstart = t.bstart.get();
+        D.nblocks = t.nblocks;
+        D.boff = t.boff.get();
+        D.rank = t.rank.get();
+        D.words = t.words.get();
+        for (int p = 0; p <= t.H; p++) {
+            D.cls_off[p] = t.cls_off[p];
+            D.cls_n[p] = t.cls_n[p];
+        }
+        D.nlow = sh.nlow;
+        D.nhigh = sh.nhigh;
+        D.ncross = sh.ncross;
+        D.own = 1;
+        D.xstride = t.maxblock + 1;
+        // LDS: shared rank and word tables; per wavefront the hop list, the own
+        // block and as many staged cross blocks as fit a 12 KB region
+        const size_t list_bytes = ((sizeof(SpHop) * (sh.nhigh + sh.ncross)) + 15) & ~(size_t)15;
+        const size_t blk = (size_t)D.xstride * esz;
+        const size_t region = 12 * 1024;
+        const int fit = (int)(region > list_bytes + blk ? (region - list_bytes - blk) / blk : 0);
+        D.nstage = std::min(sh.ncross, fit);
+        D.off_rank = 0;
+        D.off_words = (int)(nw * 2);
+        D.off_wave = (int)((nw * 4 + 15) & ~(size_t)15);
+        D.off_xw = (int)list_bytes;
+        D.wave_lds = (int)((list_bytes + (size_t)(1 + D.nstage) * blk + 15) & ~(size_t)15);
+        const size_t lds = (size_t)D.off_wave + (size_t)(kSpBlock / 64) * D.wave_lds;
+        const HopDev *hops = sh.d.get();
+        switch (dtype) {
+        case QS_F64: dispatch_split_cm<double, false>(a, D, cm, fewer, lds, s, hops, HL, x, y, 0); break;
+        case QS_F32: dispatch_split_cm<float, false>(a, D, cm, fewer, lds, s, hops, HL, x, y, 0); break;
+        case QS_C128:
+            if (cc) dispatch_split_cm<cplx_d, true>(a, D, cm, fewer, lds, s, hops, HL, x, y, 0);
+            else dispatch_split_cm<cplx_d, false>(a, D, cm, fewer, lds, s, hops, HL, x, y, 0);
+            break;
+        case QS_C64:
+            if (cc) dispatch_split_cm<cplx_f, true>(a, D, cm, fewer, lds, s, hops, HL, x, y, 0);
+            else dispatch_split_cm<cplx_f, false>(a, D, cm, fewer, lds, s, hops, HL, x, y, 0);
+            break;
+        default: fail(QS_EARG, "unknown dtype");
+        }
+    };
+    // the lean kernel: one launch over block list (bl, bs) of n entries
+    auto pass2 = [&](const SplitHops &sh, const uint32_t *bl, const uint32_t *bs, int n, int own, int chunk,
+                     const MVArgs &args) {
+        Split2Dev D;
+        memset(&D, 0, sizeof D);
+        D.H = t.H;
+        D.lmask = (1u << t.H) - 1;
+        D.wmask = t.W >= 32 ? 0xFFFFFFFFu : ((1u << t.W) - 1);
+        D.blist = bl;
+        D.bstart = bs;
+        D.nblocks = n;
+        D.own = own;
+        D.boff = t.boff.get();
+        for (int p = 0; p <= t.H; p++) {
+            D.cls_off[p] = t.cls_off[p];
+            D.cls_n[p] = t.cls_n[p];
+        }
+        D.nlow = sh.nlow;
+        D.nhigh = sh.nhigh;
+        D.ncross = sh.ncross;
+        D.sl = sh.sl;
+        D.sc = sh.sc;
+        D.words = t.words.get();
+        D.lnb = sh.lnb.get();
+        D.cnb = sh.cnb.get();
+        // LDS: word table, low-hop and cross-hop tables, 256 rows per wavefront
+        size_t o = 0;
+        auto place = [&](size_t bytes) {
+            const size_t at = (o + 15) & ~(size_t)15;
+            o = at + bytes;
+            return (int)at;
+        };
+        D.off_words = place(nw * 2);
+        D.off_lnb = place(own && sh.nlow ? nw * sh.sl : 0);
+        D.off_cnb = place(sh.ncross ? nw * sh.sc * 2 : 0);
+        D.wave_bytes = (int)(256 * esz);
+        D.off_wave = place((size_t)(kSpBlock / 64) * D.wave_bytes);
+        const size_t lds = o;
+        const HopDev *hops = sh.d.get();
+        switch (dtype) {
+        case QS_F64: dispatch_split2_cm<double, false>(args, D, cm, lds, s, hops, HL, x, y, chunk); break;
+        case QS_F32: dispatch_split2_cm<float, false>(args, D, cm, lds, s, hops, HL, x, y, chunk); break;
+        case QS_C128:
+            if (cc) dispatch_split2_cm<cplx_d, true>(args, D, cm, lds, s, hops, HL, x, y, chunk);
+            else dispatch_split2_cm<cplx_d, false>(args, D, cm, lds, s, hops, HL, x, y, chunk);
+            break;
+        case QS_C64:
+            if (cc) dispatch_split2_cm<cplx_f, true>(args, D, cm, lds, s, hops, HL, x, y, chunk);
+            else dispatch_split2_cm<cplx_f, false>(args, D, cm, lds, s, hops, HL, x, y, chunk);
+            break;
+        default: fail(QS_EARG, "unknown dtype");
+        }
+    };
+    const SplitHops &sh = adjoint ? op.split_hops_adj : op.split_hops;
+    if (lean) {
+        int kA = 0, kB = 0;
+        if (tiled >= 0 && split_tiles(op, t, esz, tiled, kA, kB)) {
+            const SplitPassHops &ph = split_pass_hops(op, t, kA, kB);
+            if (adjoint ? ph.ok_adj : ph.ok) {
+                const SplitBasis::Order &o = split_order(b, t, kA, kB);
+                // pass A: y = beta y + alpha (own terms + the groups below H + kA);
+                // pass B: y += alpha (the other groups) on the stored y
+                pass2(adjoint ? ph.a_adj : ph.a, o.blA.get(), o.bsA.get(), o.nA, 1, o.CA, a);
+                MVArgs ab = a;
+                ab.bre = 1.0;
+                ab.bim = 0.0;
+                ab.beta_zero = 0;
+                ab.user_diag = nullptr;
+                pass2(adjoint ? ph.b_adj : ph.b, o.blB.get(), o.bsB.get(), o.nB, 0, o.CB, ab);
+                return;
+            }
+        }
+        if (!sh.lean) fail(QS_EARG, "the lean split kernel does not fit this coupling list");
+        pass2(sh, t.blist.get(), t.bstart.get(), t.nblocks, 1, 0, a);
+        return;
     }
+    pass1(sh);
 }
 
 template <bool CC, int CM, bool TRACK, bool PC>
@@ -1982,14 +2575,19 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         return;
     }
 
-    // automatic choice: the split kernel for large implicit sectors
-    const bool want_split = op.lanes_per_row >= QS_VARIANT_SPLIT && op.lanes_per_row <= QS_VARIANT_SPLIT + 2;
+    // automatic choice: the split kernel (tiled form, 4 MB budget) for large
+    // implicit sectors
+    const bool want_split = op.lanes_per_row >= QS_VARIANT_SPLIT && op.lanes_per_row <= QS_VARIANT_SPLIT_MAX;
     const bool auto_split = op.lanes_per_row == 0 && b.Ns >= ((int64_t)1 << 20) && split_eligible(op, adjoint);
     if ((want_split || auto_split) && !dot_partial) {
         if (!split_eligible(op, adjoint))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
-        launch_split(op, a, dtype, cc, adjoint, x, y, s);
+        const int k = want_split ? op.lanes_per_row - QS_VARIANT_SPLIT
+                                 : (split_lean_ok(op, adjoint) ? QS_SPLIT_LEAN : 0);
+        if (k < QS_SPLIT_LEAN) launch_split(op, a, dtype, cc, adjoint, x, y, s, k, -1, false);
+        else if (k == QS_SPLIT_LEAN) launch_split(op, a, dtype, cc, adjoint, x, y, s, 0, -1, true);
+        else launch_split(op, a, dtype, cc, adjoint, x, y, s, 0, k - QS_SPLIT_TILED, true);
         QS_HIP(hipGetLastError());
         return;
     }

@@ -82,13 +82,16 @@ class SpinBasis1D:
             out["kblock"] = int(self._blocks["kblock"]) % (self.L // self.a)
         return out
 
-    @property
-    def states(self):
+    def _labels(self):
         if self._states is None:
             out = np.empty(self.Ns, dtype=np.uint64)
             check(_lib.lib().qs_basis_states(self._h, _ptr(out)))
             self._states = out
-        return self._states.copy()
+        return self._states
+
+    @property
+    def states(self):
+        return self._labels().copy()
 
     @property
     def reps(self):
@@ -103,13 +106,6 @@ class SpinBasis1D:
         check(_lib.lib().qs_basis_norms(self._h, _ptr(orb), _ptr(ref)))
         return orb, ref
 
-    def normalization(self, state=None):
-        orb, ref = self.norms()
-        if state is None:
-            return ref
-        idx, _ = self.index(np.asarray([state], dtype=np.uint64))
-        return ref[idx[0]]
-
     def index(self, states):
         """Column of each state's orbit (-1 if outside the sector) and <state|column>."""
         s = np.ascontiguousarray(np.asarray(states, dtype=np.uint64))
@@ -118,11 +114,45 @@ class SpinBasis1D:
         check(_lib.lib().qs_basis_lookup(self._h, _ptr(s), s.size, _ptr(idx), _ptr(amp)))
         return idx, amp
 
-    def state_index(self, state):
-        idx, _ = self.index([state])
+    def _column(self, state):
+        # _symmetry_column (discrete_basis.jl:2386-2395): the column holding the
+        # state's orbit and the state's amplitude there
+        if state < 0:
+            raise ArgumentError("state must be nonnegative")
+        idx, amp = self.index([state])
         if idx[0] < 0:
-            raise ArgumentError("state is outside this basis")
+            raise ArgumentError("state has zero weight in this symmetry sector")
+        return int(idx[0]), complex(amp[0])
+
+    def state_index(self, state):
+        """state_index(basis, s) -- basis.jl:1016-1027: position of s in states(basis)."""
+        if state < 0:
+            raise ArgumentError("state must be nonnegative")
+        idx, _ = self.index([state])
+        if idx[0] < 0 or int(self._labels()[idx[0]]) != int(state):
+            raise ArgumentError("state %d is not represented by this basis" % int(state))
         return int(idx[0])
+
+    def representative(self, state):
+        """representative(basis, s) -- discrete_basis.jl:2397-2403: the label of s's column."""
+        c, _ = self._column(state)
+        return int(self._labels()[c])
+
+    def get_amp(self, state):
+        """get_amp(basis, s) -- discrete_basis.jl:2414-2422: s's amplitude in its column."""
+        _, a = self._column(state)
+        return a
+
+    def normalization(self, state=None):
+        """normalization(basis, s) -- discrete_basis.jl:2405-2412: inv(abs2(amp)); without a
+        state, the normalization at every label (bit-exact with the reference's)."""
+        orb, ref = self.norms()
+        if state is None:
+            return ref
+        c, a = self._column(state)
+        if int(self._labels()[c]) == int(state):
+            return float(ref[c])
+        return 1.0 / abs(a) ** 2
 
     def project_to(self, parent_vector, parent_states):
         """P' v for a vector given on explicit parent states (used for initial states)."""

@@ -211,8 +211,10 @@ def test_expm_against_dense(gpu, case):
 def test_fullsize_c2_basis_and_ground_state(gpu):
     # J1-J2 XXZ L=28, nup=14, k=0, p=+1, z=+1 (361,270 rows).  Size-independent properties:
     # dimension from the character formula, orbit sizes summing to C(28,14),
-    # Hermiticity <x,Hy> = <Hx,y>, an FP64 ground state with a small residual,
-    # and the FP32-stream eigenvalue within 1e-8 of the FP64 one.
+    # Hermiticity <x,Hy> = <Hx,y>, an FP64 ground state with residual
+    # <= 1e-8 |E0| and a Temple-bound error <= 1e-10, and the FP32-stream
+    # eigenvalue within 1e-8 of the FP64 one.  The basis arrays themselves are
+    # checked against the oracle's digests in test_fullsize_basis_digests.
     import math
     L = 28
     print("c2 basis", flush=True)
@@ -237,12 +239,51 @@ def test_fullsize_c2_basis_and_ground_state(gpu):
     del op_a
     print("c2 eigsh fp64", flush=True)
     v0 = rng.standard_normal(b.Ns)
-    e64, v64 = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0)
-    r = op.matvec(v64[:, 0]) - e64[0] * v64[:, 0]
-    assert np.linalg.norm(r) < 1e-6
+    e64, v64 = qsgpu.eigsh(op, k=2, which="SA", tol=1e-10, v0=v0)
+    # residuals of the unit Ritz vectors, with |E0| <= ||H||
+    res = [np.linalg.norm(op.matvec(v64[:, i]) - e64[i] * v64[:, i]) / np.linalg.norm(v64[:, i]) for i in (0, 1)]
+    assert res[0] <= 1e-8 * abs(e64[0]), res
+    # Temple's inequality: with the second eigenvalue at least E1 - r1 (the
+    # interval [E1 - r1, E1 + r1] holds an eigenvalue and the k=2 solve found
+    # the two lowest), the ground-state energy lies in
+    # [E0 - r0^2 / (E1 - r1 - E0), E0]
+    gap = e64[1] - res[1] - e64[0]
+    assert gap > 0, (e64, res)
+    assert res[0] ** 2 / gap <= 1e-10, (res, gap)
     print("c2 eigsh fp32", flush=True)
     e32, _ = qsgpu.eigsh(op, k=1, which="SA", tol=1e-10, v0=v0, dtype=np.float32)
     assert abs(e32[0] - e64[0]) <= 1e-8
+
+
+def _digest(a, dtype):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.dtype(dtype).newbyteorder("<")).tobytes()).hexdigest()
+
+
+def _digest_cases():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "sector_digests.json")
+    return json.load(open(path))["sectors"]
+
+
+@pytest.mark.parametrize("rec", _digest_cases(), ids=lambda r: "L%d_%s" % (r["L"], "_".join(
+    "%s%d" % (k[0], v) for k, v in r["blocks"].items())))
+def test_fullsize_basis_digests(gpu, rec):
+    # bit-exact basis at full size: states(basis) (the reference labels), the
+    # orbit sizes and normalization(basis, label) of the L=28 ground-state
+    # sector and of the 26 L=24 quench blocks, against SHA-256 digests of the
+    # oracle's arrays (tests/golden/make_digests.py)
+    print("digests", rec["L"], rec["blocks"], flush=True)
+    b = qsgpu.SpinBasis1D(rec["L"], nup=rec["nup"], pauli=False, **rec["blocks"])
+    assert b.Ns == rec["Ns"]
+    states = b.states
+    assert [int(s) for s in states[:4]] == rec["first_states"]
+    assert _digest(states, np.uint64) == rec["states"]
+    orb, ref = b.norms()
+    assert int(orb.astype(np.int64).sum()) == rec["orbit_sum"]
+    assert _digest(orb, np.uint32) == rec["orbit_size"]
+    assert _digest(ref, np.float64) == rec["ref_norm"]
 
 
 def test_prescaled_gather_complex_sector_fullsize(gpu):
@@ -313,7 +354,7 @@ def test_every_kernel_variant_matches_oracle(gpu, case):
         with pytest.raises(qsgpu.ArgumentError, match="split"):
             op.set_variant(qsgpu.VARIANT_SPLIT)
     lpr, ms = op.tune(np.complex128, repeats=2)
-    assert lpr in (1, 2, 4, 8) + hops + (tuple(qsgpu.VARIANT_SPLIT + k for k in range(3)) if split else ()) and ms > 0
+    assert lpr in (1, 2, 4, 8) + hops + (tuple(qsgpu.VARIANT_SPLIT + k for k in range(8)) if split else ()) and ms > 0
     got = op.matvec(x)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
@@ -329,9 +370,12 @@ def test_hop_variant_rejected_for_other_couplings(gpu):
 @pytest.mark.parametrize("case", [(24, 12, dict(zblock=1)), (24, 10, {}), (20, 10, dict(zblock=-1)), (16, 7, {})],
                          ids=_id)
 def test_split_kernel_against_oracle(gpu, case):
-    # split point 12 for L >= 24 (rows of one block share bits 12..L-1), L/2
+    # split point 10 for L >= 20 (rows of one block share bits 10..L-1), L/2
     # below; next-nearest and fourth-neighbour bonds cross the split, and
-    # under spin inversion the bonds onto bit L-1 read inverted blocks
+    # under spin inversion the bonds onto bit L-1 read inverted blocks.  Every
+    # variant: the first kernel (0 or 1 workgroups per CU held back), the lean
+    # kernel in one pass and its tiled two-pass form at the four tile budgets
+    # (at L = 24 the smallest tiles cover 2^9 of the 2^14 high words)
     L, nup, blocks = case
     print("split", _id(case), flush=True)
     terms = _terms(L) + [("+-", [(0.3, i, (i + 3) % L + 1) for i in range(1, L + 1)]),
@@ -342,14 +386,45 @@ def test_split_kernel_against_oracle(gpu, case):
     rng = np.random.default_rng(17)
     x = rng.standard_normal(b.Ns)
     want = ref.matvec(x).real
-    op.set_variant(qsgpu.VARIANT_SPLIT)
-    for dt, tol in ((np.float64, 1e-12), (np.float32, 2e-6)):
-        got = op.matvec(x.astype(dt), dtype=dt)
-        assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want), dt
     y0 = rng.standard_normal(b.Ns)
-    out = y0.copy()
-    op.mul(out, x, alpha=0.75, beta=-1.5)
-    assert np.linalg.norm(out - (0.75 * want - 1.5 * y0)) <= 1e-12 * np.linalg.norm(out)
+    for k in (0, 1, 3, 4, 5, 6, 7):
+        op.set_variant(qsgpu.VARIANT_SPLIT + k)
+        for dt, tol in ((np.float64, 1e-12), (np.float32, 2e-6)):
+            got = op.matvec(x.astype(dt), dtype=dt)
+            assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want), (k, dt)
+        out = y0.copy()
+        op.mul(out, x, alpha=0.75, beta=-1.5)
+        assert np.linalg.norm(out - (0.75 * want - 1.5 * y0)) <= 1e-12 * np.linalg.norm(out), k
+
+
+@pytest.mark.parametrize("case", [(24, 12, dict(zblock=-1)), (24, 11, {})], ids=_id)
+def test_split_tiled_complex_couplings_and_flags(gpu, case):
+    # complex hop amplitudes (the adjoint list differs), transposed, conjugated
+    # and adjoint products and a user diagonal through both passes
+    L, nup, blocks = case
+    print("split tiled flags", _id(case), flush=True)
+    terms = [("+-", [(0.5 + 0.2j, i, i % L + 1) for i in range(1, L + 1)]),
+             ("-+", [(0.5 - 0.2j, i, i % L + 1) for i in range(1, L + 1)]),
+             ("zz", [(0.8, i, i % L + 1) for i in range(1, L + 1)]),
+             ("+-", [(0.1j, i, (i + 1) % L + 1) for i in range(1, L + 1)])]
+    sec = oracle.Sector(L, nup=nup, pauli=False, **blocks)
+    rng = np.random.default_rng(23)
+    diag = rng.standard_normal(sec.Ns)
+    # complex amplitudes are not inversion-symmetric: P'HP keeps their
+    # inversion average (check_symm off, as in the reference)
+    ref = oracle.Operator(sec, terms, diagonal=diag, check_symm=False)
+    b = qsgpu.SpinBasis1D(L, nup=nup, pauli=False, **blocks)
+    op = qsgpu.QuantumLinearOperator(b, terms, diagonal=diag, check_herm=False, check_symm=False)
+    x = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    flags = ((False, False, op), (True, False, op.T), (False, True, op.conj()), (True, True, op.H))
+    wants = [ref.matvec(x, transposed=tr, conjugated=cj) for tr, cj, _ in flags]
+    for k in (0, 3, 4, 6):
+        op.set_variant(qsgpu.VARIANT_SPLIT + k)
+        for (tr, cj, view), want in zip(flags, wants):
+            got = view.matvec(x)
+            assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), (k, tr, cj)
+            got64 = view.matvec(x.astype(np.complex64), dtype=np.complex64)
+            assert np.linalg.norm(got64 - want) <= 2e-6 * np.linalg.norm(want), (k, tr, cj)
 
 
 def test_split_kernel_fullsize_c3(gpu):
@@ -370,18 +445,56 @@ def test_split_kernel_fullsize_c3(gpu):
     y_split = torch.empty_like(x)
     op.set_variant(qsgpu.VARIANT_HOP)
     op.matvec_device(x, y_hop)
-    op.set_variant(qsgpu.VARIANT_SPLIT)
-    op.matvec_device(x, y_split)
     torch.cuda.synchronize()
     ref = torch.linalg.vector_norm(y_hop).item()
-    assert torch.linalg.vector_norm(y_split - y_hop).item() <= 1e-12 * ref
     x32 = x.float()
     y32 = torch.empty_like(x32)
-    op.matvec_device(x32, y32)
-    torch.cuda.synchronize()
-    assert torch.linalg.vector_norm(y32.double() - y_hop).item() <= 2e-6 * ref
+    for k in (0, 3, 4, 5, 6, 7):
+        print("  split variant", k, flush=True)
+        op.set_variant(qsgpu.VARIANT_SPLIT + k)
+        op.matvec_device(x, y_split)
+        torch.cuda.synchronize()
+        assert torch.linalg.vector_norm(y_split - y_hop).item() <= 1e-12 * ref, k
+        op.matvec_device(x32, y32)
+        torch.cuda.synchronize()
+        assert torch.linalg.vector_norm(y32.double() - y_hop).item() <= 2e-6 * ref, k
     # <x, Hx> is real and the product is symmetric: <x, H x32> == <x32, H x> in FP64
     del y32, x32
+
+
+def test_c3_eigsh_fp32_stream_against_fp64(gpu):
+    # BASELINE precision form: L=32 nup=16 z=+1 (N = 300,540,195), k=4 lowest
+    # eigenvalues (the random fields cancel in z+).  FP32-stored Krylov basis
+    # with FP64 dots / norms / tridiagonal solve against the FP64 run: every
+    # eigenvalue within 1e-8, every FP64 Ritz vector's residual within
+    # 1e-8 ||H|| (|E_i| <= ||H||).
+    import torch
+    print("c3 eigsh", flush=True)
+    b = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
+    h = np.random.default_rng(0xC3).uniform(-2.0, 2.0, 32)
+    terms = _dense.xxz_terms(32, J=1.0, delta=1.0, J2=0.0, periodic=True) + \
+        [("z", [(float(h[i]), i + 1) for i in range(32)])]
+    op = qsgpu.QuantumLinearOperator(b, terms, check_symm=False)
+    v0 = np.random.default_rng(0x5EED).standard_normal(b.Ns)
+    print("  fp64", flush=True)
+    e64, vec = qsgpu.eigsh(op, k=4, which="SA", tol=1e-10, v0=v0)
+    print("  fp64 E", e64.tolist(), qsgpu.eigsh.last_n_matvec, flush=True)
+    hmax = max(abs(e64))
+    y = torch.empty(b.Ns, dtype=torch.float64, device="cuda")
+    for i in range(4):
+        v = torch.from_numpy(np.ascontiguousarray(vec[:, i])).cuda()
+        op.matvec_device(v, y)
+        r = (torch.linalg.vector_norm(y - e64[i] * v) / torch.linalg.vector_norm(v)).item()
+        print("  residual", i, r, flush=True)
+        assert r <= 1e-8 * hmax, (i, r)
+        del v
+    del vec, y
+    torch.cuda.empty_cache()
+    print("  fp32", flush=True)
+    e32 = qsgpu.eigsh(op, k=4, which="SA", tol=1e-10, v0=v0, dtype=np.float32, return_eigenvectors=False)
+    print("  fp32 E", e32.tolist(), qsgpu.eigsh.last_n_matvec, flush=True)
+    assert np.all(np.abs(e32 - e64) <= 1e-8), (e32, e64)
+    assert np.all(np.diff(e64) >= 0)
 
 
 @pytest.mark.parametrize("case", [(8, 4, {}), (8, None, {}), (10, 5, dict(kblock=0, pblock=1, zblock=1)),
