@@ -117,6 +117,9 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     std::vector<int> wanted;
     int nconv = 0;
     uint64_t reseed = 1;
+    bool verifying = false;
+    int verifications = 0;
+    std::vector<double> verified_vals;
     // per-step scalars of one Lanczos cycle, device side: two Gram passes
     // (m + 1 complex each) and |w|^2
     const size_t per = (size_t)4 * (m + 1) + 2;
@@ -189,6 +192,48 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
         for (int idx : wanted) {
             double r = std::fabs(beta_last * Y[(size_t)(m - 1) * m + idx]);
             if (r <= tol * std::max(eps23, std::fabs(theta[idx]))) nconv++;
+        }
+        if (verifying) {
+            // the cycle started from a fresh random direction beside the
+            // converged Ritz vectors: unchanged wanted values (as a multiset)
+            // mean no eigenvalue among the wanted ones was missed
+            std::vector<double> now;
+            for (int idx : wanted) now.push_back(theta[idx]);
+            std::sort(now.begin(), now.end());
+            bool same = now.size() == verified_vals.size();
+            const double scale = std::max(1.0, std::fabs(now.empty() ? 0.0 : now.back()));
+            for (size_t i = 0; same && i < now.size(); i++)
+                same = std::fabs(now[i] - verified_vals[i]) <= std::max(1e3 * kEps, tol) * scale;
+            verifying = false;
+            if (same && nconv >= k) break;
+        }
+        if (nconv >= k && verifications < k + 2 && m - k >= 2) {
+            // Converged.  A single-vector Krylov space holds one vector of each
+            // eigenspace, so a repeated eigenvalue shows only through rounding
+            // and can be missed.  Restart once with the converged Ritz vectors
+            // and a random direction orthogonal to them (no coupling: its
+            // Krylov block starts fresh), and accept if the wanted values stay.
+            verifications++;
+            verifying = true;
+            verified_vals.clear();
+            for (int idx : wanted) verified_vals.push_back(theta[idx]);
+            std::sort(verified_vals.begin(), verified_vals.end());
+            std::vector<cd> C((size_t)m * k);
+            for (int c = 0; c < k; c++)
+                for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + wanted[c]], 0);
+            Krylov tmp(dtype, n, k, s);
+            basis_combine(dtype, K.V.get(), n, m, C, k, tmp.V.get(), s);
+            QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * k * K.esz, hipMemcpyDeviceToDevice, s));
+            void *w = K.col(k);
+            vec_random(dtype, w, n, 0x7e51ULL + (reseed++), s);
+            std::vector<cd> hh;
+            double b2 = orthogonalize(K, k - 1, w, wk, hh, s);
+            vec_scale_copy(dtype, w, w, n, cd(1.0 / b2, 0), s);
+            std::fill(T.begin(), T.end(), 0.0);
+            for (int c = 0; c < k; c++) t(c, c) = theta[wanted[c]];
+            kept = k;
+            if (iter == maxiter - 1) break;
+            continue;
         }
         if (nconv >= k || iter == maxiter - 1) break;
         // thick restart: keep the wanted Ritz vectors plus as many runners-up

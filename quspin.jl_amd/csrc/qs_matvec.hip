@@ -1423,14 +1423,246 @@ struct Split2Dev {
     int off_words, off_lnb, off_cnb, off_wave, wave_bytes;
 };
 
+// Per-lane data of the lean kernel: the high hop group this lane holds.
+struct S2Lane {
+    int lo, hi, nz, sym, top;
+    double re1, re2, im1, im2;
+};
+
+// One block of the lean kernel with NS row slots of 64 rows (n <= 64 NS).
+// Every load is issued unconditionally: rows past the block's end read its
+// last row and hops past a batch's end read the own block (weight 0), so no
+// load sits in a branch that would make the compiler wait for it there.
+template <typename V, bool CC, int CM, int NS>
+__device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D, const HopDev *__restrict__ hops,
+                                             const DiagClass *__restrict__ dcls, const TermDev *__restrict__ drest,
+                                             const V *__restrict__ x, V *__restrict__ y, const uint16_t *s_words,
+                                             const uint8_t *s_lnb, const uint16_t *s_cnb, V *s_x, const S2Lane &g,
+                                             bool is_high, int lane, uint32_t h, uint32_t off, int n, int cw) {
+    using T = VTraits<V>;
+    constexpr int kNB = sizeof(V) <= 4 ? 6 : (sizeof(V) <= 8 ? 5 : 3);
+    constexpr uint32_t ES = sizeof(V);
+    // this block's high hops: liveness by ballot; neighbour offset, row
+    // order and weight stay in the lane that holds the hop
+    bool live = false;
+    uint32_t e_off = 0;
+    double e_re = 0, e_im = 0;
+    int e_rev = 0;
+    if (is_high) {
+        const uint32_t blo = (h >> g.lo) & 1, bhi = (h >> g.hi) & 1;
+        live = (g.nz >> (blo | (bhi << 1))) & 1;
+        e_re = (g.sym || blo) ? g.re1 : g.re2;
+        e_im = CC ? -((g.sym || blo) ? g.im1 : g.im2) : 0.0;   // conj(amp): adjoint couplings
+        uint32_t h2 = h ^ ((1u << g.lo) | (1u << g.hi));
+        if (CM == CM_ZONLY && g.top) {
+            // the hop sets bit L-1: the inverted state's column, rows reversed
+            h2 = ~h2 & D.wmask;
+            e_rev = 1;
+            if (A.zchar < 0) {
+                e_re = -e_re;
+                e_im = -e_im;
+            }
+        }
+        e_off = live ? D.boff[h2] : off;
+    }
+    // hops in row order and hops onto the inverted block (rows reversed)
+    uint64_t lm = __ballot(live && !e_rev);
+    uint64_t lmr = __ballot(live && e_rev);
+    uint32_t roff[NS], rrev[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        const int r = min(lane + 64 * j, n - 1);
+        roff[j] = (uint32_t)r * ES;
+        rrev[j] = (uint32_t)(n - 1 - r) * ES;
+    }
+    const V *xo_base = x + off;
+    V xo[NS];
+    if (D.own) {
+#pragma unroll
+        for (int j = 0; j < NS; j++) xo[j] = *atb(xo_base, roff[j]);
+    }
+    double ar[NS], ai[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        ar[j] = 0;
+        ai[j] = 0;
+    }
+    bool first = true;
+    do {
+        // one batch of up to kNB live high hops (lane indices in scalars)
+        int k_idx[kNB];
+        int nb = 0;
+#pragma unroll
+        for (int kk = 0; kk < kNB; kk++) {
+            k_idx[kk] = 0;
+            if (lm) {
+                k_idx[kk] = __builtin_ctzll(lm);
+                lm &= lm - 1;
+                nb = kk + 1;
+            }
+        }
+        V v[kNB][NS];
+#pragma unroll
+        for (int kk = 0; kk < kNB; kk++) {
+            const uint32_t bo = __builtin_amdgcn_readlane(e_off, k_idx[kk]);
+            const V *base = x + (kk < nb ? bo : off);
+#pragma unroll
+            for (int j = 0; j < NS; j++) v[kk][j] = *atb(base, roff[j]);
+        }
+        if (first) {
+            first = false;
+            if (D.own) {
+                // own block: stage in LDS, diagonal, low hops (the high-hop
+                // loads above are in flight meanwhile)
+#pragma unroll
+                for (int j = 0; j < NS; j++)
+                    if (lane + 64 * j < n) s_x[lane + 64 * j] = xo[j];
+                wave_lds_sync();
+                uint32_t lw[NS];
+#pragma unroll
+                for (int j = 0; j < NS; j++) {
+                    const int r = min(lane + 64 * j, n - 1);
+                    double xr, xi;
+                    T::load(&xo[j], 0, xr, xi);
+                    if (A.conj_x) xi = -xi;
+                    const uint32_t l = s_words[cw + r];
+                    lw[j] = l;
+                    double dr = 0, di = 0;
+                    diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
+                    di = -di;   // conj(amp): the terms are the adjoint couplings
+                    if (A.user_diag) {
+                        const int64_t row = (int64_t)off + r;
+                        dr += A.user_diag[2 * row];
+                        di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+                    }
+                    acc_nb<V>(dr, di, xr, xi, ar[j], ai[j]);
+                }
+                for (int gi = 0; gi < D.nlow; gi++) {
+                    const HopDev &hp = hops[gi];
+                    const double r1 = hp.re[1], r2 = hp.re[2];
+                    const double i1 = CC ? hp.im[1] : 0.0, i2 = CC ? hp.im[2] : 0.0;
+                    const int sym = hp.sym, lo = hp.lo;
+#pragma unroll
+                    for (int j = 0; j < NS; j++) {
+                        const int r = min(lane + 64 * j, n - 1);
+                        const uint32_t nid = s_lnb[(cw + r) * D.sl + gi];
+                        double vr, vi;
+                        T::load(s_x, nid, vr, vi);
+                        if (A.conj_x) vi = -vi;
+                        double mr = r1, mi = i1;
+                        if (!sym && !((lw[j] >> lo) & 1)) {
+                            mr = r2;
+                            mi = i2;
+                        }
+                        acc_nb<V>(mr, -mi, vr, vi, ar[j], ai[j]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kNB; kk++) {
+            if (kk < nb) {
+                const uint64_t rb = __builtin_bit_cast(uint64_t, e_re);
+                const double wr = __builtin_bit_cast(
+                    double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(rb >> 32), k_idx[kk]) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((uint32_t)rb, k_idx[kk]));
+                double wi = 0;
+                if (CC) {
+                    const uint64_t ib = __builtin_bit_cast(uint64_t, e_im);
+                    wi = __builtin_bit_cast(
+                        double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(ib >> 32), k_idx[kk]) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((uint32_t)ib, k_idx[kk]));
+                }
+#pragma unroll
+                for (int j = 0; j < NS; j++) {
+                    double vr, vi;
+                    T::load(&v[kk][j], 0, vr, vi);
+                    acc_nb<V>(wr, wi, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+                }
+            }
+        }
+    } while (lm);
+    // hops that set bit L-1 under spin inversion: row r of the inverted block
+    // is row n-1-r
+    while (lmr) {
+        const int k = __builtin_ctzll(lmr);
+        lmr &= lmr - 1;
+        const V *base = x + __builtin_amdgcn_readlane(e_off, k);
+        V v[NS];
+#pragma unroll
+        for (int j = 0; j < NS; j++) v[j] = *atb(base, rrev[j]);
+        const uint64_t rb = __builtin_bit_cast(uint64_t, e_re);
+        const double wr = __builtin_bit_cast(
+            double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(rb >> 32), k) << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((uint32_t)rb, k));
+        double wi = 0;
+        if (CC) {
+            const uint64_t ib = __builtin_bit_cast(uint64_t, e_im);
+            wi = __builtin_bit_cast(
+                double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(ib >> 32), k) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((uint32_t)ib, k));
+        }
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            double vr, vi;
+            T::load(&v[j], 0, vr, vi);
+            acc_nb<V>(wr, wi, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+        }
+    }
+    // cross hops: rows of the neighbour block from the table, gathered
+    for (int c = 0; c < D.ncross; c++) {
+        const HopDev &hc = hops[D.nlow + D.nhigh + c];
+        const int hi = hc.hi - D.H;
+        const uint32_t bhi = (h >> hi) & 1;
+        double m_r = bhi ? hc.re[2] : hc.re[1];
+        double m_i = CC ? (bhi ? hc.im[2] : hc.im[1]) : 0.0;
+        uint32_t h2 = h ^ (1u << hi);
+        if (CM == CM_ZONLY && hc.top) {
+            h2 = ~h2 & D.wmask;
+            if (A.zchar < 0) {
+                m_r = -m_r;
+                m_i = -m_i;
+            }
+        }
+        const int p2 = A.nup - __popc(h2);
+        if (!((hc.nzmask >> (bhi ? 2 : 1)) & 1) || p2 < 0 || p2 > D.H) continue;
+        const uint32_t o2 = D.boff[h2];
+        if (o2 == 0xFFFFFFFFu) continue;
+        const V *b2 = x + o2;
+        V u[NS];
+        bool lv[NS];
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            const int r = min(lane + 64 * j, n - 1);
+            const uint32_t e = s_cnb[(cw + r) * D.sc + c];
+            lv[j] = (e >> 15) != bhi;
+            u[j] = *atb(b2, (lv[j] ? (e & 0x7FFFu) : 0u) * ES);
+        }
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            double vr, vi;
+            T::load(&u[j], 0, vr, vi);
+            if (!lv[j]) {
+                vr = 0;
+                vi = 0;
+            }
+            acc_nb<V>(m_r, -m_i, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+        }
+    }
+    double dot_r = 0, dot_i = 0;
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        const int r = lane + 64 * j;
+        if (r < n) finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
+    }
+}
+
 template <typename V, bool CC, int CM>
 __global__ __launch_bounds__(kSpBlock) void k_matvec_split2(MVArgs A, Split2Dev D, const HopDev *__restrict__ hops,
                                                             const DiagClass *__restrict__ dcls,
                                                             const TermDev *__restrict__ drest,
                                                             const V *__restrict__ x, V *__restrict__ y) {
     using T = VTraits<V>;
-    constexpr int kNB = sizeof(V) <= 4 ? 8 : (sizeof(V) <= 8 ? 6 : 4);
-    constexpr uint32_t ES = sizeof(V);
     extern __shared__ __align__(16) char smem[];
     uint16_t *s_words = (uint16_t *)(smem + D.off_words);
     uint8_t *s_lnb = (uint8_t *)(smem + D.off_lnb);
@@ -1449,21 +1681,19 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split2(MVArgs A, Split2Dev 
     __syncthreads();
     // lane k < nhigh holds high hop group k
     const bool is_high = lane < D.nhigh;
-    int g_lo = 0, g_hi = 0, g_nz = 0, g_sym = 0, g_top = 0;
-    double g_re1 = 0, g_re2 = 0, g_im1 = 0, g_im2 = 0;
+    S2Lane g{0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
     if (is_high) {
         const HopDev &hg = hops[D.nlow + lane];
-        g_lo = hg.lo - D.H;
-        g_hi = hg.hi - D.H;
-        g_nz = hg.nzmask;
-        g_sym = hg.sym;
-        g_top = hg.top;
-        g_re1 = hg.re[1];
-        g_re2 = hg.re[2];
-        g_im1 = hg.im[1];
-        g_im2 = hg.im[2];
+        g.lo = hg.lo - D.H;
+        g.hi = hg.hi - D.H;
+        g.nz = hg.nzmask;
+        g.sym = hg.sym;
+        g.top = hg.top;
+        g.re1 = hg.re[1];
+        g.re2 = hg.re[2];
+        g.im1 = hg.im[1];
+        g.im2 = hg.im[2];
     }
-    double dot_r = 0, dot_i = 0;
     const int xcd = blockIdx.x & 7;
     const int nwx = (int)(gridDim.x >> 3) * (kSpBlock / 64);
     const int64_t C = D.chunk;
@@ -1487,204 +1717,19 @@ __global__ __launch_bounds__(kSpBlock) void k_matvec_split2(MVArgs A, Split2Dev 
         const int p = A.nup - __popc(h);
         const int n = D.cls_n[p];
         const int cw = D.cls_off[p];
-        const int ns = (n + 63) >> 6;       // row slots in use (wave-uniform)
-        // this block's high hops: liveness by ballot, neighbour offset and
-        // weight per lane, moved to scalar registers below
-        bool live = false;
-        uint32_t e_off = 0;
-        double e_re = 0, e_im = 0;
-        int e_rev = 0;
-        if (is_high) {
-            const uint32_t blo = (h >> g_lo) & 1, bhi = (h >> g_hi) & 1;
-            live = (g_nz >> (blo | (bhi << 1))) & 1;
-            e_re = (g_sym || blo) ? g_re1 : g_re2;
-            e_im = CC ? -((g_sym || blo) ? g_im1 : g_im2) : 0.0;   // conj(amp): adjoint couplings
-            uint32_t h2 = h ^ ((1u << g_lo) | (1u << g_hi));
-            if (CM == CM_ZONLY && g_top) {
-                // the hop sets bit L-1: the inverted state's column, rows reversed
-                h2 = ~h2 & D.wmask;
-                e_rev = 1;
-                if (A.zchar < 0) {
-                    e_re = -e_re;
-                    e_im = -e_im;
-                }
-            }
-            e_off = live ? D.boff[h2] : 0u;
-        }
-        uint64_t lm = __ballot(live);
-        // per-slot byte offsets of the lane's rows (and of the reversed rows)
-        uint32_t roff[kSpRows], rrev[kSpRows];
-#pragma unroll
-        for (int j = 0; j < kSpRows; j++) {
-            const int r = lane + 64 * j;
-            roff[j] = (uint32_t)r * ES;
-            rrev[j] = (uint32_t)(n - 1 - r) * ES;
-        }
-        // own rows: loads issued now, consumed after the first high batch's loads
-        V xo[kSpRows];
-#pragma unroll
-        for (int j = 0; j < kSpRows; j++) {
-            xo[j] = V{};
-            if (D.own && j < ns && lane + 64 * j < n) xo[j] = *atb(x + off, roff[j]);
-        }
-        double ar[kSpRows], ai[kSpRows];
-#pragma unroll
-        for (int j = 0; j < kSpRows; j++) {
-            ar[j] = 0;
-            ai[j] = 0;
-        }
-        bool first = true;
-        do {
-            // one batch of up to kNB live high hops: scalar offsets and weights
-            uint32_t b_off[kNB];
-            double b_re[kNB], b_im[kNB];
-            int b_rev[kNB];
-            int nb = 0;
-#pragma unroll
-            for (int kk = 0; kk < kNB; kk++) {
-                b_off[kk] = 0;
-                b_re[kk] = 0;
-                b_im[kk] = 0;
-                b_rev[kk] = 0;
-                if (lm) {
-                    const int k = __builtin_ctzll(lm);
-                    lm &= lm - 1;
-                    b_off[kk] = __builtin_amdgcn_readlane(e_off, k);
-                    b_rev[kk] = __builtin_amdgcn_readlane(e_rev, k);
-                    {
-                        const uint64_t bits = __builtin_bit_cast(uint64_t, e_re);
-                        const uint32_t lo32 = __builtin_amdgcn_readlane((uint32_t)bits, k);
-                        const uint32_t hi32 = __builtin_amdgcn_readlane((uint32_t)(bits >> 32), k);
-                        b_re[kk] = __builtin_bit_cast(double, ((uint64_t)hi32 << 32) | lo32);
-                    }
-                    if (CC) {
-                        const uint64_t bits = __builtin_bit_cast(uint64_t, e_im);
-                        const uint32_t lo32 = __builtin_amdgcn_readlane((uint32_t)bits, k);
-                        const uint32_t hi32 = __builtin_amdgcn_readlane((uint32_t)(bits >> 32), k);
-                        b_im[kk] = __builtin_bit_cast(double, ((uint64_t)hi32 << 32) | lo32);
-                    }
-                    nb = kk + 1;
-                }
-            }
-            V v[kNB][kSpRows];
-#pragma unroll
-            for (int kk = 0; kk < kNB; kk++) {
-#pragma unroll
-                for (int j = 0; j < kSpRows; j++) {
-                    v[kk][j] = V{};
-                    if (kk < nb && j < ns && lane + 64 * j < n)
-                        v[kk][j] = *atb(x + b_off[kk], b_rev[kk] ? rrev[j] : roff[j]);
-                }
-            }
-            if (first) {
-                first = false;
-                if (D.own) {
-                    // own block: stage in LDS, diagonal, low hops
-#pragma unroll
-                    for (int j = 0; j < kSpRows; j++)
-                        if (j < ns && lane + 64 * j < n) s_x[lane + 64 * j] = xo[j];
-                    wave_lds_sync();
-#pragma unroll
-                    for (int j = 0; j < kSpRows; j++) {
-                        const int r = lane + 64 * j;
-                        if (j < ns && r < n) {
-                            double xr, xi;
-                            T::load(&xo[j], 0, xr, xi);
-                            if (A.conj_x) xi = -xi;
-                            const uint32_t l = s_words[cw + r];
-                            double dr = 0, di = 0;
-                            diag_value<CC>(A, dcls, drest, (h << D.H) | l, dr, di);
-                            di = -di;   // conj(amp): the terms are the adjoint couplings
-                            if (A.user_diag) {
-                                const int64_t row = (int64_t)off + r;
-                                dr += A.user_diag[2 * row];
-                                di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
-                            }
-                            acc_nb<V>(dr, di, xr, xi, ar[j], ai[j]);
-                        }
-                    }
-#pragma unroll 4
-                    for (int g = 0; g < D.nlow; g++) {
-                        const HopDev &hp = hops[g];
-                        const double r1 = hp.re[1], r2 = hp.re[2];
-                        const double i1 = CC ? hp.im[1] : 0.0, i2 = CC ? hp.im[2] : 0.0;
-                        const int sym = hp.sym, lo = hp.lo;
-#pragma unroll
-                        for (int j = 0; j < kSpRows; j++) {
-                            const int r = lane + 64 * j;
-                            if (j < ns && r < n) {
-                                const uint32_t nid = s_lnb[(cw + r) * D.sl + g];
-                                double vr, vi;
-                                T::load(s_x, nid, vr, vi);
-                                if (A.conj_x) vi = -vi;
-                                double mr = r1, mi = i1;
-                                if (!sym && !((s_words[cw + r] >> lo) & 1)) {
-                                    mr = r2;
-                                    mi = i2;
-                                }
-                                acc_nb<V>(mr, -mi, vr, vi, ar[j], ai[j]);
-                            }
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int kk = 0; kk < kNB; kk++) {
-                if (kk < nb) {
-#pragma unroll
-                    for (int j = 0; j < kSpRows; j++) {
-                        if (j < ns) {
-                            double vr, vi;
-                            T::load(&v[kk][j], 0, vr, vi);
-                            acc_nb<V>(b_re[kk], b_im[kk], vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
-                        }
-                    }
-                }
-            }
-        } while (lm);
-        // cross hops: the neighbour block's rows from the table, gathered
-        for (int c = 0; c < D.ncross; c++) {
-            const HopDev &hc = hops[D.nlow + D.nhigh + c];
-            const int hi = hc.hi - D.H;
-            const uint32_t bhi = (h >> hi) & 1;
-            double m_r = bhi ? hc.re[2] : hc.re[1];
-            double m_i = CC ? (bhi ? hc.im[2] : hc.im[1]) : 0.0;
-            uint32_t h2 = h ^ (1u << hi);
-            if (CM == CM_ZONLY && hc.top) {
-                h2 = ~h2 & D.wmask;
-                if (A.zchar < 0) {
-                    m_r = -m_r;
-                    m_i = -m_i;
-                }
-            }
-            const int p2 = A.nup - __popc(h2);
-            if (!((hc.nzmask >> (bhi ? 2 : 1)) & 1) || p2 < 0 || p2 > D.H) continue;
-            const uint32_t o2 = D.boff[h2];
-            if (o2 == 0xFFFFFFFFu) continue;
-            V u[kSpRows];
-#pragma unroll
-            for (int j = 0; j < kSpRows; j++) {
-                const int r = lane + 64 * j;
-                u[j] = V{};
-                if (j < ns && r < n) {
-                    const uint32_t e = s_cnb[(cw + r) * D.sc + c];
-                    if ((e >> 15) != bhi) u[j] = x[(int64_t)o2 + (e & 0x7FFFu)];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kSpRows; j++) {
-                if (j < ns) {
-                    double vr, vi;
-                    T::load(&u[j], 0, vr, vi);
-                    acc_nb<V>(m_r, -m_i, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kSpRows; j++) {
-            const int r = lane + 64 * j;
-            if (j < ns && r < n) finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
-        }
+        const int ns = (n + 63) >> 6;   // row slots in use (wave-uniform)
+        if (ns <= 1)
+            split2_block<V, CC, CM, 1>(A, D, hops, dcls, drest, x, y, s_words, s_lnb, s_cnb, s_x, g, is_high, lane, h,
+                                       off, n, cw);
+        else if (ns == 2)
+            split2_block<V, CC, CM, 2>(A, D, hops, dcls, drest, x, y, s_words, s_lnb, s_cnb, s_x, g, is_high, lane, h,
+                                       off, n, cw);
+        else if (ns == 3)
+            split2_block<V, CC, CM, 3>(A, D, hops, dcls, drest, x, y, s_words, s_lnb, s_cnb, s_x, g, is_high, lane, h,
+                                       off, n, cw);
+        else
+            split2_block<V, CC, CM, 4>(A, D, hops, dcls, drest, x, y, s_words, s_lnb, s_cnb, s_x, g, is_high, lane, h,
+                                       off, n, cw);
         wave_lds_sync();   // the block's LDS reads are done before the next block's writes
     }
 }

@@ -123,7 +123,7 @@ def test_check_rejections_carry_the_reference_messages(gpu):
         qsgpu.QuantumLinearOperator(b, [("zz", [(1.0, 1, 2)])], diagonal=np.full(b.Ns, 1j), check_symm=False)
     # with the checks off the same lists build, and act as P'HP
     op = qsgpu.QuantumLinearOperator(b, [("+-", [(0.5, 1, 2)])], check_symm=False, check_herm=False)
-    assert not op.ishermitian
+    assert not op.ishermitian()
     ref = oracle.Operator(oracle.Sector(8, nup=4, pauli=False, kblock=0), [("+-", [(0.5, 1, 2)])], check_symm=False)
     x = np.random.default_rng(1).standard_normal(b.Ns) + 0j
     assert np.linalg.norm(op.matvec(x) - ref.matvec(x)) <= 1e-13 * max(1.0, np.linalg.norm(ref.matvec(x)))
