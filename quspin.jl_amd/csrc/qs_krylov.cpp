@@ -117,9 +117,10 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     std::vector<int> wanted;
     int nconv = 0;
     uint64_t reseed = 1;
-    bool verifying = false;
+    int kw = k;                  // Ritz values converged: k, then k + 1 while verifying
+    bool have_record = false;
     int verifications = 0;
-    std::vector<double> verified_vals;
+    std::vector<double> record;  // the k wanted values of the last converged round
     // per-step scalars of one Lanczos cycle, device side: two Gram passes
     // (m + 1 complex each) and |w|^2
     const size_t per = (size_t)4 * (m + 1) + 2;
@@ -187,57 +188,68 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
         for (int i = 0; i < m; i++)
             for (int j = 0; j < m; j++) A[(size_t)i * m + j] = t(i, j);
         sym_eig(m, A, theta, Y);
-        wanted = select(theta, which, k);
+        std::vector<int> wanted_all = select(theta, which, kw);
+        // the Ritz value beyond the k wanted (verification rounds) only has to
+        // settle enough to tell whether it belongs among them
+        const double tol_extra = std::max(tol, 1e-6);
+        int nconv_all = 0;
+        for (size_t i = 0; i < wanted_all.size(); i++) {
+            const int idx = wanted_all[i];
+            double r = std::fabs(beta_last * Y[(size_t)(m - 1) * m + idx]);
+            if (r <= ((int)i < k ? tol : tol_extra) * std::max(eps23, std::fabs(theta[idx]))) nconv_all++;
+        }
+        wanted.assign(wanted_all.begin(), wanted_all.begin() + std::min<size_t>(k, wanted_all.size()));
         nconv = 0;
         for (int idx : wanted) {
             double r = std::fabs(beta_last * Y[(size_t)(m - 1) * m + idx]);
             if (r <= tol * std::max(eps23, std::fabs(theta[idx]))) nconv++;
         }
-        if (verifying) {
-            // the cycle started from a fresh random direction beside the
-            // converged Ritz vectors: unchanged wanted values (as a multiset)
-            // mean no eigenvalue among the wanted ones was missed
+        if (nconv_all >= kw) {
+            // Converged.  A single-vector Krylov space holds one vector of each
+            // eigenspace, so a repeated eigenvalue shows only through rounding
+            // and can be missed.  So, once: restart with the converged Ritz
+            // vectors beside a random direction orthogonal to them (its
+            // Krylov block starts fresh, uncoupled) and converge one Ritz value
+            // more than asked.  The k wanted values are accepted when a
+            // verification round leaves them unchanged as a multiset.
             std::vector<double> now;
             for (int idx : wanted) now.push_back(theta[idx]);
             std::sort(now.begin(), now.end());
-            bool same = now.size() == verified_vals.size();
-            const double scale = std::max(1.0, std::fabs(now.empty() ? 0.0 : now.back()));
-            for (size_t i = 0; same && i < now.size(); i++)
-                same = std::fabs(now[i] - verified_vals[i]) <= std::max(1e3 * kEps, tol) * scale;
-            verifying = false;
-            if (same && nconv >= k) break;
-        }
-        if (nconv >= k && verifications < k + 2 && m - k >= 2) {
-            // Converged.  A single-vector Krylov space holds one vector of each
-            // eigenspace, so a repeated eigenvalue shows only through rounding
-            // and can be missed.  Restart once with the converged Ritz vectors
-            // and a random direction orthogonal to them (no coupling: its
-            // Krylov block starts fresh), and accept if the wanted values stay.
+            if (have_record) {
+                bool same = now.size() == record.size();
+                const double scale = std::max(1.0, std::fabs(now.empty() ? 0.0 : now.back()));
+                for (size_t i = 0; same && i < now.size(); i++)
+                    same = std::fabs(now[i] - record[i]) <= std::max(1e3 * kEps, tol) * scale;
+                if (same) break;
+            }
+            // (with k = 1 a missed copy of the lowest value cannot change the
+            // answer, so only k >= 2 verifies)
+            const bool can = k >= 2 && which != QS_WHICH_BE && m - (k + 1) >= 2 && verifications < k + 2;
+            if (!can || iter == maxiter - 1) break;
             verifications++;
-            verifying = true;
-            verified_vals.clear();
-            for (int idx : wanted) verified_vals.push_back(theta[idx]);
-            std::sort(verified_vals.begin(), verified_vals.end());
-            std::vector<cd> C((size_t)m * k);
-            for (int c = 0; c < k; c++)
-                for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + wanted[c]], 0);
-            Krylov tmp(dtype, n, k, s);
-            basis_combine(dtype, K.V.get(), n, m, C, k, tmp.V.get(), s);
-            QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * k * K.esz, hipMemcpyDeviceToDevice, s));
-            void *w = K.col(k);
+            record = now;
+            have_record = true;
+            const int kc = (int)wanted_all.size();
+            std::vector<cd> C((size_t)m * kc);
+            for (int c = 0; c < kc; c++)
+                for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + wanted_all[c]], 0);
+            Krylov tmp(dtype, n, kc, s);
+            basis_combine(dtype, K.V.get(), n, m, C, kc, tmp.V.get(), s);
+            QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * kc * K.esz, hipMemcpyDeviceToDevice, s));
+            void *w = K.col(kc);
             vec_random(dtype, w, n, 0x7e51ULL + (reseed++), s);
             std::vector<cd> hh;
-            double b2 = orthogonalize(K, k - 1, w, wk, hh, s);
+            double b2 = orthogonalize(K, kc - 1, w, wk, hh, s);
             vec_scale_copy(dtype, w, w, n, cd(1.0 / b2, 0), s);
             std::fill(T.begin(), T.end(), 0.0);
-            for (int c = 0; c < k; c++) t(c, c) = theta[wanted[c]];
-            kept = k;
-            if (iter == maxiter - 1) break;
+            for (int c = 0; c < kc; c++) t(c, c) = theta[wanted_all[c]];
+            kept = kc;
+            kw = k + 1;
             continue;
         }
-        if (nconv >= k || iter == maxiter - 1) break;
+        if (iter == maxiter - 1) break;
         // thick restart: keep the wanted Ritz vectors plus as many runners-up
-        int keep = std::min(m - 1, k + (m - k) / 2);
+        int keep = std::min(m - 1, kw + (m - kw) / 2);
         std::vector<int> sel = select(theta, which, keep);
         std::vector<cd> C((size_t)m * keep);
         for (int c = 0; c < keep; c++)
