@@ -200,6 +200,23 @@ int qs_op_matvec(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, do
                  double alpha_im, double beta_re, double beta_im, int transposed, int conjugated,
                  int on_device);
 
+/* The same product on device vectors with no device allocation: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It runs
+ * only kernels whose tables qs_op_create built (the two-bit hop kernel or the
+ * general kernels; never the lazily assembled or split forms), and in a
+ * parent-table sector (translations / parity with a particle number) the
+ * pre-scaled x goes to a fixed scratch array of QS_NOALLOC_SCRATCH_BYTES per
+ * GPU (16 bytes per complex, 8 per real row: at most 2^21 complex or 2^22 real
+ * rows; larger sectors are rejected with QS_EARG).  x and y are device
+ * pointers and must not alias (the copy the reference makes of an aliased
+ * input, operators.jl:3158, needs a buffer).  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (concurrent streams or
+ * threads) race and give wrong results; serialise them. */
+#define QS_NOALLOC_SCRATCH_BYTES (32u << 20)
+int qs_op_matvec_noalloc(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, double alpha_re,
+                         double alpha_im, double beta_re, double beta_im, int transposed, int conjugated);
+
 /* eigsh(op; k, which, tol, maxiter, ncv, v0, return_eigenvectors) --
  * operators.jl:3367-3408 (ARPACK "SA"/"LA"/... semantics).  Thick-restart
  * Lanczos with full FP64-accumulated reorthogonalisation on the device.

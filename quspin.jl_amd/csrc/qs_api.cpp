@@ -821,6 +821,48 @@ int qs_op_matvec(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, do
     });
 }
 
+int qs_op_matvec_noalloc(qs_op *op, int dtype, const void *x, void *y, int64_t ncols, double alpha_re,
+                         double alpha_im, double beta_re, double beta_im, int transposed, int conjugated) {
+    return guard([&] {
+        check_ptr(op, "op");
+        check_ptr(x, "x");
+        check_ptr(y, "y");
+        OpHost &o = op->o;
+        if (ncols < 0) fail(QS_EDIM, "ncols must be non-negative");
+        if (dtype != QS_F64 && dtype != QS_C128 && dtype != QS_F32 && dtype != QS_C64) fail(QS_EARG, "unknown dtype");
+        if (o.is_complex && !dtype_complex(dtype))
+            fail(QS_EARG, "a complex operator needs a complex vector dtype");
+        if (x == y)
+            fail(QS_EARG, "x and y must not alias: the copy of an aliased input needs a buffer (use qs_op_matvec)");
+        const BasisHost &b = *o.basis;
+        const int64_t n = b.Ns;
+        // the parent-table pre-scale pass writes one FP64 (complex) value per row
+        const size_t need = b.ptab.get() ? (size_t)n * (dtype_complex(dtype) ? 16 : 8) : 0;
+        if (need > QS_NOALLOC_SCRATCH_BYTES)
+            fail(QS_EARG, "sector of " + std::to_string(n) + " rows needs " + std::to_string(need) +
+                              " bytes of scratch, more than the no-allocation maximum of " +
+                              std::to_string((size_t)QS_NOALLOC_SCRATCH_BYTES) + " bytes (use qs_op_matvec)");
+        hipStream_t s = o.stream ? o.stream : o.own_stream;
+        if (!s) fail(QS_EDEVICE, "operator has no stream");
+        void *scr = noalloc_scratch();
+        const size_t esz = dtype_size(dtype);
+        bool adj = transposed != 0;
+        bool cj = (transposed != 0) != (conjugated != 0);
+        double ar = alpha_re, ai = cj ? -alpha_im : alpha_im, br = beta_re, bi = cj ? -beta_im : beta_im;
+        const bool split = conjugated && alpha_im != 0 && o.extracted;
+        for (int64_t c = 0; c < ncols; c++) {
+            const char *xc = (const char *)x + (size_t)c * n * esz;
+            char *yc = (char *)y + (size_t)c * n * esz;
+            if (!split) {
+                launch_matvec(o, dtype, xc, yc, ar, ai, br, bi, adj, cj, cj, s, nullptr, scr);
+            } else {
+                launch_matvec(o, dtype, xc, yc, ar, -ai, br, bi, adj, cj, cj, s, nullptr, scr);
+                launch_matvec(*o.extracted, dtype, xc, yc, 0.0, 2 * ai, 1.0, 0.0, adj, cj, cj, s, nullptr, scr);
+            }
+        }
+    });
+}
+
 int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0, double *vals,
              void *vecs, int *nconv, int *n_matvec) {
     return guard([&] {

@@ -424,9 +424,15 @@ struct VecWork;
 // conj_x / conj_y conjugate x on load and y on load and store.
 // dot_partial (optional): matvec_grid(Ns) x 2 doubles receiving per-block
 // FP64 partial sums of <x, y_new> computed before y is rounded to its dtype.
+// xs_scratch (optional): device memory of at least 16 Ns bytes for the
+// pre-scaled x of parent-table sectors; with it the product uses only
+// kernels whose tables qs_op_create built and allocates nothing.
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim,
                    double bre, double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s,
-                   double *dot_partial = nullptr);
+                   double *dot_partial = nullptr, void *xs_scratch = nullptr);
+// Device address of the fixed scratch array of the no-allocation product on
+// the current device (looked up once per device)
+void *noalloc_scratch();
 int matvec_grid(int64_t n);
 // the two-bit hop kernel applies to this operator (plain or adjoint list)
 bool hop_eligible(const OpHost &op, bool adjoint);

@@ -30,6 +30,7 @@
 // memory latency is paid once per batch, and both renumber blocks so each
 // XCD owns a contiguous row range (its own L2 stays warm).
 #include <algorithm>
+#include <mutex>
 
 #include "qs_device.h"
 
@@ -1798,13 +1799,14 @@ void dispatch_hop_cm(const MVArgs &a, int cm, bool track, int ws, const void *x,
 // ordered by that stream alone and products on other streams use their own.
 template <typename V, bool CC>
 void hop_cm(const MVArgs &a, int cm, bool track, int ws, bool ps, const void *x, void *y, const Launch &L,
-            const HopLaunch &H, const OpHost &op) {
+            const HopLaunch &H, const OpHost &op, void *xs_given) {
     if (ps && cm == CM_PTAB) {
         using W = typename Wide<V>::type;
         const int64_t n = a.N;
-        // sized for the widest element so one buffer serves every dtype
+        // sized for the widest element so one buffer serves every dtype (or
+        // the caller's buffer: the fixed scratch of the no-allocation product)
         const size_t bytes = sizeof(cplx_d) * (size_t)std::max<int64_t>(n, 1);
-        W *xs = (W *)op.xs_for(L.s, bytes);
+        W *xs = xs_given ? (W *)xs_given : (W *)op.xs_for(L.s, bytes);
         const int g = (int)std::min<int64_t>((n + 255) / 256, 8192);
         hipLaunchKernelGGL((k_prescale<V>), dim3(std::max(g, 1)), dim3(256), 0, L.s, (const V *)x, a.inv_nrm, xs, n);
         dispatch_hop_cm<V, CC>(a, cm, track, ws, x, xs, y, L, H);
@@ -1814,17 +1816,17 @@ void hop_cm(const MVArgs &a, int cm, bool track, int ws, bool ps, const void *x,
 }
 
 void dispatch_hop(const MVArgs &a, int dtype, bool cc, int cm, bool track, int ws, bool ps, const void *x, void *y,
-                  const Launch &L, const HopLaunch &H, const OpHost &op) {
+                  const Launch &L, const HopLaunch &H, const OpHost &op, void *xs_given) {
     switch (dtype) {
-    case QS_F64: hop_cm<double, false>(a, cm, track, ws, ps, x, y, L, H, op); break;
-    case QS_F32: hop_cm<float, false>(a, cm, track, ws, ps, x, y, L, H, op); break;
+    case QS_F64: hop_cm<double, false>(a, cm, track, ws, ps, x, y, L, H, op, xs_given); break;
+    case QS_F32: hop_cm<float, false>(a, cm, track, ws, ps, x, y, L, H, op, xs_given); break;
     case QS_C128:
-        if (cc) hop_cm<cplx_d, true>(a, cm, track, ws, ps, x, y, L, H, op);
-        else hop_cm<cplx_d, false>(a, cm, track, ws, ps, x, y, L, H, op);
+        if (cc) hop_cm<cplx_d, true>(a, cm, track, ws, ps, x, y, L, H, op, xs_given);
+        else hop_cm<cplx_d, false>(a, cm, track, ws, ps, x, y, L, H, op, xs_given);
         break;
     case QS_C64:
-        if (cc) hop_cm<cplx_f, true>(a, cm, track, ws, ps, x, y, L, H, op);
-        else hop_cm<cplx_f, false>(a, cm, track, ws, ps, x, y, L, H, op);
+        if (cc) hop_cm<cplx_f, true>(a, cm, track, ws, ps, x, y, L, H, op, xs_given);
+        else hop_cm<cplx_f, false>(a, cm, track, ws, ps, x, y, L, H, op, xs_given);
         break;
     default: fail(QS_EARG, "unknown dtype");
     }
@@ -2576,15 +2578,35 @@ void sell_to_csr(const OpHost &op, std::vector<int64_t> &rowptr, std::vector<int
 }
 
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
-                   double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
+                   double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial,
+                   void *xs_scratch) {
     const BasisHost &b = *op.basis;
     MVArgs a = base_args(op, adjoint, are, aim, bre, bim, conj_x, conj_y, dot_partial);
     const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
     if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+    if (xs_scratch) {
+        // the no-allocation product: only kernels whose tables exist from
+        // qs_op_create (no lazy assembly, no split tables); parent-table
+        // sectors pre-scale x into the caller's scratch
+        if (hop_eligible(op, adjoint)) {
+            const size_t lds = hop_args(a, op, adjoint, false);
+            const int cm = hop_mode(b);
+            const size_t wsz = dtype_complex(dtype) ? 16 : 8;
+            const bool ps = cm == CM_PTAB && (uint64_t)b.Ns * wsz < ((uint64_t)1 << 32) &&
+                            (uint64_t)b.ptab.n * 4 < ((uint64_t)1 << 32);
+            Launch L{grid_for(b.Ns, kMvBlock), lds, s, nullptr, nullptr, nullptr};
+            HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(),
+                        adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
+                        adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
+            dispatch_hop(a, dtype, cc, cm, track, 1, ps, x, y, L, H, op, ps ? xs_scratch : nullptr);
+            QS_HIP(hipGetLastError());
+            return;
+        }
+    }
     // an operator marked for assembly is assembled by its first plain product,
     // on the stream that product runs on
-    if (op.want_sell && !adjoint) {
+    if (op.want_sell && !adjoint && !xs_scratch) {
         std::lock_guard<std::mutex> g(op.sell_mutex);
         if (!op.sell) {
             try {
@@ -2595,7 +2617,8 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         }
     }
     // an assembled operator takes its plain products from the stored rows
-    if (op.sell && !adjoint && !dot_partial && (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SELL)) {
+    if (op.sell && !adjoint && !dot_partial && !xs_scratch &&
+        (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SELL)) {
         const SellHost &t = *op.sell;
         if (t.cplx && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
         const int grid = (int)((b.Ns + kSellBlock - 1) / kSellBlock);
@@ -2624,7 +2647,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     // implicit sectors
     const bool want_split = op.lanes_per_row >= QS_VARIANT_SPLIT && op.lanes_per_row <= QS_VARIANT_SPLIT_MAX;
     const bool auto_split = op.lanes_per_row == 0 && b.Ns >= ((int64_t)1 << 20) && split_eligible(op, adjoint);
-    if ((want_split || auto_split) && !dot_partial) {
+    if ((want_split || auto_split) && !dot_partial && !xs_scratch) {
         if (!split_eligible(op, adjoint))
             fail(QS_EARG, "the split matvec variant needs two-bit hop couplings on a basis with a particle number "
                           "and no symmetry or spin inversion only, L <= 32");
@@ -2659,7 +2682,7 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         Launch L{grid, lds, s, nullptr, nullptr, nullptr};
         HopLaunch H{adjoint ? op.d_hops_adj.get() : op.d_hops.get(), adjoint ? op.d_dcls_adj.get() : op.d_dcls.get(),
                     adjoint ? op.d_drest_adj.get() : op.d_drest.get()};
-        dispatch_hop(a, dtype, cc, cm, track, ws, ps, x, y, L, H, op);
+        dispatch_hop(a, dtype, cc, cm, track, ws, ps, x, y, L, H, op, nullptr);
         QS_HIP(hipGetLastError());
         return;
     }
@@ -2686,6 +2709,30 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
     if (b.use64) dispatch_v<uint64_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     else dispatch_v<uint32_t>(a, dtype, cc, b.mode, track, tlds, lpr, x, y, L);
     QS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------- no-allocation product ----
+// The pre-scaled x of qs_op_matvec_noalloc lives in this fixed array, one
+// copy per GPU for the life of the process.  In host code its name stands
+// for no device address: the address comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address is passed on.
+static __device__ __align__(256) char scratch[QS_NOALLOC_SCRATCH_BYTES];
+
+void *noalloc_scratch() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex m;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    QS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(QS_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> g(m);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        QS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(QS_EDEVICE, "no device address for the no-allocation scratch");
+        addr[dev] = p;
+    }
+    return addr[dev];
 }
 
 // ------------------------------------------------------ batched matvec --
