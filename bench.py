@@ -10,9 +10,10 @@ Beside it: the FP64 Lanczos ground state (k=1, tol 1e-10) of the same
 sector, the L=32 z+ FP32-stream / FP64 matvec pair (N = 300,540,195,
 implicit basis; bytes = N * 2 * element), the batched block evolution
 (L=24, all momentum blocks, Neel quench), the serving stream (Poisson, mixed eigsh /
-evolve, solves/s and p50/p99 latency) and the CPU baseline: the oracle's restatement of QuSpin.jl's
-parent-space apply loop (operators.jl:2554-2589), single-threaded, timed on a
-bounded sample of parent states and extrapolated to the whole sector.
+evolve, solves/s and p50/p99 latency) and the CPU baselines: the oracle's restatement of QuSpin.jl's
+whole QuantumLinearOperator product (lift, parent-space apply, project; operators.jl:2591-2623) on
+the host's threads (median of warm products), the same apply loop on one thread over a bounded
+sample, and C1's CSR SpMV and ARPACK eigsh beside the GPU's.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-c3] [--no-c4] [--no-c5]
                        [--no-eigsh]
@@ -175,6 +176,109 @@ def bench_c3_eigsh(op3, n3):
     return out
 
 
+def _median(v):
+    v = sorted(v)
+    n = len(v)
+    return v[n // 2] if n % 2 else 0.5 * (v[n // 2 - 1] + v[n // 2])
+
+
+def bench_cpu(L, terms, bytes_alg, t_mv, args):
+    """CPU baselines: the oracle's restatement of QuSpin.jl's CPU algorithms on this host's cores.
+    (1) the whole C2 product of the reference's QuantumLinearOperator (lift x with P, parent-space
+    apply with one hash lookup per hit, project with P', operators.jl:2591-2623) on the host's
+    threads (at most 16, one GPU's share), median of warm products; (2) the same apply loop on one
+    thread over a bounded sample of parent states; (3) C1: the CSR SpMV of sparse_formats.jl:110-146
+    and eigsh(k=4, :SA) through ARPACK (scipy.sparse.linalg.eigsh, the ARPACK-ng family Arpack.jl
+    wraps), beside the GPU's products and eigsh on the same C1 operator."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+    import oracle
+    import qsgpu
+    out = {}
+    nth = max(1, min(os.cpu_count() or 1, 16))
+    print("bench: CPU baseline, whole C2 product on %d threads" % nth, file=sys.stderr, flush=True)
+    reps = max(1, args.cpu_repeats)
+    ns, tt = oracle.bench_qlo_matvec(L, 14, False, terms, nth, reps, kblock=0, pblock=1, zblock=1)
+    warm = tt
+    tot = [float(a) for a in warm.sum(axis=1)]
+    med = _median(tot)
+    out["cpu_baseline"] = {
+        "value": round(bytes_alg / med / 1e9, 9), "unit": "GB/s", "cores": nth, "kind": "port",
+        "sample": "the whole C2 product (N=%d; lift P x, parent-space apply over all %d parent states with a "
+                  "hash lookup per hit, project P'), median of %d products (work vectors touched beforehand): %.2f s "
+                  "(lift %.3f s, apply %.2f s, project %.3f s); no extrapolation"
+                  % (ns, math.comb(L, 14), reps, med, _median(list(warm[:, 0])), _median(list(warm[:, 1])),
+                     _median(list(warm[:, 2]))),
+        "seconds_per_product": round(med, 4), "samples": reps,
+        "speedup_gpu_over_cpu": round(med / t_mv, 1),
+    }
+    print("bench: CPU baseline, 1 thread, %d-state sample" % args.cpu_sample, file=sys.stderr, flush=True)
+    npar = math.comb(L, 14)
+    ts = oracle.bench_parent_apply(L, 14, False, terms, args.cpu_sample, 5)
+    t_full = ts * npar / args.cpu_sample
+    out["cpu_baseline_1thread"] = {
+        "value": round(bytes_alg / t_full / 1e9, 9), "unit": "GB/s", "cores": 1, "kind": "port",
+        "sample": "the parent-space apply loop (operators.jl:2554-2589, hash lookup) over the first %d of %d "
+                  "parent states, median of 5: %.2f s, scaled by %d / %d to one product (%.1f s; lift and "
+                  "project, about %.2f s on %d threads above, not included)"
+                  % (args.cpu_sample, npar, ts, npar, args.cpu_sample, t_full,
+                     _median(list(warm[:, 0] + warm[:, 2])), nth),
+        "seconds_per_product_extrapolated": round(t_full, 2), "samples": 5,
+        "speedup_gpu_over_cpu": round(t_full / t_mv, 1),
+    }
+    # C1: XXZ L=16 nup=8 pauli=false, J=1, Delta=0.5
+    print("bench: C1 CSR SpMV and ARPACK eigsh (CPU) against the GPU", file=sys.stderr, flush=True)
+    L1 = 16
+    b1 = [(i, i % L1 + 1) for i in range(1, L1 + 1)]
+    terms1 = [("+-", [(0.5, i, j) for i, j in b1]), ("-+", [(0.5, i, j) for i, j in b1]),
+              ("zz", [(0.5, i, j) for i, j in b1])]
+    sec1 = oracle.Sector(L1, nup=8, pauli=False)
+    csr = oracle.CSR(sec1, terms1)
+    rowptr, col, val = csr.export()
+    A = sp.csr_matrix((val, col, rowptr), shape=(sec1.Ns, sec1.Ns))
+    rng = np.random.default_rng(0xC1)
+    x1 = rng.standard_normal(sec1.Ns)
+    x1 /= np.linalg.norm(x1)
+    y1 = np.zeros(sec1.Ns)
+    csr.matvec(x1, y=y1)
+    spmv = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        csr.matvec(x1, y=y1)
+        spmv.append(time.perf_counter() - t0)
+    eig_cpu = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        w = spla.eigsh(A, k=4, which="SA", v0=x1, return_eigenvectors=False)
+        eig_cpu.append(time.perf_counter() - t0)
+    bg = qsgpu.SpinBasis1D(L1, nup=8, pauli=False)
+    opg = qsgpu.QuantumLinearOperator(bg, terms1)
+    xd = torch.from_numpy(x1).cuda()
+    yd = torch.zeros_like(xd)
+    t_gpu_mv, _ = time_matvec(opg, xd, yd, 200, 10)
+    eig_gpu = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        wg = qsgpu.eigsh(opg, k=4, which="SA", v0=x1, return_eigenvectors=False)
+        eig_gpu.append(time.perf_counter() - t0)
+    c1_bytes = sec1.Ns * 16
+    out["c1"] = {
+        "sector_dim": int(sec1.Ns), "csr_nnz": int(csr.nnz),
+        "cpu_csr_spmv_s": _median(spmv), "cpu_csr_spmv_samples": 20, "cpu_threads": 1,
+        "gpu_matvec_s": round(t_gpu_mv, 8), "gpu_matvec_GBps": round(c1_bytes / t_gpu_mv / 1e9, 3),
+        "cpu_arpack_eigsh_k4_s": round(_median(eig_cpu), 5), "cpu_eigsh_samples": 5,
+        "gpu_eigsh_k4_s": round(_median(eig_gpu), 5), "gpu_eigsh_samples": 5,
+        "E_cpu_arpack": sorted(float(v) for v in w), "E_gpu": [float(v) for v in wg],
+        "max_abs_dE": float(np.max(np.abs(np.sort(w) - np.asarray(wg)))),
+        "note": "CPU: the oracle's CSR SpMV (sparse_formats.jl:110-146 restated) and scipy's ARPACK; GPU: the "
+                "matrix-free product and the device thick-restart Lanczos, both on this C1 operator",
+    }
+    return out
+
+
 def bench_c5(n_requests, n_workers, assemble=True):
     """Serving form: Poisson stream (mean 200 req/s, seeded) of 60% eigsh k=1..4 / 40% evolve to
     t ~ U[1,10] on XXZ / J1-J2 chains, L in {16,20,24,28}, nup=L/2, random sectors, HBM budget 200 GB.
@@ -237,7 +341,8 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=200)
     ap.add_argument("--c5-workers", type=int, default=8)
     ap.add_argument("--c5-matrix-free", action="store_true", help="serve without assembling the operators")
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="warm products of the whole-sector CPU baseline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -363,32 +468,7 @@ def main():
     if not args.no_c5:
         result["c5_serving"] = bench_c5(args.c5_requests, args.c5_workers, assemble=not args.c5_matrix_free)
     if not args.no_cpu:
-        import oracle
-        print("bench: CPU baseline (reference parent-apply loop, 1 thread, %d-state sample)" % args.cpu_sample,
-              file=sys.stderr, flush=True)
-        npar = math.comb(L, 14)
-        ts = oracle.bench_parent_apply(L, 14, False, terms, args.cpu_sample, 1)
-        t_full = ts * npar / args.cpu_sample
-        result["cpu_baseline"] = {
-            "value": round(bytes_alg / t_full / 1e9, 9), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": "parent-space apply loop of operators.jl:2554-2589 over the first %d of %d parent states "
-                      "(%.1f s), extrapolated to one full matvec (%.1f s); lift/project SpMVs not counted"
-                      % (args.cpu_sample, npar, ts, t_full),
-            "speedup_gpu_over_cpu": round(t_full / t_mv, 1),
-        }
-        # the same loop on the host's cores (at most 16, the share of one GPU)
-        nth = max(1, min(os.cpu_count() or 1, 16))
-        sample_mt = min(npar, args.cpu_sample * 4)
-        print("bench: CPU baseline, %d threads, %d-state sample" % (nth, sample_mt), file=sys.stderr, flush=True)
-        ts_mt = oracle.bench_parent_apply_mt(L, 14, False, terms, sample_mt, nth)
-        t_full_mt = ts_mt * npar / sample_mt
-        result["cpu_baseline_threads"] = {
-            "value": round(bytes_alg / t_full_mt / 1e9, 9), "unit": "GB/s", "cores": nth, "kind": "port",
-            "sample": "the same loop on %d OpenMP threads (columns split, atomic scatter) over the first %d "
-                      "parent states (%.1f s), extrapolated to one full matvec (%.1f s)"
-                      % (nth, sample_mt, ts_mt, t_full_mt),
-            "speedup_gpu_over_cpu": round(t_full_mt / t_mv, 1),
-        }
+        result.update(bench_cpu(L, terms, bytes_alg, t_mv, args))
     print(json.dumps(result))
 
 

@@ -73,6 +73,8 @@ def lib():
     L.qo_bench_parent_apply.restype = c_dbl
     L.qo_bench_parent_apply_mt.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, c_i64, c_int, c_int]
     L.qo_bench_parent_apply_mt.restype = c_dbl
+    L.qo_bench_qlo_matvec.argtypes = [c_int] * 7 + [vp, vp, vp, vp, c_int, c_int, vp]
+    L.qo_bench_qlo_matvec.restype = c_i64
     L.qo_prim_hypot.argtypes = [c_dbl, c_dbl]
     L.qo_prim_hypot.restype = c_dbl
     L.qo_prim_cdiv.argtypes = [c_dbl, c_dbl, c_dbl, c_dbl, vp]
@@ -281,3 +283,17 @@ def bench_parent_apply_mt(L, nup, pauli, terms, n_sample, threads, repeats=1):
     ops, ncoup, coef, sites = _flatten_terms(terms)
     return lib().qo_bench_parent_apply_mt(L, nup, int(bool(pauli)), len(terms), ops, _ptr(ncoup), _ptr(coef),
                                           _ptr(sites), int(n_sample), int(repeats), int(threads))
+
+
+def bench_qlo_matvec(L, nup, pauli, terms, threads, repeats, kblock=None, pblock=None, zblock=None):
+    """Seconds of `repeats` full products of the reference's QLO (operators.jl:2591-2623: lift, parent
+    apply with a hash lookup per hit, project) on `threads` threads: (Ns, array of shape (repeats, 3)
+    with the lift / apply / project seconds)."""
+    keep = _flatten_terms(terms)
+    ops, ncoup, coef, sites = keep
+    t = np.zeros((repeats, 3))
+    ns = lib().qo_bench_qlo_matvec(L, _u(nup), int(bool(pauli)), _u(kblock), _u(pblock), _u(zblock), len(terms),
+                                   ops, _ptr(ncoup), _ptr(coef), _ptr(sites), int(threads), int(repeats), _ptr(t))
+    if ns < 0:
+        raise MemoryError("qo_bench_qlo_matvec failed")
+    return int(ns), t

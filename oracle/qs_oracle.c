@@ -1041,6 +1041,55 @@ void qo_csr_export(const qo_csr *m, int64_t *rowptr, int64_t *col, double *val)
  * extrapolated full-sector time is a lower bound on the reference's cost.
  * Returns seconds for the sample (best of `repeats`). */
 #include <time.h>
+/* ------------------------------------------- full QLO matvec baseline -- */
+/* Open-addressing hash state -> parent index (the reference's Dict lookup,
+ * basis.jl:671): linear probing, load <= 1/2, Fibonacci hashing. */
+typedef struct {
+    uint64_t *key;
+    int64_t *val;
+    uint64_t mask;
+} hash_t;
+
+static int hash_build(hash_t *h, const uint64_t *states, int64_t n)
+{
+    uint64_t cap = 1;
+    while (cap < (uint64_t)(2 * n + 2)) cap <<= 1;
+    h->mask = cap - 1;
+    h->key = malloc(sizeof(uint64_t) * cap);
+    h->val = malloc(sizeof(int64_t) * cap);
+    if (!h->key || !h->val) return -1;
+    for (uint64_t i = 0; i < cap; i++) h->key[i] = ~0ULL;
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t p = (states[i] * 0x9E3779B97F4A7C15ULL) & h->mask;
+        while (h->key[p] != ~0ULL) p = (p + 1) & h->mask;
+        h->key[p] = states[i];
+        h->val[p] = i;
+    }
+    return 0;
+}
+
+static inline int64_t hash_get(const hash_t *h, uint64_t s)
+{
+    uint64_t p = (s * 0x9E3779B97F4A7C15ULL) & h->mask;
+    for (;;) {
+        uint64_t k = h->key[p];
+        if (k == s) return h->val[p];
+        if (k == ~0ULL) return -1;
+        p = (p + 1) & h->mask;
+    }
+}
+
+static uint64_t rev_bits_u64(uint64_t u, int L)
+{
+    u = ((u >> 1) & 0x5555555555555555ULL) | ((u & 0x5555555555555555ULL) << 1);
+    u = ((u >> 2) & 0x3333333333333333ULL) | ((u & 0x3333333333333333ULL) << 2);
+    u = ((u >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((u & 0x0F0F0F0F0F0F0F0FULL) << 4);
+    u = ((u >> 8) & 0x00FF00FF00FF00FFULL) | ((u & 0x00FF00FF00FF00FFULL) << 8);
+    u = ((u >> 16) & 0x0000FFFF0000FFFFULL) | ((u & 0x0000FFFF0000FFFFULL) << 16);
+    u = (u >> 32) | (u << 32);
+    return u >> (64 - L);
+}
+
 static double bench_parent_apply(int L, int nup, int pauli, int n_terms, const char **ops, const int *ncoup,
                                  const double *coef, const int *sites, int64_t n_sample, int repeats, int threads)
 {
@@ -1066,7 +1115,9 @@ static double bench_parent_apply(int L, int nup, int pauli, int n_terms, const c
             cs[k].coef = c_make(coef[2 * k], coef[2 * k + 1]);
         }
     }
-    double best = 1e300;
+    hash_t hh;
+    if (hash_build(&hh, states, npar)) { free(states); free(x); free(y); free(cs); return -1; }
+    double *ts = malloc(sizeof(double) * (size_t)(repeats > 0 ? repeats : 1));
     for (int r = 0; r < repeats; r++) {
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
@@ -1076,7 +1127,7 @@ static double bench_parent_apply(int L, int nup, int pauli, int n_terms, const c
                     uint64_t out;
                     cplx amp;
                     if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
-                    int64_t row = lookup_state(states, npar, out);
+                    int64_t row = hash_get(&hh, out);
                     if (row < 0) continue;
                     cplx in = x[col];
                     if (in.re == 0 && in.im == 0) continue;
@@ -1094,7 +1145,7 @@ static double bench_parent_apply(int L, int nup, int pauli, int n_terms, const c
                     uint64_t out;
                     cplx amp;
                     if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
-                    int64_t row = lookup_state(states, npar, out);
+                    int64_t row = hash_get(&hh, out);
                     if (row < 0) continue;
                     cplx p = c_mul(amp, in);
 #pragma omp atomic
@@ -1105,17 +1156,185 @@ static double bench_parent_apply(int L, int nup, int pauli, int n_terms, const c
             }
         }
         clock_gettime(CLOCK_MONOTONIC, &t1);
-        double dt = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
-        if (dt < best) best = dt;
+        ts[r] = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     }
+    /* median of the repeats */
+    for (int a = 1; a < repeats; a++)
+        for (int b = a; b > 0 && ts[b - 1] > ts[b]; b--) { double t = ts[b]; ts[b] = ts[b - 1]; ts[b - 1] = t; }
+    double med = repeats > 0 ? (repeats % 2 ? ts[repeats / 2] : 0.5 * (ts[repeats / 2 - 1] + ts[repeats / 2])) : 0.0;
     volatile double sink = y[0].re;
     (void)sink;
-    free(states); free(x); free(y); free(cs);
-    return best;
+    free(states); free(x); free(y); free(cs); free(ts); free(hh.key); free(hh.val);
+    return med;
+}
+
+/* Timed full product y = P' H_par P x of the reference's QLO in a k / p / z
+ * sector (operators.jl:2591-2623: lift, parent apply with a Dict lookup per
+ * (state, coupling), project), ComplexF64 throughout.  The projector columns
+ * are built from the group orbits directly (ascending orbit minima, entries
+ * sum_g conj(chi(g)) / norm; equal to the reference's stage construction up
+ * to rounding), not timed.  Writes the seconds of each of `repeats` products
+ * to times[] (times[3 r + 0..2] = lift, apply, project); returns the sector
+ * dimension, or -1. */
+int64_t qo_bench_qlo_matvec(int L, int nup, int pauli, int kblock, int pblock, int zblock, int n_terms,
+                            const char **ops, const int *ncoup, const double *coef, const int *sites, int threads,
+                            int repeats, double *times)
+{
+    int64_t npar = qo_fixed_weight_states(L, nup, NULL);
+    uint64_t *states = malloc(sizeof(uint64_t) * (size_t)npar);
+    if (!states) return -1;
+    qo_fixed_weight_states(L, nup, states);
+    hash_t h;
+    if (hash_build(&h, states, npar)) return -1;
+    const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
+    /* group: T^t o X_q, X_q in {1, P, Z, PZ} as requested */
+    const int nT = kblock == QO_UNSET ? 1 : L;
+    int npt = 1;
+    int prev[4] = {0}, pfl[4] = {0};
+    double pch[4] = {1, 0, 0, 0};
+    if (pblock != QO_UNSET) { prev[npt] = 1; pfl[npt] = 0; pch[npt] = pblock; npt++; }
+    if (zblock != QO_UNSET) {
+        int m = npt;
+        for (int q = 0; q < m; q++) { prev[npt] = prev[q]; pfl[npt] = 1; pch[npt] = pch[q] * zblock; npt++; }
+    }
+    /* column pointers of P: orbit by orbit in ascending order of the minimum */
+    unsigned char *seen = calloc((size_t)npar, 1);
+    int64_t *colptr = malloc(sizeof(int64_t) * (size_t)(npar + 1));
+    int64_t *prow = malloc(sizeof(int64_t) * (size_t)npar);
+    cplx *pval = malloc(sizeof(cplx) * (size_t)npar);
+    if (!seen || !colptr || !prow || !pval) return -1;
+    int64_t ncol = 0, nnz = 0;
+    int G = nT * npt;
+    int64_t *orb = malloc(sizeof(int64_t) * (size_t)G);
+    cplx *acc = malloc(sizeof(cplx) * (size_t)G);
+    for (int64_t i = 0; i < npar; i++) {
+        if (seen[i]) continue;
+        int no = 0;
+        for (int t = 0; t < nT; t++)
+            for (int q = 0; q < npt; q++) {
+                uint64_t u = states[i];
+                if (prev[q]) u = rev_bits_u64(u, L);
+                if (pfl[q]) u = ~u & full;
+                if (t) u = ((u << t) | (u >> (L - t))) & full;
+                int64_t r = hash_get(&h, u);
+                /* character of T^t X_q: exp(-2 pi i k t / L) * pch */
+                double ang = kblock == QO_UNSET ? 0.0 : -2.0 * M_PI * (double)kblock * (double)t / (double)L;
+                cplx ch = c_make(pch[q] * cos(ang), -pch[q] * sin(ang));   /* conj(chi) */
+                int j = 0;
+                while (j < no && orb[j] != r) j++;
+                if (j == no) { orb[no] = r; acc[no] = c_make(0, 0); no++; }
+                acc[j] = c_add(acc[j], ch);
+            }
+        double n2 = 0;
+        for (int j = 0; j < no; j++) { seen[orb[j]] = 1; n2 += c_abs2(acc[j]); }
+        if (n2 < 1e-18) continue;
+        double inv = 1.0 / sqrt(n2);
+        colptr[ncol] = nnz;
+        /* rows ascending, as a CSC column */
+        for (int a = 1; a < no; a++)
+            for (int b = a; b > 0 && orb[b - 1] > orb[b]; b--) {
+                int64_t tr = orb[b]; orb[b] = orb[b - 1]; orb[b - 1] = tr;
+                cplx tc = acc[b]; acc[b] = acc[b - 1]; acc[b - 1] = tc;
+            }
+        for (int j = 0; j < no; j++) {
+            prow[nnz] = orb[j];
+            pval[nnz] = c_make(acc[j].re * inv, acc[j].im * inv);
+            nnz++;
+        }
+        ncol++;
+    }
+    colptr[ncol] = nnz;
+    free(seen); free(orb); free(acc);
+    /* couplings */
+    int total = 0;
+    for (int t = 0; t < n_terms; t++) total += ncoup[t];
+    qo_coupling *cs = calloc((size_t)(total ? total : 1), sizeof *cs);
+    int k = 0, sp = 0;
+    for (int t = 0; t < n_terms; t++) {
+        int len = (int)strlen(ops[t]);
+        for (int j = 0; j < ncoup[t]; j++, k++) {
+            strcpy(cs[k].op, ops[t]);
+            cs[k].nops = len;
+            for (int i = 0; i < len; i++) cs[k].sites[i] = sites[sp++];
+            cs[k].coef = c_make(coef[2 * k], coef[2 * k + 1]);
+        }
+    }
+    cplx *x = malloc(sizeof(cplx) * (size_t)(ncol ? ncol : 1));
+    cplx *y = malloc(sizeof(cplx) * (size_t)(ncol ? ncol : 1));
+    cplx *lifted = malloc(sizeof(cplx) * (size_t)npar);
+    cplx *acted = malloc(sizeof(cplx) * (size_t)npar);
+    if (!x || !y || !lifted || !acted) return -1;
+    for (int64_t c = 0; c < ncol; c++) x[c] = c_make(1.0 / (double)(1 + (c % 89)), 0.0);
+    if (threads < 1) threads = 1;
+    /* first touch of the work vectors outside the timed region */
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < npar; i++) { lifted[i] = c_make(0, 0); acted[i] = c_make(0, 0); }
+    for (int r = 0; r < repeats; r++) {
+        struct timespec t0, t1, t2, t3;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        /* lifted = P x: columns own disjoint rows */
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t i = 0; i < npar; i++) lifted[i] = c_make(0, 0);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096)
+        for (int64_t c = 0; c < ncol; c++)
+            for (int64_t q = colptr[c]; q < colptr[c + 1]; q++) lifted[prow[q]] = c_mul(pval[q], x[c]);
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        /* acted = H_par lifted: the push loop with one Dict lookup per hit */
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t i = 0; i < npar; i++) acted[i] = c_make(0, 0);
+        if (threads == 1) {
+            for (int i = 0; i < total; i++)
+                for (int64_t col = 0; col < npar; col++) {
+                    cplx in = lifted[col];
+                    if (in.re == 0 && in.im == 0) continue;
+                    uint64_t out;
+                    cplx amp;
+                    if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
+                    int64_t row = hash_get(&h, out);
+                    if (row < 0) continue;
+                    acted[row] = c_add(acted[row], c_mul(amp, in));
+                }
+        } else {
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int64_t col = 0; col < npar; col++) {
+                cplx in = lifted[col];
+                if (in.re == 0 && in.im == 0) continue;
+                for (int i = 0; i < total; i++) {
+                    uint64_t out;
+                    cplx amp;
+                    if (apply_coupling(pauli, &cs[i], states[col], &out, &amp) != 1) continue;
+                    int64_t row = hash_get(&h, out);
+                    if (row < 0) continue;
+                    cplx pr = c_mul(amp, in);
+#pragma omp atomic
+                    acted[row].re += pr.re;
+#pragma omp atomic
+                    acted[row].im += pr.im;
+                }
+            }
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t2);
+        /* y = P' acted */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096)
+        for (int64_t c = 0; c < ncol; c++) {
+            cplx sum = c_make(0, 0);
+            for (int64_t q = colptr[c]; q < colptr[c + 1]; q++) sum = c_add(sum, c_mul(c_conj(pval[q]), acted[prow[q]]));
+            y[c] = sum;
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t3);
+        times[3 * r + 0] = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+        times[3 * r + 1] = (double)(t2.tv_sec - t1.tv_sec) + 1e-9 * (double)(t2.tv_nsec - t1.tv_nsec);
+        times[3 * r + 2] = (double)(t3.tv_sec - t2.tv_sec) + 1e-9 * (double)(t3.tv_nsec - t2.tv_nsec);
+    }
+    volatile double sink = ncol ? y[0].re : 0.0;
+    (void)sink;
+    free(states); free(h.key); free(h.val); free(colptr); free(prow); free(pval); free(cs);
+    free(x); free(y); free(lifted); free(acted);
+    return ncol;
 }
 
 /* Seconds of the reference's parent-space apply loop (operators.jl:2554-2589)
-   over the first n_sample parent states, best of `repeats`: one thread. */
+   over the first n_sample parent states, median of `repeats`: one thread. */
 double qo_bench_parent_apply(int L, int nup, int pauli, int n_terms, const char **ops, const int *ncoup,
                              const double *coef, const int *sites, int64_t n_sample, int repeats)
 {
