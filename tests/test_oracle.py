@@ -167,3 +167,13 @@ def test_cpu_baseline_loops_run_on_one_and_several_threads():
     t1 = oracle.bench_parent_apply(12, 6, False, terms, 500, 1)
     t4 = oracle.bench_parent_apply_mt(12, 6, False, terms, 500, 4)
     assert t1 > 0 and t4 > 0
+
+
+def test_whole_product_baseline_builds_the_reference_sector():
+    # the timed CPU baseline builds its projector from group orbits: its
+    # column count equals the oracle's stage construction for k / p / z sectors
+    for L, blocks in ((12, dict(kblock=0, pblock=1, zblock=1)), (12, dict(kblock=3)), (14, dict(zblock=-1)),
+                      (12, dict(kblock=6, pblock=-1)), (10, {})):
+        ns, t = oracle.bench_qlo_matvec(L, L // 2, False, _dense.xxz_terms(L, J2=0.4), 2, 2, **blocks)
+        assert ns == oracle.Sector(L, nup=L // 2, pauli=False, **blocks).Ns
+        assert t.shape == (2, 3) and np.all(t >= 0)
