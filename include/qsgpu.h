@@ -238,6 +238,23 @@ int qs_eigsh(qs_op *op, int dtype, int k, int which, double tol, int maxiter, in
 int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const double *scales,
                      double tol, int krylov_dim, void *out, int *n_matvec);
 
+/* evolve(H, v0, t0, times; imag_time, max_step, rtol, atol) for a
+ * Hamiltonian with dynamic terms, H(t) = h0 + sum_j f_j(t) dyn[j] --
+ * operators.jl:2092-2128 with the adaptive Dormand-Prince step
+ * _rk45_interval (operators.jl:1964-2071), Schroedinger equation
+ * dv/dt = -i H(t) v (imag_time: -H(t) v, each state renormalised per
+ * column).  The drive values come from `drive(t, n_dyn, coef, user)`, which
+ * writes f_j(t) as 2 * n_dyn interleaved (re, im) doubles; it is called on the
+ * calling thread once per stage time.  Every operator acts on the basis of
+ * h0.  dtype QS_C128 (the reference's ComplexF64 state) or QS_C64.  v0: host,
+ * Ns x ncols column-major; out: host, Ns x ncols x n_times.  times must be
+ * sorted and not precede t0.  *n_steps (may be NULL) = attempted steps,
+ * *n_matvec (may be NULL) = operator products. */
+typedef void (*qs_drive_fn)(double t, int n_dyn, double *coef, void *user);
+int qs_evolve_dynamic(qs_op *h0, int n_dyn, qs_op *const *dyn, qs_drive_fn drive, void *user, int dtype,
+                      const void *v0, int64_t ncols, double t0, int64_t n_times, const double *times, int imag_time,
+                      double rtol, double atol, double max_step, void *out, int64_t *n_steps, int64_t *n_matvec);
+
 /* lanczos_full(op, v0, m; full_ortho, eps) and the coefficients of
  * lanczos_iter -- tools.jl:2742-2850: the reference's three-term recurrence
  * on the device.  v0: host Ns elements of dtype; eps <= 0 selects the machine

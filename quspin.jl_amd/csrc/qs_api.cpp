@@ -881,6 +881,30 @@ int qs_expm_multiply(qs_op *op, int dtype, const void *v, int64_t n_times, const
     });
 }
 
+int qs_evolve_dynamic(qs_op *h0, int n_dyn, qs_op *const *dyn, qs_drive_fn drive, void *user, int dtype,
+                      const void *v0, int64_t ncols, double t0, int64_t n_times, const double *times, int imag_time,
+                      double rtol, double atol, double max_step, void *out, int64_t *n_steps, int64_t *n_matvec) {
+    return guard([&] {
+        check_ptr(h0, "h0");
+        check_ptr(v0, "v0");
+        if (n_dyn < 0) fail(QS_EARG, "n_dyn must be non-negative");
+        if (n_times < 0) fail(QS_EARG, "n_times must be non-negative");
+        if (n_times > 0) {
+            check_ptr(times, "times");
+            check_ptr(out, "out");
+        }
+        std::vector<const OpHost *> d;
+        for (int j = 0; j < n_dyn; j++) {
+            check_ptr(dyn[j], "dyn[j]");
+            d.push_back(&dyn[j]->o);
+        }
+        ensure_stream(h0->o);
+        std::vector<double> ts(times, times + n_times);
+        evolve_dynamic_host(h0->o, d, drive, user, dtype, v0, ncols, t0, ts, imag_time, rtol, atol, max_step, out,
+                            n_steps, n_matvec);
+    });
+}
+
 int qs_lanczos(qs_op *op, int dtype, const void *v0, int m, int full_ortho, double eps, double *alphas,
                double *betas, double *E, double *V, void *Q_T, int *used) {
     return guard([&] {

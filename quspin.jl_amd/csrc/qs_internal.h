@@ -530,6 +530,12 @@ void eigsh_host(OpHost &o, hipStream_t s, int dtype, int k, int which, double to
 void expm_host(OpHost &o, hipStream_t s, int dtype, const void *v, int64_t n_times, const double *scales, double tol,
                int krylov_dim, void *out, int *n_matvec);
 
+// evolve with dynamic terms (qs_rk45.hip): the reference's adaptive RK45
+void evolve_dynamic_host(const OpHost &h0, const std::vector<const OpHost *> &dyn, qs_drive_fn drive, void *user,
+                         int dtype, const void *v0, int64_t ncols, double t0, const std::vector<double> &times,
+                         int imag_time, double rtol, double atol, double max_step, void *out, int64_t *n_steps,
+                         int64_t *n_matvec);
+
 // vector kernels (qs_vec.hip); all reductions are deterministic (fixed order)
 struct VecWork {
     explicit VecWork(hipStream_t st) : s(st) {}

@@ -14,6 +14,7 @@ from .basis import SpinBasis1D  # noqa: F401
 from .operators import (  # noqa: F401
     OperatorTerm,
     QuantumLinearOperator,
+    Hamiltonian,
     eigsh,
     evolve,
     expm_multiply,

@@ -369,9 +369,116 @@ def expm_multiply(op, v, scales, tol=1e-12, krylov_dim=30, dtype=np.complex128):
     return out
 
 
-def evolve(op, v0, t0, times, imag_time=False, iterate=False, tol=1e-12, krylov_dim=30, dtype=np.complex128):
-    """evolve(H, v0, t0, times) for a static generator -- operators.jl:2073-2192 (:SE branch)."""
+class Hamiltonian:
+    """Hamiltonian(static_list, dynamic_list; basis, check_*) -- operators.jl:578-614, 648-737, matrix-free:
+    the static terms form one QuantumLinearOperator on `basis` and each dynamic entry
+    ``[op, couplings, f, f_args]`` another, whose product is scaled by ``f(time, *f_args)``."""
+
+    def __init__(self, static_list, dynamic_list=(), basis=None, check_symm=True, check_herm=True, check_pcon=True):
+        if basis is None:
+            raise ArgumentError("a basis is required")
+        self.basis = basis
+        kw = dict(check_symm=check_symm, check_herm=check_herm, check_pcon=check_pcon)
+        self.static = QuantumLinearOperator(basis, list(static_list), **kw)
+        # entries sharing (f, f_args) form one operator, checked as one
+        # collection (_dynamic_groups, discrete_basis.jl:2260-2309)
+        groups = []
+        for entry in dynamic_list:
+            if len(entry) != 4:
+                raise ArgumentError("dynamic operator entries are [op, couplings, f, f_args]")
+            op, couplings, f, args = entry
+            key = (f, tuple(args))
+            for g in groups:
+                if g[0][0] is key[0] and g[0][1] == key[1]:
+                    g[1].append((op, couplings))
+                    break
+            else:
+                groups.append((key, [(op, couplings)]))
+        self.dynamic = [(QuantumLinearOperator(basis, terms, **kw), key[0], key[1]) for key, terms in groups]
+
+    @property
+    def Ns(self):
+        return self.basis.Ns
+
+    @property
+    def shape(self):
+        return (self.Ns, self.Ns)
+
+    def apply(self, v, time=0.0):
+        """H(time) v (operators.jl:1383-1420)."""
+        out = np.asarray(self.static.matvec(np.asarray(v, dtype=np.complex128)), dtype=np.complex128)
+        for op, f, args in self.dynamic:
+            c = f(time, *args)
+            if c == 0:
+                continue
+            out = out + c * op.matvec(np.asarray(v, dtype=np.complex128))
+        return out
+
+
+_DRIVE = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+
+
+def _evolve_dynamic(H, v0, t0, targets, imag_time, max_step, rtol, atol, dtype):
+    # operators.jl:2092-2128 on the device (qs_evolve_dynamic)
+    dt = np.dtype(dtype)
+    if not np.issubdtype(dt, np.complexfloating):
+        raise ArgumentError("evolve needs a complex dtype")
+    v = np.asarray(v0)
+    if v.shape[0] != H.Ns:
+        raise DimensionMismatch("Hamiltonian and value dimensions do not match")
+    ncols = 1 if v.ndim == 1 else v.shape[1]
+    vv = np.asfortranarray(v.astype(dt).reshape(H.Ns, ncols))
+    errors = []
+
+    def drive(t, n, coef, _user):
+        try:
+            for j, (_, f, args) in enumerate(H.dynamic):
+                c = complex(f(t, *args))
+                coef[2 * j] = c.real
+                coef[2 * j + 1] = c.imag
+        except Exception as e:   # noqa: BLE001 -- reported after the call
+            errors.append(e)
+            for j in range(n):
+                coef[2 * j] = coef[2 * j + 1] = 0.0
+
+    cb = _DRIVE(drive)
+    ops = (ctypes.c_void_p * max(1, len(H.dynamic)))(*[op._h for op, _, _ in H.dynamic])
+    tt = np.ascontiguousarray(targets, dtype=np.float64)
+    out = np.zeros((H.Ns, ncols, tt.size), dtype=dt, order="F")
+    steps, nmv = ctypes.c_int64(), ctypes.c_int64()
+    st = _lib.lib().qs_evolve_dynamic(H.static._h, len(H.dynamic), ops, cb, None, _DT[dt], _ptr(vv), ncols,
+                                       float(t0), tt.size, _ptr(tt), int(bool(imag_time)), float(rtol), float(atol),
+                                       float(max_step), _ptr(out), ctypes.byref(steps), ctypes.byref(nmv))
+    if errors:
+        raise errors[0]
+    check(st)
+    evolve.last_n_steps = steps.value
+    evolve.last_n_matvec = nmv.value
+    return out if v.ndim == 2 else out[:, 0, :]
+
+
+def evolve(op, v0, t0, times, imag_time=False, iterate=False, tol=None, krylov_dim=30, dtype=np.complex128,
+           max_step=0.01, rtol=None, atol=1e-11):
+    """evolve(H, v0, t0, times) -- operators.jl:2073-2192 (:SE branch).  A static generator
+    (QuantumLinearOperator, or a Hamiltonian without dynamic terms) takes the Krylov exponential;
+    a Hamiltonian with dynamic terms the adaptive RK45 integrator (max_step, rtol = tol or 1e-9,
+    atol)."""
     scalar = isinstance(times, numbers.Number)
+    if isinstance(op, Hamiltonian):
+        if op.dynamic:
+            if not max_step > 0:
+                raise ArgumentError("max_step must be positive")
+            targets = np.atleast_1d(np.asarray(times, dtype=np.float64))
+            r = rtol if rtol is not None else (tol if tol is not None else 1e-9)
+            states = _evolve_dynamic(op, v0, t0, targets, imag_time, max_step, r, atol, dtype)
+            if iterate:
+                return (states[..., j].copy() for j in range(states.shape[-1]))
+            if scalar:
+                return states[..., 0].copy()
+            return states
+        op = op.static
+    if tol is None:
+        tol = 1e-12
     targets = np.atleast_1d(np.asarray(times, dtype=np.float64))
     offsets = targets - float(t0)
     scales = -offsets.astype(np.complex128) if imag_time else -1j * offsets
