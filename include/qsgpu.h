@@ -218,8 +218,12 @@ int qs_op_matvec_noalloc(qs_op *op, int dtype, const void *x, void *y, int64_t n
                          double alpha_im, double beta_re, double beta_im, int transposed, int conjugated);
 
 /* eigsh(op; k, which, tol, maxiter, ncv, v0, return_eigenvectors) --
- * operators.jl:3367-3408 (ARPACK "SA"/"LA"/... semantics).  Thick-restart
- * Lanczos with full FP64-accumulated reorthogonalisation on the device.
+ * operators.jl:3367-3408 (ARPACK "SA"/"LA"/... semantics).  Hermitian
+ * operators: thick-restart Lanczos with full FP64-accumulated
+ * reorthogonalisation on the device.  Non-Hermitian operators (Arpack.eigs'
+ * complex path, which SA -> SR, LA -> LR, SM, LM; BE is rejected as ARPACK
+ * rejects it): Krylov-Schur restarted Arnoldi, complex dtypes only, complex
+ * Ritz values ordered as operators.jl:3393-3401.
  *   dtype  QS_F64/QS_C128: FP64 vectors; QS_F32/QS_C64: FP32-stored Krylov
  *          basis with FP64 dots, norms and tridiagonal solve.
  *   tol    <= 0 selects machine epsilon (ARPACK default)

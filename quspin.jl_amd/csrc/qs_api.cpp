@@ -261,6 +261,76 @@ void eigsh_host(OpHost &o, hipStream_t s, int dtype, int k, int which, double to
     }
     if (vecs) dvec.alloc((size_t)n * k * esz);
     EigshResult res;
+    if (!o.hermitian) {
+        // Arpack.eigs' complex non-symmetric path (operators.jl:3383-3390);
+        // k >= Ns - 1 takes the dense eigen decomposition (:3379-3382)
+        if (!dtype_complex(dtype)) fail(QS_EARG, "a non-Hermitian operator needs a complex vector dtype");
+        if (which == QS_WHICH_BE) fail(QS_EARG, "which=:BE only possible for real symmetric problem");
+        if (k >= n - 1) {
+            const int nn = (int)n;
+            DevBuf<char> E((size_t)nn * nn * 16), Y((size_t)nn * nn * 16);
+            std::vector<cd> eh((size_t)nn * nn, cd(0, 0));
+            for (int j = 0; j < nn; j++) eh[(size_t)j * nn + j] = 1.0;
+            QS_HIP(hipMemcpyAsync(E.get(), eh.data(), eh.size() * 16, hipMemcpyHostToDevice, s));
+            for (int j = 0; j < nn; j++)
+                launch_matvec(o, QS_C128, E.get() + (size_t)j * nn * 16, Y.get() + (size_t)j * nn * 16, 1, 0, 0, 0,
+                              false, false, false, s);
+            QS_HIP(hipMemcpyAsync(eh.data(), Y.get(), eh.size() * 16, hipMemcpyDeviceToHost, s));
+            QS_HIP(hipStreamSynchronize(s));
+            std::vector<cd> A((size_t)nn * nn), w, Vv;
+            for (int i = 0; i < nn; i++)
+                for (int j = 0; j < nn; j++) A[(size_t)i * nn + j] = eh[(size_t)j * nn + i];
+            gen_eig(nn, A, w, Vv);
+            std::vector<int> idx(nn);
+            for (int i = 0; i < nn; i++) idx[i] = i;
+            auto key = [&](int i) {
+                switch (which) {
+                case QS_WHICH_LA: return -w[i].real();
+                case QS_WHICH_SM: return std::abs(w[i]);
+                case QS_WHICH_LM: return -std::abs(w[i]);
+                default: return w[i].real();
+                }
+            };
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b2) { return key(a) < key(b2); });
+            for (int i = 0; i < k; i++) {
+                vals[2 * i] = w[idx[i]].real();
+                vals[2 * i + 1] = w[idx[i]].imag();
+            }
+            if (vecs) {
+                std::vector<char> vh((size_t)n * k * esz);
+                for (int c = 0; c < k; c++) {
+                    double nrm = 0;
+                    for (int r = 0; r < nn; r++) nrm += std::norm(Vv[(size_t)r * nn + idx[c]]);
+                    nrm = std::sqrt(nrm);
+                    for (int r = 0; r < nn; r++) {
+                        const cd z = Vv[(size_t)r * nn + idx[c]] / nrm;
+                        const size_t off = (size_t)c * n + r;
+                        if (dtype == QS_C128) {
+                            ((double *)vh.data())[2 * off] = z.real();
+                            ((double *)vh.data())[2 * off + 1] = z.imag();
+                        } else {
+                            ((float *)vh.data())[2 * off] = (float)z.real();
+                            ((float *)vh.data())[2 * off + 1] = (float)z.imag();
+                        }
+                    }
+                }
+                memcpy(vecs, vh.data(), vh.size());
+            }
+            if (nconv) *nconv = k;
+            if (n_matvec) *n_matvec = nn;
+            return;
+        }
+        eigs_device(o, dtype, k, which, tol, maxiter, ncv, v0 ? dv0.get() : nullptr, res, vecs ? dvec.get() : nullptr);
+        for (int i = 0; i < 2 * k; i++) vals[i] = res.vals[i];
+        if (vecs) QS_HIP(hipMemcpyAsync(vecs, dvec.get(), (size_t)n * k * esz, hipMemcpyDeviceToHost, s));
+        QS_HIP(hipStreamSynchronize(s));
+        if (nconv) *nconv = res.nconv;
+        if (n_matvec) *n_matvec = res.nmatvec;
+        if (res.nconv < k)
+            fail(QS_ENOCONV, "eigsh: only " + std::to_string(res.nconv) + " of " + std::to_string(k) +
+                                 " Ritz values converged within maxiter");
+        return;
+    }
     if (k >= n - 1) {
         // operators.jl:3379-3382: dense fallback for tiny problems
         int nn = (int)n;

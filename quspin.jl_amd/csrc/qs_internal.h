@@ -631,6 +631,13 @@ struct EigshResult {
     std::vector<double> vals;    // k (re) or 2k (re, im)
     int nconv = 0, nmatvec = 0;
 };
+// non-Hermitian operators: Krylov-Schur Arnoldi (qs_arnoldi.cpp); complex
+// dtypes only, vals = 2k interleaved (re, im)
+void eigs_device(OpHost &op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0_dev,
+                 EigshResult &res, void *vecs_dev);
+// eigenvalues w and right eigenvectors V (row-major, column i for w[i]) of a
+// general row-major n x n complex matrix
+void gen_eig(int n, const std::vector<cd> &A, std::vector<cd> &w, std::vector<cd> &V);
 void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0_dev,
                   EigshResult &res, void *vecs_dev);
 // out_dev: Ns x n_times (column-major) of the complex dtype

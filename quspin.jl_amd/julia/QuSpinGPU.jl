@@ -14,7 +14,9 @@ module QuSpinGPU
 
 using LinearAlgebra
 using SparseArrays
-import QuSpin: SpinBasis1D, OperatorTerm, eigsh, evolve
+import QuSpin: SpinBasis1D, OperatorTerm, QuantumLinearOperator, eigsh, evolve
+
+export GPUBasis, GPUQuantumLinearOperator, GPUHamiltonian, GPUBlockOps, assemble!, block_expm
 
 const libqsgpu = get(ENV, "QSGPU_LIB", joinpath(@__DIR__, "..", "_build", "libqsgpu.so"))
 
@@ -63,6 +65,26 @@ function GPUBasis(L::Integer; nup=nothing, pauli::Bool=true, a::Integer=1, kbloc
     check(ccall((:qs_basis_size, libqsgpu), Cint, (Ptr{Cvoid}, Ref{Int64}), h[], n))
     b = GPUBasis(h[], Int(L), Int(n[]))
     finalizer(x -> ccall((:qs_basis_destroy, libqsgpu), Cint, (Ptr{Cvoid},), x.handle), b)
+    return b
+end
+
+"""
+    GPUBasis(basis::SpinBasis1D)
+
+The GPU sector of a reference basis, from the keywords the reference stores:
+`L`, `nup` and `pauli` (basis.jl:206-213) and the symmetry blocks it records
+in `basis.symmetry.blocks` (`:a`, `:kblock`, `:pblock`, `:zblock`, `:pzblock`,
+`:zAblock`, `:zBblock`; basis.jl:671-760).  The GPU enumerates the same
+sector, so its dimension must equal `length(basis)`; a mismatch raises.
+"""
+function GPUBasis(basis::SpinBasis1D)
+    blk = basis.symmetry.blocks
+    g(name) = get(blk, name, nothing)
+    b = GPUBasis(getfield(basis, :L); nup=getfield(basis, :nup), pauli=getfield(basis, :pauli),
+                 a=something(g(:a), 1), kblock=g(:kblock), pblock=g(:pblock), zblock=g(:zblock),
+                 pzblock=g(:pzblock), zAblock=g(:zAblock), zBblock=g(:zBblock))
+    b.Ns == length(basis) ||
+        throw(DimensionMismatch("GPU sector has $(b.Ns) states, the reference basis $(length(basis))"))
     return b
 end
 
@@ -115,6 +137,37 @@ function GPUQuantumLinearOperator(basis::GPUBasis, terms::AbstractVector{<:Opera
     return op
 end
 
+# the reference's constructor signature, QuantumLinearOperator(basis, static_list; ...)
+# (operators.jl:2492-2552), on its own SpinBasis1D: unchanged host code
+# replaces `QuantumLinearOperator` by `GPUQuantumLinearOperator`
+GPUQuantumLinearOperator(basis::SpinBasis1D, terms::AbstractVector{<:OperatorTerm}; kwargs...) =
+    GPUQuantumLinearOperator(GPUBasis(basis), terms; kwargs...)
+
+# an existing reference operator: its basis, static list and user diagonal
+# (fields of QuantumLinearOperator, operators.jl:2331-2345); the reference
+# already checked the list, so the checks are not repeated
+GPUQuantumLinearOperator(op::QuantumLinearOperator) =
+    GPUQuantumLinearOperator(GPUBasis(op.basis), op.static_list;
+                             diagonal=any(!iszero, op.diagonal) ? op.diagonal : nothing,
+                             check_symm=false, check_herm=false, check_pcon=false)
+
+"""
+    GPUHamiltonian(basis, terms; static_fmt=:csr, check_symm, check_herm, check_pcon)
+
+Hamiltonian(basis, terms; static_fmt) (operators.jl:578-614) on the GPU: the
+sector matrix is assembled on the device once (`:csr` / `:csc`; the product of
+a Hermitian sector matrix reads the same stored rows either way), and every
+product then streams the stored rows.  `:dense` is not kept on the device:
+the assembled rows serve it, and `SparseMatrixCSC(H)` / `Matrix(H)` return the
+entries on the host.
+"""
+function GPUHamiltonian(basis::Union{SpinBasis1D,GPUBasis}, terms::AbstractVector{<:OperatorTerm};
+                        static_fmt=:csr, kwargs...)
+    fmt = Symbol(lowercase(String(static_fmt)))
+    fmt in (:csr, :csc, :dense) || throw(ArgumentError("static_fmt must be :dense, :csr, or :csc"))
+    return assemble!(GPUQuantumLinearOperator(basis, terms; kwargs...))
+end
+
 # AbstractMatrix protocol (operators.jl:3228-3245)
 Base.size(op::GPUQuantumLinearOperator) = (op.basis.Ns, op.basis.Ns)
 Base.getindex(op::GPUQuantumLinearOperator, i::Int, j::Int) =
@@ -153,7 +206,9 @@ function eigsh(op::GPUQuantumLinearOperator{T}; k::Integer=min(6, size(op, 1)), 
     1 <= k <= n || throw(ArgumentError("k must lie in 1:Ns"))
     w = Symbol(uppercase(String(which)))
     haskey(WHICH, w) || throw(ArgumentError("which must be SA, LA, SM, LM, or BE"))
-    vals = zeros(Float64, k)
+    # a non-Hermitian operator takes Arpack.eigs' complex path (operators.jl:3383-3390)
+    op.hermitian || dtype <: Complex || (dtype = complex(dtype))
+    vals = zeros(Float64, op.hermitian ? k : 2k)
     vecs = return_eigenvectors ? zeros(dtype, n, k) : nothing
     v = v0 === nothing ? C_NULL : Vector{dtype}(v0)
     nconv, nmv = Ref{Cint}(0), Ref{Cint}(0)
@@ -162,7 +217,8 @@ function eigsh(op::GPUQuantumLinearOperator{T}; k::Integer=min(6, size(op, 1)), 
                  Ref{Cint}, Ref{Cint}),
                 op.handle, _dtype(dtype), k, WHICH[w], tol, maxiter, ncv === nothing ? 0 : ncv, v, vals,
                 vecs === nothing ? C_NULL : vecs, nconv, nmv))
-    return return_eigenvectors ? (vals, vecs) : vals
+    values = op.hermitian ? vals : complex.(vals[1:2:end], vals[2:2:end])
+    return return_eigenvectors ? (values, vecs) : values
 end
 
 # exp(scale * op) v for every scale: _krylov_expmv_times (operators.jl:1805-1893)
@@ -229,6 +285,7 @@ function SparseArrays.SparseMatrixCSC(op::GPUQuantumLinearOperator{T}) where {T}
     rows = [r for r in 1:n for _ in 1:(rowptr[r+1] - rowptr[r])]
     return sparse(rows, col .+ 1, T <: Real ? real.(z) : T.(z), n, n)
 end
+Base.Matrix(op::GPUQuantumLinearOperator) = Matrix(SparseMatrixCSC(op))
 
 """
     GPUBlockOps(blocks, static, basis_args; basis_kwargs...)

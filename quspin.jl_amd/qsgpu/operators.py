@@ -325,14 +325,16 @@ def eigsh(op, k=None, which="SA", tol=0.0, maxiter=300, ncv=None, v0=None, retur
     w = str(which).upper()
     if w not in _lib.WHICH:
         raise ArgumentError("which must be SA, LA, SM, LM, or BE")
-    cplx = op._info()[0]
+    cplx, herm, _ = op._info()
+    # a non-Hermitian operator takes Arpack.eigs' complex path (operators.jl:3383-3390)
+    cplx = cplx or not herm
     if dtype is None:
         dt = np.dtype(np.complex128) if cplx else np.dtype(np.float64)
     else:
         dt = np.dtype(dtype)
         if cplx and not np.issubdtype(dt, np.complexfloating):
             dt = np.dtype(np.complex64) if dt == np.float32 else np.dtype(np.complex128)
-    vals = np.zeros(k, dtype=np.float64)
+    vals = np.zeros(k if herm else 2 * k, dtype=np.float64)
     vecs = np.zeros((n, k), dtype=dt, order="F") if return_eigenvectors else None
     v0p = None
     if v0 is not None:
@@ -346,6 +348,8 @@ def eigsh(op, k=None, which="SA", tol=0.0, maxiter=300, ncv=None, v0=None, retur
                               ctypes.byref(nconv), ctypes.byref(nmv))
     check(st)
     eigsh.last_n_matvec = nmv.value
+    if not herm:
+        vals = vals[0::2] + 1j * vals[1::2]
     if return_eigenvectors:
         return vals, vecs
     return vals
@@ -370,16 +374,24 @@ def expm_multiply(op, v, scales, tol=1e-12, krylov_dim=30, dtype=np.complex128):
 
 
 class Hamiltonian:
-    """Hamiltonian(static_list, dynamic_list; basis, check_*) -- operators.jl:578-614, 648-737, matrix-free:
-    the static terms form one QuantumLinearOperator on `basis` and each dynamic entry
-    ``[op, couplings, f, f_args]`` another, whose product is scaled by ``f(time, *f_args)``."""
+    """Hamiltonian(static_list, dynamic_list; basis, static_fmt, check_*) -- operators.jl:578-614, 648-737:
+    the static terms form one QuantumLinearOperator on `basis` and each group of dynamic entries
+    ``[op, couplings, f, f_args]`` sharing ``(f, f_args)`` another, whose product is scaled by
+    ``f(time, *f_args)``.  ``static_fmt`` "csr" / "csc" assembles the static sector matrix on the
+    device (its products then stream the stored rows); None / "dense" keeps it matrix-free."""
 
-    def __init__(self, static_list, dynamic_list=(), basis=None, check_symm=True, check_herm=True, check_pcon=True):
+    def __init__(self, static_list, dynamic_list=(), basis=None, static_fmt=None, check_symm=True, check_herm=True,
+                 check_pcon=True):
         if basis is None:
             raise ArgumentError("a basis is required")
+        fmt = None if static_fmt is None else str(static_fmt).lower().lstrip(":")
+        if fmt not in (None, "dense", "csr", "csc"):
+            raise ArgumentError("static_fmt must be :dense, :csr, or :csc")
         self.basis = basis
         kw = dict(check_symm=check_symm, check_herm=check_herm, check_pcon=check_pcon)
         self.static = QuantumLinearOperator(basis, list(static_list), **kw)
+        if fmt in ("csr", "csc"):
+            self.static.assemble()
         # entries sharing (f, f_args) form one operator, checked as one
         # collection (_dynamic_groups, discrete_basis.jl:2260-2309)
         groups = []

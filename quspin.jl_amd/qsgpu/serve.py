@@ -77,7 +77,8 @@ class Server:
         w = str(which).upper()
         if w not in _lib.WHICH:
             raise ArgumentError("which must be SA, LA, SM, LM, or BE")
-        cplx = op._info()[0]
+        cplx, herm, _ = op._info()
+        cplx = cplx or not herm   # non-Hermitian: complex Arnoldi
         dt = np.dtype(dtype) if dtype is not None else np.dtype(np.complex128 if cplx else np.float64)
         if cplx and not np.issubdtype(dt, np.complexfloating):
             dt = np.dtype(np.complex64) if dt == np.float32 else np.dtype(np.complex128)

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 def _setup(L, nup, blocks):
     static = _dense.xxz_terms(L, J=1.0, delta=0.6)
-    stag = ("z", [((-1.0) ** i * 0.5, i) for i in range(1, L + 1)])
+    stag = stag_of(L)
     hop = ("+-", [(0.3, i, i % L + 1) for i in range(1, L + 1)])
     hop2 = ("-+", [(0.3, i, i % L + 1) for i in range(1, L + 1)])
     f1 = lambda t, w: np.cos(w * t)
@@ -80,3 +80,13 @@ def test_dynamic_evolve_imaginary_time_matrix_input_and_errors(gpu):
     import scipy.linalg as sl
     out = qsgpu.evolve(Hs, V[:, 0], 0.0, [0.7])
     assert np.abs(out[:, 0] - sl.expm(-0.7j * H0) @ V[:, 0]).max() <= 3e-11
+    # static_fmt=:csr assembles the static part; products read the stored rows
+    Hc = qsgpu.Hamiltonian(_dense.xxz_terms(10, J=1.0, delta=0.6), [(stag_of(10)[0], stag_of(10)[1], np.cos, ())],
+                           basis=b, static_fmt="csr", check_symm=False)
+    assert np.abs(Hc.apply(V[:, 0], 0.4) - (H0 + np.cos(0.4) * dyn[0][0]) @ V[:, 0]).max() <= 1e-12
+    with pytest.raises(qsgpu.ArgumentError, match="static_fmt"):
+        qsgpu.Hamiltonian(_dense.xxz_terms(10, J=1.0, delta=0.6), [], basis=b, static_fmt="coo")
+
+
+def stag_of(L):
+    return ("z", [((-1.0) ** i * 0.5, i) for i in range(1, L + 1)])
