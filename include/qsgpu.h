@@ -270,6 +270,15 @@ int qs_evolve_dynamic(qs_op *h0, int n_dyn, qs_op *const *dyn, qs_drive_fn drive
 int qs_lanczos(qs_op *op, int dtype, const void *v0, int m, int full_ortho, double eps, double *alphas,
                double *betas, double *E, double *V, void *Q_T, int *used);
 
+/* The Gram products of the Gram-Schmidt step, G = V^H W -- the dot(basis,
+ * residual) coefficients of _krylov_factorization (operators.jl:1697-1719),
+ * for a block W of p vectors at once.  V: Ns x m, W: Ns x p, column-major
+ * host arrays of dtype; G: m x p interleaved complex (column-major), FP64
+ * accumulation.  on_mfma != 0: the block kernel on the matrix cores (the one
+ * the FP32-stream eigsh uses for its Rayleigh-Ritz products); 0: the VALU
+ * Gram kernel of the Arnoldi step, one column of W at a time. */
+int qs_block_gram(int dtype, const void *V, int64_t n, int m, const void *W, int p, int on_mfma, double *G);
+
 /* ------------------------------------------------ blocks of one parent space --
  * BlockOps' psi_0 lives in the basis built without the block keywords
  * (tools.jl:2447-2456).  kind 0: that parent is the block basis' own particle

@@ -677,6 +677,40 @@ int qs_op_create(const qs_basis *b, int n_terms, const char *const *opstr, const
     });
 }
 
+int qs_block_gram(int dtype, const void *V, int64_t n, int m, const void *W, int p, int on_mfma, double *G) {
+    return guard([&] {
+        if (!V || !W || !G) fail(QS_EARG, "V, W and G must not be NULL");
+        if (n < 1 || m < 1 || p < 1) fail(QS_EARG, "n, m and p must be positive");
+        const size_t esz = dtype_size(dtype);
+        hipStream_t s;
+        QS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        VecWork wk(s);
+        SBuf<char> dv((size_t)n * m * esz, s), dw((size_t)n * p * esz, s);
+        QS_HIP(hipMemcpyAsync(dv.get(), V, (size_t)n * m * esz, hipMemcpyHostToDevice, s));
+        QS_HIP(hipMemcpyAsync(dw.get(), W, (size_t)n * p * esz, hipMemcpyHostToDevice, s));
+        std::vector<cd> g;
+        if (on_mfma) {
+            gram_block(dtype, dv.get(), n, m, dw.get(), p, wk, s, g);
+        } else {
+            g.resize((size_t)m * p);
+            for (int c = 0; c < p; c++) {
+                std::vector<cd> h;
+                gram_dots(dtype, dv.get(), n, m, dw.get() + (size_t)c * n * esz, wk, s, h);
+                for (int i = 0; i < m; i++) g[(size_t)c * m + i] = h[i];
+            }
+        }
+        QS_HIP(hipStreamSynchronize(s));
+        for (size_t e = 0; e < g.size(); e++) {
+            G[2 * e] = g[e].real();
+            G[2 * e + 1] = g[e].imag();
+        }
+    });
+}
+
 int qs_op_destroy(qs_op *op) {
     delete op;
     return QS_OK;

@@ -557,10 +557,20 @@ double vec_norm2(int dtype, const void *x, int64_t n, VecWork &wk, hipStream_t s
 // the host.  vec_reserve sizes wk for up to mmax Gram columns beforehand
 // (the workspace must not be reallocated while launches are queued).
 void vec_reserve(VecWork &wk, int64_t n, int mmax);
+// With dgks_h1 / dgks_n2 (the first pass's h and |w'|^2, device) these are
+// the conditional second pass of ARPACK's DGKS step: skipped on the device
+// (zeros for h, |w'|^2 copied for the norm) when |w'| > 0.717 |w|.
 void gram_dots_dev(int dtype, const void *V, int64_t n, int m, const void *w, VecWork &wk, double *out,
-                   hipStream_t s);
+                   hipStream_t s, const double *dgks_h1 = nullptr, const double *dgks_n2 = nullptr);
 void gram_update_dev(int dtype, const void *V, int64_t n, int m, const double *h, void *w, VecWork &wk,
-                     double *nrm2_out, hipStream_t s);
+                     double *nrm2_out, hipStream_t s, const double *dgks_h1 = nullptr,
+                     const double *dgks_n2 = nullptr);
+// block Gram product G = V^H W (V: n x m, W: n x p, ld n) on the matrix
+// cores, FP64 accumulation; out: m * p interleaved complex, column-major
+void gram_block_dev(int dtype, const void *V, int64_t n, int m, const void *W, int p, VecWork &wk, double *out,
+                    hipStream_t s);
+void gram_block(int dtype, const void *V, int64_t n, int m, const void *W, int p, VecWork &wk, hipStream_t s,
+                std::vector<cd> &G);
 void vec_norm2_dev(int dtype, const void *x, int64_t n, VecWork &wk, double *out, hipStream_t s);
 void vec_axpby_dev(int dtype, const double *acoef, int cmode, const void *x, cd b, void *y, int64_t n,
                    hipStream_t s);

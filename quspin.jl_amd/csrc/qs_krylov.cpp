@@ -83,6 +83,90 @@ std::vector<int> select(const std::vector<double> &theta, int which, int want) {
     return idx;
 }
 
+// FP32-stored Ritz vectors Y (n x k): Rayleigh-Ritz in FP64 on span(Y).
+// Y is widened to FP64, W = A Y is formed by FP64 products, and the block
+// Gram products G = Y^H W and M = Y^H Y run on the matrix cores; the k x k
+// problem G c = lambda M c (Cholesky of M) gives the reported eigenvalues,
+// accurate to second order in the FP32 residual, and its vectors recombine Y
+// into the returned FP32 Ritz vectors.
+void rayleigh_ritz_fp64(OpHost &op, int dtype, int k, int which, const void *Yf, int64_t n, VecWork &wk,
+                        EigshResult &res, void *vecs_dev) {
+    hipStream_t s = op_stream(op);
+    const int dt64 = dtype == QS_F32 ? QS_F64 : QS_C128;
+    const size_t e64 = dtype_size(dt64);
+    SBuf<char> Y64((size_t)n * k * e64, s), W64((size_t)n * k * e64, s);
+    vec_convert(dtype, Yf, dt64, Y64.get(), n * k, s);
+    for (int c = 0; c < k; c++) {
+        launch_matvec(op, dt64, Y64.get() + (size_t)c * n * e64, W64.get() + (size_t)c * n * e64, 1.0, 0.0, 0.0,
+                      0.0, false, false, false, s);
+        res.nmatvec++;
+    }
+    std::vector<cd> G, M;
+    gram_block(dt64, Y64.get(), n, k, W64.get(), k, wk, s, G);
+    gram_block(dt64, Y64.get(), n, k, Y64.get(), k, wk, s, M);
+    auto g = [&](std::vector<cd> &X, int i, int j) -> cd & { return X[(size_t)i + (size_t)k * j]; };
+    // M = L L^H
+    std::vector<cd> Lm((size_t)k * k, cd(0, 0));
+    auto l = [&](int i, int j) -> cd & { return Lm[(size_t)i * k + j]; };
+    for (int j = 0; j < k; j++) {
+        double dsum = g(M, j, j).real();
+        for (int q = 0; q < j; q++) dsum -= std::norm(l(j, q));
+        if (!(dsum > 0)) fail(QS_ENOCONV, "eigsh: FP32 Ritz vectors are linearly dependent");
+        l(j, j) = std::sqrt(dsum);
+        for (int i = j + 1; i < k; i++) {
+            cd acc = g(M, i, j);
+            for (int q = 0; q < j; q++) acc -= l(i, q) * std::conj(l(j, q));
+            l(i, j) = acc / l(j, j);
+        }
+    }
+    // A = L^-1 G L^-H (Hermitian part)
+    std::vector<cd> X((size_t)k * k), A((size_t)k * k);
+    for (int c = 0; c < k; c++)          // X = L^-1 G, column by column
+        for (int i = 0; i < k; i++) {
+            cd acc = g(G, i, c);
+            for (int q = 0; q < i; q++) acc -= l(i, q) * X[(size_t)q * k + c];
+            X[(size_t)i * k + c] = acc / l(i, i);
+        }
+    for (int r = 0; r < k; r++)          // A = X L^-H: solve A L^H = X row by row
+        for (int j = 0; j < k; j++) {
+            cd acc = X[(size_t)r * k + j];
+            for (int q = 0; q < j; q++) acc -= A[(size_t)r * k + q] * std::conj(l(j, q));
+            A[(size_t)r * k + j] = acc / std::conj(l(j, j));
+        }
+    for (int i = 0; i < k; i++)
+        for (int j = i; j < k; j++) {
+            const cd h = 0.5 * (A[(size_t)i * k + j] + std::conj(A[(size_t)j * k + i]));
+            A[(size_t)i * k + j] = h;
+            A[(size_t)j * k + i] = std::conj(h);
+        }
+    std::vector<double> lam;
+    std::vector<cd> U;
+    herm_eig(k, A, lam, U);
+    std::vector<int> ord(k);
+    std::iota(ord.begin(), ord.end(), 0);
+    auto key = [&](int i) {
+        switch (which) {
+        case QS_WHICH_LA: return -lam[i];
+        case QS_WHICH_SM: return std::fabs(lam[i]);
+        case QS_WHICH_LM: return -std::fabs(lam[i]);
+        default: return lam[i];
+        }
+    };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key(a) < key(b); });
+    for (int c = 0; c < k; c++) res.vals[c] = lam[ord[c]];
+    if (vecs_dev) {
+        // coefficients L^-H U of the refined vectors, in the reported order
+        std::vector<cd> Cr((size_t)k * k);
+        for (int c = 0; c < k; c++)
+            for (int i = k - 1; i >= 0; i--) {
+                cd acc = U[(size_t)i * k + ord[c]];
+                for (int q = i + 1; q < k; q++) acc -= std::conj(l(q, i)) * Cr[(size_t)c * k + q];
+                Cr[(size_t)c * k + i] = acc / std::conj(l(i, i));
+            }
+        basis_combine(dtype, Yf, n, k, Cr, k, vecs_dev, s);
+    }
+}
+
 }  // namespace
 
 void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxiter, int ncv, const void *v0_dev,
@@ -122,8 +206,8 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     int verifications = 0;
     std::vector<double> record;  // the k wanted values of the last converged round
     // per-step scalars of one Lanczos cycle, device side: two Gram passes
-    // (m + 1 complex each) and |w|^2
-    const size_t per = (size_t)4 * (m + 1) + 2;
+    // (m + 1 complex each), |w|^2 after both and |w'|^2 after the first
+    const size_t per = (size_t)4 * (m + 1) + 4;
     SBuf<double> d(per * m, s);
     std::vector<double> host(per * m);
     vec_reserve(wk, n, m + 1);
@@ -133,12 +217,14 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
             // queue steps j0 .. m-1 with no host round trip
             for (int j = j0; j < m; j++) {
                 void *w = K.col(j + 1);
-                double *h1 = d.get() + per * j, *h2 = h1 + 2 * (m + 1), *n2 = h2 + 2 * (m + 1);
+                double *h1 = d.get() + per * j, *h2 = h1 + 2 * (m + 1), *n2 = h2 + 2 * (m + 1), *n2a = n2 + 2;
                 matvec(op, dtype, K.col(j), w);
+                // ARPACK's DGKS step: the second pass runs only if the first
+                // left |w'| <= 0.717 |w| (decided on the device)
                 gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h1, s);
-                gram_update_dev(dtype, K.V.get(), n, j + 1, h1, w, wk, nullptr, s);
-                gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h2, s);
-                gram_update_dev(dtype, K.V.get(), n, j + 1, h2, w, wk, n2, s);
+                gram_update_dev(dtype, K.V.get(), n, j + 1, h1, w, wk, n2a, s);
+                gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h2, s, h1, n2a);
+                gram_update_dev(dtype, K.V.get(), n, j + 1, h2, w, wk, n2, s, h1, n2a);
                 vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
             }
             QS_HIP(hipMemcpyAsync(host.data() + per * j0, d.get() + per * j0, sizeof(double) * per * (m - j0),
@@ -284,21 +370,10 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
         std::vector<cd> C((size_t)m * k);
         for (int c = 0; c < k; c++)
             for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + out[c]], 0);
-        Krylov tmp(dtype, n, vecs_dev ? 0 : k, s);
-        void *dst = vecs_dev ? vecs_dev : tmp.V.get();
+        Krylov tmp(dtype, n, (vecs_dev && !fp32) ? 0 : k, s);
+        void *dst = (vecs_dev && !fp32) ? vecs_dev : tmp.V.get();
         basis_combine(dtype, K.V.get(), n, m, C, k, dst, s);
-        if (fp32) {
-            // FP32-stored Ritz vectors: report the FP64-accumulated Rayleigh
-            // quotient <v, Hv>/<v, v> (the product is reduced in FP64 before
-            // any rounding of Hv to FP32)
-            for (int c = 0; c < k; c++) {
-                const void *v = (const char *)dst + (size_t)c * n * K.esz;
-                cd num = rayleigh_numerator(op, dtype, v, K.col(m), wk);
-                double den = vec_norm2(dtype, v, n, wk, s);
-                res.vals[c] = num.real() / den;
-                res.nmatvec++;
-            }
-        }
+        if (fp32) rayleigh_ritz_fp64(op, dtype, k, which, dst, n, wk, res, vecs_dev);
     }
     QS_HIP(hipStreamSynchronize(s));
 }

@@ -28,21 +28,41 @@ constexpr int kVB = 256;
 constexpr int kMaxBlocks = 1024;
 constexpr int kCols = 8;   // Gram columns per pass
 
+// butterfly over the 64 lanes of a wavefront: every lane ends with the same
+// sum (each step adds the same two operands on both partners)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ void block_reduce2(double &re, double &im, double *sh) {
-    // fixed-shape tree over kVB lanes; result in lane 0
-    sh[threadIdx.x] = re;
-    sh[kVB + threadIdx.x] = im;
-    __syncthreads();
-    for (int s = kVB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            sh[threadIdx.x] += sh[threadIdx.x + s];
-            sh[kVB + threadIdx.x] += sh[kVB + threadIdx.x + s];
-        }
-        __syncthreads();
+    // fixed shape: wavefront butterflies, then the four wavefront sums in a
+    // fixed order; every thread returns the block sum
+    static_assert(kVB == 256, "four wavefronts per block");
+    re = wave_sum(re);
+    im = wave_sum(im);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[w] = re;
+        sh[4 + w] = im;
     }
-    re = sh[0];
-    im = sh[kVB];
     __syncthreads();
+    re = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    im = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+    __syncthreads();
+}
+
+// DGKS test of ARPACK's Arnoldi step (dsaitr / znaitr, eta = 0.717): after
+// the first Gram-Schmidt pass w' = w - V h1 the second pass is needed only if
+// |w'| <= eta |w|, with |w|^2 = |w'|^2 + |h1|^2.  Read from device memory by
+// every block of the second-pass launches, so the decision costs no host
+// round trip; every block computes it in the same order.
+__device__ __forceinline__ bool dgks_skip(const double *h1, int m, const double *n2w) {
+    double hh = 0;
+    for (int j = 0; j < m; j++) hh += h1[2 * j] * h1[2 * j] + h1[2 * j + 1] * h1[2 * j + 1];
+    const double e2 = 0.717 * 0.717;
+    return n2w[0] > e2 * (n2w[0] + hh);
 }
 
 // Block b of nb owns one contiguous row range.  Kernel bodies take (b, nb)
@@ -274,23 +294,122 @@ __global__ __launch_bounds__(256) void k_combine_mfma(const double *__restrict__
     }
 }
 
+// The second-pass launches take (dh, dn, dm): when non-null and the DGKS
+// test says no second pass is needed, the Gram and update kernels return at
+// once, the Gram reduction writes zeros and the norm reduction copies |w'|^2.
 template <typename V>
 __global__ __launch_bounds__(kVB) void k_gram(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
-                                              double *partial) {
+                                              double *partial, const double *dh = nullptr,
+                                              const double *dn = nullptr, int dm = 0) {
     __shared__ double sh[2 * kVB];
+    if (dh && dgks_skip(dh, dm, dn)) return;
     gram_body<V>(Vb, n, m, w, partial, blockIdx.x, gridDim.x, sh);
 }
 
-__global__ __launch_bounds__(kVB) void k_reduce(const double *partial, int nb, double *out) {
+__global__ __launch_bounds__(kVB) void k_reduce(const double *partial, int nb, double *out,
+                                                const double *dh = nullptr, const double *dn = nullptr, int dm = 0,
+                                                int copy_norm = 0) {
     __shared__ double sh[2 * kVB];
+    if (dh && dgks_skip(dh, dm, dn)) {
+        if (threadIdx.x == 0) {
+            out[2 * blockIdx.x] = copy_norm ? dn[0] : 0.0;
+            out[2 * blockIdx.x + 1] = 0.0;
+        }
+        return;
+    }
     reduce_body(partial, nb, out, blockIdx.x, sh);
 }
 
 template <typename V>
 __global__ __launch_bounds__(kVB) void k_update(const V *__restrict__ Vb, int64_t n, int m, const double *h,
-                                                V *__restrict__ w, double *partial) {
+                                                V *__restrict__ w, double *partial, const double *dh = nullptr,
+                                                const double *dn = nullptr, int dm = 0) {
     __shared__ double sh[2 * kVB];
+    if (dh && dgks_skip(dh, dm, dn)) return;
     update_body<V>(Vb, n, m, h, w, partial, blockIdx.x, gridDim.x, sh);
+}
+
+// ------------------------------------------------- block Gram on MFMA ----
+// G = V^H W for V (n x m) and W (n x p), column-major with ld n: partial sums
+// per row block on the matrix cores (v_mfma_f64_16x16x4f64), one 16 x 16
+// output tile per blockIdx.y.  Each block walks its row range in chunks of
+// 64 rows staged through LDS (coalesced loads, FP32 widened to FP64 there);
+// wavefront w multiplies rows 16 w .. 16 w + 15 of the chunk as four K = 4
+// steps.  Operand layout as in k_combine_mfma: lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator register i of lane l is
+// D[l / 16 + 4 i][l % 16].  Complex: Re G = Vr'Wr + Vi'Wi, Im G = Vr'Wi - Vi'Wr.
+// The four wavefront tiles are summed in a fixed order; partial[(e nb + b) 2]
+// holds output element e = i + m j, reduced over blocks by k_reduce.
+constexpr int kGC = 64;       // rows per LDS chunk
+constexpr int kGP = kGC + 1;  // padded LDS row stride (doubles)
+
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_gram_mfma(const V *__restrict__ Vb, int64_t n, int m,
+                                                   const V *__restrict__ Wb, int p, int tiles_m,
+                                                   double *__restrict__ partial) {
+    using T = VTraits<V>;
+    constexpr bool CPLX = T::cplx;
+    __shared__ double sv[2][16 * kGP], sw[2][16 * kGP];
+    __shared__ double sred[4][2][256];
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int ti = blockIdx.y % tiles_m, tj = blockIdx.y / tiles_m;
+    const int i0 = 16 * ti, j0 = 16 * tj;
+    const int mi = m - i0 < 16 ? m - i0 : 16, pj = p - j0 < 16 ? p - j0 : 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int64_t r0, r1;
+    row_range(n, b, nb, r0, r1);
+    // columns beyond m / p stay zero for the whole launch
+    for (int e = threadIdx.x; e < 16 * kGP; e += kVB) {
+        sv[0][e] = sv[1][e] = sw[0][e] = sw[1][e] = 0.0;
+    }
+    v4d acc_r = {0, 0, 0, 0}, acc_i = {0, 0, 0, 0};
+    const int la = lane & 15, lk = lane >> 4;
+    for (int64_t c0 = r0; c0 < r1; c0 += kGC) {
+        __syncthreads();
+        // stage: thread t loads row (t % 64) of columns t / 64, t / 64 + 4, ...
+        const int rr = threadIdx.x & 63;
+        const int64_t r = c0 + rr;
+        for (int c = threadIdx.x >> 6; c < 16; c += 4) {
+            double re = 0, im = 0;
+            if (c < mi && r < r1) T::load(Vb, (int64_t)(i0 + c) * n + r, re, im);
+            sv[0][c * kGP + rr] = re;
+            if (CPLX) sv[1][c * kGP + rr] = im;
+            re = 0;
+            im = 0;
+            if (c < pj && r < r1) T::load(Wb, (int64_t)(j0 + c) * n + r, re, im);
+            sw[0][c * kGP + rr] = re;
+            if (CPLX) sw[1][c * kGP + rr] = im;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const int kr = 16 * wv + 4 * st + lk;
+            const double ar = sv[0][la * kGP + kr], br = sw[0][la * kGP + kr];
+            acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc_r, 0, 0, 0);
+            if (CPLX) {
+                const double ai = sv[1][la * kGP + kr], bi = sw[1][la * kGP + kr];
+                acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, acc_r, 0, 0, 0);
+                acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_i, 0, 0, 0);
+                acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, acc_i, 0, 0, 0);
+            }
+        }
+    }
+    // D[l / 16 + 4 q][l % 16] of each wavefront -> LDS, then a fixed-order sum
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int e = (lk + 4 * q) * 16 + la;
+        sred[wv][0][e] = acc_r[q];
+        sred[wv][1][e] = acc_i[q];
+    }
+    __syncthreads();
+    const int e = threadIdx.x, ii = e >> 4, jj = e & 15;
+    if (ii < mi && jj < pj) {
+        const double gr = (sred[0][0][e] + sred[1][0][e]) + (sred[2][0][e] + sred[3][0][e]);
+        const double gi = (sred[0][1][e] + sred[1][1][e]) + (sred[2][1][e] + sred[3][1][e]);
+        const int64_t g = (int64_t)(i0 + ii) + (int64_t)m * (j0 + jj);
+        partial[(g * nb + b) * 2] = gr;
+        partial[(g * nb + b) * 2 + 1] = CPLX ? gi : 0.0;
+    }
 }
 
 template <typename V>
@@ -544,29 +663,58 @@ void vec_reserve(VecWork &wk, int64_t n, int mmax) {
 }
 
 void gram_dots_dev(int dtype, const void *Vmat, int64_t n, int m, const void *w, VecWork &wk, double *out,
-                   hipStream_t s) {
+                   hipStream_t s, const double *dgks_h1, const double *dgks_n2) {
     const int nb = vec_blocks(n);
     if (wk.partial.n < (size_t)2 * nb * m) fail(QS_EARG, "internal: vector workspace not reserved");
     by_dtype(dtype, [&](auto *tag) {
         using V = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL((k_gram<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, (const V *)w,
-                           wk.partial.get());
+                           wk.partial.get(), dgks_h1, dgks_n2, m);
     });
-    hipLaunchKernelGGL(k_reduce, dim3(m), dim3(kVB), 0, s, wk.partial.get(), nb, out);
+    hipLaunchKernelGGL(k_reduce, dim3(m), dim3(kVB), 0, s, wk.partial.get(), nb, out, dgks_h1, dgks_n2, m, 0);
     QS_HIP(hipGetLastError());
 }
 
 void gram_update_dev(int dtype, const void *Vmat, int64_t n, int m, const double *h, void *w, VecWork &wk,
-                     double *nrm2_out, hipStream_t s) {
+                     double *nrm2_out, hipStream_t s, const double *dgks_h1, const double *dgks_n2) {
     const int nb = vec_blocks(n);
     if (wk.partial.n < (size_t)2 * nb) fail(QS_EARG, "internal: vector workspace not reserved");
     by_dtype(dtype, [&](auto *tag) {
         using V = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL((k_update<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, h, (V *)w,
-                           wk.partial.get());
+                           wk.partial.get(), dgks_h1, dgks_n2, m);
     });
-    if (nrm2_out) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, nrm2_out);
+    if (nrm2_out)
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, nrm2_out, dgks_h1, dgks_n2, m,
+                           1);
     QS_HIP(hipGetLastError());
+}
+
+void gram_block_dev(int dtype, const void *Vmat, int64_t n, int m, const void *Wmat, int p, VecWork &wk,
+                    double *out, hipStream_t s) {
+    if (m < 1 || p < 1) fail(QS_EARG, "internal: empty block Gram product");
+    const int nb = vec_blocks(n);
+    const int tm = (m + 15) / 16, tp = (p + 15) / 16;
+    ensure(wk, (size_t)2 * nb * m * p, 2);
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_gram_mfma<V>), dim3(nb, tm * tp), dim3(kVB), 0, s, (const V *)Vmat, n, m,
+                           (const V *)Wmat, p, tm, wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(m * p), dim3(kVB), 0, s, wk.partial.get(), nb, out, (const double *)nullptr,
+                       (const double *)nullptr, 0, 0);
+    QS_HIP(hipGetLastError());
+}
+
+void gram_block(int dtype, const void *Vmat, int64_t n, int m, const void *Wmat, int p, VecWork &wk, hipStream_t s,
+                std::vector<cd> &G) {
+    SBuf<double> d((size_t)2 * m * p, s);
+    gram_block_dev(dtype, Vmat, n, m, Wmat, p, wk, d.get(), s);
+    std::vector<double> h((size_t)2 * m * p);
+    QS_HIP(hipMemcpyAsync(h.data(), d.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    G.resize((size_t)m * p);
+    for (size_t e = 0; e < G.size(); e++) G[e] = cd(h[2 * e], h[2 * e + 1]);
 }
 
 void vec_norm2_dev(int dtype, const void *x, int64_t n, VecWork &wk, double *out, hipStream_t s) {

@@ -24,7 +24,7 @@ EXPORTS = (
     "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
     "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr", "qs_op_set_graphs",
-    "qs_op_matvec_noalloc", "qs_evolve_dynamic",
+    "qs_op_matvec_noalloc", "qs_evolve_dynamic", "qs_block_gram",
 )
 NOALLOC_SCRATCH_BYTES = 32 << 20   # QS_NOALLOC_SCRATCH_BYTES
 
@@ -66,6 +66,7 @@ def lib():
     L.qs_op_matvec_noalloc.argtypes = [vp, i32, vp, vp, i64, dbl, dbl, dbl, dbl, i32, i32]
     L.qs_evolve_dynamic.argtypes = [vp, i32, vp, vp, vp, i32, vp, i64, dbl, i64, vp, i32, dbl, dbl, dbl, vp, vp, vp]
     L.qs_eigsh.argtypes = [vp, i32, i32, i32, dbl, i32, i32, vp, vp, vp, vp, vp]
+    L.qs_block_gram.argtypes = [i32, vp, i64, i32, vp, i32, i32, vp]
     L.qs_expm_multiply.argtypes = [vp, i32, vp, i64, vp, dbl, i32, vp, vp]
     L.qs_op_set_variant.argtypes = [vp, i32]
     L.qs_op_tune.argtypes = [vp, i32, i32, vp, vp]
