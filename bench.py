@@ -141,6 +141,44 @@ def bench_c4(n_times):
     return out
 
 
+def bench_c3(args, skip_eigsh=False):
+    """Precision form's matvec pair: L=32 nup=16 z+ (N = 300,540,195), FP32 stream and FP64, each
+    with the autotuned (or pinned) kernel variant; then, unless skipped, the k=4 eigensolve."""
+    import numpy as np
+    import torch
+    import qsgpu
+    print("bench: C3 L=32 z+ matvec pair", file=sys.stderr, flush=True)
+    b3 = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
+    # disordered fields h_i ~ U[-2, 2]: odd under spin inversion, so the
+    # sector's group average P'HP cancels them exactly (check_symm=false)
+    h = np.random.default_rng(0xC3).uniform(-2.0, 2.0, 32)
+    terms3 = xxz_terms(32, J=1.0, delta=1.0) + [("z", [(float(h[i]), i + 1) for i in range(32)])]
+    op3 = qsgpu.QuantumLinearOperator(b3, terms3, check_symm=False)
+    n3 = b3.Ns
+    out = {"sector_dim": n3}
+    for dt, name, esz in ((torch.float32, "fp32", 4), (torch.float64, "fp64", 8)):
+        x3 = torch.rand(n3, dtype=dt, device="cuda")
+        y3 = torch.zeros_like(x3)
+        if args.c3_variant:
+            op3.set_variant(args.c3_variant)
+            lpr3 = args.c3_variant
+        else:
+            lpr3, _ = op3.tune(np.float32 if esz == 4 else np.float64, repeats=2)
+        t3, _ = time_matvec(op3, x3, y3, max(3, args.steps // 10), 2)
+        b3_alg = n3 * 2 * esz   # read x once, write y once; the z+ basis is implicit (bit 31 clear)
+        out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(b3_alg / t3 / 1e9, 2), "variant": lpr3,
+                     "kernel": variant_name(lpr3),
+                     "frac": round(b3_alg / t3 / 1e9 / HBM_PEAK_GBPS, 4), "bytes_per_matvec": b3_alg,
+                     "traffic": profiled_traffic("c3_" + name)}
+        del x3, y3
+        torch.cuda.empty_cache()
+    out["fp32_over_fp64"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)
+    eig = None if skip_eigsh else bench_c3_eigsh(op3, n3)
+    del op3, b3
+    torch.cuda.empty_cache()
+    return out, eig
+
+
 def bench_c3_eigsh(op3, n3):
     """Precision form: the k=4 lowest eigenvalues of the L=32 z+ sector in the FP32-stream form
     (FP32 Krylov basis, FP64 dots / norms / tridiagonal solve, FP64 Rayleigh quotients) and in
@@ -337,6 +375,10 @@ def main():
     ap.add_argument("--no-eigsh", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c3-only", action="store_true",
+                    help="only the L=32 z+ matvec pair (profiling runs); implies --no-c3-eigsh unless --c3-eigsh")
+    ap.add_argument("--c3-eigsh", action="store_true", help="with --c3-only: also the k=4 eigensolve")
+    ap.add_argument("--c3-variant", type=int, default=0, help="pin the C3 kernel variant instead of autotuning")
     ap.add_argument("--c4-times", type=int, default=200)
     ap.add_argument("--c5-requests", type=int, default=200)
     ap.add_argument("--c5-workers", type=int, default=8)
@@ -355,6 +397,10 @@ def main():
     import qsgpu
 
     dev = qsgpu.device_info()
+    if args.c3_only:
+        res = bench_c3(args, skip_eigsh=not args.c3_eigsh)
+        print(json.dumps({"c3_matvec": res[0], "c3_eigsh": res[1]}))
+        return
     L = 28
     print("bench: building C2 sector L=28 nup=14 k=0 p=+1 z=+1", file=sys.stderr, flush=True)
     tb = time.perf_counter()
@@ -436,33 +482,9 @@ def main():
                            "E0_fp32_stream": float(e32[0]), "seconds_fp32": round(t_e32, 4),
                            "matvecs_fp32": nmv32, "abs_err_fp32_vs_fp64": abs(float(e32[0]) - float(e64[0]))}
     if not args.no_c3:
-        print("bench: C3 L=32 z+ matvec pair", file=sys.stderr, flush=True)
-        b3 = qsgpu.SpinBasis1D(32, nup=16, pauli=False, zblock=1)
-        # disordered fields h_i ~ U[-2, 2]: odd under spin inversion, so the
-        # sector's group average P'HP cancels them exactly (check_symm=false)
-        h = np.random.default_rng(0xC3).uniform(-2.0, 2.0, 32)
-        terms3 = xxz_terms(32, J=1.0, delta=1.0) + [("z", [(float(h[i]), i + 1) for i in range(32)])]
-        op3 = qsgpu.QuantumLinearOperator(b3, terms3, check_symm=False)
-        n3 = b3.Ns
-        out = {"sector_dim": n3}
-        for dt, name, esz in ((torch.float32, "fp32", 4), (torch.float64, "fp64", 8)):
-            x3 = torch.rand(n3, dtype=dt, device="cuda")
-            y3 = torch.zeros_like(x3)
-            lpr3, _ = op3.tune(np.float32 if esz == 4 else np.float64, repeats=2)
-            t3, _ = time_matvec(op3, x3, y3, max(3, args.steps // 10), 2)
-            b3_alg = n3 * 2 * esz   # read x once, write y once; the z+ basis is implicit (bit 31 clear)
-            out[name] = {"ms": round(t3 * 1e3, 4), "GBps": round(b3_alg / t3 / 1e9, 2), "variant": lpr3,
-                         "kernel": variant_name(lpr3),
-                         "frac": round(b3_alg / t3 / 1e9 / HBM_PEAK_GBPS, 4), "bytes_per_matvec": b3_alg,
-                         "traffic": profiled_traffic("c3_" + name)}
-            del x3, y3
-            torch.cuda.empty_cache()
-        out["fp32_over_fp64"] = round(out["fp64"]["ms"] / out["fp32"]["ms"], 3)
-        result["c3_matvec"] = out
-        if not args.no_c3_eigsh:
-            result["c3_eigsh"] = bench_c3_eigsh(op3, n3)
-        del op3, b3
-        torch.cuda.empty_cache()
+        result["c3_matvec"], c3e = bench_c3(args, skip_eigsh=args.no_c3_eigsh)
+        if c3e is not None:
+            result["c3_eigsh"] = c3e
     if not args.no_c4:
         result["c4_block_evolve"] = bench_c4(args.c4_times)
     if not args.no_c5:
