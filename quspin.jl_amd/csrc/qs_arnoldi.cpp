@@ -318,10 +318,7 @@ void eigs_device(OpHost &op, int dtype, int k, int which, double tol, int maxite
                 double *h1 = d.get() + per * j, *h2 = h1 + 2 * (m + 1), *n2 = h2 + 2 * (m + 1), *n2a = n2 + 2;
                 launch_matvec(op, dtype, col(j), w, 1.0, 0.0, 0.0, 0.0, false, false, false, s);
                 // znaitr's DGKS step: a second pass only if |w'| <= 0.717 |w|
-                gram_dots_dev(dtype, V.get(), n, j + 1, w, wk, h1, s);
-                gram_update_dev(dtype, V.get(), n, j + 1, h1, w, wk, n2a, s);
-                gram_dots_dev(dtype, V.get(), n, j + 1, w, wk, h2, s, h1, n2a);
-                gram_update_dev(dtype, V.get(), n, j + 1, h2, w, wk, n2, s, h1, n2a);
+                gram_dgks_dev(dtype, V.get(), n, j + 1, w, wk, h1, h2, n2a, n2, s);
                 vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
             }
             QS_HIP(hipMemcpyAsync(host.data() + per * j0, d.get() + per * j0, sizeof(double) * per * (m - j0),

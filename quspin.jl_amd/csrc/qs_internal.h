@@ -565,6 +565,13 @@ void gram_dots_dev(int dtype, const void *V, int64_t n, int m, const void *w, Ve
 void gram_update_dev(int dtype, const void *V, int64_t n, int m, const double *h, void *w, VecWork &wk,
                      double *nrm2_out, hipStream_t s, const double *dgks_h1 = nullptr,
                      const double *dgks_n2 = nullptr);
+// ARPACK's DGKS step (dsaitr / znaitr) on w against V[:, :m]: the first
+// Gram pass (h1), w' = w - V h1 with |w'|^2 (n2a), and, only if |w'| <=
+// 0.717 |w| (decided on the device), the second pass (h2, else zeros) and
+// w'' = w' - V h2; n2 = the final |w|^2.  The first pass's update and the
+// second pass's Gram product share one read of V.  Results in device memory.
+void gram_dgks_dev(int dtype, const void *V, int64_t n, int m, void *w, VecWork &wk, double *h1, double *h2,
+                   double *n2a, double *n2, hipStream_t s);
 // block Gram product G = V^H W (V: n x m, W: n x p, ld n) on the matrix
 // cores, FP64 accumulation; out: m * p interleaved complex, column-major
 void gram_block_dev(int dtype, const void *V, int64_t n, int m, const void *W, int p, VecWork &wk, double *out,

@@ -221,10 +221,7 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
                 matvec(op, dtype, K.col(j), w);
                 // ARPACK's DGKS step: the second pass runs only if the first
                 // left |w'| <= 0.717 |w| (decided on the device)
-                gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h1, s);
-                gram_update_dev(dtype, K.V.get(), n, j + 1, h1, w, wk, n2a, s);
-                gram_dots_dev(dtype, K.V.get(), n, j + 1, w, wk, h2, s, h1, n2a);
-                gram_update_dev(dtype, K.V.get(), n, j + 1, h2, w, wk, n2, s, h1, n2a);
+                gram_dgks_dev(dtype, K.V.get(), n, j + 1, w, wk, h1, h2, n2a, n2, s);
                 vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
             }
             QS_HIP(hipMemcpyAsync(host.data() + per * j0, d.get() + per * j0, sizeof(double) * per * (m - j0),

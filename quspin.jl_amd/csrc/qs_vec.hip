@@ -27,6 +27,7 @@ namespace {
 constexpr int kVB = 256;
 constexpr int kMaxBlocks = 1024;
 constexpr int kCols = 8;   // Gram columns per pass
+constexpr int kDgksFuse = 24;   // basis columns the fused update + Gram holds in registers
 
 // butterfly over the 64 lanes of a wavefront: every lane ends with the same
 // sum (each step adds the same two operands on both partners)
@@ -74,30 +75,61 @@ __device__ __forceinline__ void row_range(int64_t n, int b, int nb, int64_t &r0,
     r1 = r0 + chunk < n ? r0 + chunk : n;
 }
 
+// Column groups of NC (<= 8) columns: every load of a row pair is issued
+// before any is used (the second row of the pair clamped to the range, its
+// terms dropped), so each lane keeps 2 (NC + 1) loads in flight.
+template <typename V, int NC>
+__device__ __forceinline__ void gram_cols(const V *__restrict__ Vb, int64_t n, int j0, const V *__restrict__ w,
+                                          int64_t r0, int64_t r1, double (&acc_r)[8], double (&acc_i)[8]) {
+    using T = VTraits<V>;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 2 * kVB) {
+        const int64_t rb = r + kVB < r1 ? r + kVB : r1 - 1;
+        const bool hb = r + kVB < r1;
+        V wa = w[r], wb = w[rb];
+        V va[NC], vb[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            va[c] = Vb[(int64_t)(j0 + c) * n + r];
+            vb[c] = Vb[(int64_t)(j0 + c) * n + rb];
+        }
+        double war, wai, wbr, wbi;
+        T::load(&wa, 0, war, wai);
+        T::load(&wb, 0, wbr, wbi);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double vr, vi;
+            T::load(&va[c], 0, vr, vi);
+            acc_r[c] += vr * war + vi * wai;
+            acc_i[c] += vr * wai - vi * war;
+            if (hb) {
+                T::load(&vb[c], 0, vr, vi);
+                acc_r[c] += vr * wbr + vi * wbi;
+                acc_i[c] += vr * wbi - vi * wbr;
+            }
+        }
+    }
+}
+
 // partial[(j * nb + b) * 2 + {0,1}] = sum over block b rows of conj(V[r, j]) * w[r]
 template <typename V>
 __device__ __forceinline__ void gram_body(const V *__restrict__ Vb, int64_t n, int m, const V *__restrict__ w,
                                           double *partial, int b, int nb, double *sh) {
-    using T = VTraits<V>;
     int64_t r0, r1;
     row_range(n, b, nb, r0, r1);
     for (int j0 = 0; j0 < m; j0 += kCols) {
         double acc_r[kCols], acc_i[kCols];
 #pragma unroll
         for (int c = 0; c < kCols; c++) { acc_r[c] = 0; acc_i[c] = 0; }
-        int nc = m - j0 < kCols ? m - j0 : kCols;
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
-            double wr, wi;
-            T::load(w, r, wr, wi);
-#pragma unroll
-            for (int c = 0; c < kCols; c++) {
-                if (c < nc) {
-                    double vr, vi;
-                    T::load(Vb, (int64_t)(j0 + c) * n + r, vr, vi);
-                    acc_r[c] += vr * wr + vi * wi;
-                    acc_i[c] += vr * wi - vi * wr;
-                }
-            }
+        const int nc = m - j0 < kCols ? m - j0 : kCols;
+        switch (nc) {
+        case 1: gram_cols<V, 1>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        case 2: gram_cols<V, 2>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        case 3: gram_cols<V, 3>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        case 4: gram_cols<V, 4>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        case 5: gram_cols<V, 5>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        case 6: gram_cols<V, 6>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        case 7: gram_cols<V, 7>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
+        default: gram_cols<V, 8>(Vb, n, j0, w, r0, r1, acc_r, acc_i); break;
         }
         for (int c = 0; c < nc; c++) {
             double re = acc_r[c], im = acc_i[c];
@@ -124,6 +156,30 @@ __device__ __forceinline__ void reduce_body(const double *partial, int nb, doubl
     }
 }
 
+// w[r] -= sum_j V[r, j] h[j] for the row's whole column list, in column
+// order; the columns' loads go out eight at a time (clamped to column m - 1,
+// their terms dropped) before any is used.
+template <typename V>
+__device__ __forceinline__ void update_row(const V *__restrict__ Vb, int64_t n, int m, const double *h, int64_t r,
+                                           double &wr, double &wi) {
+    using T = VTraits<V>;
+    for (int j0 = 0; j0 < m; j0 += 8) {
+        V vv[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) vv[c] = Vb[(int64_t)(j0 + c < m ? j0 + c : m - 1) * n + r];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            if (j0 + c < m) {
+                double vr, vi;
+                T::load(&vv[c], 0, vr, vi);
+                const double hr = h[2 * (j0 + c)], hi = h[2 * (j0 + c) + 1];
+                wr -= vr * hr - vi * hi;
+                wi -= vr * hi + vi * hr;
+            }
+        }
+    }
+}
+
 // w[r] -= sum_j V[r, j] h[j]; partial[b] = sum |w[r]|^2 (of the stored values)
 template <typename V>
 __device__ __forceinline__ void update_body(const V *__restrict__ Vb, int64_t n, int m, const double *h,
@@ -135,13 +191,7 @@ __device__ __forceinline__ void update_body(const V *__restrict__ Vb, int64_t n,
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
         double wr, wi;
         T::load(w, r, wr, wi);
-        for (int j = 0; j < m; j++) {
-            double vr, vi;
-            T::load(Vb, (int64_t)j * n + r, vr, vi);
-            double hr = h[2 * j], hi = h[2 * j + 1];
-            wr -= vr * hr - vi * hi;
-            wi -= vr * hi + vi * hr;
-        }
+        if (m > 0) update_row<V>(Vb, n, m, h, r, wr, wi);
         T::store(w, r, wr, wi);
         T::load(w, r, wr, wi);
         nrm += wr * wr + wi * wi;
@@ -151,6 +201,65 @@ __device__ __forceinline__ void update_body(const V *__restrict__ Vb, int64_t n,
     if (threadIdx.x == 0) {
         partial[2 * b] = nrm;
         partial[2 * b + 1] = 0;
+    }
+}
+
+// The first pass's update fused with the second pass's Gram product (m <=
+// MX columns, held in registers between the two): w' = w - V h1 (the same
+// per-row arithmetic as update_body), partial[j] = conj(V_j) . w' for j < m
+// and partial[m] = |w'|^2.  One read of V instead of two.
+template <typename V, int MX>
+__device__ __forceinline__ void update_gram_body(const V *__restrict__ Vb, int64_t n, int m, const double *h,
+                                                 V *__restrict__ w, double *partial, int b, int nb, double *sh) {
+    using T = VTraits<V>;
+    int64_t r0, r1;
+    row_range(n, b, nb, r0, r1);
+    double gr[MX], gi[MX];
+#pragma unroll
+    for (int c = 0; c < MX; c++) { gr[c] = 0; gi[c] = 0; }
+    double nrm = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
+        V vv[MX];
+#pragma unroll
+        for (int c = 0; c < MX; c++) vv[c] = Vb[(int64_t)(c < m ? c : m - 1) * n + r];
+        double wr, wi;
+        T::load(w, r, wr, wi);
+#pragma unroll
+        for (int c = 0; c < MX; c++) {
+            if (c < m) {
+                double vr, vi;
+                T::load(&vv[c], 0, vr, vi);
+                const double hr = h[2 * c], hi = h[2 * c + 1];
+                wr -= vr * hr - vi * hi;
+                wi -= vr * hi + vi * hr;
+            }
+        }
+        T::store(w, r, wr, wi);
+        T::load(w, r, wr, wi);
+        nrm += wr * wr + wi * wi;
+#pragma unroll
+        for (int c = 0; c < MX; c++) {
+            if (c < m) {
+                double vr, vi;
+                T::load(&vv[c], 0, vr, vi);
+                gr[c] += vr * wr + vi * wi;
+                gi[c] += vr * wi - vi * wr;
+            }
+        }
+    }
+    for (int c = 0; c < m; c++) {
+        double re = gr[c], im = gi[c];
+        block_reduce2(re, im, sh);
+        if (threadIdx.x == 0) {
+            partial[((int64_t)c * nb + b) * 2] = re;
+            partial[((int64_t)c * nb + b) * 2 + 1] = im;
+        }
+    }
+    double dummy = 0;
+    block_reduce2(nrm, dummy, sh);
+    if (threadIdx.x == 0) {
+        partial[((int64_t)m * nb + b) * 2] = nrm;
+        partial[((int64_t)m * nb + b) * 2 + 1] = 0;
     }
 }
 
@@ -203,21 +312,42 @@ __device__ __forceinline__ void axpby_body(double ar, double ai, const V *__rest
 }
 
 // Y[:, i] = sum_j V[:, j] C[j + i*m]   (C interleaved complex, column-major m x k)
+// Outputs in groups of eight per pass over the row; each group reads the
+// row's basis values eight columns at a time (clamped loads, all issued
+// before use) and adds them in column order, so each output's sum has the
+// order of a plain loop over j.
 template <typename V>
 __device__ __forceinline__ void combine_body(const V *__restrict__ Vb, int64_t n, int m, const double *C, int k,
                                              V *__restrict__ Y, int b, int nb) {
     using T = VTraits<V>;
     for (int64_t r = (int64_t)b * blockDim.x + threadIdx.x; r < n; r += (int64_t)nb * blockDim.x) {
-        for (int i = 0; i < k; i++) {
-            double ar = 0, ai = 0;
-            for (int j = 0; j < m; j++) {
-                double vr, vi;
-                T::load(Vb, (int64_t)j * n + r, vr, vi);
-                double cr = C[2 * ((int64_t)i * m + j)], ci = C[2 * ((int64_t)i * m + j) + 1];
-                ar += vr * cr - vi * ci;
-                ai += vr * ci + vi * cr;
+        for (int i0 = 0; i0 < k; i0 += 8) {
+            double ar[8], ai[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) { ar[q] = 0; ai[q] = 0; }
+            for (int j0 = 0; j0 < m; j0 += 8) {
+                V vv[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) vv[c] = Vb[(int64_t)(j0 + c < m ? j0 + c : m - 1) * n + r];
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    if (j0 + c >= m) break;
+                    double vr, vi;
+                    T::load(&vv[c], 0, vr, vi);
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        if (i0 + q < k) {
+                            const int64_t e = (int64_t)(i0 + q) * m + j0 + c;
+                            const double cr = C[2 * e], ci = C[2 * e + 1];
+                            ar[q] += vr * cr - vi * ci;
+                            ai[q] += vr * ci + vi * cr;
+                        }
+                    }
+                }
             }
-            T::store(Y, (int64_t)i * n + r, ar, ai);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (i0 + q < k) T::store(Y, (int64_t)(i0 + q) * n + r, ar[q], ai[q]);
         }
     }
 }
@@ -327,6 +457,36 @@ __global__ __launch_bounds__(kVB) void k_update(const V *__restrict__ Vb, int64_
     __shared__ double sh[2 * kVB];
     if (dh && dgks_skip(dh, dm, dn)) return;
     update_body<V>(Vb, n, m, h, w, partial, blockIdx.x, gridDim.x, sh);
+}
+
+// The fused first-pass update and second-pass Gram product (m <= MX).
+template <typename V, int MX>
+__global__ __launch_bounds__(kVB) void k_update_gram(const V *__restrict__ Vb, int64_t n, int m, const double *h,
+                                                     V *__restrict__ w, double *partial) {
+    __shared__ double sh[2 * kVB];
+    update_gram_body<V, MX>(Vb, n, m, h, w, partial, blockIdx.x, gridDim.x, sh);
+}
+
+// The second pass's update after k_update_gram: raw = the second pass's
+// reduced Gram coefficients (m complex) followed by |w'|^2.  DGKS decides on
+// the device: when no second pass is needed w stays as it is.  Block 0
+// writes the second pass's coefficients (zeros when skipped) to h2 and
+// |w'|^2 to n2a.
+template <typename V>
+__global__ __launch_bounds__(kVB) void k_update_dgks(const V *__restrict__ Vb, int64_t n, int m, const double *h1,
+                                                     const double *raw, V *__restrict__ w, double *partial,
+                                                     double *h2, double *n2a) {
+    __shared__ double sh[2 * kVB];
+    const bool skip = dgks_skip(h1, m, raw + 2 * m);
+    if (blockIdx.x == 0) {
+        for (int t = threadIdx.x; t < 2 * m; t += kVB) h2[t] = skip ? 0.0 : raw[t];
+        if (threadIdx.x == 0) {
+            n2a[0] = raw[2 * m];
+            n2a[1] = 0.0;
+        }
+    }
+    if (skip) return;
+    update_body<V>(Vb, n, m, raw, w, partial, blockIdx.x, gridDim.x, sh);
 }
 
 // ------------------------------------------------- block Gram on MFMA ----
@@ -687,6 +847,41 @@ void gram_update_dev(int dtype, const void *Vmat, int64_t n, int m, const double
     if (nrm2_out)
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, nrm2_out, dgks_h1, dgks_n2, m,
                            1);
+    QS_HIP(hipGetLastError());
+}
+
+void gram_dgks_dev(int dtype, const void *Vmat, int64_t n, int m, void *w, VecWork &wk, double *h1, double *h2,
+                   double *n2a, double *n2, hipStream_t s) {
+    const int nb = vec_blocks(n);
+    gram_dots_dev(dtype, Vmat, n, m, w, wk, h1, s);
+    const size_t need = (size_t)2 * nb * (m + 1) + (size_t)2 * (m + 1);
+    if (m > kDgksFuse || wk.partial.n < need) {
+        gram_update_dev(dtype, Vmat, n, m, h1, w, wk, n2a, s);
+        gram_dots_dev(dtype, Vmat, n, m, w, wk, h2, s, h1, n2a);
+        gram_update_dev(dtype, Vmat, n, m, h2, w, wk, n2, s, h1, n2a);
+        return;
+    }
+    double *raw = wk.partial.get() + (size_t)2 * nb * (m + 1);
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        if (m <= 8)
+            hipLaunchKernelGGL((k_update_gram<V, 8>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, h1, (V *)w,
+                               wk.partial.get());
+        else if (m <= 16)
+            hipLaunchKernelGGL((k_update_gram<V, 16>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, h1, (V *)w,
+                               wk.partial.get());
+        else
+            hipLaunchKernelGGL((k_update_gram<V, kDgksFuse>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, h1,
+                               (V *)w, wk.partial.get());
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(m + 1), dim3(kVB), 0, s, wk.partial.get(), nb, raw, (const double *)nullptr,
+                       (const double *)nullptr, 0, 0);
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_update_dgks<V>), dim3(nb), dim3(kVB), 0, s, (const V *)Vmat, n, m, h1, raw, (V *)w,
+                           wk.partial.get(), h2, n2a);
+    });
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, wk.partial.get(), nb, n2, h1, raw + 2 * m, m, 1);
     QS_HIP(hipGetLastError());
 }
 

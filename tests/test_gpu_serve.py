@@ -110,3 +110,38 @@ def test_errors_reach_the_caller(gpu):
         t = srv.submit_eigsh(op, k=1)
         res = srv.wait(t)
         assert res["vals"].shape == (1,)
+
+
+def test_concurrent_jobs_on_one_operator(gpu):
+    # one operator shared by overlapping jobs of different element types: a
+    # parent-table sector of more than 2^17 rows, whose products pre-scale x
+    # into a buffer per stream (L=26 nup=13 k=0: 400,024 rows).  Every result
+    # equals the same job run alone.
+    print("serve shared operator", flush=True)
+    L = 26
+    b = qsgpu.SpinBasis1D(L, nup=13, pauli=False, kblock=0)
+    assert b.Ns > 1 << 17
+    op = qsgpu.QuantumLinearOperator(b, serve.chain_terms(L, 0.3, 0.8))
+    rng = np.random.default_rng(31)
+    v0 = rng.standard_normal(b.Ns)
+    v1 = rng.standard_normal(b.Ns) + 1j * rng.standard_normal(b.Ns)
+    v1 /= np.linalg.norm(v1)
+    jobs = [("eigsh", np.float64, 2), ("evolve", np.complex128, 0.7), ("eigsh", np.complex128, 1),
+            ("evolve", np.complex64, 0.4), ("eigsh", np.float32, 1), ("evolve", np.complex128, 1.3)]
+
+    def submit(srv, job):
+        kind, dt, arg = job
+        if kind == "eigsh":
+            return srv.submit_eigsh(op, k=arg, which="SA", tol=1e-10, v0=v0, dtype=dt)
+        return srv.submit_evolve(op, v1, 0.0, [arg], dtype=dt)
+
+    def key(job):
+        return "vals" if job[0] == "eigsh" else "states"
+
+    with serve.Server(n_workers=1, hbm_budget=50e9) as srv:
+        alone = [srv.wait(submit(srv, j)) for j in jobs]
+    with serve.Server(n_workers=4, hbm_budget=50e9) as srv:
+        tickets = [submit(srv, j) for j in jobs]
+        mix = [srv.wait(t) for t in tickets]
+    for j, a, m in zip(jobs, alone, mix):
+        assert np.array_equal(a[key(j)], m[key(j)]), j
