@@ -25,7 +25,7 @@ bool dtype_complex(int dtype) { return dtype == QS_C128 || dtype == QS_C64; }
 namespace {
 
 constexpr int kVB = 256;
-constexpr int kMaxBlocks = 1024;
+constexpr int kMaxBlocks = 2048;
 constexpr int kCols = 8;   // Gram columns per pass
 constexpr int kDgksFuse = 24;   // basis columns the fused update + Gram holds in registers
 
@@ -156,31 +156,39 @@ __device__ __forceinline__ void reduce_body(const double *partial, int nb, doubl
     }
 }
 
-// w[r] -= sum_j V[r, j] h[j] for the row's whole column list, in column
-// order; the columns' loads go out eight at a time (clamped to column m - 1,
-// their terms dropped) before any is used.
+// w[ra], w[rb] -= sum_j V[., j] h[j] for the rows' whole column list, in
+// column order; the columns' loads for both rows go out eight at a time
+// (clamped to column m - 1, their terms dropped) before any is used.
 template <typename V>
-__device__ __forceinline__ void update_row(const V *__restrict__ Vb, int64_t n, int m, const double *h, int64_t r,
-                                           double &wr, double &wi) {
+__device__ __forceinline__ void update_rows(const V *__restrict__ Vb, int64_t n, int m, const double *h, int64_t ra,
+                                            int64_t rb, double &war, double &wai, double &wbr, double &wbi) {
     using T = VTraits<V>;
     for (int j0 = 0; j0 < m; j0 += 8) {
-        V vv[8];
+        V va[8], vb[8];
 #pragma unroll
-        for (int c = 0; c < 8; c++) vv[c] = Vb[(int64_t)(j0 + c < m ? j0 + c : m - 1) * n + r];
+        for (int c = 0; c < 8; c++) {
+            const int64_t col = (int64_t)(j0 + c < m ? j0 + c : m - 1) * n;
+            va[c] = Vb[col + ra];
+            vb[c] = Vb[col + rb];
+        }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             if (j0 + c < m) {
-                double vr, vi;
-                T::load(&vv[c], 0, vr, vi);
                 const double hr = h[2 * (j0 + c)], hi = h[2 * (j0 + c) + 1];
-                wr -= vr * hr - vi * hi;
-                wi -= vr * hi + vi * hr;
+                double vr, vi;
+                T::load(&va[c], 0, vr, vi);
+                war -= vr * hr - vi * hi;
+                wai -= vr * hi + vi * hr;
+                T::load(&vb[c], 0, vr, vi);
+                wbr -= vr * hr - vi * hi;
+                wbi -= vr * hi + vi * hr;
             }
         }
     }
 }
 
-// w[r] -= sum_j V[r, j] h[j]; partial[b] = sum |w[r]|^2 (of the stored values)
+// w[r] -= sum_j V[r, j] h[j]; partial[b] = sum |w[r]|^2 (of the stored values).
+// Rows go in pairs (r, r + kVB), the second dropped past the range end.
 template <typename V>
 __device__ __forceinline__ void update_body(const V *__restrict__ Vb, int64_t n, int m, const double *h,
                                             V *__restrict__ w, double *partial, int b, int nb, double *sh) {
@@ -188,13 +196,21 @@ __device__ __forceinline__ void update_body(const V *__restrict__ Vb, int64_t n,
     int64_t r0, r1;
     row_range(n, b, nb, r0, r1);
     double nrm = 0;
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kVB) {
-        double wr, wi;
-        T::load(w, r, wr, wi);
-        if (m > 0) update_row<V>(Vb, n, m, h, r, wr, wi);
-        T::store(w, r, wr, wi);
-        T::load(w, r, wr, wi);
-        nrm += wr * wr + wi * wi;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 2 * kVB) {
+        const bool hb = r + kVB < r1;
+        const int64_t rb = hb ? r + kVB : r;
+        double war, wai, wbr, wbi;
+        T::load(w, r, war, wai);
+        T::load(w, rb, wbr, wbi);
+        if (m > 0) update_rows<V>(Vb, n, m, h, r, rb, war, wai, wbr, wbi);
+        T::store(w, r, war, wai);
+        T::load(w, r, war, wai);
+        nrm += war * war + wai * wai;
+        if (hb) {
+            T::store(w, rb, wbr, wbi);
+            T::load(w, rb, wbr, wbi);
+            nrm += wbr * wbr + wbi * wbi;
+        }
     }
     double dummy = 0;
     block_reduce2(nrm, dummy, sh);
