@@ -179,6 +179,41 @@ def bench_c3(args, skip_eigsh=False):
     return out, eig
 
 
+def bench_gram(n=1 << 23, m=32, p=32, repeats=5):
+    """The block Gram product G = V^H W of the Krylov path on the matrix cores (k_gram_mfma,
+    v_mfma_f64_16x16x4f64, FP32 widened to FP64), V: n x m, W: n x p resident on the device; wall
+    time per call (launches, the cross-block reduction and the m x p copy to the host included)."""
+    import ctypes
+    import numpy as np
+    import torch
+    from qsgpu import _lib
+    print("bench: block Gram on MFMA", file=sys.stderr, flush=True)
+    out = {"n": n, "m": m, "p": p}
+    for dt, name, code in ((torch.float64, "fp64", _lib.QS_F64), (torch.float32, "fp32", _lib.QS_F32)):
+        g = torch.Generator(device="cpu").manual_seed(3)
+        V = torch.randn(m, n, generator=g, dtype=torch.float64).to(dt).cuda()   # column j = row j here
+        W = torch.randn(p, n, generator=g, dtype=torch.float64).to(dt).cuda()
+        G = np.zeros(2 * m * p)
+        call = lambda: _lib.lib().qs_block_gram(code, ctypes.c_void_p(V.data_ptr()), n, m,
+                                                 ctypes.c_void_p(W.data_ptr()), p, 3, G.ctypes.data)
+        assert call() == 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            call()
+        t = (time.perf_counter() - t0) / repeats
+        exact = (V.double() @ W.double().T).cpu().numpy()
+        got = G[0::2].reshape((m, p), order="F")
+        esz = 8 if dt == torch.float64 else 4
+        out[name] = {"ms": round(t * 1e3, 4), "TFLOPs": round(2.0 * n * m * p / t / 1e12, 3),
+                     "GBps": round(n * (m + p) * esz / t / 1e9, 1),
+                     "max_rel_err_vs_fp64": float(np.abs(got - exact).max() / np.abs(exact).max())}
+        del V, W
+        torch.cuda.empty_cache()
+    out["note"] = "tall-skinny: 2 m p / ((m + p) s) flop per byte, so HBM-bound; FP64 MFMA peak 78.6 TFLOP/s"
+    return out
+
+
 def bench_c3_eigsh(op3, n3):
     """Precision form: the k=4 lowest eigenvalues of the L=32 z+ sector in the FP32-stream form
     (FP32 Krylov basis, FP64 dots / norms / tridiagonal solve, FP64 Rayleigh quotients) and in
@@ -396,6 +431,7 @@ def main():
                     help="only the L=32 z+ matvec pair (profiling runs); implies --no-c3-eigsh unless --c3-eigsh")
     ap.add_argument("--c3-eigsh", action="store_true", help="with --c3-only: also the k=4 eigensolve")
     ap.add_argument("--c3-variant", type=int, default=0, help="pin the C3 kernel variant instead of autotuning")
+    ap.add_argument("--gram-only", action="store_true", help="only the block Gram product on the matrix cores")
     ap.add_argument("--c4-times", type=int, default=200)
     ap.add_argument("--c5-requests", type=int, default=200)
     ap.add_argument("--c5-workers", type=int, default=8)
@@ -414,6 +450,9 @@ def main():
     import qsgpu
 
     dev = qsgpu.device_info()
+    if args.gram_only:
+        print(json.dumps({"gram_mfma": bench_gram()}))
+        return
     if args.c3_only:
         res = bench_c3(args, skip_eigsh=not args.c3_eigsh)
         print(json.dumps({"c3_matvec": res[0], "c3_eigsh": res[1]}))
@@ -502,6 +541,7 @@ def main():
         result["c3_matvec"], c3e = bench_c3(args, skip_eigsh=args.no_c3_eigsh)
         if c3e is not None:
             result["c3_eigsh"] = c3e
+    result["gram_mfma"] = bench_gram()
     if not args.no_c4:
         result["c4_block_evolve"] = bench_c4(args.c4_times)
     if not args.no_c5:

@@ -83,6 +83,20 @@ int qs_device_info(char *name, size_t len, int64_t *hbm_bytes, int *n_cu);
  * ascending order, which is the reference's column order).  flags: 0. */
 int qs_basis_create(int L, int nup, int pauli, int a, int kblock, int pblock, int zblock,
                     int pzblock, int zAblock, int zBblock, int flags, qs_basis **out);
+/* SpinBasisGeneral(L; Nup, pauli, block_order, blocks...) -- basis.jl:530-620
+ * (spin 1/2): the nup sector (QS_UNSET: every state) reduced by nblocks site
+ * maps in stage order (the reference's block_order).  maps: nblocks x L
+ * entries, each row one map in QuSpin's zero-based labels (any entry zero or
+ * negative; -v inverts the site and sends it to site v) or Julia's one-based
+ * labels (negative = inverted), as _normalize_general_spin_map reads them
+ * (basis.jl:282-304); qnumbers: each block's quantum number q, eigenvalue
+ * cis(2 pi mod(q, order) / order) with the map's order (basis.jl:306-323).
+ * L <= 32 on the GPU; the maps may generate at most 256 elements.  Sectors
+ * of such a basis take two-bit hop couplings (the parent-table kernels).
+ * Errors: the reference's ArgumentError messages for a map that is not a
+ * permutation, a flip that leaves the particle sector, an empty sector. */
+int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_t *maps, const int32_t *qnumbers,
+                            int flags, qs_basis **out);
 int qs_basis_destroy(qs_basis *b);
 
 /* length(basis) / basis.Ns -- basis.jl:779 */
@@ -273,10 +287,11 @@ int qs_lanczos(qs_op *op, int dtype, const void *v0, int m, int full_ortho, doub
 /* The Gram products of the Gram-Schmidt step, G = V^H W -- the dot(basis,
  * residual) coefficients of _krylov_factorization (operators.jl:1697-1719),
  * for a block W of p vectors at once.  V: Ns x m, W: Ns x p, column-major
- * host arrays of dtype; G: m x p interleaved complex (column-major), FP64
- * accumulation.  on_mfma != 0: the block kernel on the matrix cores (the one
- * the FP32-stream eigsh uses for its Rayleigh-Ritz products); 0: the VALU
- * Gram kernel of the Arnoldi step, one column of W at a time. */
+ * arrays of dtype; G: m x p interleaved complex (column-major), FP64
+ * accumulation.  Bit 0 of on_mfma: the block kernel on the matrix cores (the
+ * one the FP32-stream eigsh uses for its Rayleigh-Ritz products), else the
+ * VALU Gram kernel of the Arnoldi step, one column of W at a time; bit 1: V
+ * and W are device arrays (else host arrays, copied in). */
 int qs_block_gram(int dtype, const void *V, int64_t n, int m, const void *W, int p, int on_mfma, double *G);
 
 /* ------------------------------------------------ blocks of one parent space --

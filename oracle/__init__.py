@@ -37,6 +37,8 @@ def lib():
     L.qo_fixed_weight_states.restype = c_i64
     L.qo_sector_build.argtypes = [c_int] * 10
     L.qo_sector_build.restype = vp
+    L.qo_sector_build_general.argtypes = [c_int, c_int, c_int, c_int, vp, vp]
+    L.qo_sector_build_general.restype = vp
     for name in ("qo_sector_ncols", "qo_sector_nparent", "qo_sector_nnz"):
         getattr(L, name).argtypes = [vp]
         getattr(L, name).restype = c_i64
@@ -136,6 +138,25 @@ class Sector:
         self.Ns = lb.qo_sector_ncols(h)
         self.npar = lb.qo_sector_nparent(h)
         self.reduced = bool(lb.qo_sector_reduced(h))
+
+    @classmethod
+    def general(cls, L, nup=None, pauli=True, blocks=()):
+        """Reference-semantics SpinBasisGeneral (basis.jl:530-620): `blocks` is a
+        sequence of (map, quantum_number) in stage order, each map one entry per
+        site in QuSpin's zero-based or Julia's one-based convention."""
+        lb = lib()
+        maps = np.ascontiguousarray(np.asarray([list(m) for m, _ in blocks] or [[0]], dtype=np.int32))
+        qn = np.ascontiguousarray(np.asarray([int(q) for _, q in blocks] or [0], dtype=np.int32))
+        h = lb.qo_sector_build_general(L, _u(nup), int(bool(pauli)), len(blocks), _ptr(maps), _ptr(qn))
+        if not h:
+            raise ValueError(lb.qo_last_error().decode())
+        self = cls.__new__(cls)
+        self._h = h
+        self.L, self.nup, self.pauli = L, nup, pauli
+        self.Ns = lb.qo_sector_ncols(h)
+        self.npar = lb.qo_sector_nparent(h)
+        self.reduced = bool(lb.qo_sector_reduced(h))
+        return self
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -10,6 +10,7 @@
  * What it restates (reference file:line):
  *   qo_fixed_weight_states  basis.jl:229-250      Gosper enumeration, ascending
  *   qo_sector_build         basis.jl:622-777      stage order k, p, z, pz, zA, zB
+ *   qo_sector_build_general basis.jl:282-338,530-620  SpinBasisGeneral site maps, stages in block order
  *                           symmetry_basis.jl:38-55   signed permutation S
  *                           symmetry_basis.jl:57-108  cyclic projector (orbit walk)
  *                           symmetry_basis.jl:110-213 eigenspace intersection P'SP
@@ -353,9 +354,12 @@ static void csc_droptol(csc_t *a, double tol)
 }
 
 /* --------------------------------------------------------- transforms --- */
-enum { TR_TRANSLATE = 0, TR_PARITY, TR_FLIP, TR_PARITY_FLIP, TR_FLIP_A, TR_FLIP_B };
+enum { TR_TRANSLATE = 0, TR_PARITY, TR_FLIP, TR_PARITY_FLIP, TR_FLIP_A, TR_FLIP_B, TR_GENERAL };
 
-typedef struct { int kind; int L; int a; } transform_t;
+/* TR_GENERAL: a site map of SpinBasisGeneral (basis.jl:325-338): the bit of
+ * site `source` (1-based), inverted when flips[source - 1], moves to site
+ * perm[source - 1]. */
+typedef struct { int kind; int L; int a; const int *perm; const unsigned char *flips; } transform_t;
 
 static uint64_t full_mask(int L) { return L == 64 ? ~0ULL : ((1ULL << L) - 1); }
 
@@ -394,6 +398,13 @@ static uint64_t apply_transform(const transform_t *t, uint64_t s)
         for (int site = 2; site <= L; site += 2) m |= 1ULL << (site - 1);
         return s ^ m;
     }
+    case TR_GENERAL:
+        for (int source = 1; source <= L; source++) {
+            int occupied = (s >> (source - 1)) & 1;
+            if (t->flips[source - 1]) occupied = !occupied;
+            if (occupied) out |= 1ULL << (t->perm[source - 1] - 1);
+        }
+        return out;
     }
     return s;
 }
@@ -641,6 +652,119 @@ qo_sector *qo_sector_build(int L, int nup, int pauli, int a, int kblock, int pbl
         s->labels[c] = s->parent[s->P.row[best]];
     }
     return s;
+}
+
+/* basis.jl:282-304 (_normalize_general_spin_map): one entry per site; maps
+ * holding a zero or a negative entry are QuSpin's zero-based labels (a
+ * negative entry -v flips the site and sends it to site v), otherwise
+ * Julia's one-based labels (negative = flipped).  0 or an error message. */
+static int normalize_general_map(const int *values, int L, int *perm, unsigned char *flips)
+{
+    int zero_based = 0;
+    for (int i = 0; i < L; i++)
+        if (values[i] <= 0) zero_based = 1;
+    for (int i = 0; i < L; i++) {
+        int v = values[i];
+        if (zero_based) {
+            perm[i] = v >= 0 ? v + 1 : -v;
+            flips[i] = v < 0;
+        } else {
+            perm[i] = v < 0 ? -v : v;
+            flips[i] = v < 0;
+        }
+    }
+    char *hit = calloc((size_t)L + 1, 1);
+    int ok = 1;
+    for (int i = 0; i < L; i++) {
+        if (perm[i] < 1 || perm[i] > L || hit[perm[i]]) ok = 0;
+        else hit[perm[i]] = 1;
+    }
+    free(hit);
+    if (!ok) { set_err("a general symmetry map must be a site permutation"); return -1; }
+    return 0;
+}
+
+static int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a; }
+
+/* basis.jl:306-323: lcm over cycles of the cycle length, doubled when the
+ * cycle flips an odd number of sites */
+static int general_map_order(const int *perm, const unsigned char *flips, int L)
+{
+    char *visited = calloc((size_t)L, 1);
+    int64_t order = 1;
+    for (int start = 0; start < L; start++) {
+        if (visited[start]) continue;
+        int cur = start, len = 0, parity = 0;
+        while (!visited[cur]) {
+            visited[cur] = 1;
+            len++;
+            parity ^= flips[cur];
+            cur = perm[cur] - 1;
+        }
+        int64_t c = parity ? 2 * len : len;
+        order = order / gcd64(order, c) * c;
+    }
+    free(visited);
+    return (int)order;
+}
+
+/* basis.jl:530-620 (SpinBasisGeneral, L <= 63, spin 1/2): the parent is the
+ * nup sector (or every state), and each block in the given order adds one
+ * stage: the cyclic projector for the first, P'SP intersections after it,
+ * with eigenvalue cis(2 pi mod(q, order) / order).  maps: nblocks x L. */
+qo_sector *qo_sector_build_general(int L, int nup, int pauli, int nblocks, const int *maps, const int *qnumbers)
+{
+    if (L < 1 || L > 30) { set_err("sector too large for the oracle"); return NULL; }
+    if (nup != QO_UNSET && (nup < 0 || nup > L)) { set_err("nup must lie between 0 and L"); return NULL; }
+    qo_sector *s = calloc(1, sizeof *s);
+    s->L = L; s->nup = nup; s->pauli = pauli;
+    int w = nup == QO_UNSET ? -1 : nup;
+    s->npar = qo_fixed_weight_states(L, w, NULL);
+    s->parent = malloc(sizeof(uint64_t) * (size_t)s->npar);
+    qo_fixed_weight_states(L, w, s->parent);
+    if (nblocks == 0) {
+        s->reduced = 0;
+        s->ncol = s->npar;
+        s->labels = malloc(sizeof(uint64_t) * (size_t)s->npar);
+        memcpy(s->labels, s->parent, sizeof(uint64_t) * (size_t)s->npar);
+        return s;
+    }
+    int *perm = malloc(sizeof(int) * (size_t)L);
+    unsigned char *flips = malloc((size_t)L);
+    for (int bi = 0; bi < nblocks; bi++) {
+        if (normalize_general_map(maps + (size_t)bi * L, L, perm, flips)) goto fail;
+        int order = general_map_order(perm, flips, L);
+        int m = ((qnumbers[bi] % order) + order) % order;
+        double ang = (6.283185307179586 * (double)m) / (double)order;
+        cplx lam = c_make(cos(ang), sin(ang));
+        transform_t tr = {TR_GENERAL, L, 1, perm, flips};
+        if (bi == 0) {
+            if (cyclic_projector(&s->P, s->parent, s->npar, &tr, lam)) goto fail;
+        } else {
+            csc_t S = {0};
+            if (signed_permutation(&S, s->parent, s->npar, &tr)) { csc_free(&S); goto fail; }
+            int rc = intersect_eigenspace(&s->P, &S, lam);
+            csc_free(&S);
+            if (rc) goto fail;
+        }
+    }
+    free(perm); free(flips);
+    if (s->P.n == 0) { set_err("the requested symmetry sector is empty"); qo_sector_free(s); return NULL; }
+    s->reduced = 1;
+    s->ncol = s->P.n;
+    /* symmetry_basis.jl:215-232 */
+    s->labels = malloc(sizeof(uint64_t) * (size_t)s->ncol);
+    for (int64_t c = 0; c < s->ncol; c++) {
+        int64_t p0 = s->P.colptr[c], best = p0;
+        for (int64_t p = p0 + 1; p < s->P.colptr[c + 1]; p++)
+            if (c_abs(s->P.val[p]) > c_abs(s->P.val[best])) best = p;
+        s->labels[c] = s->parent[s->P.row[best]];
+    }
+    return s;
+fail:
+    free(perm); free(flips);
+    qo_sector_free(s);
+    return NULL;
 }
 
 int64_t qo_sector_ncols(const qo_sector *s) { return s->ncol; }

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <numeric>
 
 #include "qs_internal.h"
 
@@ -65,6 +66,16 @@ uint64_t canon_host(const BasisHost &b, uint64_t u, cd &chi) {
     const GroupHost &g = b.group;
     uint64_t best = ~0ULL;
     chi = cd(1, 0);
+    if (g.general) {
+        for (int e = 0; e < g.order(); e++) {
+            const uint64_t w = group_apply(g, e, u);
+            if (w < best) {
+                best = w;
+                chi = g.echar[(size_t)e];
+            }
+        }
+        return best;
+    }
     for (int p = 0; p < g.npoint; p++) {
         uint64_t v = apply_point(g, p, u);
         for (int t = 0; t < g.nT; t++) {
@@ -224,6 +235,8 @@ void setup_terms(OpHost &o, const std::vector<CTerm> &Havg, const double *diag) 
     o.fast_adj = compile_fast(o.diag_terms_adj, o.off_terms_adj, o.groups_adj, bb.L, bb.nup);
     upload_fast(o.fast, o.d_hops, o.d_dcls, o.d_drest);
     upload_fast(o.fast_adj, o.d_hops_adj, o.d_dcls_adj, o.d_drest_adj);
+    if (bb.group.general && !(hop_eligible(o, false) && hop_eligible(o, true)))
+        fail(QS_EARG, "a basis of general symmetry maps takes two-bit hop couplings (the parent-table kernels)");
     if (o.has_user_diag) {
         o.d_user_diag.alloc((size_t)2 * bb.Ns);
         QS_HIP(hipMemcpy(o.d_user_diag.get(), diag, sizeof(double) * 2 * bb.Ns, hipMemcpyHostToDevice));
@@ -475,6 +488,75 @@ int qs_basis_create(int L, int nup, int pauli, int a, int kblock, int pblock, in
     });
 }
 
+int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_t *maps, const int32_t *qnumbers,
+                            int flags, qs_basis **out) {
+    (void)flags;
+    return guard([&] {
+        check_ptr(out, "out");
+        *out = nullptr;
+        if (L < 1 || L > 63) fail(QS_EARG, "L must be between 1 and 63");
+        if (nup != QS_UNSET && (nup < 0 || nup > L)) fail(QS_EARG, "nup must lie between 0 and L");
+        if (nblocks < 0) fail(QS_EARG, "nblocks must be non-negative");
+        if (nblocks > 0) {
+            check_ptr(maps, "maps");
+            check_ptr(qnumbers, "qnumbers");
+        }
+        auto h = std::make_unique<qs_basis>();
+        BasisHost &b = h->b;
+        b.L = L;
+        b.nup = nup == QS_UNSET ? -1 : nup;
+        b.pauli = pauli ? 1 : 0;
+        b.a = 1;
+        for (int i = 0; i < 6; i++) b.blocks[i] = QS_UNSET;
+        if (b.nup < 0 && L > 40) fail(QS_EARG, "a basis without nup is limited to L <= 40 here");
+        for (int bi = 0; bi < nblocks; bi++) {
+            // basis.jl:282-304: QuSpin's zero-based labels when any entry is
+            // zero or negative (-v: flipped, to site v), else one-based
+            const int32_t *v = maps + (size_t)bi * L;
+            bool zero_based = false;
+            for (int i = 0; i < L; i++) zero_based |= v[i] <= 0;
+            GenMap m;
+            m.perm.assign((size_t)L, -1);
+            std::vector<char> hit((size_t)L + 1, 0);
+            bool ok = true;
+            for (int i = 0; i < L; i++) {
+                const int64_t target = zero_based ? (v[i] >= 0 ? (int64_t)v[i] + 1 : -(int64_t)v[i])
+                                                  : (v[i] < 0 ? -(int64_t)v[i] : (int64_t)v[i]);
+                if (v[i] < 0) m.flip |= 1ULL << i;
+                if (target < 1 || target > L || hit[(size_t)target]) {
+                    ok = false;
+                    break;
+                }
+                hit[(size_t)target] = 1;
+                m.perm[(size_t)i] = (int)target - 1;
+            }
+            if (!ok) fail(QS_EARG, "a general symmetry map must be a site permutation");
+            // basis.jl:306-323: lcm of the cycle lengths, doubled for cycles with an odd number of flips
+            std::vector<char> seen((size_t)L, 0);
+            int64_t order = 1;
+            for (int st = 0; st < L; st++) {
+                if (seen[(size_t)st]) continue;
+                int cur = st, len = 0, par = 0;
+                while (!seen[(size_t)cur]) {
+                    seen[(size_t)cur] = 1;
+                    len++;
+                    par ^= (int)(m.flip >> cur & 1);
+                    cur = m.perm[(size_t)cur];
+                }
+                const int64_t c = par ? 2 * len : len;
+                order = order / std::gcd(order, c) * c;
+                if (order > kMaxGeneral) fail(QS_EARG, "symmetry group too large");
+            }
+            m.order = (int)order;
+            m.q = qnumbers[bi];
+            b.gmaps.push_back(m);
+        }
+        build_group_general(b);
+        build_basis_device(b);
+        *out = h.release();
+    });
+}
+
 int qs_basis_destroy(qs_basis *b) {
     delete b;
     return QS_OK;
@@ -689,17 +771,22 @@ int qs_block_gram(int dtype, const void *V, int64_t n, int m, const void *W, int
             ~StreamGuard() { (void)hipStreamDestroy(s); }
         } sg{s};
         VecWork wk(s);
-        SBuf<char> dv((size_t)n * m * esz, s), dw((size_t)n * p * esz, s);
-        QS_HIP(hipMemcpyAsync(dv.get(), V, (size_t)n * m * esz, hipMemcpyHostToDevice, s));
-        QS_HIP(hipMemcpyAsync(dw.get(), W, (size_t)n * p * esz, hipMemcpyHostToDevice, s));
+        // bit 1 of the flags: V and W are device arrays
+        const bool dev = (on_mfma & 2) != 0;
+        SBuf<char> dv(dev ? 0 : (size_t)n * m * esz, s), dw(dev ? 0 : (size_t)n * p * esz, s);
+        if (!dev) {
+            QS_HIP(hipMemcpyAsync(dv.get(), V, (size_t)n * m * esz, hipMemcpyHostToDevice, s));
+            QS_HIP(hipMemcpyAsync(dw.get(), W, (size_t)n * p * esz, hipMemcpyHostToDevice, s));
+        }
+        const char *pv = dev ? (const char *)V : dv.get(), *pw = dev ? (const char *)W : dw.get();
         std::vector<cd> g;
-        if (on_mfma) {
-            gram_block(dtype, dv.get(), n, m, dw.get(), p, wk, s, g);
+        if (on_mfma & 1) {
+            gram_block(dtype, pv, n, m, pw, p, wk, s, g);
         } else {
             g.resize((size_t)m * p);
             for (int c = 0; c < p; c++) {
                 std::vector<cd> h;
-                gram_dots(dtype, dv.get(), n, m, dw.get() + (size_t)c * n * esz, wk, s, h);
+                gram_dots(dtype, pv, n, m, pw + (size_t)c * n * esz, wk, s, h);
                 for (int i = 0; i < m; i++) g[(size_t)c * m + i] = h[i];
             }
         }

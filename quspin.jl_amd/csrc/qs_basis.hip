@@ -198,7 +198,135 @@ __global__ __launch_bounds__(kBlock) void k_ptab(GroupDev g, int nup, int64_t np
     }
 }
 
+// ------------------------------------------------- general site maps ----
+// A SpinBasisGeneral group on the device (L <= 32): element e maps u to
+// T_e[u & 255] | T_e[256 + (u >> 8 & 255)] | T_e[512 + (u >> 16 & 255)] |
+// T_e[768 + (u >> 24)], xor c_e (tab: 1024 words per element), with
+// character chi[2 e] + i chi[2 e + 1]; element 0 is the identity.
+struct GenGroup {
+    int n, L;
+    const uint32_t *tab, *c;
+    const double *chi;
+};
+
+__device__ __forceinline__ uint32_t gen_apply(const GenGroup &g, int e, uint32_t u) {
+    const uint32_t *t = g.tab + (size_t)e * 1024;
+    return (t[u & 255] | t[256 + ((u >> 8) & 255)] | t[512 + ((u >> 16) & 255)] | t[768 + (u >> 24)]) ^ g.c[e];
+}
+
+// the stabiliser size when u is its orbit's minimum and the character is
+// trivial on its stabiliser, else 0 (keep_rep for the general group)
+__device__ int keep_rep_gen(const GenGroup &g, uint32_t u) {
+    int cnt = 0;
+    double cre = 0, cim = 0;
+    for (int e = 0; e < g.n; e++) {
+        const uint32_t w = gen_apply(g, e, u);
+        if (w < u) return 0;
+        if (w == u) {
+            cnt++;
+            cre += g.chi[2 * e];
+            cim += g.chi[2 * e + 1];
+        }
+    }
+    return (cre * cre + cim * cim > 0.25) ? cnt : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_gen(GenGroup g, int nup, int64_t npar, const uint64_t *binom,
+                                                      uint32_t *block_count) {
+    __shared__ uint32_t red[kBlock];
+    int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    uint32_t cnt = 0;
+    if (base < npar) {
+        uint32_t u = parent_state<uint32_t>(base, g.L, nup, binom);
+        int64_t end = base + kChunk < npar ? base + kChunk : npar;
+        for (int64_t i = base; i < end; i++) {
+            if (keep_rep_gen(g, u)) cnt++;
+            if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : u + 1;
+        }
+    }
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_count[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_write_gen(GenGroup g, int nup, int64_t npar, const uint64_t *binom,
+                                                      const uint64_t *block_offset, uint32_t *reps, uint32_t *orbit,
+                                                      double *inv_nrm) {
+    __shared__ uint32_t scan[kBlock];
+    int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    uint32_t cnt = 0;
+    uint32_t u0 = 0;
+    int64_t end = base;
+    if (base < npar) {
+        u0 = parent_state<uint32_t>(base, g.L, nup, binom);
+        end = base + kChunk < npar ? base + kChunk : npar;
+        uint32_t u = u0;
+        for (int64_t i = base; i < end; i++) {
+            if (keep_rep_gen(g, u)) cnt++;
+            if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : u + 1;
+        }
+    }
+    scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t pos = block_offset[blockIdx.x] + scan[threadIdx.x] - cnt;
+    if (cnt == 0) return;
+    uint32_t u = u0;
+    for (int64_t i = base; i < end; i++) {
+        const int st = keep_rep_gen(g, u);
+        if (st) {
+            const uint32_t d = (uint32_t)(g.n / st);
+            reps[pos] = u;
+            orbit[pos] = d;
+            inv_nrm[pos] = 1.0 / sqrt((double)d);
+            pos++;
+        }
+        if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : u + 1;
+    }
+}
+
+// Parent lookup table of a general group: column of the orbit minimum | the
+// first element reaching it << colbits (the element canon_host picks too)
+__global__ __launch_bounds__(kBlock) void k_ptab_gen(GenGroup g, int nup, int64_t npar, const uint64_t *binom,
+                                                     const uint64_t *hkeys, uint64_t hmask, int max_probe,
+                                                     int colbits, int track, uint32_t *ptab) {
+    const int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kChunk;
+    if (base >= npar) return;
+    uint32_t u = parent_state<uint32_t>(base, g.L, nup, binom);
+    const int64_t end = base + kChunk < npar ? base + kChunk : npar;
+    for (int64_t i = base; i < end; i++) {
+        uint32_t best = 0xFFFFFFFFu;
+        int be = 0;
+        for (int e = 0; e < g.n; e++) {
+            const uint32_t w = gen_apply(g, e, u);
+            if (w < best) {
+                best = w;
+                be = e;
+            }
+        }
+        const int64_t c = hash_find32(hkeys, hmask, max_probe, best);
+        uint32_t ent = 0xFFFFFFFFu;
+        if (c >= 0) ent = (uint32_t)c | (track ? (uint32_t)be << colbits : 0u);
+        ptab[i] = ent;
+        if (i + 1 < end) u = (nup >= 0) ? gosper_next(u) : u + 1;
+    }
+}
+
 bool group_tracks(const GroupHost &g) {
+    if (g.general) {
+        for (const cd &c : g.echar)
+            if (c != cd(1, 0)) return true;
+        return false;
+    }
     for (int p = 0; p < g.npoint; p++)
         if (g.pchar[p] != cd(1, 0)) return true;
     for (int t = 0; t < g.nT; t++)
@@ -230,6 +358,13 @@ void build_ptab(BasisHost &b) {
     GroupDev g = group_to_dev(b.group);
     const int64_t lanes = (b.npar + kChunk - 1) / kChunk;
     const unsigned nblk = (unsigned)((lanes + kBlock - 1) / kBlock);
+    if (b.group.general) {
+        GenGroup gg{order, b.L, b.gtab.get(), b.gc.get(), b.gchi.get()};
+        hipLaunchKernelGGL(k_ptab_gen, dim3(nblk), dim3(kBlock), 0, 0, gg, b.nup, b.npar, b.binom.get(),
+                           b.hkeys.get(), b.hmask, b.hmax_probe, colbits, (int)track, b.ptab.get());
+        QS_HIP(hipGetLastError());
+        return;
+    }
     if (track)
         hipLaunchKernelGGL((k_ptab<true>), dim3(nblk), dim3(kBlock), 0, 0, g, b.nup, b.npar, b.binom.get(),
                            b.hkeys.get(), b.hmask, b.hmax_probe, colbits, b.ptab.get());
@@ -249,6 +384,34 @@ void build_implicit_states(BasisHost &b) {
     b.states_stored = true;
 }
 
+// the general group's element tables on the device
+void upload_general(BasisHost &b) {
+    const GroupHost &h = b.group;
+    const int n = h.order();
+    std::vector<uint32_t> tab((size_t)n * 1024, 0), c((size_t)n);
+    std::vector<double> chi((size_t)2 * n);
+    for (int e = 0; e < n; e++) {
+        for (int k = 0; k < 4; k++)
+            for (int v = 0; v < 256; v++) {
+                uint32_t r = 0;
+                for (int bit = 0; bit < 8; bit++) {
+                    const int i = 8 * k + bit;
+                    if (i < b.L && (v >> bit & 1)) r |= 1u << h.eperm[(size_t)e][(size_t)i];
+                }
+                tab[(size_t)e * 1024 + 256 * k + v] = r;
+            }
+        c[(size_t)e] = (uint32_t)h.ec[(size_t)e];
+        chi[(size_t)2 * e] = h.echar[(size_t)e].real();
+        chi[(size_t)2 * e + 1] = h.echar[(size_t)e].imag();
+    }
+    b.gtab.alloc(tab.size());
+    b.gc.alloc(c.size());
+    b.gchi.alloc(chi.size());
+    QS_HIP(hipMemcpy(b.gtab.get(), tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    QS_HIP(hipMemcpy(b.gc.get(), c.data(), c.size() * 4, hipMemcpyHostToDevice));
+    QS_HIP(hipMemcpy(b.gchi.get(), chi.data(), chi.size() * 8, hipMemcpyHostToDevice));
+}
+
 template <typename S>
 void build_stored(BasisHost &b, int64_t npar) {
     GroupDev g = group_to_dev(b.group);
@@ -257,8 +420,14 @@ void build_stored(BasisHost &b, int64_t npar) {
     int64_t nblk = (lanes + kBlock - 1) / kBlock;
     if (nblk > 0x7fffffff) fail(QS_EARG, "sector too large");
     DevBuf<uint32_t> counts((size_t)nblk);
-    hipLaunchKernelGGL((k_count<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
-                       b.binom.get(), counts.get());
+    const bool gen = b.group.general;
+    GenGroup gg{b.group.order(), b.L, b.gtab.get(), b.gc.get(), b.gchi.get()};
+    if (gen)
+        hipLaunchKernelGGL(k_count_gen, dim3((unsigned)nblk), dim3(kBlock), 0, 0, gg, b.nup, npar, b.binom.get(),
+                           counts.get());
+    else
+        hipLaunchKernelGGL((k_count<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
+                           b.binom.get(), counts.get());
     QS_HIP(hipGetLastError());
     std::vector<uint32_t> hc((size_t)nblk);
     QS_HIP(hipMemcpy(hc.data(), counts.get(), sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost));
@@ -279,8 +448,12 @@ void build_stored(BasisHost &b, int64_t npar) {
     QS_HIP(hipMemset(reps.get(), 0xFF, padded * sizeof(S)));
     b.orbit.alloc((size_t)total);
     b.inv_nrm.alloc((size_t)total);
-    hipLaunchKernelGGL((k_write<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
-                       b.binom.get(), d_off.get(), reps.get(), b.orbit.get(), b.inv_nrm.get());
+    if (gen)
+        hipLaunchKernelGGL(k_write_gen, dim3((unsigned)nblk), dim3(kBlock), 0, 0, gg, b.nup, npar, b.binom.get(),
+                           d_off.get(), (uint32_t *)reps.get(), b.orbit.get(), b.inv_nrm.get());
+    else
+        hipLaunchKernelGGL((k_write<S>), dim3((unsigned)nblk), dim3(kBlock), 0, 0, g, b.nup, npar, mask,
+                           b.binom.get(), d_off.get(), reps.get(), b.orbit.get(), b.inv_nrm.get());
     QS_HIP(hipGetLastError());
     // column hash table: power of two with load factor <= 1/2
     uint64_t hsize = 1024;
@@ -328,6 +501,7 @@ void build_basis_device(BasisHost &b) {
         QS_HIP(hipDeviceSynchronize());
         return;
     }
+    if (b.group.general) upload_general(b);
     if (b.use64) {
         build_stored<uint64_t>(b, npar);
     } else {

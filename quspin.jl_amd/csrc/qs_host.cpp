@@ -294,6 +294,83 @@ void build_group(BasisHost &b)
     }
 }
 
+// basis.jl:530-620 (SpinBasisGeneral, spin 1/2, L <= 32 here): the group the
+// blocks' maps generate, every element with its character (the product of
+// the generators' eigenvalues along any word; a clash means no common
+// eigenvector, so the sector is empty).
+void build_group_general(BasisHost &b)
+{
+    const int L = b.L;
+    GroupHost &g = b.group;
+    g = GroupHost();
+    g.L = L;
+    g.a = 1;
+    g.lam[0] = cd(1, 0);
+    g.pchar[0] = cd(1, 0);
+    if (b.gmaps.empty()) {
+        b.mode = MODE_FULL;
+        return;
+    }
+    if (L > 32)
+        fail(QS_EARG, "general symmetry maps on more than 32 sites are not supported on the GPU");
+    std::vector<std::pair<Affine, cd>> gens;
+    for (const GenMap &m : b.gmaps) {
+        uint64_t f = m.flip;
+        // the particle sector is kept only by no flip, or by inverting every site at half filling
+        if (b.nup >= 0 && f != 0 && !(f == full_mask(L) && 2 * b.nup == L))
+            fail(QS_EARG, "symmetry transformation leaves the selected particle sector");
+        Affine a;
+        a.perm = m.perm;
+        a.c = permute_bits(m.perm, f);
+        gens.push_back({a, exact_root(((m.q % m.order) + m.order) % m.order, m.order)});
+    }
+    std::map<Affine, cd> chi;
+    std::vector<Affine> queue{identity_map(L)};
+    chi[queue[0]] = cd(1, 0);
+    bool consistent = true;
+    for (size_t qi = 0; qi < queue.size(); qi++) {
+        const Affine cur = queue[qi];
+        const cd cc = chi[cur];
+        for (auto &gg : gens) {
+            Affine nx = compose(gg.first, cur);
+            cd nc = gg.second * cc;
+            auto it = chi.find(nx);
+            if (it == chi.end()) {
+                chi[nx] = nc;
+                queue.push_back(nx);
+                if (queue.size() > (size_t)kMaxGeneral) fail(QS_EARG, "symmetry group too large");
+            } else if (!close_to(it->second, nc)) {
+                consistent = false;
+            }
+        }
+    }
+    if (!consistent) fail(QS_EARG, "the requested symmetry sector is empty");
+    g.general = true;
+    for (const Affine &e : queue) {
+        g.eperm.push_back(e.perm);
+        g.ec.push_back(e.c);
+        g.echar.push_back(chi[e]);
+    }
+    b.is_complex = 0;
+    for (const cd &c : g.echar)
+        if (std::abs(c.imag()) > 1e-12) b.is_complex = 1;
+    b.mode = MODE_STORED;
+}
+
+uint64_t group_apply(const GroupHost &g, int e, uint64_t u)
+{
+    if (g.general) return permute_bits(g.eperm[(size_t)e], u) ^ g.ec[(size_t)e];
+    const int t = e / g.npoint, p = e - t * g.npoint;
+    return rotl(apply_point(g, p, u), t * g.a, g.L);
+}
+
+cd group_char(const GroupHost &g, int e)
+{
+    if (g.general) return g.echar[(size_t)e];
+    const int t = e / g.npoint, p = e - t * g.npoint;
+    return g.lam[t] * g.pchar[p];
+}
+
 // ----------------------------------------------------------------- terms --
 namespace {
 
@@ -433,6 +510,36 @@ std::vector<CTerm> symmetrize_cterms(const std::vector<CTerm> &t, const BasisHos
     const GroupHost &g = b.group;
     if (b.mode == MODE_FULL) return consolidate_cterms(t);
     std::vector<CTerm> base = consolidate_cterms(t);
+    if (g.general) {
+        // the same test and average over the explicit elements
+        auto image = [&](const CTerm &x, size_t e) {
+            auto pi = [&](uint64_t v) { return permute_bits(g.eperm[e], v); };
+            CTerm o;
+            o.flip = pi(x.flip);
+            o.cmask = pi(x.cmask);
+            o.cval = pi(x.cval) ^ (g.ec[e] & o.cmask);
+            o.smask = pi(x.smask);
+            o.A = x.A;
+            if (__builtin_popcountll(g.ec[e] & o.smask) & 1) o.A = -o.A;
+            return o;
+        };
+        bool inv = true;
+        for (size_t e = 0; e < g.eperm.size() && inv; e++) {
+            std::vector<CTerm> img;
+            for (auto &x : base) img.push_back(image(x, e));
+            inv = same_dict(consolidate_cterms(img), base);
+        }
+        if (inv) return base;
+        std::vector<CTerm> all;
+        const double w = 1.0 / (double)g.eperm.size();
+        for (size_t e = 0; e < g.eperm.size(); e++)
+            for (auto &x : base) {
+                CTerm y = image(x, e);
+                y.A *= w;
+                all.push_back(y);
+            }
+        return consolidate_cterms(all);
+    }
     bool invariant = true;
     for (int p = 0; p < g.npoint && invariant; p++)
         for (int tt = 0; tt < g.nT && invariant; tt++) {

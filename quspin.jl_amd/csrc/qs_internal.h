@@ -140,7 +140,24 @@ struct GroupHost {
     uint64_t pflip[kMaxPoint] = {0};
     cd pchar[kMaxPoint];     // character of X_p
     cd lam[kMaxRot];         // character of T^t
-    int order() const { return nT * npoint; }
+    // general site maps (SpinBasisGeneral): every element as an explicit
+    // map u -> perm(u) ^ c (bit i moves to bit perm[i]) with its character,
+    // identity first; the (t, p) fields above are then unused
+    bool general = false;
+    std::vector<std::vector<int>> eperm;
+    std::vector<uint64_t> ec;
+    std::vector<cd> echar;
+    int order() const { return general ? (int)eperm.size() : nT * npoint; }
+};
+constexpr int kMaxGeneral = 256;   // elements of a general-map group
+
+// A SpinBasisGeneral symmetry block (basis.jl:282-338): the bit of source
+// site i, inverted when bit i of `flip` is set, moves to site perm[i]
+// (0-based); eigenvalue cis(2 pi q / order).
+struct GenMap {
+    std::vector<int> perm;
+    uint64_t flip = 0;
+    int order = 1, q = 0;
 };
 
 // Device-side copy passed by value in kernel arguments.
@@ -316,6 +333,12 @@ struct BasisHost {
     // split-kernel tables (built once, on first use)
     mutable std::once_flag split_once;
     mutable std::unique_ptr<SplitBasis> split;
+    // SpinBasisGeneral: the blocks in stage order, and the group on the
+    // device: per element four 256-entry byte tables of the permutation, its
+    // xor constant and its character (interleaved complex)
+    std::vector<GenMap> gmaps;
+    DevBuf<uint32_t> gtab, gc;
+    DevBuf<double> gchi;
 };
 
 struct OpHost {
@@ -399,6 +422,10 @@ uint64_t apply_point(const GroupHost &g, int p, uint64_t u);
 int64_t binom(int n, int k);
 
 void build_group(BasisHost &b);   // throws Error(QS_EARG) for invalid blocks
+void build_group_general(BasisHost &b);   // from b.gmaps (SpinBasisGeneral)
+// element e of the group (e = t * npoint + p, or the general list index)
+uint64_t group_apply(const GroupHost &g, int e, uint64_t u);
+cd group_char(const GroupHost &g, int e);
 void build_basis_device(BasisHost &b);
 void compute_reference_labels(BasisHost &b);
 

@@ -116,18 +116,27 @@ double gnorm2(const jc *v, int n) {
 }
 
 struct StageT {
-    int kind;   // 0 translate, 1 parity, 2 flip, 3 parity+flip, 4 flip odd, 5 flip even
+    int kind;   // 0 translate, 1 parity, 2 flip, 3 parity+flip, 4 flip odd, 5 flip even, 6 general map
     jc lam;
+    const GenMap *map = nullptr;
 };
 
-uint64_t stage_apply(int kind, uint64_t u, int L, int a, uint64_t odd, uint64_t even) {
-    switch (kind) {
+uint64_t stage_apply(const StageT &st, uint64_t u, int L, int a, uint64_t odd, uint64_t even) {
+    switch (st.kind) {
     case 0: return rotl(u, a, L);
     case 1: return bit_reverse(u, L);
     case 2: return u ^ full_mask(L);
     case 3: return bit_reverse(u, L) ^ full_mask(L);
     case 4: return u ^ odd;
     case 5: return u ^ even;
+    case 6: {
+        // basis.jl:325-338: the (inverted) bit of each source site moves to its target
+        const uint64_t v = u ^ st.map->flip;
+        uint64_t r = 0;
+        for (int i = 0; i < L; i++)
+            if (v >> i & 1) r |= 1ULL << st.map->perm[(size_t)i];
+        return r;
+    }
     }
     return u;
 }
@@ -136,10 +145,7 @@ void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint6
                     uint64_t &label, double &rnorm) {
     const GroupHost &g = b.group;
     std::vector<uint64_t> O;
-    for (int p = 0; p < g.npoint; p++) {
-        uint64_t v = apply_point(g, p, s0);
-        for (int t = 0; t < g.nT; t++) O.push_back(rotl(v, t * g.a, g.L));
-    }
+    for (int e = 0; e < g.order(); e++) O.push_back(group_apply(g, e, s0));
     std::sort(O.begin(), O.end());
     O.erase(std::unique(O.begin(), O.end()), O.end());
     const int R = (int)O.size();
@@ -157,7 +163,7 @@ void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint6
             cyc.assign(1, i);
             int cur = i;
             for (;;) {
-                int nx = idx(stage_apply(st.kind, O[cur], b.L, b.a, odd, even));
+                int nx = idx(stage_apply(st, O[cur], b.L, b.a, odd, even));
                 if (nx == i) break;
                 cyc.push_back(nx);
                 cur = nx;
@@ -178,58 +184,61 @@ void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint6
             }
         }
     }
-    // later stages: P'SP intersections (symmetry_basis.jl:110-213)
+    // later stages: P'SP intersections (symmetry_basis.jl:110-213): the
+    // stage maps each column onto one column with a phase; the cycles of that
+    // column map, walked from their first column, give the new columns
     for (size_t si = 1; si < stages.size(); si++) {
         const StageT &st = stages[si];
         std::vector<int> ncol(R, -1);
         std::vector<jc> nval(val);
-        for (int i = 0; i < R; i++) {
-            if (col[i] != i || ncol[i] != -1) continue;
-            int j = col[idx(stage_apply(st.kind, O[i], b.L, b.a, odd, even))];
-            if (j < 0) {
-                for (int r = 0; r < R; r++)
-                    if (col[r] == i) ncol[r] = DEAD;
-                continue;
+        std::vector<char> seen(R, 0);
+        auto overlap = [&](int from) {
+            // R[to, from] = sum over rows r of `from` (ascending) of conj(P[S r, to]) * P[r, from]
+            jc acc{0, 0};
+            bool first = true;
+            for (int r = 0; r < R; r++) {
+                if (col[r] != from) continue;
+                jc prod = jmul(jconj(val[idx(stage_apply(st, O[r], b.L, b.a, odd, even))]), val[r]);
+                acc = first ? prod : jadd(acc, prod);
+                first = false;
             }
-            auto overlap = [&](int from) {
-                // R[to, from] = sum over rows r of `from` (ascending) of conj(P[S r, to]) * P[r, from]
-                jc acc{0, 0};
-                bool first = true;
-                for (int r = 0; r < R; r++) {
-                    if (col[r] != from) continue;
-                    jc prod = jmul(jconj(val[idx(stage_apply(st.kind, O[r], b.L, b.a, odd, even))]), val[r]);
-                    acc = first ? prod : jadd(acc, prod);
-                    first = false;
+            return acc;
+        };
+        std::vector<int> cyc;
+        std::vector<jc> ph, c;
+        for (int i = 0; i < R; i++) {
+            if (col[i] != i || seen[i]) continue;
+            cyc.clear();
+            ph.clear();
+            bool dead = false;
+            int cur = i;
+            for (;;) {
+                cyc.push_back(cur);
+                seen[cur] = 1;
+                ph.push_back(overlap(cur));
+                const int nx = col[idx(stage_apply(st, O[cur], b.L, b.a, odd, even))];
+                if (nx == i) break;
+                if (nx < 0 || seen[nx]) {
+                    dead = true;   // no column cycle through i (not reached by a monomial stage)
+                    break;
                 }
-                return acc;
-            };
-            if (j == i) {
-                jc phi = overlap(i);
-                jc closure = jmul(jc{1.0, 0.0}, phi);
-                bool keep = japprox(closure, jmul(st.lam, jc{1.0, 0.0}), 2e-10, 2e-10);
-                jc one{1.0, 0.0};
-                double nrm = gnorm2(&one, 1);
-                jc w{1.0 / nrm, 0.0 / nrm};
+                cur = nx;
+            }
+            const int len = (int)cyc.size();
+            bool keep = !dead;
+            c.assign(len, jc{1.0, 0.0});
+            if (keep) {
+                for (int q = 0; q + 1 < len; q++) c[q + 1] = jdiv(jmul(c[q], ph[q]), st.lam);
+                jc closure = jmul(c[len - 1], ph[len - 1]);
+                keep = japprox(closure, jmul(st.lam, c[0]), 2e-10, 2e-10);
+            }
+            const double nrm = gnorm2(c.data(), len);
+            for (int q = 0; q < len; q++) {
+                const jc w{c[q].re / nrm, c[q].im / nrm};
                 for (int r = 0; r < R; r++) {
-                    if (col[r] != i) continue;
+                    if (col[r] != cyc[q]) continue;
                     ncol[r] = keep ? i : DEAD;
                     if (keep) nval[r] = jmul(val[r], w);
-                }
-            } else {
-                jc phi_ba = overlap(i), phi_ab = overlap(j);
-                jc c[2] = {jc{1.0, 0.0}, jdiv(jmul(jc{1.0, 0.0}, phi_ba), st.lam)};
-                jc closure = jmul(c[1], phi_ab);
-                bool keep = japprox(closure, jmul(st.lam, c[0]), 2e-10, 2e-10);
-                double nrm = gnorm2(c, 2);
-                jc w0{c[0].re / nrm, c[0].im / nrm}, w1{c[1].re / nrm, c[1].im / nrm};
-                for (int r = 0; r < R; r++) {
-                    if (col[r] == i) {
-                        ncol[r] = keep ? i : DEAD;
-                        if (keep) nval[r] = jmul(val[r], w0);
-                    } else if (col[r] == j) {
-                        ncol[r] = keep ? i : DEAD;
-                        if (keep) nval[r] = jmul(val[r], w1);
-                    }
                 }
             }
         }
@@ -265,8 +274,14 @@ void compute_reference_labels(BasisHost &b) {
         QS_HIP(hipMemcpy(r32.data(), b.reps32.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; i++) reps[i] = r32[i];
     }
-    // the reference's stage order: k first, then p, z, pz, zA, zB (basis.jl:704-763)
+    // the reference's stage order: k first, then p, z, pz, zA, zB (basis.jl:704-763);
+    // SpinBasisGeneral: its blocks in block order (basis.jl:575-602)
     std::vector<StageT> stages;
+    for (const GenMap &m : b.gmaps) {
+        const int q = ((m.q % m.order) + m.order) % m.order;
+        const double ang = (6.283185307179586 * (double)q) / (double)m.order;
+        stages.push_back(StageT{6, jc{std::cos(ang), std::sin(ang)}, &m});
+    }
     if (b.blocks[0] != QS_UNSET) {
         int order = b.L / b.a;
         int m = ((b.blocks[0] % order) + order) % order;

@@ -72,7 +72,26 @@ struct MVArgs {
     int colbits;
     uint32_t cfull;        // C(L, nup): spin inversion reverses the colex order
     int ndcls, ndrest;
+    // SpinBasisGeneral: the characters of the parent table's group elements
+    // (interleaved complex), else null
+    const double *gchi;
 };
+
+// Character of the group element a parent-table entry names: (t, p) =
+// divmod(elem, npoint) of the translation / point group, or the general
+// group's element list.
+__device__ __forceinline__ void elem_char(const MVArgs &A, int elem, double &cr, double &ci) {
+    if (A.gchi) {
+        cr = A.gchi[2 * elem];
+        ci = A.gchi[2 * elem + 1];
+        return;
+    }
+    const int bt = elem / A.g.npoint, bp = elem - bt * A.g.npoint;
+    const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
+    const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
+    cr = lr * pr - li * pi;
+    ci = lr * pi + li * pr;
+}
 
 // Amplitude of one compiled term on state s: A * (-1)^popcount(s & smask),
 // or zero when the conditions fail.
@@ -667,11 +686,10 @@ __device__ __forceinline__ void hop_groups(const MVArgs &A, const HopDev *__rest
             if (CM == CM_PTAB) {
                 const double fac = f[j] * own_scale;
                 if (TRACK) {
-                    const int bt = elem[j] / A.g.npoint, bp = elem[j] - bt * A.g.npoint;
-                    const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
-                    const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
-                    const double cr = (lr * pr - li * pi) * fac;
-                    const double ci = (lr * pi + li * pr) * fac;
+                    double cr, ci;
+                    elem_char(A, elem[j], cr, ci);
+                    cr *= fac;
+                    ci *= fac;
                     wr = CC ? mr[j] * cr - mi[j] * ci : mr[j] * cr;
                     wi = CC ? mr[j] * ci + mi[j] * cr : mr[j] * ci;
                 } else {
@@ -861,14 +879,7 @@ __device__ __forceinline__ void row_entries(const MVArgs &A, const HopDev *__res
             if (e == 0xFFFFFFFFu) continue;
             col = e & colmask;
             double cr = 1.0, ci = 0.0;
-            if (TRACK && A.colbits < 32) {
-                const int elem = (int)(e >> A.colbits);
-                const int bt = elem / A.g.npoint, bp = elem - bt * A.g.npoint;
-                const double lr = A.g.lam_re[bt], li = A.g.lam_im[bt];
-                const double pr = A.g.pch_re[bp], pi = A.g.pch_im[bp];
-                cr = lr * pr - li * pi;
-                ci = lr * pi + li * pr;
-            }
+            if (TRACK && A.colbits < 32) elem_char(A, (int)(e >> A.colbits), cr, ci);
             const double fac = A.inv_nrm[col] * own_scale;
             cr *= fac;
             ci *= fac;
@@ -1939,6 +1950,7 @@ MVArgs base_args(const OpHost &op, bool adjoint, double are, double aim, double 
     a.hmask = b.hmask;
     a.hmax_probe = b.hmax_probe;
     a.binom = b.binom.get();
+    a.gchi = b.group.general ? b.gchi.get() : nullptr;
     // plain products use the adjoint couplings (pull form), adjoint products the couplings
     const std::vector<CTerm> &dg = adjoint ? op.diag_terms_adj : op.diag_terms;
     const std::vector<CTerm> &of = adjoint ? op.off_terms_adj : op.off_terms;
@@ -1960,6 +1972,11 @@ MVArgs base_args(const OpHost &op, bool adjoint, double are, double aim, double 
 }
 
 bool basis_tracks(const BasisHost &b) {
+    if (b.group.general) {
+        for (const cd &c : b.group.echar)
+            if (c != cd(1, 0)) return true;
+        return false;
+    }
     for (int p = 0; p < b.group.npoint; p++)
         if (b.group.pchar[p] != cd(1, 0)) return true;
     for (int t = 0; t < b.group.nT; t++)
@@ -2686,6 +2703,10 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
         QS_HIP(hipGetLastError());
         return;
     }
+    // the general kernels canonicalise with the translation / point group;
+    // a general-map basis has only its parent table
+    if (b.group.general)
+        fail(QS_EARG, "a basis of general symmetry maps takes two-bit hop couplings (the parent-table kernels)");
     int lpr = op.lanes_per_row > 0 ? op.lanes_per_row : default_lanes_per_row(b);
     if (b.mode != MODE_STORED) lpr = 1;
     if (dot_partial) lpr = 8;   // the partial-sum grid is sized by matvec_grid

@@ -189,6 +189,7 @@ int64_t parent_length(const BasisHost &b, int kind) {
 }
 
 void basis_project(const BasisHost &b, int dtype, int kind, const void *parent, void *out, hipStream_t s) {
+    if (b.group.general) fail(QS_EARG, "projection to and from the parent is not supported for general symmetry maps");
     if (b.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
     if (kind != 0 && kind != 1) fail(QS_EARG, "parent kind must be 0 (own parent) or 1 (all states)");
     const size_t esz = dtype_size(dtype);
@@ -214,6 +215,7 @@ void basis_project(const BasisHost &b, int dtype, int kind, const void *parent, 
 
 void basis_expand(const BasisHost &b, int dtype, int kind, const void *x, void *parent, int ncols, bool accumulate,
                   hipStream_t s) {
+    if (b.group.general) fail(QS_EARG, "projection to and from the parent is not supported for general symmetry maps");
     if (b.is_complex && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
     if (kind != 0 && kind != 1) fail(QS_EARG, "parent kind must be 0 (own parent) or 1 (all states)");
     if (b.mode == MODE_STORED && !b.ptab.get())

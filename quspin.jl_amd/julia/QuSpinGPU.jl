@@ -16,7 +16,7 @@ using LinearAlgebra
 using SparseArrays
 import QuSpin: SpinBasis1D, OperatorTerm, QuantumLinearOperator, eigsh, evolve
 
-export GPUBasis, GPUQuantumLinearOperator, GPUHamiltonian, GPUBlockOps, assemble!, block_expm
+export GPUBasis, GPUBasisGeneral, GPUQuantumLinearOperator, GPUHamiltonian, GPUBlockOps, assemble!, block_expm
 
 const libqsgpu = get(ENV, "QSGPU_LIB", joinpath(@__DIR__, "..", "_build", "libqsgpu.so"))
 
@@ -85,6 +85,41 @@ function GPUBasis(basis::SpinBasis1D)
                  pzblock=g(:pzblock), zAblock=g(:zAblock), zBblock=g(:zBblock))
     b.Ns == length(basis) ||
         throw(DimensionMismatch("GPU sector has $(b.Ns) states, the reference basis $(length(basis))"))
+    return b
+end
+
+"""
+    GPUBasisGeneral(L; nup=nothing, pauli=true, block_order=nothing, blocks...)
+
+The GPU sector of `SpinBasisGeneral(L; nup, pauli, block_order, blocks...)`
+(basis.jl:530-620, spin 1/2): each block `name=(map, quantum_number)` with the
+map in QuSpin's zero-based or Julia's one-based site labels, stages in
+`block_order` (default: keyword order).  The reference basis keeps only the
+quantum numbers, so the maps are given again here.  L <= 32.
+"""
+function GPUBasisGeneral(L::Integer; nup=nothing, pauli::Bool=true, block_order=nothing, blocks...)
+    names = block_order === nothing ? Symbol[Symbol(n) for (n, _) in blocks] : Symbol.(collect(block_order))
+    spec = Dict{Symbol,Any}(Symbol(n) => v for (n, v) in blocks)
+    for n in keys(spec)
+        n in names || push!(names, n)
+    end
+    all(n -> haskey(spec, n), names) || throw(ArgumentError("block_order refers to an unknown symmetry block"))
+    maps = Matrix{Int32}(undef, L, length(names))
+    qn = Vector{Int32}(undef, length(names))
+    for (j, n) in enumerate(names)
+        m, q = spec[n][1], spec[n][2]
+        length(m) == L || throw(ArgumentError("a general symmetry map must contain one entry per site"))
+        maps[:, j] .= Int32.(collect(m))
+        qn[j] = Int32(q)
+    end
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:qs_basis_create_general, libqsgpu), Cint,
+                (Cint, Cint, Cint, Cint, Ptr{Int32}, Ptr{Int32}, Cint, Ref{Ptr{Cvoid}}),
+                L, _blk(nup), pauli, length(names), maps, qn, 0, h))
+    n = Ref{Int64}(0)
+    check(ccall((:qs_basis_size, libqsgpu), Cint, (Ptr{Cvoid}, Ref{Int64}), h[], n))
+    b = GPUBasis(h[], Int(L), Int(n[]))
+    finalizer(x -> ccall((:qs_basis_destroy, libqsgpu), Cint, (Ptr{Cvoid},), x.handle), b)
     return b
 end
 

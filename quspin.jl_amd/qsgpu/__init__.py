@@ -1,7 +1,7 @@
 """qsgpu -- MI355X host interface for QuSpin.jl's spin-1/2 sector Hamiltonian action.
 
 This package mirrors the part of QuSpin.jl's public API that sits on the
-matrix-free path (``SpinBasis1D``, ``OperatorTerm``, ``QuantumLinearOperator``,
+matrix-free path (``SpinBasis1D``, ``SpinBasisGeneral``, ``OperatorTerm``, ``QuantumLinearOperator``,
 ``eigsh``, ``evolve``, ``ExpmMultiplyParallel``) with the same argument names,
 meanings and exception classes, and forwards every call through the C ABI of
 ``libqsgpu.so`` (include/qsgpu.h) to hand-written gfx950 kernels.  There is no
@@ -10,7 +10,7 @@ works but every call raises ImportError.
 """
 from ._lib import build, lib, LIB_PATH, VARIANT_HOP, VARIANT_SPLIT, VARIANT_SELL  # noqa: F401
 from .errors import ArgumentError, DimensionMismatch, ErrorException  # noqa: F401
-from .basis import SpinBasis1D  # noqa: F401
+from .basis import SpinBasis1D, SpinBasisGeneral  # noqa: F401
 from .operators import (  # noqa: F401
     OperatorTerm,
     QuantumLinearOperator,

@@ -24,7 +24,7 @@ EXPORTS = (
     "qs_basis_parent_size", "qs_basis_project", "qs_basis_expand", "qs_batch_create", "qs_batch_destroy",
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
     "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr", "qs_op_set_graphs",
-    "qs_op_matvec_noalloc", "qs_evolve_dynamic", "qs_block_gram",
+    "qs_op_matvec_noalloc", "qs_evolve_dynamic", "qs_block_gram", "qs_basis_create_general",
 )
 NOALLOC_SCRATCH_BYTES = 32 << 20   # QS_NOALLOC_SCRATCH_BYTES
 
@@ -49,6 +49,7 @@ def lib():
     L.qs_last_error.restype = ctypes.c_size_t
     L.qs_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, vp, vp]
     L.qs_basis_create.argtypes = [i32] * 11 + [pp]
+    L.qs_basis_create_general.argtypes = [i32, i32, i32, i32, vp, vp, i32, pp]
     L.qs_basis_destroy.argtypes = [vp]
     L.qs_basis_size.argtypes = [vp, vp]
     L.qs_basis_states.argtypes = [vp, vp]

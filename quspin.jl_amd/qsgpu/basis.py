@@ -1,4 +1,4 @@
-"""SpinBasis1D on the GPU (basis.jl:622-777)."""
+"""SpinBasis1D and SpinBasisGeneral on the GPU (basis.jl:622-777, 530-620)."""
 import ctypes
 
 import numpy as np
@@ -160,4 +160,70 @@ class SpinBasis1D:
         out = np.zeros(self.Ns, dtype=np.complex128)
         ok = idx >= 0
         np.add.at(out, idx[ok], np.conj(amp[ok]) * np.asarray(parent_vector)[ok])
+        return out
+
+
+class SpinBasisGeneral(SpinBasis1D):
+    """SpinBasisGeneral(L; nup, Nup, m, S, pauli, block_order, blocks...) -- basis.jl:530-620.
+
+    Spin-1/2 states reduced by arbitrary site maps: each block is
+    ``name=(map, quantum_number)``, the map one entry per site in QuSpin's
+    zero-based labels (any entry zero or negative; ``-v`` inverts the site and
+    sends it to site ``v``) or Julia's one-based labels (negative = inverted),
+    the eigenvalue ``cis(2 pi q / order)`` with the map's order.  Stages run in
+    ``block_order`` (default: the keyword order).  L <= 32 here; operators on
+    such a basis take two-bit hop couplings.
+    """
+
+    def __init__(self, L, nup=None, Nup=None, m=None, S="1/2", pauli=True, block_order=None, **blocks):
+        if str(S) not in ("1/2", "0.5"):
+            raise ArgumentError("only spin-1/2 chains are on the GPU path")
+        if nup is not None and Nup is not None:
+            raise ArgumentError("specify only one of nup and Nup")
+        sel = Nup if nup is None else nup
+        if m is not None:
+            if sel is not None:
+                raise ArgumentError("m cannot be combined with nup or Nup")
+            sel = int(round((float(m) + 0.5) * L))
+        names = list(blocks) if block_order is None else [str(n) for n in block_order]
+        for n in blocks:
+            if n not in names:
+                names.append(n)
+        if any(n not in blocks for n in names):
+            raise ArgumentError("block_order refers to an unknown symmetry block")
+        maps, qn = [], []
+        for n in names:
+            spec = blocks[n]
+            if not isinstance(spec, (tuple, list)) or len(spec) < 2:
+                raise ArgumentError("general symmetry blocks must be (map, quantum_number)")
+            mp = [int(v) for v in spec[0]]
+            if len(mp) != int(L):
+                raise ArgumentError("a general symmetry map must contain one entry per site")
+            maps.append(mp)
+            qn.append(int(spec[1]))
+        self.L = int(L)
+        self.nup = None if sel is None else int(sel)
+        self.pauli = bool(pauli)
+        self.a = 1
+        self._blocks = dict(kblock=None, pblock=None, zblock=None, pzblock=None, zAblock=None, zBblock=None)
+        self._general = [(n, tuple(mp), q) for n, mp, q in zip(names, maps, qn)]
+        mats = np.ascontiguousarray(np.asarray(maps or [[0]], dtype=np.int32))
+        qs = np.ascontiguousarray(np.asarray(qn or [0], dtype=np.int32))
+        h = ctypes.c_void_p()
+        check(_lib.lib().qs_basis_create_general(self.L, _blk(self.nup), int(self.pauli), len(maps), _ptr(mats),
+                                                  _ptr(qs), 0, ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        check(_lib.lib().qs_basis_size(self._h, ctypes.byref(n)))
+        self.Ns = int(n.value)
+        c = ctypes.c_int()
+        check(_lib.lib().qs_basis_is_complex(self._h, ctypes.byref(c)))
+        self.is_complex = bool(c.value)
+        self._states = None
+
+    @property
+    def blocks(self):
+        out = {"nup": self.nup, "pauli": self.pauli, "block_order": [n for n, _, _ in self._general]}
+        for n, mp, q in self._general:
+            out[n] = q
         return out

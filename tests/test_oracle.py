@@ -177,3 +177,26 @@ def test_whole_product_baseline_builds_the_reference_sector():
         ns, t = oracle.bench_qlo_matvec(L, L // 2, False, _dense.xxz_terms(L, J2=0.4), 2, 2, **blocks)
         assert ns == oracle.Sector(L, nup=L // 2, pauli=False, **blocks).Ns
         assert t.shape == (2, 3) and np.all(t >= 0)
+
+
+@pytest.mark.parametrize("blocks,kw", [
+    ([("t", 0), ("p", 0)], dict(kblock=0, pblock=1)),
+    ([("t", 0), ("p", 1)], dict(kblock=0, pblock=-1)),
+    ([("t", 0), ("z", 0)], dict(kblock=0, zblock=1)),
+    ([("t", 3), ("z", 1)], dict(kblock=3, zblock=-1)),
+    ([("t", 0), ("p", 1), ("z", 1)], dict(kblock=0, pblock=-1, zblock=-1)),
+    ([("t", 1)], dict(kblock=1)),
+])
+def test_general_maps_reproduce_spin_basis_1d(blocks, kw):
+    # SpinBasisGeneral's site maps (basis.jl:282-338) written for translation
+    # by two sites, parity and spin inversion give SpinBasis1D's a=2 sectors:
+    # same columns, labels and normalizations to the bit (the quantum number q
+    # of an order-2 map selects cis(pi q))
+    L = 12
+    maps = {"t": [(i + 2) % L for i in range(L)], "p": [L - 1 - i for i in range(L)],
+            "z": [-(i + 1) for i in range(L)]}
+    g = oracle.Sector.general(L, nup=6, pauli=False, blocks=[(maps[n], q) for n, q in blocks])
+    a = oracle.Sector(L, nup=6, pauli=False, a=2, **kw)
+    assert g.Ns == a.Ns
+    assert np.array_equal(g.states(), a.states())
+    assert np.array_equal(g.norms()[1].view(np.uint64), a.norms()[1].view(np.uint64))
