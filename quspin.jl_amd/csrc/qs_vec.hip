@@ -5,6 +5,7 @@
 // second fixed-shape tree over the block partials.  Results therefore do not
 // depend on what else shares the device.  Vectors stored in FP32 (the
 // FP32-stream form) are widened to FP64 before every product.
+#include <type_traits>
 #include <algorithm>
 
 #include "qs_device.h"
@@ -507,85 +508,153 @@ __global__ __launch_bounds__(kVB) void k_update_dgks(const V *__restrict__ Vb, i
 
 // ------------------------------------------------- block Gram on MFMA ----
 // G = V^H W for V (n x m) and W (n x p), column-major with ld n: partial sums
-// per row block on the matrix cores (v_mfma_f64_16x16x4f64), one 16 x 16
-// output tile per blockIdx.y.  Each block walks its row range in chunks of
-// 64 rows staged through LDS (coalesced loads, FP32 widened to FP64 there);
-// wavefront w multiplies rows 16 w .. 16 w + 15 of the chunk as four K = 4
-// steps.  Operand layout as in k_combine_mfma: lane l holds A[l % 16][l / 16]
-// and B[l / 16][l % 16]; accumulator register i of lane l is
-// D[l / 16 + 4 i][l % 16].  Complex: Re G = Vr'Wr + Vi'Wi, Im G = Vr'Wi - Vi'Wr.
-// The four wavefront tiles are summed in a fixed order; partial[(e nb + b) 2]
-// holds output element e = i + m j, reduced over blocks by k_reduce.
-constexpr int kGC = 64;       // rows per LDS chunk
-constexpr int kGP = kGC + 1;  // padded LDS row stride (doubles)
+// per row block on the matrix cores (v_mfma_f64_16x16x4f64).  One block owns
+// a group of TM x TP output tiles of 16 x 16 (columns i0 .. i0 + 16 TM of V,
+// j0 .. j0 + 16 TP of W, blockIdx.y numbering the groups), so each row of V
+// and W is read once per group.  The block walks its row range in chunks of
+// 64 R rows (R = 2 for FP32 storage, so a lane keeps as many bytes in flight
+// as in FP64): the next chunk's values are loaded into registers (coalesced,
+// 64 consecutive rows per column per load) while the matrix cores work on the
+// current chunk from LDS, where FP32 values stay FP32 and are widened exactly
+// as operands are read.  Wavefront w multiplies rows 16 R w .. 16 R w + 16 R - 1
+// of a chunk as 4 R steps of K = 4 into every tile of the group.  Operand
+// layout as in k_combine_mfma: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator register i of lane l is D[l / 16 + 4 i][l % 16].
+// Complex: Re G = Vr'Wr + Vi'Wi, Im G = Vr'Wi - Vi'Wr.  The four wavefront
+// tiles are summed in a fixed order; partial[(e nb + b) 2] holds output
+// element e = i + m j, reduced over blocks by k_reduce.
+template <typename V> struct GramStage {
+    static constexpr bool f32 = std::is_same<V, float>::value || std::is_same<V, cplx_f>::value;
+    using S = typename std::conditional<f32, float, double>::type;  // LDS element
+    static constexpr int R = f32 ? 2 : 1;                            // rows per lane per chunk
+    static constexpr int RC = 64 * R;                                // rows per chunk
+    // LDS row stride: operand reads by (la, lk) land on distinct banks
+    static constexpr int P = f32 ? RC + 4 : RC + 1;
+};
 
-template <typename V>
-__global__ __launch_bounds__(kVB) void k_gram_mfma(const V *__restrict__ Vb, int64_t n, int m,
-                                                   const V *__restrict__ Wb, int p, int tiles_m,
-                                                   double *__restrict__ partial) {
+template <typename V, int TM, int TP>
+__global__ __launch_bounds__(kVB, 2) void k_gram_mfma(const V *__restrict__ Vb, int64_t n, int m,
+                                                      const V *__restrict__ Wb, int p, int groups_m,
+                                                      double *__restrict__ partial) {
     using T = VTraits<V>;
+    using G = GramStage<V>;
+    using S = typename G::S;
     constexpr bool CPLX = T::cplx;
-    __shared__ double sv[2][16 * kGP], sw[2][16 * kGP];
-    __shared__ double sred[4][2][256];
+    constexpr int NP = CPLX ? 2 : 1;             // planes: re (, im)
+    constexpr int R = G::R, RC = G::RC, P = G::P;
+    constexpr int CV = 16 * TM, CW = 16 * TP;    // columns staged per chunk
+    constexpr int QV = CV / 4, QW = CW / 4;      // columns per lane
+    constexpr size_t kStage = sizeof(S) * NP * (CV + CW) * P;
+    constexpr size_t kRed = sizeof(double) * 2 * 4 * 256;
+    __shared__ __align__(16) unsigned char smem[kStage > kRed ? kStage : kRed];
+    S *sv = (S *)smem, *sw = sv + NP * CV * P;
     const int b = blockIdx.x, nb = gridDim.x;
-    const int ti = blockIdx.y % tiles_m, tj = blockIdx.y / tiles_m;
-    const int i0 = 16 * ti, j0 = 16 * tj;
-    const int mi = m - i0 < 16 ? m - i0 : 16, pj = p - j0 < 16 ? p - j0 : 16;
+    const int gi = blockIdx.y % groups_m, gj = blockIdx.y / groups_m;
+    const int i0 = CV * gi, j0 = CW * gj;
+    const int mc = m - i0 < CV ? m - i0 : CV, pc = p - j0 < CW ? p - j0 : CW;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rr = threadIdx.x & 63, cq = threadIdx.x >> 6;  // staging row / first column
     int64_t r0, r1;
     row_range(n, b, nb, r0, r1);
-    // columns beyond m / p stay zero for the whole launch
-    for (int e = threadIdx.x; e < 16 * kGP; e += kVB) {
-        sv[0][e] = sv[1][e] = sw[0][e] = sw[1][e] = 0.0;
-    }
-    v4d acc_r = {0, 0, 0, 0}, acc_i = {0, 0, 0, 0};
-    const int la = lane & 15, lk = lane >> 4;
-    for (int64_t c0 = r0; c0 < r1; c0 += kGC) {
-        __syncthreads();
-        // stage: thread t loads row (t % 64) of columns t / 64, t / 64 + 4, ...
-        const int rr = threadIdx.x & 63;
-        const int64_t r = c0 + rr;
-        for (int c = threadIdx.x >> 6; c < 16; c += 4) {
-            double re = 0, im = 0;
-            if (c < mi && r < r1) T::load(Vb, (int64_t)(i0 + c) * n + r, re, im);
-            sv[0][c * kGP + rr] = re;
-            if (CPLX) sv[1][c * kGP + rr] = im;
-            re = 0;
-            im = 0;
-            if (c < pj && r < r1) T::load(Wb, (int64_t)(j0 + c) * n + r, re, im);
-            sw[0][c * kGP + rr] = re;
-            if (CPLX) sw[1][c * kGP + rr] = im;
-        }
-        __syncthreads();
+    S pv[NP][R][QV], pw[NP][R][QW];
+    // lane t stages rows t % 64 + 64 u of columns t / 64 + 4 q; absent entries are zero
+    auto fetch = [&](int64_t c0) {
 #pragma unroll
-        for (int st = 0; st < 4; st++) {
-            const int kr = 16 * wv + 4 * st + lk;
-            const double ar = sv[0][la * kGP + kr], br = sw[0][la * kGP + kr];
-            acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc_r, 0, 0, 0);
-            if (CPLX) {
-                const double ai = sv[1][la * kGP + kr], bi = sw[1][la * kGP + kr];
-                acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, acc_r, 0, 0, 0);
-                acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_i, 0, 0, 0);
-                acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, acc_i, 0, 0, 0);
+        for (int u = 0; u < R; u++) {
+            const int64_t r = c0 + rr + 64 * u;
+            const bool in = r < r1;
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                const int c = cq + 4 * q;
+                double re = 0, im = 0;
+                if (in && c < mc) T::load(Vb, (int64_t)(i0 + c) * n + r, re, im);
+                pv[0][u][q] = (S)re;
+                if (CPLX) pv[NP - 1][u][q] = (S)im;
+            }
+#pragma unroll
+            for (int q = 0; q < QW; q++) {
+                const int c = cq + 4 * q;
+                double re = 0, im = 0;
+                if (in && c < pc) T::load(Wb, (int64_t)(j0 + c) * n + r, re, im);
+                pw[0][u][q] = (S)re;
+                if (CPLX) pw[NP - 1][u][q] = (S)im;
             }
         }
-    }
-    // D[l / 16 + 4 q][l % 16] of each wavefront -> LDS, then a fixed-order sum
+    };
+    v4d acc_r[TM][TP], acc_i[TM][TP];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int e = (lk + 4 * q) * 16 + la;
-        sred[wv][0][e] = acc_r[q];
-        sred[wv][1][e] = acc_i[q];
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int c = 0; c < TP; c++) {
+            acc_r[a][c] = v4d{0, 0, 0, 0};
+            acc_i[a][c] = v4d{0, 0, 0, 0};
+        }
+    const int la = lane & 15, lk = lane >> 4;
+    if (r0 < r1) fetch(r0);
+    for (int64_t c0 = r0; c0 < r1; c0 += RC) {
+        __syncthreads();  // every wavefront is done with the previous chunk
+#pragma unroll
+        for (int h = 0; h < NP; h++)
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+#pragma unroll
+                for (int q = 0; q < QV; q++) sv[(h * CV + cq + 4 * q) * P + rr + 64 * u] = pv[h][u][q];
+#pragma unroll
+                for (int q = 0; q < QW; q++) sw[(h * CW + cq + 4 * q) * P + rr + 64 * u] = pw[h][u][q];
+            }
+        __syncthreads();
+        if (c0 + RC < r1) fetch(c0 + RC);  // in flight during the products below
+#pragma unroll
+        for (int st = 0; st < 4 * R; st++) {
+            const int kr = 16 * R * wv + 4 * st + lk;
+            double ar[TM], ai[TM], br[TP], bi[TP];
+#pragma unroll
+            for (int a = 0; a < TM; a++) {
+                ar[a] = (double)sv[(16 * a + la) * P + kr];
+                ai[a] = CPLX ? (double)sv[(CV + 16 * a + la) * P + kr] : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < TP; c++) {
+                br[c] = (double)sw[(16 * c + la) * P + kr];
+                bi[c] = CPLX ? (double)sw[(CW + 16 * c + la) * P + kr] : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < TM; a++)
+#pragma unroll
+                for (int c = 0; c < TP; c++) {
+                    acc_r[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[c], acc_r[a][c], 0, 0, 0);
+                    if (CPLX) {
+                        acc_r[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[c], acc_r[a][c], 0, 0, 0);
+                        acc_i[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[c], acc_i[a][c], 0, 0, 0);
+                        acc_i[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], br[c], acc_i[a][c], 0, 0, 0);
+                    }
+                }
+        }
     }
-    __syncthreads();
-    const int e = threadIdx.x, ii = e >> 4, jj = e & 15;
-    if (ii < mi && jj < pj) {
-        const double gr = (sred[0][0][e] + sred[1][0][e]) + (sred[2][0][e] + sred[3][0][e]);
-        const double gi = (sred[0][1][e] + sred[1][1][e]) + (sred[2][1][e] + sred[3][1][e]);
-        const int64_t g = (int64_t)(i0 + ii) + (int64_t)m * (j0 + jj);
-        partial[(g * nb + b) * 2] = gr;
-        partial[(g * nb + b) * 2 + 1] = CPLX ? gi : 0.0;
-    }
+    // per tile: D[l / 16 + 4 q][l % 16] of each wavefront -> LDS, then a
+    // fixed-order sum over the four wavefronts
+    double *sred = (double *)smem;  // [wavefront][re, im][256]
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int c = 0; c < TP; c++) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int e = (lk + 4 * q) * 16 + la;
+                sred[(wv * 2) * 256 + e] = acc_r[a][c][q];
+                sred[(wv * 2 + 1) * 256 + e] = acc_i[a][c][q];
+            }
+            __syncthreads();
+            const int e = threadIdx.x, ii = 16 * a + (e >> 4), jj = 16 * c + (e & 15);
+            if (ii < mc && jj < pc) {
+                const double gr = (sred[0 * 256 + e] + sred[2 * 256 + e]) + (sred[4 * 256 + e] + sred[6 * 256 + e]);
+                const double gim = (sred[1 * 256 + e] + sred[3 * 256 + e]) + (sred[5 * 256 + e] + sred[7 * 256 + e]);
+                const int64_t g = (int64_t)(i0 + ii) + (int64_t)m * (j0 + jj);
+                partial[(g * nb + b) * 2] = gr;
+                partial[(g * nb + b) * 2 + 1] = CPLX ? gim : 0.0;
+            }
+        }
 }
 
 template <typename V>
@@ -904,13 +973,30 @@ void gram_dgks_dev(int dtype, const void *Vmat, int64_t n, int m, void *w, VecWo
 void gram_block_dev(int dtype, const void *Vmat, int64_t n, int m, const void *Wmat, int p, VecWork &wk,
                     double *out, hipStream_t s) {
     if (m < 1 || p < 1) fail(QS_EARG, "internal: empty block Gram product");
-    const int nb = vec_blocks(n);
+    // tile groups of up to 2 x 2 tiles (32 x 32 outputs) per block
     const int tm = (m + 15) / 16, tp = (p + 15) / 16;
-    ensure(wk, (size_t)2 * nb * m * p, 2);
+    const int TM = tm > 1 ? 2 : 1, TP = tp > 1 ? 2 : 1;
+    const int gm = (tm + TM - 1) / TM, gp = (tp + TP - 1) / TP;
+    int dev = 0, cus = 0;
+    QS_HIP(hipGetDevice(&dev));
+    QS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int nb = 0;
     by_dtype(dtype, [&](auto *tag) {
         using V = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL((k_gram_mfma<V>), dim3(nb, tm * tp), dim3(kVB), 0, s, (const V *)Vmat, n, m,
-                           (const V *)Wmat, p, tm, wk.partial.get());
+        const void *kern = TM == 2 && TP == 2 ? (const void *)k_gram_mfma<V, 2, 2>
+                           : TM == 2          ? (const void *)k_gram_mfma<V, 2, 1>
+                           : TP == 2          ? (const void *)k_gram_mfma<V, 1, 2>
+                                              : (const void *)k_gram_mfma<V, 1, 1>;
+        // one round of resident blocks over the groups: no partly filled
+        // last round; fewer blocks when n is small
+        int occ = 0;
+        QS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kVB, 0));
+        const int per_group = std::max(1, occ * cus / (gm * gp));
+        nb = std::max(1, std::min(vec_blocks(n), per_group));
+        ensure(wk, (size_t)2 * nb * m * p, 2);
+        void *args[] = {(void *)&Vmat, (void *)&n, (void *)&m, (void *)&Wmat, (void *)&p, (void *)&gm,
+                        (void *)&wk.partial.p};
+        QS_HIP(hipLaunchKernel(kern, dim3(nb, gm * gp), dim3(kVB), args, 0, s));
     });
     hipLaunchKernelGGL(k_reduce, dim3(m * p), dim3(kVB), 0, s, wk.partial.get(), nb, out, (const double *)nullptr,
                        (const double *)nullptr, 0, 0);

@@ -32,7 +32,8 @@ def _gram(V, W, mfma):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32, np.complex64])
-@pytest.mark.parametrize("shape", [(5, 2, 1), (1000, 4, 4), (100003, 20, 3), (70001, 33, 17), (4096, 16, 16)])
+@pytest.mark.parametrize("shape", [(5, 2, 1), (1000, 4, 4), (100003, 20, 3), (70001, 33, 17), (4096, 16, 16),
+                                   (30011, 7, 40), (3001, 48, 50)])
 def test_block_gram_mfma_against_valu_and_numpy(gpu, dtype, shape):
     n, m, p = shape
     print("block gram", np.dtype(dtype).name, shape, flush=True)
