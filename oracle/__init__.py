@@ -5,7 +5,8 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 The product library (``libqsgpu.so``) never loads it.
 
 ``qs_oracle.c`` restates QuSpin.jl's CPU algorithms (cited per function there);
-``krylov.py`` restates the Krylov exponential action (operators.jl:1655-1893) in numpy.
+``WideSector`` / ``parent_hamiltonian`` below cover chains of up to 256 sites
+(basis.jl:361-445, 1231-1300).
 """
 import ctypes
 import os
@@ -53,6 +54,22 @@ def lib():
     L.qo_sector_norms.restype = None
     L.qo_sector_free.argtypes = [vp]
     L.qo_sector_free.restype = None
+    L.qo_wsector_build.argtypes = [c_int, c_int, c_int, c_int, vp, vp]
+    L.qo_wsector_build.restype = vp
+    for name in ("qo_wsector_ncols", "qo_wsector_nparent", "qo_wsector_nnz"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = c_i64
+    L.qo_wsector_reduced.argtypes = [vp]
+    L.qo_wsector_reduced.restype = c_int
+    for name in ("qo_wsector_parent_states", "qo_wsector_labels"):
+        getattr(L, name).argtypes = [vp, vp]
+        getattr(L, name).restype = None
+    L.qo_wsector_projector.argtypes = [vp, vp, vp, vp]
+    L.qo_wsector_projector.restype = None
+    L.qo_wsector_norms.argtypes = [vp, vp, vp]
+    L.qo_wsector_norms.restype = None
+    L.qo_wsector_free.argtypes = [vp]
+    L.qo_wsector_free.restype = None
     L.qo_op_create.argtypes = [vp, c_int, vp, vp, vp, vp, c_int, vp]
     L.qo_op_create.restype = vp
     L.qo_op_matvec.argtypes = [vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, c_int, c_int]
@@ -198,6 +215,123 @@ class Sector:
         for c in range(self.Ns):
             P[row[colptr[c]:colptr[c + 1]], c] = val[colptr[c]:colptr[c + 1]]
         return P
+
+
+def _words_to_int(w):
+    return int(w[0]) | int(w[1]) << 64 | int(w[2]) << 128 | int(w[3]) << 192
+
+
+class WideSector:
+    """Reference-semantics SpinBasisGeneral on up to 256 sites (basis.jl:361-445,
+    530-620; the reference's UInt256 states): states are Python ints, the
+    parent is the nup sector in ascending order, `blocks` as in Sector.general."""
+
+    def __init__(self, L, nup, pauli=True, blocks=()):
+        lb = lib()
+        maps = np.ascontiguousarray(np.asarray([list(m) for m, _ in blocks] or [[0]], dtype=np.int32))
+        qn = np.ascontiguousarray(np.asarray([int(q) for _, q in blocks] or [0], dtype=np.int32))
+        h = lb.qo_wsector_build(L, int(nup), int(bool(pauli)), len(blocks), _ptr(maps), _ptr(qn))
+        if not h:
+            raise ValueError(lb.qo_last_error().decode())
+        self._h = h
+        self.L, self.nup, self.pauli = L, nup, pauli
+        self.Ns = lb.qo_wsector_ncols(h)
+        self.npar = lb.qo_wsector_nparent(h)
+        self.reduced = bool(lb.qo_wsector_reduced(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().qo_wsector_free(self._h)
+            self._h = None
+
+    def parent_states(self):
+        out = np.empty((self.npar, 4), dtype=np.uint64)
+        lib().qo_wsector_parent_states(self._h, _ptr(out))
+        return [_words_to_int(r) for r in out]
+
+    def states(self):
+        out = np.empty((self.Ns, 4), dtype=np.uint64)
+        lib().qo_wsector_labels(self._h, _ptr(out))
+        return [_words_to_int(r) for r in out]
+
+    def norms(self):
+        ref = np.empty(self.Ns, dtype=np.float64)
+        orb = np.empty(self.Ns, dtype=np.uint32)
+        lib().qo_wsector_norms(self._h, _ptr(ref), _ptr(orb))
+        return orb, ref
+
+    def projector_sparse(self):
+        """P as a scipy CSC matrix (npar x Ns); the identity without blocks."""
+        import scipy.sparse as sp
+        if not self.reduced:
+            return sp.identity(self.npar, dtype=np.complex128, format="csc")
+        nnz = lib().qo_wsector_nnz(self._h)
+        colptr = np.empty(self.Ns + 1, dtype=np.int64)
+        row = np.empty(nnz, dtype=np.int64)
+        val = np.empty(nnz, dtype=np.complex128)
+        lib().qo_wsector_projector(self._h, _ptr(colptr), _ptr(row), _ptr(val))
+        return sp.csc_matrix((val, row, colptr), shape=(self.npar, self.Ns))
+
+
+def _local(op, state, site, pauli):
+    """basis.jl:1231-1259 (_apply_wide_spin_local) / operators.jl:343-379:
+    (new state, factor, alive) of one local operator on 1-based `site`."""
+    mask = 1 << (site - 1)
+    occ = bool(state & mask)
+    half = 1.0 if pauli else 0.5
+    if op == "I":
+        return state, 1.0, True
+    if op == "z":
+        return state, (half if occ else -half), True
+    if op == "+":
+        if occ:
+            return state, 0.0, False
+        return state | mask, (2.0 if pauli else 1.0), True
+    if op == "-":
+        if not occ:
+            return state, 0.0, False
+        return state & ~mask, (2.0 if pauli else 1.0), True
+    if op == "x":
+        return state ^ mask, half, True
+    if op == "y":
+        return state ^ mask, (-1j * half if occ else 1j * half), True
+    raise ValueError("unsupported spin operator %r" % op)
+
+
+def parent_hamiltonian(states, terms, pauli=True):
+    """basis.jl:1261-1300 (_wide_spin_operator_triplets): the operator list on
+    the parent states (ascending Python ints) as a scipy CSC matrix; operator
+    strings act right to left, couplings (coef, site1, site2, ...) 1-based."""
+    import scipy.sparse as sp
+    index = {s: i for i, s in enumerate(states)}
+    rows, cols, vals = [], [], []
+    for op, couplings in terms:
+        for c in couplings:
+            sites = c[1:]
+            for col, init in enumerate(states):
+                st, amp, alive = init, complex(c[0]), True
+                for k in range(len(op) - 1, -1, -1):
+                    st, f, alive = _local(op[k], st, int(sites[k]), pauli)
+                    if not alive:
+                        break
+                    amp *= f
+                if not alive:
+                    continue
+                r = index.get(st)
+                if r is None:
+                    continue
+                rows.append(r)
+                cols.append(col)
+                vals.append(amp)
+    n = len(states)
+    return sp.csc_matrix((np.asarray(vals, dtype=np.complex128), (rows, cols)), shape=(n, n))
+
+
+def wide_sector_matrix(sector, terms):
+    """P' H P of a WideSector (the projected triplet matrix of basis.jl:1302-1326)."""
+    P = sector.projector_sparse()
+    H = parent_hamiltonian(sector.parent_states(), terms, sector.pauli)
+    return (P.conj().T @ H @ P).toarray()
 
 
 def _flatten_terms(terms):

@@ -11,6 +11,7 @@
  *   qo_fixed_weight_states  basis.jl:229-250      Gosper enumeration, ascending
  *   qo_sector_build         basis.jl:622-777      stage order k, p, z, pz, zA, zB
  *   qo_sector_build_general basis.jl:282-338,530-620  SpinBasisGeneral site maps, stages in block order
+ *   qo_wsector_build        basis.jl:361-445,530-620  the same for up to 256 sites (UInt256 states)
  *                           symmetry_basis.jl:38-55   signed permutation S
  *                           symmetry_basis.jl:57-108  cyclic projector (orbit walk)
  *                           symmetry_basis.jl:110-213 eigenspace intersection P'SP
@@ -409,9 +410,9 @@ static uint64_t apply_transform(const transform_t *t, uint64_t s)
     return s;
 }
 
-/* symmetry_basis.jl:57-108 */
-static int cyclic_projector(csc_t *out, const uint64_t *states, int64_t n,
-                            const transform_t *tr, cplx lambda)
+/* symmetry_basis.jl:57-108, on the image table of the map: img[i] = index
+ * of the image of parent state i (-1: outside the parent states). */
+static int cyclic_projector_img(csc_t *out, const int64_t *img, int64_t n, cplx lambda)
 {
     char *visited = calloc((size_t)n, 1);
     int64_t cap = n, nt = 0, ncol = 0;
@@ -428,8 +429,7 @@ static int cyclic_projector(csc_t *out, const uint64_t *states, int64_t n,
             if (len >= 128) { set_err("symmetry map produced a non-cyclic orbit"); rc = -1; goto done; }
             orbit[len++] = cur;
             visited[cur] = 1;
-            uint64_t img = apply_transform(tr, states[cur]);
-            int64_t nxt = lookup_state(states, n, img);
+            int64_t nxt = img[cur];
             if (nxt < 0) { set_err("symmetry transformation leaves the selected particle sector"); rc = -1; goto done; }
             if (nxt == start) break;
             for (int q = 0; q < len; q++)
@@ -456,8 +456,8 @@ done:
     return rc;
 }
 
-/* symmetry_basis.jl:38-55 */
-static int signed_permutation(csc_t *out, const uint64_t *states, int64_t n, const transform_t *tr)
+/* symmetry_basis.jl:38-55, on the image table of the map */
+static int signed_permutation_img(csc_t *out, const int64_t *img, int64_t n)
 {
     out->m = n; out->n = n; out->nnz = n;
     out->colptr = malloc(sizeof(int64_t) * (size_t)(n + 1));
@@ -465,14 +465,40 @@ static int signed_permutation(csc_t *out, const uint64_t *states, int64_t n, con
     out->val = malloc(sizeof(cplx) * (size_t)(n ? n : 1));
     if (!out->colptr || !out->row || !out->val) { set_err("out of memory"); return -1; }
     for (int64_t c = 0; c < n; c++) {
-        int64_t r = lookup_state(states, n, apply_transform(tr, states[c]));
-        if (r < 0) { set_err("symmetry transformation leaves the selected particle sector"); return -1; }
+        if (img[c] < 0) { set_err("symmetry transformation leaves the selected particle sector"); return -1; }
         out->colptr[c] = c;
-        out->row[c] = r;
+        out->row[c] = img[c];
         out->val[c] = c_make(1.0, 0.0);
     }
     out->colptr[n] = n;
     return 0;
+}
+
+static int64_t *image_table(const uint64_t *states, int64_t n, const transform_t *tr)
+{
+    int64_t *img = malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+    if (!img) { set_err("out of memory"); return NULL; }
+    for (int64_t i = 0; i < n; i++) img[i] = lookup_state(states, n, apply_transform(tr, states[i]));
+    return img;
+}
+
+static int cyclic_projector(csc_t *out, const uint64_t *states, int64_t n,
+                            const transform_t *tr, cplx lambda)
+{
+    int64_t *img = image_table(states, n, tr);
+    if (!img) return -1;
+    int rc = cyclic_projector_img(out, img, n, lambda);
+    free(img);
+    return rc;
+}
+
+static int signed_permutation(csc_t *out, const uint64_t *states, int64_t n, const transform_t *tr)
+{
+    int64_t *img = image_table(states, n, tr);
+    if (!img) return -1;
+    int rc = signed_permutation_img(out, img, n);
+    free(img);
+    return rc;
 }
 
 /* symmetry_basis.jl:110-213 (the permutation-like branch; the dense nullspace
@@ -800,6 +826,188 @@ void qo_sector_norms(const qo_sector *s, double *ref_norm, uint32_t *orbit)
             continue;
         }
         int64_t row = lookup_state(s->parent, s->npar, s->labels[c]);
+        cplx amp = c_make(0, 0);
+        for (int64_t p = s->P.colptr[c]; p < s->P.colptr[c + 1]; p++)
+            if (s->P.row[p] == row) amp = s->P.val[p];
+        if (ref_norm) ref_norm[c] = 1.0 / c_abs2(amp);
+        if (orbit) orbit[c] = (uint32_t)(s->P.colptr[c + 1] - s->P.colptr[c]);
+    }
+}
+
+/* ------------------------------------------------------- wide sector --- */
+/* basis.jl:361-445 (_wide_spin_general_basis) and basis.jl:530-620: the same
+ * stages for chains of up to 256 sites (the reference's UInt256 states),
+ * states as four 64-bit words (site i, 1-based, is bit (i - 1) % 64 of word
+ * (i - 1) / 64), the parent the nup sector in ascending order. */
+typedef struct { uint64_t w[4]; } wstate;
+
+static int ws_less(const wstate *a, const wstate *b)
+{
+    for (int k = 3; k >= 0; k--)
+        if (a->w[k] != b->w[k]) return a->w[k] < b->w[k];
+    return 0;
+}
+
+static int ws_eq(const wstate *a, const wstate *b)
+{
+    return a->w[0] == b->w[0] && a->w[1] == b->w[1] && a->w[2] == b->w[2] && a->w[3] == b->w[3];
+}
+
+static int64_t ws_lookup(const wstate *states, int64_t n, const wstate *s)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (ws_less(&states[mid], s)) lo = mid + 1; else hi = mid;
+    }
+    return (lo < n && ws_eq(&states[lo], s)) ? lo : -1;
+}
+
+/* basis.jl:325-338: the (inverted) bit of each source site moves to its target */
+static wstate ws_apply_general(int L, const int *perm, const unsigned char *flips, const wstate *s)
+{
+    wstate out = {{0, 0, 0, 0}};
+    for (int source = 1; source <= L; source++) {
+        int occupied = (int)((s->w[(source - 1) >> 6] >> ((source - 1) & 63)) & 1);
+        if (flips[source - 1]) occupied = !occupied;
+        if (occupied) {
+            int t = perm[source - 1] - 1;
+            out.w[t >> 6] |= 1ULL << (t & 63);
+        }
+    }
+    return out;
+}
+
+typedef struct qo_wsector {
+    int L, nup, pauli, reduced;
+    int64_t npar, ncol;
+    wstate *parent;     /* ascending parent states */
+    csc_t P;
+    wstate *labels;
+} qo_wsector;
+
+void qo_wsector_free(qo_wsector *s)
+{
+    if (!s) return;
+    free(s->parent);
+    free(s->labels);
+    csc_free(&s->P);
+    free(s);
+}
+
+/* the particle sector of L sites and w particles in ascending order: the
+ * occupied sites as an increasing tuple, advanced in colexicographic order */
+static int64_t ws_fixed_weight(int L, int w, wstate *out, int64_t cap)
+{
+    int pos[256];
+    for (int i = 0; i < w; i++) pos[i] = i;
+    int64_t n = 0;
+    for (;;) {
+        if (n >= cap) return -1;
+        wstate st = {{0, 0, 0, 0}};
+        for (int i = 0; i < w; i++) st.w[pos[i] >> 6] |= 1ULL << (pos[i] & 63);
+        out[n++] = st;
+        int i = 0;
+        while (i < w && pos[i] + 1 == (i + 1 < w ? pos[i + 1] : L)) i++;
+        if (i == w) break;
+        pos[i]++;
+        for (int j = 0; j < i; j++) pos[j] = j;
+    }
+    return n;
+}
+
+qo_wsector *qo_wsector_build(int L, int nup, int pauli, int nblocks, const int *maps, const int *qnumbers)
+{
+    if (L < 1 || L > 256) { set_err("L must be between 1 and 256"); return NULL; }
+    if (nup < 0 || nup > L) { set_err("nup must lie between 0 and L"); return NULL; }
+    double cnt = 1;
+    for (int i = 0; i < nup; i++) cnt = cnt * (double)(L - i) / (double)(i + 1);
+    if (cnt > (double)(1 << 24)) { set_err("sector too large for the oracle"); return NULL; }
+    qo_wsector *s = calloc(1, sizeof *s);
+    s->L = L; s->nup = nup; s->pauli = pauli;
+    int64_t cap = (int64_t)(cnt + 0.5) + 1;
+    s->parent = malloc(sizeof(wstate) * (size_t)cap);
+    s->npar = ws_fixed_weight(L, nup, s->parent, cap);
+    if (s->npar < 0) { set_err("particle sector count"); qo_wsector_free(s); return NULL; }
+    const int64_t n = s->npar;
+    if (nblocks == 0) {
+        s->reduced = 0;
+        s->ncol = n;
+        s->labels = malloc(sizeof(wstate) * (size_t)n);
+        memcpy(s->labels, s->parent, sizeof(wstate) * (size_t)n);
+        return s;
+    }
+    int *perm = malloc(sizeof(int) * (size_t)L);
+    unsigned char *flips = malloc((size_t)L);
+    int64_t *img = malloc(sizeof(int64_t) * (size_t)n);
+    for (int bi = 0; bi < nblocks; bi++) {
+        if (normalize_general_map(maps + (size_t)bi * L, L, perm, flips)) goto fail;
+        int order = general_map_order(perm, flips, L);
+        int m = ((qnumbers[bi] % order) + order) % order;
+        double ang = (6.283185307179586 * (double)m) / (double)order;
+        cplx lam = c_make(cos(ang), sin(ang));
+        for (int64_t i = 0; i < n; i++) {
+            wstate im = ws_apply_general(L, perm, flips, &s->parent[i]);
+            img[i] = ws_lookup(s->parent, n, &im);
+        }
+        if (bi == 0) {
+            if (cyclic_projector_img(&s->P, img, n, lam)) goto fail;
+        } else {
+            csc_t S = {0};
+            if (signed_permutation_img(&S, img, n)) { csc_free(&S); goto fail; }
+            int rc = intersect_eigenspace(&s->P, &S, lam);
+            csc_free(&S);
+            if (rc) goto fail;
+        }
+    }
+    free(perm); free(flips); free(img);
+    if (s->P.n == 0) { set_err("the requested symmetry sector is empty"); qo_wsector_free(s); return NULL; }
+    s->reduced = 1;
+    s->ncol = s->P.n;
+    /* symmetry_basis.jl:215-232 */
+    s->labels = malloc(sizeof(wstate) * (size_t)s->ncol);
+    for (int64_t c = 0; c < s->ncol; c++) {
+        int64_t p0 = s->P.colptr[c], best = p0;
+        for (int64_t p = p0 + 1; p < s->P.colptr[c + 1]; p++)
+            if (c_abs(s->P.val[p]) > c_abs(s->P.val[best])) best = p;
+        s->labels[c] = s->parent[s->P.row[best]];
+    }
+    return s;
+fail:
+    free(perm); free(flips); free(img);
+    qo_wsector_free(s);
+    return NULL;
+}
+
+int64_t qo_wsector_ncols(const qo_wsector *s) { return s->ncol; }
+int64_t qo_wsector_nparent(const qo_wsector *s) { return s->npar; }
+int64_t qo_wsector_nnz(const qo_wsector *s) { return s->reduced ? s->P.nnz : 0; }
+int qo_wsector_reduced(const qo_wsector *s) { return s->reduced; }
+void qo_wsector_parent_states(const qo_wsector *s, uint64_t *out)
+{
+    memcpy(out, s->parent, sizeof(wstate) * (size_t)s->npar);
+}
+void qo_wsector_labels(const qo_wsector *s, uint64_t *out)
+{
+    memcpy(out, s->labels, sizeof(wstate) * (size_t)s->ncol);
+}
+void qo_wsector_projector(const qo_wsector *s, int64_t *colptr, int64_t *row, double *val)
+{
+    if (!s->reduced) return;
+    memcpy(colptr, s->P.colptr, sizeof(int64_t) * (size_t)(s->P.n + 1));
+    memcpy(row, s->P.row, sizeof(int64_t) * (size_t)s->P.nnz);
+    memcpy(val, s->P.val, sizeof(cplx) * (size_t)s->P.nnz);
+}
+/* discrete_basis.jl:2386-2413, as qo_sector_norms */
+void qo_wsector_norms(const qo_wsector *s, double *ref_norm, uint32_t *orbit)
+{
+    for (int64_t c = 0; c < s->ncol; c++) {
+        if (!s->reduced) {
+            if (ref_norm) ref_norm[c] = 1.0;
+            if (orbit) orbit[c] = 1;
+            continue;
+        }
+        int64_t row = ws_lookup(s->parent, s->npar, &s->labels[c]);
         cplx amp = c_make(0, 0);
         for (int64_t p = s->P.colptr[c]; p < s->P.colptr[c + 1]; p++)
             if (s->P.row[p] == row) amp = s->P.val[p];
