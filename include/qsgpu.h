@@ -66,6 +66,8 @@ extern "C" {
  * promote_type with eltype(diagonal), operators.jl:2523-2527), so products
  * and solvers take complex vectors. */
 #define QS_FORCE_COMPLEX 8
+/* qs_basis_create_general flags: the wide (four-word, stored) form at any L */
+#define QS_BASIS_EXPLICIT 1
 
 typedef struct qs_basis qs_basis;
 typedef struct qs_op qs_op;
@@ -91,10 +93,15 @@ int qs_basis_create(int L, int nup, int pauli, int a, int kblock, int pblock, in
  * labels (negative = inverted), as _normalize_general_spin_map reads them
  * (basis.jl:282-304); qnumbers: each block's quantum number q, eigenvalue
  * cis(2 pi mod(q, order) / order) with the map's order (basis.jl:306-323).
- * L <= 32 on the GPU; the maps may generate at most 256 elements.  Sectors
- * of such a basis take two-bit hop couplings (the parent-table kernels).
- * Errors: the reference's ArgumentError messages for a map that is not a
- * permutation, a flip that leaves the particle sector, an empty sector. */
+ * L <= 256; the maps may generate at most 256 elements.  L <= 32: sectors
+ * take two-bit hop couplings (the parent-table kernels).  L > 32 with
+ * blocks, L > 63, or flags & QS_BASIS_EXPLICIT: a wide basis (basis.jl:
+ * 361-445; the reference's WideSpinBasis above 63 sites), which needs nup:
+ * its orbit minima are stored as four 64-bit words each (site i = bit i % 64
+ * of word i / 64) and products take any monomial couplings (one kernel,
+ * group orbit and binary search per neighbour).  Errors: the reference's
+ * ArgumentError messages for a map that is not a permutation, a flip that
+ * leaves the particle sector, an empty sector, a missing nup above 32 sites. */
 int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_t *maps, const int32_t *qnumbers,
                             int flags, qs_basis **out);
 int qs_basis_destroy(qs_basis *b);
@@ -109,6 +116,13 @@ int qs_basis_states(const qs_basis *b, uint64_t *out);
 /* Orbit-minimum canonical state per column (the key the GPU kernels index
  * by; equals states(basis) wherever the reference label is the orbit min). */
 int qs_basis_reps(const qs_basis *b, uint64_t *out);
+
+/* states(basis) (which = 0) or the orbit minima (which = 1) as four 64-bit
+ * words per column (site i = bit i % 64 of word i / 64), for any basis; the
+ * only form that holds the states of a basis of more than 64 sites
+ * (basis.jl:802 states(::WideSpinBasis), UInt256 values).  out: 4 Ns words.
+ * qs_basis_states / qs_basis_reps refuse such a basis. */
+int qs_basis_states_wide(const qs_basis *b, int which, uint64_t *out);
 
 /* Per column: number of distinct parent states in the orbit, and the
  * reference's normalization(basis, states(basis)[i]) (discrete_basis.jl:2401-2408). */

@@ -417,8 +417,8 @@ static int cyclic_projector_img(csc_t *out, const int64_t *img, int64_t n, cplx 
     char *visited = calloc((size_t)n, 1);
     int64_t cap = n, nt = 0, ncol = 0;
     trip_t *t = malloc(sizeof(trip_t) * (size_t)cap);
-    int64_t *orbit = malloc(sizeof(int64_t) * 130);
-    cplx *coef = malloc(sizeof(cplx) * 130);
+    int64_t *orbit = malloc(sizeof(int64_t) * (size_t)(n + 1));
+    cplx *coef = malloc(sizeof(cplx) * (size_t)(n + 1));
     if (!visited || !t || !orbit || !coef) { set_err("out of memory"); return -1; }
     int rc = 0;
     for (int64_t start = 0; start < n; start++) {
@@ -426,7 +426,7 @@ static int cyclic_projector_img(csc_t *out, const int64_t *img, int64_t n, cplx 
         int len = 0;
         int64_t cur = start;
         for (;;) {
-            if (len >= 128) { set_err("symmetry map produced a non-cyclic orbit"); rc = -1; goto done; }
+            if (len >= n) { set_err("symmetry map produced a non-cyclic orbit"); rc = -1; goto done; }
             orbit[len++] = cur;
             visited[cur] = 1;
             int64_t nxt = img[cur];

@@ -113,8 +113,24 @@ uint64_t unrank_host(int64_t i, int L, int w) {
     return u;
 }
 
+// the reference labels and normalizations, computed on first use
+std::mutex g_label_mutex;
+void ensure_labels(BasisHost &b) {
+    std::lock_guard<std::mutex> g(g_label_mutex);
+    if (b.labels_ready) return;
+    if (b.wide) compute_wide_labels(b);
+    else compute_reference_labels(b);
+}
+
 void host_reps(const BasisHost &b, std::vector<uint64_t> &out) {
     out.resize((size_t)b.Ns);
+    if (b.wide) {
+        if (b.L > 64) fail(QS_EARG, "states of more than 64 sites take four words: use qs_basis_states_wide");
+        std::vector<uint64_t> w((size_t)4 * b.Ns);
+        QS_HIP(hipMemcpy(w.data(), b.wreps.get(), sizeof(uint64_t) * w.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < b.Ns; i++) out[(size_t)i] = w[(size_t)4 * i];
+        return;
+    }
     if (b.mode == MODE_STORED) {
         if (b.use64) {
             QS_HIP(hipMemcpy(out.data(), b.reps64.get(), sizeof(uint64_t) * b.Ns, hipMemcpyDeviceToHost));
@@ -490,11 +506,13 @@ int qs_basis_create(int L, int nup, int pauli, int a, int kblock, int pblock, in
 
 int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_t *maps, const int32_t *qnumbers,
                             int flags, qs_basis **out) {
-    (void)flags;
     return guard([&] {
         check_ptr(out, "out");
         *out = nullptr;
-        if (L < 1 || L > 63) fail(QS_EARG, "L must be between 1 and 63");
+        if (L < 1 || L > kMaxWideL) fail(QS_EARG, "L must be between 1 and 256");
+        const bool wide = (flags & QS_BASIS_EXPLICIT) || L > 63 || (L > 32 && nblocks > 0);
+        if (wide && nup == QS_UNSET)
+            fail(QS_EARG, "wide spin bases require nup, Nup, or m to avoid enumerating 2^L states");
         if (nup != QS_UNSET && (nup < 0 || nup > L)) fail(QS_EARG, "nup must lie between 0 and L");
         if (nblocks < 0) fail(QS_EARG, "nblocks must be non-negative");
         if (nblocks > 0) {
@@ -522,7 +540,10 @@ int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_
             for (int i = 0; i < L; i++) {
                 const int64_t target = zero_based ? (v[i] >= 0 ? (int64_t)v[i] + 1 : -(int64_t)v[i])
                                                   : (v[i] < 0 ? -(int64_t)v[i] : (int64_t)v[i]);
-                if (v[i] < 0) m.flip |= 1ULL << i;
+                if (v[i] < 0) {
+                    m.flipw.set(i);
+                    if (i < 64) m.flip |= 1ULL << i;
+                }
                 if (target < 1 || target > L || hit[(size_t)target]) {
                     ok = false;
                     break;
@@ -540,7 +561,7 @@ int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_
                 while (!seen[(size_t)cur]) {
                     seen[(size_t)cur] = 1;
                     len++;
-                    par ^= (int)(m.flip >> cur & 1);
+                    par ^= (int)m.flipw.bit(cur);
                     cur = m.perm[(size_t)cur];
                 }
                 const int64_t c = par ? 2 * len : len;
@@ -551,8 +572,12 @@ int qs_basis_create_general(int L, int nup, int pauli, int nblocks, const int32_
             m.q = qnumbers[bi];
             b.gmaps.push_back(m);
         }
-        build_group_general(b);
-        build_basis_device(b);
+        if (wide) {
+            build_wide_basis(b);
+        } else {
+            build_group_general(b);
+            build_basis_device(b);
+        }
         *out = h.release();
     });
 }
@@ -591,8 +616,40 @@ int qs_basis_states(const qs_basis *b, uint64_t *out) {
             memcpy(out, r.data(), sizeof(uint64_t) * r.size());
             return;
         }
-        if (!bb.labels_ready) compute_reference_labels(bb);
+        if (bb.wide && bb.L > 64) fail(QS_EARG, "states of more than 64 sites take four words: use qs_basis_states_wide");
+        ensure_labels(bb);
         memcpy(out, bb.labels.data(), sizeof(uint64_t) * bb.labels.size());
+    });
+}
+
+int qs_basis_states_wide(const qs_basis *b, int which, uint64_t *out) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        check_ptr(out, "out");
+        if (which != 0 && which != 1) fail(QS_EARG, "which must be 0 (states) or 1 (orbit minima)");
+        BasisHost &bb = const_cast<BasisHost &>(b->b);
+        const int64_t n = bb.Ns;
+        if (bb.wide) {
+            if (which == 1) {
+                QS_HIP(hipMemcpy(out, bb.wreps.get(), sizeof(uint64_t) * 4 * n, hipMemcpyDeviceToHost));
+                return;
+            }
+            ensure_labels(bb);
+            for (int64_t i = 0; i < n; i++)
+                for (int k = 0; k < 4; k++) out[4 * i + k] = bb.wlabels[(size_t)i].w[k];
+            return;
+        }
+        std::vector<uint64_t> r;
+        if (which == 1 || bb.mode != MODE_STORED) {
+            host_reps(bb, r);
+        } else {
+            ensure_labels(bb);
+            r = bb.labels;
+        }
+        for (int64_t i = 0; i < n; i++) {
+            out[4 * i] = r[(size_t)i];
+            out[4 * i + 1] = out[4 * i + 2] = out[4 * i + 3] = 0;
+        }
     });
 }
 
@@ -616,7 +673,7 @@ int qs_basis_norms(const qs_basis *b, uint32_t *orbit_size, double *ref_norm) {
         }
         if (orbit_size) QS_HIP(hipMemcpy(orbit_size, bb.orbit.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         if (ref_norm) {
-            if (!bb.labels_ready) compute_reference_labels(bb);
+            ensure_labels(bb);
             memcpy(ref_norm, bb.ref_norm.data(), sizeof(double) * n);
         }
     });
@@ -634,6 +691,7 @@ int qs_basis_lookup(const qs_basis *b, const uint64_t *states, int64_t n, int64_
     return guard([&] {
         check_ptr(b, "basis");
         const BasisHost &bb = b->b;
+        if (bb.wide && bb.L > 64) fail(QS_EARG, "states of more than 64 sites do not fit the 64-bit lookup");
         std::vector<uint64_t> reps;
         if (bb.mode == MODE_STORED) host_reps(bb, reps);
         uint64_t mask = full_mask(bb.L);
@@ -707,6 +765,47 @@ int qs_op_create(const qs_basis *b, int n_terms, const char *const *opstr, const
         bool check_herm = flags & QS_CHECK_HERM, check_pcon = flags & QS_CHECK_PCON, check_symm = flags & QS_CHECK_SYMM;
         if (check_herm && !herm) fail(QS_EARG, "operator list is not Hermitian");
         if (check_pcon && bb.nup >= 0 && !pcon_ok(cs)) fail(QS_EARG, "operator list violates the selected particle sector");
+        if (bb.wide) {
+            // bases of up to 256 sites: four-word term masks, one kernel
+            bool diag_complex = false;
+            if (diag) {
+                o.user_diag.resize((size_t)bb.Ns);
+                for (int64_t i = 0; i < bb.Ns; i++) {
+                    o.user_diag[i] = cd(diag[2 * i], diag[2 * i + 1]);
+                    if (diag[2 * i + 1] != 0) diag_complex = true;
+                }
+                o.has_user_diag = true;
+                if (check_herm && diag_complex) fail(QS_EARG, "a Hermitian operator requires a real diagonal");
+            }
+            o.is_complex = (any_complex || bb.is_complex || diag_complex || (flags & 8)) ? 1 : 0;
+            setup_wide_terms(o, cs, check_symm);
+            o.hermitian = check_herm ? 1 : ((herm && !diag_complex) ? 1 : 0);
+            auto upload_diag = [&](OpHost &t) {
+                if (!t.has_user_diag) return;
+                t.d_user_diag.alloc((size_t)2 * bb.Ns);
+                QS_HIP(hipMemcpy(t.d_user_diag.get(), diag, sizeof(double) * 2 * bb.Ns, hipMemcpyHostToDevice));
+            };
+            upload_diag(o);
+            if (bb.group.order() == 1) {
+                // a basis without symmetry: the structurally diagonal terms
+                // apart, as on 64-bit states (operators.jl:2372-2429)
+                std::vector<Coupling> dcs;
+                for (auto &c : cs)
+                    if (c.op.find_first_not_of("Izn|") == std::string::npos) dcs.push_back(c);
+                o.extracted = std::make_unique<OpHost>();
+                OpHost &e = *o.extracted;
+                e.basis = &bb;
+                e.is_complex = o.is_complex;
+                e.has_user_diag = o.has_user_diag;
+                e.user_diag = o.user_diag;
+                setup_wide_terms(e, dcs, false);
+                e.is_complex = o.is_complex;
+                upload_diag(e);
+            }
+            ensure_stream(o);
+            *out = h.release();
+            return;
+        }
         std::vector<CTerm> compiled = compile_couplings(cs, bb.L, bb.pauli);
         std::vector<CTerm> H = consolidate_cterms(compiled);
         if (check_symm && bb.mode != MODE_FULL) {
@@ -847,6 +946,8 @@ int qs_op_stream(const qs_op *op, void **stream) {
 int qs_op_set_variant(qs_op *op, int lanes_per_row) {
     return guard([&] {
         check_ptr(op, "op");
+        if (op->o.basis->wide && lanes_per_row != 0)
+            fail(QS_EARG, "a wide basis has one matvec kernel (variant 0)");
         if (lanes_per_row != 0 && lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 &&
             lanes_per_row != 8 && (lanes_per_row < QS_VARIANT_SPLIT || lanes_per_row > QS_VARIANT_SPLIT_MAX) &&
             (lanes_per_row < QS_VARIANT_HOP || lanes_per_row > QS_VARIANT_HOP + 3) && lanes_per_row != QS_VARIANT_SELL)
@@ -896,7 +997,8 @@ int qs_op_tune(qs_op *op, int dtype, int repeats, int *lanes_per_row, double *be
                 if (k < QS_VARIANT_SPLIT + QS_SPLIT_LEAN || (split_lean_ok(o, false) && split_lean_ok(o, true)))
                     cands.push_back(k);
         if (o.sell) cands.push_back(QS_VARIANT_SELL);
-        if (o.basis->mode == MODE_STORED) cands.insert(cands.end(), {1, 2, 4, 8});
+        if (o.basis->wide) cands.assign(1, 0);
+        else if (o.basis->mode == MODE_STORED) cands.insert(cands.end(), {1, 2, 4, 8});
         else cands.push_back(1);
         for (int lpr : cands) {
             o.lanes_per_row = lpr;
@@ -933,6 +1035,7 @@ int qs_op_assemble(qs_op *op) {
     return guard([&] {
         check_ptr(op, "op");
         OpHost &o = op->o;
+        if (o.basis->wide) fail(QS_EARG, "assembly is not available for a wide basis");
         hipStream_t s = ensure_stream(o);
         sell_assemble(o, s);
     });
@@ -1190,6 +1293,7 @@ int qs_batch_create(qs_op *const *ops, int n_ops, int dtype, qs_batch **out) {
         auto h = std::make_unique<qs_batch>();
         for (int i = 0; i < n_ops; i++) {
             check_ptr(ops[i], "ops[i]");
+            if (ops[i]->o.basis->wide) fail(QS_EARG, "the batch engine does not take wide bases");
             h->ops.push_back(&ops[i]->o);
         }
         dtype_size(dtype);

@@ -283,4 +283,58 @@ __device__ __forceinline__ S comb_unrank(int64_t i, int L, int w, const B *binom
     return u;
 }
 
+// ---------------------------------------------------------- wide states --
+// States of up to 256 sites as four words (site i = bit i % 64 of word i / 64),
+// ordered as 256-bit integers.
+struct W4 { uint64_t w[4]; };
+
+__device__ __forceinline__ bool w4_less(const W4 &a, const W4 &b) {
+    if (a.w[3] != b.w[3]) return a.w[3] < b.w[3];
+    if (a.w[2] != b.w[2]) return a.w[2] < b.w[2];
+    if (a.w[1] != b.w[1]) return a.w[1] < b.w[1];
+    return a.w[0] < b.w[0];
+}
+__device__ __forceinline__ bool w4_eq(const W4 &a, const W4 &b) {
+    return a.w[0] == b.w[0] && a.w[1] == b.w[1] && a.w[2] == b.w[2] && a.w[3] == b.w[3];
+}
+__device__ __forceinline__ W4 w4_at(const uint64_t *p, int64_t i) {
+    const uint64_t *q = p + 4 * i;
+    return W4{{q[0], q[1], q[2], q[3]}};
+}
+
+// A wide group: element e sends site i to site perm[e L + i], then xors the
+// four words c[4 e ..]; character chi[2 e] + i chi[2 e + 1]; element 0 is
+// the identity.
+struct WGroup {
+    int n, L;
+    const uint8_t *perm;
+    const uint64_t *c;
+    const double *chi;
+};
+
+__device__ __forceinline__ W4 wg_apply(const WGroup &g, int e, const W4 &u) {
+    const uint8_t *pe = g.perm + (size_t)e * g.L;
+    W4 r = w4_at(g.c, e);
+    for (int k = 0; k < 4; k++) {
+        uint64_t bits = u.w[k];
+        while (bits) {
+            const int t = pe[64 * k + __builtin_ctzll(bits)];
+            r.w[t >> 6] ^= 1ULL << (t & 63);
+            bits &= bits - 1;
+        }
+    }
+    return r;
+}
+
+// Index of s in the ascending array st[0 .. n), or -1.
+__device__ __forceinline__ int64_t w4_find(const uint64_t *st, int64_t n, const W4 &s) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (w4_less(w4_at(st, mid), s)) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && w4_eq(w4_at(st, lo), s)) ? lo : -1;
+}
+
 }  // namespace qs

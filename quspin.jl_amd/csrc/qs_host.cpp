@@ -371,6 +371,104 @@ cd group_char(const GroupHost &g, int e)
     return g.lam[t] * g.pchar[p];
 }
 
+// ------------------------------------------------------------ wide groups --
+namespace {
+
+// A group element on up to 256 sites: u -> perm(u) ^ c.
+struct WAffine {
+    std::vector<int> perm;
+    WState c;
+    bool operator<(const WAffine &o) const {
+        if (perm != o.perm) return perm < o.perm;
+        return c < o.c;
+    }
+};
+
+WState wpermute(const std::vector<int> &perm, const WState &u)
+{
+    WState r;
+    for (size_t i = 0; i < perm.size(); i++)
+        if (u.bit((int)i)) r.set(perm[i]);
+    return r;
+}
+
+WAffine wcompose(const WAffine &g1, const WAffine &g2)   // g1 o g2
+{
+    WAffine r;
+    r.perm.resize(g1.perm.size());
+    for (size_t i = 0; i < g1.perm.size(); i++) r.perm[i] = g1.perm[g2.perm[i]];
+    r.c = wpermute(g1.perm, g2.c) ^ g1.c;
+    return r;
+}
+
+}  // namespace
+
+// basis.jl:361-445 and 530-620 on more than 32 sites (up to 256): the group
+// the blocks' maps generate, every element with its character, as in
+// build_group_general but with four-word xor constants.
+void build_group_wide(BasisHost &b)
+{
+    const int L = b.L;
+    GroupHost &g = b.group;
+    g = GroupHost();
+    g.L = L;
+    g.a = 1;
+    g.lam[0] = cd(1, 0);
+    g.pchar[0] = cd(1, 0);
+    WState full;
+    for (int i = 0; i < L; i++) full.set(i);
+    std::vector<std::pair<WAffine, cd>> gens;
+    for (const GenMap &m : b.gmaps) {
+        const WState f = m.flipw;
+        if (b.nup >= 0 && !f.zero() && !(f == full && 2 * b.nup == L))
+            fail(QS_EARG, "symmetry transformation leaves the selected particle sector");
+        WAffine a;
+        a.perm = m.perm;
+        a.c = wpermute(m.perm, f);
+        gens.push_back({a, exact_root(((m.q % m.order) + m.order) % m.order, m.order)});
+    }
+    WAffine id;
+    id.perm.resize(L);
+    for (int i = 0; i < L; i++) id.perm[i] = i;
+    std::map<WAffine, cd> chi;
+    std::vector<WAffine> queue{id};
+    chi[id] = cd(1, 0);
+    bool consistent = true;
+    for (size_t qi = 0; qi < queue.size(); qi++) {
+        const WAffine cur = queue[qi];
+        const cd cc = chi[cur];
+        for (auto &gg : gens) {
+            WAffine nx = wcompose(gg.first, cur);
+            cd nc = gg.second * cc;
+            auto it = chi.find(nx);
+            if (it == chi.end()) {
+                chi[nx] = nc;
+                queue.push_back(nx);
+                if (queue.size() > (size_t)kMaxGeneral) fail(QS_EARG, "symmetry group too large");
+            } else if (!close_to(it->second, nc)) {
+                consistent = false;
+            }
+        }
+    }
+    if (!consistent) fail(QS_EARG, "the requested symmetry sector is empty");
+    g.general = true;
+    for (const WAffine &e : queue) {
+        g.eperm.push_back(e.perm);
+        g.ecw.push_back(e.c);
+        g.ec.push_back(e.c.w[0]);
+        g.echar.push_back(chi[e]);
+    }
+    b.is_complex = 0;
+    for (const cd &c : g.echar)
+        if (std::abs(c.imag()) > 1e-12) b.is_complex = 1;
+    b.mode = MODE_STORED;
+}
+
+WState wide_group_apply(const GroupHost &g, int e, const WState &u)
+{
+    return wpermute(g.eperm[(size_t)e], u) ^ g.ecw[(size_t)e];
+}
+
 // ----------------------------------------------------------------- terms --
 namespace {
 
@@ -396,6 +494,14 @@ LocalAct local_op(char op, int b, int pauli)
 }
 
 }  // namespace
+
+void wide_local(char op, int bit, int pauli, bool &alive, int &out, cd &amp)
+{
+    const LocalAct a = local_op(op, bit, pauli);
+    alive = a.alive;
+    out = a.out;
+    amp = a.amp;
+}
 
 std::vector<CTerm> compile_couplings(const std::vector<Coupling> &cs, int L, int pauli)
 {

@@ -133,6 +133,39 @@ private:
 constexpr int kMaxPoint = 16;
 constexpr int kMaxRot = 64;
 
+// A state of up to 256 sites (the reference's UInt256 basis integers):
+// site i (0-based) is bit i % 64 of word i / 64.
+constexpr int kMaxWideL = 256;
+struct WState {
+    uint64_t w[4] = {0, 0, 0, 0};
+    bool operator<(const WState &o) const {
+        for (int k = 3; k >= 0; k--)
+            if (w[k] != o.w[k]) return w[k] < o.w[k];
+        return false;
+    }
+    bool operator==(const WState &o) const {
+        return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
+    }
+    bool operator!=(const WState &o) const { return !(*this == o); }
+    WState operator^(const WState &o) const {
+        WState r;
+        for (int k = 0; k < 4; k++) r.w[k] = w[k] ^ o.w[k];
+        return r;
+    }
+    WState operator&(const WState &o) const {
+        WState r;
+        for (int k = 0; k < 4; k++) r.w[k] = w[k] & o.w[k];
+        return r;
+    }
+    bool bit(int i) const { return (w[i >> 6] >> (i & 63)) & 1; }
+    void set(int i) { w[i >> 6] |= 1ULL << (i & 63); }
+    int popcount() const {
+        return __builtin_popcountll(w[0]) + __builtin_popcountll(w[1]) + __builtin_popcountll(w[2]) +
+               __builtin_popcountll(w[3]);
+    }
+    bool zero() const { return (w[0] | w[1] | w[2] | w[3]) == 0; }
+};
+
 struct GroupHost {
     int L = 0, a = 1, nT = 1;
     int npoint = 1;
@@ -147,6 +180,7 @@ struct GroupHost {
     std::vector<std::vector<int>> eperm;
     std::vector<uint64_t> ec;
     std::vector<cd> echar;
+    std::vector<WState> ecw;    // wide bases: the xor constants on up to 256 sites
     int order() const { return general ? (int)eperm.size() : nT * npoint; }
 };
 constexpr int kMaxGeneral = 256;   // elements of a general-map group
@@ -157,6 +191,7 @@ constexpr int kMaxGeneral = 256;   // elements of a general-map group
 struct GenMap {
     std::vector<int> perm;
     uint64_t flip = 0;
+    WState flipw;          // the same flips on up to 256 sites
     int order = 1, q = 0;
 };
 
@@ -181,6 +216,19 @@ struct CTerm {
     cd A;
 };
 
+// The same on states of up to 256 sites (wide bases)
+struct WTerm {
+    WState flip, cmask, cval, smask;
+    cd A;
+};
+struct WTermDev {
+    uint64_t cmask[4], cval[4], smask[4];
+    double are, aim;
+};
+struct WGroupDev {     // off-diagonal wide terms sharing one flip mask
+    uint64_t flip[4];
+    int t0, t1;
+};
 struct TermDev {
     uint64_t cmask, cval, smask;
     double are, aim;
@@ -339,6 +387,15 @@ struct BasisHost {
     std::vector<GenMap> gmaps;
     DevBuf<uint32_t> gtab, gc;
     DevBuf<double> gchi;
+    // wide bases (general maps on more than 32 sites, up to 256): the orbit
+    // minima as four words each, ascending; the group as a site table per
+    // element (byte e * L + i = target of site i) and its xor constants
+    // (four words per element); characters in gchi
+    int wide = 0;
+    DevBuf<uint64_t> wreps;
+    DevBuf<uint8_t> wperm;
+    DevBuf<uint64_t> wc;
+    std::vector<WState> wlabels;   // reference labels (host, on demand)
 };
 
 struct OpHost {
@@ -372,6 +429,12 @@ struct OpHost {
     // and for the passes of its tiled form, per (kA, kB)
     mutable std::mutex pass_mutex;
     mutable std::vector<std::unique_ptr<SplitPassHops>> pass_hops;
+    // wide bases: the term lists (plain products apply the adjoint
+    // couplings, as above), grouped by flip mask
+    std::vector<WTerm> wdiag, woff, wdiag_adj, woff_adj;
+    std::vector<WGroupDev> wgroups, wgroups_adj;
+    DevBuf<WTermDev> d_wdiag, d_woff, d_wdiag_adj, d_woff_adj;
+    DevBuf<WGroupDev> d_wgroups, d_wgroups_adj;
     // user diagonal
     std::vector<cd> user_diag;
     bool has_user_diag = false;
@@ -428,12 +491,24 @@ uint64_t group_apply(const GroupHost &g, int e, uint64_t u);
 cd group_char(const GroupHost &g, int e);
 void build_basis_device(BasisHost &b);
 void compute_reference_labels(BasisHost &b);
+// wide bases (qs_wide.hip, qs_wterms.cpp, qs_labels.cpp)
+void build_group_wide(BasisHost &b);            // the group of b.gmaps on up to 256 sites
+void build_wide_basis(BasisHost &b);            // group from b.gmaps, orbit minima on the device
+void wide_enumerate(BasisHost &b);              // device enumeration of the orbit minima
+void compute_wide_labels(BasisHost &b);         // reference labels and normalizations
+WState wide_group_apply(const GroupHost &g, int e, const WState &u);
+void wide_local(char op, int bit, int pauli, bool &alive, int &out, cd &amp);
+void launch_matvec_wide(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
+                        double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial);
 
 struct Coupling {
     std::string op;
     std::vector<int> sites;   // 1-based
     cd coef;
 };
+// wide-basis term lists of an operator (qs_wterms.cpp); check_symm: reject a
+// list the group does not leave invariant
+void setup_wide_terms(OpHost &o, const std::vector<Coupling> &cs, bool check_symm);
 std::vector<CTerm> compile_couplings(const std::vector<Coupling> &cs, int L, int pauli);
 std::vector<CTerm> consolidate_cterms(const std::vector<CTerm> &t);
 std::vector<CTerm> symmetrize_cterms(const std::vector<CTerm> &t, const BasisHost &b);

@@ -141,15 +141,18 @@ uint64_t stage_apply(const StageT &st, uint64_t u, int L, int a, uint64_t odd, u
     return u;
 }
 
-void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint64_t s0, uint64_t odd, uint64_t even,
-                    uint64_t &label, double &rnorm) {
-    const GroupHost &g = b.group;
-    std::vector<uint64_t> O;
-    for (int e = 0; e < g.order(); e++) O.push_back(group_apply(g, e, s0));
+// One column's label and normalization: the orbit O of s0 under the group
+// (gapply(e, s)), each stage's map sapply(stage, s).  S: uint64_t, or WState
+// for states of up to 256 sites.
+template <typename S, typename GA, typename SA>
+void emulate_column(int order, const std::vector<StageT> &stages, const S &s0, GA gapply, SA sapply, S &label,
+                    double &rnorm) {
+    std::vector<S> O;
+    for (int e = 0; e < order; e++) O.push_back(gapply(e, s0));
     std::sort(O.begin(), O.end());
     O.erase(std::unique(O.begin(), O.end()), O.end());
     const int R = (int)O.size();
-    auto idx = [&](uint64_t u) { return (int)(std::lower_bound(O.begin(), O.end(), u) - O.begin()); };
+    auto idx = [&](const S &u) { return (int)(std::lower_bound(O.begin(), O.end(), u) - O.begin()); };
     std::vector<jc> val(R);
     std::vector<int> col(R, -1);
     const int DEAD = -2;
@@ -163,7 +166,7 @@ void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint6
             cyc.assign(1, i);
             int cur = i;
             for (;;) {
-                int nx = idx(stage_apply(st, O[cur], b.L, b.a, odd, even));
+                int nx = idx(sapply(st, O[cur]));
                 if (nx == i) break;
                 cyc.push_back(nx);
                 cur = nx;
@@ -198,7 +201,7 @@ void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint6
             bool first = true;
             for (int r = 0; r < R; r++) {
                 if (col[r] != from) continue;
-                jc prod = jmul(jconj(val[idx(stage_apply(st, O[r], b.L, b.a, odd, even))]), val[r]);
+                jc prod = jmul(jconj(val[idx(sapply(st, O[r]))]), val[r]);
                 acc = first ? prod : jadd(acc, prod);
                 first = false;
             }
@@ -216,7 +219,7 @@ void emulate_column(const BasisHost &b, const std::vector<StageT> &stages, uint6
                 cyc.push_back(cur);
                 seen[cur] = 1;
                 ph.push_back(overlap(cur));
-                const int nx = col[idx(stage_apply(st, O[cur], b.L, b.a, odd, even))];
+                const int nx = col[idx(sapply(st, O[cur]))];
                 if (nx == i) break;
                 if (nx < 0 || seen[nx]) {
                     dead = true;   // no column cycle through i (not reached by a monomial stage)
@@ -299,9 +302,59 @@ void compute_reference_labels(BasisHost &b) {
     std::vector<std::thread> pool;
     for (unsigned w = 0; w < nt; w++)
         pool.emplace_back([&, w] {
-            for (int64_t i = w; i < n; i += nt) emulate_column(b, stages, reps[i], odd, even, b.labels[i], b.ref_norm[i]);
+            const GroupHost &g = b.group;
+            auto ga = [&](int e, uint64_t u) { return group_apply(g, e, u); };
+            auto sa = [&](const StageT &st, uint64_t u) { return stage_apply(st, u, b.L, b.a, odd, even); };
+            for (int64_t i = w; i < n; i += nt)
+                emulate_column(g.order(), stages, reps[i], ga, sa, b.labels[i], b.ref_norm[i]);
         });
     for (auto &t : pool) t.join();
+    b.labels_ready = true;
+}
+
+// Labels and normalizations of a wide basis: the same stage sequence on
+// states of up to 256 sites (general maps only).
+void compute_wide_labels(BasisHost &b) {
+    const int64_t n = b.Ns;
+    std::vector<uint64_t> words((size_t)4 * n);
+    QS_HIP(hipMemcpy(words.data(), b.wreps.get(), sizeof(uint64_t) * words.size(), hipMemcpyDeviceToHost));
+    std::vector<StageT> stages;
+    for (const GenMap &m : b.gmaps) {
+        const int q = ((m.q % m.order) + m.order) % m.order;
+        const double ang = (6.283185307179586 * (double)q) / (double)m.order;
+        stages.push_back(StageT{6, jc{std::cos(ang), std::sin(ang)}, &m});
+    }
+    const int L = b.L;
+    auto sa = [L](const StageT &st, const WState &u) {
+        // basis.jl:325-338: the (inverted) bit of each source site moves to its target
+        const WState v = u ^ st.map->flipw;
+        WState r;
+        for (int i = 0; i < L; i++)
+            if (v.bit(i)) r.set(st.map->perm[(size_t)i]);
+        return r;
+    };
+    const GroupHost &g = b.group;
+    auto ga = [&g](int e, const WState &u) { return wide_group_apply(g, e, u); };
+    b.wlabels.assign((size_t)n, WState());
+    b.ref_norm.assign((size_t)n, 0.0);
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; w++)
+        pool.emplace_back([&, w] {
+            for (int64_t i = w; i < n; i += nt) {
+                WState s0;
+                for (int k = 0; k < 4; k++) s0.w[k] = words[(size_t)4 * i + k];
+                if (stages.empty()) {
+                    b.wlabels[(size_t)i] = s0;
+                    b.ref_norm[(size_t)i] = 1.0;
+                    continue;
+                }
+                emulate_column(g.order(), stages, s0, ga, sa, b.wlabels[(size_t)i], b.ref_norm[(size_t)i]);
+            }
+        });
+    for (auto &t : pool) t.join();
+    b.labels.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) b.labels[(size_t)i] = b.wlabels[(size_t)i].w[0];
     b.labels_ready = true;
 }
 

@@ -437,6 +437,107 @@ __global__ __launch_bounds__(kStBlock) void k_matvec_stored(MVArgs A, const Grou
     if (A.dot_partial) write_dot_partial<kStBlock>(A, s_red, dot_r, dot_i);
 }
 
+// ----------------------------------------------------------- wide bases --
+// Bases of up to 256 sites (states as four words).  LPR = 8 lanes per row,
+// lane `sub` taking the flip groups g = sub (mod 8), as in the stored
+// kernel: per live group, the flipped state's orbit minimum under the group
+// (the first element reaching it, with its character), its column by binary
+// search over the ascending orbit minima, and the weight m chi sqrt(D_r / D_c).
+struct WideArgs {
+    const uint64_t *reps;
+    WGroup g;
+    const WGroupDev *grp;
+    const WTermDev *off, *diag;
+};
+
+template <bool CC>
+__device__ __forceinline__ void wterm_amp(const WTermDev &t, const W4 &s, double &re, double &im) {
+    int par = 0;
+    for (int k = 0; k < 4; k++) {
+        if ((s.w[k] & t.cmask[k]) != t.cval[k]) return;
+        par ^= __builtin_popcountll(s.w[k] & t.smask[k]) & 1;
+    }
+    const double sg = par ? -1.0 : 1.0;
+    re += sg * t.are;
+    if (CC) im += sg * t.aim;
+}
+
+constexpr int kWideLPR = 8;
+
+template <typename V, bool CC, bool TRACK>
+__global__ __launch_bounds__(kStBlock) void k_matvec_wide(MVArgs A, WideArgs W, const V *__restrict__ x,
+                                                          V *__restrict__ y) {
+    using T = VTraits<V>;
+    __shared__ double s_red[2 * kStBlock];
+    constexpr int kRows = kStBlock / kWideLPR;
+    const int sub = threadIdx.x % kWideLPR;
+    double dot_r = 0, dot_i = 0;
+    for (int64_t base = (int64_t)blockIdx.x * kRows; base < A.N; base += (int64_t)gridDim.x * kRows) {
+        const int64_t row = base + threadIdx.x / kWideLPR;
+        const bool valid = row < A.N;
+        double ar = 0, ai = 0, xr = 0, xi = 0;
+        if (valid) {
+            const W4 s = w4_at(W.reps, row);
+            const double own_scale = 1.0 / A.inv_nrm[row];
+            if (sub == 0) {
+                T::load(x, row, xr, xi);
+                if (A.conj_x) xi = -xi;
+                double dr = 0, di = 0;
+                for (int k = 0; k < A.ndiag; k++) wterm_amp<CC>(W.diag[k], s, dr, di);
+                di = -di;
+                if (A.user_diag) {
+                    dr += A.user_diag[2 * row];
+                    di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
+                }
+                ar = dr * xr - di * xi;
+                ai = dr * xi + di * xr;
+            }
+            for (int gi = sub; gi < A.ngroups; gi += kWideLPR) {
+                const WGroupDev gr = W.grp[gi];
+                double mr = 0, mi = 0;
+                for (int k = gr.t0; k < gr.t1; k++) wterm_amp<CC>(W.off[k], s, mr, mi);
+                if (mr == 0.0 && mi == 0.0) continue;
+                mi = -mi;
+                W4 u;
+                for (int k = 0; k < 4; k++) u.w[k] = s.w[k] ^ gr.flip[k];
+                W4 best = u;
+                int be = 0;
+                for (int e = 1; e < W.g.n; e++) {
+                    const W4 w = wg_apply(W.g, e, u);
+                    if (w4_less(w, best)) {
+                        best = w;
+                        be = e;
+                    }
+                }
+                const int64_t col = w4_find(W.reps, A.N, best);
+                if (col < 0) continue;
+                double cr = 1.0, ci = 0.0;
+                if (TRACK) {
+                    cr = W.g.chi[2 * be];
+                    ci = W.g.chi[2 * be + 1];
+                }
+                const double fac = A.inv_nrm[col] * own_scale;
+                cr *= fac;
+                ci *= fac;
+                const double wr = mr * cr - mi * ci;
+                const double wi = mr * ci + mi * cr;
+                double vr, vi;
+                T::load(x, col, vr, vi);
+                if (A.conj_x) vi = -vi;
+                ar += wr * vr - wi * vi;
+                ai += wr * vi + wi * vr;
+            }
+        }
+#pragma unroll
+        for (int o = kWideLPR / 2; o >= 1; o >>= 1) {
+            ar += __shfl_xor(ar, o);
+            ai += __shfl_xor(ai, o);
+        }
+        if (valid && sub == 0) finish_row<V>(A, y, row, ar, ai, xr, xi, dot_r, dot_i);
+    }
+    if (A.dot_partial) write_dot_partial<kStBlock>(A, s_red, dot_r, dot_i);
+}
+
 // ------------------------------------------------------------ hop kernel --
 // Every flip group is a two-bit hop whose amplitude is a function of its two
 // bits (HopDev), L <= 32 and all indices fit 31 bits.  One lane per row and a
@@ -2598,6 +2699,10 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
                    double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial,
                    void *xs_scratch) {
     const BasisHost &b = *op.basis;
+    if (b.wide) {
+        launch_matvec_wide(op, dtype, x, y, are, aim, bre, bim, adjoint, conj_x, conj_y, s, dot_partial);
+        return;
+    }
     MVArgs a = base_args(op, adjoint, are, aim, bre, bim, conj_x, conj_y, dot_partial);
     const bool track = basis_tracks(b);
     const bool cc = op.is_complex != 0;
@@ -2917,6 +3022,51 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
             QS_HIP(hipGetLastError());
         }
     }
+}
+
+void launch_matvec_wide(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,
+                        double bim, bool adjoint, bool conj_x, bool conj_y, hipStream_t s, double *dot_partial) {
+    const BasisHost &b = *op.basis;
+    MVArgs a = base_args(op, adjoint, are, aim, bre, bim, conj_x, conj_y, dot_partial);
+    const std::vector<WTerm> &dg = adjoint ? op.wdiag_adj : op.wdiag;
+    const std::vector<WTerm> &of = adjoint ? op.woff_adj : op.woff;
+    const std::vector<WGroupDev> &gr = adjoint ? op.wgroups_adj : op.wgroups;
+    a.ndiag = (int)dg.size();
+    a.noff = (int)of.size();
+    a.ngroups = (int)gr.size();
+    WideArgs w;
+    w.reps = b.wreps.get();
+    w.g = WGroup{b.group.order(), b.L, b.wperm.get(), b.wc.get(), b.gchi.get()};
+    w.grp = adjoint ? op.d_wgroups_adj.get() : op.d_wgroups.get();
+    w.off = adjoint ? op.d_woff_adj.get() : op.d_woff.get();
+    w.diag = adjoint ? op.d_wdiag_adj.get() : op.d_wdiag.get();
+    const bool track = basis_tracks(b);
+    const bool cc = op.is_complex != 0;
+    if (cc && !dtype_complex(dtype)) fail(QS_EARG, "a complex operator needs a complex vector dtype");
+    // one block per 32 rows: the grid of the dot partials (matvec_grid)
+    const int grid = matvec_grid(b.Ns);
+    auto go = [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        if (cc) {
+            if (track) hipLaunchKernelGGL((k_matvec_wide<V, true, true>), dim3(grid), dim3(kStBlock), 0, s, a, w,
+                                          (const V *)x, (V *)y);
+            else hipLaunchKernelGGL((k_matvec_wide<V, true, false>), dim3(grid), dim3(kStBlock), 0, s, a, w,
+                                    (const V *)x, (V *)y);
+        } else {
+            if (track) hipLaunchKernelGGL((k_matvec_wide<V, false, true>), dim3(grid), dim3(kStBlock), 0, s, a, w,
+                                          (const V *)x, (V *)y);
+            else hipLaunchKernelGGL((k_matvec_wide<V, false, false>), dim3(grid), dim3(kStBlock), 0, s, a, w,
+                                    (const V *)x, (V *)y);
+        }
+    };
+    switch (dtype) {
+    case QS_F64: go((double *)nullptr); break;
+    case QS_F32: go((float *)nullptr); break;
+    case QS_C128: go((cplx_d *)nullptr); break;
+    case QS_C64: go((cplx_f *)nullptr); break;
+    default: fail(QS_EARG, "unknown dtype");
+    }
+    QS_HIP(hipGetLastError());
 }
 
 }  // namespace qs

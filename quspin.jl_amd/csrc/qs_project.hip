@@ -181,6 +181,7 @@ bool tracks(const GroupHost &g) {
 }  // namespace
 
 int64_t parent_length(const BasisHost &b, int kind) {
+    if (b.wide) fail(QS_EARG, "projection to and from the parent is not supported for a wide basis");
     if (kind == 1) {
         if (b.L > 40) fail(QS_EARG, "a state-indexed parent vector needs L <= 40");
         return (int64_t)1 << b.L;

@@ -15,6 +15,7 @@ QS_CHECK_SYMM, QS_CHECK_HERM, QS_CHECK_PCON, QS_FORCE_COMPLEX = 1, 2, 4, 8
 VARIANT_HOP = 16   # QS_VARIANT_HOP: the two-bit hop matvec kernel
 VARIANT_SPLIT = 32   # QS_VARIANT_SPLIT: the split (high word / low word) hop kernel
 VARIANT_SELL = 48    # QS_VARIANT_SELL: plain products from the assembled sector matrix
+QS_BASIS_EXPLICIT = 1   # qs_basis_create_general: the wide (four-word, stored) form at any L
 
 EXPORTS = (
     "qs_last_error", "qs_device_info", "qs_basis_create", "qs_basis_destroy", "qs_basis_size",
@@ -25,6 +26,7 @@ EXPORTS = (
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
     "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr", "qs_op_set_graphs",
     "qs_op_matvec_noalloc", "qs_evolve_dynamic", "qs_block_gram", "qs_basis_create_general",
+    "qs_basis_states_wide",
 )
 NOALLOC_SCRATCH_BYTES = 32 << 20   # QS_NOALLOC_SCRATCH_BYTES
 
@@ -53,6 +55,7 @@ def lib():
     L.qs_basis_destroy.argtypes = [vp]
     L.qs_basis_size.argtypes = [vp, vp]
     L.qs_basis_states.argtypes = [vp, vp]
+    L.qs_basis_states_wide.argtypes = [vp, i32, vp]
     L.qs_basis_reps.argtypes = [vp, vp]
     L.qs_basis_norms.argtypes = [vp, vp, vp]
     L.qs_basis_is_complex.argtypes = [vp, vp]

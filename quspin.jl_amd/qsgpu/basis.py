@@ -171,11 +171,16 @@ class SpinBasisGeneral(SpinBasis1D):
     zero-based labels (any entry zero or negative; ``-v`` inverts the site and
     sends it to site ``v``) or Julia's one-based labels (negative = inverted),
     the eigenvalue ``cis(2 pi q / order)`` with the map's order.  Stages run in
-    ``block_order`` (default: the keyword order).  L <= 32 here; operators on
-    such a basis take two-bit hop couplings.
+    ``block_order`` (default: the keyword order).  L <= 32: operators take
+    two-bit hop couplings (the parent-table kernels).  More than 32 sites with
+    blocks, or more than 63 sites (the reference's WideSpinBasis, basis.jl:
+    361-445), up to 256: a wide basis (needs nup; states of more than 64
+    sites are Python ints); operators take any monomial couplings.
+    ``_explicit=True`` builds the wide form at any L.
     """
 
-    def __init__(self, L, nup=None, Nup=None, m=None, S="1/2", pauli=True, block_order=None, **blocks):
+    def __init__(self, L, nup=None, Nup=None, m=None, S="1/2", pauli=True, block_order=None, _explicit=False,
+                 **blocks):
         if str(S) not in ("1/2", "0.5"):
             raise ArgumentError("only spin-1/2 chains are on the GPU path")
         if nup is not None and Nup is not None:
@@ -210,8 +215,10 @@ class SpinBasisGeneral(SpinBasis1D):
         mats = np.ascontiguousarray(np.asarray(maps or [[0]], dtype=np.int32))
         qs = np.ascontiguousarray(np.asarray(qn or [0], dtype=np.int32))
         h = ctypes.c_void_p()
+        flags = _lib.QS_BASIS_EXPLICIT if _explicit else 0
+        self.wide = bool(_explicit) or self.L > 63 or (self.L > 32 and len(maps) > 0)
         check(_lib.lib().qs_basis_create_general(self.L, _blk(self.nup), int(self.pauli), len(maps), _ptr(mats),
-                                                  _ptr(qs), 0, ctypes.byref(h)))
+                                                  _ptr(qs), flags, ctypes.byref(h)))
         self._h = h
         n = ctypes.c_int64()
         check(_lib.lib().qs_basis_size(self._h, ctypes.byref(n)))
@@ -220,6 +227,29 @@ class SpinBasisGeneral(SpinBasis1D):
         check(_lib.lib().qs_basis_is_complex(self._h, ctypes.byref(c)))
         self.is_complex = bool(c.value)
         self._states = None
+
+    def state_words(self, which=0):
+        """(Ns, 4) uint64 words of states(basis) (which=0) or of the orbit
+        minima (which=1); site i is bit i % 64 of word i // 64."""
+        out = np.empty((self.Ns, 4), dtype=np.uint64)
+        check(_lib.lib().qs_basis_states_wide(self._h, int(which), _ptr(out)))
+        return out
+
+    def _labels(self):
+        if self.L <= 64:
+            return SpinBasis1D._labels(self)
+        if self._states is None:
+            w = self.state_words(0)
+            self._states = np.array([int(r[0]) | int(r[1]) << 64 | int(r[2]) << 128 | int(r[3]) << 192 for r in w],
+                                    dtype=object)
+        return self._states
+
+    @property
+    def reps(self):
+        if self.L <= 64:
+            return SpinBasis1D.reps.fget(self)
+        w = self.state_words(1)
+        return np.array([int(r[0]) | int(r[1]) << 64 | int(r[2]) << 128 | int(r[3]) << 192 for r in w], dtype=object)
 
     @property
     def blocks(self):
