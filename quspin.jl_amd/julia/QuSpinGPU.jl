@@ -95,7 +95,10 @@ The GPU sector of `SpinBasisGeneral(L; nup, pauli, block_order, blocks...)`
 (basis.jl:530-620, spin 1/2): each block `name=(map, quantum_number)` with the
 map in QuSpin's zero-based or Julia's one-based site labels, stages in
 `block_order` (default: keyword order).  The reference basis keeps only the
-quantum numbers, so the maps are given again here.  L <= 32.
+quantum numbers, so the maps are given again here.  Up to 256 sites: beyond
+32 sites with blocks, or beyond 63 (the reference's WideSpinBasis), a wide
+basis that needs `nup`; its states beyond 64 sites come back as UInt256-sized
+`BigInt`s from `states`.
 """
 function GPUBasisGeneral(L::Integer; nup=nothing, pauli::Bool=true, block_order=nothing, blocks...)
     names = block_order === nothing ? Symbol[Symbol(n) for (n, _) in blocks] : Symbol.(collect(block_order))
@@ -124,6 +127,12 @@ function GPUBasisGeneral(L::Integer; nup=nothing, pauli::Bool=true, block_order=
 end
 
 function states(b::GPUBasis)
+    if b.L > 64
+        # four 64-bit words per state (site i = bit i % 64 of word i / 64)
+        w = Vector{UInt64}(undef, 4 * b.Ns)
+        check(ccall((:qs_basis_states_wide, libqsgpu), Cint, (Ptr{Cvoid}, Cint, Ptr{UInt64}), b.handle, 0, w))
+        return [sum(BigInt(w[4 * (i - 1) + k + 1]) << (64 * k) for k in 0:3) for i in 1:b.Ns]
+    end
     out = Vector{UInt64}(undef, b.Ns)
     check(ccall((:qs_basis_states, libqsgpu), Cint, (Ptr{Cvoid}, Ptr{UInt64}), b.handle, out))
     return out
