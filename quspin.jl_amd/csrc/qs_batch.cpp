@@ -293,9 +293,17 @@ struct BFact {
 // queued with their scalars in device memory (FactScalars layout per
 // request); one copy at the end feeds KrylovProj::replay, which applies the
 // reference's bookkeeping and breakdown test in step order.
+// Krylov bases of a lockstep call in one device allocation reused by every
+// round: slot i holds request i's n x (steps + 1) columns (stream order keeps
+// a round's reads before the next round's writes).
+struct Arena {
+    std::shared_ptr<SBuf<char>> buf;
+    size_t per = 0, slots = 0;
+};
+
 void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dtype,
                 const std::vector<std::pair<int, const void *>> &req, int krylov_dim, BWork &wk, hipStream_t s,
-                std::vector<std::unique_ptr<BFact>> &F, std::vector<int> &nmv) {
+                std::vector<std::unique_ptr<BFact>> &F, std::vector<int> &nmv, const Arena *arena = nullptr) {
     const size_t esz = dtype_size(dtype);
     const size_t R = req.size();
     F.clear();
@@ -320,7 +328,15 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         max_steps = std::max(max_steps, f->P.steps);
         F.push_back(std::move(f));
     }
-    {
+    bool in_arena = arena && arena->buf && R <= arena->slots;
+    if (in_arena)
+        for (auto &f : F) in_arena &= (size_t)f->n * (f->P.steps + 1) * esz <= arena->per;
+    if (in_arena) {
+        for (size_t i = 0; i < R; i++) {
+            F[i]->store = arena->buf;
+            F[i]->V = arena->buf->get() + i * arena->per;
+        }
+    } else {
         size_t bytes = 0;
         for (auto &f : F) bytes += ((size_t)f->n * (f->P.steps + 1) * esz + 255) & ~(size_t)255;
         auto store = std::make_shared<SBuf<char>>(std::max<size_t>(bytes, 1), s);
@@ -419,7 +435,33 @@ struct Call {
     double tol;
     int depth;
     void *out;
-    std::shared_ptr<SBuf<char>> vbuf, obuf;
+    std::shared_ptr<char> vbuf, obuf;
+};
+
+// Split midpoints of one call of expm_times_batch: fixed-size device blocks
+// carved from a few growing chunks and recycled through a free list, so the
+// thousands of halvings cost no allocation each (every use is on the one
+// stream, whose order keeps a recycled block's earlier reads first).
+class MidPool {
+public:
+    MidPool(size_t bytes, hipStream_t s) : bytes_((bytes + 255) & ~(size_t)255), s_(s) {}
+    std::shared_ptr<char> get() {
+        if (free_.empty()) {
+            const size_t k = std::max<size_t>(16, total_);
+            chunks_.push_back(std::make_unique<SBuf<char>>(k * bytes_, s_));
+            for (size_t i = 0; i < k; i++) free_.push_back(chunks_.back()->get() + i * bytes_);
+            total_ += k;
+        }
+        char *p = free_.back();
+        free_.pop_back();
+        return std::shared_ptr<char>(p, [this](char *q) { free_.push_back(q); });
+    }
+private:
+    size_t bytes_;
+    hipStream_t s_;
+    size_t total_ = 0;
+    std::vector<std::unique_ptr<SBuf<char>>> chunks_;
+    std::vector<char *> free_;
 };
 struct Chain {              // the calls of one (task, time point), run in order
     int task;
@@ -461,9 +503,21 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
     // one factorization per task, all in lockstep
     std::vector<std::pair<int, const void *>> req;
     for (size_t t : live) req.push_back({tasks[t].op, tasks[t].v});
+    int64_t nmax = 1;
+    for (auto *op : ops) nmax = std::max<int64_t>(nmax, op->basis->Ns);
+    Arena arena;
+    arena.per = ((size_t)nmax * (krylov_dim + 1) * esz + 255) & ~(size_t)255;
+    auto reserve = [&](size_t slots) {
+        if (slots <= arena.slots) return;
+        arena.buf.reset();
+        arena.buf = std::make_shared<SBuf<char>>(slots * arena.per, s);
+        arena.slots = slots;
+    };
+    reserve(req.size());
+    MidPool mids((size_t)nmax * esz, s);   // before the chains that hold its blocks
     std::vector<std::unique_ptr<BFact>> F;
     std::vector<int> nmv(req.size(), 0);
-    bfactorize(*bm, ops, dtype, req, krylov_dim, wk, s, F, nmv);
+    bfactorize(*bm, ops, dtype, req, krylov_dim, wk, s, F, nmv, &arena);
     for (size_t i = 0; i < live.size(); i++) nmatvec[live[i]] += nmv[i];
     // every time point from its task's factorization; failures become chains
     std::vector<Chain> chains;
@@ -522,12 +576,11 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
     F.clear();
     // halving recursion, the next call of as many chains as device memory
     // holds Krylov bases for, in lockstep
-    int64_t nmax = 1;
-    for (auto *op : ops) nmax = std::max<int64_t>(nmax, op->basis->Ns);
     size_t fr = 0, tot = 0;
     QS_HIP(hipMemGetInfo(&fr, &tot));
     const size_t per_chain = (size_t)(krylov_dim + 3) * nmax * esz;
     const size_t cap = std::max<size_t>(1, std::min<size_t>(4096, fr / 3 / per_chain));
+    if (!chains.empty()) reserve(std::min(cap, chains.size()));
     while (true) {
         std::vector<size_t> run;
         for (size_t c = 0; c < chains.size() && run.size() < cap; c++)
@@ -552,7 +605,7 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
         std::vector<std::pair<int, const void *>> rq;
         for (size_t c : fac) rq.push_back({tasks[chains[c].task].op, chains[c].stack.back().v});
         std::vector<int> m2(rq.size(), 0);
-        bfactorize(*bm, ops, dtype, rq, krylov_dim, wk, s, F, m2);
+        bfactorize(*bm, ops, dtype, rq, krylov_dim, wk, s, F, m2, &arena);
         std::vector<CombReq> pq;
         std::vector<double> est(fac.size());
         std::vector<std::vector<cd>> coefs(fac.size());
@@ -577,10 +630,10 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
             ch.stack.pop_back();
             if (est[r] > cl.tol * std::max(1.0, std::sqrt(onr[r]))) {
                 if (cl.depth >= 14) fail(QS_ENOCONV, "Krylov exponential action did not converge; increase krylov_dim");
-                auto mid = std::make_shared<SBuf<char>>((size_t)nrows(ch.task) * esz, s);
+                std::shared_ptr<char> mid = mids.get();
                 // second half (mid -> out) runs after the first (v -> mid): pushed first
-                ch.stack.push_back(Call{mid->get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid, cl.obuf});
-                ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid->get(), cl.vbuf, mid});
+                ch.stack.push_back(Call{mid.get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid, cl.obuf});
+                ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid.get(), cl.vbuf, mid});
             }
         }
     }

@@ -1037,6 +1037,48 @@ __global__ __launch_bounds__(kSellBlock) void k_sell_fill(MVArgs A, const HopDev
     }
 }
 
+// One assembled row: sum over its w stored entries (column c, weight e) of
+// e x[c], entries at base + 64 k.  Groups of kSellU entries load their
+// columns and weights first and then gather x, so a lane keeps kSellU
+// gathers in flight; the sum runs in entry order either way.
+constexpr int kSellU = 4;
+template <typename V>
+__device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, const double *__restrict__ wre,
+                                         const double *__restrict__ wim, uint64_t base, uint32_t w,
+                                         const V *__restrict__ x, int conj_x, double &ar, double &ai) {
+    using T = VTraits<V>;
+    uint32_t k = 0;
+    for (; k + kSellU <= w; k += kSellU) {
+        uint32_t c[kSellU];
+        double er[kSellU], ei[kSellU], vr[kSellU], vi[kSellU];
+#pragma unroll
+        for (int u = 0; u < kSellU; u++) {
+            const uint64_t at = base + 64 * (uint64_t)(k + u);
+            c[u] = cols[at];
+            er[u] = wre[at];
+            ei[u] = wim ? wim[at] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSellU; u++) T::load(x, c[u], vr[u], vi[u]);
+#pragma unroll
+        for (int u = 0; u < kSellU; u++) {
+            if (conj_x) vi[u] = -vi[u];
+            ar += er[u] * vr[u] - ei[u] * vi[u];
+            ai += er[u] * vi[u] + ei[u] * vr[u];
+        }
+    }
+    for (; k < w; k++) {
+        const uint64_t at = base + 64 * (uint64_t)k;
+        const uint32_t c = cols[at];
+        const double er = wre[at], ei = wim ? wim[at] : 0.0;
+        double vr, vi;
+        T::load(x, c, vr, vi);
+        if (conj_x) vi = -vi;
+        ar += er * vr - ei * vi;
+        ai += er * vi + ei * vr;
+    }
+}
+
 // y = beta*y + alpha*M x from the assembled rows (WC: complex weights)
 template <typename V, bool WC>
 __global__ __launch_bounds__(kSellBlock) void k_sell_spmv(MVArgs A, const uint64_t *__restrict__ sptr,
@@ -1045,23 +1087,11 @@ __global__ __launch_bounds__(kSellBlock) void k_sell_spmv(MVArgs A, const uint64
                                                           const double *__restrict__ wre,
                                                           const double *__restrict__ wim,
                                                           const V *__restrict__ x, V *__restrict__ y) {
-    using T = VTraits<V>;
     const int64_t row = (int64_t)blockIdx.x * kSellBlock + threadIdx.x;
     if (row >= A.N) return;
     const uint64_t q = (uint64_t)row >> 6;
-    const uint64_t base = sptr[q] + (row & 63);
-    const uint32_t w = width[q];
     double ar = 0, ai = 0;
-    for (uint32_t k = 0; k < w; k++) {
-        const uint64_t at = base + 64 * (uint64_t)k;
-        const uint32_t c = cols[at];
-        const double er = wre[at], ei = WC ? wim[at] : 0.0;
-        double vr, vi;
-        T::load(x, c, vr, vi);
-        if (A.conj_x) vi = -vi;
-        ar += er * vr - ei * vi;
-        ai += er * vi + ei * vr;
-    }
+    sell_row<V>(cols, wre, WC ? wim : nullptr, sptr[q] + (row & 63), width[q], x, A.conj_x, ar, ai);
     double dot_r = 0, dot_i = 0;
     finish_row<V>(A, y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
 }
@@ -1086,24 +1116,12 @@ struct SellJobs {
 // block b of job q: its kSellBlock rows
 template <typename V>
 __device__ __forceinline__ void sell_job_block(const SellJob &q, int b) {
-    using T = VTraits<V>;
     const MVArgs &A = *q.A;
     const int64_t row = (int64_t)b * kSellBlock + threadIdx.x;
     if (row >= A.N) return;
-    const uint64_t base = q.sptr[(uint64_t)row >> 6] + (row & 63);
-    const uint32_t w = q.width[(uint64_t)row >> 6];
-    const V *x = (const V *)q.x;
     double ar = 0, ai = 0;
-    for (uint32_t k = 0; k < w; k++) {
-        const uint64_t at = base + 64 * (uint64_t)k;
-        const uint32_t c = q.cols[at];
-        const double er = q.wre[at], ei = q.wim ? q.wim[at] : 0.0;
-        double vr, vi;
-        T::load(x, c, vr, vi);
-        if (A.conj_x) vi = -vi;
-        ar += er * vr - ei * vi;
-        ai += er * vi + ei * vr;
-    }
+    sell_row<V>(q.cols, q.wre, q.wim, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
+                (const V *)q.x, A.conj_x, ar, ai);
     double dot_r = 0, dot_i = 0;
     finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
 }
@@ -2932,7 +2950,11 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
         for (size_t j = 0; j < jobs.size(); j++)
             if (bm.ops[jobs[j].op]->sell) asm_jobs.push_back(j);
         if (asm_jobs.size() > 32) {
-            // one launch through a device job table, each job on nbmax blocks
+            // one launch through a device job table, each job on nbmax blocks;
+            // the jobs of one operator side by side, so its stored rows stay
+            // in the L2 / MALL while they run
+            std::stable_sort(asm_jobs.begin(), asm_jobs.end(),
+                             [&](size_t a, size_t b) { return jobs[a].op < jobs[b].op; });
             std::vector<SellJob> tab;
             int nbmax = 0;
             for (size_t j : asm_jobs) {

@@ -245,10 +245,11 @@ class BlockOps:
         return result
 
     def evolve(self, psi_0, t0, times, iterate=False, n_jobs=1, block_diag=False, stack_state=False,
-               imag_time=False, tol=1e-12, krylov_dim=30, dtype=np.complex128):
+               imag_time=False, tol=1e-12, krylov_dim=30, dtype=np.complex128, device=False):
         """evolve(::BlockOps, psi_0, t0, times; ...) -- tools.jl:2588-2646.  Returns Npar x n_t
         (or Npar for a scalar time).  n_jobs and block_diag are accepted for API parity: every
-        block always runs in the one batched engine."""
+        block always runs in the one batched engine.  device=True returns the states as a
+        device-resident torch tensor (Npar x n_t view) instead of copying them to the host."""
         if n_jobs <= 0:
             raise ArgumentError("n_jobs must be positive")
         scalar = isinstance(times, numbers.Number)
@@ -257,6 +258,11 @@ class BlockOps:
         scales = -offsets.astype(np.complex128) if imag_time else -1j * offsets
         entries = self._entries(psi_0, dtype)
         res = self._evolution(entries, scales, self._parent.Ns, dtype, tol, krylov_dim)
+        if device:
+            out = res.T
+            if iterate:
+                return (out[:, j] for j in range(out.shape[1]))
+            return out[:, 0] if scalar else out
         out = res.cpu().numpy().T
         if iterate:
             return (out[:, j].copy() for j in range(out.shape[1]))
