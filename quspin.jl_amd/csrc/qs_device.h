@@ -337,4 +337,47 @@ __device__ __forceinline__ int64_t w4_find(const uint64_t *st, int64_t n, const 
     return (lo < n && w4_eq(w4_at(st, lo), s)) ? lo : -1;
 }
 
+// ------------------------------------------------------- assembled rows --
+// One assembled row: sum over its w stored entries (column c, weight e) of
+// e x[c], entries at base + 64 k.  Groups of kSellU entries load their
+// columns and weights first and then gather x, so a lane keeps kSellU
+// gathers in flight; the sum runs in entry order either way.
+constexpr int kSellU = 4;
+template <typename V>
+__device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, const double *__restrict__ wre,
+                                         const double *__restrict__ wim, uint64_t base, uint32_t w,
+                                         const V *__restrict__ x, int conj_x, double &ar, double &ai) {
+    using T = VTraits<V>;
+    uint32_t k = 0;
+    for (; k + kSellU <= w; k += kSellU) {
+        uint32_t c[kSellU];
+        double er[kSellU], ei[kSellU], vr[kSellU], vi[kSellU];
+#pragma unroll
+        for (int u = 0; u < kSellU; u++) {
+            const uint64_t at = base + 64 * (uint64_t)(k + u);
+            c[u] = cols[at];
+            er[u] = wre[at];
+            ei[u] = wim ? wim[at] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSellU; u++) T::load(x, c[u], vr[u], vi[u]);
+#pragma unroll
+        for (int u = 0; u < kSellU; u++) {
+            if (conj_x) vi[u] = -vi[u];
+            ar += er[u] * vr[u] - ei[u] * vi[u];
+            ai += er[u] * vi[u] + ei[u] * vr[u];
+        }
+    }
+    for (; k < w; k++) {
+        const uint64_t at = base + 64 * (uint64_t)k;
+        const uint32_t c = cols[at];
+        const double er = wre[at], ei = wim ? wim[at] : 0.0;
+        double vr, vi;
+        T::load(x, c, vr, vi);
+        if (conj_x) vi = -vi;
+        ar += er * vr - ei * vi;
+        ai += er * vi + ei * vr;
+    }
+}
+
 }  // namespace qs

@@ -591,6 +591,21 @@ void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::ve
                      JobTable *table = nullptr);
 int ew_blocks(int64_t n);
 
+// Krylov steps of a small assembled sector in one single-workgroup launch
+// (qs_vec.hip): SMALL_DGKS runs steps j0 .. steps-1 of a thick-restart
+// Lanczos cycle (per-step scalars `per` apart, as eigsh_device lays them
+// out); SMALL_LANCZOS / SMALL_ARNOLDI a whole expm factorization from vin
+// (FactScalars layout).  krylov_small_ok: the operator's plain products read
+// stored rows, n <= kSmallMaxN and at most kSmallMaxCols basis columns.
+enum SmallMode { SMALL_DGKS = 0, SMALL_LANCZOS = 1, SMALL_ARNOLDI = 2 };
+constexpr int64_t kSmallMaxN = 8192;
+constexpr int kSmallMaxCols = 128;
+bool krylov_small_ok(const OpHost &op, int cols);
+void krylov_small(const OpHost &op, int dtype, int mode, int steps, int j0, void *V, const void *vin, double *dd,
+                  int per, hipStream_t s);
+// assemble an operator marked for assembly (QS_VARIANT_SELL) now, on stream s
+void ensure_assembled(const OpHost &op, hipStream_t s);
+
 // Batched matvec: y_j = M_j x_j for several operators in one launch (all
 // must take the hop kernel with one wave per row tile).
 struct MatvecJob {

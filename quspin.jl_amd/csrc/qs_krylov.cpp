@@ -214,15 +214,21 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     for (int iter = 0; iter < maxiter; iter++) {
         int j0 = kept;
         while (j0 < m) {
-            // queue steps j0 .. m-1 with no host round trip
-            for (int j = j0; j < m; j++) {
-                void *w = K.col(j + 1);
-                double *h1 = d.get() + per * j, *h2 = h1 + 2 * (m + 1), *n2 = h2 + 2 * (m + 1), *n2a = n2 + 2;
-                matvec(op, dtype, K.col(j), w);
-                // ARPACK's DGKS step: the second pass runs only if the first
-                // left |w'| <= 0.717 |w| (decided on the device)
-                gram_dgks_dev(dtype, K.V.get(), n, j + 1, w, wk, h1, h2, n2a, n2, s);
-                vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
+            // queue steps j0 .. m-1 with no host round trip (a small
+            // assembled sector: all of them in one single-workgroup launch)
+            ensure_assembled(op, s);
+            if (krylov_small_ok(op, m + 1)) {
+                krylov_small(op, dtype, SMALL_DGKS, m, j0, K.V.get(), nullptr, d.get(), (int)per, s);
+            } else {
+                for (int j = j0; j < m; j++) {
+                    void *w = K.col(j + 1);
+                    double *h1 = d.get() + per * j, *h2 = h1 + 2 * (m + 1), *n2 = h2 + 2 * (m + 1), *n2a = n2 + 2;
+                    matvec(op, dtype, K.col(j), w);
+                    // ARPACK's DGKS step: the second pass runs only if the first
+                    // left |w'| <= 0.717 |w| (decided on the device)
+                    gram_dgks_dev(dtype, K.V.get(), n, j + 1, w, wk, h1, h2, n2a, n2, s);
+                    vec_axpby_dev(dtype, n2, CO_INV_SQRT, w, cd(0, 0), w, n, s);
+                }
             }
             QS_HIP(hipMemcpyAsync(host.data() + per * j0, d.get() + per * j0, sizeof(double) * per * (m - j0),
                                   hipMemcpyDeviceToHost, s));
@@ -418,6 +424,12 @@ void queue_factorization(OpHost &op, int dtype, int64_t n, int steps, bool hermi
     const size_t esz = dtype_size(dtype);
     auto col = [&](int j) { return (void *)(V + (size_t)j * n * esz); };
     FactScalars L(steps);
+    ensure_assembled(op, s);
+    if (krylov_small_ok(op, steps + 1)) {
+        // a small assembled sector: the whole factorization in one launch
+        krylov_small(op, dtype, hermitian ? SMALL_LANCZOS : SMALL_ARNOLDI, steps, 0, V, vin, dd, 0, s);
+        return;
+    }
     vec_norm2_dev(dtype, vin, n, wk, dd + L.beta2(), s);
     vec_axpby_dev(dtype, dd + L.beta2(), CO_INV_SQRT, vin, cd(0, 0), col(0), n, s);
     for (int c = 0; c < steps; c++) {

@@ -1037,48 +1037,6 @@ __global__ __launch_bounds__(kSellBlock) void k_sell_fill(MVArgs A, const HopDev
     }
 }
 
-// One assembled row: sum over its w stored entries (column c, weight e) of
-// e x[c], entries at base + 64 k.  Groups of kSellU entries load their
-// columns and weights first and then gather x, so a lane keeps kSellU
-// gathers in flight; the sum runs in entry order either way.
-constexpr int kSellU = 4;
-template <typename V>
-__device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, const double *__restrict__ wre,
-                                         const double *__restrict__ wim, uint64_t base, uint32_t w,
-                                         const V *__restrict__ x, int conj_x, double &ar, double &ai) {
-    using T = VTraits<V>;
-    uint32_t k = 0;
-    for (; k + kSellU <= w; k += kSellU) {
-        uint32_t c[kSellU];
-        double er[kSellU], ei[kSellU], vr[kSellU], vi[kSellU];
-#pragma unroll
-        for (int u = 0; u < kSellU; u++) {
-            const uint64_t at = base + 64 * (uint64_t)(k + u);
-            c[u] = cols[at];
-            er[u] = wre[at];
-            ei[u] = wim ? wim[at] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kSellU; u++) T::load(x, c[u], vr[u], vi[u]);
-#pragma unroll
-        for (int u = 0; u < kSellU; u++) {
-            if (conj_x) vi[u] = -vi[u];
-            ar += er[u] * vr[u] - ei[u] * vi[u];
-            ai += er[u] * vi[u] + ei[u] * vr[u];
-        }
-    }
-    for (; k < w; k++) {
-        const uint64_t at = base + 64 * (uint64_t)k;
-        const uint32_t c = cols[at];
-        const double er = wre[at], ei = wim ? wim[at] : 0.0;
-        double vr, vi;
-        T::load(x, c, vr, vi);
-        if (conj_x) vi = -vi;
-        ar += er * vr - ei * vi;
-        ai += er * vi + ei * vr;
-    }
-}
-
 // y = beta*y + alpha*M x from the assembled rows (WC: complex weights)
 template <typename V, bool WC>
 __global__ __launch_bounds__(kSellBlock) void k_sell_spmv(MVArgs A, const uint64_t *__restrict__ sptr,
@@ -2710,6 +2668,18 @@ void sell_to_csr(const OpHost &op, std::vector<int64_t> &rowptr, std::vector<int
             i = j;
         }
         rowptr[(size_t)r + 1] = (int64_t)col.size();
+    }
+}
+
+void ensure_assembled(const OpHost &op, hipStream_t s) {
+    if (!op.want_sell || op.sell) return;
+    std::lock_guard<std::mutex> g(op.sell_mutex);
+    if (!op.sell) {
+        try {
+            sell_assemble(const_cast<OpHost &>(op), s);
+        } catch (const Error &) {
+            op.want_sell = false;   // couplings it cannot assemble: stay matrix-free
+        }
     }
 }
 

@@ -506,6 +506,265 @@ __global__ __launch_bounds__(kVB) void k_update_dgks(const V *__restrict__ Vb, i
     update_body<V>(Vb, n, m, raw, w, partial, blockIdx.x, gridDim.x, sh);
 }
 
+// ------------------------------------------ small sectors, one workgroup --
+// A whole run of Krylov steps of a small assembled sector in one launch of
+// one workgroup (kSmallT threads; thread t owns rows t, t + kSmallT, ...):
+// the same step sequence and device scalars as the per-operation launches
+// (DGKS cycle of the thick-restart Lanczos, three-term Lanczos or two-pass
+// Arnoldi of the expm factorization), with workgroup-wide sums in a fixed
+// order instead of per-block partials.  Phases meet at workgroup barriers;
+// every value a phase reads from global memory was written before the
+// barrier that opens it.
+constexpr int kSmallT = 512;
+constexpr int kSmallW = kSmallT / 64;
+constexpr int kSmallC = 8;    // columns per Gram pass
+
+struct SmallKrylov {
+    int mode;           // SMALL_DGKS, SMALL_LANCZOS, SMALL_ARNOLDI
+    int64_t n;
+    int steps, j0;      // DGKS: cycle length m and first step; else the factorization's steps
+    char *V;            // basis, column-major with ld n
+    const void *vin;    // factorization: starting vector
+    double *dd;         // device scalars (layouts of the per-operation forms)
+    int per;            // DGKS: scalars per step
+    const uint64_t *sptr;
+    const uint32_t *width, *cols;
+    const double *wre, *wim;   // wim null: real weights
+};
+
+// sum over the workgroup of every thread's (re, im): fixed order (wavefront
+// butterflies, then the wavefront sums in order); every thread gets it
+__device__ __forceinline__ void wg_sum2(double &re, double &im, double *red) {
+    re = wave_sum(re);
+    im = wave_sum(im);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = re;
+        red[kSmallW + w] = im;
+    }
+    __syncthreads();
+    re = 0;
+    im = 0;
+    for (int i = 0; i < kSmallW; i++) {
+        re += red[i];
+        im += red[kSmallW + i];
+    }
+}
+
+// h[c] = <V_c, w> for c < m (conj(V) . w), to LDS sh and global out
+template <typename V>
+__device__ void wg_gram(const V *Vb, int64_t n, int m, const V *w, double *sh, double *out, double *red) {
+    using T = VTraits<V>;
+    for (int c0 = 0; c0 < m; c0 += kSmallC) {
+        double ar[kSmallC], ai[kSmallC];
+#pragma unroll
+        for (int c = 0; c < kSmallC; c++) ar[c] = ai[c] = 0.0;
+        for (int64_t r = threadIdx.x; r < n; r += kSmallT) {
+            double wr, wi;
+            T::load(w, r, wr, wi);
+#pragma unroll
+            for (int c = 0; c < kSmallC; c++) {
+                if (c0 + c < m) {
+                    double vr, vi;
+                    T::load(Vb, (int64_t)(c0 + c) * n + r, vr, vi);
+                    ar[c] += vr * wr + vi * wi;
+                    ai[c] += vr * wi - vi * wr;
+                }
+            }
+        }
+        // every column's wavefront butterflies at once, then the wavefront
+        // sums of each column in order
+#pragma unroll
+        for (int c = 0; c < kSmallC; c++) {
+            ar[c] = wave_sum(ar[c]);
+            ai[c] = wave_sum(ai[c]);
+        }
+        const int wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int c = 0; c < kSmallC; c++) {
+                red[(2 * c) * kSmallW + wv] = ar[c];
+                red[(2 * c + 1) * kSmallW + wv] = ai[c];
+            }
+        }
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (t < 2 * kSmallC && c0 + t / 2 < m) {
+            double v = 0;
+            for (int i = 0; i < kSmallW; i++) v += red[t * kSmallW + i];
+            sh[2 * c0 + t] = v;
+            if (out) out[2 * c0 + t] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// w -= V h (h: m complex in LDS), then |w|^2 of the stored w
+template <typename V>
+__device__ double wg_update(const V *Vb, int64_t n, int m, const double *h, V *w, double *red) {
+    using T = VTraits<V>;
+    double nrm = 0;
+    for (int64_t r = threadIdx.x; r < n; r += kSmallT) {
+        double wr, wi;
+        T::load(w, r, wr, wi);
+        // columns in groups of kSmallC: the group's loads in flight together,
+        // then the updates in column order
+        for (int c0 = 0; c0 < m; c0 += kSmallC) {
+            double vr[kSmallC], vi[kSmallC];
+#pragma unroll
+            for (int u = 0; u < kSmallC; u++) {
+                vr[u] = vi[u] = 0.0;
+                if (c0 + u < m) T::load(Vb, (int64_t)(c0 + u) * n + r, vr[u], vi[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kSmallC; u++) {
+                if (c0 + u < m) {
+                    const double hr = h[2 * (c0 + u)], hi = h[2 * (c0 + u) + 1];
+                    wr -= vr[u] * hr - vi[u] * hi;
+                    wi -= vr[u] * hi + vi[u] * hr;
+                }
+            }
+        }
+        T::store(w, r, wr, wi);
+        T::load(w, r, wr, wi);
+        nrm += wr * wr + wi * wi;
+    }
+    double dummy = 0;
+    wg_sum2(nrm, dummy, red);
+    return nrm;
+}
+
+template <typename V>
+__device__ double wg_norm2(const V *x, int64_t n, double *red) {
+    using T = VTraits<V>;
+    double nrm = 0;
+    for (int64_t r = threadIdx.x; r < n; r += kSmallT) {
+        double a, b;
+        T::load(x, r, a, b);
+        nrm += a * a + b * b;
+    }
+    double dummy = 0;
+    wg_sum2(nrm, dummy, red);
+    return nrm;
+}
+
+// y = a x + b y (b_zero: y = a x), real a, b
+template <typename V>
+__device__ void wg_axpby(double a, const V *x, double b, bool b_zero, V *y, int64_t n) {
+    using T = VTraits<V>;
+    for (int64_t r = threadIdx.x; r < n; r += kSmallT) {
+        double xr, xi, yr = 0, yi = 0;
+        T::load(x, r, xr, xi);
+        if (!b_zero) T::load(y, r, yr, yi);
+        const double orr = a * xr - 0.0 * xi + (b_zero ? 0.0 : b * yr - 0.0 * yi);
+        const double oi = a * xi + 0.0 * xr + (b_zero ? 0.0 : b * yi + 0.0 * yr);
+        T::store(y, r, orr, oi);
+    }
+}
+
+template <typename V>
+__device__ void wg_matvec(const SmallKrylov &K, const V *x, V *y) {
+    using T = VTraits<V>;
+    for (int64_t r = threadIdx.x; r < K.n; r += kSmallT) {
+        double ar = 0, ai = 0;
+        sell_row<V>(K.cols, K.wre, K.wim, K.sptr[(uint64_t)r >> 6] + (r & 63), K.width[(uint64_t)r >> 6], x, 0, ar,
+                    ai);
+        T::store(y, r, ar, ai);
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kSmallT) void k_krylov_small(SmallKrylov K) {
+    __shared__ double red[2 * kSmallC * kSmallW];
+    __shared__ double sh1[2 * (kSmallMaxCols + 1)], sh2[2 * (kSmallMaxCols + 1)];
+    const int64_t n = K.n;
+    const size_t esz = sizeof(V);
+    auto col = [&](int j) { return (V *)(K.V + (size_t)j * n * esz); };
+    if (K.mode == SMALL_DGKS) {
+        const int mc = K.steps;
+        for (int j = K.j0; j < mc; j++) {
+            double *h1 = K.dd + (size_t)K.per * j, *h2 = h1 + 2 * (mc + 1), *n2 = h2 + 2 * (mc + 1), *n2a = n2 + 2;
+            V *w = col(j + 1);
+            const int m = j + 1;
+            wg_matvec<V>(K, col(j), w);
+            __syncthreads();
+            wg_gram<V>(col(0), n, m, w, sh1, h1, red);
+            const double nw1 = wg_update<V>(col(0), n, m, sh1, w, red);
+            __syncthreads();
+            wg_gram<V>(col(0), n, m, w, sh2, nullptr, red);
+            // ARPACK's DGKS test (dgks_skip): |w'|^2 > 0.717^2 (|w'|^2 + |h1|^2)
+            double hh = 0;
+            for (int c = 0; c < m; c++) hh += sh1[2 * c] * sh1[2 * c] + sh1[2 * c + 1] * sh1[2 * c + 1];
+            const double e2 = 0.717 * 0.717;
+            const bool skip = nw1 > e2 * (nw1 + hh);
+            if (threadIdx.x == 0) {
+                for (int c = 0; c < m; c++) {
+                    h2[2 * c] = skip ? 0.0 : sh2[2 * c];
+                    h2[2 * c + 1] = skip ? 0.0 : sh2[2 * c + 1];
+                }
+                n2a[0] = nw1;
+                n2a[1] = 0.0;
+            }
+            double nw = nw1;
+            if (!skip) {
+                nw = wg_update<V>(col(0), n, m, sh2, w, red);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                n2[0] = nw;
+                n2[1] = 0.0;
+            }
+            wg_axpby<V>(1.0 / sqrt(nw), w, 0.0, true, w, n);
+            __syncthreads();
+        }
+        return;
+    }
+    // the expm factorization (FactScalars layout: beta2 at 0, then per step
+    // h1, h2 (2 (steps + 1) each) and |w|^2)
+    const int steps = K.steps;
+    const size_t per = (size_t)4 * (steps + 1) + 2;
+    const double b2 = wg_norm2<V>((const V *)K.vin, n, red);
+    if (threadIdx.x == 0) {
+        K.dd[0] = b2;
+        K.dd[1] = 0.0;
+    }
+    wg_axpby<V>(1.0 / sqrt(b2), (const V *)K.vin, 0.0, true, col(0), n);
+    __syncthreads();
+    double nprev = 0;
+    for (int c = 0; c < steps; c++) {
+        double *h1 = K.dd + 2 + (size_t)c * per, *h2 = h1 + 2 * (steps + 1), *n2 = h2 + 2 * (steps + 1);
+        V *w = col(c + 1);
+        wg_matvec<V>(K, col(c), w);
+        __syncthreads();
+        double nw;
+        if (K.mode == SMALL_LANCZOS) {
+            if (c > 0) {
+                wg_axpby<V>(-sqrt(nprev), col(c - 1), 1.0, false, w, n);
+                __syncthreads();
+            }
+            wg_gram<V>(col(c), n, 1, w, sh1, h1, red);
+            nw = wg_update<V>(col(c), n, 1, sh1, w, red);
+        } else {
+            wg_gram<V>(col(0), n, c + 1, w, sh1, h1, red);
+            wg_update<V>(col(0), n, c + 1, sh1, w, red);
+            __syncthreads();
+            wg_gram<V>(col(0), n, c + 1, w, sh2, h2, red);
+            nw = wg_update<V>(col(0), n, c + 1, sh2, w, red);
+        }
+        if (threadIdx.x == 0) {
+            n2[0] = nw;
+            n2[1] = 0.0;
+        }
+        __syncthreads();
+        if (c + 1 < steps) {
+            wg_axpby<V>(1.0 / sqrt(nw), w, 0.0, true, w, n);
+            __syncthreads();
+        }
+        nprev = nw;
+    }
+}
+
 // ------------------------------------------------- block Gram on MFMA ----
 // G = V^H W for V (n x m) and W (n x p), column-major with ld n: partial sums
 // per row block on the matrix cores (v_mfma_f64_16x16x4f64).  One block owns
@@ -1082,6 +1341,37 @@ void vec_convert(int from, const void *x, int to, void *y, int64_t n, hipStream_
             using B = std::remove_pointer_t<decltype(tb)>;
             hipLaunchKernelGGL((k_convert<A, B>), dim3(ew_grid(n)), dim3(256), 0, s, (const A *)x, (B *)y, n);
         });
+    });
+    QS_HIP(hipGetLastError());
+}
+
+bool krylov_small_ok(const OpHost &op, int cols) {
+    const BasisHost &b = *op.basis;
+    return op.sell && !b.wide && b.Ns <= kSmallMaxN && cols <= kSmallMaxCols &&
+           (op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SELL);
+}
+
+void krylov_small(const OpHost &op, int dtype, int mode, int steps, int j0, void *V, const void *vin, double *dd,
+                  int per, hipStream_t s) {
+    const SellHost &t = *op.sell;
+    if (t.cplx && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
+    SmallKrylov K;
+    K.mode = mode;
+    K.n = op.basis->Ns;
+    K.steps = steps;
+    K.j0 = j0;
+    K.V = (char *)V;
+    K.vin = vin;
+    K.dd = dd;
+    K.per = per;
+    K.sptr = t.sptr.get();
+    K.width = t.width.get();
+    K.cols = t.cols.get();
+    K.wre = t.wre.get();
+    K.wim = t.cplx ? t.wim.get() : nullptr;
+    by_dtype(dtype, [&](auto *tag) {
+        using VT = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_krylov_small<VT>), dim3(1), dim3(kSmallT), 0, s, K);
     });
     QS_HIP(hipGetLastError());
 }
