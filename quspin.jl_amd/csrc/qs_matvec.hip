@@ -1167,6 +1167,23 @@ __device__ __forceinline__ void acc_nb(double wr, double wi, double vr, double v
     }
 }
 
+// Row-sum type of the split kernels: FP32 storage sums in FP32 (the FP32
+// stream's products; its dots, norms and Rayleigh-Ritz step stay FP64),
+// FP64 storage in FP64.
+template <typename V> struct SplitAcc { using type = double; };
+template <> struct SplitAcc<float> { using type = float; };
+template <> struct SplitAcc<cplx_f> { using type = float; };
+
+template <typename V, typename R>
+__device__ __forceinline__ void acc_r(R wr, R wi, R vr, R vi, R &ar, R &ai) {
+    if (VTraits<V>::cplx) {
+        ar += wr * vr - wi * vi;
+        ai += wr * vi + wi * vr;
+    } else {
+        ar += wr * vr;
+    }
+}
+
 // LDS written by some lanes of a wavefront, read by others
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1529,6 +1546,7 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
                                              const uint8_t *s_lnb, const uint16_t *s_cnb, V *s_x, const S2Lane &g,
                                              bool is_high, int lane, uint32_t h, uint32_t off, int n, int cw) {
     using T = VTraits<V>;
+    using R = typename SplitAcc<V>::type;   // row sums: FP32 for FP32 storage
     constexpr int kNB = sizeof(V) <= 4 ? 6 : (sizeof(V) <= 8 ? 5 : 3);
     constexpr uint32_t ES = sizeof(V);
     // this block's high hops: liveness by ballot; neighbour offset, row
@@ -1570,7 +1588,7 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
 #pragma unroll
         for (int j = 0; j < NS; j++) xo[j] = *atb(xo_base, roff[j]);
     }
-    double ar[NS], ai[NS];
+    R ar[NS], ai[NS];
 #pragma unroll
     for (int j = 0; j < NS; j++) {
         ar[j] = 0;
@@ -1624,12 +1642,12 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
                         dr += A.user_diag[2 * row];
                         di += A.user_diag_conj ? -A.user_diag[2 * row + 1] : A.user_diag[2 * row + 1];
                     }
-                    acc_nb<V>(dr, di, xr, xi, ar[j], ai[j]);
+                    acc_r<V, R>((R)dr, (R)di, (R)xr, (R)xi, ar[j], ai[j]);
                 }
                 for (int gi = 0; gi < D.nlow; gi++) {
                     const HopDev &hp = hops[gi];
-                    const double r1 = hp.re[1], r2 = hp.re[2];
-                    const double i1 = CC ? hp.im[1] : 0.0, i2 = CC ? hp.im[2] : 0.0;
+                    const R r1 = (R)hp.re[1], r2 = (R)hp.re[2];
+                    const R i1 = CC ? (R)hp.im[1] : (R)0, i2 = CC ? (R)hp.im[2] : (R)0;
                     const int sym = hp.sym, lo = hp.lo;
 #pragma unroll
                     for (int j = 0; j < NS; j++) {
@@ -1638,12 +1656,12 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
                         double vr, vi;
                         T::load(s_x, nid, vr, vi);
                         if (A.conj_x) vi = -vi;
-                        double mr = r1, mi = i1;
+                        R mr = r1, mi = i1;
                         if (!sym && !((lw[j] >> lo) & 1)) {
                             mr = r2;
                             mi = i2;
                         }
-                        acc_nb<V>(mr, -mi, vr, vi, ar[j], ai[j]);
+                        acc_r<V, R>(mr, -mi, (R)vr, (R)vi, ar[j], ai[j]);
                     }
                 }
             }
@@ -1666,7 +1684,7 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
                 for (int j = 0; j < NS; j++) {
                     double vr, vi;
                     T::load(&v[kk][j], 0, vr, vi);
-                    acc_nb<V>(wr, wi, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+                    acc_r<V, R>((R)wr, (R)wi, (R)vr, (R)(A.conj_x ? -vi : vi), ar[j], ai[j]);
                 }
             }
         }
@@ -1695,7 +1713,7 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
         for (int j = 0; j < NS; j++) {
             double vr, vi;
             T::load(&v[j], 0, vr, vi);
-            acc_nb<V>(wr, wi, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+            acc_r<V, R>((R)wr, (R)wi, (R)vr, (R)(A.conj_x ? -vi : vi), ar[j], ai[j]);
         }
     }
     // cross hops: rows of the neighbour block from the table, gathered
@@ -1735,14 +1753,14 @@ __device__ __forceinline__ void split2_block(const MVArgs &A, const Split2Dev &D
                 vr = 0;
                 vi = 0;
             }
-            acc_nb<V>(m_r, -m_i, vr, A.conj_x ? -vi : vi, ar[j], ai[j]);
+            acc_r<V, R>((R)m_r, (R)(-m_i), (R)vr, (R)(A.conj_x ? -vi : vi), ar[j], ai[j]);
         }
     }
     double dot_r = 0, dot_i = 0;
 #pragma unroll
     for (int j = 0; j < NS; j++) {
         const int r = lane + 64 * j;
-        if (r < n) finish_row<V>(A, y, (int64_t)off + r, ar[j], ai[j], 0.0, 0.0, dot_r, dot_i);
+        if (r < n) finish_row<V>(A, y, (int64_t)off + r, (double)ar[j], (double)ai[j], 0.0, 0.0, dot_r, dot_i);
     }
 }
 
