@@ -124,6 +124,12 @@ int qs_basis_reps(const qs_basis *b, uint64_t *out);
  * qs_basis_states / qs_basis_reps refuse such a basis. */
 int qs_basis_states_wide(const qs_basis *b, int which, uint64_t *out);
 
+/* qs_basis_lookup for states given as four 64-bit words each (any basis; the
+ * only form for bases of more than 64 sites): column of each state's orbit
+ * (-1 outside the sector) and its amplitude there -- state_index /
+ * get_amp of a WideSpinBasis (basis.jl:1049-1075). */
+int qs_basis_lookup_wide(const qs_basis *b, const uint64_t *states, int64_t n, int64_t *index, double *amplitude);
+
 /* Per column: number of distinct parent states in the orbit, and the
  * reference's normalization(basis, states(basis)[i]) (discrete_basis.jl:2401-2408). */
 int qs_basis_norms(const qs_basis *b, uint32_t *orbit_size, double *ref_norm);

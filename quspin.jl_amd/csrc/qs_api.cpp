@@ -734,6 +734,85 @@ int qs_basis_lookup(const qs_basis *b, const uint64_t *states, int64_t n, int64_
     });
 }
 
+int qs_basis_lookup_wide(const qs_basis *b, const uint64_t *states, int64_t n, int64_t *index, double *amplitude) {
+    return guard([&] {
+        check_ptr(b, "basis");
+        if (n > 0) check_ptr(states, "states");
+        const BasisHost &bb = b->b;
+        if (!bb.wide) {
+            // states of at most 64 sites: the one-word lookup, beyond-word bits invalid
+            std::vector<uint64_t> one((size_t)n);
+            std::vector<char> bad((size_t)n, 0);
+            for (int64_t i = 0; i < n; i++) {
+                one[(size_t)i] = states[4 * i];
+                bad[(size_t)i] = (states[4 * i + 1] | states[4 * i + 2] | states[4 * i + 3]) != 0;
+            }
+            std::vector<int64_t> idx((size_t)std::max<int64_t>(n, 1));
+            std::vector<double> amp((size_t)2 * std::max<int64_t>(n, 1));
+            const int st = qs_basis_lookup(b, one.data(), n, idx.data(), amp.data());
+            if (st != QS_OK) {
+                char msg[512];
+                qs_last_error(msg, sizeof msg);
+                fail(st, msg);
+            }
+            for (int64_t i = 0; i < n; i++) {
+                if (index) index[i] = bad[(size_t)i] ? -1 : idx[(size_t)i];
+                if (amplitude) {
+                    amplitude[2 * i] = bad[(size_t)i] ? 0.0 : amp[2 * (size_t)i];
+                    amplitude[2 * i + 1] = bad[(size_t)i] ? 0.0 : amp[2 * (size_t)i + 1];
+                }
+            }
+            return;
+        }
+        // wide basis: orbit minimum under the group (first element reaching
+        // it, its character), column by binary search over the minima
+        const GroupHost &g = bb.group;
+        std::vector<uint64_t> words((size_t)4 * bb.Ns);
+        QS_HIP(hipMemcpy(words.data(), bb.wreps.get(), sizeof(uint64_t) * words.size(), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> orb((size_t)bb.Ns);
+        QS_HIP(hipMemcpy(orb.data(), bb.orbit.get(), sizeof(uint32_t) * orb.size(), hipMemcpyDeviceToHost));
+        auto rep = [&](int64_t c) {
+            WState w;
+            for (int k = 0; k < 4; k++) w.w[k] = words[(size_t)4 * c + k];
+            return w;
+        };
+        for (int64_t i = 0; i < n; i++) {
+            WState u;
+            for (int k = 0; k < 4; k++) u.w[k] = states[4 * i + k];
+            bool valid = u.popcount() == bb.nup;
+            for (int site = bb.L; site < 256 && valid; site++) valid = !u.bit(site);
+            int64_t idx = -1;
+            cd amp(0, 0);
+            if (valid) {
+                WState best = u;
+                cd ch = g.echar[0];
+                for (int e = 1; e < g.order(); e++) {
+                    const WState w = wide_group_apply(g, e, u);
+                    if (w < best) {
+                        best = w;
+                        ch = g.echar[(size_t)e];
+                    }
+                }
+                int64_t lo = 0, hi = bb.Ns;
+                while (lo < hi) {
+                    const int64_t mid = lo + (hi - lo) / 2;
+                    if (rep(mid) < best) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < bb.Ns && rep(lo) == best) {
+                    idx = lo;
+                    amp = ch / std::sqrt((double)orb[(size_t)lo]);
+                }
+            }
+            if (index) index[i] = idx;
+            if (amplitude) {
+                amplitude[2 * i] = amp.real();
+                amplitude[2 * i + 1] = amp.imag();
+            }
+        }
+    });
+}
+
 int qs_op_create(const qs_basis *b, int n_terms, const char *const *opstr, const int32_t *n_couplings,
                  const double *coeff, const int32_t *sites, const double *diag, int flags, qs_op **out) {
     return guard([&] {

@@ -26,7 +26,7 @@ EXPORTS = (
     "qs_batch_matvec", "qs_batch_expm_multiply", "qs_server_create", "qs_server_destroy", "qs_server_submit",
     "qs_server_wait", "qs_lanczos", "qs_blocks_evolve", "qs_op_assemble", "qs_op_csr", "qs_op_set_graphs",
     "qs_op_matvec_noalloc", "qs_evolve_dynamic", "qs_block_gram", "qs_basis_create_general",
-    "qs_basis_states_wide",
+    "qs_basis_states_wide", "qs_basis_lookup_wide",
 )
 NOALLOC_SCRATCH_BYTES = 32 << 20   # QS_NOALLOC_SCRATCH_BYTES
 
@@ -56,6 +56,7 @@ def lib():
     L.qs_basis_size.argtypes = [vp, vp]
     L.qs_basis_states.argtypes = [vp, vp]
     L.qs_basis_states_wide.argtypes = [vp, i32, vp]
+    L.qs_basis_lookup_wide.argtypes = [vp, vp, i64, vp, vp]
     L.qs_basis_reps.argtypes = [vp, vp]
     L.qs_basis_norms.argtypes = [vp, vp, vp]
     L.qs_basis_is_complex.argtypes = [vp, vp]

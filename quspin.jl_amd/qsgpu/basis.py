@@ -244,6 +244,20 @@ class SpinBasisGeneral(SpinBasis1D):
                                     dtype=object)
         return self._states
 
+    def index(self, states):
+        """Column of each state's orbit (-1 if outside the sector) and <state|column>;
+        states of more than 64 sites as Python ints."""
+        if self.L <= 64:
+            return SpinBasis1D.index(self, states)
+        vals = [int(v) for v in np.atleast_1d(np.asarray(states, dtype=object))]
+        words = np.array([[(v >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for k in range(4)] for v in vals],
+                         dtype=np.uint64).reshape(-1, 4)
+        words = np.ascontiguousarray(words)
+        idx = np.empty(len(vals), dtype=np.int64)
+        amp = np.empty(len(vals), dtype=np.complex128)
+        check(_lib.lib().qs_basis_lookup_wide(self._h, _ptr(words), len(vals), _ptr(idx), _ptr(amp)))
+        return idx, amp
+
     @property
     def reps(self):
         if self.L <= 64:

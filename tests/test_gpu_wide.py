@@ -177,6 +177,58 @@ def test_wide_closed_forms(gpu):
     assert abs(e0 - np.linalg.eigvalsh(M)[0]) <= 1e-10
 
 
+@pytest.mark.parametrize("L,blocks", [(70, [("t", _t(70), 3)]), (66, [("t", _t(66), 0), ("p", _p(66), 0)]),
+                                      (12, [("t", _t(12), 2)])])
+def test_wide_lookup_against_projector(gpu, L, blocks):
+    # state_index / representative / get_amp (basis.jl:1049-1075,
+    # discrete_basis.jl:2386-2422): every parent state's column and amplitude
+    # against the oracle's projector row (one entry per row), within 1e-13
+    print("wide lookup", L, flush=True)
+    nup = 2
+    sec = oracle.WideSector(L, nup, pauli=False, blocks=[(m, q) for _, m, q in blocks])
+    b = qsgpu.SpinBasisGeneral(L, nup=nup, pauli=False, block_order=[n for n, _, _ in blocks], _explicit=True,
+                               **{n: (m, q) for n, m, q in blocks})
+    P = sec.projector_sparse().tocsr()
+    parent = sec.parent_states()
+    col_of = np.full(len(parent), -1, dtype=np.int64)
+    amp_of = np.zeros(len(parent), dtype=np.complex128)
+    for r in range(len(parent)):
+        a, b_ = P.indptr[r], P.indptr[r + 1]
+        if b_ > a:
+            col_of[r] = P.indices[a]
+            amp_of[r] = P.data[a]
+    idx, amp = b.index(parent)
+    assert idx.tolist() == col_of.tolist()
+    ok = col_of >= 0
+    assert np.abs(amp[ok] - amp_of[ok]).max() <= 1e-13
+    labels = sec.states()
+    assert [b.state_index(s) for s in labels[:5]] == list(range(5))
+    for r in np.flatnonzero(ok)[:: max(1, int(ok.sum()) // 20)]:
+        assert b.representative(parent[r]) == labels[col_of[r]]
+        assert abs(b.get_amp(parent[r]) - amp_of[r]) <= 1e-13
+    # outside the particle sector, and beyond the chain
+    idx, amp = b.index([1, (1 << L) | 1])
+    assert idx.tolist() == [-1, -1] and np.all(amp == 0)
+
+
+def test_wide_lookup_entry_on_a_64_bit_basis(gpu):
+    # qs_basis_lookup_wide on a SpinBasis1D sector equals qs_basis_lookup
+    import ctypes
+    from qsgpu import _lib
+    b = qsgpu.SpinBasis1D(12, nup=6, pauli=False, kblock=1)
+    parent = oracle.fixed_weight_states(12, 6)
+    idx1, amp1 = b.index(parent)
+    words = np.zeros((len(parent) + 1, 4), dtype=np.uint64)
+    words[:-1, 0] = parent
+    words[-1, 0], words[-1, 1] = parent[0], 1          # a bit beyond 64 sites: invalid
+    idx = np.empty(len(words), dtype=np.int64)
+    amp = np.empty(len(words), dtype=np.complex128)
+    qsgpu.errors.check(_lib.lib().qs_basis_lookup_wide(b._h, words.ctypes.data, len(words), idx.ctypes.data,
+                                                        amp.ctypes.data))
+    assert idx[:-1].tolist() == idx1.tolist() and idx[-1] == -1
+    assert np.array_equal(amp[:-1], amp1) and amp[-1] == 0
+
+
 def test_wide_errors(gpu):
     with pytest.raises(qsgpu.ArgumentError, match="require nup"):
         qsgpu.SpinBasisGeneral(70, t=(_t(70), 0))
