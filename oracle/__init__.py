@@ -6,7 +6,9 @@ The product library (``libqsgpu.so``) never loads it.
 
 ``qs_oracle.c`` restates QuSpin.jl's CPU algorithms (cited per function there);
 ``WideSector`` / ``parent_hamiltonian`` below cover chains of up to 256 sites
-(basis.jl:361-445, 1231-1300).
+(basis.jl:361-445, 1231-1300); ``krylov.py`` restates the Krylov exponential
+action (operators.jl:1655-1893) and ``rk45.py`` the dynamic evolution
+(operators.jl:1964-2071) in numpy.
 """
 import ctypes
 import os
