@@ -367,13 +367,49 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
         for (size_t i = 0; i < R; i++)
             if (col < F[i]->P.steps) act.push_back(i);
         if (act.empty()) break;
-        std::vector<MatvecJob> mj;
-        for (size_t i : act) mj.push_back({F[i]->op, F[i]->col(col, esz), F[i]->col(col + 1, esz)});
-        launch_batch_matvec(bm, mj, s);
         std::vector<size_t> herm, arn;
         for (size_t i : act) (F[i]->P.hermitian ? herm : arn).push_back(i);
         auto at = [&](size_t i, size_t o) { return d.get() + off[i] + o; };
-        if (!herm.empty()) {
+        // assembled Hermitian requests: the product also subtracts the
+        // previous basis vector and leaves the partials of <v_col, w>
+        bool fused = !herm.empty();
+        for (size_t i : herm) fused = fused && ops[F[i]->op]->sell;
+        size_t np = 0;
+        if (fused)
+            for (size_t i : herm) np += (size_t)2 * ((F[i]->n + 255) / 256);
+        wk.need(np, 0, 0);   // (grows the partials in stream order before any pointer into them is taken)
+        std::vector<MatvecJob> mj;
+        std::vector<VecJob> fr;
+        std::vector<int> frb;
+        np = 0;
+        for (size_t i : act) {
+            MatvecJob j{F[i]->op, F[i]->col(col, esz), F[i]->col(col + 1, esz)};
+            if (fused && F[i]->P.hermitian) {
+                if (col > 0) {
+                    j.z = F[i]->col(col - 1, esz);
+                    j.zc = at(i, sc(i).nrm2(col - 1));
+                    j.zmode = CO_NEG_SQRT;
+                }
+                j.dot = wk.partial.get() + np;
+                VecJob r{};
+                r.partial = j.dot;
+                r.nb = (int)((F[i]->n + 255) / 256);
+                r.aux = at(i, sc(i).h1(col));
+                np += (size_t)2 * r.nb;
+                fr.push_back(r);
+                frb.push_back(1);
+            }
+            mj.push_back(j);
+        }
+        launch_batch_matvec(bm, mj, s);
+        if (fused) {
+            launch_vec_jobs(dtype, VOP_REDUCE, fr, frb, s, &wk.jt);
+            std::vector<UpdDev> uq;
+            for (size_t i : herm)
+                uq.push_back({F[i]->col(col, esz), F[i]->n, 1, at(i, sc(i).h1(col)), F[i]->col(col + 1, esz),
+                              at(i, sc(i).nrm2(col))});
+            bupdate_dev(dtype, uq, wk, s);
+        } else if (!herm.empty()) {
             // three-term Lanczos without reorthogonalisation
             std::vector<AxpbyDev> ax;
             for (size_t i : herm)

@@ -380,4 +380,16 @@ __device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, cons
     }
 }
 
+// Coefficient of an axpby read from device memory (a reduction result still
+// on the device), computed with the same IEEE operations the host would use.
+__device__ __forceinline__ void dev_coef(const double *c, int mode, double &ar, double &ai) {
+    switch (mode) {
+    case CO_NEG: ar = -c[0]; ai = -c[1]; break;
+    case CO_NEG_RE: ar = -c[0]; ai = 0.0; break;
+    case CO_NEG_SQRT: ar = -sqrt(c[0]); ai = 0.0; break;
+    case CO_INV_SQRT: ar = 1.0 / sqrt(c[0]); ai = 0.0; break;
+    default: break;
+    }
+}
+
 }  // namespace qs

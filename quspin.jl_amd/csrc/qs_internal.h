@@ -612,6 +612,13 @@ struct MatvecJob {
     int op;            // index into the batch's operator list
     const void *x;
     void *y;
+    // assembled operators only: y += a z with a from device memory
+    // (CoefMode zmode of zc), then per-block FP64 partials of <x, y> to
+    // dot[2 b] (b < ceil(N / 256)); the next step of a Lanczos recurrence
+    const void *z = nullptr;
+    const double *zc = nullptr;
+    int zmode = 0;
+    double *dot = nullptr;
 };
 struct BatchMatvec;   // device copies of the operators' kernel arguments
 std::unique_ptr<BatchMatvec, void (*)(BatchMatvec *)> make_batch_matvec(const std::vector<const OpHost *> &ops,

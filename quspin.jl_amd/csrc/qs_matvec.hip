@@ -1065,6 +1065,10 @@ struct SellJob {
     const void *x;
     void *y;
     int b0, nb;
+    const void *z;     // optional: y += a z, a from zc (CoefMode zmode)
+    const double *zc;
+    int zmode;
+    double *dot;       // optional: per-block partials of <x, y>
 };
 struct SellJobs {
     int njobs;
@@ -1074,14 +1078,58 @@ struct SellJobs {
 // block b of job q: its kSellBlock rows
 template <typename V>
 __device__ __forceinline__ void sell_job_block(const SellJob &q, int b) {
+    using T = VTraits<V>;
     const MVArgs &A = *q.A;
     const int64_t row = (int64_t)b * kSellBlock + threadIdx.x;
-    if (row >= A.N) return;
-    double ar = 0, ai = 0;
-    sell_row<V>(q.cols, q.wre, q.wim, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
-                (const V *)q.x, A.conj_x, ar, ai);
-    double dot_r = 0, dot_i = 0;
-    finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
+    if (!q.dot) {
+        if (row >= A.N) return;
+        double ar = 0, ai = 0;
+        sell_row<V>(q.cols, q.wre, q.wim, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
+                    (const V *)q.x, A.conj_x, ar, ai);
+        double dot_r = 0, dot_i = 0;
+        finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
+        return;
+    }
+    // the Lanczos form: y = M x + a z (the axpby's operation order), then
+    // this block's partial of <x, y> (the Gram product's per-row terms)
+    __shared__ double red[2 * (kSellBlock / 64)];
+    double dr = 0, di = 0;
+    if (row < A.N) {
+        double ar = 0, ai = 0;
+        sell_row<V>(q.cols, q.wre, q.wim, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
+                    (const V *)q.x, A.conj_x, ar, ai);
+        double dot_r = 0, dot_i = 0;
+        finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
+        double yr, yi, xr, xi;
+        T::load((const V *)q.y, row, yr, yi);
+        if (q.z) {
+            double cr = 0, ci = 0, zr, zi;
+            dev_coef(q.zc, q.zmode, cr, ci);
+            T::load((const V *)q.z, row, zr, zi);
+            const double orr = cr * zr - ci * zi + (1.0 * yr - 0.0 * yi);
+            const double oi = cr * zi + ci * zr + (1.0 * yi + 0.0 * yr);
+            T::store((V *)q.y, row, orr, oi);
+            T::load((const V *)q.y, row, yr, yi);
+        }
+        T::load((const V *)q.x, row, xr, xi);
+        dr = xr * yr + xi * yi;
+        di = xr * yi - xi * yr;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        dr += __shfl_xor(dr, o, 64);
+        di += __shfl_xor(di, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = dr;
+        red[kSellBlock / 64 + w] = di;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        q.dot[2 * b] = (red[0] + red[1]) + (red[2] + red[3]);
+        q.dot[2 * b + 1] = (red[4] + red[5]) + (red[6] + red[7]);
+    }
 }
 
 template <typename V>
@@ -2951,7 +2999,8 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
                 const SellHost &t = *op.sell;
                 const int nb = (int)((op.basis->Ns + kSellBlock - 1) / kSellBlock);
                 tab.push_back(SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(),
-                                      t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, 0, nb});
+                                      t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, 0, nb, mj.z, mj.zc, mj.zmode,
+                                      mj.dot});
                 nbmax = std::max(nbmax, nb);
                 done[j] = 1;
             }
@@ -2978,7 +3027,8 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
                 const SellHost &t = *op.sell;
                 const int nb = (int)((op.basis->Ns + kSellBlock - 1) / kSellBlock);
                 J.job[i] = SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(),
-                                   t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, total, nb};
+                                   t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, total, nb, mj.z, mj.zc, mj.zmode,
+                                   mj.dot};
                 total += nb;
                 done[asm_jobs[c0 + i]] = 1;
             }
@@ -2995,6 +3045,7 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
     // the others: one launch per (column mode, particle number) class, jobs in order
     for (size_t first = 0; first < jobs.size(); first++) {
         if (done[first]) continue;
+        if (jobs[first].dot || jobs[first].z) fail(QS_EARG, "internal: the fused Lanczos product needs an assembled operator");
         const int key_cm = bm.cm[jobs[first].op], key_pc = bm.pc[jobs[first].op];
         std::vector<size_t> group;
         for (size_t j = first; j < jobs.size(); j++)

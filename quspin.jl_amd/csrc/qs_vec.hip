@@ -300,17 +300,6 @@ __device__ __forceinline__ void norm2_body(const V *__restrict__ x, int64_t n, d
     }
 }
 
-// Coefficient of an axpby read from device memory (a reduction result still
-// on the device), computed with the same IEEE operations the host would use.
-__device__ __forceinline__ void dev_coef(const double *c, int mode, double &ar, double &ai) {
-    switch (mode) {
-    case CO_NEG: ar = -c[0]; ai = -c[1]; break;
-    case CO_NEG_RE: ar = -c[0]; ai = 0.0; break;
-    case CO_NEG_SQRT: ar = -sqrt(c[0]); ai = 0.0; break;
-    case CO_INV_SQRT: ar = 1.0 / sqrt(c[0]); ai = 0.0; break;
-    default: break;
-    }
-}
 
 template <typename V>
 __device__ __forceinline__ void axpby_body(double ar, double ai, const V *__restrict__ x, double br, double bi,
