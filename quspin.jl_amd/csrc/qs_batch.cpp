@@ -439,8 +439,32 @@ void bfactorize(BatchMatvec &bm, const std::vector<const OpHost *> &ops, int dty
                 u2.push_back({F[i]->V, F[i]->n, col + 1, at(i, sc(i).h2(col)), w, at(i, sc(i).nrm2(col))});
             }
             bdots_dev(dtype, d1, wk, s);
-            bupdate_dev(dtype, u1, wk, s);
-            bdots_dev(dtype, d2, wk, s);
+            // first update and second Gram product in one read of the basis
+            std::vector<VecJob> ug, rg;
+            std::vector<int> rgb;
+            size_t np = 0;
+            for (size_t i : arn) {
+                VecJob j{};
+                j.V = F[i]->V;
+                j.w = F[i]->col(col + 1, esz);
+                j.n = F[i]->n;
+                j.m = col + 1;
+                j.nb = vec_blocks(F[i]->n);
+                j.aux = at(i, sc(i).h1(col));
+                j.partial = wk.partial.get() + np;
+                np += (size_t)2 * j.nb * (col + 2);
+                ug.push_back(j);
+                VecJob r = j;
+                r.aux = at(i, sc(i).h2(col));   // h2[0 .. m) and |w'|^2 after them (unused)
+                rg.push_back(r);
+                rgb.push_back(col + 2);
+            }
+            if (np <= wk.partial.n && launch_updgram_jobs(dtype, ug, s, &wk.jt)) {
+                launch_vec_jobs(dtype, VOP_REDUCE, rg, rgb, s, &wk.jt);
+            } else {
+                bupdate_dev(dtype, u1, wk, s);
+                bdots_dev(dtype, d2, wk, s);
+            }
             bupdate_dev(dtype, u2, wk, s);
         }
         std::vector<AxpbyDev> sc2;

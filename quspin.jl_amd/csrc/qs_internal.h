@@ -590,6 +590,10 @@ struct VecJobs {
 void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::vector<int> &blocks, hipStream_t s,
                      JobTable *table = nullptr);
 int ew_blocks(int64_t n);
+// w_j -= V_j h_j then partials of V_j^H w_j and |w_j|^2 (m + 1 per block, as
+// k_update_gram) for jobs sharing one column count m (job.aux = h); false
+// (nothing launched) when m is too large for the fused kernel
+bool launch_updgram_jobs(int dtype, std::vector<VecJob> &jobs, hipStream_t s, JobTable *table);
 
 // Krylov steps of a small assembled sector in one single-workgroup launch
 // (qs_vec.hip): SMALL_DGKS runs steps j0 .. steps-1 of a thick-restart

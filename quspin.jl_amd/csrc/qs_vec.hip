@@ -264,12 +264,16 @@ __device__ __forceinline__ void update_gram_body(const V *__restrict__ Vb, int64
             }
         }
     }
-    for (int c = 0; c < m; c++) {
-        double re = gr[c], im = gi[c];
-        block_reduce2(re, im, sh);
-        if (threadIdx.x == 0) {
-            partial[((int64_t)c * nb + b) * 2] = re;
-            partial[((int64_t)c * nb + b) * 2 + 1] = im;
+    // unrolled with the column test inside, so gr / gi stay in registers
+#pragma unroll
+    for (int c = 0; c < MX; c++) {
+        if (c < m) {   // uniform over the block
+            double re = gr[c], im = gi[c];
+            block_reduce2(re, im, sh);
+            if (threadIdx.x == 0) {
+                partial[((int64_t)c * nb + b) * 2] = re;
+                partial[((int64_t)c * nb + b) * 2 + 1] = im;
+            }
         }
     }
     double dummy = 0;
@@ -957,6 +961,18 @@ __global__ __launch_bounds__(kVB) void kb_vec(VecJobs J, int op) {
     vec_job_block<V>(q, b, op, sh);
 }
 
+// The fused first-pass update and second-pass Gram product (update_gram_body)
+// over a device job table, every job with the same column count m <= MX:
+// job j owns blocks [j nbmax, (j+1) nbmax) and uses the first tab[j].b0.
+template <typename V, int MX>
+__global__ __launch_bounds__(kVB) void kb_updgram_tab(const VecJob *__restrict__ tab, int nbmax) {
+    __shared__ double sh[2 * kVB];
+    const int j = (int)blockIdx.x / nbmax, b = (int)blockIdx.x - j * nbmax;
+    const VecJob q = tab[j];
+    if (b >= q.b0) return;
+    update_gram_body<V, MX>((const V *)q.V, q.n, q.m, q.aux, (V *)q.w, q.partial, b, q.nb, sh);
+}
+
 // The same over a device job table: job j owns blocks [j nbmax, (j+1) nbmax)
 // and uses the first tab[j].b0 of them (b0 holds its block count here).
 template <typename V>
@@ -1068,6 +1084,29 @@ void launch_vec_jobs(int dtype, int op, std::vector<VecJob> &jobs, const std::ve
         });
         QS_HIP(hipGetLastError());
     }
+}
+
+bool launch_updgram_jobs(int dtype, std::vector<VecJob> &jobs, hipStream_t s, JobTable *table) {
+    if (jobs.empty()) return true;
+    const int m = jobs[0].m;
+    if (m > kDgksFuse) return false;
+    int nbmax = 0;
+    for (auto &j : jobs) {
+        if (j.m != m) fail(QS_EARG, "internal: fused Gram jobs of different column counts");
+        nbmax = std::max(nbmax, j.nb);
+        j.b0 = j.nb;
+    }
+    if ((int64_t)nbmax * (int64_t)jobs.size() > INT32_MAX) fail(QS_EARG, "internal: batched grid too large");
+    const VecJob *tab = (const VecJob *)table->stage(jobs.data(), jobs.size() * sizeof(VecJob), s);
+    const int grid = nbmax * (int)jobs.size();
+    by_dtype(dtype, [&](auto *tag) {
+        using V = std::remove_pointer_t<decltype(tag)>;
+        if (m <= 8) hipLaunchKernelGGL((kb_updgram_tab<V, 8>), dim3(grid), dim3(kVB), 0, s, tab, nbmax);
+        else if (m <= 16) hipLaunchKernelGGL((kb_updgram_tab<V, 16>), dim3(grid), dim3(kVB), 0, s, tab, nbmax);
+        else hipLaunchKernelGGL((kb_updgram_tab<V, kDgksFuse>), dim3(grid), dim3(kVB), 0, s, tab, nbmax);
+    });
+    QS_HIP(hipGetLastError());
+    return true;
 }
 
 int vec_blocks(int64_t n) {
