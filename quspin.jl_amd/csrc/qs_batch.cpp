@@ -121,26 +121,25 @@ void bcombine(int dtype, const std::vector<CombReq> &rq, BWork &wk, hipStream_t 
             cc[co[i] + 2 * t] = rq[i].C[t].real();
             cc[co[i] + 2 * t + 1] = rq[i].C[t].imag();
         }
-    if (nc) QS_HIP(hipMemcpyAsync(wk.coef.get(), cc.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    // staged through the pinned job table: no wait before the next call
+    const double *coef = nc ? (const double *)wk.jt.stage(cc.data(), sizeof(double) * nc, s) : nullptr;
     std::vector<VecJob> a;
     std::vector<int> ab;
     for (size_t i = 0; i < rq.size(); i++) {
         // a job with many output columns is a GEMM: its own launch on the matrix cores
-        if (combine_on_mfma(dtype, rq[i].V, rq[i].n, rq[i].m, wk.coef.get() + co[i], rq[i].k, rq[i].Y, s)) continue;
+        if (combine_on_mfma(dtype, rq[i].V, rq[i].n, rq[i].m, coef + co[i], rq[i].k, rq[i].Y, s)) continue;
         VecJob j{};
         j.V = rq[i].V;
         j.n = rq[i].n;
         j.m = rq[i].m;
         j.k = rq[i].k;
         j.w = rq[i].Y;
-        j.aux = wk.coef.get() + co[i];
+        j.aux = (double *)coef + co[i];
         j.nb = ew_blocks(rq[i].n);
         a.push_back(j);
         ab.push_back(j.nb);
     }
     if (!a.empty()) launch_vec_jobs(dtype, VOP_COMBINE, a, ab, s, &wk.jt);
-    // the coefficient buffer is reused by the next call
-    QS_HIP(hipStreamSynchronize(s));
 }
 
 // Device-scalar forms: results land in device memory at the caller's
@@ -496,6 +495,7 @@ struct Call {
     int depth;
     void *out;
     std::shared_ptr<char> vbuf, obuf;
+    double v2 = -1;   // |v|^2 when known on the host (the previous call's output norm)
 };
 
 // Split midpoints of one call of expm_times_batch: fixed-size device blocks
@@ -631,7 +631,8 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
     for (size_t q = 0; q < idx.size(); q++) {
         const size_t i = idx[q].first, j = idx[q].second, t = live[i];
         if (ests[i][j] > tol * std::max(1.0, std::sqrt(o2[q])))
-            chains.push_back({(int)t, {Call{tasks[t].v, tasks[t].scales[j], tol, 0, outcol(t, j), nullptr, nullptr}}});
+            chains.push_back(
+                {(int)t, {Call{tasks[t].v, tasks[t].scales[j], tol, 0, outcol(t, j), nullptr, nullptr, v2[t]}}});
     }
     F.clear();
     // halving recursion, the next call of as many chains as device memory
@@ -646,20 +647,38 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
         for (size_t c = 0; c < chains.size() && run.size() < cap; c++)
             if (!chains[c].stack.empty()) run.push_back(c);
         if (run.empty()) break;
+        // input norms: known on the host for every call but the second half
+        // of a split whose first half finished in the same round
         std::vector<NormReq> vq;
-        for (size_t c : run) vq.push_back({chains[c].stack.back().v, nrows(chains[c].task)});
-        std::vector<double> vn;
-        bnorm2(dtype, vq, wk, s, vn);
+        std::vector<size_t> unk;
+        std::vector<double> vn(run.size());
+        for (size_t r = 0; r < run.size(); r++) {
+            const Call &cl = chains[run[r]].stack.back();
+            vn[r] = cl.v2;
+            if (cl.v2 < 0) {
+                vq.push_back({cl.v, nrows(chains[run[r]].task)});
+                unk.push_back(r);
+            }
+        }
+        if (!vq.empty()) {
+            std::vector<double> got;
+            bnorm2(dtype, vq, wk, s, got);
+            for (size_t q = 0; q < unk.size(); q++) vn[unk[q]] = got[q];
+        }
         std::vector<size_t> fac;
+        std::vector<double> fv2;
         for (size_t r = 0; r < run.size(); r++) {
             Chain &ch = chains[run[r]];
             Call &cl = ch.stack.back();
             if (cl.scale == cd(0, 0) || vn[r] == 0) {
                 QS_HIP(hipMemcpyAsync(cl.out, cl.v, nrows(ch.task) * esz, hipMemcpyDeviceToDevice, s));
+                void *out = cl.out;
                 ch.stack.pop_back();
+                if (!ch.stack.empty() && ch.stack.back().v == out) ch.stack.back().v2 = vn[r];
                 continue;
             }
             fac.push_back(run[r]);
+            fv2.push_back(vn[r]);
         }
         if (fac.empty()) continue;
         std::vector<std::pair<int, const void *>> rq;
@@ -693,7 +712,10 @@ void expm_times_batch(const std::vector<const OpHost *> &ops, int dtype, const s
                 std::shared_ptr<char> mid = mids.get();
                 // second half (mid -> out) runs after the first (v -> mid): pushed first
                 ch.stack.push_back(Call{mid.get(), cl.scale / 2.0, cl.tol / 2, cl.depth + 1, cl.out, mid, cl.obuf});
-                ch.stack.push_back(Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid.get(), cl.vbuf, mid});
+                ch.stack.push_back(
+                    Call{cl.v, cl.scale / 2.0, cl.tol / 2, cl.depth + 1, mid.get(), cl.vbuf, mid, fv2[r]});
+            } else if (!ch.stack.empty() && ch.stack.back().v == cl.out) {
+                ch.stack.back().v2 = onr[r];   // the next call reads this output
             }
         }
     }

@@ -769,57 +769,122 @@ FastTerms compile_fast(const std::vector<CTerm> &diag, const std::vector<CTerm> 
 }
 
 // ---------------------------------------------------- dense eigensolvers --
-// Cyclic Jacobi for a real symmetric n x n matrix (row-major A, destroyed).
-// Eigenvalues ascending in w, eigenvectors in the columns of V (row-major).
+// Real symmetric n x n eigenproblem (row-major A, destroyed): Householder
+// reduction to tridiagonal form, then implicit symmetric QR steps with the
+// Wilkinson shift (Golub & Van Loan, Matrix Computations, algorithms 8.3.1
+// and 8.3.3), the Givens rotations accumulated onto the reduction's
+// orthogonal factor.  A matrix that is already tridiagonal (the Lanczos
+// projections) skips the reduction.  Eigenvalues ascending in w,
+// eigenvectors in the columns of V (row-major).
 void sym_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V)
 {
-    V.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
     auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
-    for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0, tot = 0;
-        for (int i = 0; i < n; i++)
-            for (int j = 0; j < n; j++) {
-                double v = a(i, j) * a(i, j);
-                tot += v;
-                if (i != j) off += v;
+    // the orthogonal factor is accumulated transposed (Zt row c = column c),
+    // so each rotation and reflection streams contiguous rows
+    std::vector<double> Zt((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) Zt[(size_t)i * n + i] = 1.0;
+    auto zrow = [&](int c) { return Zt.data() + (size_t)c * n; };
+    V = Zt;
+    w.assign(n, 0.0);
+    if (n == 0) return;
+    bool finite = true;
+    for (double x : A) finite = finite && std::isfinite(x);
+    if (!finite) {
+        w.assign(n, std::nan(""));
+        return;
+    }
+    // Householder reduction: A = Q T Q^T
+    std::vector<double> v(n), p(n), t(n);
+    for (int k = 0; k + 2 < n; k++) {
+        const int m = n - k - 1;
+        double sig = 0;
+        for (int i = 0; i < m; i++) sig += a(k + 1 + i, k) * a(k + 1 + i, k);
+        double tail = sig - a(k + 1, k) * a(k + 1, k);
+        if (tail == 0) continue;   // column already reduced
+        sig = std::sqrt(sig);
+        const double x0 = a(k + 1, k);
+        const double alpha = x0 >= 0 ? -sig : sig;
+        for (int i = 0; i < m; i++) v[i] = a(k + 1 + i, k);
+        v[0] = x0 - alpha;
+        double vv = 0;
+        for (int i = 0; i < m; i++) vv += v[i] * v[i];
+        const double beta = 2.0 / vv;
+        // trailing block: A22 -= v w^T + w v^T, p = beta A22 v, w = p - (beta p.v / 2) v
+        double pv = 0;
+        for (int i = 0; i < m; i++) {
+            double acc = 0;
+            for (int j = 0; j < m; j++) acc += a(k + 1 + i, k + 1 + j) * v[j];
+            p[i] = beta * acc;
+            pv += p[i] * v[i];
+        }
+        const double kf = 0.5 * beta * pv;
+        for (int i = 0; i < m; i++) p[i] -= kf * v[i];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) a(k + 1 + i, k + 1 + j) -= v[i] * p[j] + p[i] * v[j];
+        a(k + 1, k) = a(k, k + 1) = alpha;
+        for (int i = 1; i < m; i++) a(k + 1 + i, k) = a(k, k + 1 + i) = 0.0;
+        // Q <- Q (I - beta v v^T) on columns k+1 ..
+        std::fill(t.begin(), t.end(), 0.0);
+        for (int j = 0; j < m; j++) {
+            const double *zr = zrow(k + 1 + j);
+            for (int r = 0; r < n; r++) t[r] += v[j] * zr[r];
+        }
+        for (int j = 0; j < m; j++) {
+            double *zr = zrow(k + 1 + j);
+            const double f = beta * v[j];
+            for (int r = 0; r < n; r++) zr[r] -= f * t[r];
+        }
+    }
+    std::vector<double> d(n), e(n, 0.0);
+    for (int i = 0; i < n; i++) d[i] = a(i, i);
+    for (int i = 0; i + 1 < n; i++) e[i] = 0.5 * (a(i, i + 1) + a(i + 1, i));
+    // implicit QR on the unreduced trailing block [lo, hi], bulge chased
+    // down by rotations G = [[c, s], [-s, c]] on (k, k+1)
+    const double eps = 2.220446049250313e-16;
+    auto small = [&](int i) { return std::fabs(e[i]) <= eps * (std::fabs(d[i]) + std::fabs(d[i + 1])); };
+    int hi = n - 1, sweeps = 0;
+    while (hi > 0) {
+        if (small(hi - 1)) {
+            e[hi - 1] = 0;
+            hi--;
+            continue;
+        }
+        int lo = hi - 1;
+        while (lo > 0 && !small(lo - 1)) lo--;
+        if (lo > 0) e[lo - 1] = 0;
+        if (++sweeps > 64 * n) break;   // not reached for finite input
+        const double dl = 0.5 * (d[hi - 1] - d[hi]), eb = e[hi - 1];
+        const double mu = d[hi] - eb * eb / (dl + std::copysign(std::hypot(dl, eb), dl));
+        double x = d[lo] - mu, z = e[lo];
+        for (int k = lo; k < hi; k++) {
+            const double r = std::sqrt(x * x + z * z);
+            const double c = r == 0 ? 1.0 : x / r, sn = r == 0 ? 0.0 : -z / r;
+            if (k > lo) e[k - 1] = r;
+            const double dk = d[k], dk1 = d[k + 1], ek = e[k];
+            d[k] = c * c * dk - 2 * c * sn * ek + sn * sn * dk1;
+            d[k + 1] = sn * sn * dk + 2 * c * sn * ek + c * c * dk1;
+            e[k] = c * sn * (dk - dk1) + (c * c - sn * sn) * ek;
+            if (k + 1 < hi) {
+                z = -sn * e[k + 1];
+                e[k + 1] *= c;
+                x = e[k];
             }
-        if (off <= 1e-34 * tot || off == 0) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                double apq = a(p, q);
-                if (apq == 0) continue;
-                double app = a(p, p), aqq = a(q, q);
-                double theta = (aqq - app) / (2 * apq);
-                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
-                double c = 1 / std::sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < n; k++) {
-                    double akp = a(k, p), akq = a(k, q);
-                    a(k, p) = c * akp - s * akq;
-                    a(k, q) = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; k++) {
-                    double apk = a(p, k), aqk = a(q, k);
-                    a(p, k) = c * apk - s * aqk;
-                    a(q, k) = s * apk + c * aqk;
-                }
-                for (int k = 0; k < n; k++) {
-                    double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
-                    V[(size_t)k * n + p] = c * vkp - s * vkq;
-                    V[(size_t)k * n + q] = s * vkp + c * vkq;
-                }
+            double *z0 = zrow(k), *z1 = zrow(k + 1);
+            for (int r2 = 0; r2 < n; r2++) {
+                const double zk = z0[r2], zk1 = z1[r2];
+                z0[r2] = c * zk - sn * zk1;
+                z1[r2] = sn * zk + c * zk1;
             }
+        }
     }
     std::vector<int> ord(n);
     for (int i = 0; i < n; i++) ord[i] = i;
-    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
-    w.resize(n);
-    std::vector<double> V2((size_t)n * n);
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return d[x] < d[y]; });
     for (int j = 0; j < n; j++) {
-        w[j] = a(ord[j], ord[j]);
-        for (int k = 0; k < n; k++) V2[(size_t)k * n + j] = V[(size_t)k * n + ord[j]];
+        w[j] = d[ord[j]];
+        const double *zr = zrow(ord[j]);
+        for (int k = 0; k < n; k++) V[(size_t)k * n + j] = zr[k];
     }
-    V.swap(V2);
 }
 
 // Complex Hermitian eigenproblem via the real symmetric embedding
@@ -864,14 +929,22 @@ void herm_eig(int n, std::vector<cd> &A, std::vector<double> &w, std::vector<cd>
 // Pade approximant (Higham, SIAM J. Matrix Anal. Appl. 26 (2005) 1179).
 void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
 {
+    // complex products on the interleaved (re, im) doubles, which vectorise
     auto mm = [n](const std::vector<cd> &X, const std::vector<cd> &Y) {
         std::vector<cd> Z((size_t)n * n, cd(0, 0));
-        for (int i = 0; i < n; i++)
+        const double *y = reinterpret_cast<const double *>(Y.data());
+        for (int i = 0; i < n; i++) {
+            double *z = reinterpret_cast<double *>(Z.data() + (size_t)i * n);
             for (int k = 0; k < n; k++) {
-                cd x = X[(size_t)i * n + k];
-                if (x == cd(0, 0)) continue;
-                for (int j = 0; j < n; j++) Z[(size_t)i * n + j] += x * Y[(size_t)k * n + j];
+                const double xr = X[(size_t)i * n + k].real(), xi = X[(size_t)i * n + k].imag();
+                if (xr == 0 && xi == 0) continue;
+                const double *yk = y + (size_t)2 * k * n;
+                for (int j = 0; j < n; j++) {
+                    z[2 * j] += xr * yk[2 * j] - xi * yk[2 * j + 1];
+                    z[2 * j + 1] += xr * yk[2 * j + 1] + xi * yk[2 * j];
+                }
             }
+        }
         return Z;
     };
     double norm1 = 0;

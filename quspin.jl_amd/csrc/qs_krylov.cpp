@@ -676,6 +676,12 @@ int KrylovProj::replay(const FactScalars &L, const std::vector<double> &d) {
 
 double KrylovProj::expm_column(cd scale, std::vector<cd> &coef) const {
     const int c = completed;
+    if (hermitian && scale != cd(0, 0)) {
+        // real symmetric tridiagonal projection: through its eigenvectors
+        std::vector<double> est;
+        lanczos_times({scale}, coef, est);
+        return est[0];
+    }
     std::vector<cd> A((size_t)c * c), E;
     for (int i = 0; i < c; i++)
         for (int j = 0; j < c; j++) A[(size_t)i * c + j] = scale * h(i, j);
