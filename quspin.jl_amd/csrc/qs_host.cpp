@@ -9,9 +9,12 @@
 #include <tuple>
 
 #include <atomic>
+#include <condition_variable>
 #include <exception>
 #include <mutex>
 #include <thread>
+
+#include <unistd.h>
 
 #include "qs_internal.h"
 
@@ -1014,12 +1017,94 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
     for (int i = 0; i < s; i++) E = mm(E, E);
 }
 
+namespace {
+
+// Host workers kept for the life of the process, so a lockstep round's
+// per-request host work (scalar replay, small exponentials) costs no thread
+// creation.  One job at a time: a caller that finds the pool busy (another
+// serving thread, a nested call) or finds itself in a forked child runs its
+// own threads instead.
+class WorkPool {
+public:
+    static WorkPool &get() {
+        static WorkPool *p = new WorkPool;   // never destroyed: workers outlive static teardown
+        return *p;
+    }
+    bool run(size_t n, size_t nt, const std::function<void(size_t)> &fn) {
+        if (getpid() != pid_) return false;
+        std::unique_lock<std::mutex> own(owner_, std::try_to_lock);
+        if (!own.owns_lock()) return false;
+        while (workers_.size() + 1 < nt) {
+            const size_t id = workers_.size();
+            workers_.emplace_back([this, id] { loop(id); });
+            workers_.back().detach();
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            fn_ = &fn;
+            n_ = n;
+            next_ = 0;
+            err_ = nullptr;
+            want_ = nt - 1;
+            busy_ = nt - 1;
+            gen_++;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+        std::exception_ptr e = err_;
+        g.unlock();
+        if (e) std::rethrow_exception(e);
+        return true;
+    }
+
+private:
+    WorkPool() : pid_(getpid()) {}
+    void drain() {
+        for (size_t i; (i = next_.fetch_add(1)) < n_;) {
+            try {
+                (*fn_)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(m_);
+                if (!err_) err_ = std::current_exception();
+            }
+        }
+    }
+    void loop(size_t id) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> g(m_);
+        for (;;) {
+            cv_.wait(g, [&] { return gen_ != seen; });
+            seen = gen_;
+            if (id >= want_) continue;
+            g.unlock();
+            drain();
+            g.lock();
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    const pid_t pid_;
+    std::mutex owner_, m_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> workers_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0, want_ = 0, busy_ = 0;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    std::exception_ptr err_;
+};
+
+}  // namespace
+
 void parallel_for(size_t n, const std::function<void(size_t)> &fn) {
     const size_t nt = std::min<size_t>({n, 16, std::max(1u, std::thread::hardware_concurrency())});
     if (nt <= 1) {
         for (size_t i = 0; i < n; i++) fn(i);
         return;
     }
+    if (WorkPool::get().run(n, nt, fn)) return;
     std::atomic<size_t> next{0};
     std::exception_ptr err;
     std::mutex em;

@@ -751,8 +751,9 @@ struct KrylovProj {
     cd h(int i, int j) const { return H[(size_t)i * steps + j]; }
     void lanczos_entry(int col, cd alpha, double tn);
     void arnoldi_column(int col, const std::vector<cd> &h1, const std::vector<cd> &h2, double tn);
-    // records tn; true (and completed = col + 1) on breakdown
-    bool breakdown(int col, double tn);
+    // records tn; true (and completed = col + 1) on breakdown; f = squared
+    // Frobenius norm of H's entries so far
+    bool breakdown(int col, double tn, double f);
     // the per-step bookkeeping of a factorization queued with its scalars on
     // the device (beta, H entries, breakdown test in step order); returns the
     // number of steps the reference performs (= matvecs)

@@ -647,10 +647,8 @@ void KrylovProj::arnoldi_column(int col, const std::vector<cd> &h1, const std::v
     H[(size_t)(col + 1) * steps + col] = tn;
 }
 
-bool KrylovProj::breakdown(int col, double tn) {
+bool KrylovProj::breakdown(int col, double tn, double f) {
     terminal = tn;
-    double f = 0;
-    for (auto &z : H) f += std::norm(z);
     if (tn <= 100 * kEps * std::max(1.0, std::sqrt(f))) {
         completed = col + 1;
         return true;
@@ -660,6 +658,7 @@ bool KrylovProj::breakdown(int col, double tn) {
 
 int KrylovProj::replay(const FactScalars &L, const std::vector<double> &d) {
     beta = std::sqrt(d[L.beta2()]);
+    double f = 0;   // squared Frobenius norm of the entries set so far
     for (int col = 0; col < steps; col++) {
         const double tn = std::sqrt(d[L.nrm2(col)]);
         auto hv = [&](size_t off, int count) {
@@ -667,9 +666,16 @@ int KrylovProj::replay(const FactScalars &L, const std::vector<double> &d) {
             for (int i = 0; i < count; i++) h[i] = cd(d[off + 2 * i], d[off + 2 * i + 1]);
             return h;
         };
-        if (hermitian) lanczos_entry(col, cd(d[L.h1(col)], d[L.h1(col) + 1]), tn);
-        else arnoldi_column(col, hv(L.h1(col), col + 1), hv(L.h2(col), col + 1), tn);
-        if (breakdown(col, tn)) return col + 1;
+        if (hermitian) {
+            const cd alpha(d[L.h1(col)], d[L.h1(col) + 1]);
+            lanczos_entry(col, alpha, tn);
+            f += std::norm(alpha) + tn * tn + (col + 1 < steps ? tn * tn : 0.0);
+        } else {
+            arnoldi_column(col, hv(L.h1(col), col + 1), hv(L.h2(col), col + 1), tn);
+            for (int r = 0; r <= col; r++) f += std::norm(h(r, col));
+            f += tn * tn;
+        }
+        if (breakdown(col, tn, f)) return col + 1;
     }
     return steps;
 }
@@ -706,11 +712,13 @@ void KrylovProj::lanczos_times(const std::vector<cd> &scales, std::vector<cd> &c
     sym_eig(c, A, w, Vv);
     coefs.assign((size_t)c * scales.size(), cd(0, 0));
     ests.assign(scales.size(), 0.0);
+    std::vector<cd> g(c);
     for (size_t j = 0; j < scales.size(); j++) {
         if (scales[j] == cd(0, 0)) continue;
+        for (int q = 0; q < c; q++) g[q] = std::exp(scales[j] * w[q]) * Vv[(size_t)0 * c + q];
         for (int i = 0; i < c; i++) {
             cd acc = 0;
-            for (int q = 0; q < c; q++) acc += Vv[(size_t)i * c + q] * std::exp(scales[j] * w[q]) * Vv[(size_t)0 * c + q];
+            for (int q = 0; q < c; q++) acc += Vv[(size_t)i * c + q] * g[q];
             coefs[j * c + i] = beta * acc;
         }
         ests[j] = full ? std::abs(scales[j]) * terminal * std::abs(coefs[j * c + c - 1]) : 0.0;
