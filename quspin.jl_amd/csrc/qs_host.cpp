@@ -932,22 +932,29 @@ void herm_eig(int n, std::vector<cd> &A, std::vector<double> &w, std::vector<cd>
 // Pade approximant (Higham, SIAM J. Matrix Anal. Appl. 26 (2005) 1179).
 void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
 {
-    // complex products on the interleaved (re, im) doubles, which vectorise
-    auto mm = [n](const std::vector<cd> &X, const std::vector<cd> &Y) {
-        std::vector<cd> Z((size_t)n * n, cd(0, 0));
-        const double *y = reinterpret_cast<const double *>(Y.data());
+    // complex products on split real / imaginary planes, which vectorise
+    std::vector<double> yr((size_t)n * n), yi((size_t)n * n), zr((size_t)n * n), zi((size_t)n * n);
+    auto mm = [&](const std::vector<cd> &X, const std::vector<cd> &Y) {
+        for (size_t t = 0; t < (size_t)n * n; t++) {
+            yr[t] = Y[t].real();
+            yi[t] = Y[t].imag();
+        }
+        std::fill(zr.begin(), zr.end(), 0.0);
+        std::fill(zi.begin(), zi.end(), 0.0);
         for (int i = 0; i < n; i++) {
-            double *z = reinterpret_cast<double *>(Z.data() + (size_t)i * n);
+            double *qr = zr.data() + (size_t)i * n, *qi = zi.data() + (size_t)i * n;
             for (int k = 0; k < n; k++) {
-                const double xr = X[(size_t)i * n + k].real(), xi = X[(size_t)i * n + k].imag();
-                if (xr == 0 && xi == 0) continue;
-                const double *yk = y + (size_t)2 * k * n;
+                const double a = X[(size_t)i * n + k].real(), b = X[(size_t)i * n + k].imag();
+                if (a == 0 && b == 0) continue;
+                const double *pr = yr.data() + (size_t)k * n, *pi = yi.data() + (size_t)k * n;
                 for (int j = 0; j < n; j++) {
-                    z[2 * j] += xr * yk[2 * j] - xi * yk[2 * j + 1];
-                    z[2 * j + 1] += xr * yk[2 * j + 1] + xi * yk[2 * j];
+                    qr[j] += a * pr[j] - b * pi[j];
+                    qi[j] += a * pi[j] + b * pr[j];
                 }
             }
         }
+        std::vector<cd> Z((size_t)n * n);
+        for (size_t t = 0; t < (size_t)n * n; t++) Z[t] = cd(zr[t], zi[t]);
         return Z;
     };
     double norm1 = 0;
@@ -987,7 +994,13 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
         P[i] = V1[i] - U[i];
         Q[i] = V1[i] + U[i];
     }
-    // Gaussian elimination with partial pivoting on P, applied to Q
+    // Gaussian elimination with partial pivoting on P, applied to Q (Q on
+    // split planes: its row updates are the bulk of the work)
+    std::vector<double> &qr = yr, &qi = yi;
+    for (size_t t = 0; t < (size_t)n * n; t++) {
+        qr[t] = Q[t].real();
+        qi[t] = Q[t].imag();
+    }
     for (int c = 0; c < n; c++) {
         int piv = c;
         for (int r = c + 1; r < n; r++)
@@ -995,24 +1008,41 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E)
         if (piv != c)
             for (int j = 0; j < n; j++) {
                 std::swap(P[(size_t)c * n + j], P[(size_t)piv * n + j]);
-                std::swap(Q[(size_t)c * n + j], Q[(size_t)piv * n + j]);
+                std::swap(qr[(size_t)c * n + j], qr[(size_t)piv * n + j]);
+                std::swap(qi[(size_t)c * n + j], qi[(size_t)piv * n + j]);
             }
-        cd d = P[(size_t)c * n + c];
+        const cd d = P[(size_t)c * n + c];
         for (int r = c + 1; r < n; r++) {
-            cd f = P[(size_t)r * n + c] / d;
+            const cd f = P[(size_t)r * n + c] / d;
             if (f == cd(0, 0)) continue;
             for (int j = c; j < n; j++) P[(size_t)r * n + j] -= f * P[(size_t)c * n + j];
-            for (int j = 0; j < n; j++) Q[(size_t)r * n + j] -= f * Q[(size_t)c * n + j];
+            const double a = f.real(), b = f.imag();
+            double *rr = qr.data() + (size_t)r * n, *ri = qi.data() + (size_t)r * n;
+            const double *cr = qr.data() + (size_t)c * n, *ci = qi.data() + (size_t)c * n;
+            for (int j = 0; j < n; j++) {
+                rr[j] -= a * cr[j] - b * ci[j];
+                ri[j] -= a * ci[j] + b * cr[j];
+            }
         }
     }
     for (int c = n - 1; c >= 0; c--) {
-        cd d = P[(size_t)c * n + c];
+        double *rr = qr.data() + (size_t)c * n, *ri = qi.data() + (size_t)c * n;
+        for (int k = c + 1; k < n; k++) {
+            const double a = P[(size_t)c * n + k].real(), b = P[(size_t)c * n + k].imag();
+            const double *kr = qr.data() + (size_t)k * n, *ki = qi.data() + (size_t)k * n;
+            for (int j = 0; j < n; j++) {
+                rr[j] -= a * kr[j] - b * ki[j];
+                ri[j] -= a * ki[j] + b * kr[j];
+            }
+        }
+        const cd dinv = 1.0 / P[(size_t)c * n + c];
         for (int j = 0; j < n; j++) {
-            cd acc = Q[(size_t)c * n + j];
-            for (int k = c + 1; k < n; k++) acc -= P[(size_t)c * n + k] * Q[(size_t)k * n + j];
-            Q[(size_t)c * n + j] = acc / d;
+            const cd q = cd(rr[j], ri[j]) * dinv;
+            rr[j] = q.real();
+            ri[j] = q.imag();
         }
     }
+    for (size_t t = 0; t < (size_t)n * n; t++) Q[t] = cd(qr[t], qi[t]);
     E = Q;
     for (int i = 0; i < s; i++) E = mm(E, E);
 }

@@ -628,6 +628,13 @@ struct BatchMatvec;   // device copies of the operators' kernel arguments
 std::unique_ptr<BatchMatvec, void (*)(BatchMatvec *)> make_batch_matvec(const std::vector<const OpHost *> &ops,
                                                                          int dtype);
 void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hipStream_t s);
+// One Lanczos product of an assembled operator (the MatvecJob fields z, zc,
+// zmode, dot for a single product, arguments by value); false when the
+// operator's plain products do not read stored rows
+bool matvec_lanczos(const OpHost &op, int dtype, const void *x, void *y, const void *z, const double *zc, int zmode,
+                    double *dot, hipStream_t s);
+// out[0..1] = the sum of nb interleaved complex partials
+void vec_reduce_dev(const double *partial, int nb, double *out, hipStream_t s);
 bool batch_matvec_ok(const std::vector<const OpHost *> &ops, int dtype);
 
 // Sector <-> parent space (qs_project.hip).  kind 0: the parent vector is

@@ -435,13 +435,22 @@ void queue_factorization(OpHost &op, int dtype, int64_t n, int steps, bool hermi
     for (int c = 0; c < steps; c++) {
         void *w = col(c + 1);
         double *h1 = dd + L.h1(c), *h2 = dd + L.h2(c), *n2 = dd + L.nrm2(c);
-        matvec(op, dtype, col(c), w);
-        if (hermitian) {
+        const int nbs = (int)((n + 255) / 256);
+        if (hermitian && (size_t)2 * nbs <= wk.partial.n &&
+            matvec_lanczos(op, dtype, col(c), w, c > 0 ? col(c - 1) : nullptr, c > 0 ? dd + L.nrm2(c - 1) : nullptr,
+                           CO_NEG_SQRT, wk.partial.get(), s)) {
+            // assembled: the product also subtracts beta v_{c-1} and leaves
+            // the per-block partials of <v_c, w>
+            vec_reduce_dev(wk.partial.get(), nbs, h1, s);
+            gram_update_dev(dtype, col(c), n, 1, h1, w, wk, n2, s);
+        } else if (hermitian) {
             // three-term Lanczos without reorthogonalisation
+            matvec(op, dtype, col(c), w);
             if (c > 0) vec_axpby_dev(dtype, dd + L.nrm2(c - 1), CO_NEG_SQRT, col(c - 1), cd(1, 0), w, n, s);
             gram_dots_dev(dtype, col(c), n, 1, w, wk, h1, s);
             gram_update_dev(dtype, col(c), n, 1, h1, w, wk, n2, s);
         } else {
+            matvec(op, dtype, col(c), w);
             // Arnoldi, two Gram-Schmidt passes
             gram_dots_dev(dtype, V, n, c + 1, w, wk, h1, s);
             gram_update_dev(dtype, V, n, c + 1, h1, w, wk, nullptr, s);

@@ -1077,9 +1077,8 @@ struct SellJobs {
 
 // block b of job q: its kSellBlock rows
 template <typename V>
-__device__ __forceinline__ void sell_job_block(const SellJob &q, int b) {
+__device__ __forceinline__ void sell_job_block(const MVArgs &A, const SellJob &q, int b) {
     using T = VTraits<V>;
-    const MVArgs &A = *q.A;
     const int64_t row = (int64_t)b * kSellBlock + threadIdx.x;
     if (!q.dot) {
         if (row >= A.N) return;
@@ -1140,7 +1139,14 @@ __global__ __launch_bounds__(kSellBlock) void kb_sell_spmv(SellJobs J) {
         if ((int)blockIdx.x >= J.job[mid].b0) lo = mid;
         else hi = mid - 1;
     }
-    sell_job_block<V>(J.job[lo], (int)blockIdx.x - J.job[lo].b0);
+    sell_job_block<V>(*J.job[lo].A, J.job[lo], (int)blockIdx.x - J.job[lo].b0);
+}
+
+// One operator's Lanczos product on its stored rows (the job's A unused:
+// the arguments come by value)
+template <typename V>
+__global__ __launch_bounds__(kSellBlock) void k_sell_lanczos(MVArgs A, SellJob q) {
+    sell_job_block<V>(A, q, (int)blockIdx.x);
 }
 
 // The same over a device job table: job j owns blocks [j nbmax, (j+1) nbmax)
@@ -1150,7 +1156,7 @@ __global__ __launch_bounds__(kSellBlock) void kb_sell_tab(const SellJob *__restr
     const int j = (int)blockIdx.x / nbmax, b = (int)blockIdx.x - j * nbmax;
     const SellJob q = tab[j];
     if (b >= q.nb) return;
-    sell_job_block<V>(q, b);
+    sell_job_block<V>(*q.A, q, b);
 }
 
 // ------------------------------------------------------- split kernel --
@@ -2747,6 +2753,27 @@ void ensure_assembled(const OpHost &op, hipStream_t s) {
             op.want_sell = false;   // couplings it cannot assemble: stay matrix-free
         }
     }
+}
+
+bool matvec_lanczos(const OpHost &op, int dtype, const void *x, void *y, const void *z, const double *zc, int zmode,
+                    double *dot, hipStream_t s) {
+    const BasisHost &b = *op.basis;
+    if (b.wide || !op.sell || !(op.lanes_per_row == 0 || op.lanes_per_row == QS_VARIANT_SELL)) return false;
+    const SellHost &t = *op.sell;
+    if (t.cplx && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
+    const MVArgs a = base_args(op, false, 1.0, 0.0, 0.0, 0.0, false, false, nullptr);
+    const int nb = (int)((b.Ns + kSellBlock - 1) / kSellBlock);
+    const SellJob q{nullptr, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(), t.cplx ? t.wim.get() : nullptr,
+                    x, y, 0, nb, z, zc, zmode, dot};
+    switch (dtype) {
+    case QS_F64: hipLaunchKernelGGL(k_sell_lanczos<double>, dim3(nb), dim3(kSellBlock), 0, s, a, q); break;
+    case QS_F32: hipLaunchKernelGGL(k_sell_lanczos<float>, dim3(nb), dim3(kSellBlock), 0, s, a, q); break;
+    case QS_C128: hipLaunchKernelGGL(k_sell_lanczos<cplx_d>, dim3(nb), dim3(kSellBlock), 0, s, a, q); break;
+    case QS_C64: hipLaunchKernelGGL(k_sell_lanczos<cplx_f>, dim3(nb), dim3(kSellBlock), 0, s, a, q); break;
+    default: fail(QS_EARG, "unsupported dtype");
+    }
+    QS_HIP(hipGetLastError());
+    return true;
 }
 
 void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double are, double aim, double bre,

@@ -1207,6 +1207,11 @@ void gram_dots_dev(int dtype, const void *Vmat, int64_t n, int m, const void *w,
     QS_HIP(hipGetLastError());
 }
 
+void vec_reduce_dev(const double *partial, int nb, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kVB), 0, s, partial, nb, out, nullptr, nullptr, 0, 0);
+    QS_HIP(hipGetLastError());
+}
+
 void gram_update_dev(int dtype, const void *Vmat, int64_t n, int m, const double *h, void *w, VecWork &wk,
                      double *nrm2_out, hipStream_t s, const double *dgks_h1, const double *dgks_n2) {
     const int nb = vec_blocks(n);
