@@ -343,11 +343,20 @@ __device__ __forceinline__ int64_t w4_find(const uint64_t *st, int64_t n, const 
 // columns and weights first and then gather x, so a lane keeps kSellU
 // gathers in flight; the sum runs in entry order either way.
 constexpr int kSellU = 4;
-template <typename V>
-__device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, const double *__restrict__ wre,
-                                         const double *__restrict__ wim, uint64_t base, uint32_t w,
-                                         const V *__restrict__ x, int conj_x, double &ar, double &ai) {
+template <typename V, bool D>
+__device__ __forceinline__ void sell_row_w(const uint32_t *__restrict__ cols, const SellW &W, uint64_t base,
+                                           uint32_t w, const V *__restrict__ x, int conj_x, double &ar, double &ai) {
     using T = VTraits<V>;
+    auto weight = [&](uint64_t at, double &er, double &ei) {
+        if (D) {
+            const uint32_t k = W.idx[at];
+            er = W.dict[2 * k];
+            ei = W.dict[2 * k + 1];
+        } else {
+            er = W.re[at];
+            ei = W.im ? W.im[at] : 0.0;
+        }
+    };
     uint32_t k = 0;
     for (; k + kSellU <= w; k += kSellU) {
         uint32_t c[kSellU];
@@ -356,8 +365,7 @@ __device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, cons
         for (int u = 0; u < kSellU; u++) {
             const uint64_t at = base + 64 * (uint64_t)(k + u);
             c[u] = cols[at];
-            er[u] = wre[at];
-            ei[u] = wim ? wim[at] : 0.0;
+            weight(at, er[u], ei[u]);
         }
 #pragma unroll
         for (int u = 0; u < kSellU; u++) T::load(x, c[u], vr[u], vi[u]);
@@ -371,13 +379,20 @@ __device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, cons
     for (; k < w; k++) {
         const uint64_t at = base + 64 * (uint64_t)k;
         const uint32_t c = cols[at];
-        const double er = wre[at], ei = wim ? wim[at] : 0.0;
-        double vr, vi;
+        double er, ei, vr, vi;
+        weight(at, er, ei);
         T::load(x, c, vr, vi);
         if (conj_x) vi = -vi;
         ar += er * vr - ei * vi;
         ai += er * vi + ei * vr;
     }
+}
+
+template <typename V>
+__device__ __forceinline__ void sell_row(const uint32_t *__restrict__ cols, const SellW &W, uint64_t base, uint32_t w,
+                                         const V *__restrict__ x, int conj_x, double &ar, double &ai) {
+    if (W.dict) sell_row_w<V, true>(cols, W, base, w, x, conj_x, ar, ai);
+    else sell_row_w<V, false>(cols, W, base, w, x, conj_x, ar, ai);
 }
 
 // Coefficient of an axpby read from device memory (a reduction result still

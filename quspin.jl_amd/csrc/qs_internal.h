@@ -325,6 +325,15 @@ struct SplitPassHops {
 
 // The assembled sector matrix of an operator (plain products), SELL-64:
 // rows in slices of 64, slice q padded to its longest row, column-major.
+// Stored weights of assembled rows: per entry (re, im; im null for real
+// weights) or, when the operator has few distinct weights, a 16-bit index
+// per entry into a table of (re, im) pairs (dict non-null).
+struct SellW {
+    const double *re, *im;
+    const uint16_t *idx;
+    const double *dict;
+};
+
 struct SellHost {
     int64_t nslices = 0;
     uint64_t stored = 0;          // entries including padding
@@ -333,7 +342,14 @@ struct SellHost {
     DevBuf<uint64_t> sptr;        // first entry of each slice
     DevBuf<uint32_t> width;       // entries per row of each slice
     DevBuf<uint32_t> cols;
-    DevBuf<double> wre, wim;
+    DevBuf<double> wre, wim;      // per-entry weights (freed when the table form is kept)
+    DevBuf<uint16_t> widx;        // table form: per-entry index into dict
+    DevBuf<double> dict;          // table form: (re, im) pairs
+    uint32_t ndict = 0;           // distinct weights in the table form
+    SellW weights() const {
+        if (dict.get()) return SellW{nullptr, nullptr, widx.get(), dict.get()};
+        return SellW{wre.get(), cplx ? wim.get() : nullptr, nullptr, nullptr};
+    }
 };
 
 struct BasisHost {

@@ -30,6 +30,7 @@
 // memory latency is paid once per batch, and both renumber blocks so each
 // XCD owns a contiguous row range (its own L2 stays warm).
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 
 #include "qs_device.h"
@@ -1041,15 +1042,14 @@ __global__ __launch_bounds__(kSellBlock) void k_sell_fill(MVArgs A, const HopDev
 template <typename V, bool WC>
 __global__ __launch_bounds__(kSellBlock) void k_sell_spmv(MVArgs A, const uint64_t *__restrict__ sptr,
                                                           const uint32_t *__restrict__ width,
-                                                          const uint32_t *__restrict__ cols,
-                                                          const double *__restrict__ wre,
-                                                          const double *__restrict__ wim,
+                                                          const uint32_t *__restrict__ cols, SellW W,
                                                           const V *__restrict__ x, V *__restrict__ y) {
     const int64_t row = (int64_t)blockIdx.x * kSellBlock + threadIdx.x;
     if (row >= A.N) return;
     const uint64_t q = (uint64_t)row >> 6;
     double ar = 0, ai = 0;
-    sell_row<V>(cols, wre, WC ? wim : nullptr, sptr[q] + (row & 63), width[q], x, A.conj_x, ar, ai);
+    if (!WC) W.im = nullptr;
+    sell_row<V>(cols, W, sptr[q] + (row & 63), width[q], x, A.conj_x, ar, ai);
     double dot_r = 0, dot_i = 0;
     finish_row<V>(A, y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
 }
@@ -1061,7 +1061,7 @@ struct SellJob {
     const MVArgs *A;   // device copy
     const uint64_t *sptr;
     const uint32_t *width, *cols;
-    const double *wre, *wim;   // wim null: real weights
+    SellW wt;
     const void *x;
     void *y;
     int b0, nb;
@@ -1070,9 +1070,10 @@ struct SellJob {
     int zmode;
     double *dot;       // optional: per-block partials of <x, y>
 };
+constexpr int kSellJobsInline = 24;   // jobs passed as kernel arguments (within 4 KB)
 struct SellJobs {
     int njobs;
-    SellJob job[32];
+    SellJob job[kSellJobsInline];
 };
 
 // block b of job q: its kSellBlock rows
@@ -1083,7 +1084,7 @@ __device__ __forceinline__ void sell_job_block(const MVArgs &A, const SellJob &q
     if (!q.dot) {
         if (row >= A.N) return;
         double ar = 0, ai = 0;
-        sell_row<V>(q.cols, q.wre, q.wim, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
+        sell_row<V>(q.cols, q.wt, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
                     (const V *)q.x, A.conj_x, ar, ai);
         double dot_r = 0, dot_i = 0;
         finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
@@ -1095,7 +1096,7 @@ __device__ __forceinline__ void sell_job_block(const MVArgs &A, const SellJob &q
     double dr = 0, di = 0;
     if (row < A.N) {
         double ar = 0, ai = 0;
-        sell_row<V>(q.cols, q.wre, q.wim, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
+        sell_row<V>(q.cols, q.wt, q.sptr[(uint64_t)row >> 6] + (row & 63), q.width[(uint64_t)row >> 6],
                     (const V *)q.x, A.conj_x, ar, ai);
         double dot_r = 0, dot_i = 0;
         finish_row<V>(A, (V *)q.y, row, ar, ai, 0.0, 0.0, dot_r, dot_i);
@@ -2650,6 +2651,126 @@ void sell_pass(const MVArgs &a, bool cc, int cm, bool track, bool pc, const HopL
 // operators.jl:461-576 (_assemble) builds P'.H_par.P as a sparse matrix; here
 // the rows of the plain product come straight from the pull kernel's
 // neighbour enumeration (two passes: lengths, then entries).
+// ----------------------------------------------- weight table of the rows --
+// Assembled rows of a symmetry sector hold few distinct weights (couplings
+// times characters times orbit-norm ratios), so each stored entry can keep a
+// 16-bit index into a table of (re, im) pairs instead of 8 or 16 bytes of
+// weight: 6 bytes per entry with its column.  The table is an open-addressed
+// hash of the weights' bit patterns (linear probing, bounded); any overflow,
+// long probe or 64-bit hash collision leaves the per-entry weights in place.
+constexpr uint32_t kDictSlots = 1u << 16;
+constexpr int kDictProbe = 512;
+constexpr unsigned long long kDictEmpty = ~0ull;
+
+__device__ __forceinline__ unsigned long long dict_key(double re, double im) {
+    const unsigned long long a = (unsigned long long)__double_as_longlong(re);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(im);
+    unsigned long long h = (a ^ (b * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull)) * 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+    h *= 0x94D049BB133111EBull;
+    h ^= h >> 29;
+    return h == kDictEmpty ? h - 1 : h;
+}
+
+// slot of every distinct weight; the thread that claims a slot writes its
+// pair into the table
+__global__ __launch_bounds__(256) void k_dict_insert(const double *__restrict__ re, const double *__restrict__ im,
+                                                     uint64_t n, unsigned long long *keys, double *dict,
+                                                     uint32_t *status) {
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (uint64_t)gridDim.x * 256) {
+        const double wr = re[e], wi = im ? im[e] : 0.0;
+        const unsigned long long h = dict_key(wr, wi);
+        uint32_t p = (uint32_t)h & (kDictSlots - 1);
+        bool ok = false;
+        for (int t = 0; t < kDictProbe; t++) {
+            unsigned long long cur = keys[p];
+            if (cur == kDictEmpty) {
+                cur = atomicCAS(&keys[p], kDictEmpty, h);
+                if (cur == kDictEmpty) {
+                    dict[2 * p] = wr;
+                    dict[2 * p + 1] = wi;
+                    atomicAdd(&status[1], 1u);
+                    ok = true;
+                    break;
+                }
+            }
+            if (cur == h) {
+                ok = true;
+                break;
+            }
+            p = (p + 1) & (kDictSlots - 1);
+        }
+        if (!ok) atomicOr(&status[0], 1u);
+    }
+}
+
+// each entry's slot, checked against the table's pair bit for bit
+__global__ __launch_bounds__(256) void k_dict_index(const double *__restrict__ re, const double *__restrict__ im,
+                                                    uint64_t n, const unsigned long long *__restrict__ keys,
+                                                    const double *__restrict__ dict, uint16_t *idx,
+                                                    uint32_t *status) {
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (uint64_t)gridDim.x * 256) {
+        const double wr = re[e], wi = im ? im[e] : 0.0;
+        const unsigned long long h = dict_key(wr, wi);
+        uint32_t p = (uint32_t)h & (kDictSlots - 1);
+        bool ok = false;
+        for (int t = 0; t < kDictProbe; t++) {
+            const unsigned long long cur = keys[p];
+            if (cur == h) {
+                ok = __double_as_longlong(dict[2 * p]) == __double_as_longlong(wr) &&
+                     __double_as_longlong(dict[2 * p + 1]) == __double_as_longlong(wi);
+                break;
+            }
+            if (cur == kDictEmpty) break;
+            p = (p + 1) & (kDictSlots - 1);
+        }
+        idx[e] = (uint16_t)p;
+        if (!ok) atomicOr(&status[0], 1u);
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// The table form of freshly assembled rows (QS_SELL_TABLE=0 keeps the
+// per-entry weights); true when it was kept.
+bool sell_compress(SellHost &t, hipStream_t s) {
+    const char *env = std::getenv("QS_SELL_TABLE");
+    if (env && env[0] == '0') return false;
+    if (t.stored == 0) return false;
+    SBuf<unsigned long long> keys(kDictSlots, s);
+    SBuf<uint32_t> status(2, s);
+    DevBuf<double> dict((size_t)2 * kDictSlots);
+    const unsigned grid = (unsigned)std::min<uint64_t>(8192, (t.stored + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, keys.get(), (uint64_t)kDictSlots, kDictEmpty);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)status.get(), (uint64_t)1, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (unsigned long long *)dict.get(),
+                       (uint64_t)2 * kDictSlots, 0ull);
+    const double *im = t.cplx ? t.wim.get() : nullptr;
+    hipLaunchKernelGGL(k_dict_insert, dim3(grid), dim3(256), 0, s, t.wre.get(), im, t.stored, keys.get(), dict.get(),
+                       status.get());
+    QS_HIP(hipGetLastError());
+    uint32_t st[2] = {0, 0};
+    QS_HIP(hipMemcpyAsync(st, status.get(), sizeof(st), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    if (st[0]) return false;
+    DevBuf<uint16_t> widx((size_t)t.stored);
+    hipLaunchKernelGGL(k_dict_index, dim3(grid), dim3(256), 0, s, t.wre.get(), im, t.stored, keys.get(), dict.get(),
+                       widx.get(), status.get());
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipMemcpyAsync(st, status.get(), sizeof(st), hipMemcpyDeviceToHost, s));
+    QS_HIP(hipStreamSynchronize(s));
+    if (st[0]) return false;
+    t.widx = std::move(widx);
+    t.dict = std::move(dict);
+    t.ndict = st[1];
+    t.wre = DevBuf<double>();
+    t.wim = DevBuf<double>();
+    return true;
+}
+
 void sell_assemble(OpHost &op, hipStream_t s) {
     const BasisHost &b = *op.basis;
     if (!hop_eligible(op, false))
@@ -2700,6 +2821,7 @@ void sell_assemble(OpHost &op, hipStream_t s) {
     sell_pass(a, cc, cm, track, pc, H, lds, s, grid, nullptr, t.get());
     QS_HIP(hipGetLastError());
     QS_HIP(hipStreamSynchronize(s));
+    sell_compress(*t, s);
     op.sell = std::move(t);
 }
 
@@ -2713,8 +2835,19 @@ void sell_to_csr(const OpHost &op, std::vector<int64_t> &rowptr, std::vector<int
     QS_HIP(hipMemcpy(sp.data(), t.sptr.get(), sp.size() * 8, hipMemcpyDeviceToHost));
     QS_HIP(hipMemcpy(w.data(), t.width.get(), w.size() * 4, hipMemcpyDeviceToHost));
     QS_HIP(hipMemcpy(c.data(), t.cols.get(), c.size() * 4, hipMemcpyDeviceToHost));
-    QS_HIP(hipMemcpy(re.data(), t.wre.get(), re.size() * 8, hipMemcpyDeviceToHost));
-    if (t.cplx) QS_HIP(hipMemcpy(im.data(), t.wim.get(), im.size() * 8, hipMemcpyDeviceToHost));
+    if (t.dict.get()) {
+        std::vector<uint16_t> ix((size_t)t.stored);
+        std::vector<double> tab((size_t)2 * kDictSlots);
+        QS_HIP(hipMemcpy(ix.data(), t.widx.get(), ix.size() * 2, hipMemcpyDeviceToHost));
+        QS_HIP(hipMemcpy(tab.data(), t.dict.get(), tab.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < ix.size(); e++) {
+            re[e] = tab[2 * (size_t)ix[e]];
+            if (t.cplx) im[e] = tab[2 * (size_t)ix[e] + 1];
+        }
+    } else {
+        QS_HIP(hipMemcpy(re.data(), t.wre.get(), re.size() * 8, hipMemcpyDeviceToHost));
+        if (t.cplx) QS_HIP(hipMemcpy(im.data(), t.wim.get(), im.size() * 8, hipMemcpyDeviceToHost));
+    }
     rowptr.assign((size_t)n + 1, 0);
     col.clear();
     val.clear();
@@ -2763,8 +2896,7 @@ bool matvec_lanczos(const OpHost &op, int dtype, const void *x, void *y, const v
     if (t.cplx && !dtype_complex(dtype)) fail(QS_EARG, "a complex sector needs a complex vector dtype");
     const MVArgs a = base_args(op, false, 1.0, 0.0, 0.0, 0.0, false, false, nullptr);
     const int nb = (int)((b.Ns + kSellBlock - 1) / kSellBlock);
-    const SellJob q{nullptr, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(), t.cplx ? t.wim.get() : nullptr,
-                    x, y, 0, nb, z, zc, zmode, dot};
+    const SellJob q{nullptr, t.sptr.get(), t.width.get(), t.cols.get(), t.weights(), x, y, 0, nb, z, zc, zmode, dot};
     switch (dtype) {
     case QS_F64: hipLaunchKernelGGL(k_sell_lanczos<double>, dim3(nb), dim3(kSellBlock), 0, s, a, q); break;
     case QS_F32: hipLaunchKernelGGL(k_sell_lanczos<float>, dim3(nb), dim3(kSellBlock), 0, s, a, q); break;
@@ -2829,11 +2961,10 @@ void launch_matvec(const OpHost &op, int dtype, const void *x, void *y, double a
             using V = std::remove_pointer_t<decltype(tag)>;
             if (t.cplx)
                 hipLaunchKernelGGL((k_sell_spmv<V, true>), dim3(grid), dim3(kSellBlock), 0, s, a, t.sptr.get(),
-                                   t.width.get(), t.cols.get(), t.wre.get(), t.wim.get(), (const V *)x, (V *)y);
+                                   t.width.get(), t.cols.get(), t.weights(), (const V *)x, (V *)y);
             else
                 hipLaunchKernelGGL((k_sell_spmv<V, false>), dim3(grid), dim3(kSellBlock), 0, s, a, t.sptr.get(),
-                                   t.width.get(), t.cols.get(), t.wre.get(), (const double *)nullptr,
-                                   (const V *)x, (V *)y);
+                                   t.width.get(), t.cols.get(), t.weights(), (const V *)x, (V *)y);
         };
         switch (dtype) {
         case QS_F64: go((double *)nullptr); break;
@@ -3012,7 +3143,7 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
         std::vector<size_t> asm_jobs;
         for (size_t j = 0; j < jobs.size(); j++)
             if (bm.ops[jobs[j].op]->sell) asm_jobs.push_back(j);
-        if (asm_jobs.size() > 32) {
+        if (asm_jobs.size() > (size_t)kSellJobsInline) {
             // one launch through a device job table, each job on nbmax blocks;
             // the jobs of one operator side by side, so its stored rows stay
             // in the L2 / MALL while they run
@@ -3025,9 +3156,8 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
                 const OpHost &op = *bm.ops[mj.op];
                 const SellHost &t = *op.sell;
                 const int nb = (int)((op.basis->Ns + kSellBlock - 1) / kSellBlock);
-                tab.push_back(SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(),
-                                      t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, 0, nb, mj.z, mj.zc, mj.zmode,
-                                      mj.dot});
+                tab.push_back(SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.weights(),
+                                      mj.x, mj.y, 0, nb, mj.z, mj.zc, mj.zmode, mj.dot});
                 nbmax = std::max(nbmax, nb);
                 done[j] = 1;
             }
@@ -3044,18 +3174,17 @@ void launch_batch_matvec(BatchMatvec &bm, const std::vector<MatvecJob> &jobs, hi
             QS_HIP(hipGetLastError());
             asm_jobs.clear();
         }
-        for (size_t c0 = 0; c0 < asm_jobs.size(); c0 += 32) {
+        for (size_t c0 = 0; c0 < asm_jobs.size(); c0 += kSellJobsInline) {
             SellJobs J;
-            J.njobs = (int)std::min<size_t>(32, asm_jobs.size() - c0);
+            J.njobs = (int)std::min<size_t>(kSellJobsInline, asm_jobs.size() - c0);
             int total = 0;
             for (int i = 0; i < J.njobs; i++) {
                 const MatvecJob &mj = jobs[asm_jobs[c0 + i]];
                 const OpHost &op = *bm.ops[mj.op];
                 const SellHost &t = *op.sell;
                 const int nb = (int)((op.basis->Ns + kSellBlock - 1) / kSellBlock);
-                J.job[i] = SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.wre.get(),
-                                   t.cplx ? t.wim.get() : nullptr, mj.x, mj.y, total, nb, mj.z, mj.zc, mj.zmode,
-                                   mj.dot};
+                J.job[i] = SellJob{bm.args.get() + mj.op, t.sptr.get(), t.width.get(), t.cols.get(), t.weights(),
+                                   mj.x, mj.y, total, nb, mj.z, mj.zc, mj.zmode, mj.dot};
                 total += nb;
                 done[asm_jobs[c0 + i]] = 1;
             }

@@ -522,7 +522,7 @@ struct SmallKrylov {
     int per;            // DGKS: scalars per step
     const uint64_t *sptr;
     const uint32_t *width, *cols;
-    const double *wre, *wim;   // wim null: real weights
+    SellW wt;
 };
 
 // sum over the workgroup of every thread's (re, im): fixed order (wavefront
@@ -661,7 +661,7 @@ __device__ void wg_matvec(const SmallKrylov &K, const V *x, V *y) {
     using T = VTraits<V>;
     for (int64_t r = threadIdx.x; r < K.n; r += kSmallT) {
         double ar = 0, ai = 0;
-        sell_row<V>(K.cols, K.wre, K.wim, K.sptr[(uint64_t)r >> 6] + (r & 63), K.width[(uint64_t)r >> 6], x, 0, ar,
+        sell_row<V>(K.cols, K.wt, K.sptr[(uint64_t)r >> 6] + (r & 63), K.width[(uint64_t)r >> 6], x, 0, ar,
                     ai);
         T::store(y, r, ar, ai);
     }
@@ -1400,8 +1400,7 @@ void krylov_small(const OpHost &op, int dtype, int mode, int steps, int j0, void
     K.sptr = t.sptr.get();
     K.width = t.width.get();
     K.cols = t.cols.get();
-    K.wre = t.wre.get();
-    K.wim = t.cplx ? t.wim.get() : nullptr;
+    K.wt = t.weights();
     by_dtype(dtype, [&](auto *tag) {
         using VT = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL((k_krylov_small<VT>), dim3(1), dim3(kSmallT), 0, s, K);
