@@ -31,6 +31,30 @@ struct Krylov {
         V.alloc((size_t)n * c * esz, s);
     }
     void *col(int j) const { return (void *)(V.get() + (size_t)j * n * esz); }
+    void swap(Krylov &o) {
+        std::swap(V.p, o.V.p);
+        std::swap(V.n, o.V.n);
+        std::swap(cols, o.cols);
+    }
+};
+
+// Second basis of a thick-restart solve: the restart's recombination writes
+// into it and the two swap, instead of recombining into scratch and copying
+// the kept columns back.  Allocated at the first restart; where device memory
+// cannot hold it the restart recombines through scratch as before.
+struct SpareBasis {
+    std::unique_ptr<Krylov> K;
+    bool failed = false;
+    Krylov *get(int dtype, int64_t n, int cols, hipStream_t s) {
+        if (!K && !failed) {
+            try {
+                K = std::make_unique<Krylov>(dtype, n, cols, s);
+            } catch (const Error &) {
+                failed = true;
+            }
+        }
+        return K.get();
+    }
 };
 
 void matvec(OpHost &op, int dtype, const void *x, void *y) {
@@ -181,6 +205,7 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
     if (ncv <= k) fail(QS_EARG, "ncv must exceed k");
     const double eps23 = std::pow(kEps, 2.0 / 3.0);
     Krylov K(dtype, n, ncv + 1, s);
+    SpareBasis spare;
     VecWork wk(s);
     // starting vector
     if (v0_dev) {
@@ -322,9 +347,14 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
             std::vector<cd> C((size_t)m * kc);
             for (int c = 0; c < kc; c++)
                 for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + wanted_all[c]], 0);
-            Krylov tmp(dtype, n, kc, s);
-            basis_combine(dtype, K.V.get(), n, m, C, kc, tmp.V.get(), s);
-            QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * kc * K.esz, hipMemcpyDeviceToDevice, s));
+            if (Krylov *K2 = spare.get(dtype, n, ncv + 1, s)) {
+                basis_combine(dtype, K.V.get(), n, m, C, kc, K2->V.get(), s);
+                K.swap(*K2);
+            } else {
+                Krylov tmp(dtype, n, kc, s);
+                basis_combine(dtype, K.V.get(), n, m, C, kc, tmp.V.get(), s);
+                QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * kc * K.esz, hipMemcpyDeviceToDevice, s));
+            }
             void *w = K.col(kc);
             vec_random(dtype, w, n, 0x7e51ULL + (reseed++), s);
             std::vector<cd> hh;
@@ -343,10 +373,16 @@ void eigsh_device(OpHost &op, int dtype, int k, int which, double tol, int maxit
         std::vector<cd> C((size_t)m * keep);
         for (int c = 0; c < keep; c++)
             for (int j = 0; j < m; j++) C[(size_t)c * m + j] = cd(Y[(size_t)j * m + sel[c]], 0);
-        Krylov tmp(dtype, n, keep, s);
-        basis_combine(dtype, K.V.get(), n, m, C, keep, tmp.V.get(), s);
-        QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * keep * K.esz, hipMemcpyDeviceToDevice, s));
-        QS_HIP(hipMemcpyAsync(K.col(keep), K.col(m), (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
+        if (Krylov *K2 = spare.get(dtype, n, ncv + 1, s)) {
+            basis_combine(dtype, K.V.get(), n, m, C, keep, K2->V.get(), s);
+            QS_HIP(hipMemcpyAsync(K2->col(keep), K.col(m), (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
+            K.swap(*K2);
+        } else {
+            Krylov tmp(dtype, n, keep, s);
+            basis_combine(dtype, K.V.get(), n, m, C, keep, tmp.V.get(), s);
+            QS_HIP(hipMemcpyAsync(K.col(0), tmp.V.get(), (size_t)n * keep * K.esz, hipMemcpyDeviceToDevice, s));
+            QS_HIP(hipMemcpyAsync(K.col(keep), K.col(m), (size_t)n * K.esz, hipMemcpyDeviceToDevice, s));
+        }
         std::fill(T.begin(), T.end(), 0.0);
         for (int c = 0; c < keep; c++) {
             t(c, c) = theta[sel[c]];
