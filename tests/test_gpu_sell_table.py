@@ -81,3 +81,23 @@ def test_table_rows_in_the_batch_engine(gpu):
         outs.append([s.cpu().numpy() for s in batch.expm_multiply(jobs, tol=1e-12)])
     for a, c in zip(*outs):
         assert np.array_equal(a.view(np.uint8), c.view(np.uint8))
+
+
+def test_table_overflow_keeps_per_entry_weights(gpu):
+    # all-pairs zz couplings with random strengths give nearly every state its
+    # own diagonal value (far more than the table's 65,536 slots): the rows
+    # keep their per-entry weights and the products stay exact
+    print("sell table overflow", flush=True)
+    L = 20
+    rng = np.random.default_rng(7)
+    pairs = [(float(rng.uniform(-1, 1)), i, j) for i in range(1, L + 1) for j in range(i + 1, L + 1)]
+    terms = serve.chain_terms(L, 0.0, 0.0) + [("zz", pairs)]
+    b = qsgpu.SpinBasis1D(L, nup=L // 2, pauli=False)
+    assert b.Ns > 65536
+    tab = _assembled(b, terms, True)
+    per = _assembled(b, terms, False)
+    free = qsgpu.QuantumLinearOperator(b, terms)
+    x = rng.standard_normal(b.Ns)
+    a, c, f = tab.matvec(x), per.matvec(x), free.matvec(x)
+    assert np.array_equal(a.view(np.uint8), c.view(np.uint8))
+    assert np.linalg.norm(a - f) <= 1e-13 * np.linalg.norm(f)
