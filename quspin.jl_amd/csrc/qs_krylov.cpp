@@ -40,13 +40,22 @@ struct Krylov {
 
 // Second basis of a thick-restart solve: the restart's recombination writes
 // into it and the two swap, instead of recombining into scratch and copying
-// the kept columns back.  Allocated at the first restart; where device memory
-// cannot hold it the restart recombines through scratch as before.
+// the kept columns back.  Allocated at the first restart when free device
+// memory holds three times its size; otherwise the restart recombines
+// through scratch.
 struct SpareBasis {
     std::unique_ptr<Krylov> K;
     bool failed = false;
     Krylov *get(int dtype, int64_t n, int cols, hipStream_t s) {
         if (!K && !failed) {
+            // only where it leaves room for the rest of the solve
+            size_t fr = 0, tot = 0;
+            const size_t bytes = (size_t)n * cols * dtype_size(dtype);
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < 3 * bytes) {
+                (void)hipGetLastError();
+                failed = true;
+                return nullptr;
+            }
             try {
                 K = std::make_unique<Krylov>(dtype, n, cols, s);
             } catch (const Error &) {
