@@ -463,7 +463,7 @@ def test_split_kernel_fullsize_c3(gpu):
 
 
 def test_c3_eigsh_fp32_stream_against_fp64(gpu):
-    # BASELINE precision form: L=32 nup=16 z=+1 (N = 300,540,195), k=4 lowest
+    # The precision-form sector: L=32 nup=16 z=+1 (N = 300,540,195), k=4 lowest
     # eigenvalues (the random fields cancel in z+).  FP32-stored Krylov basis
     # with FP64 dots / norms / tridiagonal solve against the FP64 run: every
     # eigenvalue within 1e-8, every FP64 Ritz vector's residual within

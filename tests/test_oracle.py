@@ -152,7 +152,7 @@ def test_c1_csr_structure_and_product():
     assert np.allclose(csr.matvec(x, nthreads=4), y1, atol=0)
 
 
-def test_burnside_dimensions_match_survey_configs():
+def test_burnside_dimensions_match_benchmark_sectors():
     # sector sizes of the benchmark configurations, from the character formula
     assert _dense.burnside_count(12, 6, kblock=0, pblock=1, zblock=1) == \
         oracle.Sector(12, nup=6, kblock=0, pblock=1, zblock=1).Ns
