@@ -618,7 +618,10 @@ bool launch_updgram_jobs(int dtype, std::vector<VecJob> &jobs, hipStream_t s, Jo
 // (FactScalars layout).  krylov_small_ok: the operator's plain products read
 // stored rows, n <= kSmallMaxN and at most kSmallMaxCols basis columns.
 enum SmallMode { SMALL_DGKS = 0, SMALL_LANCZOS = 1, SMALL_ARNOLDI = 2 };
-constexpr int64_t kSmallMaxN = 8192;
+// above ~2k rows one CU's L2 bandwidth makes the launch slower than the
+// per-operation launches over the whole chip (measured: 4.6k rows 32 vs 13 ms
+// per evolve)
+constexpr int64_t kSmallMaxN = 2048;
 constexpr int kSmallMaxCols = 128;
 bool krylov_small_ok(const OpHost &op, int cols);
 void krylov_small(const OpHost &op, int dtype, int mode, int steps, int j0, void *V, const void *vin, double *dd,

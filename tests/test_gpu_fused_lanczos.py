@@ -1,6 +1,6 @@
 """GPU tests of the fused Lanczos product of an assembled operator.
 
-Above the single-workgroup size (8192 rows) a Hermitian expm factorization of
+Above the single-workgroup size (2048 rows) a Hermitian expm factorization of
 an assembled operator runs each three-term Lanczos step as one product that
 also subtracts beta v_{j-1} and leaves the per-block partials of <v_j, w>
 (operators.jl:1655-1731, the same operation order as the separate product,
@@ -24,7 +24,7 @@ def test_fused_lanczos_expm_matches_restatement(gpu, L, blocks):
     print("fused lanczos expm", L, blocks, flush=True)
     terms = serve.chain_terms(L, 0.35, 0.8)
     b = qsgpu.SpinBasis1D(L, nup=L // 2, pauli=False, **blocks)
-    assert b.Ns > 8192 or blocks.get("pblock")
+    assert b.Ns > 2048
     free = qsgpu.QuantumLinearOperator(b, terms)
     asm = qsgpu.QuantumLinearOperator(b, terms).assemble()
     rng = np.random.default_rng(L + b.Ns)
@@ -47,7 +47,7 @@ def test_fused_lanczos_evolve_in_place_of_separate_steps(gpu):
     print("fused lanczos vs separate on one operator", flush=True)
     L = 22
     b = qsgpu.SpinBasis1D(L, nup=L // 2, pauli=False, kblock=3)
-    assert b.Ns > 8192
+    assert b.Ns > 2048
     op = qsgpu.QuantumLinearOperator(b, serve.chain_terms(L, 0.2, 1.5)).assemble()
     v = np.random.default_rng(9).standard_normal(b.Ns) + 0j
     v /= np.linalg.norm(v)

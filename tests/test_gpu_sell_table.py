@@ -68,7 +68,7 @@ def test_table_rows_in_the_batch_engine(gpu):
     L = 20
     terms = serve.chain_terms(L, 0.25, 1.2)
     bases = [qsgpu.SpinBasis1D(L, nup=L // 2, pauli=False, kblock=k) for k in (1, 2, 3)]
-    assert min(bb.Ns for bb in bases) > 8192
+    assert min(bb.Ns for bb in bases) > 2048
     outs = []
     for table in (True, False):
         ops = [_assembled(bb, terms, table) for bb in bases]

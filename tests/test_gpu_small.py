@@ -1,6 +1,6 @@
 """GPU tests of the single-workgroup Krylov path for small assembled sectors.
 
-An assembled operator of at most 8192 rows runs each thick-restart Lanczos
+An assembled operator of at most 2048 rows runs each thick-restart Lanczos
 cycle of eigsh, and each expm factorization, in one launch of one workgroup
 (k_krylov_small); the same operator left matrix-free runs the per-operation
 launches.  Both follow the same step sequence (operators.jl:1655-1731 for the
@@ -33,7 +33,7 @@ def test_small_eigsh_matches_per_operation_path(gpu, L, blocks):
     print("small eigsh", L, blocks, flush=True)
     terms = serve.chain_terms(L, 0.3, 0.9)
     b, free, asm = _pair(L, terms, **blocks)
-    assert b.Ns <= 8192
+    assert b.Ns <= 2048
     v0 = np.random.default_rng(L).standard_normal(b.Ns) + 0j
     for k in (1, 4):
         e_free = qsgpu.eigsh(free, k=k, which="SA", tol=1e-12, v0=v0, return_eigenvectors=False)
